@@ -1,0 +1,86 @@
+"""Loader of the gfx950 library and ctypes declarations of its C ABI."""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+lib_path = os.path.join(_HERE, "lib", "libsimpois_hip.so")
+
+
+class SimpoisError(RuntimeError):
+    """A status code other than SP_OK returned by the library."""
+
+
+class sp_params(ctypes.Structure):
+    _fields_ = [("tstep", ctypes.c_float), ("niter", ctypes.c_int),
+                ("scale", ctypes.c_int), ("cgit", ctypes.c_int),
+                ("pmsfac", ctypes.c_double), ("ponlit", ctypes.c_double),
+                ("cg_eps", ctypes.c_double), ("trace", ctypes.c_int)]
+
+
+class sp_stats(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_int), ("cg_iters_finest", ctypes.c_int),
+                ("elapsed_ms", ctypes.c_double)]
+
+
+_P = ctypes.c_void_p
+_F = ctypes.c_float
+_I = ctypes.c_int
+
+# name -> (restype, argtypes); every symbol declared in include/simpois_hip.h
+_SIGNATURES = {
+    "poisson_solver_separable": (None, [_P, _P, _P, _I, _I, _I, _F, _I, _I, _F]),
+    "poisson_rec": (None, [_P, _P, _P, _I, _I, _F, _I, _I, _I]),
+    "poisson_extension_by_cg": (None, [_P, _P, _P, _I, _I, _P, _I, _F]),
+    "biharmonic_extension_by_cg": (None, [_P, _P, _P, _I, _I, _P, _I, _F]),
+    "main_simpois": (_I, [_I, ctypes.POINTER(ctypes.c_char_p)]),
+    "sp_params_default": (None, [ctypes.POINTER(sp_params)]),
+    "sp_last_error": (ctypes.c_char_p, []),
+    "sp_solve_host": (_I, [_P, _P, _P, _I, _I, _I, ctypes.POINTER(sp_params),
+                           ctypes.POINTER(sp_stats)]),
+    "sp_solve_device": (_I, [_P, _P, _P, _I, _I, _I, ctypes.POINTER(sp_params),
+                             _P, ctypes.POINTER(sp_stats)]),
+    "sp_residual_device": (_I, [_P, _P, _P, _I, _I, _I, _I,
+                                ctypes.POINTER(ctypes.c_double), _P]),
+}
+
+
+def exported_symbols():
+    """Names the C ABI must export (the declarations of the public header)."""
+    return sorted(_SIGNATURES)
+
+
+class _Lib:
+    _handle = None
+
+    def __getattr__(self, name):
+        if _Lib._handle is None:
+            if not os.path.exists(lib_path):
+                raise ImportError(
+                    f"{lib_path} is missing: build it with "
+                    "`make -C imscript_amd/csrc` (or __graft_entry__.build())")
+            h = ctypes.CDLL(lib_path)
+            for k, (res, args) in _SIGNATURES.items():
+                fn = getattr(h, k)
+                fn.restype = res
+                fn.argtypes = args
+            _Lib._handle = h
+        return getattr(_Lib._handle, name)
+
+
+lib = _Lib()
+
+
+def default_params(**kw):
+    """sp_params with the reference defaults (-t 0.25 -n 10 -s 99 -c 0) and
+    the PMSFAC / PONLIT / CG_EPS environment knobs, updated by ``kw``."""
+    p = sp_params()
+    lib.sp_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def check(rc):
+    if rc != 0:
+        raise SimpoisError(f"simpois_hip status {rc}: "
+                           f"{lib.sp_last_error().decode(errors='replace')}")

@@ -1,0 +1,124 @@
+"""Array-level interface over the C ABI."""
+import ctypes
+
+import numpy as np
+
+from ._lib import lib, check, default_params, sp_stats
+
+
+def _f32(a, name):
+    a = np.asarray(a)
+    if a.dtype != np.float32 or not a.flags.c_contiguous:
+        raise TypeError(f"{name} must be a C-contiguous float32 array")
+    return a
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def poisson_solver_separable(out, inp, dat, w, h, pd, tstep=0.25, niter=10,
+                             scale=99, cgrad=0.0):
+    """Same arguments and effect as simpois.c:288-302, on numpy float32
+    planar buffers of w*h*pd values (``dat`` may be None).  ``out`` is
+    written in place."""
+    out = _f32(out, "out")
+    inp = _f32(inp, "inp")
+    n = w * h * pd
+    if out.size != n or inp.size != n or (dat is not None and dat.size != n):
+        raise ValueError("buffers must hold w*h*pd floats")
+    if dat is not None:
+        dat = _f32(dat, "dat")
+    if np.shares_memory(out, inp):
+        raise ValueError("out must not alias inp")
+    lib.poisson_solver_separable(_ptr(out), _ptr(inp), _ptr(dat), w, h, pd,
+                                 tstep, int(niter), int(scale), float(cgrad))
+    return out
+
+
+def poisson_rec(u, g, f, w, h, tstep=0.25, niter=10, scale=99, cgit=0):
+    """simpois.c:247-285 on one plane."""
+    u, g = _f32(u, "u"), _f32(g, "g")
+    if f is not None:
+        f = _f32(f, "f")
+    lib.poisson_rec(_ptr(u), _ptr(g), _ptr(f), w, h, tstep, niter, scale, cgit)
+    return u
+
+
+def _cg(fn, out, inp, dat, w, h, init, maxit, eps):
+    out, inp, init = _f32(out, "out"), _f32(inp, "inp"), _f32(init, "init")
+    if dat is not None:
+        dat = _f32(dat, "dat")
+    fn(_ptr(out), _ptr(inp), _ptr(dat), w, h, _ptr(init), maxit, eps)
+    return out
+
+
+def poisson_extension_by_cg(out, inp, dat, w, h, init, maxit, eps):
+    """cleant_cgpois.c:266-271."""
+    return _cg(lib.poisson_extension_by_cg, out, inp, dat, w, h, init, maxit, eps)
+
+
+def biharmonic_extension_by_cg(out, inp, dat, w, h, init, maxit, eps):
+    """cleant_cgpois.c:273-278."""
+    return _cg(lib.biharmonic_extension_by_cg, out, inp, dat, w, h, init,
+               maxit, eps)
+
+
+def simpois(g, f=None, tstep=0.25, niter=10, scale=99, cgit=0, trace=True,
+            stats=False, **knobs):
+    """Fill the NaN pixels of ``g`` (shape (h, w) or (pd, h, w), planar).
+
+    Returns the filled array (and an sp_stats when ``stats``).  Status codes
+    are raised as SimpoisError."""
+    g = np.ascontiguousarray(g, dtype=np.float32)
+    shape = g.shape
+    g3 = g.reshape((1,) + shape) if g.ndim == 2 else g
+    pd, h, w = g3.shape
+    if f is not None:
+        f = np.ascontiguousarray(f, dtype=np.float32).reshape(g3.shape)
+    out = np.empty_like(g3)
+    p = default_params(tstep=tstep, niter=niter, scale=scale, cgit=cgit,
+                       trace=int(bool(trace)), **knobs)
+    st = sp_stats()
+    check(lib.sp_solve_host(_ptr(out), _ptr(g3), _ptr(f), w, h, pd,
+                            ctypes.byref(p), ctypes.byref(st)))
+    out = out.reshape(shape)
+    return (out, st) if stats else out
+
+
+def solve_device(g, f=None, out=None, stream=None, tstep=0.25, niter=10,
+                 scale=99, cgit=0, trace=False, **knobs):
+    """Solve on device tensors (torch, float32, planar (pd,h,w) or (h,w)).
+    Returns (out, sp_stats)."""
+    import torch
+    if g.dtype != torch.float32 or not g.is_cuda or not g.is_contiguous():
+        raise TypeError("g must be a contiguous float32 device tensor")
+    pd, h, w = (1,) + tuple(g.shape) if g.dim() == 2 else tuple(g.shape)
+    if f is not None and (f.shape != g.shape or f.dtype != torch.float32
+                          or not f.is_contiguous()):
+        raise TypeError("f must match g")
+    if out is None:
+        out = torch.empty_like(g)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    p = default_params(tstep=tstep, niter=niter, scale=scale, cgit=cgit,
+                       trace=int(bool(trace)), **knobs)
+    st = sp_stats()
+    check(lib.sp_solve_device(out.data_ptr(), g.data_ptr(),
+                              None if f is None else f.data_ptr(), w, h, pd,
+                              ctypes.byref(p), ctypes.c_void_p(s.cuda_stream),
+                              ctypes.byref(st)))
+    return out, st
+
+
+def residual_device(u, g, f=None, bilap=False, stream=None):
+    """FP64 relative residual per plane of the masked Poisson (or
+    biharmonic) system, on device tensors."""
+    import torch
+    pd, h, w = (1,) + tuple(g.shape) if g.dim() == 2 else tuple(g.shape)
+    rel = (ctypes.c_double * pd)()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib.sp_residual_device(u.data_ptr(), g.data_ptr(),
+                                 None if f is None else f.data_ptr(), w, h, pd,
+                                 int(bool(bilap)), rel,
+                                 ctypes.c_void_p(s.cuda_stream)))
+    return list(rel)

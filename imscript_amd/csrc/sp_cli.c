@@ -1,0 +1,513 @@
+/*
+ * sp_cli.c -- the simpois command line and the image I/O it needs.
+ *
+ * main_simpois keeps the reference's flags, defaults, stdin/stdout
+ * handling, mask semantics and exit codes (simpois.c:351-439), and calls
+ * the GPU solver through poisson_solver_separable.
+ *
+ * Image I/O is the subset of iio that runs without external libraries:
+ *   read : NPY 1.0 (iio.c:3216-3295), PFM (iio.c:2599-2616, see below)
+ *   write: NPY '<f4' shape (h, w, pd) (iio.c:4337-4373), PFM for a ".pfm"
+ *          suffix (iio.c:4151-4164); anything else, and pipes, get NPY
+ *          (iio.c:6495); a terminal on stdout gets a one-line summary
+ *          (iio.c:6171-6185, without the sixel preview).
+ * Pixels are converted to planar float as iio_read_image_float_split does
+ * (iio.c:1416-1420, 5763-5771).
+ */
+#include <ctype.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#include "../../include/simpois_hip.h"
+
+/* ------------------------------------------------------------ failures */
+
+static const char *g_cmdline = "simpois";
+
+/* message and exit(-1), the behaviour of iio's fail() (iio.c:321-350) */
+static void io_fail(const char *fmt, ...)
+{
+	va_list ap;
+	fprintf(stderr, "\nIIO_ERROR(\"%s\"): ", g_cmdline);
+	va_start(ap, fmt);
+	vfprintf(stderr, fmt, ap);
+	va_end(ap);
+	fprintf(stderr, "\n\n");
+	fflush(NULL);
+	exit(-1);
+}
+
+static void *xalloc(size_t n)
+{
+	void *p = malloc(n ? n : 1);
+	if (!p) {
+		fprintf(stderr, "xmalloc: can not malloc %zu bytes\n", n);
+		exit(1);
+	}
+	return p;
+}
+
+/* --------------------------------------------------------- option parse */
+
+/* Remove "-o VALUE" from argv and return VALUE, or the default.  Same
+ * contract as pick_option (pickopt.c:7-22): the scan starts at argv[0]
+ * and the last argument cannot be an option name. */
+static char *take_option(int *argc, char ***argv, const char *name, char *dflt)
+{
+	int n = *argc;
+	char **v = *argv;
+	int skip = dflt ? 1 : 0;
+	for (int i = 0; i < n - skip; i++) {
+		if (v[i][0] != '-' || strcmp(v[i] + 1, name)) continue;
+		char *val = skip ? v[i + 1] : v[i] + 1;
+		*argc -= skip + 1;
+		for (int j = i; j < n - skip; j++)
+			v[j] = v[j + skip + 1];
+		return val;
+	}
+	return dflt;
+}
+
+/* ------------------------------------------------------------ reading */
+
+typedef struct raw_image {
+	int w, h, pd;
+	float *px;      /* interleaved, w*h*pd */
+} raw_image;
+
+static FILE *open_in(const char *name)
+{
+	if (!strcmp(name, "-")) return stdin;
+	FILE *f = fopen(name, "rb");
+	if (!f) io_fail("can not open file \"%s\" in mode \"rb\"", name);
+	return f;
+}
+
+static void read_exact(FILE *f, void *buf, size_t n)
+{
+	if (fread(buf, 1, n, f) != n) io_fail("input file ended before expected");
+}
+
+/* sample types of NPY descr strings (iio.c:3266-3283) */
+enum { T_F4, T_F8, T_U1, T_U2, T_U4, T_U8, T_I1, T_I2, T_I4, T_I8 };
+
+static int npy_type(const char *d, int *bytes, int *cplx)
+{
+	*cplx = 0;
+	if (*d == '<' || *d == '>' || *d == '=' || *d == '|') d++;
+	static const struct { const char *s; int t, b, c; } tab[] = {
+		{"f8", T_F8, 8, 0}, {"f4", T_F4, 4, 0}, {"u1", T_U1, 1, 0},
+		{"u2", T_U2, 2, 0}, {"u4", T_U4, 4, 0}, {"u8", T_U8, 8, 0},
+		{"i1", T_I1, 1, 0}, {"i2", T_I2, 2, 0}, {"i4", T_I4, 4, 0},
+		{"i8", T_I8, 8, 0}, {"c8", T_F4, 4, 1}, {"c16", T_F8, 8, 1},
+		{"b1", T_I1, 1, 0}, {"B1", T_U1, 1, 0} };
+	for (size_t k = 0; k < sizeof tab / sizeof *tab; k++)
+		if (!strcmp(d, tab[k].s)) {
+			*bytes = tab[k].b;
+			*cplx = tab[k].c;
+			return tab[k].t;
+		}
+	return -1;
+}
+
+static float sample_to_float(const unsigned char *p, int type)
+{
+	switch (type) {
+	case T_F4: { float v; memcpy(&v, p, 4); return v; }
+	case T_F8: { double v; memcpy(&v, p, 8); return (float)v; }
+	case T_U1: return (float)p[0];
+	case T_U2: { uint16_t v; memcpy(&v, p, 2); return (float)v; }
+	case T_U4: { uint32_t v; memcpy(&v, p, 4); return (float)v; }
+	case T_U8: { uint64_t v; memcpy(&v, p, 8); return (float)v; }
+	case T_I1: return (float)(int8_t)p[0];
+	case T_I2: { int16_t v; memcpy(&v, p, 2); return (float)v; }
+	case T_I4: { int32_t v; memcpy(&v, p, 4); return (float)v; }
+	default: { int64_t v; memcpy(&v, p, 8); return (float)v; }
+	}
+}
+
+/* NPY 1.0 after the 4-byte magic "\x93NUM" (iio.c:3216-3295).  Like iio,
+ * the byte-order character is skipped, not honoured. */
+static int read_npy(FILE *f, raw_image *im, unsigned char **keep_raw,
+		int *raw_type)
+{
+	unsigned char s[6];
+	read_exact(f, s, 6);
+	if (s[2] != 1 || s[3] != 0) io_fail("only NPY 1.0 is supported");
+	int hl = s[4] + 256 * s[5];
+	char *hdr = xalloc(hl + 1);
+	read_exact(f, hdr, hl);
+	hdr[hl] = 0;
+	char descr[16] = {0}, order[16] = {0};
+	int dims[3] = {0, 0, 0}, nd = 0;
+	const char *d = strstr(hdr, "'descr'"), *o = strstr(hdr, "'fortran_order'"),
+		*sh = strstr(hdr, "'shape'");
+	if (!d || !o || !sh) io_fail("badly formed npy header");
+	if (sscanf(d, "'descr': '%15[^']'", descr) != 1) io_fail("badly formed npy header");
+	if (sscanf(o, "'fortran_order': %15[^,}]", order) != 1) io_fail("badly formed npy header");
+	const char *par = strchr(sh, '(');
+	if (!par) io_fail("badly formed npy header");
+	par++;
+	while (nd < 3) {
+		while (*par == ' ') par++;
+		if (*par == ')' || !isdigit((unsigned char)*par)) break;
+		dims[nd++] = (int)strtol(par, (char **)&par, 10);
+		while (*par == ' ') par++;
+		if (*par == ',') par++;
+	}
+	free(hdr);
+	if (nd < 1) io_fail("badly formed npy header");
+	int h = dims[0], w = nd > 1 ? dims[1] : 1, pd = nd > 2 ? dims[2] : 1;
+	if (nd < 2) { w = dims[0]; h = 1; }
+	if (h == 1 && w > 1 && pd > 1) { h = w; w = pd; pd = 1; }   /* :3240-3245 */
+	int fortran = order[0] == 'T';
+	if (fortran) { int t = h; h = w; w = t; }
+	int bytes, cplx, type = npy_type(descr, &bytes, &cplx);
+	if (type < 0) io_fail("unrecognized npy type \"%s\"", descr);
+	if (cplx) pd *= 2;
+	size_t n = (size_t)w * h * pd;
+	unsigned char *raw = xalloc(n * bytes);
+	size_t got = fread(raw, bytes, n, f);
+	if (got != n) {
+		fprintf(stderr, "IIO WARNING: npy file smaller than expected\n");
+		memset(raw + got * bytes, 0, (n - got) * bytes);
+	}
+	if (fortran) {   /* stored column-major: transpose to row-major */
+		unsigned char *t = xalloc(n * bytes);
+		for (int j = 0; j < h; j++)
+		for (int i = 0; i < w; i++)
+			memcpy(t + ((size_t)j * w + i) * pd * bytes,
+				raw + ((size_t)i * h + j) * pd * bytes, (size_t)pd * bytes);
+		free(raw);
+		raw = t;
+	}
+	im->w = w; im->h = h; im->pd = pd;
+	im->px = xalloc(n * sizeof(float));
+	for (size_t k = 0; k < n; k++)
+		im->px[k] = sample_to_float(raw + k * bytes, type);
+	if (keep_raw) {   /* keep raw samples for the mask conversion */
+		*raw_type = type;
+		*keep_raw = raw;
+	} else {
+		free(raw);
+	}
+	return 0;
+}
+
+/* PFM: "Pf" (1 channel) or "PF" (3 channels), "w h\nscale\n", float rows.
+ * iio.c:2599-2616 declares every PFM as 2-channel, which misreads both
+ * kinds; this reader follows the header, as the matching PFM writer
+ * (iio.c:4151-4164) intends. */
+static void read_pfm(FILE *f, int upper, raw_image *im)
+{
+	int w, h;
+	float scale;
+	int c = fgetc(f);
+	if (!isspace(c)) io_fail("bad PFM header");
+	if (fscanf(f, "%d %d\n%g", &w, &h, &scale) != 3) io_fail("bad PFM header");
+	c = fgetc(f);
+	if (!isspace(c)) io_fail("bad PFM header");
+	int pd = upper ? 3 : 1;
+	size_t n = (size_t)w * h * pd;
+	im->w = w; im->h = h; im->pd = pd;
+	im->px = xalloc(n * sizeof(float));
+	read_exact(f, im->px, n * sizeof(float));
+}
+
+static void read_any(const char *name, raw_image *im, unsigned char **raw,
+		int *raw_type)
+{
+	FILE *f = open_in(name);
+	unsigned char m[4];
+	read_exact(f, m, 2);
+	if (m[0] == 'P' && (m[1] == 'f' || m[1] == 'F')) {
+		read_pfm(f, m[1] == 'F', im);
+		if (raw) *raw = NULL;
+	} else {
+		read_exact(f, m + 2, 2);
+		if (m[0] != 0x93 || m[1] != 'N' || m[2] != 'U' || m[3] != 'M')
+			io_fail("unsupported image format (NPY and PFM are supported)");
+		read_npy(f, im, raw, raw_type);
+	}
+	if (f != stdin) fclose(f);
+}
+
+/* planar float image, iio_read_image_float_split (iio.c:5763-5771) */
+static float *read_split(const char *name, int *w, int *h, int *pd)
+{
+	raw_image im;
+	read_any(name, &im, NULL, NULL);
+	size_t n = (size_t)im.w * im.h;
+	float *out = xalloc(n * im.pd * sizeof(float));
+	for (size_t i = 0; i < n; i++)
+		for (int l = 0; l < im.pd; l++)
+			out[n * l + i] = im.px[im.pd * i + l];
+	free(im.px);
+	*w = im.w; *h = im.h; *pd = im.pd;
+	return out;
+}
+
+/* gray value of a colour sample triple, computed in double and stored in
+ * the sample type, as iio_hacky_uncolorize does (iio.c:1230-1300) */
+static float gray_of(const unsigned char *raw, int type, size_t k, int pd)
+{
+	int bytes = type == T_F8 || type == T_U8 || type == T_I8 ? 8
+		: type == T_F4 || type == T_U4 || type == T_I4 ? 4
+		: type == T_U2 || type == T_I2 ? 2 : 1;
+	double a = sample_to_float(raw + (k * pd + 0) * bytes, type);
+	double b = sample_to_float(raw + (k * pd + 1) * bytes, type);
+	double c = sample_to_float(raw + (k * pd + 2) * bytes, type);
+	double v = .299 * a + .587 * b + .114 * c;
+	switch (type) {
+	case T_U1: return (float)(uint8_t)v;
+	case T_U2: return (float)(uint16_t)v;
+	case T_F4: return (float)v;
+	default: io_fail("uncolorize type not supported");
+	}
+	return 0;
+}
+
+/* scalar mask, iio_read_image_float (iio.c:6029-6048) */
+static float *read_mask(const char *name, int *w, int *h)
+{
+	raw_image im;
+	unsigned char *raw = NULL;
+	int type = T_F4;
+	read_any(name, &im, &raw, &type);
+	size_t n = (size_t)im.w * im.h;
+	float *out = xalloc(n * sizeof(float));
+	if (im.pd == 1) {
+		memcpy(out, im.px, n * sizeof(float));
+	} else if (im.pd == 3 || im.pd == 4) {
+		for (size_t i = 0; i < n; i++) {
+			if (raw) out[i] = gray_of(raw, type, i, im.pd);
+			else out[i] = (float)(.299 * im.px[im.pd * i]
+					+ .587 * im.px[im.pd * i + 1]
+					+ .114 * im.px[im.pd * i + 2]);
+		}
+	} else {
+		io_fail("could not read image");
+	}
+	free(raw);
+	free(im.px);
+	*w = im.w; *h = im.h;
+	return out;
+}
+
+/* ------------------------------------------------------------ writing */
+
+static int has_suffix(const char *s, const char *suf)
+{
+	size_t a = strlen(s), b = strlen(suf);
+	return a >= b && !strcmp(s + a - b, suf);
+}
+
+static FILE *open_out(const char *name)
+{
+	if (!strcmp(name, "-")) return stdout;
+	FILE *f = fopen(name, "wb");
+	if (!f) io_fail("can not open file \"%s\" in mode \"w\"", name);
+	return f;
+}
+
+/* NPY header exactly as iio_write_image_as_npy lays it out
+ * (iio.c:4337-4373): padded with spaces to a multiple of 16, ending "\n" */
+static void write_npy(const char *name, const float *px, int w, int h, int pd)
+{
+	char buf[1000];
+	buf[0] = (char)0x93;
+	memcpy(buf + 1, "NUMPY", 5);
+	buf[6] = 1; buf[7] = 0;
+	int n = 10;
+	n += snprintf(buf + n, sizeof buf - n,
+		"{'descr': '<f4', 'fortran_order': False, 'shape': (%d, %d, %d)}",
+		h, w, pd);
+	int m = ((n + 15 + 1) / 16) * 16;
+	buf[8] = (char)((m - 10) % 256);
+	buf[9] = (char)((m - 10) / 256);
+	for (int i = n; i < m - 1; i++) buf[i] = ' ';
+	buf[m - 1] = '\n';
+	FILE *f = open_out(name);
+	fwrite(buf, 1, m, f);
+	fwrite(px, sizeof(float), (size_t)w * h * pd, f);
+	if (f != stdout) fclose(f); else fflush(f);
+}
+
+static void write_pfm(const char *name, const float *px, int w, int h, int pd)
+{
+	FILE *f = open_out(name);
+	fprintf(f, "P%c\n%d %d\n%g\n", pd > 1 ? 'F' : 'f', w, h, -1.0f);
+	fwrite(px, sizeof(float), (size_t)w * h * pd, f);
+	if (f != stdout) fclose(f); else fflush(f);
+}
+
+/* iio_write_image_float_split (iio.c:6525-6531) and the writer dispatch of
+ * iio_write_image_default (iio.c:6166-6497) for the formats kept here */
+static void write_split(const char *name, const float *planar, int w, int h, int pd)
+{
+	size_t n = (size_t)w * h;
+	if (!strcmp(name, "-") && isatty(1)) {
+		printf("IMAGE %dx%d,%d %s\n", w, h, pd, "FLOAT");
+		return;
+	}
+	float *px = xalloc(n * pd * sizeof(float));
+	for (int l = 0; l < pd; l++)
+		for (size_t i = 0; i < n; i++)
+			px[pd * i + l] = planar[n * l + i];
+	if (has_suffix(name, ".pfm") && (pd == 1 || pd == 3))
+		write_pfm(name, px, w, h, pd);
+	else
+		write_npy(name, px, w, h, pd);
+	free(px);
+}
+
+/* --------------------------------------------------------------- help */
+
+/* The help texts are the reference's (simpois.c:307-347), kept verbatim
+ * because they are part of the command-line interface. */
+static const char *help_usage = "usage:\n\t"
+"simpois [options] [-m mask.png] [-f data.npy] [-i in.npy] [-o out.npy]";
+static const char *help_oneliner = "a simple Poisson solver for images";
+static const char *help_version = "simpois 1.0\n\nWritten by mnhrdt";
+static const char *help_long =
+"Simpois is a simple Poisson solver for image processing.\n"
+"\n"
+"This program can solve Poisson, Laplace and Biharmonic equations\n"
+"over an arbitrary part of the image domain, with arbitrary data terms\n"
+"and boundary conditions.  Fast approximations to the solution can be\n"
+"obtained by setting the parameters.  Solving over the whole image domain\n"
+"is discouraged, as it is always better to use fft-based methods.\n"
+"\n"
+"Usage: simpois [options] -i in.npy -o out.npy\n"
+"   or: simpois [options] -i in.npy > out.npy\n"
+"   or: cat in.npy | simpois [options] -o out.npy\n"
+"   or: cat in.npy | simpois [options] > out.npy\n"
+"\n"
+"Options:\n"
+" -i IMG\tInput image with boundary data (default=stdin)\n"
+" -f IMG\tPoisson data term (default=zeros)\n"
+" -m IMG\tMask for the region of interest (default=zeros)\n"
+" -o IMG\tOutput image (default=stdout)\n"
+" -t 0.25\tTime step for Gauss-Seidel iterations (negative=biharmonic)\n"
+" -n 10\tNumber of Gauss-Seidel iterations\n"
+" -s 99\tMaximum number of multi-scale octaves\n"
+" -c 0\tNumber of Conjugate Gradient iterations\n"
+" -h\tdisplay short help message\n"
+" --help\tdisplay longer help message\n"
+"\n"
+"Examples:\n"
+" cat in.npy | simpois > out.npy           Fill NANs by Laplace equation\n"
+" cat in.npy | simpois -n 1 > out.npy      Fill NANs, fast (one iteration)\n"
+" cat in.npy | simpois -t -0.08 > out.npy  Fill NANs, smooth (Biharmonic)\n"
+" simpois -i in.npy -o out.npy             Laplace, with explicit data\n"
+" simpois -m mask.png ...                  Use mask instead of NANs\n"
+" simpois -f lap.npy ...                   Poisson editor\n"
+"\n"
+"Report bugs to <enric.meinhardt@ens-paris-saclay.fr>.";
+
+/* help flags, help_stuff.c:23-34 (the --man* pages need help2man and are
+ * not provided) */
+static void maybe_help(const char *s)
+{
+	if (!s || !strcmp(s, "-h")) { puts(help_usage); exit(0); }
+	if (!strcmp(s, "-?")) { puts(help_oneliner); exit(0); }
+	if (!strcmp(s, "--help")) { puts(help_long); exit(0); }
+	if (!strcmp(s, "--version")) { puts(help_version); exit(0); }
+	if (!strncmp(s, "--man", 5)) {
+		fprintf(stderr, "simpois: man pages are not available\n");
+		exit(1);
+	}
+	if (!strcmp(s, "--help-oneliner")) puts(help_oneliner);
+}
+
+/* --------------------------------------------------------------- main */
+
+int main_simpois(int argc, char *argv[])
+{
+	static char cmd[1024];
+	cmd[0] = 0;
+	for (int i = 0; i < argc; i++) {
+		if (i) strncat(cmd, " ", sizeof cmd - strlen(cmd) - 1);
+		strncat(cmd, argv[i], sizeof cmd - strlen(cmd) - 1);
+	}
+	g_cmdline = cmd;
+	if (argc == 2) maybe_help(argv[1]);
+
+	float tstep = atof(take_option(&argc, &argv, "t", "0.25"));
+	float niter = atof(take_option(&argc, &argv, "n", "10"));
+	float nscal = atof(take_option(&argc, &argv, "s", "99"));
+	float cgrad = atof(take_option(&argc, &argv, "c", "0"));
+	char *fi = take_option(&argc, &argv, "i", "-");
+	char *fo = take_option(&argc, &argv, "o", "-");
+	char *fm = take_option(&argc, &argv, "m", "");
+	char *ff = take_option(&argc, &argv, "f", "");
+	if (!strcmp(ff, "-") && !strcmp(fi, "-"))
+		fi = "NAN";
+
+	if (argc > 1) {   /* simpois.c:369-386 */
+		fprintf(stderr, "Usage:\n\t%s [options]\n", *argv);
+		fprintf(stderr, "\nComputes approximate solutions of Poisson"
+			" or Laplace equations for images\n");
+		fprintf(stderr, "\nOptions with their default values:\n"
+			"\t-i stdin   Input image with boundary data\n"
+			"\t-f (zeros) Optional image with Poisson data term\n"
+			"\t-m (zeros) Optional image with region of interest\n"
+			"\t-o stdout  Output image\n"
+			"\t-t 0.25    Time step for Gauss-Seidel iterations\n"
+			"\t-n 10      Number of Gauss-Seidel iterations\n"
+			"\t-s 99      Number of Multi-Scale octaves\n"
+			"\t-c 0       Number of Conjugate Gradient iterations\n"
+		);
+		fprintf(stderr, "\nNote: NAN values in the input image"
+				" are added to the region of interest\n");
+		return 1;
+	}
+
+	int w = 0, h = 0, pd = 0;
+	float *img_i = NULL;
+	if (*fi && strcmp(fi, "NAN"))
+		img_i = read_split(fi, &w, &h, &pd);
+
+	float *img_f = NULL;
+	if (*ff) {
+		int w2, h2, pd2;
+		img_f = read_split(ff, &w2, &h2, &pd2);
+		if (!img_i) {
+			w = w2; h = h2; pd = pd2;
+			img_i = xalloc((size_t)w * h * pd * sizeof(float));
+			for (size_t k = 0; k < (size_t)w * h * pd; k++) img_i[k] = NAN;
+		} else if (w != w2 || h != h2 || pd != pd2) {
+			return fprintf(stderr, "input sizes mismatch (i,f)");
+		}
+	}
+	if (!img_i) io_fail("could not read image");
+
+	float *img_m = NULL;
+	if (*fm) {
+		int w2, h2;
+		img_m = read_mask(fm, &w2, &h2);
+		if (w != w2 || h != h2)
+			return fprintf(stderr, "input sizes mismatch (i,m)");
+	}
+
+	float *out = xalloc((size_t)w * h * pd * sizeof(float));
+	if (img_m)   /* mask > 0 makes a hole in every channel, :421-424 */
+		for (size_t k = 0; k < (size_t)w * h * pd; k++)
+			if (img_m[k % ((size_t)w * h)] > 0)
+				img_i[k] = NAN;
+
+	poisson_solver_separable(out, img_i, img_f, w, h, pd,
+			tstep, niter, nscal, cgrad);
+
+	write_split(fo, out, w, h, pd);
+	free(out);
+	free(img_i);
+	free(img_m);
+	free(img_f);
+	return 0;
+}

@@ -1,0 +1,268 @@
+// Gauss-Seidel tile geometry and per-thread phases, shared by the device
+// kernels and by the CPU emulator used in the tests (the same code, so the
+// CPU run checks the device kernels' indexing and schedule).
+//
+// Gauss-Seidel (simpois.c:93-119) updates the NaN pixels in raster order,
+// in place.  The update of pixel (i,j) at sweep k only depends on updates
+// with a smaller value of s = i + R*j + (R*R+1)*k (R = stencil radius), so
+// all updates on one hyperplane s are independent and hyperplanes executed
+// in increasing s reproduce the raster order bit for bit.
+//
+// Skewed coordinates (i', j') and the tile box [I'BI, +BI) x [J'BJ, +BJ) x
+// [T'BT, +BT) of (i', j', k):
+//   R = 1: i' = i + k,                 j' = j + k      in-tile step a + b
+//   R > 1: i' = i + (R-1) j + R(R-1) k, j' = j + R k   in-tile step a + b + tau
+// Every dependence between updates (true, and the anti-dependence of the
+// in-place update) has a componentwise non-negative skewed distance, so a
+// tile only depends on componentwise-smaller tiles and tiles with equal
+// I'+J'+T' are independent.  A tile loads its whole footprint from the
+// in-place image, runs its hyperplanes in LDS and writes back the pixels
+// it updated; the image then always holds the version each tile needs.
+#pragma once
+#include <stdint.h>
+#include <string.h>
+#include <math.h>
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define SP_HD __host__ __device__ __forceinline__
+#else
+#define SP_HD inline
+#endif
+
+#define SP_KNOWN_BITS 0x7fbadbadu
+#define SP_QNAN_BITS 0x7fc00000u
+
+// Gauss-Seidel work item: one plane with its sweep count and effective step.
+struct GsInst {
+	float *u;
+	const float *c;
+	int w, h, nsweep;
+	float t;
+	const int *active;   // optional per-instance flag (nullptr = always on)
+};
+
+// Skewed temporal tile: (instance, I', J', T').
+struct GsTile { int inst, ti, tj, tt; };
+
+// Tile shapes (skewed extents and sweeps per tile) per stencil radius.
+template <int R> struct TileShape;
+template <> struct TileShape<1> { static constexpr int BI = 32, BJ = 128, BT = 16; };
+template <> struct TileShape<2> { static constexpr int BI = 32, BJ = 32, BT = 8; };
+template <> struct TileShape<3> { static constexpr int BI = 32, BJ = 32, BT = 4; };
+
+SP_HD uint32_t sp_bits(float x) { return __builtin_bit_cast(uint32_t, x); }
+SP_HD float sp_float(uint32_t b) { return __builtin_bit_cast(float, b); }
+SP_HD bool sp_known(float c) { return sp_bits(c) == SP_KNOWN_BITS; }
+SP_HD int sp_clamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// coefficient of a pixel: the data term at a hole (NaN made canonical so it
+// can never equal the known marker), the known marker elsewhere
+SP_HD float sp_coef(float g, float d)
+{
+	if (!isnan(g)) return sp_float(SP_KNOWN_BITS);
+	return isnan(d) ? sp_float(SP_QNAN_BITS) : d;
+}
+
+// Nested clamped Laplacian of depth D centred at offset (CX, CY) inside a
+// register window v of side N = 2R+1: -4*c + E + S + W + N, left to right
+// (simpois.c:35-68).  Depth 0 is the pixel value itself.
+template <int D, int CX, int CY, int N> struct NestedLap {
+	static SP_HD float eval(const float (&v)[N][N])
+	{
+		float r = -4.0f * NestedLap<D - 1, CX, CY, N>::eval(v);
+		r = r + NestedLap<D - 1, CX + 1, CY, N>::eval(v);
+		r = r + NestedLap<D - 1, CX, CY + 1, N>::eval(v);
+		r = r + NestedLap<D - 1, CX - 1, CY, N>::eval(v);
+		r = r + NestedLap<D - 1, CX, CY - 1, N>::eval(v);
+		return r;
+	}
+};
+template <int CX, int CY, int N> struct NestedLap<0, CX, CY, N> {
+	static SP_HD float eval(const float (&v)[N][N]) { return v[CY + N / 2][CX + N / 2]; }
+};
+
+// One in-place update of hole pixel (i,j) whose value sits at U[idx];
+// neighbour (di,dj) sits at U[idx + dj*RS + di].
+//  R = 1: Laplacian that skips non-finite neighbours (simpois.c:72-90).
+//         A clamped neighbour is the pixel itself and adds (x - x) = +0 or
+//         is skipped, which leaves the sum unchanged, so it is left out.
+//  R = 2, 3: nested clamped Laplacians (simpois.c:48-68); every read is of
+//         the clamped coordinate, so the window is read with clamping.
+template <int R>
+SP_HD void gs_update(float *U, float cv, int idx, int i, int j, int W, int H,
+		int RS, float t)
+{
+	if constexpr (R == 1) {
+		const float x = U[idx];
+		float r = 0.0f;
+		if (i + 1 < W) { float v = U[idx + 1]; if (isfinite(v)) r = r + (v - x); }
+		if (j + 1 < H) { float v = U[idx + RS]; if (isfinite(v)) r = r + (v - x); }
+		if (i > 0) { float v = U[idx - 1]; if (isfinite(v)) r = r + (v - x); }
+		if (j > 0) { float v = U[idx - RS]; if (isfinite(v)) r = r + (v - x); }
+		U[idx] = x + t * (r - cv);
+	} else {
+		constexpr int N = 2 * R + 1;
+		float v[N][N];
+#pragma unroll
+		for (int dy = -R; dy <= R; dy++) {
+			const int cj = sp_clamp(j + dy, 0, H - 1) - j;
+#pragma unroll
+			for (int dx = -R; dx <= R; dx++) {
+				if ((dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy) > R) {
+					v[dy + R][dx + R] = 0.0f;
+					continue;
+				}
+				const int ci = sp_clamp(i + dx, 0, W - 1) - i;
+				v[dy + R][dx + R] = U[idx + cj * RS + ci];
+			}
+		}
+		const float l = NestedLap<R, 0, 0, N>::eval(v);
+		U[idx] = v[R][R] + t * (l - cv);
+	}
+}
+
+// ------------------------------------------------------------- tiles
+
+template <int R> struct Geo {
+	static constexpr int BI = TileShape<R>::BI, BJ = TileShape<R>::BJ,
+		BT = TileShape<R>::BT;
+	static constexpr int SH = R - 1;                       // col shear per row
+	static constexpr int HC = R == 1 ? 1 : R * (R - 1);    // column halo
+	static constexpr int LW0 = R == 1 ? BI + BT + 1 : BI + R * (R - 1) * (BT - 1) + 2 * HC;
+	static constexpr int LW = LW0 + (LW0 & 1);             // even: odd diagonal stride
+	static constexpr int LH = R == 1 ? BJ + BT + 1 : BJ + R * (BT - 1) + 2 * R;
+	static constexpr int NT = BI * BT;                      // threads per tile
+	static constexpr int NSTEP = R == 1 ? BI + BJ - 1 : BI + BJ + BT - 2;
+	static constexpr int LDS_FLOATS = 2 * LW * LH;          // u and coefficient
+};
+
+// placement of one tile: LDS cell (row, col) holds pixel
+// j = row + jlo, i = col + K - SH*j
+template <int R> struct TileCtx {
+	int x0, y0, jlo, K, bte, W, H;
+	float t;
+	float *U, *C;
+};
+
+template <int R>
+SP_HD TileCtx<R> tile_ctx(const GsTile &tl, const GsInst &in, float *lds)
+{
+	using G = Geo<R>;
+	TileCtx<R> c;
+	const int k0 = tl.tt * G::BT;
+	c.bte = in.nsweep - k0 < G::BT ? in.nsweep - k0 : G::BT;
+	if (R == 1) {
+		c.x0 = tl.ti * G::BI - k0;
+		c.y0 = tl.tj * G::BJ - k0;
+		c.jlo = c.y0 - G::BT;
+		c.K = c.x0 - G::BT;
+	} else {
+		c.x0 = tl.ti * G::BI - G::SH * tl.tj * G::BJ;
+		c.y0 = tl.tj * G::BJ - R * k0;
+		c.jlo = c.y0 - R * (G::BT - 1) - R;
+		c.K = c.x0 + G::SH * c.y0 - R * G::SH * (G::BT - 1) - G::HC;
+	}
+	c.W = in.w;
+	c.H = in.h;
+	c.t = in.t;
+	c.U = lds;
+	c.C = lds + G::LW * G::LH;
+	return c;
+}
+
+// phase 1 (thread tid): copy the tile's footprint into LDS
+template <int R>
+SP_HD void tile_load(const TileCtx<R> &c, const GsInst &in, int tid)
+{
+	using G = Geo<R>;
+	for (int e = tid; e < G::LW * G::LH; e += G::NT) {
+		const int row = e / G::LW, col = e - row * G::LW;
+		const int j = row + c.jlo, i = col + c.K - G::SH * j;
+		if (i >= 0 && i < c.W && j >= 0 && j < c.H) {
+			const size_t q = (size_t)j * c.W + i;
+			c.U[e] = in.u[q];
+			c.C[e] = in.c[q];
+		}
+	}
+}
+
+// phase 2 (thread tid, step sg): the update of thread (a, tau) on the
+// in-tile hyperplane sg; steps run in increasing sg with a barrier between
+template <int R>
+SP_HD void tile_step(const TileCtx<R> &c, int tid, int sg)
+{
+	using G = Geo<R>;
+	const int a = tid % G::BI, tau = tid / G::BI;
+	if (tau >= c.bte) return;
+	const int b = R == 1 ? sg - a : sg - a - tau;
+	if (b < 0 || b >= G::BJ) return;
+	int i, j;
+	if (R == 1) { i = c.x0 + a - tau; j = c.y0 + b - tau; }
+	else { i = c.x0 + a - G::SH * b; j = c.y0 + b - R * tau; }
+	if (i < 0 || i >= c.W || j < 0 || j >= c.H) return;
+	const int idx = (j - c.jlo) * G::LW + (i + G::SH * j - c.K);
+	const float cv = c.C[idx];
+	if (!sp_known(cv))
+		gs_update<R>(c.U, cv, idx, i, j, c.W, c.H, G::LW + G::SH, c.t);
+}
+
+// does this tile update pixel (i, j) at some sweep?
+template <int R>
+SP_HD bool tile_owns(const TileCtx<R> &c, int i, int j)
+{
+	using G = Geo<R>;
+	for (int tt = 0; tt < c.bte; tt++) {
+		int aa, bb;
+		if (R == 1) { aa = i - c.x0 + tt; bb = j - c.y0 + tt; }
+		else { bb = j - c.y0 + R * tt; aa = i - c.x0 + G::SH * bb; }
+		if (aa >= 0 && aa < G::BI && bb >= 0 && bb < G::BJ) return true;
+	}
+	return false;
+}
+
+// phase 3 (thread tid): write back the hole pixels this tile updated
+template <int R>
+SP_HD void tile_store(const TileCtx<R> &c, const GsInst &in, int tid)
+{
+	using G = Geo<R>;
+	for (int e = tid; e < G::LW * G::LH; e += G::NT) {
+		const int row = e / G::LW, col = e - row * G::LW;
+		const int j = row + c.jlo, i = col + c.K - G::SH * j;
+		if (i < 0 || i >= c.W || j < 0 || j >= c.H || sp_known(c.C[e])) continue;
+		if (tile_owns<R>(c, i, j)) in.u[(size_t)j * c.W + i] = c.U[e];
+	}
+}
+
+// ------------------------------------------ whole level in one work-group
+
+// For levels that fit in LDS: one block runs every sweep of one plane,
+// hyperplane by hyperplane.  Step s, thread slot e: (row j, k-offset qk).
+template <int R>
+SP_HD void small_step(float *U, const float *C, int W, int H, int T, float t,
+		long s, int tid, int nthreads)
+{
+	constexpr int CT = R * R + 1;
+	const int KT = (W - 1) / CT + 1;     // sweeps meeting one row on a plane
+	const int slots = H * KT;
+	for (int e = tid; e < slots; e += nthreads) {
+		const int j = e % H, qk = e / H;
+		const long base = s - (long)R * j;   // = i + CT*k
+		if (base < 0) continue;
+		long k = base / CT;
+		if (k > T - 1) k = T - 1;
+		k -= qk;
+		if (k < 0) continue;
+		const long i = base - CT * k;
+		if (i >= W) continue;
+		const int idx = j * W + (int)i;
+		const float cv = C[idx];
+		if (!sp_known(cv))
+			gs_update<R>(U, cv, idx, (int)i, j, W, H, W, t);
+	}
+}
+
+SP_HD long small_nsteps(int R, int W, int H, int T)
+{
+	return (long)(W - 1) + (long)R * (H - 1) + (long)(R * R + 1) * (T - 1) + 1;
+}

@@ -1,0 +1,63 @@
+// Internal interface between the host driver (sp_host.hip) and the device
+// kernels (sp_kernels.hip).  Not part of the public C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sp_gs.h"
+
+// One plane at one pyramid level.
+struct SpPlane {
+	float *g;        // boundary data, NaN = hole
+	float *f;        // data term or nullptr
+	float *u;        // solution (in place)
+	float *c;        // coefficient: data term at holes, SP_KNOWN_BITS elsewhere
+	int *cnt;        // number of holes (device counter)
+	int w, h;
+};
+
+// Work block for elementwise kernels over many planes of uneven size.
+struct BlockWork { int plane; int start; };
+
+namespace spk {
+
+// elementwise / pyramid
+void launch_prepare(const SpPlane *planes, const BlockWork *bw, int nblocks,
+		hipStream_t s);
+void launch_restrict(const SpPlane *fine, const SpPlane *coarse,
+		const BlockWork *bw, int nblocks, double pmsfac, hipStream_t s);
+void launch_init(const SpPlane *lv, const SpPlane *coarse, const BlockWork *bw,
+		int nblocks, hipStream_t s);
+void launch_finalize(const SpPlane *lv, float *const *outs,
+		const BlockWork *bw, int nblocks, hipStream_t s);
+
+// Gauss-Seidel
+int tile_lds_bytes(int R);
+int small_lds_bytes(int w, int h);
+void launch_gs_tiles(int R, const GsTile *tiles, int ntiles, const GsInst *insts,
+		hipStream_t s);
+void launch_gs_small(int R, const GsInst *insts, const int *ids, int nids,
+		int lds_bytes, int threads, hipStream_t s);
+
+// CG (FP64 vectors).  Full-grid vectors, zero at known pixels.
+struct CgDev {
+	const float *g, *f;   // plane data
+	float *u;             // solution plane (read for x0, written at the end)
+	double *x, *r, *p, *q;
+	double *part;         // per-block partial sums (2 * nblk)
+	double *rr;           // rr history, rr[0] = r0.r0, rr[k+1] = after iter k
+	int *done;            // [0] = stop flag, [1] = iterations run
+	int w, h, bilap, nblk;
+	double stop;          // absolute stop threshold on sqrt(rr)
+};
+void launch_cg_setup(const CgDev &d, hipStream_t s);
+void launch_cg_iter(const CgDev &d, int it, hipStream_t s);
+void launch_cg_finish(const CgDev &d, hipStream_t s);
+
+// FP64 relative residual of the Poisson (bilap=0) or biharmonic system
+void launch_residual(const SpPlane *lv, const BlockWork *bw, int nblocks,
+		int bilap, double *part, hipStream_t s);
+void launch_residual_reduce(const double *part, const int *plane_first,
+		int nplanes, double *out, hipStream_t s);
+
+} // namespace spk

@@ -1,0 +1,562 @@
+// Device kernels for the simpois hole-filling path on gfx950 (CDNA4).
+//
+// All arithmetic that the reference performs in float is performed here in
+// float with the same association order; the library is compiled with
+// -ffp-contract=off so no multiply-add is fused.  That, plus the hyperplane
+// schedule below, makes the Gauss-Seidel sweeps and the pyramid bit-exact
+// with simpois.c.
+//
+// The Gauss-Seidel schedule and its correctness argument are in sp_gs.h.
+#include "sp_internal.h"
+#include <math.h>
+
+namespace spk {
+
+static __device__ __forceinline__ float coef_of(float g, const float *f, size_t q)
+{
+	return sp_coef(g, f && isnan(g) ? f[q] : 0.0f);
+}
+
+static __device__ __forceinline__ int clampi(int v, int lo, int hi)
+{
+	return v < lo ? lo : (v > hi ? hi : v);
+}
+
+static __device__ __forceinline__ float at(const float *a, int w, int h, int i, int j)
+{
+	return a[(size_t)clampi(j, 0, h - 1) * w + clampi(i, 0, w - 1)];
+}
+
+// block-wide sum of an int, result valid in thread 0
+template <int NT>
+static __device__ int block_sum_int(int v, int *red)
+{
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+	int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	if (lane == 0) red[wv] = v;
+	__syncthreads();
+	int r = 0;
+	if (threadIdx.x == 0)
+		for (int k = 0; k < NT / 64; k++) r += red[k];
+	__syncthreads();
+	return r;
+}
+
+// block-wide sum of a double (fixed tree), result valid in thread 0
+template <int NT>
+static __device__ double block_sum_dbl(double v, double *red)
+{
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+	int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	if (lane == 0) red[wv] = v;
+	__syncthreads();
+	double r = 0;
+	if (threadIdx.x == 0)
+		for (int k = 0; k < NT / 64; k++) r += red[k];
+	__syncthreads();
+	return r;
+}
+
+constexpr int EW_NT = 256;   // threads per elementwise block
+constexpr int EW_PPT = 4;    // pixels per thread
+constexpr int EW_SPAN = EW_NT * EW_PPT;
+
+// ------------------------------------------------------------ pyramid ops
+
+// finest level: coefficient plane and hole count
+__global__ __launch_bounds__(EW_NT) void k_prepare(const SpPlane *planes,
+		const BlockWork *bw)
+{
+	__shared__ int red[EW_NT / 64];
+	const BlockWork b = bw[blockIdx.x];
+	const SpPlane P = planes[b.plane];
+	const size_t n = (size_t)P.w * P.h;
+	int holes = 0;
+	for (int k = 0; k < EW_PPT; k++) {
+		size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
+		if (q >= n) break;
+		float g = P.g[q];
+		holes += isnan(g) ? 1 : 0;
+		P.c[q] = coef_of(g, P.f, q);
+	}
+	int s = block_sum_int<EW_NT>(holes, red);
+	if (threadIdx.x == 0 && s) atomicAdd(P.cnt, s);
+}
+
+// 2x2 NaN-aware mean of g (and f, times pmsfac), simpois.c:172-196,260-262
+__global__ __launch_bounds__(EW_NT) void k_restrict(const SpPlane *fine,
+		const SpPlane *coarse, const BlockWork *bw, double pmsfac)
+{
+	__shared__ int red[EW_NT / 64];
+	const BlockWork b = bw[blockIdx.x];
+	const SpPlane F = fine[b.plane], C = coarse[b.plane];
+	const size_t n = (size_t)C.w * C.h;
+	int holes = 0;
+	for (int k = 0; k < EW_PPT; k++) {
+		size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
+		if (q >= n) break;
+		int i = (int)(q % C.w), j = (int)(q / C.w);
+		float v[4], m = 0;
+		int cnt = 0;
+		v[0] = at(F.g, F.w, F.h, 2 * i, 2 * j);
+		v[1] = at(F.g, F.w, F.h, 2 * i + 1, 2 * j);
+		v[2] = at(F.g, F.w, F.h, 2 * i, 2 * j + 1);
+		v[3] = at(F.g, F.w, F.h, 2 * i + 1, 2 * j + 1);
+		for (int a = 0; a < 4; a++)
+			if (isfinite(v[a])) { m = m + v[a]; cnt++; }
+		float g = cnt ? m / (float)cnt : __uint_as_float(SP_QNAN_BITS);
+		C.g[q] = g;
+		if (C.f) {
+			v[0] = at(F.f, F.w, F.h, 2 * i, 2 * j);
+			v[1] = at(F.f, F.w, F.h, 2 * i + 1, 2 * j);
+			v[2] = at(F.f, F.w, F.h, 2 * i, 2 * j + 1);
+			v[3] = at(F.f, F.w, F.h, 2 * i + 1, 2 * j + 1);
+			m = 0; cnt = 0;
+			for (int a = 0; a < 4; a++)
+				if (isfinite(v[a])) { m = m + v[a]; cnt++; }
+			float fv = cnt ? m / (float)cnt : __uint_as_float(SP_QNAN_BITS);
+			C.f[q] = (float)((double)fv * pmsfac);
+		}
+		holes += isnan(g) ? 1 : 0;
+		C.c[q] = coef_of(g, C.f, q);
+	}
+	int s = block_sum_int<EW_NT>(holes, red);
+	if (threadIdx.x == 0 && s) atomicAdd(C.cnt, s);
+}
+
+// value of the coarse output at q: after a level, simpois writes the input
+// back at every non-NaN pixel when the level has holes (cleant_cgpois.c:251)
+static __device__ __forceinline__ float level_out(const SpPlane &P, size_t q)
+{
+	float g = P.g[q];
+	return (*P.cnt > 0 && !isnan(g)) ? g : P.u[q];
+}
+
+// u = isfinite(g) ? g : init, init = bilinear x2 of the coarse output
+// (simpois.c:159-160, 199-239) or 0 at the coarsest level
+__global__ __launch_bounds__(EW_NT) void k_init(const SpPlane *lv,
+		const SpPlane *coarse, const BlockWork *bw)
+{
+	const BlockWork b = bw[blockIdx.x];
+	const SpPlane P = lv[b.plane];
+	const size_t n = (size_t)P.w * P.h;
+	for (int k = 0; k < EW_PPT; k++) {
+		size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
+		if (q >= n) break;
+		float g = P.g[q];
+		float init = 0.0f;
+		if (coarse && !isfinite(g)) {
+			const SpPlane C = coarse[b.plane];
+			int i = (int)(q % P.w), j = (int)(q / P.w);
+			float x = (float)((i - 0.5) / 2), y = (float)((j - 0.5) / 2);
+			int ix = (int)floorf(x), iy = (int)floorf(y);
+			int x0 = clampi(ix, 0, C.w - 1), x1 = clampi(ix + 1, 0, C.w - 1);
+			int y0 = clampi(iy, 0, C.h - 1), y1 = clampi(iy + 1, 0, C.h - 1);
+			float a = level_out(C, (size_t)y0 * C.w + x0);
+			float bb = level_out(C, (size_t)y0 * C.w + x1);
+			float cc = level_out(C, (size_t)y1 * C.w + x0);
+			float d = level_out(C, (size_t)y1 * C.w + x1);
+			float fx = x - (float)ix, fy = y - (float)iy;
+			float r = 0.0f;
+			r = r + a * (1 - fx) * (1 - fy);
+			r = r + bb * fx * (1 - fy);
+			r = r + cc * (1 - fx) * fy;
+			r = r + d * fx * fy;
+			init = r;
+		}
+		P.u[q] = isfinite(g) ? g : init;
+	}
+}
+
+__global__ __launch_bounds__(EW_NT) void k_finalize(const SpPlane *lv,
+		float *const *outs, const BlockWork *bw)
+{
+	const BlockWork b = bw[blockIdx.x];
+	const SpPlane P = lv[b.plane];
+	float *out = outs[b.plane];
+	const size_t n = (size_t)P.w * P.h;
+	for (int k = 0; k < EW_PPT; k++) {
+		size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
+		if (q >= n) break;
+		out[q] = level_out(P, q);
+	}
+}
+
+void launch_prepare(const SpPlane *planes, const BlockWork *bw, int nblocks,
+		hipStream_t s)
+{
+	if (nblocks) k_prepare<<<nblocks, EW_NT, 0, s>>>(planes, bw);
+}
+
+void launch_restrict(const SpPlane *fine, const SpPlane *coarse,
+		const BlockWork *bw, int nblocks, double pmsfac, hipStream_t s)
+{
+	if (nblocks) k_restrict<<<nblocks, EW_NT, 0, s>>>(fine, coarse, bw, pmsfac);
+}
+
+void launch_init(const SpPlane *lv, const SpPlane *coarse, const BlockWork *bw,
+		int nblocks, hipStream_t s)
+{
+	if (nblocks) k_init<<<nblocks, EW_NT, 0, s>>>(lv, coarse, bw);
+}
+
+void launch_finalize(const SpPlane *lv, float *const *outs,
+		const BlockWork *bw, int nblocks, hipStream_t s)
+{
+	if (nblocks) k_finalize<<<nblocks, EW_NT, 0, s>>>(lv, outs, bw);
+}
+
+// ------------------------------------------------------------ GS kernels
+
+template <int R>
+__global__ __launch_bounds__(Geo<R>::NT) void k_gs_tile(const GsTile *tiles,
+		const GsInst *insts)
+{
+	using G = Geo<R>;
+	extern __shared__ float smem[];
+	const GsTile tl = tiles[blockIdx.x];
+	const GsInst in = insts[tl.inst];
+	if (in.active && !*in.active) return;
+	const TileCtx<R> c = tile_ctx<R>(tl, in, smem);
+	tile_load<R>(c, in, threadIdx.x);
+	__syncthreads();
+	for (int sg = 0; sg < G::NSTEP; sg++) {
+		tile_step<R>(c, threadIdx.x, sg);
+		__syncthreads();
+	}
+	tile_store<R>(c, in, threadIdx.x);
+}
+
+int tile_lds_bytes(int R)
+{
+	switch (R) {
+	case 1: return Geo<1>::LDS_FLOATS * 4;
+	case 2: return Geo<2>::LDS_FLOATS * 4;
+	default: return Geo<3>::LDS_FLOATS * 4;
+	}
+}
+
+void launch_gs_tiles(int R, const GsTile *tiles, int ntiles, const GsInst *insts,
+		hipStream_t s)
+{
+	if (!ntiles) return;
+	const int lds = tile_lds_bytes(R);
+	switch (R) {
+	case 1: k_gs_tile<1><<<ntiles, Geo<1>::NT, lds, s>>>(tiles, insts); break;
+	case 2: k_gs_tile<2><<<ntiles, Geo<2>::NT, lds, s>>>(tiles, insts); break;
+	default: k_gs_tile<3><<<ntiles, Geo<3>::NT, lds, s>>>(tiles, insts); break;
+	}
+}
+
+// whole level in one block (levels that fit in LDS)
+template <int R>
+__global__ void k_gs_small(const GsInst *insts, const int *ids)
+{
+	extern __shared__ float smem[];
+	const GsInst in = insts[ids[blockIdx.x]];
+	if (in.active && !*in.active) return;
+	const int W = in.w, H = in.h, n = W * H, T = in.nsweep;
+	if (T <= 0) return;
+	float *U = smem, *C = smem + n;
+	for (int e = threadIdx.x; e < n; e += blockDim.x) {
+		U[e] = in.u[e];
+		C[e] = in.c[e];
+	}
+	__syncthreads();
+	const long ns = small_nsteps(R, W, H, T);
+	for (long s = 0; s < ns; s++) {
+		small_step<R>(U, C, W, H, T, in.t, s, threadIdx.x, blockDim.x);
+		__syncthreads();
+	}
+	for (int e = threadIdx.x; e < n; e += blockDim.x)
+		if (!sp_known(C[e])) in.u[e] = U[e];
+}
+
+int small_lds_bytes(int w, int h) { return 8 * w * h; }
+
+void launch_gs_small(int R, const GsInst *insts, const int *ids, int nids,
+		int lds_bytes, int threads, hipStream_t s)
+{
+	if (!nids) return;
+	switch (R) {
+	case 1: k_gs_small<1><<<nids, threads, lds_bytes, s>>>(insts, ids); break;
+	case 2: k_gs_small<2><<<nids, threads, lds_bytes, s>>>(insts, ids); break;
+	default: k_gs_small<3><<<nids, threads, lds_bytes, s>>>(insts, ids); break;
+	}
+}
+
+// ------------------------------------------------------------------ CG
+//
+// Masked conjugate gradient of cleant_cgpois.c:197-264 / minicg.c:24-56.
+// Vectors are full-grid doubles, zero at known pixels.  The operator reads
+// each unknown rounded to float and evaluates its stencil in float, as the
+// reference's float-returning getter does; dot products are double.
+
+constexpr int CG_NT = 256;
+
+// stencil of A at (i,j) over values val(q) (float), Poisson = -(5-point),
+// biharmonic = 13-point, in the reference's term order
+template <typename V>
+static __device__ __forceinline__ double cg_stencil(int bilap, int i, int j,
+		int w, int h, V val)
+{
+	auto p = [&](int a, int b) {
+		return val((size_t)clampi(b, 0, h - 1) * w + clampi(a, 0, w - 1));
+	};
+	if (!bilap) {
+		float r = -4.0f * p(i, j);
+		r = r + p(i + 1, j);
+		r = r + p(i, j + 1);
+		r = r + p(i - 1, j);
+		r = r + p(i, j - 1);
+		return -(double)r;
+	}
+	float r = 20.0f * p(i, j);
+	r = r - 8.0f * p(i + 1, j);
+	r = r - 8.0f * p(i, j + 1);
+	r = r - 8.0f * p(i - 1, j);
+	r = r - 8.0f * p(i, j - 1);
+	r = r + 2.0f * p(i + 1, j + 1);
+	r = r + 2.0f * p(i + 1, j - 1);
+	r = r + 2.0f * p(i - 1, j + 1);
+	r = r + 2.0f * p(i - 1, j - 1);
+	r = r + p(i + 2, j);
+	r = r + p(i, j + 2);
+	r = r + p(i - 2, j);
+	r = r + p(i, j - 2);
+	return (double)r;
+}
+
+// sum of the nblk partials at part[0..nblk), same fixed tree in every block
+static __device__ double sum_partials(const double *part, int nblk, double *red)
+{
+	double v = 0;
+	for (int k = threadIdx.x; k < nblk; k += CG_NT) v += part[k];
+	double s = block_sum_dbl<CG_NT>(v, red);
+	__shared__ double bc;
+	if (threadIdx.x == 0) bc = s;
+	__syncthreads();
+	return bc;
+}
+
+// x0 = u at holes; b = -f - A(boundary); r = b - A x0; p = r; partial r.r
+__global__ __launch_bounds__(CG_NT) void k_cg_setup(CgDev d)
+{
+	__shared__ double red[CG_NT / 64];
+	const size_t n = (size_t)d.w * d.h;
+	double acc = 0;
+	for (size_t q = (size_t)blockIdx.x * CG_NT + threadIdx.x; q < n;
+			q += (size_t)d.nblk * CG_NT) {
+		const float gq = d.g[q];
+		if (!isnan(gq)) {
+			d.x[q] = 0; d.r[q] = 0; d.p[q] = 0; d.q[q] = 0;
+			continue;
+		}
+		const int i = (int)(q % d.w), j = (int)(q / d.w);
+		const float *g = d.g, *u = d.u;
+		double a0 = cg_stencil(d.bilap, i, j, d.w, d.h, [&](size_t k) {
+			float v = g[k]; return isnan(v) ? 0.0f : v; });
+		double ax = cg_stencil(d.bilap, i, j, d.w, d.h, [&](size_t k) {
+			return isnan(g[k]) ? u[k] : 0.0f; });
+		float dv = d.f ? -d.f[q] : 0.0f;
+		double b = (double)dv - a0;
+		double r = b - ax;
+		d.x[q] = (double)u[q];
+		d.r[q] = r;
+		d.p[q] = r;
+		d.q[q] = 0;
+		acc += r * r;
+	}
+	double s = block_sum_dbl<CG_NT>(acc, red);
+	if (threadIdx.x == 0) d.part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(CG_NT) void k_cg_rr0(CgDev d)
+{
+	__shared__ double red[CG_NT / 64];
+	double s = sum_partials(d.part, d.nblk, red);
+	if (threadIdx.x == 0) {
+		d.rr[0] = s;
+		d.done[0] = 0;
+		d.done[1] = 0;
+	}
+}
+
+static __device__ __forceinline__ bool cg_stopped(const CgDev &d)
+{
+	return __hip_atomic_load(d.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+// q = A p, partial p.q
+__global__ __launch_bounds__(CG_NT) void k_cg_spmv(CgDev d)
+{
+	__shared__ double red[CG_NT / 64];
+	if (cg_stopped(d)) return;
+	const size_t n = (size_t)d.w * d.h;
+	const double *p = d.p;
+	double acc = 0;
+	for (size_t q = (size_t)blockIdx.x * CG_NT + threadIdx.x; q < n;
+			q += (size_t)d.nblk * CG_NT) {
+		if (!isnan(d.g[q])) continue;
+		const int i = (int)(q % d.w), j = (int)(q / d.w);
+		double y = cg_stencil(d.bilap, i, j, d.w, d.h, [&](size_t k) {
+			return (float)p[k]; });
+		d.q[q] = y;
+		acc += y * p[q];
+	}
+	double s = block_sum_dbl<CG_NT>(acc, red);
+	if (threadIdx.x == 0) d.part[blockIdx.x] = s;
+}
+
+// alpha = rr_old / p.q ; x += alpha p ; r -= alpha q ; partial r.r
+__global__ __launch_bounds__(CG_NT) void k_cg_update(CgDev d, int it)
+{
+	__shared__ double red[CG_NT / 64];
+	if (cg_stopped(d)) return;
+	const double pq = sum_partials(d.part, d.nblk, red);
+	const double alpha = d.rr[it] / pq;
+	const size_t n = (size_t)d.w * d.h;
+	double acc = 0;
+	for (size_t q = (size_t)blockIdx.x * CG_NT + threadIdx.x; q < n;
+			q += (size_t)d.nblk * CG_NT) {
+		if (!isnan(d.g[q])) continue;
+		d.x[q] = d.x[q] + alpha * d.p[q];
+		double r = d.r[q] - alpha * d.q[q];
+		d.r[q] = r;
+		acc += r * r;
+	}
+	double s = block_sum_dbl<CG_NT>(acc, red);
+	if (threadIdx.x == 0) d.part[d.nblk + blockIdx.x] = s;
+}
+
+// rr_new; absolute stop test; beta; p = r + beta p
+__global__ __launch_bounds__(CG_NT) void k_cg_dir(CgDev d, int it)
+{
+	__shared__ double red[CG_NT / 64];
+	if (cg_stopped(d)) return;
+	const double rr1 = sum_partials(d.part + d.nblk, d.nblk, red);
+	const double rr0 = d.rr[it];
+	if (blockIdx.x == 0 && threadIdx.x == 0) d.rr[it + 1] = rr1;
+	if (sqrt(rr1) < d.stop) {
+		if (blockIdx.x == 0 && threadIdx.x == 0) {
+			d.done[1] = it + 1;
+			__hip_atomic_store(d.done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		}
+		return;
+	}
+	const double beta = rr1 / rr0;
+	const size_t n = (size_t)d.w * d.h;
+	for (size_t q = (size_t)blockIdx.x * CG_NT + threadIdx.x; q < n;
+			q += (size_t)d.nblk * CG_NT) {
+		if (!isnan(d.g[q])) continue;
+		d.p[q] = d.r[q] + beta * d.p[q];
+	}
+}
+
+// u[holes] = (float) x
+__global__ __launch_bounds__(CG_NT) void k_cg_finish(CgDev d)
+{
+	const size_t n = (size_t)d.w * d.h;
+	for (size_t q = (size_t)blockIdx.x * CG_NT + threadIdx.x; q < n;
+			q += (size_t)d.nblk * CG_NT)
+		if (isnan(d.g[q])) d.u[q] = (float)d.x[q];
+}
+
+void launch_cg_setup(const CgDev &d, hipStream_t s)
+{
+	k_cg_setup<<<d.nblk, CG_NT, 0, s>>>(d);
+	k_cg_rr0<<<1, CG_NT, 0, s>>>(d);
+}
+
+void launch_cg_iter(const CgDev &d, int it, hipStream_t s)
+{
+	k_cg_spmv<<<d.nblk, CG_NT, 0, s>>>(d);
+	k_cg_update<<<d.nblk, CG_NT, 0, s>>>(d, it);
+	k_cg_dir<<<d.nblk, CG_NT, 0, s>>>(d, it);
+}
+
+void launch_cg_finish(const CgDev &d, hipStream_t s)
+{
+	k_cg_finish<<<d.nblk, CG_NT, 0, s>>>(d);
+}
+
+// ------------------------------------------------------------- residual
+
+// per block: sum over holes of (b - A u)^2 and b^2, in double from float
+// data, for the masked system of cleant_cgpois.c:225-236
+__global__ __launch_bounds__(EW_NT) void k_residual(const SpPlane *lv,
+		const BlockWork *bw, int bilap, double *part)
+{
+	__shared__ double red[EW_NT / 64];
+	const BlockWork b = bw[blockIdx.x];
+	const SpPlane P = lv[b.plane];
+	const size_t n = (size_t)P.w * P.h;
+	double rr = 0, bb = 0;
+	for (int k = 0; k < EW_PPT; k++) {
+		size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
+		if (q >= n) break;
+		if (!isnan(P.g[q])) continue;
+		const int i = (int)(q % P.w), j = (int)(q / P.w);
+		double au = 0, a0 = 0;
+		auto term = [&](int di, int dj, double cf) {
+			size_t k2 = (size_t)clampi(j + dj, 0, P.h - 1) * P.w
+				+ clampi(i + di, 0, P.w - 1);
+			au += cf * (double)P.u[k2];
+			float gv = P.g[k2];
+			if (!isnan(gv)) a0 += cf * (double)gv;
+		};
+		if (!bilap) {
+			term(0, 0, -4); term(1, 0, 1); term(0, 1, 1); term(-1, 0, 1);
+			term(0, -1, 1);
+			au = -au; a0 = -a0;
+		} else {
+			term(0, 0, 20); term(1, 0, -8); term(0, 1, -8); term(-1, 0, -8);
+			term(0, -1, -8); term(1, 1, 2); term(1, -1, 2); term(-1, 1, 2);
+			term(-1, -1, 2); term(2, 0, 1); term(0, 2, 1); term(-2, 0, 1);
+			term(0, -2, 1);
+		}
+		double bv = (P.f ? -(double)P.f[q] : 0.0) - a0;
+		double res = bv - (au - a0);
+		rr += res * res;
+		bb += bv * bv;
+	}
+	double s1 = block_sum_dbl<EW_NT>(rr, red);
+	double s2 = block_sum_dbl<EW_NT>(bb, red);
+	if (threadIdx.x == 0) {
+		part[2 * blockIdx.x] = s1;
+		part[2 * blockIdx.x + 1] = s2;
+	}
+}
+
+// out[3p..3p+2] = (rr, bb, rel) for plane p; blocks of plane p are
+// plane_first[p] .. plane_first[p+1]-1
+__global__ void k_residual_reduce(const double *part, const int *plane_first,
+		int nplanes, double *out)
+{
+	int p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= nplanes) return;
+	double rr = 0, bb = 0;
+	for (int k = plane_first[p]; k < plane_first[p + 1]; k++) {
+		rr += part[2 * k];
+		bb += part[2 * k + 1];
+	}
+	out[3 * p] = rr;
+	out[3 * p + 1] = bb;
+	out[3 * p + 2] = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+}
+
+void launch_residual(const SpPlane *lv, const BlockWork *bw, int nblocks,
+		int bilap, double *part, hipStream_t s)
+{
+	if (nblocks) k_residual<<<nblocks, EW_NT, 0, s>>>(lv, bw, bilap, part);
+}
+
+void launch_residual_reduce(const double *part, const int *plane_first,
+		int nplanes, double *out, hipStream_t s)
+{
+	if (nplanes)
+		k_residual_reduce<<<(nplanes + 63) / 64, 64, 0, s>>>(part, plane_first,
+				nplanes, out);
+}
+
+} // namespace spk

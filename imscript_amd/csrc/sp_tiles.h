@@ -1,0 +1,55 @@
+// Host-side construction of the Gauss-Seidel tile schedule (sp_gs.h):
+// the skewed tiles of one level, grouped by wavefront I'+J'+T'.  Shared by
+// the host driver and the CPU emulator of the tests.
+#pragma once
+#include <algorithm>
+#include <vector>
+#include "sp_gs.h"
+
+// skewed tiles of one level, grouped by wavefront I'+J'+T'
+template <int R>
+inline void build_tiles(int W, int H, int T, int pd, std::vector<GsTile> &out,
+		std::vector<int> &off)
+{
+	using S = TileShape<R>;
+	const int SH = R - 1;
+	long ispan, jspan;
+	if (R == 1) { ispan = (long)W + T - 1; jspan = (long)H + T - 1; }
+	else {
+		ispan = (long)(W - 1) + (long)SH * (H - 1) + (long)R * SH * (T - 1) + 1;
+		jspan = (long)(H - 1) + (long)R * (T - 1) + 1;
+	}
+	const int nI = (int)((ispan + S::BI - 1) / S::BI);
+	const int nJ = (int)((jspan + S::BJ - 1) / S::BJ);
+	const int nT = (T + S::BT - 1) / S::BT;
+	std::vector<std::vector<GsTile>> wf(nI + nJ + nT);
+	for (int tt = 0; tt < nT; tt++) {
+		const long k0 = (long)tt * S::BT;
+		const long bte = std::min<long>(S::BT, T - k0);
+		for (int tj = 0; tj < nJ; tj++) {
+			long jl, jh;
+			if (R == 1) { jl = (long)tj * S::BJ - k0 - (bte - 1); jh = (long)tj * S::BJ + S::BJ - 1 - k0; }
+			else { jl = (long)tj * S::BJ - R * (k0 + bte - 1); jh = (long)tj * S::BJ + S::BJ - 1 - R * k0; }
+			if (jh < 0 || jl >= H) continue;
+			for (int ti = 0; ti < nI; ti++) {
+				long il, ih;
+				if (R == 1) { il = (long)ti * S::BI - k0 - (bte - 1); ih = (long)ti * S::BI + S::BI - 1 - k0; }
+				else {
+					il = (long)ti * S::BI - (long)SH * ((long)tj * S::BJ + S::BJ - 1);
+					ih = (long)ti * S::BI + S::BI - 1 - (long)SH * tj * S::BJ;
+				}
+				if (ih < 0 || il >= W) continue;
+				for (int p = 0; p < pd; p++)
+					wf[ti + tj + tt].push_back({p, ti, tj, tt});
+			}
+		}
+	}
+	out.clear();
+	off.assign(1, 0);
+	for (auto &v : wf) {
+		if (v.empty()) continue;
+		out.insert(out.end(), v.begin(), v.end());
+		off.push_back((int)out.size());
+	}
+}
+

@@ -1,0 +1,116 @@
+/*
+ * simpois_hip.h -- C ABI of the MI355X (gfx950) implementation of imscript's
+ * simpois hole-filling path.
+ *
+ * Two groups of entry points:
+ *
+ *  1. Drop-in replacements with the reference's exact signatures and
+ *     semantics.  Buffers are host memory, planar (plane l at +w*h*l), and
+ *     the call is synchronous.  Failures print a message and exit(1), as the
+ *     reference's xmalloc does (simpois.c:10-19).
+ *
+ *  2. sp_* entry points with status codes that never exit, working on host
+ *     or device buffers, plus the additions that have no reference
+ *     counterpart (batched solve, adjoint, residual).
+ *
+ * Environment knobs read once per process, as SMART_PARAMETER_SILENT does
+ * (smapa.h:44-60): PMSFAC (default 3), PONLIT (0), CG_EPS (-1).
+ * SIMPOIS_HIP_QUIET=1 suppresses the per-iteration CG trace on stderr.
+ */
+#ifndef SIMPOIS_HIP_H
+#define SIMPOIS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ drop-in */
+
+/* Fill the NaN pixels of each plane of `in` (w*h*pd floats, planar) and
+ * write the result to `out`.  `dat` is the Poisson data term (same layout)
+ * or NULL.  tstep > 0: Laplace/Poisson Gauss-Seidel with step tstep;
+ * tstep < 0: biharmonic; tstep < -1000: triharmonic.  niter sweeps at each
+ * of at most `scale` pyramid levels; cgrad CG iterations per level (-1: as
+ * many as unknowns).  `out` must not alias `in`.
+ * Replaces simpois.c:288-290 (poisson_solver_separable). */
+void poisson_solver_separable(float *out, float *in, float *dat,
+		int w, int h, int pd,
+		float tstep, int niter, int scale, float cgrad);
+
+/* One plane of the above.  Replaces simpois.c:247-248 (poisson_rec). */
+void poisson_rec(float *u, float *g, float *f, int w, int h,
+		float tstep, int niter, int scale, int cgit);
+
+/* Masked CG refinement of one plane from the initial guess `init`
+ * (Poisson: -Laplacian; biharmonic: 13-point bilaplacian), absolute stop
+ * sqrt(r.r) < eps (CG_EPS overrides eps), maxit == -1: as many iterations
+ * as unknowns.  `out` may alias `init`.
+ * Replace cleant_cgpois.c:266-271 and :273-278. */
+void poisson_extension_by_cg(float *out, float *in, float *dat, int w, int h,
+		float *init, int maxit, float eps);
+void biharmonic_extension_by_cg(float *out, float *in, float *dat, int w, int h,
+		float *init, int maxit, float eps);
+
+/* The simpois command line (flags -i -o -m -f -t -n -s -c, NPY/PFM I/O).
+ * Replaces simpois.c:351-439 (main_simpois). */
+int main_simpois(int argc, char *argv[]);
+
+/* ------------------------------------------------------- status API */
+
+enum {
+	SP_OK = 0,
+	SP_ERR_ARG = 1,     /* invalid argument (sizes, aliasing, NULL) */
+	SP_ERR_NOMEM = 2,   /* device or host allocation failed */
+	SP_ERR_HIP = 3,     /* HIP runtime error */
+	SP_ERR_NODEV = 4    /* no usable gfx950 device */
+};
+
+typedef struct sp_params {
+	float tstep;     /* -t, default 0.25 */
+	int niter;       /* -n, default 10 */
+	int scale;       /* -s, default 99 */
+	int cgit;        /* -c, default 0 */
+	double pmsfac;   /* PMSFAC */
+	double ponlit;   /* PONLIT */
+	double cg_eps;   /* CG_EPS (< 0: built-in 1e-6 / 1e-9) */
+	int trace;       /* print the CG trace to stderr like minicg.c:46 */
+} sp_params;
+
+typedef struct sp_stats {
+	int levels;            /* pyramid levels of the solve */
+	int cg_iters_finest;   /* CG iterations at the finest level (plane 0) */
+	double elapsed_ms;     /* device time of the solve */
+} sp_stats;
+
+/* Defaults of the reference (simpois.c:357-360) with the env knobs. */
+void sp_params_default(sp_params *p);
+
+/* Text of the last error of the calling thread. */
+const char *sp_last_error(void);
+
+/* Host-buffer solve with a status code (same semantics as
+ * poisson_solver_separable). */
+int sp_solve_host(float *out, const float *in, const float *dat,
+		int w, int h, int pd, const sp_params *p, sp_stats *st);
+
+/* Device-buffer solve on `stream` (a hipStream_t, NULL = default).
+ * d_out, d_in, d_dat are device pointers, planar; d_dat may be NULL.
+ * The call returns when the work is queued and the result is complete on
+ * the stream, except for the CG stage, which synchronises the stream to
+ * test for convergence. */
+int sp_solve_device(float *d_out, const float *d_in, const float *d_dat,
+		int w, int h, int pd, const sp_params *p, void *stream,
+		sp_stats *st);
+
+/* FP64 relative residual ||b - A u|| / ||b|| of each plane over the NaN
+ * pixels of g, for the masked Poisson (bilap = 0) or biharmonic system of
+ * cleant_cgpois.c:225-236.  Device pointers; writes pd doubles to host
+ * array `rel`. */
+int sp_residual_device(const float *d_u, const float *d_g, const float *d_f,
+		int w, int h, int pd, int bilap, double *rel, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIMPOIS_HIP_H */

@@ -1,0 +1,144 @@
+// CPU emulation of the Gauss-Seidel kernels, used by the test-suite.
+//
+// Runs the device kernels' own phase functions (imscript_amd/csrc/sp_gs.h)
+// on the CPU, thread by thread, with a barrier between phases, and compares
+// the result with the oracle's raster-order sweeps (oracle/simpois_oracle.c)
+// bit for bit.  Every tile of a wavefront loads before any tile of that
+// wavefront computes or stores, which is the worst interleaving a launch
+// allows, and a pixel stored by two tiles of one wavefront is reported.
+// Built with AddressSanitizer by the tests, so any LDS or image access out
+// of bounds aborts the run.
+//
+// usage: gs_emulate W H T tstep seed holefrac withf mode
+//   mode: tiles | small
+// prints "OK <n>" (n = holes) or "MISMATCH ..." / "CONFLICT ..." and exits 1
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include "../../imscript_amd/csrc/sp_gs.h"
+#include "../../imscript_amd/csrc/sp_tiles.h"
+
+extern "C" void orc_gs_sweeps(float *u, const float *g, const float *f, int w,
+		int h, float tstep, int niter);
+
+static unsigned rng_state;
+static double urand()
+{
+	rng_state = rng_state * 1664525u + 1013904223u;
+	return (rng_state >> 8) * (1.0 / 16777216.0);
+}
+
+template <int R>
+static int run_tiles(float *u, const float *c, int W, int H, int T, float t)
+{
+	using G = Geo<R>;
+	std::vector<GsTile> tiles;
+	std::vector<int> off;
+	build_tiles<R>(W, H, T, 1, tiles, off);
+	GsInst in{u, c, W, H, T, t, nullptr};
+	std::vector<int> owner((size_t)W * H, -1);
+	for (size_t w = 0; w + 1 < off.size(); w++) {
+		const int n = off[w + 1] - off[w];
+		std::vector<float *> lds(n);
+		std::vector<TileCtx<R>> ctx(n);
+		for (int k = 0; k < n; k++) {
+			lds[k] = new float[G::LDS_FLOATS];
+			for (int e = 0; e < G::LDS_FLOATS; e++) lds[k][e] = -12345.0f;
+			ctx[k] = tile_ctx<R>(tiles[off[w] + k], in, lds[k]);
+		}
+		for (int k = 0; k < n; k++)
+			for (int tid = 0; tid < G::NT; tid++) tile_load<R>(ctx[k], in, tid);
+		for (int k = 0; k < n; k++)
+			for (int sg = 0; sg < G::NSTEP; sg++)
+				for (int tid = 0; tid < G::NT; tid++) tile_step<R>(ctx[k], tid, sg);
+		// write-write conflicts between tiles of one wavefront
+		for (int k = 0; k < n; k++) {
+			const TileCtx<R> &cc = ctx[k];
+			for (int e = 0; e < G::LW * G::LH; e++) {
+				int row = e / G::LW, col = e - row * G::LW;
+				int j = row + cc.jlo, i = col + cc.K - G::SH * j;
+				if (i < 0 || i >= W || j < 0 || j >= H || sp_known(cc.C[e])) continue;
+				if (!tile_owns<R>(cc, i, j)) continue;
+				size_t q = (size_t)j * W + i;
+				if (owner[q] == (int)w * 1000000 + k) continue;
+				if (owner[q] >= (int)w * 1000000) {
+					printf("CONFLICT wavefront %zu pixel %d,%d\n", w, i, j);
+					return 1;
+				}
+				owner[q] = (int)w * 1000000 + k;
+			}
+		}
+		for (int k = 0; k < n; k++)
+			for (int tid = 0; tid < G::NT; tid++) tile_store<R>(ctx[k], in, tid);
+		for (int k = 0; k < n; k++) delete[] lds[k];
+	}
+	return 0;
+}
+
+template <int R>
+static int run_small(float *u, const float *c, int W, int H, int T, float t)
+{
+	const int n = W * H;
+	float *lds = new float[2 * n];
+	float *U = lds, *C = lds + n;
+	for (int e = 0; e < n; e++) { U[e] = u[e]; C[e] = c[e]; }
+	const int ct = R * R + 1;
+	long slots = (long)H * ((W - 1) / ct + 1);
+	int nt = (int)std::min<long>(1024, std::max<long>(64, (slots + 63) / 64 * 64));
+	const long ns = small_nsteps(R, W, H, T);
+	for (long s = 0; s < ns; s++)
+		for (int tid = 0; tid < nt; tid++)
+			small_step<R>(U, C, W, H, T, t, s, tid, nt);
+	for (int e = 0; e < n; e++)
+		if (!sp_known(C[e])) u[e] = U[e];
+	delete[] lds;
+	return 0;
+}
+
+int main(int argc, char **argv)
+{
+	if (argc != 9) {
+		fprintf(stderr, "usage: gs_emulate W H T tstep seed holefrac withf tiles|small\n");
+		return 2;
+	}
+	const int W = atoi(argv[1]), H = atoi(argv[2]), T = atoi(argv[3]);
+	const float tstep = (float)atof(argv[4]);
+	rng_state = (unsigned)atoi(argv[5]) * 2654435761u + 7;
+	const double frac = atof(argv[6]);
+	const bool withf = atoi(argv[7]) != 0;
+	const bool small = !strcmp(argv[8], "small");
+	const size_t n = (size_t)W * H;
+	float *g = new float[n], *f = withf ? new float[n] : nullptr;
+	float *u = new float[n], *uref = new float[n], *c = new float[n];
+	int holes = 0;
+	for (size_t q = 0; q < n; q++) {
+		g[q] = (float)(urand() * 100);
+		if (urand() < frac) { g[q] = NAN; holes++; }
+		if (f) f[q] = (float)(urand() * 4 - 2);
+		u[q] = isfinite(g[q]) ? g[q] : (float)(urand() * 100);
+		uref[q] = u[q];
+	}
+	for (size_t q = 0; q < n; q++) c[q] = sp_coef(g[q], f ? f[q] : 0.0f);
+	orc_gs_sweeps(uref, g, f, W, H, tstep, T);
+	float teff = tstep;
+	int R = 1;
+	if (tstep < 0) {
+		R = 2;
+		if (tstep < -1000) { R = 3; teff = tstep; teff += 1000; teff *= -1; }
+	}
+	int rc;
+	if (small) rc = R == 1 ? run_small<1>(u, c, W, H, T, teff)
+		: R == 2 ? run_small<2>(u, c, W, H, T, teff) : run_small<3>(u, c, W, H, T, teff);
+	else rc = R == 1 ? run_tiles<1>(u, c, W, H, T, teff)
+		: R == 2 ? run_tiles<2>(u, c, W, H, T, teff) : run_tiles<3>(u, c, W, H, T, teff);
+	if (rc) return 1;
+	for (size_t q = 0; q < n; q++)
+		if (memcmp(&u[q], &uref[q], 4)) {
+			printf("MISMATCH at %zu,%zu: %.9g vs %.9g\n", q % W, q / W, u[q], uref[q]);
+			return 1;
+		}
+	printf("OK %d\n", holes);
+	delete[] g; delete[] f; delete[] u; delete[] uref; delete[] c;
+	return 0;
+}
