@@ -14,13 +14,15 @@ Entry points mirror the reference (``simpois.c:288``, ``:247``,
 plus array-level helpers (:func:`simpois`) and device-tensor helpers.
 """
 from ._lib import (lib, lib_path, SimpoisError, sp_params, sp_stats,
-                   default_params, exported_symbols)
+                   sp_image, default_params, exported_symbols)
 from .api import (poisson_solver_separable, poisson_rec,
                   poisson_extension_by_cg, biharmonic_extension_by_cg,
-                  simpois, solve_device, residual_device)
+                  simpois, solve_device, solve_batch_device, residual_device,
+                  profile, profile_read)
 
 __all__ = ["lib", "lib_path", "SimpoisError", "sp_params", "sp_stats",
-           "default_params", "exported_symbols", "poisson_solver_separable",
-           "poisson_rec", "poisson_extension_by_cg",
-           "biharmonic_extension_by_cg", "simpois", "solve_device",
-           "residual_device"]
+           "sp_image", "default_params", "exported_symbols",
+           "poisson_solver_separable", "poisson_rec",
+           "poisson_extension_by_cg", "biharmonic_extension_by_cg", "simpois",
+           "solve_device", "solve_batch_device", "residual_device", "profile",
+           "profile_read"]

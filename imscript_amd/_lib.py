@@ -14,7 +14,15 @@ class sp_params(ctypes.Structure):
     _fields_ = [("tstep", ctypes.c_float), ("niter", ctypes.c_int),
                 ("scale", ctypes.c_int), ("cgit", ctypes.c_int),
                 ("pmsfac", ctypes.c_double), ("ponlit", ctypes.c_double),
-                ("cg_eps", ctypes.c_double), ("trace", ctypes.c_int)]
+                ("cg_eps", ctypes.c_double), ("trace", ctypes.c_int),
+                ("cg_fp32", ctypes.c_int), ("cg_rel", ctypes.c_double),
+                ("cg_chunk", ctypes.c_int)]
+
+
+class sp_image(ctypes.Structure):
+    _fields_ = [("w", ctypes.c_int), ("h", ctypes.c_int), ("pd", ctypes.c_int),
+                ("g", ctypes.c_void_p), ("f", ctypes.c_void_p),
+                ("out", ctypes.c_void_p)]
 
 
 class sp_stats(ctypes.Structure):
@@ -41,6 +49,12 @@ _SIGNATURES = {
                              _P, ctypes.POINTER(sp_stats)]),
     "sp_residual_device": (_I, [_P, _P, _P, _I, _I, _I, _I,
                                 ctypes.POINTER(ctypes.c_double), _P]),
+    "sp_solve_batch_device": (_I, [_I, _P, ctypes.POINTER(sp_params), _P,
+                                   ctypes.POINTER(sp_stats)]),
+    "sp_profile": (None, [_I]),
+    "sp_profile_read": (_I, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                             ctypes.POINTER(ctypes.c_long),
+                             ctypes.POINTER(ctypes.c_double)]),
 }
 
 
@@ -54,6 +68,13 @@ class _Lib:
 
     def __getattr__(self, name):
         if _Lib._handle is None:
+            # torch ships its own HIP runtime; when torch is present it must
+            # be the one the library binds to, so that device pointers of
+            # torch tensors and of the library live in one runtime
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
             if not os.path.exists(lib_path):
                 raise ImportError(
                     f"{lib_path} is missing: build it with "

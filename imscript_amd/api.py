@@ -3,7 +3,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import lib, check, default_params, sp_stats
+from ._lib import lib, check, default_params, sp_stats, sp_image
 
 
 def _f32(a, name):
@@ -122,3 +122,43 @@ def residual_device(u, g, f=None, bilap=False, stream=None):
                                  int(bool(bilap)), rel,
                                  ctypes.c_void_p(s.cuda_stream)))
     return list(rel)
+
+
+def solve_batch_device(gs, fs=None, outs=None, stream=None, **params):
+    """Batched solve of device tensors of uneven shapes ((h,w) or (pd,h,w)),
+    one pyramid per plane, all planes advancing together.  ``params`` are
+    sp_params fields (tstep, niter, scale, cgit, cg_fp32, cg_rel, ...).
+    Returns (outs, sp_stats)."""
+    import torch
+    n = len(gs)
+    fs = fs if fs is not None else [None] * n
+    if outs is None:
+        outs = [torch.empty_like(g) for g in gs]
+    arr = (sp_image * n)()
+    for k, (g, f, o) in enumerate(zip(gs, fs, outs)):
+        if g.dtype != torch.float32 or not g.is_contiguous() or not g.is_cuda:
+            raise TypeError("inputs must be contiguous float32 device tensors")
+        pd, h, w = (1,) + tuple(g.shape) if g.dim() == 2 else tuple(g.shape)
+        arr[k] = sp_image(w, h, pd, g.data_ptr(),
+                          None if f is None else f.data_ptr(), o.data_ptr())
+    s = stream if stream is not None else torch.cuda.current_stream()
+    params.setdefault("trace", 0)
+    p = default_params(**params)
+    st = sp_stats()
+    check(lib.sp_solve_batch_device(n, arr, ctypes.byref(p),
+                                    ctypes.c_void_p(s.cuda_stream),
+                                    ctypes.byref(st)))
+    return outs, st
+
+
+def profile(on=True):
+    """Enable (and clear) or disable the solver's phase timers."""
+    lib.sp_profile(1 if on else 0)
+
+
+def profile_read(name):
+    """(device ms, kernel launches, algorithmic bytes) summed over a phase."""
+    ms, n, by = ctypes.c_double(), ctypes.c_long(), ctypes.c_double()
+    check(lib.sp_profile_read(name.encode(), ctypes.byref(ms), ctypes.byref(n),
+                              ctypes.byref(by)))
+    return ms.value, n.value, by.value

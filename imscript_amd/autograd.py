@@ -1,0 +1,95 @@
+"""Differentiable Poisson hole filling (torch.autograd).
+
+Forward: the converged masked Poisson fill of simpois (simpois.c:247-285 with
+CG to a relative tolerance): holes u_h solve A u_h = -f_h - A_hk g_k with
+A = -Laplacian (clamped 5-point, cleant_cgpois.c:67-89), known pixels are
+copied from g.
+
+Backward: A is symmetric, so the adjoint is one more masked Laplacian solve,
+A lambda = dL/du on the holes, run on the GPU by the same pyramid and CG
+(sp_adjoint_batch_device); then dL/df = -lambda on the holes and
+dL/dg = dL/du + (sum of lambda over hole neighbours) on the known pixels.
+"""
+import ctypes
+
+import torch
+
+from ._lib import lib, check, default_params, sp_stats
+
+
+class sp_adjoint_image(ctypes.Structure):
+    _fields_ = [("w", ctypes.c_int), ("h", ctypes.c_int), ("pd", ctypes.c_int),
+                ("g", ctypes.c_void_p), ("grad_u", ctypes.c_void_p),
+                ("grad_g", ctypes.c_void_p), ("grad_f", ctypes.c_void_p)]
+
+
+lib_sig = None
+
+
+def _adjoint(g, grad_u, grad_g, grad_f, params):
+    global lib_sig
+    if lib_sig is None:
+        fn = lib.sp_adjoint_batch_device
+        fn.restype = ctypes.c_int
+        fn.argtypes = [ctypes.c_int, ctypes.POINTER(sp_adjoint_image),
+                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        lib_sig = fn
+    pd, h, w = g.shape
+    img = sp_adjoint_image(w, h, pd, g.data_ptr(), grad_u.data_ptr(),
+                           grad_g.data_ptr(),
+                           None if grad_f is None else grad_f.data_ptr())
+    st = sp_stats()
+    s = torch.cuda.current_stream()
+    check(lib_sig(1, ctypes.byref(img), ctypes.byref(params),
+                  ctypes.c_void_p(s.cuda_stream), ctypes.byref(st)))
+    return st
+
+
+def _params(tol, niter, scale):
+    return default_params(tstep=0.25, niter=niter, scale=scale, cgit=-1,
+                          cg_fp32=1, cg_rel=tol, trace=0)
+
+
+class PoissonFill(torch.autograd.Function):
+    """u = fill(g, f) for planar float32 device tensors of shape (B, H, W)."""
+
+    @staticmethod
+    def forward(ctx, g, f, tol, niter, scale):
+        g = g.contiguous()
+        f = None if f is None else f.contiguous()
+        out = torch.empty_like(g)
+        p = _params(tol, niter, scale)
+        pd, h, w = g.shape
+        st = sp_stats()
+        s = torch.cuda.current_stream()
+        check(lib.sp_solve_device(out.data_ptr(), g.data_ptr(),
+                                  None if f is None else f.data_ptr(), w, h, pd,
+                                  ctypes.byref(p), ctypes.c_void_p(s.cuda_stream),
+                                  ctypes.byref(st)))
+        ctx.save_for_backward(g)
+        ctx.has_f = f is not None
+        ctx.cfg = (tol, niter, scale)
+        ctx.iters = st.cg_iters_finest
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_u):
+        (g,) = ctx.saved_tensors
+        grad_u = grad_u.contiguous()
+        grad_g = torch.empty_like(g)
+        need_f = ctx.has_f and ctx.needs_input_grad[1]
+        grad_f = torch.empty_like(g) if need_f else None
+        _adjoint(g, grad_u, grad_g, grad_f, _params(*ctx.cfg))
+        return grad_g, grad_f, None, None, None
+
+
+def poisson_fill(g, f=None, tol=1e-6, niter=10, scale=99):
+    """Differentiable fill of the NaN pixels of g ((H,W) or (B,H,W) float32
+    on the GPU) with optional guidance f; gradients flow to the known pixels
+    of g and to f."""
+    squeeze = g.dim() == 2
+    if squeeze:
+        g = g[None]
+        f = None if f is None else f[None]
+    u = PoissonFill.apply(g, f, tol, niter, scale)
+    return u[0] if squeeze else u

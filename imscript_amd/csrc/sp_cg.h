@@ -1,0 +1,41 @@
+// Batched FP32-vector CG (sp_cg.hip): device-side descriptors.
+#pragma once
+#include "sp_internal.h"
+
+// one plane of the batched CG: full-grid float vectors, zero at known pixels
+struct Cg32Plane {
+	const float *g;    // boundary data, NaN = unknown
+	const float *f;    // data term or nullptr
+	float *u;          // initial guess in, solution out (unknowns only)
+	double *x;         // solution accumulates in double: ~1e3 float updates
+	                   // would otherwise cost ~1e-5 of relative residual
+	float *r, *p, *q;
+	const int *cnt;    // number of unknowns (device)
+	int w, h, bilap;
+	int blk0, nblk;    // this plane's work blocks / partial-sum slots
+};
+
+// per-plane scalars (device)
+struct Cg32Scal {
+	double rr, alpha, beta, stop, eps, bnorm;
+	int rel;           // stop = eps * ||b|| instead of eps
+	int active, iters, maxit;
+	int maxit_from_count;   // maxit = number of unknowns (the -c -1 rule)
+	int trace0, trace_cap;  // slots of this plane in the rr trace
+};
+
+struct Cg32Batch {
+	Cg32Plane *planes;
+	Cg32Scal *scal;
+	BlockWork *bw;
+	double *part;      // 2 per work block
+	double *trace;     // optional rr_new trace
+	int *nactive;      // planes still iterating
+	int nplanes, nblk;
+};
+
+namespace spk {
+void cg32_setup(const Cg32Batch &B, hipStream_t s);
+void cg32_iter(const Cg32Batch &B, hipStream_t s);
+void cg32_finish(const Cg32Batch &B, hipStream_t s);
+}

@@ -1,15 +1,16 @@
 // Host driver of the simpois path: pyramid plan, Gauss-Seidel scheduling,
-// CG driver and the C ABI (include/simpois_hip.h).
+// CG drivers, profiler and the C ABI (include/simpois_hip.h).
 //
 // Call structure restated from the reference (simpois.c:247-302):
 //   per plane: restrict g (and f, times PMSFAC) down the pyramid; from the
 //   coarsest level up: u = isfinite(g) ? g : prolong(coarse output);
 //   niter Gauss-Seidel sweeps; CG when -c is non-zero; the level's output
 //   is the input at non-NaN pixels when the level has holes.
-// Here all planes of a call advance together, level by level, on one HIP
-// stream; results per plane are identical to the reference's
-// channel-by-channel loop because planes never interact.
+// Here every plane of every image of a call advances together, level by
+// level (levels aligned at the finest), on one HIP stream.  Planes never
+// interact, so each result equals the reference's channel-by-channel loop.
 #include "sp_internal.h"
+#include "sp_cg.h"
 #include "sp_tiles.h"
 #include "../../include/simpois_hip.h"
 
@@ -66,6 +67,7 @@ static double env_knob(const char *name, double dflt)
 
 extern "C" void sp_params_default(sp_params *p)
 {
+	memset(p, 0, sizeof *p);
 	p->tstep = 0.25f;
 	p->niter = 10;
 	p->scale = 99;
@@ -75,6 +77,9 @@ extern "C" void sp_params_default(sp_params *p)
 	p->cg_eps = env_knob("CG_EPS", -1);
 	const char *q = getenv("SIMPOIS_HIP_QUIET");
 	p->trace = !(q && *q && strcmp(q, "0"));
+	p->cg_fp32 = 0;
+	p->cg_rel = 0;
+	p->cg_chunk = 32;
 }
 
 // ------------------------------------------------------------- device
@@ -85,13 +90,15 @@ static int ensure_device()
 {
 	static std::mutex mu;
 	std::lock_guard<std::mutex> lk(mu);
-	if (g_dev_checked) return g_dev_ok ? SP_OK : set_err(SP_ERR_NODEV, "%s", g_err.c_str());
+	if (g_dev_checked)
+		return g_dev_ok ? SP_OK : set_err(SP_ERR_NODEV, "no usable gfx950 device");
 	g_dev_checked = 1;
 	int n = 0;
 	if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
 		return set_err(SP_ERR_NODEV, "no HIP device visible");
 	int dev = 0;
-	hipGetDevice(&dev);
+	if (hipGetDevice(&dev) != hipSuccess)
+		return set_err(SP_ERR_NODEV, "hipGetDevice failed");
 	hipDeviceProp_t prop;
 	if (hipGetDeviceProperties(&prop, dev) != hipSuccess)
 		return set_err(SP_ERR_NODEV, "hipGetDeviceProperties failed");
@@ -105,49 +112,125 @@ static int ensure_device()
 	return SP_OK;
 }
 
-// --------------------------------------------------------------- plan
+// ------------------------------------------------------------ profiler
+//
+// When enabled, each phase records an event pair around its launches on the
+// solve's stream, with the launch count and the algorithmic bytes of the
+// phase (definitions next to each use).  sp_profile_read sums them.
 
-struct LevelPlan {
-	int w = 0, h = 0;
-	int nsweep = 0, cgit = 0;
-	bool small = false;
-	int small_threads = 64, small_lds = 0;
-	std::vector<int> wf_off;                // tile wavefront offsets
-	GsTile *d_tiles = nullptr;
-	GsInst *d_insts = nullptr;
-	int *d_ids = nullptr;
-	SpPlane *d_planes = nullptr;
-	BlockWork *d_bw = nullptr;
-	int nbw = 0;
-	std::vector<SpPlane> planes;            // host copy of d_planes
+struct ProfRec { std::string name; hipEvent_t a, b; long launches; double bytes; };
+static std::mutex g_prof_mu;
+static bool g_prof_on = false;
+static std::vector<ProfRec> g_prof;
+
+struct ProfScope {
+	bool on;
+	ProfRec r;
+	hipStream_t s;
+	ProfScope(const char *name, hipStream_t st) : on(false), s(st)
+	{
+		std::lock_guard<std::mutex> lk(g_prof_mu);
+		if (!g_prof_on) return;
+		on = hipEventCreate(&r.a) == hipSuccess && hipEventCreate(&r.b) == hipSuccess;
+		if (!on) return;
+		r.name = name; r.launches = 0; r.bytes = 0;
+		(void)hipEventRecord(r.a, s);
+	}
+	void add(long launches, double bytes) { r.launches += launches; r.bytes += bytes; }
+	~ProfScope()
+	{
+		if (!on) return;
+		(void)hipEventRecord(r.b, s);
+		std::lock_guard<std::mutex> lk(g_prof_mu);
+		g_prof.push_back(r);
+	}
 };
 
-struct CgTrace { int n; std::vector<double> rr; };
+extern "C" void sp_profile(int on)
+{
+	std::lock_guard<std::mutex> lk(g_prof_mu);
+	for (auto &r : g_prof) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+	g_prof.clear();
+	g_prof_on = on != 0;
+}
+
+extern "C" int sp_profile_read(const char *name, double *ms, long *launches,
+		double *bytes)
+{
+	std::lock_guard<std::mutex> lk(g_prof_mu);
+	double t = 0, by = 0;
+	long n = 0;
+	for (auto &r : g_prof) {
+		if (r.name != name) continue;
+		HIPCHK(hipEventSynchronize(r.b));
+		float e = 0;
+		HIPCHK(hipEventElapsedTime(&e, r.a, r.b));
+		t += e; n += r.launches; by += r.bytes;
+	}
+	if (ms) *ms = t;
+	if (launches) *launches = n;
+	if (bytes) *bytes = by;
+	return SP_OK;
+}
+
+// --------------------------------------------------------------- plan
+
+struct PlaneIn {            // one plane of one image (device pointers)
+	int w, h;
+	const float *g, *f;
+	float *out;
+};
+
+struct LevelPlan {
+	std::vector<int> members;        // plane ids present at this level
+	std::vector<SpPlane> sp;         // their descriptors at this level
+	std::vector<int> nsweep, cgit;   // after the PONLIT rule
+	SpPlane *d_sp = nullptr;
+	SpPlane *d_parent = nullptr;     // finer-level descriptor of each member
+	SpPlane *d_coarse = nullptr;     // coarser-level descriptor (g = null: none)
+	BlockWork *d_bw = nullptr;
+	int nbw = 0;
+	int *d_first = nullptr;          // first work block of each member, + end
+	int *d_bcnt = nullptr;           // holes per work block
+	GsInst *d_insts = nullptr;
+	std::vector<int> wf_off;
+	GsTile *d_tiles = nullptr;
+	long tile_pass_px = 0;           // for the profiler
+	int *d_small = nullptr;
+	int nsmall = 0, small_lds = 0, small_threads = 64;
+	bool any_cg = false;
+};
+
+struct CgTrace { int n = 0; std::vector<double> rr; };
 
 struct Plan {
-	int w = 0, h = 0, pd = 0;
-	bool has_f = false;
 	sp_params P{};
 	int R = 1;
 	float teff = 0;
+	std::vector<PlaneIn> planes;
+	std::vector<int> depth;          // pyramid levels of each plane
 	std::vector<LevelPlan> lv;
 	std::vector<void *> allocs;
-	int *d_cnt = nullptr;          // nlev * pd hole counters
-	float **d_outs = nullptr;      // pd output pointers
-	float *c0 = nullptr;           // level-0 coefficient planes
-	// CG scratch, sized for the finest level
+	int *d_cnt = nullptr;
+	float **d_outs = nullptr;
+	// FP64 CG scratch, sized for the largest plane
 	double *cg_x = nullptr, *cg_r = nullptr, *cg_p = nullptr, *cg_q = nullptr,
 		*cg_part = nullptr, *cg_rr = nullptr;
 	int cg_rr_cap = 0;
 	int *cg_done = nullptr, *h_done = nullptr;
+	// FP32 batched CG scratch: r p q (float) and x (double) per pixel
+	float *cg32_vec = nullptr;
+	double *cg32_x = nullptr;
+	size_t cg32_cap = 0;
 	std::vector<std::vector<CgTrace>> traces;   // [plane][level]
-	const float *bound_in = nullptr, *bound_dat = nullptr;
-	float *bound_out = nullptr;
+	std::vector<int> cg_iters_finest;
+	hipStream_t cap_stream = nullptr;
 
 	~Plan()
 	{
-		for (void *p : allocs) hipFree(p);
-		if (h_done) hipHostFree(h_done);
+		for (void *p : allocs) (void)hipFree(p);
+		if (h_done) (void)hipHostFree(h_done);
+		if (cap_stream) (void)hipStreamDestroy(cap_stream);
 	}
 	template <typename T> int alloc(T **p, size_t n)
 	{
@@ -180,148 +263,198 @@ static int op_radius(float tstep, float *teff)
 	return 2;
 }
 
-static std::vector<BlockWork> block_list(int w, int h, int pd)
-{
-	std::vector<BlockWork> bw;
-	size_t n = (size_t)w * h, span = 256 * 4;
-	for (int p = 0; p < pd; p++)
-		for (size_t s = 0; s < n; s += span)
-			bw.push_back({p, (int)s});
-	return bw;
-}
+static constexpr int EW_SPAN = 256 * 4;   // pixels per elementwise work block
 
-static int plan_create(Plan &pl, int w, int h, int pd, bool has_f,
-		const sp_params &P)
+static int plan_create(Plan &pl, const std::vector<PlaneIn> &planes,
+		const sp_params &P, bool has_f)
 {
-	pl.w = w; pl.h = h; pl.pd = pd; pl.has_f = has_f; pl.P = P;
+	pl.P = P;
+	pl.planes = planes;
 	pl.R = op_radius(P.tstep, &pl.teff);
-	// pyramid shapes, simpois.c:253-256
-	{
-		int cw = w, ch = h, sc = P.scale;
+	const int np = (int)planes.size();
+	// pyramid depth of each plane, simpois.c:253-256
+	std::vector<std::vector<std::pair<int, int>>> shapes(np);
+	int nlev = 0;
+	for (int p = 0; p < np; p++) {
+		int cw = planes[p].w, ch = planes[p].h, sc = P.scale;
 		for (;;) {
-			LevelPlan L;
-			L.w = cw; L.h = ch;
-			pl.lv.push_back(L);
+			shapes[p].push_back({cw, ch});
 			if (!(sc > 1 && (cw > 1 || ch > 1))) break;
 			cw = (int)ceil(cw / 2.0);
 			ch = (int)ceil(ch / 2.0);
 			sc--;
 		}
+		pl.depth.push_back((int)shapes[p].size());
+		nlev = std::max(nlev, (int)shapes[p].size());
 	}
-	const int nlev = (int)pl.lv.size();
-	TRY(pl.alloc(&pl.d_cnt, (size_t)nlev * pd));
-	TRY(pl.alloc(&pl.d_outs, pd));
-	TRY(pl.alloc(&pl.c0, (size_t)w * h * pd));
+	pl.lv.resize(nlev);
+	size_t ncnt = 0;
+	for (int k = 0; k < nlev; k++)
+		for (int p = 0; p < np; p++)
+			if (pl.depth[p] > k) { pl.lv[k].members.push_back(p); ncnt++; }
+	TRY(pl.alloc(&pl.d_cnt, ncnt));
 	HIPCHK(hipHostMalloc((void **)&pl.h_done, 4 * sizeof(int)));
-	pl.traces.assign(pd, std::vector<CgTrace>(nlev));
+	pl.traces.assign(np, std::vector<CgTrace>(nlev));
+	pl.cg_iters_finest.assign(np, 0);
+	{
+		std::vector<float *> outs(np);
+		for (int p = 0; p < np; p++) outs[p] = planes[p].out;
+		TRY(pl.upload(&pl.d_outs, outs));
+	}
+	size_t cnt_off = 0;
 	for (int k = 0; k < nlev; k++) {
 		LevelPlan &L = pl.lv[k];
-		const size_t n = (size_t)L.w * L.h;
-		float *g = nullptr, *f = nullptr, *u = nullptr, *c = nullptr;
-		if (k > 0) {
-			TRY(pl.alloc(&g, n * pd));
-			if (has_f) TRY(pl.alloc(&f, n * pd));
-			TRY(pl.alloc(&u, n * pd));
-			TRY(pl.alloc(&c, n * pd));
-		} else {
-			c = pl.c0;
-		}
-		L.planes.resize(pd);
-		for (int p = 0; p < pd; p++) {
-			SpPlane &S = L.planes[p];
-			S.g = g ? g + n * p : nullptr;
-			S.f = f ? f + n * p : nullptr;
-			S.u = u ? u + n * p : nullptr;
-			S.c = c + n * p;
-			S.cnt = pl.d_cnt + (size_t)k * pd + p;
-			S.w = L.w; S.h = L.h;
-		}
-		TRY(pl.upload(&L.d_planes, L.planes));
-		auto bw = block_list(L.w, L.h, pd);
-		L.nbw = (int)bw.size();
-		TRY(pl.upload(&L.d_bw, bw));
-
-		// PONLIT restricts the work to the level of that width (:272-273)
-		L.nsweep = std::max(P.niter, 0);
-		L.cgit = P.cgit;
-		if (P.ponlit != 0 && P.ponlit != (double)L.w) L.nsweep = L.cgit = 0;
-
-		std::vector<GsInst> gi(pd);
-		for (int p = 0; p < pd; p++)
-			gi[p] = {L.planes[p].u, L.planes[p].c, L.w, L.h, L.nsweep, pl.teff, nullptr};
-		TRY(pl.upload(&L.d_insts, gi));
-		std::vector<int> ids(pd);
-		for (int p = 0; p < pd; p++) ids[p] = p;
-		TRY(pl.upload(&L.d_ids, ids));
-		if (L.nsweep > 0) {
-			L.small_lds = spk::small_lds_bytes(L.w, L.h);
-			L.small = (long)8 * L.w * L.h <= g_max_lds;
-			if (L.small) {
-				const int ct = pl.R * pl.R + 1;
-				long slots = (long)L.h * ((L.w - 1) / ct + 1);
-				long th = (slots + 63) / 64 * 64;
-				L.small_threads = (int)std::min<long>(1024, std::max<long>(64, th));
+		const int nm = (int)L.members.size();
+		size_t pool_px = 0;
+		for (int p : L.members)
+			pool_px += (size_t)shapes[p][k].first * shapes[p][k].second;
+		float *pool = nullptr;
+		const int arrays = k > 0 ? (has_f ? 4 : 3) : 1;   // g u c [f] | c
+		TRY(pl.alloc(&pool, pool_px * arrays));
+		size_t off = 0;
+		L.sp.resize(nm);
+		for (int m = 0; m < nm; m++) {
+			const int p = L.members[m];
+			const int w = shapes[p][k].first, h = shapes[p][k].second;
+			const size_t n = (size_t)w * h;
+			SpPlane &S = L.sp[m];
+			S.w = w; S.h = h;
+			S.cnt = pl.d_cnt + cnt_off + m;
+			if (k == 0) {
+				S.g = const_cast<float *>(planes[p].g);
+				S.f = const_cast<float *>(planes[p].f);
+				S.u = planes[p].out;
+				S.c = pool + off; off += n;
 			} else {
-				std::vector<GsTile> tiles;
-				if (pl.R == 1) build_tiles<1>(L.w, L.h, L.nsweep, pd, tiles, L.wf_off);
-				else if (pl.R == 2) build_tiles<2>(L.w, L.h, L.nsweep, pd, tiles, L.wf_off);
-				else build_tiles<3>(L.w, L.h, L.nsweep, pd, tiles, L.wf_off);
-				TRY(pl.upload(&L.d_tiles, tiles));
+				S.g = pool + off; off += n;
+				S.u = pool + off; off += n;
+				S.c = pool + off; off += n;
+				S.f = nullptr;
+				if (has_f) { S.f = pool + off; off += n; }
 			}
 		}
+		cnt_off += nm;
+		TRY(pl.upload(&L.d_sp, L.sp));
+		// work blocks, contiguous per member
+		std::vector<BlockWork> bw;
+		std::vector<int> first;
+		for (int m = 0; m < nm; m++) {
+			first.push_back((int)bw.size());
+			const size_t n = (size_t)L.sp[m].w * L.sp[m].h;
+			for (size_t s = 0; s < n; s += EW_SPAN) bw.push_back({m, (int)s});
+		}
+		first.push_back((int)bw.size());
+		L.nbw = (int)bw.size();
+		TRY(pl.upload(&L.d_bw, bw));
+		TRY(pl.upload(&L.d_first, first));
+		TRY(pl.alloc(&L.d_bcnt, bw.size()));
+		// PONLIT restricts the work to the level of that width (:272-273)
+		L.nsweep.assign(nm, std::max(P.niter, 0));
+		L.cgit.assign(nm, P.cgit);
+		for (int m = 0; m < nm; m++)
+			if (P.ponlit != 0 && P.ponlit != (double)L.sp[m].w)
+				L.nsweep[m] = L.cgit[m] = 0;
+		for (int m = 0; m < nm; m++) L.any_cg |= L.cgit[m] != 0;
+		std::vector<GsInst> gi(nm);
+		for (int m = 0; m < nm; m++)
+			gi[m] = {L.sp[m].u, L.sp[m].c, L.sp[m].w, L.sp[m].h, L.nsweep[m],
+				pl.teff, nullptr};
+		TRY(pl.upload(&L.d_insts, gi));
+		// Gauss-Seidel: members that fit in LDS run whole; others in tiles
+		std::vector<int> small;
+		std::vector<std::vector<GsTile>> wf;
+		const int bt = pl.R == 1 ? TileShape<1>::BT : pl.R == 2
+			? TileShape<2>::BT : TileShape<3>::BT;
+		for (int m = 0; m < nm; m++) {
+			const int w = L.sp[m].w, h = L.sp[m].h, T = L.nsweep[m];
+			if (T <= 0) continue;
+			if ((long)8 * w * h <= g_max_lds) {
+				small.push_back(m);
+				const int ct = pl.R * pl.R + 1;
+				const long slots = (long)h * ((w - 1) / ct + 1);
+				const long th = (slots + 63) / 64 * 64;
+				L.small_threads = (int)std::max<long>(L.small_threads,
+						std::min<long>(1024, th));
+				L.small_lds = std::max(L.small_lds, spk::small_lds_bytes(w, h));
+				continue;
+			}
+			std::vector<GsTile> t;
+			std::vector<int> o;
+			if (pl.R == 1) build_tiles<1>(w, h, T, 1, t, o);
+			else if (pl.R == 2) build_tiles<2>(w, h, T, 1, t, o);
+			else build_tiles<3>(w, h, T, 1, t, o);
+			if (wf.size() + 1 < o.size()) wf.resize(o.size() - 1);
+			for (size_t z = 0; z + 1 < o.size(); z++)
+				for (int e = o[z]; e < o[z + 1]; e++) {
+					GsTile tl = t[e];
+					tl.inst = m;
+					wf[z].push_back(tl);
+				}
+			L.tile_pass_px += (long)((T + bt - 1) / bt) * w * h;
+		}
+		L.nsmall = (int)small.size();
+		TRY(pl.upload(&L.d_small, small));
+		std::vector<GsTile> tiles;
+		L.wf_off.assign(1, 0);
+		for (auto &v : wf) {
+			tiles.insert(tiles.end(), v.begin(), v.end());
+			L.wf_off.push_back((int)tiles.size());
+		}
+		TRY(pl.upload(&L.d_tiles, tiles));
 	}
-	return SP_OK;
-}
-
-// point level 0 at the caller's planes (device pointers)
-static int plan_bind(Plan &pl, float *d_out, const float *d_in, const float *d_dat,
-		hipStream_t s)
-{
-	if (pl.bound_in == d_in && pl.bound_dat == d_dat && pl.bound_out == d_out)
-		return SP_OK;
-	LevelPlan &L = pl.lv[0];
-	const size_t n = (size_t)pl.w * pl.h;
-	std::vector<float *> outs(pl.pd);
-	for (int p = 0; p < pl.pd; p++) {
-		SpPlane &S = L.planes[p];
-		S.g = const_cast<float *>(d_in) + n * p;
-		S.f = d_dat ? const_cast<float *>(d_dat) + n * p : nullptr;
-		S.u = d_out + n * p;
-		outs[p] = d_out + n * p;
+	// parent (finer) and coarse descriptors aligned with each level's members
+	for (int k = 0; k < nlev; k++) {
+		LevelPlan &L = pl.lv[k];
+		std::vector<SpPlane> par, co;
+		for (size_t m = 0; m < L.members.size(); m++) {
+			const int p = L.members[m];
+			if (k > 0) {
+				const LevelPlan &F = pl.lv[k - 1];
+				const size_t fm = std::find(F.members.begin(), F.members.end(), p)
+					- F.members.begin();
+				par.push_back(F.sp[fm]);
+			}
+			SpPlane none{};
+			if (k + 1 < nlev && pl.depth[p] > k + 1) {
+				const LevelPlan &C = pl.lv[k + 1];
+				const size_t cm = std::find(C.members.begin(), C.members.end(), p)
+					- C.members.begin();
+				co.push_back(C.sp[cm]);
+			} else {
+				co.push_back(none);
+			}
+		}
+		if (k > 0) TRY(pl.upload(&L.d_parent, par));
+		TRY(pl.upload(&L.d_coarse, co));
 	}
-	HIPCHK(hipMemcpyAsync(L.d_planes, L.planes.data(), pl.pd * sizeof(SpPlane),
-			hipMemcpyHostToDevice, s));
-	std::vector<GsInst> gi(pl.pd);
-	for (int p = 0; p < pl.pd; p++)
-		gi[p] = {L.planes[p].u, L.planes[p].c, L.w, L.h, L.nsweep, pl.teff, nullptr};
-	HIPCHK(hipMemcpyAsync(L.d_insts, gi.data(), pl.pd * sizeof(GsInst),
-			hipMemcpyHostToDevice, s));
-	HIPCHK(hipMemcpyAsync(pl.d_outs, outs.data(), pl.pd * sizeof(float *),
-			hipMemcpyHostToDevice, s));
-	HIPCHK(hipStreamSynchronize(s));   // host vectors die here
-	pl.bound_in = d_in; pl.bound_dat = d_dat; pl.bound_out = d_out;
 	return SP_OK;
 }
 
 static int run_gs(Plan &pl, LevelPlan &L, hipStream_t s)
 {
-	if (L.nsweep <= 0) return SP_OK;
-	if (L.small) {
-		spk::launch_gs_small(pl.R, L.d_insts, L.d_ids, pl.pd, L.small_lds,
+	if (L.nsmall)
+		spk::launch_gs_small(pl.R, L.d_insts, L.d_small, L.nsmall, L.small_lds,
 				L.small_threads, s);
-	} else {
+	if (L.wf_off.size() > 1) {
+		// algorithmic bytes: per pass of BT sweeps, read u and the
+		// coefficient plane and write u once (12 B per pixel)
+		ProfScope pr(pl.R == 1 ? "gs_tile" : "gs_tile_r23", s);
 		for (size_t k = 0; k + 1 < L.wf_off.size(); k++)
 			spk::launch_gs_tiles(pl.R, L.d_tiles + L.wf_off[k],
 					L.wf_off[k + 1] - L.wf_off[k], L.d_insts, s);
+		pr.add((long)L.wf_off.size() - 1, 12.0 * L.tile_pass_px);
 	}
 	HIPCHK(hipGetLastError());
 	return SP_OK;
 }
 
-static int cg_ensure(Plan &pl, size_t n, int nblk, int maxit)
+// ---------------------------------------------------------- FP64 CG
+
+static int cg64_ensure(Plan &pl, int maxit)
 {
 	if (!pl.cg_x) {
-		const size_t n0 = (size_t)pl.w * pl.h;
+		size_t n0 = 0;
+		for (auto &p : pl.planes) n0 = std::max(n0, (size_t)p.w * p.h);
 		TRY(pl.alloc(&pl.cg_x, n0));
 		TRY(pl.alloc(&pl.cg_r, n0));
 		TRY(pl.alloc(&pl.cg_p, n0));
@@ -329,45 +462,44 @@ static int cg_ensure(Plan &pl, size_t n, int nblk, int maxit)
 		TRY(pl.alloc(&pl.cg_part, 2 * 1024));
 		TRY(pl.alloc(&pl.cg_done, 4));
 	}
-	(void)n; (void)nblk;
 	if (maxit + 2 > pl.cg_rr_cap) {
-		int cap = std::max(maxit + 2, 1024);
+		const int cap = std::max(maxit + 2, 1024);
 		TRY(pl.alloc(&pl.cg_rr, cap));
 		pl.cg_rr_cap = cap;
 	}
 	return SP_OK;
 }
 
-// masked CG on one plane of one level (cleant_cgpois.c:197-264), bilap
-// selects the biharmonic operator; eps is the built-in threshold
-static int run_cg_plane(Plan &pl, int k, int p, int cgit, int bilap, float eps,
+// masked CG on one plane of one level in FP64 (cleant_cgpois.c:197-264)
+static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 		hipStream_t s)
 {
 	LevelPlan &L = pl.lv[k];
-	SpPlane &S = L.planes[p];
+	const SpPlane &S = L.sp[m];
 	int nmask = 0;
 	HIPCHK(hipMemcpyAsync(&nmask, S.cnt, sizeof(int), hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
-	if (nmask == 0) return SP_OK;   // :203-207, the level rule covers it
-	float e = pl.P.cg_eps >= 0 ? (float)pl.P.cg_eps : eps;
-	float fm = cgit == -1 ? (float)nmask : (float)cgit;
-	int maxit = (int)fm;
+	if (nmask == 0) return SP_OK;   // :203-207, the level output rule covers it
+	const float e = pl.P.cg_eps >= 0 ? (float)pl.P.cg_eps : eps;
+	const float fm = cgit == -1 ? (float)nmask : (float)cgit;
+	const int maxit = (int)fm;
 	if (maxit <= 0) return SP_OK;   // x = x0: u is unchanged
-	const size_t n = (size_t)L.w * L.h;
-	int nblk = (int)std::min<size_t>(1024, std::max<size_t>(1, (n + 2047) / 2048));
-	TRY(cg_ensure(pl, n, nblk, maxit));
+	const size_t n = (size_t)S.w * S.h;
+	const int nblk = (int)std::min<size_t>(1024, std::max<size_t>(1, (n + 2047) / 2048));
+	TRY(cg64_ensure(pl, maxit));
 	spk::CgDev d;
 	d.g = S.g; d.f = S.f; d.u = S.u;
 	d.x = pl.cg_x; d.r = pl.cg_r; d.p = pl.cg_p; d.q = pl.cg_q;
 	d.part = pl.cg_part; d.rr = pl.cg_rr; d.done = pl.cg_done;
-	d.w = L.w; d.h = L.h; d.bilap = bilap; d.nblk = nblk;
+	d.w = S.w; d.h = S.h; d.bilap = bilap; d.nblk = nblk;
 	d.stop = (double)e;
+	ProfScope pr("cg64", s);
 	spk::launch_cg_setup(d, s);
 	int it = 0, chunk = 16, ran = maxit;
 	while (it < maxit) {
-		int m = std::min(chunk, maxit - it);
-		for (int c = 0; c < m; c++) spk::launch_cg_iter(d, it + c, s);
-		it += m;
+		const int c = std::min(chunk, maxit - it);
+		for (int z = 0; z < c; z++) spk::launch_cg_iter(d, it + z, s);
+		it += c;
 		HIPCHK(hipGetLastError());
 		HIPCHK(hipMemcpyAsync(pl.h_done, pl.cg_done, 2 * sizeof(int),
 				hipMemcpyDeviceToHost, s));
@@ -377,14 +509,147 @@ static int run_cg_plane(Plan &pl, int k, int p, int cgit, int bilap, float eps,
 	}
 	spk::launch_cg_finish(d, s);
 	HIPCHK(hipGetLastError());
-	CgTrace &tr = pl.traces[p][k];
+	// FP64 vectors: 88 B per unknown and iteration (twice the FP32 vectors)
+	pr.add(3L * it, 88.0 * ran * nmask);
+	CgTrace &tr = pl.traces[L.members[m]][k];
 	tr.n = nmask;
 	tr.rr.resize(ran);
 	HIPCHK(hipMemcpyAsync(tr.rr.data(), pl.cg_rr + 1, ran * sizeof(double),
 			hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
+	if (k == 0) pl.cg_iters_finest[L.members[m]] = ran;
 	return SP_OK;
 }
+
+// ---------------------------------------------------------- FP32 CG
+
+// batched FP32 CG over the members of level k with a non-zero -c
+static int run_cg32(Plan &pl, int k, int bilap, float eps, hipStream_t s)
+{
+	LevelPlan &L = pl.lv[k];
+	std::vector<int> ms;
+	for (size_t m = 0; m < L.members.size(); m++)
+		if (L.cgit[m] != 0) ms.push_back((int)m);
+	if (ms.empty()) return SP_OK;
+	size_t need = 0;
+	for (int m : ms) need += (size_t)L.sp[m].w * L.sp[m].h;
+	if (need > pl.cg32_cap) {
+		TRY(pl.alloc(&pl.cg32_vec, need * 3));
+		TRY(pl.alloc(&pl.cg32_x, need));
+		pl.cg32_cap = need;
+	}
+	std::vector<Cg32Plane> cp;
+	std::vector<Cg32Scal> sc;
+	std::vector<BlockWork> bw;
+	size_t off = 0, offx = 0;
+	long maxit_host = 0;
+	int trace_len = 0;
+	for (size_t z = 0; z < ms.size(); z++) {
+		const int m = ms[z];
+		const SpPlane &S = L.sp[m];
+		const size_t n = (size_t)S.w * S.h;
+		Cg32Plane c{};
+		c.g = S.g; c.f = S.f; c.u = S.u; c.cnt = S.cnt;
+		c.x = pl.cg32_x + offx; offx += n;
+		c.r = pl.cg32_vec + off; off += n;
+		c.p = pl.cg32_vec + off; off += n;
+		c.q = pl.cg32_vec + off; off += n;
+		c.w = S.w; c.h = S.h; c.bilap = bilap;
+		c.blk0 = (int)bw.size();
+		for (size_t st = 0; st < n; st += EW_SPAN) bw.push_back({(int)z, (int)st});
+		c.nblk = (int)bw.size() - c.blk0;
+		cp.push_back(c);
+		Cg32Scal q{};
+		const int cg = L.cgit[m];
+		q.rel = pl.P.cg_rel > 0;
+		q.eps = q.rel ? pl.P.cg_rel : (pl.P.cg_eps >= 0 ? (double)(float)pl.P.cg_eps
+				: (double)eps);
+		q.maxit_from_count = cg == -1;
+		q.maxit = cg == -1 ? 0 : std::max(cg, 0);
+		const int cap = cg == -1 ? (int)std::min<size_t>(n, 1 << 20) : std::max(cg, 0);
+		q.trace0 = trace_len;
+		q.trace_cap = pl.P.trace ? cap : 0;
+		trace_len += q.trace_cap;
+		maxit_host = std::max<long>(maxit_host, cg == -1 ? (long)n : cg);
+		sc.push_back(q);
+	}
+	Cg32Batch B{};
+	TRY(pl.upload(&B.planes, cp));
+	TRY(pl.upload(&B.scal, sc));
+	TRY(pl.upload(&B.bw, bw));
+	TRY(pl.alloc(&B.part, 2 * bw.size()));
+	TRY(pl.alloc(&B.nactive, 1));
+	if (trace_len) TRY(pl.alloc(&B.trace, trace_len));
+	B.nplanes = (int)ms.size();
+	B.nblk = (int)bw.size();
+	HIPCHK(hipMemsetAsync(B.nactive, 0, sizeof(int), s));
+	ProfScope pr("cg32", s);
+	spk::cg32_setup(B, s);
+	// iterations in chunks captured as one graph, replayed until every
+	// plane has stopped
+	const int chunk = std::max(1, pl.P.cg_chunk);
+	if (!pl.cap_stream)
+		HIPCHK(hipStreamCreateWithFlags(&pl.cap_stream, hipStreamNonBlocking));
+	hipGraphExec_t exec = nullptr;
+	{
+		hipGraph_t graph;
+		HIPCHK(hipStreamBeginCapture(pl.cap_stream, hipStreamCaptureModeThreadLocal));
+		for (int z = 0; z < chunk; z++) spk::cg32_iter(B, pl.cap_stream);
+		HIPCHK(hipStreamEndCapture(pl.cap_stream, &graph));
+		const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+		(void)hipGraphDestroy(graph);
+		HIPCHK(ie);
+	}
+	long launched = 0;
+	int rc = SP_OK;
+	for (long done = 0; done < maxit_host; done += chunk) {
+		if (hipGraphLaunch(exec, s) != hipSuccess) {
+			rc = set_err(SP_ERR_HIP, "hipGraphLaunch failed");
+			break;
+		}
+		launched += chunk;
+		if (hipMemcpyAsync(pl.h_done, B.nactive, sizeof(int), hipMemcpyDeviceToHost, s)
+				!= hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+			rc = set_err(SP_ERR_HIP, "CG progress check failed");
+			break;
+		}
+		if (pl.h_done[0] == 0) break;
+	}
+	(void)hipGraphExecDestroy(exec);
+	TRY(rc);
+	spk::cg32_finish(B, s);
+	HIPCHK(hipGetLastError());
+	// per-plane results: iterations and trace
+	HIPCHK(hipMemcpyAsync(sc.data(), B.scal, sc.size() * sizeof(Cg32Scal),
+			hipMemcpyDeviceToHost, s));
+	std::vector<double> trace(trace_len);
+	if (trace_len)
+		HIPCHK(hipMemcpyAsync(trace.data(), B.trace, trace_len * sizeof(double),
+				hipMemcpyDeviceToHost, s));
+	std::vector<int> cnts(ms.size());
+	for (size_t z = 0; z < ms.size(); z++)
+		HIPCHK(hipMemcpyAsync(&cnts[z], L.sp[ms[z]].cnt, sizeof(int),
+				hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
+	double unk = 0;
+	for (size_t z = 0; z < ms.size(); z++) {
+		const int p = L.members[ms[z]];
+		CgTrace &tr = pl.traces[p][k];
+		tr.n = cnts[z];
+		const int it = sc[z].iters;
+		tr.rr.assign(trace.begin() + sc[z].trace0,
+				trace.begin() + sc[z].trace0 + std::min(it, sc[z].trace_cap));
+		if (k == 0) pl.cg_iters_finest[p] = it;
+		unk += (double)it * cnts[z];
+	}
+	// algorithmic bytes per unknown and iteration: spmv reads p, writes q
+	// (8); update reads x (double) r p q, writes x r (32); direction reads
+	// r p, writes p (12): 52 B (44 B with a float x)
+	pr.add(5 * launched, 52.0 * unk);
+	return SP_OK;
+}
+
+// -------------------------------------------------------------- solve
 
 static int plan_execute(Plan &pl, hipStream_t s, sp_stats *st)
 {
@@ -394,43 +659,55 @@ static int plan_execute(Plan &pl, hipStream_t s, sp_stats *st)
 	HIPCHK(hipEventCreate(&e1));
 	HIPCHK(hipEventRecord(e0, s));
 	for (auto &v : pl.traces) for (auto &t : v) { t.n = 0; t.rr.clear(); }
-	HIPCHK(hipMemsetAsync(pl.d_cnt, 0, sizeof(int) * nlev * pl.pd, s));
 	// pyramid of data, simpois.c:260-262
-	spk::launch_prepare(pl.lv[0].d_planes, pl.lv[0].d_bw, pl.lv[0].nbw, s);
-	for (int k = 1; k < nlev; k++)
-		spk::launch_restrict(pl.lv[k - 1].d_planes, pl.lv[k].d_planes,
-				pl.lv[k].d_bw, pl.lv[k].nbw, pl.P.pmsfac, s);
-	HIPCHK(hipGetLastError());
+	{
+		ProfScope pr("pyramid", s);
+		LevelPlan &L0 = pl.lv[0];
+		spk::launch_prepare(L0.d_sp, L0.d_bw, L0.nbw, L0.d_bcnt, s);
+		spk::launch_count(L0.d_sp, (int)L0.members.size(), L0.d_first, L0.d_bcnt, s);
+		for (int k = 1; k < nlev; k++) {
+			LevelPlan &L = pl.lv[k];
+			spk::launch_restrict(L.d_parent, L.d_sp, L.d_bw, L.nbw, pl.P.pmsfac,
+					L.d_bcnt, s);
+			spk::launch_count(L.d_sp, (int)L.members.size(), L.d_first, L.d_bcnt, s);
+		}
+		HIPCHK(hipGetLastError());
+	}
 	// coarse to fine
+	const bool pois = pl.P.tstep > 0;
 	for (int k = nlev - 1; k >= 0; k--) {
 		LevelPlan &L = pl.lv[k];
-		spk::launch_init(L.d_planes, k + 1 < nlev ? pl.lv[k + 1].d_planes : nullptr,
-				L.d_bw, L.nbw, s);
+		spk::launch_init(L.d_sp, L.d_coarse, L.d_bw, L.nbw, s);
 		TRY(run_gs(pl, L, s));
-		if (L.cgit) {   // simpois.c:278-284; cgit == 0 is a no-op there
-			const bool pois = pl.P.tstep > 0;
-			for (int p = 0; p < pl.pd; p++)
-				TRY(run_cg_plane(pl, k, p, L.cgit, pois ? 0 : 1,
-						pois ? 1e-6f : 1e-9f, s));
+		if (L.any_cg) {   // simpois.c:278-284; cgit == 0 is a no-op there
+			const float eps = pois ? 1e-6f : 1e-9f;
+			if (pl.P.cg_fp32) {
+				TRY(run_cg32(pl, k, pois ? 0 : 1, eps, s));
+			} else {
+				for (size_t m = 0; m < L.members.size(); m++)
+					if (L.cgit[m])
+						TRY(run_cg64(pl, k, (int)m, L.cgit[m], pois ? 0 : 1, eps, s));
+			}
 		}
 	}
-	spk::launch_finalize(pl.lv[0].d_planes, pl.d_outs, pl.lv[0].d_bw,
-			pl.lv[0].nbw, s);
+	spk::launch_finalize(pl.lv[0].d_sp, pl.d_outs, pl.lv[0].d_bw, pl.lv[0].nbw, s);
 	HIPCHK(hipGetLastError());
 	HIPCHK(hipEventRecord(e1, s));
 	HIPCHK(hipEventSynchronize(e1));
 	float ms = 0;
-	hipEventElapsedTime(&ms, e0, e1);
-	hipEventDestroy(e0);
-	hipEventDestroy(e1);
+	(void)hipEventElapsedTime(&ms, e0, e1);
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
 	if (st) {
 		st->levels = nlev;
-		st->cg_iters_finest = (int)pl.traces[0][0].rr.size();
+		int mx = 0;
+		for (int v : pl.cg_iters_finest) mx = std::max(mx, v);
+		st->cg_iters_finest = mx;
 		st->elapsed_ms = ms;
 	}
-	if (pl.P.trace)   // reference order: channel by channel, coarse to fine
-		for (int p = 0; p < pl.pd; p++)
-			for (int k = nlev - 1; k >= 0; k--) {
+	if (pl.P.trace)   // reference order: plane by plane, coarse to fine
+		for (size_t p = 0; p < pl.planes.size(); p++)
+			for (int k = pl.depth[p] - 1; k >= 0; k--) {
 				const CgTrace &t = pl.traces[p][k];
 				for (size_t i = 0; i < t.rr.size(); i++)
 					fprintf(stderr, "n=%d iter=%d, rr_new=%g\n", t.n, (int)i, t.rr[i]);
@@ -438,37 +715,57 @@ static int plan_execute(Plan &pl, hipStream_t s, sp_stats *st)
 	return SP_OK;
 }
 
-static int check_args(const void *out, const void *in, int w, int h, int pd,
-		const sp_params *p)
+static int check_plane_args(const void *out, const void *in, int w, int h, int pd)
 {
-	if (!out || !in || !p) return set_err(SP_ERR_ARG, "NULL buffer or params");
+	if (!out || !in) return set_err(SP_ERR_ARG, "NULL buffer");
 	if (w < 1 || h < 1 || pd < 1) return set_err(SP_ERR_ARG, "bad size %dx%dx%d", w, h, pd);
 	if ((long)w * h > (1L << 30)) return set_err(SP_ERR_ARG, "image too large");
 	if (out == in) return set_err(SP_ERR_ARG, "out must not alias in");
 	return SP_OK;
 }
 
+extern "C" int sp_solve_batch_device(int n, const sp_image *imgs,
+		const sp_params *p, void *stream, sp_stats *st)
+{
+	if (n < 1 || !imgs || !p) return set_err(SP_ERR_ARG, "empty batch");
+	TRY(ensure_device());
+	std::vector<PlaneIn> planes;
+	bool has_f = false;
+	for (int i = 0; i < n; i++) {
+		const sp_image &I = imgs[i];
+		TRY(check_plane_args(I.out, I.g, I.w, I.h, I.pd));
+		const size_t wh = (size_t)I.w * I.h;
+		has_f |= I.f != nullptr;
+		for (int l = 0; l < I.pd; l++)
+			planes.push_back({I.w, I.h, I.g + wh * l, I.f ? I.f + wh * l : nullptr,
+				I.out + wh * l});
+	}
+	Plan pl;
+	TRY(plan_create(pl, planes, *p, has_f));
+	return plan_execute(pl, (hipStream_t)stream, st);
+}
+
 extern "C" int sp_solve_device(float *d_out, const float *d_in, const float *d_dat,
 		int w, int h, int pd, const sp_params *p, void *stream, sp_stats *st)
 {
-	TRY(check_args(d_out, d_in, w, h, pd, p));
-	TRY(ensure_device());
-	hipStream_t s = (hipStream_t)stream;
-	Plan pl;
-	TRY(plan_create(pl, w, h, pd, d_dat != nullptr, *p));
-	TRY(plan_bind(pl, d_out, d_in, d_dat, s));
-	return plan_execute(pl, s, st);
+	if (!p) return set_err(SP_ERR_ARG, "NULL params");
+	TRY(check_plane_args(d_out, d_in, w, h, pd));
+	sp_image I{w, h, pd, d_in, d_dat, d_out};
+	return sp_solve_batch_device(1, &I, p, stream, st);
 }
 
 extern "C" int sp_solve_host(float *out, const float *in, const float *dat,
 		int w, int h, int pd, const sp_params *p, sp_stats *st)
 {
-	TRY(check_args(out, in, w, h, pd, p));
+	if (!p) return set_err(SP_ERR_ARG, "NULL params");
+	TRY(check_plane_args(out, in, w, h, pd));
 	TRY(ensure_device());
 	const size_t n = (size_t)w * h * pd;
 	float *d_in = nullptr, *d_dat = nullptr, *d_out = nullptr;
 	int rc = SP_OK;
-	auto cleanup = [&]() { hipFree(d_in); hipFree(d_dat); hipFree(d_out); };
+	auto cleanup = [&]() {
+		(void)hipFree(d_in); (void)hipFree(d_dat); (void)hipFree(d_out);
+	};
 	if (hipMalloc(&d_in, n * 4) != hipSuccess || hipMalloc(&d_out, n * 4) != hipSuccess
 			|| (dat && hipMalloc(&d_dat, n * 4) != hipSuccess)) {
 		cleanup();
@@ -482,6 +779,59 @@ extern "C" int sp_solve_host(float *out, const float *in, const float *dat,
 		rc = set_err(SP_ERR_HIP, "download failed");
 	cleanup();
 	return rc;
+}
+
+// ------------------------------------------------------------- adjoint
+
+extern "C" int sp_adjoint_batch_device(int n, const sp_adjoint_image *imgs,
+		const sp_params *p, void *stream, sp_stats *st)
+{
+	if (n < 1 || !imgs || !p) return set_err(SP_ERR_ARG, "empty batch");
+	if (!(p->tstep > 0))
+		return set_err(SP_ERR_ARG, "the adjoint covers the Poisson/Laplace fill "
+				"(tstep > 0): the clamped bilaplacian is not symmetric");
+	TRY(ensure_device());
+	hipStream_t s = (hipStream_t)stream;
+	Plan tmp;   // owns the scratch planes
+	size_t total = 0;
+	for (int i = 0; i < n; i++) {
+		const sp_adjoint_image &I = imgs[i];
+		if (!I.g || !I.grad_u || !I.grad_g || I.w < 1 || I.h < 1 || I.pd < 1)
+			return set_err(SP_ERR_ARG, "bad adjoint image %d", i);
+		total += (size_t)I.w * I.h * I.pd;
+	}
+	float *scratch;
+	TRY(tmp.alloc(&scratch, 3 * total));
+	std::vector<AdjPlane> ap;
+	std::vector<BlockWork> bw;
+	std::vector<sp_image> fwd(n);
+	size_t off = 0;
+	for (int i = 0; i < n; i++) {
+		const sp_adjoint_image &I = imgs[i];
+		const size_t wh = (size_t)I.w * I.h, m = wh * I.pd;
+		float *g2 = scratch + off, *f2 = scratch + total + off,
+			*lam = scratch + 2 * total + off;
+		off += m;
+		fwd[i] = sp_image{I.w, I.h, I.pd, g2, f2, lam};
+		for (int l = 0; l < I.pd; l++) {
+			const int id = (int)ap.size();
+			ap.push_back({I.g + wh * l, I.grad_u + wh * l, g2 + wh * l, f2 + wh * l,
+				lam + wh * l, I.grad_g + wh * l, I.grad_f ? I.grad_f + wh * l : nullptr,
+				I.w, I.h});
+			for (size_t st2 = 0; st2 < wh; st2 += EW_SPAN) bw.push_back({id, (int)st2});
+		}
+	}
+	AdjPlane *d_ap;
+	BlockWork *d_bw;
+	TRY(tmp.upload(&d_ap, ap));
+	TRY(tmp.upload(&d_bw, bw));
+	spk::launch_adj_prepare(d_ap, d_bw, (int)bw.size(), s);
+	HIPCHK(hipGetLastError());
+	TRY(sp_solve_batch_device(n, fwd.data(), p, stream, st));
+	spk::launch_adj_gather(d_ap, d_bw, (int)bw.size(), s);
+	HIPCHK(hipGetLastError());
+	HIPCHK(hipStreamSynchronize(s));
+	return SP_OK;
 }
 
 // ------------------------------------------------------------ residual
@@ -500,9 +850,12 @@ extern "C" int sp_residual_device(const float *d_u, const float *d_g,
 		planes[p] = {const_cast<float *>(d_g) + n * p,
 			d_f ? const_cast<float *>(d_f) + n * p : nullptr,
 			const_cast<float *>(d_u) + n * p, nullptr, nullptr, w, h};
-	auto bw = block_list(w, h, pd);
-	std::vector<int> first(pd + 1, 0);
-	for (size_t k = 0; k < bw.size(); k++) first[bw[k].plane + 1] = (int)k + 1;
+	std::vector<BlockWork> bw;
+	std::vector<int> first(1, 0);
+	for (int p = 0; p < pd; p++) {
+		for (size_t st = 0; st < n; st += EW_SPAN) bw.push_back({p, (int)st});
+		first.push_back((int)bw.size());
+	}
 	SpPlane *dp; BlockWork *dbw; int *dfirst; double *part, *out;
 	TRY(pl.upload(&dp, planes));
 	TRY(pl.upload(&dbw, bw));
@@ -529,26 +882,26 @@ static int cg_extend_host(float *out, const float *in, const float *dat, int w,
 	if (!out || !in || !init || w < 1 || h < 1)
 		return set_err(SP_ERR_ARG, "bad CG arguments");
 	TRY(ensure_device());
+	const size_t n = (size_t)w * h;
 	sp_params P;
 	sp_params_default(&P);
 	P.niter = 0; P.scale = 1; P.cgit = maxit;
 	P.tstep = bilap ? -1.0f : 1.0f;
 	P.ponlit = 0;
-	Plan pl;
-	TRY(plan_create(pl, w, h, 1, dat != nullptr, P));
-	const size_t n = (size_t)w * h;
 	float *d_in, *d_dat = nullptr, *d_u;
+	Plan pl;
 	TRY(pl.alloc(&d_in, n));
 	TRY(pl.alloc(&d_u, n));
 	if (dat) TRY(pl.alloc(&d_dat, n));
 	HIPCHK(hipMemcpy(d_in, in, n * 4, hipMemcpyHostToDevice));
 	if (dat) HIPCHK(hipMemcpy(d_dat, dat, n * 4, hipMemcpyHostToDevice));
 	HIPCHK(hipMemcpy(d_u, init, n * 4, hipMemcpyHostToDevice));
+	TRY(plan_create(pl, {PlaneIn{w, h, d_in, d_dat, d_u}}, P, dat != nullptr));
 	hipStream_t s = nullptr;
-	TRY(plan_bind(pl, d_u, d_in, d_dat, s));
-	HIPCHK(hipMemsetAsync(pl.d_cnt, 0, sizeof(int), s));
-	spk::launch_prepare(pl.lv[0].d_planes, pl.lv[0].d_bw, pl.lv[0].nbw, s);
-	TRY(run_cg_plane(pl, 0, 0, maxit, bilap, eps, s));
+	LevelPlan &L0 = pl.lv[0];
+	spk::launch_prepare(L0.d_sp, L0.d_bw, L0.nbw, L0.d_bcnt, s);
+	spk::launch_count(L0.d_sp, 1, L0.d_first, L0.d_bcnt, s);
+	TRY(run_cg64(pl, 0, 0, maxit, bilap, eps, s));
 	// :251-256 and the nmask == 0 branch :203-207
 	int nmask = 0;
 	HIPCHK(hipMemcpy(&nmask, pl.d_cnt, sizeof(int), hipMemcpyDeviceToHost));

@@ -19,13 +19,27 @@ struct SpPlane {
 // Work block for elementwise kernels over many planes of uneven size.
 struct BlockWork { int plane; int start; };
 
+// One plane of an adjoint (gradient) computation.
+struct AdjPlane {
+	const float *g;    // forward boundary data, NaN = hole
+	const float *v;    // dL/du
+	float *g2, *f2;    // adjoint system: 0 / NaN boundary, data -v
+	float *lam;        // adjoint solution
+	float *grad_g, *grad_f;   // outputs (grad_f may be null)
+	int w, h;
+};
+
 namespace spk {
 
 // elementwise / pyramid
+// prepare / restrict write the holes of each work block to bcnt;
+// launch_count folds them into each member's counter
 void launch_prepare(const SpPlane *planes, const BlockWork *bw, int nblocks,
-		hipStream_t s);
+		int *bcnt, hipStream_t s);
 void launch_restrict(const SpPlane *fine, const SpPlane *coarse,
-		const BlockWork *bw, int nblocks, double pmsfac, hipStream_t s);
+		const BlockWork *bw, int nblocks, double pmsfac, int *bcnt, hipStream_t s);
+void launch_count(const SpPlane *planes, int nplanes, const int *first,
+		const int *bcnt, hipStream_t s);
 void launch_init(const SpPlane *lv, const SpPlane *coarse, const BlockWork *bw,
 		int nblocks, hipStream_t s);
 void launch_finalize(const SpPlane *lv, float *const *outs,
@@ -53,6 +67,12 @@ struct CgDev {
 void launch_cg_setup(const CgDev &d, hipStream_t s);
 void launch_cg_iter(const CgDev &d, int it, hipStream_t s);
 void launch_cg_finish(const CgDev &d, hipStream_t s);
+
+// adjoint of the converged Poisson fill
+void launch_adj_prepare(const AdjPlane *ap, const BlockWork *bw, int nblocks,
+		hipStream_t s);
+void launch_adj_gather(const AdjPlane *ap, const BlockWork *bw, int nblocks,
+		hipStream_t s);
 
 // FP64 relative residual of the Poisson (bilap=0) or biharmonic system
 void launch_residual(const SpPlane *lv, const BlockWork *bw, int nblocks,

@@ -59,13 +59,12 @@ static __device__ double block_sum_dbl(double v, double *red)
 
 constexpr int EW_NT = 256;   // threads per elementwise block
 constexpr int EW_PPT = 4;    // pixels per thread
-constexpr int EW_SPAN = EW_NT * EW_PPT;
 
 // ------------------------------------------------------------ pyramid ops
 
 // finest level: coefficient plane and hole count
 __global__ __launch_bounds__(EW_NT) void k_prepare(const SpPlane *planes,
-		const BlockWork *bw)
+		const BlockWork *bw, int *bcnt)
 {
 	__shared__ int red[EW_NT / 64];
 	const BlockWork b = bw[blockIdx.x];
@@ -80,12 +79,12 @@ __global__ __launch_bounds__(EW_NT) void k_prepare(const SpPlane *planes,
 		P.c[q] = coef_of(g, P.f, q);
 	}
 	int s = block_sum_int<EW_NT>(holes, red);
-	if (threadIdx.x == 0 && s) atomicAdd(P.cnt, s);
+	if (threadIdx.x == 0) bcnt[blockIdx.x] = s;
 }
 
 // 2x2 NaN-aware mean of g (and f, times pmsfac), simpois.c:172-196,260-262
 __global__ __launch_bounds__(EW_NT) void k_restrict(const SpPlane *fine,
-		const SpPlane *coarse, const BlockWork *bw, double pmsfac)
+		const SpPlane *coarse, const BlockWork *bw, double pmsfac, int *bcnt)
 {
 	__shared__ int red[EW_NT / 64];
 	const BlockWork b = bw[blockIdx.x];
@@ -121,7 +120,19 @@ __global__ __launch_bounds__(EW_NT) void k_restrict(const SpPlane *fine,
 		C.c[q] = coef_of(g, C.f, q);
 	}
 	int s = block_sum_int<EW_NT>(holes, red);
-	if (threadIdx.x == 0 && s) atomicAdd(C.cnt, s);
+	if (threadIdx.x == 0) bcnt[blockIdx.x] = s;
+}
+
+// one block per plane: its hole count is the sum of its work blocks' counts
+__global__ __launch_bounds__(EW_NT) void k_count(const SpPlane *planes,
+		const int *first, const int *bcnt)
+{
+	__shared__ int red[EW_NT / 64];
+	int v = 0;
+	for (int k = first[blockIdx.x] + threadIdx.x; k < first[blockIdx.x + 1]; k += EW_NT)
+		v += bcnt[k];
+	const int s = block_sum_int<EW_NT>(v, red);
+	if (threadIdx.x == 0) *planes[blockIdx.x].cnt = s;
 }
 
 // value of the coarse output at q: after a level, simpois writes the input
@@ -145,8 +156,8 @@ __global__ __launch_bounds__(EW_NT) void k_init(const SpPlane *lv,
 		if (q >= n) break;
 		float g = P.g[q];
 		float init = 0.0f;
-		if (coarse && !isfinite(g)) {
-			const SpPlane C = coarse[b.plane];
+		const SpPlane C = coarse[b.plane];   // C.g == nullptr: coarsest level
+		if (C.g && !isfinite(g)) {
 			int i = (int)(q % P.w), j = (int)(q / P.w);
 			float x = (float)((i - 0.5) / 2), y = (float)((j - 0.5) / 2);
 			int ix = (int)floorf(x), iy = (int)floorf(y);
@@ -183,15 +194,21 @@ __global__ __launch_bounds__(EW_NT) void k_finalize(const SpPlane *lv,
 }
 
 void launch_prepare(const SpPlane *planes, const BlockWork *bw, int nblocks,
-		hipStream_t s)
+		int *bcnt, hipStream_t s)
 {
-	if (nblocks) k_prepare<<<nblocks, EW_NT, 0, s>>>(planes, bw);
+	if (nblocks) k_prepare<<<nblocks, EW_NT, 0, s>>>(planes, bw, bcnt);
 }
 
 void launch_restrict(const SpPlane *fine, const SpPlane *coarse,
-		const BlockWork *bw, int nblocks, double pmsfac, hipStream_t s)
+		const BlockWork *bw, int nblocks, double pmsfac, int *bcnt, hipStream_t s)
 {
-	if (nblocks) k_restrict<<<nblocks, EW_NT, 0, s>>>(fine, coarse, bw, pmsfac);
+	if (nblocks) k_restrict<<<nblocks, EW_NT, 0, s>>>(fine, coarse, bw, pmsfac, bcnt);
+}
+
+void launch_count(const SpPlane *planes, int nplanes, const int *first,
+		const int *bcnt, hipStream_t s)
+{
+	if (nplanes) k_count<<<nplanes, EW_NT, 0, s>>>(planes, first, bcnt);
 }
 
 void launch_init(const SpPlane *lv, const SpPlane *coarse, const BlockWork *bw,
@@ -557,6 +574,71 @@ void launch_residual_reduce(const double *part, const int *plane_first,
 	if (nplanes)
 		k_residual_reduce<<<(nplanes + 63) / 64, 64, 0, s>>>(part, plane_first,
 				nplanes, out);
+}
+
+} // namespace spk
+
+// ------------------------------------------------------------- adjoint
+//
+// Converged solve of the masked Poisson system A u_h = -f_h - A_hk g_k
+// (A = -Laplacian, clamped, symmetric).  For a loss with gradient v = dL/du:
+//   lambda = A^-1 v_h                         (the adjoint: a masked solve)
+//   dL/df_h = -lambda_h,   dL/df_k = 0
+//   dL/dg_k = v_k + sum of lambda over the hole 4-neighbours of k
+//   dL/dg_h = 0 at holes (NaN inputs)
+namespace spk {
+
+// adjoint system of one plane: boundary 0 at known pixels, data -v at holes
+__global__ __launch_bounds__(EW_NT) void k_adj_prepare(const AdjPlane *ap,
+		const BlockWork *bw)
+{
+	const BlockWork b = bw[blockIdx.x];
+	const AdjPlane A = ap[b.plane];
+	const size_t n = (size_t)A.w * A.h;
+	for (int k = 0; k < EW_PPT; k++) {
+		const size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
+		if (q >= n) break;
+		const bool hole = isnan(A.g[q]);
+		A.g2[q] = hole ? __uint_as_float(SP_QNAN_BITS) : 0.0f;
+		A.f2[q] = hole ? -A.v[q] : 0.0f;
+	}
+}
+
+__global__ __launch_bounds__(EW_NT) void k_adj_gather(const AdjPlane *ap,
+		const BlockWork *bw)
+{
+	const BlockWork b = bw[blockIdx.x];
+	const AdjPlane A = ap[b.plane];
+	const size_t n = (size_t)A.w * A.h;
+	for (int k = 0; k < EW_PPT; k++) {
+		const size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
+		if (q >= n) break;
+		const int i = (int)(q % A.w), j = (int)(q / A.w);
+		if (isnan(A.g[q])) {
+			A.grad_g[q] = 0.0f;
+			if (A.grad_f) A.grad_f[q] = -A.lam[q];
+			continue;
+		}
+		float s = A.v[q];
+		if (i + 1 < A.w && isnan(A.g[q + 1])) s += A.lam[q + 1];
+		if (j + 1 < A.h && isnan(A.g[q + A.w])) s += A.lam[q + A.w];
+		if (i > 0 && isnan(A.g[q - 1])) s += A.lam[q - 1];
+		if (j > 0 && isnan(A.g[q - A.w])) s += A.lam[q - A.w];
+		A.grad_g[q] = s;
+		if (A.grad_f) A.grad_f[q] = 0.0f;
+	}
+}
+
+void launch_adj_prepare(const AdjPlane *ap, const BlockWork *bw, int nblocks,
+		hipStream_t s)
+{
+	if (nblocks) k_adj_prepare<<<nblocks, EW_NT, 0, s>>>(ap, bw);
+}
+
+void launch_adj_gather(const AdjPlane *ap, const BlockWork *bw, int nblocks,
+		hipStream_t s)
+{
+	if (nblocks) k_adj_gather<<<nblocks, EW_NT, 0, s>>>(ap, bw);
 }
 
 } // namespace spk

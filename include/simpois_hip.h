@@ -74,7 +74,21 @@ typedef struct sp_params {
 	double ponlit;   /* PONLIT */
 	double cg_eps;   /* CG_EPS (< 0: built-in 1e-6 / 1e-9) */
 	int trace;       /* print the CG trace to stderr like minicg.c:46 */
+	/* additions (no reference counterpart) */
+	int cg_fp32;     /* 1: CG with float vectors and double reductions,
+	                    all planes batched; 0: double vectors (parity) */
+	double cg_rel;   /* > 0: CG stops at sqrt(r.r) < cg_rel * ||b|| per plane
+	                    (FP32 CG only); 0: absolute threshold as minicg.c:47 */
+	int cg_chunk;    /* CG iterations per captured graph (FP32 CG) */
 } sp_params;
+
+/* one image of a batch: planar w*h*pd device buffers, f may be NULL */
+typedef struct sp_image {
+	int w, h, pd;
+	const float *g;
+	const float *f;
+	float *out;
+} sp_image;
 
 typedef struct sp_stats {
 	int levels;            /* pyramid levels of the solve */
@@ -101,6 +115,38 @@ int sp_solve_host(float *out, const float *in, const float *dat,
 int sp_solve_device(float *d_out, const float *d_in, const float *d_dat,
 		int w, int h, int pd, const sp_params *p, void *stream,
 		sp_stats *st);
+
+/* Batched device solve: every plane of every image runs its own pyramid,
+ * all advancing together level by level (levels aligned at the finest),
+ * with per-plane CG convergence.  No reference counterpart: the reference
+ * solves one image per process (doc/misc/optimization.txt:44-50). */
+int sp_solve_batch_device(int n, const sp_image *imgs, const sp_params *p,
+		void *stream, sp_stats *st);
+
+/* one image of an adjoint batch: planar w*h*pd device buffers */
+typedef struct sp_adjoint_image {
+	int w, h, pd;
+	const float *g;        /* forward input, NaN = hole */
+	const float *grad_u;   /* dL/du of the filled output */
+	float *grad_g;         /* dL/dg: v at known pixels plus the adjoint
+	                          flux from neighbouring holes; 0 at holes */
+	float *grad_f;         /* dL/df: -lambda at holes, 0 elsewhere; may be NULL */
+} sp_adjoint_image;
+
+/* Gradient of the converged Poisson fill (tstep > 0 systems only; the
+ * clamped 5-point operator is symmetric, so the adjoint is the masked
+ * Laplacian solve A lambda = grad_u on the holes, run by the same pyramid
+ * and CG as the forward solve with the parameters in p).  No reference
+ * counterpart. */
+int sp_adjoint_batch_device(int n, const sp_adjoint_image *imgs,
+		const sp_params *p, void *stream, sp_stats *st);
+
+/* Profiler: sp_profile(1) clears and enables event timing of the solver
+ * phases ("gs_tile", "gs_tile_r23", "cg32", "cg64", "pyramid");
+ * sp_profile_read sums device milliseconds, kernel launches and
+ * algorithmic bytes of a phase. */
+void sp_profile(int on);
+int sp_profile_read(const char *name, double *ms, long *launches, double *bytes);
 
 /* FP64 relative residual ||b - A u|| / ||b|| of each plane over the NaN
  * pixels of g, for the masked Poisson (bilap = 0) or biharmonic system of

@@ -1,0 +1,69 @@
+"""FP64 CPU reference of the converged masked Poisson fill and its gradient.
+
+TEST INFRASTRUCTURE: only tests/ and the benchmark's checks use it.
+
+The system is the reference's CG system (cleant_cgpois.c:197-264 with the
+operator of :67-89): unknowns are the NaN pixels of g,
+    A u_h = b,  b = -f_h - A_hk g_k,  A = -Laplacian with clamped reads,
+assembled here in FP64 from scratch and solved with a sparse direct solver.
+The gradient of a loss L(u) is computed with the general adjoint formulas
+(no use of symmetry):  lambda = A_hh^-T v_h,  dL/df_h = -lambda,
+dL/dg_k = v_k - (A_hk^T lambda)_k.
+"""
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+_ST5 = [(0, 0, -4.0), (1, 0, 1.0), (0, 1, 1.0), (-1, 0, 1.0), (0, -1, 1.0)]
+
+
+def masked_operator(g):
+    """(A_hh, A_hk, holes, known) for the -Laplacian with clamped reads."""
+    h, w = g.shape
+    hole = np.isnan(g).ravel()
+    idx_h = np.flatnonzero(hole)
+    idx_k = np.flatnonzero(~hole)
+    pos_h = -np.ones(h * w, int)
+    pos_h[idx_h] = np.arange(len(idx_h))
+    pos_k = -np.ones(h * w, int)
+    pos_k[idx_k] = np.arange(len(idx_k))
+    rh, ch, vh, rk, ck, vk = [], [], [], [], [], []
+    for p, q in enumerate(idx_h):
+        i, j = q % w, q // w
+        for di, dj, c in _ST5:
+            a, b = min(max(i + di, 0), w - 1), min(max(j + dj, 0), h - 1)
+            t = b * w + a
+            if hole[t]:
+                rh.append(p); ch.append(pos_h[t]); vh.append(-c)
+            else:
+                rk.append(p); ck.append(pos_k[t]); vk.append(-c)
+    Ahh = sp.csr_matrix((vh, (rh, ch)), shape=(len(idx_h), len(idx_h)))
+    Ahk = sp.csr_matrix((vk, (rk, ck)), shape=(len(idx_h), len(idx_k)))
+    return Ahh, Ahk, idx_h, idx_k
+
+
+def solve(g, f=None):
+    """Converged fill in FP64 (float64 array, known pixels copied)."""
+    g = np.asarray(g, np.float64)
+    Ahh, Ahk, ih, ik = masked_operator(g)
+    gf = g.ravel()
+    b = -Ahk @ gf[ik]
+    if f is not None:
+        b -= np.asarray(f, np.float64).ravel()[ih]
+    u = gf.copy()
+    if len(ih):
+        u[ih] = spla.spsolve(Ahh.tocsc(), b)
+    return u.reshape(g.shape)
+
+
+def gradient(g, f, v):
+    """(dL/dg, dL/df) for dL/du = v, FP64."""
+    g = np.asarray(g, np.float64)
+    v = np.asarray(v, np.float64).ravel()
+    Ahh, Ahk, ih, ik = masked_operator(g)
+    lam = spla.spsolve(Ahh.T.tocsc(), v[ih]) if len(ih) else np.zeros(0)
+    gg = np.zeros(g.size)
+    gf = np.zeros(g.size)
+    gg[ik] = v[ik] - Ahk.T @ lam
+    gf[ih] = -lam
+    return gg.reshape(g.shape), gf.reshape(g.shape)

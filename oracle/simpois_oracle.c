@@ -369,8 +369,8 @@ int orc_cg_extend(float *out, const float *in, const float *dat, int w, int h,
 	}
 	/* right-hand side: data term minus the known-pixel contribution */
 	double *b = orc_alloc(sizeof(double) * n);
-	double *z = orc_alloc(sizeof(double) * n);
-	for (int p = 0; p < n; p++) z[p] = 0;
+	double *z = calloc(n ? n : 1, sizeof(double));
+	if (!z) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
 	cg_apply(&s, b, z);
 	for (int p = 0; p < n; p++) {
 		float d = dat ? -dat[s.unk[p]] : 0;

@@ -1,0 +1,164 @@
+"""Shared test helpers: the oracle library, the golden fixtures, NPY bytes."""
+import ctypes
+import io
+import json
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+REF_SO = os.path.join(ORACLE_DIR, "_ref", "libsimpois_ref.so")
+REF_BIN = os.path.join(ORACLE_DIR, "_ref", "simpois")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "cases.npz")
+SIMPOIS_BIN = os.path.join(ROOT, "imscript_amd", "lib", "simpois")
+
+V = ctypes.c_void_p
+I = ctypes.c_int
+F = ctypes.c_float
+
+
+class Knobs(ctypes.Structure):
+    _fields_ = [("pmsfac", ctypes.c_double), ("ponlit", ctypes.c_double),
+                ("cg_eps", ctypes.c_double)]
+
+
+class Trace(ctypes.Structure):
+    _fields_ = [("cap", I), ("len", I), ("n", ctypes.POINTER(I)),
+                ("iter", ctypes.POINTER(I)), ("rr", ctypes.POINTER(ctypes.c_double))]
+
+
+_oracle = None
+
+
+def oracle():
+    """ctypes handle of oracle/liboracle.so (built on first use)."""
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-s", "-C", ORACLE_DIR, "liboracle.so"],
+                           check=True)
+        o = ctypes.CDLL(ORACLE_SO)
+        o.orc_solver_separable.argtypes = [V, V, V, I, I, I, F, I, I, F, V, V]
+        o.orc_poisson_rec.argtypes = [V, V, V, I, I, F, I, I, I, V, V]
+        o.orc_gs_sweeps.argtypes = [V, V, V, I, I, F, I]
+        o.orc_gs_sweeps_hyperplane.argtypes = [V, V, V, I, I, F, I, ctypes.c_uint]
+        o.orc_restrict.argtypes = [V, I, I, V, I, I]
+        o.orc_prolong.argtypes = [V, I, I, V, I, I]
+        o.orc_cg_extend.argtypes = [V, V, V, I, I, V, I, F, I, V, V]
+        o.orc_cg_extend.restype = I
+        o.orc_rel_residual.argtypes = [V, V, V, I, I, I, V]
+        o.orc_rel_residual.restype = ctypes.c_double
+        _oracle = o
+    return _oracle
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def oracle_solve(g, f=None, tstep=0.25, niter=10, scale=99, cgit=0,
+                 pmsfac=3.0, ponlit=0.0, cg_eps=-1.0, trace_cap=0):
+    """Oracle run on planar (pd,h,w) or (h,w) float32; returns (out, trace)."""
+    g = np.ascontiguousarray(g, np.float32)
+    g3 = g.reshape((1,) + g.shape) if g.ndim == 2 else g
+    pd, h, w = g3.shape
+    f3 = None if f is None else np.ascontiguousarray(f, np.float32).reshape(g3.shape)
+    out = np.zeros_like(g3)
+    kn = Knobs(pmsfac, ponlit, cg_eps)
+    tr = None
+    bufs = None
+    if trace_cap:
+        bufs = ((I * trace_cap)(), (I * trace_cap)(),
+                (ctypes.c_double * trace_cap)())
+        tr = Trace(trace_cap, 0, bufs[0], bufs[1], bufs[2])
+    oracle().orc_solver_separable(ptr(out), ptr(g3), ptr(f3), w, h, pd, tstep,
+                                  int(niter), int(scale), float(cgit),
+                                  ctypes.byref(kn),
+                                  ctypes.byref(tr) if tr else None)
+    trace = None
+    if tr is not None:
+        m = min(tr.len, trace_cap)
+        trace = [(bufs[0][k], bufs[1][k], bufs[2][k]) for k in range(m)]
+    return out.reshape(g.shape), trace
+
+
+def npy_from_bytes(b):
+    return np.load(io.BytesIO(bytes(b)), allow_pickle=False)
+
+
+def load_golden():
+    """List of fixture cases (dicts) from tests/golden/cases.npz."""
+    z = np.load(GOLDEN, allow_pickle=False)
+    meta = json.loads(bytes(z["meta"]).decode())
+    cases = []
+    for k, m in enumerate(meta):
+        pre = f"c{k}_"
+        c = dict(m)
+        c["g"] = z[pre + "g"] if m["has_g"] else None
+        c["f"] = z[pre + "f"] if m["has_f"] else None
+        c["mask"] = z[pre + "mask"] if m["has_mask"] else None
+        c["out_bytes"] = z[pre + "out_bytes"]
+        c["trace"] = z[pre + "trace"]
+        cases.append(c)
+    return cases
+
+
+def flag_values(flags):
+    """-t -n -s -c of a fixture, converted as main_simpois does."""
+    d = {"t": "0.25", "n": "10", "s": "99", "c": "0"}
+    for a, b in zip(flags[::2], flags[1::2]):
+        d[a[1:]] = b
+    return (np.float32(float(d["t"])), int(np.float32(float(d["n"]))),
+            int(np.float32(float(d["s"]))), int(np.float32(float(d["c"]))))
+
+
+def knobs_of(env):
+    return dict(pmsfac=float(env.get("PMSFAC", 3)),
+                ponlit=float(env.get("PONLIT", 0)),
+                cg_eps=float(env.get("CG_EPS", -1)))
+
+
+def planar_inputs(c):
+    """Planar (pd,h,w) solver inputs of a fixture, after the CLI's mask and
+    "-f -" handling (simpois.c:365-366, 399-403, 421-424)."""
+    f = None
+    if c["f"] is not None:
+        f = np.ascontiguousarray(np.moveaxis(c["f"], -1, 0), np.float32)
+    if c["g"] is None:
+        g = np.full(f.shape, np.nan, np.float32)
+    else:
+        g = np.ascontiguousarray(np.moveaxis(c["g"], -1, 0), np.float32)
+    if c["mask"] is not None:
+        m = c["mask"][..., 0] > 0
+        g = g.copy()
+        g[:, m] = np.nan
+    return g, f
+
+
+def expected_planar(c):
+    """Reference output of a fixture as planar (pd,h,w)."""
+    a = npy_from_bytes(c["out_bytes"])
+    return np.ascontiguousarray(np.moveaxis(a, -1, 0))
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    if a.shape != b.shape:
+        return False
+    ua, ub = a.view(np.uint32), b.view(np.uint32)
+    same = ua == ub
+    # NaN payloads are not part of the contract: any NaN matches any NaN
+    same |= np.isnan(a) & np.isnan(b)
+    return bool(same.all())
+
+
+def gpu_available():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False

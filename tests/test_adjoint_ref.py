@@ -1,0 +1,79 @@
+"""The FP64 adjoint reference (oracle/adjoint_ref.py) pinned on the CPU:
+central finite differences of its own FP64 solve, the symmetry of the
+clamped 5-point operator, and agreement of its solve with the oracle's
+restatement run to convergence."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import ROOT, oracle_solve
+
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import adjoint_ref as ar  # noqa: E402
+
+
+def problem(seed, h=12, w=14):
+    rng = np.random.default_rng(seed)
+    g = rng.random((h, w)) * 50
+    g[3:9, 4:11] = np.nan
+    g[0, 0:3] = np.nan
+    g[h - 1, w - 2] = np.nan
+    f = rng.standard_normal(g.shape)
+    t = rng.random(g.shape) * 50
+    return g, f, t
+
+
+def loss(g, f, t):
+    u = ar.solve(g, f)
+    hole = np.isnan(g)
+    return np.sum((u - t)[hole] ** 2)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_gradient_matches_central_differences(seed):
+    g, f, t = problem(seed)
+    hole = np.isnan(g)
+    u = ar.solve(g, f)
+    v = np.where(hole, 2 * (u - t), 0)
+    gg, gf = ar.gradient(g, f, v)
+    eps = 1e-3
+    known = list(zip(*np.nonzero(~hole)))
+    rng = np.random.default_rng(seed)
+    for k in rng.choice(len(known), 12, replace=False):
+        a, b = known[k]
+        g1, g2 = g.copy(), g.copy()
+        g1[a, b] += eps
+        g2[a, b] -= eps
+        fd = (loss(g1, f, t) - loss(g2, f, t)) / (2 * eps)
+        assert abs(fd - gg[a, b]) <= 1e-6 * max(1, abs(fd))
+    holes = list(zip(*np.nonzero(hole)))
+    for k in rng.choice(len(holes), 8, replace=False):
+        a, b = holes[k]
+        f1, f2 = f.copy(), f.copy()
+        f1[a, b] += eps
+        f2[a, b] -= eps
+        fd = (loss(g, f1, t) - loss(g, f2, t)) / (2 * eps)
+        assert abs(fd - gf[a, b]) <= 1e-6 * max(1, abs(fd))
+    assert np.all(gg[hole] == 0) and np.all(gf[~hole] == 0)
+
+
+def test_operator_is_symmetric():
+    # <A x, y> = <x, A y> for the clamped 5-point masked operator: the reason
+    # the adjoint is another masked Laplacian solve
+    g, _, _ = problem(3, 17, 19)
+    g[:, -1] = np.nan            # holes on the image border (clamped reads)
+    Ahh, _, _, _ = ar.masked_operator(g)
+    rng = np.random.default_rng(4)
+    x, y = rng.standard_normal((2, Ahh.shape[0]))
+    assert abs((Ahh @ x) @ y - x @ (Ahh @ y)) < 1e-9 * np.abs(x).sum()
+    assert (Ahh - Ahh.T).count_nonzero() == 0
+
+
+def test_fp64_solve_matches_converged_oracle():
+    g, f, _ = problem(5)
+    u = ar.solve(g, f)
+    o, _ = oracle_solve(g.astype(np.float32), f.astype(np.float32), niter=0,
+                        scale=1, cgit=-1, cg_eps=1e-9)
+    assert np.max(np.abs(o - u)) < 1e-3

@@ -1,0 +1,62 @@
+"""The Gauss-Seidel kernels' own phase functions (imscript_amd/csrc/sp_gs.h)
+run on the CPU under AddressSanitizer, against the oracle's raster sweeps.
+
+This checks, without a GPU, the tile geometry, the LDS and image indexing,
+the wavefront schedule (every tile of a wavefront loads before any tile
+stores) and the write-back rule of the device kernels.
+"""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+from helpers import ROOT
+
+BUILD = os.path.join(ROOT, "tests", "cpu", "_build")
+EXE = os.path.join(BUILD, "gs_emulate")
+
+
+@pytest.fixture(scope="module")
+def emulator():
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    os.makedirs(BUILD, exist_ok=True)
+    obj = os.path.join(BUILD, "oracle_asan.o")
+    san = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all"]
+    subprocess.run(["gcc", "-O1", "-g", "-std=gnu99", "-ffp-contract=off", *san,
+                    "-c", os.path.join(ROOT, "oracle", "simpois_oracle.c"),
+                    "-o", obj], check=True)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-ffp-contract=off", *san,
+                    "-o", EXE, os.path.join(ROOT, "tests", "cpu", "gs_emulate.cpp"),
+                    obj, "-lm"], check=True)
+    return EXE
+
+
+CASES = [
+    # W H T tstep seed holefrac withf mode
+    ("40 30 5 0.25 1 0.4 0 tiles"),
+    ("150 140 40 0.25 2 0.3 1 tiles"),
+    ("150 140 40 0.4 3 0.5 0 tiles"),
+    ("200 3 33 0.25 1 0.5 1 tiles"),
+    ("33 1 50 0.25 3 0.5 1 tiles"),
+    ("1 1 7 0.25 1 1 1 tiles"),
+    ("70 45 17 -0.08 4 0.3 0 tiles"),
+    ("100 90 20 -0.08 7 0.4 1 tiles"),
+    ("1 9 7 -0.08 1 0.5 1 tiles"),
+    ("50 37 6 -1000.01 5 0.3 0 tiles"),
+    ("97 83 9 -1000.02 8 0.4 1 tiles"),
+    ("40 30 5 0.25 1 0.4 0 small"),
+    ("70 45 17 -0.08 4 0.3 1 small"),
+    ("50 37 6 -1000.01 5 0.3 1 small"),
+    ("2 2 100 -0.08 3 0.5 1 small"),
+]
+
+
+@pytest.mark.parametrize("args", CASES)
+def test_kernel_phases_match_raster_gauss_seidel(emulator, args):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    p = subprocess.run([emulator, *args.split()], capture_output=True,
+                       text=True, timeout=600, env=env)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert p.stdout.startswith("OK")
