@@ -86,7 +86,7 @@ def cpu_baseline(g, f, rel, side, params):
             env = dict(os.environ, CG_EPS=repr(eps))
             t0 = time.perf_counter()
             subprocess.run([ref, "-i", os.path.join(td, "g.npy"), "-f",
-                            os.path.join(td, "f.npy"), "-n", str(params["niter"]),
+                            os.path.join(td, "f.npy"), "-n", "10",
                             "-c", "-1", "-o", os.path.join(td, "o.npy")],
                            env=env, check=True, stdout=subprocess.DEVNULL,
                            stderr=subprocess.DEVNULL)
@@ -95,15 +95,16 @@ def cpu_baseline(g, f, rel, side, params):
     else:
         kind = "port"
         t0 = time.perf_counter()
-        out, _ = oracle_solve(gc, fc, niter=params["niter"], cgit=-1, cg_eps=eps)
+        out, _ = oracle_solve(gc, fc, niter=10, cgit=-1, cg_eps=eps)
         dt = time.perf_counter() - t0
     res = oracle().orc_rel_residual(ptr(np.ascontiguousarray(out, np.float32)),
                                     ptr(gc), ptr(fc), side, side, 0, None)
     return {"value": round(side * side / dt / 1e6, 4), "unit": "Mpix/s", "cores": 1,
             "kind": kind, "seconds": round(dt, 3), "rel_residual": res,
-            "sample": f"channel 0, top-left {side}x{side} crop, simpois -n "
-                      f"{params['niter']} -c -1 with CG_EPS = {rel:g} * ||b|| of "
-                      "the crop, single thread, file I/O included"}
+            "sample": f"channel 0, top-left {side}x{side} crop, simpois -n 10 "
+                      f"-c -1 with CG_EPS = {rel:g} * ||b|| of the crop (the "
+                      "reference's pyramid with plain CG at every level), single "
+                      "thread, file I/O included"}
 
 
 def main():
@@ -113,7 +114,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--channels", type=int, default=3)
-    ap.add_argument("--niter", type=int, default=10)
+    ap.add_argument("--niter", type=int, default=None)
+    ap.add_argument("--method", default="mgpcg", choices=["mgpcg", "mgpcg1", "cg"],
+                    help="mgpcg: pyramid with multigrid-preconditioned CG at "
+                         "every level; mgpcg1: the same on the finest level "
+                         "only; cg: plain CG at every level (the reference's "
+                         "method)")
     ap.add_argument("--target", type=float, default=1e-5)
     ap.add_argument("--cg-rel", type=float, default=5e-6)
     ap.add_argument("--seed", type=int, default=2)
@@ -130,8 +136,11 @@ def main():
     dev = "cuda:0"
     g, f, hole_frac = make_inputs(torch, a.size, a.channels, a.seed, dev)
     out = torch.empty_like(g)
-    params = dict(tstep=0.25, niter=a.niter, scale=99, cgit=-1, cg_fp32=1,
-                  cg_rel=a.cg_rel, trace=0, cg_chunk=32)
+    if a.niter is None:
+        a.niter = 10 if a.method == "cg" else 0
+    params = dict(tstep=0.25, niter=a.niter, scale=1 if a.method == "mgpcg1" else 99,
+                  cgit=-1, cg_fp32=1, cg_rel=a.cg_rel, trace=0, cg_chunk=32,
+                  cg_mg=0 if a.method == "cg" else 1)
     torch.cuda.synchronize()
 
     def step():
@@ -176,17 +185,25 @@ def main():
             "workload": f"Poisson fill with guidance, {a.size}x{a.size}x{a.channels} "
                         "float32, 50% rectangle holes 8-512 px, multiscale "
                         "coarse-to-fine to rel residual 1e-5",
-            "method": f"simpois pyramid, -n {a.niter} Gauss-Seidel sweeps and CG "
+            "method": {
+                "cg": f"simpois pyramid, -n {a.niter} Gauss-Seidel sweeps and CG "
                       f"(float vectors, double reductions) to {a.cg_rel:g}*||b|| "
                       "at every level, all channels batched",
+                "mgpcg": f"simpois pyramid (-n {a.niter}) with CG preconditioned "
+                         "by an aggregation multigrid V-cycle to "
+                         f"{a.cg_rel:g}*||b|| at every level, channels batched",
+                "mgpcg1": "multigrid-preconditioned CG on the finest level from "
+                          f"a zero fill, to {a.cg_rel:g}*||b||, channels batched",
+            }[a.method],
             "hole_fraction": round(hole_frac, 4),
             "levels": stats.levels,
+            "cg_ms": round(cg_ms, 3),
             "cg_iters_finest_max": stats.cg_iters_finest,
             "rel_residual_max": max(rel),
             "residual_target_met": ok,
         },
     }
-    if cg_ms > 0:
+    if cg_ms > 0 and a.method == "cg":
         achieved = cg_bytes / (cg_ms * 1e-3) / 1e9
         result["roofline"] = {
             "kernel": "CG iteration (k32_spmv, k32_update, k32_dir + 2 per-plane "

@@ -16,7 +16,7 @@ class sp_params(ctypes.Structure):
                 ("pmsfac", ctypes.c_double), ("ponlit", ctypes.c_double),
                 ("cg_eps", ctypes.c_double), ("trace", ctypes.c_int),
                 ("cg_fp32", ctypes.c_int), ("cg_rel", ctypes.c_double),
-                ("cg_chunk", ctypes.c_int)]
+                ("cg_chunk", ctypes.c_int), ("cg_mg", ctypes.c_int)]
 
 
 class sp_image(ctypes.Structure):
@@ -51,6 +51,8 @@ _SIGNATURES = {
                                 ctypes.POINTER(ctypes.c_double), _P]),
     "sp_solve_batch_device": (_I, [_I, _P, ctypes.POINTER(sp_params), _P,
                                    ctypes.POINTER(sp_stats)]),
+    "sp_adjoint_batch_device": (_I, [_I, _P, ctypes.POINTER(sp_params), _P,
+                                     ctypes.POINTER(sp_stats)]),
     "sp_profile": (None, [_I]),
     "sp_profile_read": (_I, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                              ctypes.POINTER(ctypes.c_long),

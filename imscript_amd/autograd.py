@@ -23,25 +23,18 @@ class sp_adjoint_image(ctypes.Structure):
                 ("grad_g", ctypes.c_void_p), ("grad_f", ctypes.c_void_p)]
 
 
-lib_sig = None
-
-
 def _adjoint(g, grad_u, grad_g, grad_f, params):
-    global lib_sig
-    if lib_sig is None:
-        fn = lib.sp_adjoint_batch_device
-        fn.restype = ctypes.c_int
-        fn.argtypes = [ctypes.c_int, ctypes.POINTER(sp_adjoint_image),
-                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
-        lib_sig = fn
     pd, h, w = g.shape
     img = sp_adjoint_image(w, h, pd, g.data_ptr(), grad_u.data_ptr(),
                            grad_g.data_ptr(),
                            None if grad_f is None else grad_f.data_ptr())
     st = sp_stats()
     s = torch.cuda.current_stream()
-    check(lib_sig(1, ctypes.byref(img), ctypes.byref(params),
-                  ctypes.c_void_p(s.cuda_stream), ctypes.byref(st)))
+    check(lib.sp_adjoint_batch_device(1, ctypes.cast(ctypes.byref(img),
+                                                     ctypes.c_void_p),
+                                      ctypes.byref(params),
+                                      ctypes.c_void_p(s.cuda_stream),
+                                      ctypes.byref(st)))
     return st
 
 

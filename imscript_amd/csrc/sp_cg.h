@@ -10,6 +10,7 @@ struct Cg32Plane {
 	double *x;         // solution accumulates in double: ~1e3 float updates
 	                   // would otherwise cost ~1e-5 of relative residual
 	float *r, *p, *q;
+	float *z;          // preconditioned residual (== r without multigrid)
 	const int *cnt;    // number of unknowns (device)
 	int w, h, bilap;
 	int blk0, nblk;    // this plane's work blocks / partial-sum slots
@@ -17,11 +18,29 @@ struct Cg32Plane {
 
 // per-plane scalars (device)
 struct Cg32Scal {
-	double rr, alpha, beta, stop, eps, bnorm;
+	double rr, rz, alpha, beta, stop, eps, bnorm;
 	int rel;           // stop = eps * ||b|| instead of eps
 	int active, iters, maxit;
 	int maxit_from_count;   // maxit = number of unknowns (the -c -1 rule)
 	int trace0, trace_cap;  // slots of this plane in the rr trace
+};
+
+// one plane at one level of the multigrid preconditioner
+struct MgLevel {
+	const float *g;        // level 0: boundary data, NaN = unknown; else null
+	float *D, *wE, *wS;    // levels > 0: Galerkin coefficients, D = 0 inactive
+	float *z, *b;          // correction and right-hand side
+	int w, h;
+};
+
+// work lists of one multigrid level (device arrays indexed by batch plane)
+struct MgLevelWork {
+	MgLevel *lv;           // descriptor of each plane at this level
+	MgLevel *coarse;       // descriptor at the next level (planes that have it)
+	BlockWork *mid;        // cells of planes that have a coarser level
+	BlockWork *last;       // cells of planes for which this is the coarsest
+	BlockWork *cells_c;    // coarse cells (next level) of planes that have it
+	int nmid, nlast, ncells_c;
 };
 
 struct Cg32Batch {
@@ -32,10 +51,15 @@ struct Cg32Batch {
 	double *trace;     // optional rr_new trace
 	int *nactive;      // planes still iterating
 	int nplanes, nblk;
+	// multigrid preconditioner (nlev == 0: plain CG)
+	MgLevelWork *mg;   // host array of nlev levels
+	int nlev, nu, coarse_sweeps;
+	float wc;          // coarse-correction factor
 };
 
 namespace spk {
 void cg32_setup(const Cg32Batch &B, hipStream_t s);
 void cg32_iter(const Cg32Batch &B, hipStream_t s);
 void cg32_finish(const Cg32Batch &B, hipStream_t s);
+void mg_build(const Cg32Batch &B, hipStream_t s);
 }

@@ -141,6 +141,7 @@ __global__ __launch_bounds__(NT) void k32_reduce0(const Cg32Plane *pl, Cg32Scal 
 	if (threadIdx.x == 0) {
 		Cg32Scal &s = sc[blockIdx.x];
 		s.rr = rr;
+		s.rz = rr;   // replaced by (r, M r) when preconditioned
 		s.bnorm = sqrt(bb);
 		s.stop = s.rel ? s.eps * s.bnorm : s.eps;
 		s.iters = 0;
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(NT) void k32_reduce_a(const Cg32Plane *pl, Cg32Scal
 	__shared__ double pq;
 	if (!sc[blockIdx.x].active) return;
 	fold(part, pl[blockIdx.x], 2, 0, red, &pq);
-	if (threadIdx.x == 0) sc[blockIdx.x].alpha = sc[blockIdx.x].rr / pq;
+	if (threadIdx.x == 0) sc[blockIdx.x].alpha = sc[blockIdx.x].rz / pq;
 }
 
 // x += alpha p ; r -= alpha q ; partial r.r
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(NT) void k32_update(const BlockWork *bw,
 
 // rr_new, absolute stop test on sqrt(rr_new), beta
 __global__ __launch_bounds__(NT) void k32_reduce_b(const Cg32Plane *pl, Cg32Scal *sc,
-		const double *part, int *nactive, double *trace)
+		const double *part, int *nactive, double *trace, int precond)
 {
 	__shared__ double red[NT / 64];
 	__shared__ double rr1;
@@ -223,7 +224,10 @@ __global__ __launch_bounds__(NT) void k32_reduce_b(const Cg32Plane *pl, Cg32Scal
 		Cg32Scal &s = sc[blockIdx.x];
 		if (trace && s.iters < s.trace_cap) trace[s.trace0 + s.iters] = rr1;
 		s.iters++;
-		s.beta = rr1 / s.rr;
+		if (!precond) {   // z = r: rho = r.r
+			s.beta = rr1 / s.rz;
+			s.rz = rr1;
+		}
 		s.rr = rr1;
 		if (sqrt(rr1) < s.stop || s.iters >= s.maxit) {
 			s.active = 0;
@@ -232,7 +236,7 @@ __global__ __launch_bounds__(NT) void k32_reduce_b(const Cg32Plane *pl, Cg32Scal
 	}
 }
 
-// p = r + beta p
+// p = z + beta p  (z = r without preconditioner)
 __global__ __launch_bounds__(NT) void k32_dir(const BlockWork *bw,
 		const Cg32Plane *pl, const Cg32Scal *sc)
 {
@@ -246,7 +250,7 @@ __global__ __launch_bounds__(NT) void k32_dir(const BlockWork *bw,
 		if (qq < 0) break;
 		const size_t q = (size_t)qq;
 		if (!isnan(P.g[q])) continue;
-		P.p[q] = P.r[q] + be * P.p[q];
+		P.p[q] = P.z[q] + be * P.p[q];
 	}
 }
 
@@ -263,18 +267,268 @@ __global__ __launch_bounds__(NT) void k32_finish(const BlockWork *bw,
 	}
 }
 
+// ------------------------------------------------------------ multigrid
+//
+// Preconditioner M ~ A^-1: one symmetric V-cycle over a hierarchy built by
+// aggregation.  Level l+1 cell (I,J) aggregates the active cells of
+// (2I..2I+1, 2J..2J+1); the interpolation P is piecewise constant and the
+// coarse operator is the exact Galerkin product P^T A P, again a symmetric
+// 5-point operator stored as a diagonal D and east/south couplings wE, wS
+// (level 0 is the masked -Laplacian itself, read from the mask).  Smoother:
+// red-black Gauss-Seidel, red then black before the coarse correction and
+// black then red after it, so that M is symmetric; the coarse correction is
+// scaled by wc.  The coarsest level runs many such sweeps.
+
+// level-0 coefficients from the mask: D = in-image neighbours, w = 1
+// between two unknowns
+static __device__ __forceinline__ bool unk0(const MgLevel &L, int i, int j)
+{
+	return isnan(L.g[(size_t)j * L.w + i]);
+}
+
+static __device__ __forceinline__ float diag_at(const MgLevel &L, int i, int j,
+		bool l0)
+{
+	if (!l0) return L.D[(size_t)j * L.w + i];
+	if (!unk0(L, i, j)) return 0.0f;
+	return (float)((i > 0) + (i + 1 < L.w) + (j > 0) + (j + 1 < L.h));
+}
+
+static __device__ __forceinline__ float we_at(const MgLevel &L, int i, int j, bool l0)
+{   // coupling between (i,j) and (i+1,j)
+	if (i + 1 >= L.w) return 0.0f;
+	if (!l0) return L.wE[(size_t)j * L.w + i];
+	return (unk0(L, i, j) && unk0(L, i + 1, j)) ? 1.0f : 0.0f;
+}
+
+static __device__ __forceinline__ float ws_at(const MgLevel &L, int i, int j, bool l0)
+{   // coupling between (i,j) and (i,j+1)
+	if (j + 1 >= L.h) return 0.0f;
+	if (!l0) return L.wS[(size_t)j * L.w + i];
+	return (unk0(L, i, j) && unk0(L, i, j + 1)) ? 1.0f : 0.0f;
+}
+
+// sum of couplings times z over the four neighbours of (i,j)
+static __device__ __forceinline__ float offdiag(const MgLevel &L, int i, int j,
+		bool l0)
+{
+	const size_t q = (size_t)j * L.w + i;
+	float s = 0.0f;
+	if (i + 1 < L.w) s += we_at(L, i, j, l0) * L.z[q + 1];
+	if (i > 0) s += we_at(L, i - 1, j, l0) * L.z[q - 1];
+	if (j + 1 < L.h) s += ws_at(L, i, j, l0) * L.z[q + L.w];
+	if (j > 0) s += ws_at(L, i, j - 1, l0) * L.z[q - L.w];
+	return s;
+}
+
+// Galerkin coarse operator of the next level (one thread per coarse cell)
+__global__ __launch_bounds__(NT) void k_mg_coarsen(const BlockWork *bw,
+		const MgLevel *fine, const MgLevel *coarse, int l0)
+{
+	const BlockWork b = bw[blockIdx.x];
+	const MgLevel F = fine[b.plane], C = coarse[b.plane];
+	for (int k_ = 0; k_ < PPT; k_++) {
+		const long qq = pix(b, k_, (size_t)C.w * C.h);
+		if (qq < 0) break;
+		const size_t q = (size_t)qq;
+		const int I = (int)(q % C.w), J = (int)(q / C.w);
+		float d = 0, inner = 0, e = 0, sth = 0;
+		for (int c = 0; c < 4; c++) {
+			const int i = 2 * I + (c & 1), j = 2 * J + (c >> 1);
+			if (i >= F.w || j >= F.h) continue;
+			d += diag_at(F, i, j, l0);
+			const float we = we_at(F, i, j, l0), ws = ws_at(F, i, j, l0);
+			if (c & 1) e += we; else inner += we;      // east edge leaves the cell
+			if (c >> 1) sth += ws; else inner += ws;   // south edge leaves the cell
+		}
+		C.D[q] = d - 2.0f * inner;
+		C.wE[q] = I + 1 < C.w ? e : 0.0f;
+		C.wS[q] = J + 1 < C.h ? sth : 0.0f;
+		C.z[q] = 0.0f;
+	}
+}
+
+// one red-black half sweep on colour `color`; `first` starts from z = 0
+__global__ __launch_bounds__(NT) void k_mg_smooth(const BlockWork *bw,
+		const MgLevel *lv, const Cg32Scal *sc, int color, int first, int l0)
+{
+	const BlockWork b = bw[blockIdx.x];
+	if (!sc[b.plane].active) return;
+	const MgLevel L = lv[b.plane];
+	for (int k_ = 0; k_ < PPT; k_++) {
+		const long qq = pix(b, k_, (size_t)L.w * L.h);
+		if (qq < 0) break;
+		const size_t q = (size_t)qq;
+		const int i = (int)(q % L.w), j = (int)(q / L.w);
+		if (((i + j) & 1) != color) {
+			if (first) L.z[q] = 0.0f;
+			continue;
+		}
+		const float d = diag_at(L, i, j, l0);
+		if (d == 0.0f) continue;
+		const float s = first ? L.b[q] : L.b[q] + offdiag(L, i, j, l0);
+		L.z[q] = s / d;
+	}
+}
+
+// coarse right-hand side: sum of the fine residuals of the children
+__global__ __launch_bounds__(NT) void k_mg_restrict(const BlockWork *bw,
+		const MgLevel *fine, const MgLevel *coarse, const Cg32Scal *sc, int l0)
+{
+	const BlockWork b = bw[blockIdx.x];
+	if (!sc[b.plane].active) return;
+	const MgLevel F = fine[b.plane], C = coarse[b.plane];
+	for (int k_ = 0; k_ < PPT; k_++) {
+		const long qq = pix(b, k_, (size_t)C.w * C.h);
+		if (qq < 0) break;
+		const size_t q = (size_t)qq;
+		const int I = (int)(q % C.w), J = (int)(q / C.w);
+		float r = 0;
+		for (int c = 0; c < 4; c++) {
+			const int i = 2 * I + (c & 1), j = 2 * J + (c >> 1);
+			if (i >= F.w || j >= F.h) continue;
+			const float d = diag_at(F, i, j, l0);
+			if (d == 0.0f) continue;
+			const size_t f = (size_t)j * F.w + i;
+			r += F.b[f] - (d * F.z[f] - offdiag(F, i, j, l0));
+		}
+		C.b[q] = r;
+		C.z[q] = 0.0f;
+	}
+}
+
+// z += wc * P z_coarse on the active fine cells
+__global__ __launch_bounds__(NT) void k_mg_prolong(const BlockWork *bw,
+		const MgLevel *fine, const MgLevel *coarse, const Cg32Scal *sc, float wc,
+		int l0)
+{
+	const BlockWork b = bw[blockIdx.x];
+	if (!sc[b.plane].active) return;
+	const MgLevel F = fine[b.plane], C = coarse[b.plane];
+	for (int k_ = 0; k_ < PPT; k_++) {
+		const long qq = pix(b, k_, (size_t)F.w * F.h);
+		if (qq < 0) break;
+		const size_t q = (size_t)qq;
+		const int i = (int)(q % F.w), j = (int)(q / F.w);
+		if (diag_at(F, i, j, l0) == 0.0f) continue;
+		F.z[q] += wc * C.z[(size_t)(j >> 1) * C.w + (i >> 1)];
+	}
+}
+
+// partial r.z over the unknowns
+__global__ __launch_bounds__(NT) void k_mg_dot(const BlockWork *bw,
+		const Cg32Plane *pl, const Cg32Scal *sc, double *part)
+{
+	__shared__ double red[NT / 64];
+	const BlockWork b = bw[blockIdx.x];
+	if (!sc[b.plane].active) return;
+	const Cg32Plane P = pl[b.plane];
+	double acc = 0;
+	for (int k_ = 0; k_ < PPT; k_++) {
+		const long qq = pix(b, k_, (size_t)P.w * P.h);
+		if (qq < 0) break;
+		const size_t q = (size_t)qq;
+		if (!isnan(P.g[q])) continue;
+		acc += (double)P.r[q] * P.z[q];
+	}
+	const double s = bsum<NT>(acc, red);
+	if (threadIdx.x == 0) part[2 * blockIdx.x] = s;
+}
+
+// rho_new = r.z; beta = rho_new / rho (iterations) or p = z (setup)
+__global__ __launch_bounds__(NT) void k_mg_reduce(const Cg32Plane *pl, Cg32Scal *sc,
+		const double *part, int setup)
+{
+	__shared__ double red[NT / 64];
+	__shared__ double rz;
+	if (!sc[blockIdx.x].active) return;
+	fold(part, pl[blockIdx.x], 2, 0, red, &rz);
+	if (threadIdx.x == 0) {
+		Cg32Scal &s = sc[blockIdx.x];
+		s.beta = setup ? 0.0 : rz / s.rz;
+		s.rz = rz;
+	}
+}
+
+void cg32_setup(const Cg32Batch &B, hipStream_t s);
+
+static void vcycle(const Cg32Batch &B, hipStream_t s)
+{
+	const int L = B.nlev;
+	// down: pre-smooth, restrict; coarsest levels: many sweeps
+	for (int l = 0; l < L; l++) {
+		const MgLevelWork &W = B.mg[l];
+		const int l0 = l == 0;
+		if (W.nmid) {
+			for (int it = 0; it < B.nu; it++) {
+				k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 0, it == 0, l0);
+				k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 1, 0, l0);
+			}
+			k_mg_restrict<<<W.ncells_c, NT, 0, s>>>(W.cells_c, W.lv, W.coarse,
+					B.scal, l0);
+		}
+		if (W.nlast) {
+			for (int it = 0; it < B.coarse_sweeps; it++) {
+				k_mg_smooth<<<W.nlast, NT, 0, s>>>(W.last, W.lv, B.scal, 0, it == 0, l0);
+				k_mg_smooth<<<W.nlast, NT, 0, s>>>(W.last, W.lv, B.scal, 1, 0, l0);
+			}
+			for (int it = 0; it < B.coarse_sweeps; it++) {
+				k_mg_smooth<<<W.nlast, NT, 0, s>>>(W.last, W.lv, B.scal, 1, 0, l0);
+				k_mg_smooth<<<W.nlast, NT, 0, s>>>(W.last, W.lv, B.scal, 0, 0, l0);
+			}
+		}
+	}
+	// up: correct, post-smooth in the mirrored order
+	for (int l = L - 1; l >= 0; l--) {
+		const MgLevelWork &W = B.mg[l];
+		if (!W.nmid) continue;
+		const int l0 = l == 0;
+		k_mg_prolong<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, W.coarse, B.scal, B.wc, l0);
+		for (int it = 0; it < B.nu; it++) {
+			k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 1, 0, l0);
+			k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 0, 0, l0);
+		}
+	}
+}
+
+// z = M r, rho = r.z (and p = z at setup)
+static void precondition(const Cg32Batch &B, int setup, hipStream_t s)
+{
+	vcycle(B, s);
+	k_mg_dot<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
+	k_mg_reduce<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, setup);
+	if (setup)   // p = z + 0 * p
+		k32_dir<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal);
+}
+
+void mg_build(const Cg32Batch &B, hipStream_t s)
+{
+	for (int l = 0; l + 1 < B.nlev; l++) {
+		const MgLevelWork &W = B.mg[l];
+		if (W.ncells_c)
+			k_mg_coarsen<<<W.ncells_c, NT, 0, s>>>(W.cells_c, W.lv, W.coarse, l == 0);
+	}
+}
+
 void cg32_setup(const Cg32Batch &B, hipStream_t s)
 {
 	k32_setup<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.part);
 	k32_reduce0<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive);
+	if (B.nlev) {
+		mg_build(B, s);
+		precondition(B, 1, s);
+	}
 }
 
 void cg32_iter(const Cg32Batch &B, hipStream_t s)
 {
+	const int pc = B.nlev > 0;
 	k32_spmv<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
 	k32_reduce_a<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part);
 	k32_update<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
-	k32_reduce_b<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive, B.trace);
+	k32_reduce_b<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive,
+			B.trace, pc);
+	if (pc) precondition(B, 0, s);
 	k32_dir<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal);
 }
 

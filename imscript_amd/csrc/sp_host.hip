@@ -80,6 +80,7 @@ extern "C" void sp_params_default(sp_params *p)
 	p->cg_fp32 = 0;
 	p->cg_rel = 0;
 	p->cg_chunk = 32;
+	p->cg_mg = 0;
 }
 
 // ------------------------------------------------------------- device
@@ -199,6 +200,17 @@ struct LevelPlan {
 	int *d_small = nullptr;
 	int nsmall = 0, small_lds = 0, small_threads = 64;
 	bool any_cg = false;
+	// FP32 CG of this level, built on first use and kept with the plan
+	struct {
+		bool built = false;
+		Cg32Batch B{};
+		std::vector<MgLevelWork> mgw;
+		std::vector<Cg32Scal> sc0;      // initial scalars, uploaded per solve
+		std::vector<int> ms;            // members in the batch
+		long maxit_host = 0;
+		int trace_len = 0, chunk = 1;
+		hipGraphExec_t exec = nullptr;
+	} cg;
 };
 
 struct CgTrace { int n = 0; std::vector<double> rr; };
@@ -226,8 +238,12 @@ struct Plan {
 	std::vector<int> cg_iters_finest;
 	hipStream_t cap_stream = nullptr;
 
+	std::vector<char> key;           // identity of the call this plan serves
+
 	~Plan()
 	{
+		for (auto &L : lv)
+			if (L.cg.exec) (void)hipGraphExecDestroy(L.cg.exec);
 		for (void *p : allocs) (void)hipFree(p);
 		if (h_done) (void)hipHostFree(h_done);
 		if (cap_stream) (void)hipStreamDestroy(cap_stream);
@@ -523,29 +539,113 @@ static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 
 // ---------------------------------------------------------- FP32 CG
 
-// batched FP32 CG over the members of level k with a non-zero -c
-static int run_cg32(Plan &pl, int k, int bilap, float eps, hipStream_t s)
+// Multigrid hierarchy of the preconditioner for the planes of one CG batch:
+// aggregation by 2x2 down to a smallest side of 8 cells (at most 16 levels);
+// coarse coefficients are computed on the device by mg_build.
+static int mg_plan(Plan &pl, const std::vector<Cg32Plane> &cp, int bilap,
+		std::vector<MgLevelWork> &mgw, Cg32Batch &B)
+{
+	if (bilap)
+		return set_err(SP_ERR_ARG, "multigrid preconditioning covers the "
+				"Poisson system only");
+	const int np = (int)cp.size();
+	std::vector<std::vector<std::pair<int, int>>> shp(np);
+	int nlev = 0;
+	for (int z = 0; z < np; z++) {
+		int w = cp[z].w, h = cp[z].h;
+		shp[z].push_back({w, h});
+		while (std::min(w, h) > 8 && shp[z].size() < 16) {
+			w = (w + 1) / 2;
+			h = (h + 1) / 2;
+			shp[z].push_back({w, h});
+		}
+		nlev = std::max(nlev, (int)shp[z].size());
+	}
+	// storage: level 0 uses the CG vectors; levels > 0 own D wE wS z b
+	size_t cells = 0;
+	for (int z = 0; z < np; z++)
+		for (size_t l = 1; l < shp[z].size(); l++)
+			cells += (size_t)shp[z][l].first * shp[z][l].second;
+	float *pool = nullptr;
+	TRY(pl.alloc(&pool, 5 * std::max<size_t>(cells, 1)));
+	std::vector<std::vector<MgLevel>> lv(nlev, std::vector<MgLevel>(np));
+	size_t off = 0;
+	for (int z = 0; z < np; z++)
+		for (size_t l = 0; l < shp[z].size(); l++) {
+			MgLevel &L = lv[l][z];
+			L.w = shp[z][l].first;
+			L.h = shp[z][l].second;
+			if (l == 0) {
+				L.g = cp[z].g;
+				L.z = cp[z].z;
+				L.b = cp[z].r;
+				continue;
+			}
+			const size_t n = (size_t)L.w * L.h;
+			L.D = pool + off; off += n;
+			L.wE = pool + off; off += n;
+			L.wS = pool + off; off += n;
+			L.z = pool + off; off += n;
+			L.b = pool + off; off += n;
+		}
+	mgw.assign(nlev, MgLevelWork{});
+	for (int l = 0; l < nlev; l++) {
+		std::vector<BlockWork> mid, last, cc;
+		for (int z = 0; z < np; z++) {
+			const int d = (int)shp[z].size();
+			if (d <= l) continue;
+			const size_t n = (size_t)lv[l][z].w * lv[l][z].h;
+			auto &dst = d > l + 1 ? mid : last;
+			for (size_t st = 0; st < n; st += EW_SPAN) dst.push_back({z, (int)st});
+			if (d > l + 1) {
+				const size_t nc = (size_t)lv[l + 1][z].w * lv[l + 1][z].h;
+				for (size_t st = 0; st < nc; st += EW_SPAN) cc.push_back({z, (int)st});
+			}
+		}
+		MgLevelWork &W = mgw[l];
+		TRY(pl.upload(&W.lv, lv[l]));
+		if (l + 1 < nlev) TRY(pl.upload(&W.coarse, lv[l + 1]));
+		TRY(pl.upload(&W.mid, mid));
+		TRY(pl.upload(&W.last, last));
+		TRY(pl.upload(&W.cells_c, cc));
+		W.nmid = (int)mid.size();
+		W.nlast = (int)last.size();
+		W.ncells_c = (int)cc.size();
+	}
+	B.mg = mgw.data();
+	B.nlev = nlev;
+	B.nu = 2;
+	B.coarse_sweeps = 16;
+	B.wc = 1.6f;
+	return SP_OK;
+}
+
+// batched FP32 CG over the members of level k with a non-zero -c: device
+// descriptors, multigrid hierarchy and the captured iteration graph are
+// built once per plan
+static int cg32_build(Plan &pl, int k, int bilap, float eps)
 {
 	LevelPlan &L = pl.lv[k];
-	std::vector<int> ms;
+	auto &C = L.cg;
+	C.ms.clear();
 	for (size_t m = 0; m < L.members.size(); m++)
-		if (L.cgit[m] != 0) ms.push_back((int)m);
-	if (ms.empty()) return SP_OK;
+		if (L.cgit[m] != 0) C.ms.push_back((int)m);
+	if (C.ms.empty()) { C.built = true; return SP_OK; }
 	size_t need = 0;
-	for (int m : ms) need += (size_t)L.sp[m].w * L.sp[m].h;
+	for (int m : C.ms) need += (size_t)L.sp[m].w * L.sp[m].h;
 	if (need > pl.cg32_cap) {
-		TRY(pl.alloc(&pl.cg32_vec, need * 3));
+		TRY(pl.alloc(&pl.cg32_vec, need * 4));
 		TRY(pl.alloc(&pl.cg32_x, need));
 		pl.cg32_cap = need;
 	}
 	std::vector<Cg32Plane> cp;
-	std::vector<Cg32Scal> sc;
 	std::vector<BlockWork> bw;
 	size_t off = 0, offx = 0;
-	long maxit_host = 0;
-	int trace_len = 0;
-	for (size_t z = 0; z < ms.size(); z++) {
-		const int m = ms[z];
+	C.maxit_host = 0;
+	C.trace_len = 0;
+	C.sc0.clear();
+	for (size_t z = 0; z < C.ms.size(); z++) {
+		const int m = C.ms[z];
 		const SpPlane &S = L.sp[m];
 		const size_t n = (size_t)S.w * S.h;
 		Cg32Plane c{};
@@ -554,6 +654,8 @@ static int run_cg32(Plan &pl, int k, int bilap, float eps, hipStream_t s)
 		c.r = pl.cg32_vec + off; off += n;
 		c.p = pl.cg32_vec + off; off += n;
 		c.q = pl.cg32_vec + off; off += n;
+		c.z = c.r;
+		if (pl.P.cg_mg) { c.z = pl.cg32_vec + off; off += n; }
 		c.w = S.w; c.h = S.h; c.bilap = bilap;
 		c.blk0 = (int)bw.size();
 		for (size_t st = 0; st < n; st += EW_SPAN) bw.push_back({(int)z, (int)st});
@@ -567,73 +669,78 @@ static int run_cg32(Plan &pl, int k, int bilap, float eps, hipStream_t s)
 		q.maxit_from_count = cg == -1;
 		q.maxit = cg == -1 ? 0 : std::max(cg, 0);
 		const int cap = cg == -1 ? (int)std::min<size_t>(n, 1 << 20) : std::max(cg, 0);
-		q.trace0 = trace_len;
+		q.trace0 = C.trace_len;
 		q.trace_cap = pl.P.trace ? cap : 0;
-		trace_len += q.trace_cap;
-		maxit_host = std::max<long>(maxit_host, cg == -1 ? (long)n : cg);
-		sc.push_back(q);
+		C.trace_len += q.trace_cap;
+		C.maxit_host = std::max<long>(C.maxit_host, cg == -1 ? (long)n : cg);
+		C.sc0.push_back(q);
 	}
-	Cg32Batch B{};
+	Cg32Batch &B = C.B;
+	B = Cg32Batch{};
 	TRY(pl.upload(&B.planes, cp));
-	TRY(pl.upload(&B.scal, sc));
+	TRY(pl.upload(&B.scal, C.sc0));
 	TRY(pl.upload(&B.bw, bw));
 	TRY(pl.alloc(&B.part, 2 * bw.size()));
 	TRY(pl.alloc(&B.nactive, 1));
-	if (trace_len) TRY(pl.alloc(&B.trace, trace_len));
-	B.nplanes = (int)ms.size();
+	if (C.trace_len) TRY(pl.alloc(&B.trace, C.trace_len));
+	B.nplanes = (int)C.ms.size();
 	B.nblk = (int)bw.size();
+	if (pl.P.cg_mg) TRY(mg_plan(pl, cp, bilap, C.mgw, B));
+	// iterations in chunks, captured once as a graph
+	C.chunk = pl.P.cg_mg ? std::max(1, std::min(pl.P.cg_chunk, 4))
+		: std::max(1, pl.P.cg_chunk);
+	if (!pl.cap_stream)
+		HIPCHK(hipStreamCreateWithFlags(&pl.cap_stream, hipStreamNonBlocking));
+	hipGraph_t graph;
+	HIPCHK(hipStreamBeginCapture(pl.cap_stream, hipStreamCaptureModeThreadLocal));
+	for (int z = 0; z < C.chunk; z++) spk::cg32_iter(B, pl.cap_stream);
+	HIPCHK(hipStreamEndCapture(pl.cap_stream, &graph));
+	const hipError_t ie = hipGraphInstantiate(&C.exec, graph, nullptr, nullptr, 0);
+	(void)hipGraphDestroy(graph);
+	HIPCHK(ie);
+	C.built = true;
+	return SP_OK;
+}
+
+static int run_cg32(Plan &pl, int k, int bilap, float eps, hipStream_t s)
+{
+	LevelPlan &L = pl.lv[k];
+	auto &C = L.cg;
+	if (!C.built) TRY(cg32_build(pl, k, bilap, eps));
+	if (C.ms.empty()) return SP_OK;
+	Cg32Batch &B = C.B;
+	B.mg = C.mgw.data();
+	HIPCHK(hipMemcpyAsync(B.scal, C.sc0.data(), C.sc0.size() * sizeof(Cg32Scal),
+			hipMemcpyHostToDevice, s));
 	HIPCHK(hipMemsetAsync(B.nactive, 0, sizeof(int), s));
 	ProfScope pr("cg32", s);
 	spk::cg32_setup(B, s);
-	// iterations in chunks captured as one graph, replayed until every
-	// plane has stopped
-	const int chunk = std::max(1, pl.P.cg_chunk);
-	if (!pl.cap_stream)
-		HIPCHK(hipStreamCreateWithFlags(&pl.cap_stream, hipStreamNonBlocking));
-	hipGraphExec_t exec = nullptr;
-	{
-		hipGraph_t graph;
-		HIPCHK(hipStreamBeginCapture(pl.cap_stream, hipStreamCaptureModeThreadLocal));
-		for (int z = 0; z < chunk; z++) spk::cg32_iter(B, pl.cap_stream);
-		HIPCHK(hipStreamEndCapture(pl.cap_stream, &graph));
-		const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-		(void)hipGraphDestroy(graph);
-		HIPCHK(ie);
-	}
 	long launched = 0;
-	int rc = SP_OK;
-	for (long done = 0; done < maxit_host; done += chunk) {
-		if (hipGraphLaunch(exec, s) != hipSuccess) {
-			rc = set_err(SP_ERR_HIP, "hipGraphLaunch failed");
-			break;
-		}
-		launched += chunk;
-		if (hipMemcpyAsync(pl.h_done, B.nactive, sizeof(int), hipMemcpyDeviceToHost, s)
-				!= hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
-			rc = set_err(SP_ERR_HIP, "CG progress check failed");
-			break;
-		}
+	for (long done = 0; done < C.maxit_host; done += C.chunk) {
+		HIPCHK(hipGraphLaunch(C.exec, s));
+		launched += C.chunk;
+		HIPCHK(hipMemcpyAsync(pl.h_done, B.nactive, sizeof(int), hipMemcpyDeviceToHost, s));
+		HIPCHK(hipStreamSynchronize(s));
 		if (pl.h_done[0] == 0) break;
 	}
-	(void)hipGraphExecDestroy(exec);
-	TRY(rc);
 	spk::cg32_finish(B, s);
 	HIPCHK(hipGetLastError());
 	// per-plane results: iterations and trace
+	std::vector<Cg32Scal> sc(C.sc0.size());
 	HIPCHK(hipMemcpyAsync(sc.data(), B.scal, sc.size() * sizeof(Cg32Scal),
 			hipMemcpyDeviceToHost, s));
-	std::vector<double> trace(trace_len);
-	if (trace_len)
-		HIPCHK(hipMemcpyAsync(trace.data(), B.trace, trace_len * sizeof(double),
+	std::vector<double> trace(C.trace_len);
+	if (C.trace_len)
+		HIPCHK(hipMemcpyAsync(trace.data(), B.trace, C.trace_len * sizeof(double),
 				hipMemcpyDeviceToHost, s));
-	std::vector<int> cnts(ms.size());
-	for (size_t z = 0; z < ms.size(); z++)
-		HIPCHK(hipMemcpyAsync(&cnts[z], L.sp[ms[z]].cnt, sizeof(int),
+	std::vector<int> cnts(C.ms.size());
+	for (size_t z = 0; z < C.ms.size(); z++)
+		HIPCHK(hipMemcpyAsync(&cnts[z], L.sp[C.ms[z]].cnt, sizeof(int),
 				hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
 	double unk = 0;
-	for (size_t z = 0; z < ms.size(); z++) {
-		const int p = L.members[ms[z]];
+	for (size_t z = 0; z < C.ms.size(); z++) {
+		const int p = L.members[C.ms[z]];
 		CgTrace &tr = pl.traces[p][k];
 		tr.n = cnts[z];
 		const int it = sc[z].iters;
@@ -644,8 +751,8 @@ static int run_cg32(Plan &pl, int k, int bilap, float eps, hipStream_t s)
 	}
 	// algorithmic bytes per unknown and iteration: spmv reads p, writes q
 	// (8); update reads x (double) r p q, writes x r (32); direction reads
-	// r p, writes p (12): 52 B (44 B with a float x)
-	pr.add(5 * launched, 52.0 * unk);
+	// z p, writes p (12): 52 B.  The multigrid V-cycle is not counted.
+	pr.add((pl.P.cg_mg ? 0 : 5) * launched, 52.0 * unk);
 	return SP_OK;
 }
 
@@ -740,9 +847,35 @@ extern "C" int sp_solve_batch_device(int n, const sp_image *imgs,
 			planes.push_back({I.w, I.h, I.g + wh * l, I.f ? I.f + wh * l : nullptr,
 				I.out + wh * l});
 	}
-	Plan pl;
-	TRY(plan_create(pl, planes, *p, has_f));
-	return plan_execute(pl, (hipStream_t)stream, st);
+	// the last plan is kept and reused by a call with the same planes,
+	// pointers and parameters (repeated solves skip allocation, tiling and
+	// graph instantiation)
+	std::vector<char> key;
+	auto put = [&](const void *v, size_t n) {
+		key.insert(key.end(), (const char *)v, (const char *)v + n);
+	};
+	for (const PlaneIn &q : planes) {
+		put(&q.w, sizeof q.w); put(&q.h, sizeof q.h);
+		put(&q.g, sizeof q.g); put(&q.f, sizeof q.f); put(&q.out, sizeof q.out);
+	}
+	const sp_params &P = *p;
+	put(&P.tstep, sizeof P.tstep); put(&P.niter, sizeof P.niter);
+	put(&P.scale, sizeof P.scale); put(&P.cgit, sizeof P.cgit);
+	put(&P.pmsfac, sizeof P.pmsfac); put(&P.ponlit, sizeof P.ponlit);
+	put(&P.cg_eps, sizeof P.cg_eps); put(&P.trace, sizeof P.trace);
+	put(&P.cg_fp32, sizeof P.cg_fp32); put(&P.cg_rel, sizeof P.cg_rel);
+	put(&P.cg_chunk, sizeof P.cg_chunk); put(&P.cg_mg, sizeof P.cg_mg);
+	static std::mutex cache_mu;
+	static Plan *cache = nullptr;
+	std::lock_guard<std::mutex> lk(cache_mu);
+	if (!cache || cache->key != key) {
+		delete cache;
+		cache = new Plan;
+		const int rc = plan_create(*cache, planes, *p, has_f);
+		if (rc) { delete cache; cache = nullptr; return rc; }
+		cache->key = key;
+	}
+	return plan_execute(*cache, (hipStream_t)stream, st);
 }
 
 extern "C" int sp_solve_device(float *d_out, const float *d_in, const float *d_dat,

@@ -80,6 +80,8 @@ typedef struct sp_params {
 	double cg_rel;   /* > 0: CG stops at sqrt(r.r) < cg_rel * ||b|| per plane
 	                    (FP32 CG only); 0: absolute threshold as minicg.c:47 */
 	int cg_chunk;    /* CG iterations per captured graph (FP32 CG) */
+	int cg_mg;       /* 1: precondition the FP32 CG with a multigrid V-cycle
+	                    (Poisson only); 0: plain CG as the reference */
 } sp_params;
 
 /* one image of a batch: planar w*h*pd device buffers, f may be NULL */

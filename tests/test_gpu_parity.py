@@ -216,3 +216,37 @@ def test_batch_of_uneven_images_matches_oracle():
     for g, f, o in zip(gs, fs, outs):
         ref, _ = oracle_solve(g, f, niter=12)
         assert bits_equal(o.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("scale", [1, 99])
+def test_multigrid_preconditioned_cg(scale):
+    """Multigrid-preconditioned FP32 CG: reaches the relative target in few
+    iterations and agrees with the plain CG solution."""
+    import torch
+    m = _imscript()
+    g, f = random_case(51, 2, 700, 650, 0.5, True)
+    dg, df = torch.from_numpy(g).cuda(), torch.from_numpy(f).cuda()
+    mg, st = m.solve_device(dg, df, niter=0, scale=scale, cgit=-1, cg_fp32=1,
+                            cg_rel=1e-6, cg_mg=1)
+    rel = m.residual_device(mg, dg, df)
+    assert max(rel) <= 2e-6, rel
+    assert st.cg_iters_finest < 60
+    plain, _ = m.solve_device(dg, df, niter=0, scale=1, cgit=-1, cg_fp32=1,
+                              cg_rel=1e-6)
+    a, b = mg.cpu().numpy(), plain.cpu().numpy()
+    assert np.max(np.abs(a - b)) <= 1e-3 * np.max(np.abs(b))
+
+
+def test_multigrid_batch_of_uneven_planes():
+    import torch
+    m = _imscript()
+    shapes = [(1, 33, 1000), (1, 300, 257), (2, 9, 9), (1, 1500, 40)]
+    gs, fs = [], []
+    for k, (pd, h, w) in enumerate(shapes):
+        g, f = random_case(60 + k, pd, h, w, 0.5, True)
+        gs.append(torch.from_numpy(g).cuda())
+        fs.append(torch.from_numpy(f).cuda())
+    outs, st = m.solve_batch_device(gs, fs, niter=0, scale=1, cgit=-1, cg_fp32=1,
+                                    cg_rel=1e-6, cg_mg=1)
+    for g, f, o in zip(gs, fs, outs):
+        assert max(m.residual_device(o, g, f)) <= 2e-6
