@@ -39,8 +39,10 @@ def _adjoint(g, grad_u, grad_g, grad_f, params):
 
 
 def _params(tol, niter, scale):
+    # multigrid-preconditioned FP32 CG: converges to ~1e-7 relative where
+    # the plain float CG stalls
     return default_params(tstep=0.25, niter=niter, scale=scale, cgit=-1,
-                          cg_fp32=1, cg_rel=tol, trace=0)
+                          cg_fp32=1, cg_rel=tol, cg_mg=1, trace=0)
 
 
 class PoissonFill(torch.autograd.Function):
@@ -76,7 +78,7 @@ class PoissonFill(torch.autograd.Function):
         return grad_g, grad_f, None, None, None
 
 
-def poisson_fill(g, f=None, tol=1e-6, niter=10, scale=99):
+def poisson_fill(g, f=None, tol=1e-6, niter=0, scale=1):
     """Differentiable fill of the NaN pixels of g ((H,W) or (B,H,W) float32
     on the GPU) with optional guidance f; gradients flow to the known pixels
     of g and to f."""

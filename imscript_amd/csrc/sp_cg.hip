@@ -177,13 +177,24 @@ __global__ __launch_bounds__(NT) void k32_spmv(const BlockWork *bw,
 }
 
 __global__ __launch_bounds__(NT) void k32_reduce_a(const Cg32Plane *pl, Cg32Scal *sc,
-		const double *part)
+		const double *part, int *nactive)
 {
 	__shared__ double red[NT / 64];
 	__shared__ double pq;
 	if (!sc[blockIdx.x].active) return;
 	fold(part, pl[blockIdx.x], 2, 0, red, &pq);
-	if (threadIdx.x == 0) sc[blockIdx.x].alpha = sc[blockIdx.x].rz / pq;
+	if (threadIdx.x == 0) {
+		Cg32Scal &s = sc[blockIdx.x];
+		// breakdown (p.Ap = 0 or not finite once float round-off has taken
+		// over): keep the current x and stop this plane
+		if (!(pq > 0) || !isfinite(s.rz / pq)) {
+			s.alpha = 0.0;
+			s.active = 0;
+			atomicSub(nactive, 1);
+		} else {
+			s.alpha = s.rz / pq;
+		}
+	}
 }
 
 // x += alpha p ; r -= alpha q ; partial r.r
@@ -524,7 +535,7 @@ void cg32_iter(const Cg32Batch &B, hipStream_t s)
 {
 	const int pc = B.nlev > 0;
 	k32_spmv<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
-	k32_reduce_a<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part);
+	k32_reduce_a<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive);
 	k32_update<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
 	k32_reduce_b<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive,
 			B.trace, pc);

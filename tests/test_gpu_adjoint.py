@@ -31,7 +31,7 @@ def problem(seed, b=3, h=40, w=48):
     return g, f, t
 
 
-def gpu_grad(g, f, t, tol=1e-7):
+def gpu_grad(g, f, t, tol=1e-7):  # multigrid CG reaches it in float
     import torch
     from imscript_amd.autograd import poisson_fill
     dg = torch.from_numpy(g).cuda().requires_grad_(True)
