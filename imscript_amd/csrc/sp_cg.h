@@ -14,6 +14,7 @@ struct Cg32Plane {
 	const int *cnt;    // number of unknowns (device)
 	int w, h, bilap;
 	int blk0, nblk;    // this plane's work blocks / partial-sum slots
+	int t0, nt;        // this plane's level-0 multigrid tiles / their slots
 };
 
 // per-plane scalars (device)
@@ -25,9 +26,13 @@ struct Cg32Scal {
 	int trace0, trace_cap;  // slots of this plane in the rr trace
 };
 
+// level-0 multigrid tile (pixels); origins are even
+constexpr int MG_TX = 120, MG_TY = 32;
+
 // one plane at one level of the multigrid preconditioner
 struct MgLevel {
 	const float *g;        // level 0: boundary data, NaN = unknown; else null
+	unsigned char *code;   // level 0: neighbour code per pixel (see sp_cg.hip)
 	float *D, *wE, *wS;    // levels > 0: Galerkin coefficients, D = 0 inactive
 	float *z, *b;          // correction and right-hand side
 	int w, h;
@@ -55,6 +60,9 @@ struct Cg32Batch {
 	MgLevelWork *mg;   // host array of nlev levels
 	int nlev, nu, coarse_sweeps;
 	float wc;          // coarse-correction factor
+	int4 *tiles0;      // level-0 tiles (plane, x0, y0, -), contiguous per plane
+	int ntiles0;
+	double *tpart;     // one partial per level-0 tile
 };
 
 namespace spk {

@@ -461,59 +461,299 @@ __global__ __launch_bounds__(NT) void k_mg_reduce(const Cg32Plane *pl, Cg32Scal 
 	}
 }
 
+
+// ----------------------------------------------- fused level-0 V-cycle ends
+//
+// Level 0 is the masked -Laplacian itself.  A byte per pixel encodes it:
+// bit 0 unknown, bits 1-4 the E, S, W, N neighbour is an in-image unknown,
+// bits 5-7 the number of in-image neighbours (the diagonal, 2..4).
+// The two kernels below run on tiles of TX x TY pixels (even origin) with a
+// halo of H = 2*nu: every red-black half sweep shrinks the region where z is
+// exact by one cell, and the first half sweep from z = 0 is exact
+// everywhere, so after 2*nu half sweeps z is exact on the tile plus one
+// cell (down: enough for the residual) or on the tile (up).
+
+constexpr int TX = MG_TX, TY = MG_TY, MG_NU = 2, HALO = 2 * MG_NU;
+constexpr int RW = TX + 2 * HALO, RH = TY + 2 * HALO;
+// Thread layout: a thread owns the 2x2 blocks (column pair pc, row pair m)
+// for m = m0, m0+4, ...; b and the codes of its cells stay in registers
+// across half sweeps, and a block is also one coarse cell (origins even),
+// so the restriction and the coarse correction are thread-local.
+constexpr int MG_T = 256, PAIRS_X = RW / 2, ROWPAIRS = RH / 2;
+constexpr int MG_RG = MG_T / PAIRS_X, MG_K = ROWPAIRS / MG_RG;
+static_assert(PAIRS_X * MG_RG == MG_T && MG_RG * MG_K == ROWPAIRS,
+		"multigrid tile shape must divide the thread layout");
+
+__global__ __launch_bounds__(NT) void k_mg_code(const BlockWork *bw,
+		const MgLevel *lv)
+{
+	const BlockWork b = bw[blockIdx.x];
+	const MgLevel L = lv[b.plane];
+	for (int k_ = 0; k_ < PPT; k_++) {
+		const long qq = pix(b, k_, (size_t)L.w * L.h);
+		if (qq < 0) break;
+		const size_t q = (size_t)qq;
+		const int i = (int)(q % L.w), j = (int)(q / L.w);
+		unsigned c = 0;
+		if (isnan(L.g[q])) {
+			c = 1;
+			if (i + 1 < L.w && isnan(L.g[q + 1])) c |= 2;
+			if (j + 1 < L.h && isnan(L.g[q + L.w])) c |= 4;
+			if (i > 0 && isnan(L.g[q - 1])) c |= 8;
+			if (j > 0 && isnan(L.g[q - L.w])) c |= 16;
+			c |= (unsigned)((i > 0) + (i + 1 < L.w) + (j > 0) + (j + 1 < L.h)) << 5;
+		}
+		L.code[q] = (unsigned char)c;
+	}
+}
+
+static __device__ __forceinline__ float inv_diag(unsigned k)
+{
+	const unsigned d = k >> 5;
+	return d == 4 ? 0.25f : d == 3 ? (1.0f / 3.0f) : 0.5f;
+}
+
+// per-thread cells: 2x2 blocks, cell c of block kb at (2m + (c>>1), 2pc + (c&1))
+struct MgCells {
+	float b[MG_K][4];
+	unsigned char k[MG_K][4];
+	int pc, m0;
+};
+
+static __device__ __forceinline__ int cell_idx(const MgCells &C, int kb, int c)
+{
+	return (2 * (C.m0 + MG_RG * kb) + (c >> 1)) * RW + 2 * C.pc + (c & 1);
+}
+
+// load b and codes (zero outside the image); z of the region from zsrc or 0
+static __device__ void mg_load(const MgLevel &L, int ox, int oy, MgCells &C,
+		float *z, const float *zsrc)
+{
+	C.pc = threadIdx.x % PAIRS_X;
+	C.m0 = threadIdx.x / PAIRS_X;
+#pragma unroll
+	for (int kb = 0; kb < MG_K; kb++)
+#pragma unroll
+		for (int c = 0; c < 4; c++) {
+			const int r = 2 * (C.m0 + MG_RG * kb) + (c >> 1), col = 2 * C.pc + (c & 1);
+			const int i = ox + col, j = oy + r;
+			float bv = 0.0f, zv = 0.0f;
+			unsigned char kv = 0;
+			if (i >= 0 && i < L.w && j >= 0 && j < L.h) {
+				const size_t q = (size_t)j * L.w + i;
+				bv = L.b[q];
+				kv = L.code[q];
+				if (zsrc) zv = zsrc[q];
+			}
+			C.b[kb][c] = bv;
+			C.k[kb][c] = kv;
+			z[r * RW + col] = zv;
+		}
+}
+
+// red-black half sweep over the thread's cells of colour `color`: all
+// neighbour loads first (cells of one colour do not depend on each other)
+static __device__ __forceinline__ void mg_half(const MgCells &C, float *z, int color,
+		bool first)
+{
+	float nz[MG_K][2];
+#pragma unroll
+	for (int kb = 0; kb < MG_K; kb++)
+#pragma unroll
+		for (int h = 0; h < 2; h++) {
+			// colour of cell c is ((c>>1) + (c&1) + pair origin parity) & 1;
+			// origins are even, so cells 0,3 have colour 0 and 1,2 colour 1
+			const int c = color == 0 ? (h ? 3 : 0) : (h ? 2 : 1);
+			const unsigned k = C.k[kb][c];
+			const int idx = cell_idx(C, kb, c);
+			const int r = idx / RW, col = idx - r * RW;
+			float s = C.b[kb][c];
+			if (!first) {
+				if ((k & 2) && col + 1 < RW) s += z[idx + 1];
+				if ((k & 4) && r + 1 < RH) s += z[idx + RW];
+				if ((k & 8) && col > 0) s += z[idx - 1];
+				if ((k & 16) && r > 0) s += z[idx - RW];
+			}
+			nz[kb][h] = s * inv_diag(k);
+		}
+#pragma unroll
+	for (int kb = 0; kb < MG_K; kb++)
+#pragma unroll
+		for (int h = 0; h < 2; h++) {
+			const int c = color == 0 ? (h ? 3 : 0) : (h ? 2 : 1);
+			if (C.k[kb][c] & 1) z[cell_idx(C, kb, c)] = nz[kb][h];
+		}
+}
+
+static __device__ __forceinline__ bool in_tile(const MgCells &C, int kb)
+{
+	const int m = C.m0 + MG_RG * kb;
+	return C.pc >= HALO / 2 && C.pc < HALO / 2 + TX / 2
+		&& m >= HALO / 2 && m < HALO / 2 + TY / 2;
+}
+
+// pre-smoothing from z = 0, residual, 2x2 restriction into the next level
+__global__ __launch_bounds__(MG_T) void k_mg_down0(const int4 *tiles,
+		const MgLevel *lv, const MgLevel *coarse, const Cg32Scal *sc)
+{
+	__shared__ float z[RW * RH];
+	const int4 t = tiles[blockIdx.x];
+	if (!sc[t.x].active) return;
+	const MgLevel L = lv[t.x], Cc = coarse[t.x];
+	const int ox = t.y - HALO, oy = t.z - HALO;
+	MgCells C;
+	mg_load(L, ox, oy, C, z, nullptr);
+	__syncthreads();
+	for (int s = 0; s < 2 * MG_NU; s++) {
+		mg_half(C, z, s & 1, s == 0);
+		__syncthreads();
+	}
+#pragma unroll
+	for (int kb = 0; kb < MG_K; kb++) {
+		if (!in_tile(C, kb)) continue;
+		float acc = 0.0f;
+#pragma unroll
+		for (int c = 0; c < 4; c++) {
+			const int idx = cell_idx(C, kb, c);
+			const unsigned k = C.k[kb][c];
+			const int r = idx / RW, col = idx - r * RW;
+			const int i = ox + col, j = oy + r;
+			if (i < L.w && j < L.h) L.z[(size_t)j * L.w + i] = z[idx];
+			if (!(k & 1)) continue;
+			float az = (float)(k >> 5) * z[idx];
+			if (k & 2) az -= z[idx + 1];
+			if (k & 4) az -= z[idx + RW];
+			if (k & 8) az -= z[idx - 1];
+			if (k & 16) az -= z[idx - RW];
+			acc += C.b[kb][c] - az;
+		}
+		const int I = (ox >> 1) + C.pc, J = (oy >> 1) + C.m0 + MG_RG * kb;
+		if (I < Cc.w && J < Cc.h) {
+			const size_t cq = (size_t)J * Cc.w + I;
+			Cc.b[cq] = acc;
+			Cc.z[cq] = 0.0f;
+		}
+	}
+}
+
+// coarse correction, post-smoothing, partial r.z of the tile
+__global__ __launch_bounds__(MG_T) void k_mg_up0(const int4 *tiles, const MgLevel *lv,
+		const MgLevel *coarse, const Cg32Scal *sc, float wc, double *tpart)
+{
+	__shared__ float z[RW * RH];
+	__shared__ double red[MG_T / 64];
+	const int4 t = tiles[blockIdx.x];
+	if (!sc[t.x].active) return;
+	const MgLevel L = lv[t.x], Cc = coarse[t.x];
+	const int ox = t.y - HALO, oy = t.z - HALO;
+	MgCells C;
+	mg_load(L, ox, oy, C, z, L.z);
+	// every block of the region is one coarse cell: add its correction
+#pragma unroll
+	for (int kb = 0; kb < MG_K; kb++) {
+		const int I = (ox >> 1) + C.pc, J = (oy >> 1) + C.m0 + MG_RG * kb;
+		if (I < 0 || J < 0 || I >= Cc.w || J >= Cc.h) continue;
+		const float e = wc * Cc.z[(size_t)J * Cc.w + I];
+#pragma unroll
+		for (int c = 0; c < 4; c++)
+			if (C.k[kb][c] & 1) z[cell_idx(C, kb, c)] += e;
+	}
+	__syncthreads();
+	for (int s = 0; s < 2 * MG_NU; s++) {
+		mg_half(C, z, (s & 1) ^ 1, false);
+		__syncthreads();
+	}
+	double acc = 0;
+#pragma unroll
+	for (int kb = 0; kb < MG_K; kb++) {
+		if (!in_tile(C, kb)) continue;
+#pragma unroll
+		for (int c = 0; c < 4; c++) {
+			const int idx = cell_idx(C, kb, c);
+			const int r = idx / RW, col = idx - r * RW;
+			const int i = ox + col, j = oy + r;
+			if (i >= L.w || j >= L.h) continue;
+			const float zz = z[idx];
+			L.z[(size_t)j * L.w + i] = zz;
+			if (C.k[kb][c] & 1) acc += (double)C.b[kb][c] * zz;
+		}
+	}
+	const double sum = bsum<MG_T>(acc, red);
+	if (threadIdx.x == 0) tpart[blockIdx.x] = sum;
+}
+
+// rho_new = r.z folded over the plane's level-0 tiles
+__global__ __launch_bounds__(NT) void k_mg_reduce_t(const Cg32Plane *pl,
+		Cg32Scal *sc, const double *tpart, int setup)
+{
+	__shared__ double red[NT / 64];
+	if (!sc[blockIdx.x].active) return;
+	const Cg32Plane P = pl[blockIdx.x];
+	double v = 0;
+	for (int k = P.t0 + threadIdx.x; k < P.t0 + P.nt; k += NT) v += tpart[k];
+	const double rz = bsum<NT>(v, red);
+	if (threadIdx.x == 0) {
+		Cg32Scal &s = sc[blockIdx.x];
+		s.beta = setup ? 0.0 : rz / s.rz;
+		s.rz = rz;
+	}
+}
+
 void cg32_setup(const Cg32Batch &B, hipStream_t s);
 
 static void vcycle(const Cg32Batch &B, hipStream_t s)
 {
 	const int L = B.nlev;
-	// down: pre-smooth, restrict; coarsest levels: many sweeps
-	for (int l = 0; l < L; l++) {
+	// level 0 (never the coarsest): fused pre-smoothing + restriction
+	k_mg_down0<<<B.ntiles0, MG_T, 0, s>>>(B.tiles0, B.mg[0].lv, B.mg[0].coarse, B.scal);
+	// levels >= 1: pre-smooth, restrict; coarsest levels: many sweeps
+	for (int l = 1; l < L; l++) {
 		const MgLevelWork &W = B.mg[l];
-		const int l0 = l == 0;
 		if (W.nmid) {
 			for (int it = 0; it < B.nu; it++) {
-				k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 0, it == 0, l0);
-				k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 1, 0, l0);
+				k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 0, it == 0, 0);
+				k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 1, 0, 0);
 			}
 			k_mg_restrict<<<W.ncells_c, NT, 0, s>>>(W.cells_c, W.lv, W.coarse,
-					B.scal, l0);
+					B.scal, 0);
 		}
 		if (W.nlast) {
 			for (int it = 0; it < B.coarse_sweeps; it++) {
-				k_mg_smooth<<<W.nlast, NT, 0, s>>>(W.last, W.lv, B.scal, 0, it == 0, l0);
-				k_mg_smooth<<<W.nlast, NT, 0, s>>>(W.last, W.lv, B.scal, 1, 0, l0);
+				k_mg_smooth<<<W.nlast, NT, 0, s>>>(W.last, W.lv, B.scal, 0, it == 0, 0);
+				k_mg_smooth<<<W.nlast, NT, 0, s>>>(W.last, W.lv, B.scal, 1, 0, 0);
 			}
 			for (int it = 0; it < B.coarse_sweeps; it++) {
-				k_mg_smooth<<<W.nlast, NT, 0, s>>>(W.last, W.lv, B.scal, 1, 0, l0);
-				k_mg_smooth<<<W.nlast, NT, 0, s>>>(W.last, W.lv, B.scal, 0, 0, l0);
+				k_mg_smooth<<<W.nlast, NT, 0, s>>>(W.last, W.lv, B.scal, 1, 0, 0);
+				k_mg_smooth<<<W.nlast, NT, 0, s>>>(W.last, W.lv, B.scal, 0, 0, 0);
 			}
 		}
 	}
 	// up: correct, post-smooth in the mirrored order
-	for (int l = L - 1; l >= 0; l--) {
+	for (int l = L - 1; l >= 1; l--) {
 		const MgLevelWork &W = B.mg[l];
 		if (!W.nmid) continue;
-		const int l0 = l == 0;
-		k_mg_prolong<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, W.coarse, B.scal, B.wc, l0);
+		k_mg_prolong<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, W.coarse, B.scal, B.wc, 0);
 		for (int it = 0; it < B.nu; it++) {
-			k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 1, 0, l0);
-			k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 0, 0, l0);
+			k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 1, 0, 0);
+			k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 0, 0, 0);
 		}
 	}
+	k_mg_up0<<<B.ntiles0, MG_T, 0, s>>>(B.tiles0, B.mg[0].lv, B.mg[0].coarse, B.scal,
+			B.wc, B.tpart);
 }
 
 // z = M r, rho = r.z (and p = z at setup)
 static void precondition(const Cg32Batch &B, int setup, hipStream_t s)
 {
 	vcycle(B, s);
-	k_mg_dot<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
-	k_mg_reduce<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, setup);
+	k_mg_reduce_t<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.tpart, setup);
 	if (setup)   // p = z + 0 * p
 		k32_dir<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal);
 }
 
 void mg_build(const Cg32Batch &B, hipStream_t s)
 {
+	k_mg_code<<<B.nblk, NT, 0, s>>>(B.bw, B.mg[0].lv);
 	for (int l = 0; l + 1 < B.nlev; l++) {
 		const MgLevelWork &W = B.mg[l];
 		if (W.ncells_c)

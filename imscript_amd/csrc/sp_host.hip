@@ -542,7 +542,7 @@ static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 // Multigrid hierarchy of the preconditioner for the planes of one CG batch:
 // aggregation by 2x2 down to a smallest side of 8 cells (at most 16 levels);
 // coarse coefficients are computed on the device by mg_build.
-static int mg_plan(Plan &pl, const std::vector<Cg32Plane> &cp, int bilap,
+static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 		std::vector<MgLevelWork> &mgw, Cg32Batch &B)
 {
 	if (bilap)
@@ -554,7 +554,8 @@ static int mg_plan(Plan &pl, const std::vector<Cg32Plane> &cp, int bilap,
 	for (int z = 0; z < np; z++) {
 		int w = cp[z].w, h = cp[z].h;
 		shp[z].push_back({w, h});
-		while (std::min(w, h) > 8 && shp[z].size() < 16) {
+		// level 0 is never the coarsest (its fused kernels restrict)
+		while (shp[z].size() < 2 || (std::min(w, h) > 8 && shp[z].size() < 16)) {
 			w = (w + 1) / 2;
 			h = (h + 1) / 2;
 			shp[z].push_back({w, h});
@@ -568,6 +569,12 @@ static int mg_plan(Plan &pl, const std::vector<Cg32Plane> &cp, int bilap,
 			cells += (size_t)shp[z][l].first * shp[z][l].second;
 	float *pool = nullptr;
 	TRY(pl.alloc(&pool, 5 * std::max<size_t>(cells, 1)));
+	size_t px0 = 0;
+	for (int z = 0; z < np; z++) px0 += (size_t)cp[z].w * cp[z].h;
+	unsigned char *codes = nullptr;
+	TRY(pl.alloc(&codes, px0));
+	size_t coff = 0;
+	std::vector<int4> tiles0;
 	std::vector<std::vector<MgLevel>> lv(nlev, std::vector<MgLevel>(np));
 	size_t off = 0;
 	for (int z = 0; z < np; z++)
@@ -579,6 +586,13 @@ static int mg_plan(Plan &pl, const std::vector<Cg32Plane> &cp, int bilap,
 				L.g = cp[z].g;
 				L.z = cp[z].z;
 				L.b = cp[z].r;
+				L.code = codes + coff;
+				coff += (size_t)L.w * L.h;
+				cp[z].t0 = (int)tiles0.size();
+				for (int y0 = 0; y0 < L.h; y0 += MG_TY)
+					for (int x0 = 0; x0 < L.w; x0 += MG_TX)
+						tiles0.push_back(make_int4(z, x0, y0, 0));
+				cp[z].nt = (int)tiles0.size() - cp[z].t0;
 				continue;
 			}
 			const size_t n = (size_t)L.w * L.h;
@@ -612,6 +626,9 @@ static int mg_plan(Plan &pl, const std::vector<Cg32Plane> &cp, int bilap,
 		W.nlast = (int)last.size();
 		W.ncells_c = (int)cc.size();
 	}
+	TRY(pl.upload(&B.tiles0, tiles0));
+	B.ntiles0 = (int)tiles0.size();
+	TRY(pl.alloc(&B.tpart, tiles0.size()));
 	B.mg = mgw.data();
 	B.nlev = nlev;
 	B.nu = 2;
@@ -677,6 +694,7 @@ static int cg32_build(Plan &pl, int k, int bilap, float eps)
 	}
 	Cg32Batch &B = C.B;
 	B = Cg32Batch{};
+	if (pl.P.cg_mg) TRY(mg_plan(pl, cp, bilap, C.mgw, B));
 	TRY(pl.upload(&B.planes, cp));
 	TRY(pl.upload(&B.scal, C.sc0));
 	TRY(pl.upload(&B.bw, bw));
@@ -685,7 +703,6 @@ static int cg32_build(Plan &pl, int k, int bilap, float eps)
 	if (C.trace_len) TRY(pl.alloc(&B.trace, C.trace_len));
 	B.nplanes = (int)C.ms.size();
 	B.nblk = (int)bw.size();
-	if (pl.P.cg_mg) TRY(mg_plan(pl, cp, bilap, C.mgw, B));
 	// iterations in chunks, captured once as a graph
 	C.chunk = pl.P.cg_mg ? std::max(1, std::min(pl.P.cg_chunk, 4))
 		: std::max(1, pl.P.cg_chunk);

@@ -249,4 +249,10 @@ def test_multigrid_batch_of_uneven_planes():
     outs, st = m.solve_batch_device(gs, fs, niter=0, scale=1, cgit=-1, cg_fp32=1,
                                     cg_rel=1e-6, cg_mg=1)
     for g, f, o in zip(gs, fs, outs):
-        assert max(m.residual_device(o, g, f)) <= 2e-6
+        rel = m.residual_device(o, g, f)
+        for l in range(g.shape[0]):
+            # a plane without known pixels is the pure-Neumann problem, which
+            # is singular: no residual target applies
+            if torch.isnan(g[l]).all():
+                continue
+            assert rel[l] <= 2e-6, (tuple(g.shape), rel)
