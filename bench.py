@@ -107,6 +107,299 @@ def cpu_baseline(g, f, rel, side, params):
                       "thread, file I/O included"}
 
 
+# ------------------------------------------------------------ other configs
+
+def emit(d):
+    print(json.dumps(d), flush=True)
+
+
+def sinusoids_np(rng, h, w, k=8, pmin=16, pmax=256):
+    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+    img = np.zeros((h, w))
+    for _ in range(k):
+        th, per, ph = rng.uniform(0, 2 * np.pi), rng.uniform(pmin, pmax), rng.uniform(0, 2 * np.pi)
+        img += np.sin(2 * np.pi * (x * np.cos(th) + y * np.sin(th)) / per + ph)
+    img -= img.min()
+    return (255 * img / img.max()).astype(np.float32)
+
+
+def run_c1(a):
+    """512x512 Laplace fill, 30% disc holes, simpois -n 2000 (bit-exact
+    Gauss-Seidel on every pyramid level)."""
+    import torch
+    import imscript_amd as sp
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from helpers import oracle_solve, bits_equal
+    import adjoint_ref as ar
+    rng = np.random.default_rng(a.seed)
+    n = 512
+    g = sinusoids_np(rng, n, n)
+    cy, cx = rng.uniform(0, n, 40), rng.uniform(0, n, 40)
+    r0 = rng.uniform(5, 40, 40)
+    y, x = np.mgrid[0:n, 0:n]
+    lo, hi = 0.5, 2.0   # scale the radii until 29-31 % of the pixels are holes
+    for _ in range(40):
+        sc = 0.5 * (lo + hi)
+        m = np.zeros((n, n), bool)
+        for k in range(40):
+            m |= (y - cy[k]) ** 2 + (x - cx[k]) ** 2 < (sc * r0[k]) ** 2
+        if m.mean() < 0.29:
+            lo = sc
+        elif m.mean() > 0.31:
+            hi = sc
+        else:
+            break
+    g[m] = np.nan
+    dg = torch.from_numpy(g).cuda()
+    out = torch.empty_like(dg)
+    p = dict(tstep=0.25, niter=2000, scale=99, cgit=0, trace=0)
+    sp.solve_device(dg, out=out, **p)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        sp.solve_device(dg, out=out, **p)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps
+    got = out.cpu().numpy()
+    t1 = time.perf_counter()
+    want, _ = oracle_solve(g, niter=2000)
+    t_orc = time.perf_counter() - t1
+    exact = ar.solve(g)
+    res = {"config": "c1", "workload": "512x512 Laplace, 30% disc holes, simpois -n 2000",
+           "hole_fraction": round(float(m.mean()), 4), "gpu_ms": round(dt * 1e3, 2),
+           "gpu_Mpix_s": round(n * n / dt / 1e6, 3),
+           "bit_exact_vs_oracle": bits_equal(got, want),
+           "oracle_port_s": round(t_orc, 3),
+           "max_abs_err_vs_fp64_solve": float(np.max(np.abs(got - exact))),
+           "rel_residual": sp.residual_device(out, dg)[0]}
+    ref = os.path.join(ROOT, "oracle", "_ref", "simpois")
+    if os.path.exists(ref):
+        with tempfile.TemporaryDirectory() as td:
+            np.save(os.path.join(td, "g.npy"), g)
+            t1 = time.perf_counter()
+            subprocess.run([ref, "-i", os.path.join(td, "g.npy"), "-n", "2000", "-o",
+                            os.path.join(td, "o.npy")], check=True)
+            res["reference_cpu_s"] = round(time.perf_counter() - t1, 3)
+            res["reference_bitexact"] = bits_equal(
+                np.load(os.path.join(td, "o.npy"))[..., 0], got)
+    emit(res)
+
+
+def grf(rng, n, power=3.0):
+    """Gaussian random field with |k|^-power spectrum, scaled to [0, 255]."""
+    w = rng.standard_normal((n, n))
+    ky = np.fft.fftfreq(n)[:, None]
+    kx = np.fft.rfftfreq(n)[None, :]
+    k = np.sqrt(kx * kx + ky * ky)
+    k[0, 0] = 1.0
+    amp = k ** (-power / 2)
+    amp[0, 0] = 0
+    v = np.fft.irfft2(np.fft.rfft2(w) * amp, s=(n, n))
+    v -= v.min()
+    return (255 * v / v.max()).astype(np.float32)
+
+
+def stripes(rng, n, frac):
+    y, x = np.mgrid[0:n, 0:n].astype(np.float32)
+    m = np.zeros((n, n), bool)
+    while m.mean() < frac:
+        wid = rng.uniform(2, 16)
+        th = rng.uniform(0, np.pi)
+        cy, cx = rng.uniform(0, n, 2)
+        ln = rng.uniform(128, 1024)
+        dx, dy = np.cos(th), np.sin(th)
+        along = (x - cx) * dx + (y - cy) * dy
+        across = -(x - cx) * dy + (y - cy) * dx
+        m |= (np.abs(across) < wid / 2) & (np.abs(along) < ln / 2)
+    return m
+
+
+def run_c3(a):
+    """2048x2048 biharmonic fill of a k^-3 Gaussian random field, 20% thin
+    stripes; FP64 CG on the masked bilaplacian at every level to 1e-6*||b||
+    (simpois -t -0.08 -n 0 -c -1 with CG_EPS)."""
+    import torch
+    import imscript_amd as sp
+    rng = np.random.default_rng(a.seed + 3)
+    n = a.c3_size
+    g = grf(rng, n)
+    m = stripes(rng, n, 0.2)
+    g[m] = np.nan
+    dg = torch.from_numpy(g).cuda()
+    bn = bnorm_bilap(dg)
+    out = torch.empty_like(dg)
+    # the CG stops on its own recursive residual evaluated with the float
+    # operator; a 10% margin keeps the FP64 residual under 1e-6
+    p = dict(tstep=-0.08, niter=0, scale=99, cgit=-1, cg_eps=0.9e-6 * bn, trace=0)
+    t0 = time.perf_counter()
+    _, st = sp.solve_device(dg, out=out, **p)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rel = sp.residual_device(out, dg, bilap=True)[0]
+    res = {"config": "c3", "workload": f"{n}x{n} biharmonic, GRF k^-3, 20% stripes, "
+           "FP64 CG at every level to 1e-6*||b||", "hole_fraction": round(float(m.mean()), 4),
+           "gpu_s": round(dt, 3), "gpu_Mpix_s": round(n * n / dt / 1e6, 3),
+           "cg_iters_finest": st.cg_iters_finest, "rel_residual": rel}
+    emit(res)
+
+
+def bnorm_bilap(dg):
+    """||b|| of the biharmonic masked system, b = -A(known pixels)
+    (cleant_cgpois.c:225-236 with the 13-point operator of :91-121)."""
+    g = dg.cpu().numpy().astype(np.float64)
+    h, w = g.shape
+    st = [(0, 0, 20), (1, 0, -8), (0, 1, -8), (-1, 0, -8), (0, -1, -8), (1, 1, 2),
+          (1, -1, 2), (-1, 1, 2), (-1, -1, 2), (2, 0, 1), (0, 2, 1), (-2, 0, 1), (0, -2, 1)]
+    known = np.where(np.isnan(g), 0.0, g)
+    yy, xx = np.mgrid[0:h, 0:w]
+    acc = np.zeros((h, w))
+    for di, dj, c in st:
+        acc += c * known[np.clip(yy + dj, 0, h - 1), np.clip(xx + di, 0, w - 1)]
+    b = -acc[np.isnan(g)]
+    return float(np.sqrt(np.sum(b * b)))
+
+
+def c4_images(torch, rng, count, dev):
+    gs, fr = [], []
+    amin, amax = np.log(128 * 128), np.log(2048 * 2048)
+    for _ in range(count):
+        area = np.exp(rng.uniform(amin, amax))
+        asp = rng.uniform(1, 4)
+        h = max(8, int(round(np.sqrt(area / asp))))
+        w = max(8, int(round(area / h)))
+        if rng.random() < 0.5:
+            h, w = w, h
+        yy = torch.arange(h, device=dev, dtype=torch.float32).view(h, 1)
+        xx = torch.arange(w, device=dev, dtype=torch.float32).view(1, w)
+        img = torch.zeros(h, w, device=dev)
+        for _k in range(8):
+            th, per, ph = rng.uniform(0, 2 * np.pi), rng.uniform(16, 256), rng.uniform(0, 2 * np.pi)
+            img += torch.sin((xx * math.cos(th) + yy * math.sin(th)) * (2 * math.pi / per) + ph)
+        img = (img - img.min()) / (img.max() - img.min()) * 255
+        frac = rng.uniform(0.10, 0.60)
+        m = torch.zeros(h, w, dtype=torch.bool, device=dev)
+        while m.float().mean().item() < frac:
+            if rng.random() < 0.5:
+                r = rng.uniform(3, max(4, min(h, w) / 6))
+                cy, cx = rng.uniform(0, h), rng.uniform(0, w)
+                m |= (yy - cy) ** 2 + (xx - cx) ** 2 < r * r
+            else:
+                a_, b_ = rng.uniform(4, max(5, min(h, w) / 4), 2).astype(int)
+                y0, x0 = rng.integers(0, h), rng.integers(0, w)
+                m[y0:y0 + a_, x0:x0 + b_] = True
+        img[m] = float("nan")
+        gs.append(img.contiguous())
+        fr.append(float(m.float().mean()))
+    return gs, fr
+
+
+def run_c4(a):
+    """256 grayscale images, areas log-uniform 128^2..2048^2, aspect 1:1-1:4,
+    10-60% holes; each solved to rel residual 1e-5: one batched call versus
+    the plain per-image loop."""
+    import torch
+    import imscript_amd as sp
+    rng = np.random.default_rng(a.seed + 4)
+    gs, fr = c4_images(torch, rng, a.c4_count, "cuda:0")
+    mpix = sum(g.numel() for g in gs) / 1e6
+    p = dict(tstep=0.25, niter=0, scale=1, cgit=-1, cg_fp32=1, cg_rel=a.cg_rel,
+             cg_mg=1)
+    outs = [torch.empty_like(g) for g in gs]
+    sp.solve_batch_device(gs, outs=outs, **p)       # warm-up (plan + graphs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        _, st = sp.solve_batch_device(gs, outs=outs, **p)
+    torch.cuda.synchronize()
+    tb = (time.perf_counter() - t0) / a.steps
+    relb = [sp.residual_device(o, g)[0] for o, g in zip(outs, gs)]
+    # plain loop: one solve call per image
+    t0 = time.perf_counter()
+    for g, o in zip(gs, outs):
+        sp.solve_device(g, out=o, **p)
+    torch.cuda.synchronize()
+    tl = time.perf_counter() - t0
+    rell = [sp.residual_device(o, g)[0] for o, g in zip(outs, gs)]
+    emit({"config": "c4", "workload": f"{len(gs)} images, {mpix:.1f} Mpx total, "
+          "log-uniform 128^2-2048^2, aspect 1-4, 10-60% holes, rel residual 1e-5",
+          "method": "multigrid-preconditioned FP32 CG per image",
+          "batched_s": round(tb, 4), "batched_Mpix_s": round(mpix / tb, 1),
+          "loop_s": round(tl, 4), "loop_Mpix_s": round(mpix / tl, 1),
+          "speedup_batched_vs_loop": round(tl / tb, 2),
+          "max_rel_residual_batched": max(relb), "max_rel_residual_loop": max(rell),
+          "hole_fraction_mean": round(float(np.mean(fr)), 3)})
+
+
+def run_c5(a):
+    """Batch 64 x 1024x1024, 35% disc holes: forward Poisson fill, loss on
+    the filled pixels, backward by the adjoint solve."""
+    import torch
+    import imscript_amd as sp
+    from imscript_amd.autograd import poisson_fill
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import adjoint_ref as ar
+    rng = np.random.default_rng(a.seed + 5)
+    B, n = a.c5_batch, a.c5_size
+    dev = "cuda:0"
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(a.seed)
+    yy = torch.arange(n, device=dev, dtype=torch.float32).view(n, 1)
+    xx = torch.arange(n, device=dev, dtype=torch.float32).view(1, n)
+    g = torch.empty(B, n, n, device=dev)
+    for k in range(B):
+        img = torch.zeros(n, n, device=dev)
+        for _ in range(8):
+            th, per, ph = rng.uniform(0, 2 * np.pi), rng.uniform(16, 256), rng.uniform(0, 2 * np.pi)
+            img += torch.sin((xx * math.cos(th) + yy * math.sin(th)) * (2 * math.pi / per) + ph)
+        m = torch.zeros(n, n, dtype=torch.bool, device=dev)
+        while m.float().mean().item() < 0.35:
+            r = rng.uniform(5, 60)
+            cy, cx = rng.uniform(0, n, 2)
+            m |= (yy - cy) ** 2 + (xx - cx) ** 2 < r * r
+        img = (img - img.min()) / (img.max() - img.min()) * 255
+        img[m] = float("nan")
+        g[k] = img
+    f = torch.randn(B, n, n, device=dev, generator=gen)
+    target = torch.rand(B, n, n, device=dev, generator=gen) * 255
+    hole = torch.isnan(g)
+
+    def step():
+        gg = g.clone().requires_grad_(True)
+        ff = f.clone().requires_grad_(True)
+        u = poisson_fill(gg, ff, tol=a.cg_rel)
+        loss = ((u - target) ** 2)[hole].sum()
+        loss.backward()
+        return gg.grad, ff.grad
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps
+    # gradient check on a small instance against the FP64 reference
+    rs = np.random.default_rng(9)
+    gs = sinusoids_np(rs, 48, 56).astype(np.float64)
+    gs[10:30, 12:40] = np.nan
+    fs, ts = rs.standard_normal(gs.shape), rs.random(gs.shape) * 255
+    dgs = torch.tensor(gs, dtype=torch.float32, device=dev).requires_grad_(True)
+    dfs = torch.tensor(fs, dtype=torch.float32, device=dev).requires_grad_(True)
+    u = poisson_fill(dgs, dfs, tol=1e-7)
+    hs = torch.isnan(dgs.detach())
+    ((u - torch.tensor(ts, dtype=torch.float32, device=dev)) ** 2)[hs].sum().backward()
+    uref = ar.solve(gs, fs)
+    v = np.where(np.isnan(gs), 2 * (uref - ts), 0)
+    rg, rf = ar.gradient(gs, fs, v)
+    eg = np.max(np.abs(dgs.grad.cpu().numpy() - rg)) / np.max(np.abs(rg))
+    ef = np.max(np.abs(dfs.grad.cpu().numpy() - rf)) / np.max(np.abs(rf))
+    emit({"config": "c5", "workload": f"batch {B} x {n}x{n}, 35% disc holes, "
+          "forward fill + loss + adjoint backward",
+          "step_s": round(dt, 4), "Mpix_s_fwd_bwd": round(B * n * n / dt / 1e6, 1),
+          "grad_rel_err_vs_fp64_g": float(eg), "grad_rel_err_vs_fp64_f": float(ef)})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,11 +418,21 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-side", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+                    help="c2 (default) is the headline; the others print their "
+                         "own JSON line")
+    ap.add_argument("--c3-size", type=int, default=2048)
+    ap.add_argument("--c4-count", type=int, default=256)
+    ap.add_argument("--c5-batch", type=int, default=64)
+    ap.add_argument("--c5-size", type=int, default=1024)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this path runs on one GPU; --gpus {a.gpus} is not "
               "supported (see DESIGN.md for how it would shard)", file=sys.stderr)
         sys.exit(2)
+    if a.config != "c2":
+        {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5}[a.config](a)
+        return
 
     import torch
     import imscript_amd as sp
@@ -164,6 +467,7 @@ def main():
     torch.cuda.synchronize()
     cg_ms, cg_launch, cg_bytes = sp.profile_read("cg32")
     gs_ms, gs_launch, gs_bytes = sp.profile_read("gs_tile")
+    kern = {k: sp.profile_read(k) for k in ("mg_up0", "mg_down0", "k32_spmv")}
     sp.profile(False)
 
     mpix = a.size * a.size * a.channels / 1e6
@@ -213,6 +517,26 @@ def main():
             "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
             "algorithmic_bytes": cg_bytes, "ms": round(cg_ms, 3),
             "launches": cg_launch}
+    if a.method != "cg" and kern["mg_up0"][0] > 0:
+        rep = {}
+        for k, (ms, n, by) in kern.items():
+            rep[k] = {"avg_us": round(ms * 1e3 / max(n, 1), 2), "launches": n,
+                      "GBps": round(by / (ms * 1e-3) / 1e9, 1)}
+        dom = max(kern, key=lambda k: kern[k][0])
+        ms, n, by = kern[dom]
+        achieved = by / (ms * 1e-3) / 1e9
+        result["roofline"] = {
+            "kernel": {"mg_up0": "k_mg_up0 (fused level-0 coarse correction + "
+                                 "4 red-black half sweeps + r.z)",
+                       "mg_down0": "k_mg_down0 (fused level-0 4 red-black half "
+                                   "sweeps + residual + restriction)",
+                       "k32_spmv": "k32_spmv"}[dom],
+            "bound": "hbm", "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
+            "algorithmic_bytes_per_launch": by / max(n, 1),
+            "avg_launch_us": round(ms * 1e3 / max(n, 1), 2), "launches": n}
+        result["kernels"] = rep
     if gs_ms > 0:
         result["gs_tile"] = {"ms": round(gs_ms, 3), "launches": gs_launch,
                              "pass_GBps": round(gs_bytes / (gs_ms * 1e-3) / 1e9, 1)}

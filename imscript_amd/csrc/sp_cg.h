@@ -44,9 +44,19 @@ struct MgLevelWork {
 	MgLevel *coarse;       // descriptor at the next level (planes that have it)
 	BlockWork *mid;        // cells of planes that have a coarser level
 	BlockWork *last;       // cells of planes for which this is the coarsest
-	BlockWork *cells_c;    // coarse cells (next level) of planes that have it
-	int nmid, nlast, ncells_c;
+	BlockWork *cells_c;    // coarse cells (next level) of the mid planes
+	BlockWork *build_c;    // coarse cells (next level) of every plane that has it
+	int nmid, nlast, ncells_c, nbuild_c;
 };
+
+// per plane: the levels [first, depth) run inside one work-group
+constexpr int MG_MAXLEV = 16;
+constexpr int MG_TAIL_CELLS = 4096;    // first level at or below this size
+constexpr int MG_TAIL_T = 1024;
+
+// optional timer around individual launches (profiling runs only; never
+// set while a graph is being captured)
+typedef void (*ProfFn)(const char *name, int begin, double bytes, hipStream_t s);
 
 struct Cg32Batch {
 	Cg32Plane *planes;
@@ -60,9 +70,13 @@ struct Cg32Batch {
 	MgLevelWork *mg;   // host array of nlev levels
 	int nlev, nu, coarse_sweeps;
 	float wc;          // coarse-correction factor
+	MgLevel *tail_lv;  // [plane * MG_MAXLEV + level]
+	int2 *tail_rng;    // per plane: (first tail level, depth)
 	int4 *tiles0;      // level-0 tiles (plane, x0, y0, -), contiguous per plane
 	int ntiles0;
 	double *tpart;     // one partial per level-0 tile
+	double px0;        // level-0 pixels of the batch (profiler bytes)
+	ProfFn prof;
 };
 
 namespace spk {

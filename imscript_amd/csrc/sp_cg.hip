@@ -701,33 +701,114 @@ __global__ __launch_bounds__(NT) void k_mg_reduce_t(const Cg32Plane *pl,
 
 void cg32_setup(const Cg32Batch &B, hipStream_t s);
 
+
+// ------------------------------------------------------ coarse-level tail
+//
+// The small levels of a plane (from the first level of at most
+// MG_TAIL_CELLS cells down to the coarsest) run inside one work-group: the
+// same smoothing, restriction and correction as the per-level kernels, with
+// a barrier between phases instead of a launch.
+static __device__ void tail_smooth(const MgLevel &L, int color, bool first)
+{
+	const int n = L.w * L.h;
+	for (int q = threadIdx.x; q < n; q += blockDim.x) {
+		const int i = q % L.w, j = q / L.w;
+		if (((i + j) & 1) != color) {
+			if (first) L.z[q] = 0.0f;
+			continue;
+		}
+		const float d = L.D[q];
+		if (d == 0.0f) continue;
+		L.z[q] = (first ? L.b[q] : L.b[q] + offdiag(L, i, j, false)) / d;
+	}
+	__syncthreads();
+}
+
+static __device__ void tail_restrict(const MgLevel &F, const MgLevel &C)
+{
+	const int n = C.w * C.h;
+	for (int q = threadIdx.x; q < n; q += blockDim.x) {
+		const int I = q % C.w, J = q / C.w;
+		float r = 0;
+		for (int c = 0; c < 4; c++) {
+			const int i = 2 * I + (c & 1), j = 2 * J + (c >> 1);
+			if (i >= F.w || j >= F.h) continue;
+			const size_t f = (size_t)j * F.w + i;
+			const float d = F.D[f];
+			if (d == 0.0f) continue;
+			r += F.b[f] - (d * F.z[f] - offdiag(F, i, j, false));
+		}
+		C.b[q] = r;
+		C.z[q] = 0.0f;
+	}
+	__syncthreads();
+}
+
+static __device__ void tail_prolong(const MgLevel &F, const MgLevel &C, float wc)
+{
+	const int n = F.w * F.h;
+	for (int q = threadIdx.x; q < n; q += blockDim.x) {
+		if (F.D[q] == 0.0f) continue;
+		const int i = q % F.w, j = q / F.w;
+		F.z[q] += wc * C.z[(j >> 1) * C.w + (i >> 1)];
+	}
+	__syncthreads();
+}
+
+__global__ __launch_bounds__(MG_TAIL_T) void k_mg_tail(const MgLevel *lvall,
+		const int2 *rng, const Cg32Scal *sc, int nu, int csw, float wc)
+{
+	const int z = blockIdx.x;
+	if (!sc[z].active) return;
+	const int l0 = rng[z].x, d = rng[z].y;
+	const MgLevel *lv = lvall + (size_t)z * MG_MAXLEV;
+	for (int l = l0; l + 1 < d; l++) {
+		for (int it = 0; it < nu; it++) {
+			tail_smooth(lv[l], 0, it == 0);
+			tail_smooth(lv[l], 1, false);
+		}
+		tail_restrict(lv[l], lv[l + 1]);
+	}
+	for (int it = 0; it < csw; it++) {
+		tail_smooth(lv[d - 1], 0, it == 0);
+		tail_smooth(lv[d - 1], 1, false);
+	}
+	for (int it = 0; it < csw; it++) {
+		tail_smooth(lv[d - 1], 1, false);
+		tail_smooth(lv[d - 1], 0, false);
+	}
+	for (int l = d - 2; l >= l0; l--) {
+		tail_prolong(lv[l], lv[l + 1], wc);
+		for (int it = 0; it < nu; it++) {
+			tail_smooth(lv[l], 1, false);
+			tail_smooth(lv[l], 0, false);
+		}
+	}
+}
+
 static void vcycle(const Cg32Batch &B, hipStream_t s)
 {
 	const int L = B.nlev;
-	// level 0 (never the coarsest): fused pre-smoothing + restriction
+	// level 0 (never in the tail): fused pre-smoothing + restriction
+	// algorithmic bytes per level-0 pixel: down reads b and the code, writes
+	// z and a quarter coarse b (10 B); up reads z, b, the code and a quarter
+	// coarse z, writes z (14 B)
+	if (B.prof) B.prof("mg_down0", 1, 0, s);
 	k_mg_down0<<<B.ntiles0, MG_T, 0, s>>>(B.tiles0, B.mg[0].lv, B.mg[0].coarse, B.scal);
-	// levels >= 1: pre-smooth, restrict; coarsest levels: many sweeps
+	if (B.prof) B.prof("mg_down0", 0, 10.0 * B.px0, s);
+	// large levels >= 1 not in a tail: pre-smooth, restrict
 	for (int l = 1; l < L; l++) {
 		const MgLevelWork &W = B.mg[l];
-		if (W.nmid) {
-			for (int it = 0; it < B.nu; it++) {
-				k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 0, it == 0, 0);
-				k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 1, 0, 0);
-			}
-			k_mg_restrict<<<W.ncells_c, NT, 0, s>>>(W.cells_c, W.lv, W.coarse,
-					B.scal, 0);
+		if (!W.nmid) continue;
+		for (int it = 0; it < B.nu; it++) {
+			k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 0, it == 0, 0);
+			k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 1, 0, 0);
 		}
-		if (W.nlast) {
-			for (int it = 0; it < B.coarse_sweeps; it++) {
-				k_mg_smooth<<<W.nlast, NT, 0, s>>>(W.last, W.lv, B.scal, 0, it == 0, 0);
-				k_mg_smooth<<<W.nlast, NT, 0, s>>>(W.last, W.lv, B.scal, 1, 0, 0);
-			}
-			for (int it = 0; it < B.coarse_sweeps; it++) {
-				k_mg_smooth<<<W.nlast, NT, 0, s>>>(W.last, W.lv, B.scal, 1, 0, 0);
-				k_mg_smooth<<<W.nlast, NT, 0, s>>>(W.last, W.lv, B.scal, 0, 0, 0);
-			}
-		}
+		k_mg_restrict<<<W.ncells_c, NT, 0, s>>>(W.cells_c, W.lv, W.coarse, B.scal, 0);
 	}
+	// small levels of every plane in one work-group each
+	k_mg_tail<<<B.nplanes, MG_TAIL_T, 0, s>>>(B.tail_lv, B.tail_rng, B.scal, B.nu,
+			B.coarse_sweeps, B.wc);
 	// up: correct, post-smooth in the mirrored order
 	for (int l = L - 1; l >= 1; l--) {
 		const MgLevelWork &W = B.mg[l];
@@ -738,8 +819,10 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 			k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 0, 0, 0);
 		}
 	}
+	if (B.prof) B.prof("mg_up0", 1, 0, s);
 	k_mg_up0<<<B.ntiles0, MG_T, 0, s>>>(B.tiles0, B.mg[0].lv, B.mg[0].coarse, B.scal,
 			B.wc, B.tpart);
+	if (B.prof) B.prof("mg_up0", 0, 14.0 * B.px0, s);
 }
 
 // z = M r, rho = r.z (and p = z at setup)
@@ -756,8 +839,8 @@ void mg_build(const Cg32Batch &B, hipStream_t s)
 	k_mg_code<<<B.nblk, NT, 0, s>>>(B.bw, B.mg[0].lv);
 	for (int l = 0; l + 1 < B.nlev; l++) {
 		const MgLevelWork &W = B.mg[l];
-		if (W.ncells_c)
-			k_mg_coarsen<<<W.ncells_c, NT, 0, s>>>(W.cells_c, W.lv, W.coarse, l == 0);
+		if (W.nbuild_c)
+			k_mg_coarsen<<<W.nbuild_c, NT, 0, s>>>(W.build_c, W.lv, W.coarse, l == 0);
 	}
 }
 
@@ -774,7 +857,10 @@ void cg32_setup(const Cg32Batch &B, hipStream_t s)
 void cg32_iter(const Cg32Batch &B, hipStream_t s)
 {
 	const int pc = B.nlev > 0;
+	// spmv: read p (stencil) and the mask, write q: 12 B per pixel
+	if (B.prof) B.prof("k32_spmv", 1, 0, s);
 	k32_spmv<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
+	if (B.prof) B.prof("k32_spmv", 0, 12.0 * B.px0, s);
 	k32_reduce_a<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive);
 	k32_update<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
 	k32_reduce_b<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive,

@@ -547,19 +547,23 @@ __global__ __launch_bounds__(EW_NT) void k_residual(const SpPlane *lv,
 
 // out[3p..3p+2] = (rr, bb, rel) for plane p; blocks of plane p are
 // plane_first[p] .. plane_first[p+1]-1
-__global__ void k_residual_reduce(const double *part, const int *plane_first,
-		int nplanes, double *out)
+__global__ __launch_bounds__(EW_NT) void k_residual_reduce(const double *part,
+		const int *plane_first, int nplanes, double *out)
 {
-	int p = blockIdx.x * blockDim.x + threadIdx.x;
-	if (p >= nplanes) return;
+	__shared__ double red[EW_NT / 64];
+	const int p = blockIdx.x;
 	double rr = 0, bb = 0;
-	for (int k = plane_first[p]; k < plane_first[p + 1]; k++) {
+	for (int k = plane_first[p] + threadIdx.x; k < plane_first[p + 1]; k += EW_NT) {
 		rr += part[2 * k];
 		bb += part[2 * k + 1];
 	}
-	out[3 * p] = rr;
-	out[3 * p + 1] = bb;
-	out[3 * p + 2] = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+	rr = block_sum_dbl<EW_NT>(rr, red);
+	bb = block_sum_dbl<EW_NT>(bb, red);
+	if (threadIdx.x == 0) {
+		out[3 * p] = rr;
+		out[3 * p + 1] = bb;
+		out[3 * p + 2] = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+	}
 }
 
 void launch_residual(const SpPlane *lv, const BlockWork *bw, int nblocks,
@@ -572,8 +576,7 @@ void launch_residual_reduce(const double *part, const int *plane_first,
 		int nplanes, double *out, hipStream_t s)
 {
 	if (nplanes)
-		k_residual_reduce<<<(nplanes + 63) / 64, 64, 0, s>>>(part, plane_first,
-				nplanes, out);
+		k_residual_reduce<<<nplanes, EW_NT, 0, s>>>(part, plane_first, nplanes, out);
 }
 
 } // namespace spk
