@@ -34,9 +34,15 @@ struct MgLevel {
 	const float *g;        // level 0: boundary data, NaN = unknown; else null
 	unsigned char *code;   // level 0: neighbour code per pixel (see sp_cg.hip)
 	float *D, *wE, *wS;    // levels > 0: Galerkin coefficients, D = 0 inactive
+	unsigned *gcode;       // levels 1..MG_FUSE_MAX: the same coefficients packed
 	float *z, *b;          // correction and right-hand side
 	int w, h;
 };
+
+// levels 1..MG_FUSE_MAX run the fused tile kernels: their coefficients are
+// integers small enough to pack (diagonal <= 4^(l+1) <= 511, couplings
+// <= 2^l <= 15)
+constexpr int MG_FUSE_MAX = 3;
 
 // work lists of one multigrid level (device arrays indexed by batch plane)
 struct MgLevelWork {
@@ -46,7 +52,10 @@ struct MgLevelWork {
 	BlockWork *last;       // cells of planes for which this is the coarsest
 	BlockWork *cells_c;    // coarse cells (next level) of the mid planes
 	BlockWork *build_c;    // coarse cells (next level) of every plane that has it
-	int nmid, nlast, ncells_c, nbuild_c;
+	int4 *tiles;           // fused levels: tiles (plane, x0, y0, -) of the
+	                       // planes that run this level outside the tail
+	int nmid, nlast, ncells_c, nbuild_c, ntiles;
+	double cells;          // cells covered by `tiles` (profiler bytes)
 };
 
 // per plane: the levels [first, depth) run inside one work-group

@@ -462,13 +462,16 @@ __global__ __launch_bounds__(NT) void k_mg_reduce(const Cg32Plane *pl, Cg32Scal 
 }
 
 
-// ----------------------------------------------- fused level-0 V-cycle ends
+// ------------------------------------------------------ fused V-cycle ends
 //
 // Level 0 is the masked -Laplacian itself.  A byte per pixel encodes it:
 // bit 0 unknown, bits 1-4 the E, S, W, N neighbour is an in-image unknown,
 // bits 5-7 the number of in-image neighbours (the diagonal, 2..4).
-// The two kernels below run on tiles of TX x TY pixels (even origin) with a
-// halo of H = 2*nu: every red-black half sweep shrinks the region where z is
+// Levels 1..MG_FUSE_MAX pack their integer Galerkin coefficients into 32
+// bits: the diagonal in bits 0-8, the E, S, W, N couplings in 4 bits each
+// from bit 9 (a cell is active when its diagonal is not 0).
+// The kernels below run on tiles of TX x TY cells (even origin) with a halo
+// of H = 2*nu: every red-black half sweep shrinks the region where z is
 // exact by one cell, and the first half sweep from z = 0 is exact
 // everywhere, so after 2*nu half sweeps z is exact on the tile plus one
 // cell (down: enough for the residual) or on the tile (up).
@@ -507,54 +510,242 @@ __global__ __launch_bounds__(NT) void k_mg_code(const BlockWork *bw,
 	}
 }
 
-static __device__ __forceinline__ float inv_diag(unsigned k)
+// packed coefficients of a level 1..MG_FUSE_MAX (one thread per cell)
+__global__ __launch_bounds__(NT) void k_mg_gcode(const BlockWork *bw,
+		const MgLevel *lv)
 {
-	const unsigned d = k >> 5;
-	return d == 4 ? 0.25f : d == 3 ? (1.0f / 3.0f) : 0.5f;
+	const BlockWork b = bw[blockIdx.x];
+	const MgLevel L = lv[b.plane];
+	for (int k_ = 0; k_ < PPT; k_++) {
+		const long qq = pix(b, k_, (size_t)L.w * L.h);
+		if (qq < 0) break;
+		const size_t q = (size_t)qq;
+		const int i = (int)(q % L.w), j = (int)(q / L.w);
+		const unsigned d = (unsigned)L.D[q];
+		const unsigned e = (unsigned)L.wE[q], so = (unsigned)L.wS[q];
+		const unsigned we = i > 0 ? (unsigned)L.wE[q - 1] : 0u;
+		const unsigned no = j > 0 ? (unsigned)L.wS[q - L.w] : 0u;
+		L.gcode[q] = d | e << 9 | so << 13 | we << 17 | no << 21;
+	}
 }
 
+// The region's z sits in LDS with a zero frame: one row above and below
+// and one column after each row (the column before a row is the previous
+// row's frame), so the four neighbour reads of every region cell stay in
+// bounds and read 0 outside the region, with no tests.
+constexpr int ZS = RW + 1, ZOFF = ZS, ZN = (RH + 2) * ZS;
+
+static __device__ __forceinline__ int zid(int r, int col) { return ZOFF + r * ZS + col; }
+
+// level 0: byte codes, four to a register
+struct Lev0 {
+	struct Codes { unsigned v[MG_K]; };
+	typedef unsigned char Raw;
+	typedef uchar2 Raw2;
+	static __device__ __forceinline__ const Raw *codes(const MgLevel &L) { return L.code; }
+	static __device__ __forceinline__ void set(Codes &K, int kb, int c, unsigned x)
+	{ K.v[kb] = c ? K.v[kb] | x << (8 * c) : x; }
+	static __device__ __forceinline__ unsigned get(const Codes &K, int kb, int c)
+	{ return (K.v[kb] >> (8 * c)) & 255u; }
+	static __device__ __forceinline__ unsigned fresh(unsigned k)
+	{
+		asm volatile("" : "+v"(k));
+		return k;
+	}
+	static __device__ __forceinline__ float mask(unsigned k) { return (float)(k & 1u); }
+	static __device__ __forceinline__ float diag(unsigned k) { return (float)(k >> 5); }
+	// 1/diagonal, 0 for an inactive cell (code 0)
+	static __device__ __forceinline__ float rdiag(unsigned k)
+	{
+		const unsigned d = k >> 5;
+		return d == 4 ? 0.25f : d == 3 ? (1.0f / 3.0f) : d == 2 ? 0.5f : 0.0f;
+	}
+	// s + the neighbours of the cell at z[id] (E, S, W, N)
+	static __device__ __forceinline__ float gather(unsigned k, float s,
+			const float *z, int id)
+	{
+		const float e = z[id + 1], so = z[id + ZS], w = z[id - 1], n = z[id - ZS];
+		s += (k & 2) ? e : 0.0f;
+		s += (k & 4) ? so : 0.0f;
+		s += (k & 8) ? w : 0.0f;
+		s += (k & 16) ? n : 0.0f;
+		return s;
+	}
+};
+
+// levels 1..MG_FUSE_MAX: packed integer coefficients
+struct LevG {
+	struct Codes { unsigned v[MG_K][4]; };
+	typedef unsigned Raw;
+	typedef uint2 Raw2;
+	static __device__ __forceinline__ const Raw *codes(const MgLevel &L) { return L.gcode; }
+	static __device__ __forceinline__ void set(Codes &K, int kb, int c, unsigned x)
+	{ K.v[kb][c] = x; }
+	static __device__ __forceinline__ unsigned get(const Codes &K, int kb, int c)
+	{ return K.v[kb][c]; }
+	// decode the coefficients again in every half sweep: hoisting the
+	// decoded floats out of the sweep loop costs 80 registers per thread
+	static __device__ __forceinline__ unsigned fresh(unsigned k)
+	{
+		asm volatile("" : "+v"(k));
+		return k;
+	}
+	static __device__ __forceinline__ float mask(unsigned k)
+	{ return (k & 511u) ? 1.0f : 0.0f; }
+	static __device__ __forceinline__ float diag(unsigned k) { return (float)(k & 511u); }
+	// 1/diagonal, 0 for an inactive cell (diagonal 0).  The hardware
+	// reciprocal (1 ulp) is the same in the pre- and post-smoother, so the
+	// V-cycle stays symmetric.
+	static __device__ __forceinline__ float rdiag(unsigned k)
+	{
+		const unsigned d = k & 511u;
+		return d ? __builtin_amdgcn_rcpf((float)d) : 0.0f;
+	}
+	static __device__ __forceinline__ float gather(unsigned k, float s,
+			const float *z, int id)
+	{
+		s += (float)((k >> 9) & 15u) * z[id + 1];
+		s += (float)((k >> 13) & 15u) * z[id + ZS];
+		s += (float)((k >> 17) & 15u) * z[id - 1];
+		s += (float)((k >> 21) & 15u) * z[id - ZS];
+		return s;
+	}
+};
+
 // per-thread cells: 2x2 blocks, cell c of block kb at (2m + (c>>1), 2pc + (c&1))
-struct MgCells {
+template <class P> struct MgCells {
 	float b[MG_K][4];
-	unsigned char k[MG_K][4];
+	typename P::Codes k;
 	int pc, m0;
 };
 
-static __device__ __forceinline__ int cell_idx(const MgCells &C, int kb, int c)
+template <class P>
+static __device__ __forceinline__ int cell_z(const MgCells<P> &C, int kb, int c)
 {
-	return (2 * (C.m0 + MG_RG * kb) + (c >> 1)) * RW + 2 * C.pc + (c & 1);
+	return zid(2 * (C.m0 + MG_RG * kb) + (c >> 1), 2 * C.pc + (c & 1));
 }
 
-// load b and codes (zero outside the image); z of the region from zsrc or 0
-static __device__ void mg_load(const MgLevel &L, int ox, int oy, MgCells &C,
+// Region geometry of one thread: its columns i0, i0+1 and the rows
+// j0 + 2*MG_RG*kb + (0, 1); q0 = j0 * w + i0 (may lie outside the image).
+struct MgSpan {
+	int i0, j0, w, h;
+	long q0;
+	bool colok[2];
+};
+
+static __device__ __forceinline__ MgSpan mg_span(const MgLevel &L, int ox, int oy,
+		int pc, int m0)
+{
+	MgSpan S;
+	S.i0 = ox + 2 * pc;
+	S.j0 = oy + 2 * m0;
+	S.w = L.w;
+	S.h = L.h;
+	S.q0 = (long)S.j0 * L.w + S.i0;
+	S.colok[0] = S.i0 >= 0 && S.i0 < L.w;
+	S.colok[1] = S.i0 + 1 >= 0 && S.i0 + 1 < L.w;
+	return S;
+}
+
+static __device__ __forceinline__ bool row_ok(const MgSpan &S, int kb, int hr)
+{
+	const int j = S.j0 + 2 * MG_RG * kb + hr;
+	return j >= 0 && j < S.h;
+}
+
+static __device__ __forceinline__ long row_q(const MgSpan &S, int kb, int hr)
+{
+	return S.q0 + (long)(2 * MG_RG * kb + hr) * S.w;
+}
+
+// zero frame of the region's z
+static __device__ __forceinline__ void z_frame(float *z)
+{
+	for (int e = threadIdx.x; e < 2 * ZS + RH; e += MG_T) {
+		const int at = e < ZS ? e : e < 2 * ZS ? ZOFF + RH * ZS + (e - ZS)
+			: ZOFF + (e - 2 * ZS) * ZS + RW;
+		z[at] = 0.0f;
+	}
+}
+
+// load b and codes (zero outside the image); z of the region from zsrc or 0.
+// With an even width the two cells of a block row are one aligned pair
+// (region origins are even), loaded together.
+template <class P>
+static __device__ void mg_load(const MgLevel &L, int ox, int oy, MgCells<P> &C,
 		float *z, const float *zsrc)
 {
 	C.pc = threadIdx.x % PAIRS_X;
 	C.m0 = threadIdx.x / PAIRS_X;
+	const MgSpan S = mg_span(L, ox, oy, C.pc, C.m0);
+	const typename P::Raw *code = P::codes(L);
+	const bool pair = !(L.w & 1);
 #pragma unroll
 	for (int kb = 0; kb < MG_K; kb++)
 #pragma unroll
-		for (int c = 0; c < 4; c++) {
-			const int r = 2 * (C.m0 + MG_RG * kb) + (c >> 1), col = 2 * C.pc + (c & 1);
-			const int i = ox + col, j = oy + r;
-			float bv = 0.0f, zv = 0.0f;
-			unsigned char kv = 0;
-			if (i >= 0 && i < L.w && j >= 0 && j < L.h) {
-				const size_t q = (size_t)j * L.w + i;
-				bv = L.b[q];
-				kv = L.code[q];
-				if (zsrc) zv = zsrc[q];
+		for (int hr = 0; hr < 2; hr++) {
+			float bv[2] = {0.0f, 0.0f}, zv[2] = {0.0f, 0.0f};
+			unsigned kv[2] = {0u, 0u};
+			if (row_ok(S, kb, hr)) {
+				const long q = row_q(S, kb, hr);
+				if (pair) {
+					if (S.colok[0]) {
+						const float2 b2 = *(const float2 *)(L.b + q);
+						const typename P::Raw2 k2 = *(const typename P::Raw2 *)(code + q);
+						bv[0] = b2.x; bv[1] = b2.y;
+						kv[0] = k2.x; kv[1] = k2.y;
+						if (zsrc) {
+							const float2 z2 = *(const float2 *)(zsrc + q);
+							zv[0] = z2.x; zv[1] = z2.y;
+						}
+					}
+				} else {
+#pragma unroll
+					for (int hc = 0; hc < 2; hc++)
+						if (S.colok[hc]) {
+							bv[hc] = L.b[q + hc];
+							kv[hc] = code[q + hc];
+							if (zsrc) zv[hc] = zsrc[q + hc];
+						}
+				}
 			}
-			C.b[kb][c] = bv;
-			C.k[kb][c] = kv;
-			z[r * RW + col] = zv;
+#pragma unroll
+			for (int hc = 0; hc < 2; hc++) {
+				const int c = 2 * hr + hc;
+				C.b[kb][c] = bv[hc];
+				P::set(C.k, kb, c, kv[hc]);
+				z[cell_z(C, kb, c)] = zv[hc];
+			}
 		}
 }
 
+// store z of the thread's tile cells (inside the image)
+template <class P>
+static __device__ __forceinline__ void mg_store_z(const MgLevel &L, int ox, int oy,
+		const MgCells<P> &C, const float *z, int kb)
+{
+	const MgSpan S = mg_span(L, ox, oy, C.pc, C.m0);
+#pragma unroll
+	for (int hr = 0; hr < 2; hr++) {
+		if (!row_ok(S, kb, hr)) continue;
+		const long q = row_q(S, kb, hr);
+		const float z0 = z[cell_z(C, kb, 2 * hr)], z1 = z[cell_z(C, kb, 2 * hr + 1)];
+		if (!(L.w & 1)) {
+			if (S.colok[0]) *(float2 *)(L.z + q) = make_float2(z0, z1);
+		} else {
+			if (S.colok[0]) L.z[q] = z0;
+			if (S.colok[1]) L.z[q + 1] = z1;
+		}
+	}
+}
+
 // red-black half sweep over the thread's cells of colour `color`: all
-// neighbour loads first (cells of one colour do not depend on each other)
-static __device__ __forceinline__ void mg_half(const MgCells &C, float *z, int color,
-		bool first)
+// neighbour loads first (cells of one colour do not depend on each other).
+// An inactive cell (known pixel, empty aggregate, outside the image) has
+// b = 0, no couplings and 1/diagonal = 0, so it keeps z = 0 without a test.
+template <class P>
+static __device__ __forceinline__ void mg_half(const MgCells<P> &C, float *z,
+		int color, bool first)
 {
 	float nz[MG_K][2];
 #pragma unroll
@@ -564,28 +755,22 @@ static __device__ __forceinline__ void mg_half(const MgCells &C, float *z, int c
 			// colour of cell c is ((c>>1) + (c&1) + pair origin parity) & 1;
 			// origins are even, so cells 0,3 have colour 0 and 1,2 colour 1
 			const int c = color == 0 ? (h ? 3 : 0) : (h ? 2 : 1);
-			const unsigned k = C.k[kb][c];
-			const int idx = cell_idx(C, kb, c);
-			const int r = idx / RW, col = idx - r * RW;
+			const unsigned k = P::fresh(P::get(C.k, kb, c));
 			float s = C.b[kb][c];
-			if (!first) {
-				if ((k & 2) && col + 1 < RW) s += z[idx + 1];
-				if ((k & 4) && r + 1 < RH) s += z[idx + RW];
-				if ((k & 8) && col > 0) s += z[idx - 1];
-				if ((k & 16) && r > 0) s += z[idx - RW];
-			}
-			nz[kb][h] = s * inv_diag(k);
+			if (!first) s = P::gather(k, s, z, cell_z(C, kb, c));
+			nz[kb][h] = s * P::rdiag(k);
 		}
 #pragma unroll
 	for (int kb = 0; kb < MG_K; kb++)
 #pragma unroll
 		for (int h = 0; h < 2; h++) {
 			const int c = color == 0 ? (h ? 3 : 0) : (h ? 2 : 1);
-			if (C.k[kb][c] & 1) z[cell_idx(C, kb, c)] = nz[kb][h];
+			z[cell_z(C, kb, c)] = nz[kb][h];
 		}
 }
 
-static __device__ __forceinline__ bool in_tile(const MgCells &C, int kb)
+template <class P>
+static __device__ __forceinline__ bool in_tile(const MgCells<P> &C, int kb)
 {
 	const int m = C.m0 + MG_RG * kb;
 	return C.pc >= HALO / 2 && C.pc < HALO / 2 + TX / 2
@@ -593,15 +778,18 @@ static __device__ __forceinline__ bool in_tile(const MgCells &C, int kb)
 }
 
 // pre-smoothing from z = 0, residual, 2x2 restriction into the next level
-__global__ __launch_bounds__(MG_T) void k_mg_down0(const int4 *tiles,
+template <class P>
+__global__ __launch_bounds__(MG_T) __attribute__((amdgpu_waves_per_eu(4)))
+void k_mg_down(const int4 *tiles,
 		const MgLevel *lv, const MgLevel *coarse, const Cg32Scal *sc)
 {
-	__shared__ float z[RW * RH];
+	__shared__ float z[ZN];
 	const int4 t = tiles[blockIdx.x];
 	if (!sc[t.x].active) return;
 	const MgLevel L = lv[t.x], Cc = coarse[t.x];
 	const int ox = t.y - HALO, oy = t.z - HALO;
-	MgCells C;
+	MgCells<P> C;
+	z_frame(z);
 	mg_load(L, ox, oy, C, z, nullptr);
 	__syncthreads();
 	for (int s = 0; s < 2 * MG_NU; s++) {
@@ -611,20 +799,14 @@ __global__ __launch_bounds__(MG_T) void k_mg_down0(const int4 *tiles,
 #pragma unroll
 	for (int kb = 0; kb < MG_K; kb++) {
 		if (!in_tile(C, kb)) continue;
+		mg_store_z(L, ox, oy, C, z, kb);
 		float acc = 0.0f;
 #pragma unroll
 		for (int c = 0; c < 4; c++) {
-			const int idx = cell_idx(C, kb, c);
-			const unsigned k = C.k[kb][c];
-			const int r = idx / RW, col = idx - r * RW;
-			const int i = ox + col, j = oy + r;
-			if (i < L.w && j < L.h) L.z[(size_t)j * L.w + i] = z[idx];
-			if (!(k & 1)) continue;
-			float az = (float)(k >> 5) * z[idx];
-			if (k & 2) az -= z[idx + 1];
-			if (k & 4) az -= z[idx + RW];
-			if (k & 8) az -= z[idx - 1];
-			if (k & 16) az -= z[idx - RW];
+			const unsigned k = P::get(C.k, kb, c);
+			const int id = cell_z(C, kb, c);
+			// inactive cells add b - A z = 0 - 0
+			const float az = P::diag(k) * z[id] - P::gather(k, 0.0f, z, id);
 			acc += C.b[kb][c] - az;
 		}
 		const int I = (ox >> 1) + C.pc, J = (oy >> 1) + C.m0 + MG_RG * kb;
@@ -636,17 +818,21 @@ __global__ __launch_bounds__(MG_T) void k_mg_down0(const int4 *tiles,
 	}
 }
 
-// coarse correction, post-smoothing, partial r.z of the tile
-__global__ __launch_bounds__(MG_T) void k_mg_up0(const int4 *tiles, const MgLevel *lv,
+// coarse correction, post-smoothing; at level 0 also the partial r.z of
+// the tile
+template <class P, bool DOT>
+__global__ __launch_bounds__(MG_T) __attribute__((amdgpu_waves_per_eu(4)))
+void k_mg_up(const int4 *tiles, const MgLevel *lv,
 		const MgLevel *coarse, const Cg32Scal *sc, float wc, double *tpart)
 {
-	__shared__ float z[RW * RH];
+	__shared__ float z[ZN];
 	__shared__ double red[MG_T / 64];
 	const int4 t = tiles[blockIdx.x];
 	if (!sc[t.x].active) return;
 	const MgLevel L = lv[t.x], Cc = coarse[t.x];
 	const int ox = t.y - HALO, oy = t.z - HALO;
-	MgCells C;
+	MgCells<P> C;
+	z_frame(z);
 	mg_load(L, ox, oy, C, z, L.z);
 	// every block of the region is one coarse cell: add its correction
 #pragma unroll
@@ -656,7 +842,7 @@ __global__ __launch_bounds__(MG_T) void k_mg_up0(const int4 *tiles, const MgLeve
 		const float e = wc * Cc.z[(size_t)J * Cc.w + I];
 #pragma unroll
 		for (int c = 0; c < 4; c++)
-			if (C.k[kb][c] & 1) z[cell_idx(C, kb, c)] += e;
+			z[cell_z(C, kb, c)] += e * P::mask(P::get(C.k, kb, c));
 	}
 	__syncthreads();
 	for (int s = 0; s < 2 * MG_NU; s++) {
@@ -667,19 +853,17 @@ __global__ __launch_bounds__(MG_T) void k_mg_up0(const int4 *tiles, const MgLeve
 #pragma unroll
 	for (int kb = 0; kb < MG_K; kb++) {
 		if (!in_tile(C, kb)) continue;
+		mg_store_z(L, ox, oy, C, z, kb);
+		if (DOT) {
 #pragma unroll
-		for (int c = 0; c < 4; c++) {
-			const int idx = cell_idx(C, kb, c);
-			const int r = idx / RW, col = idx - r * RW;
-			const int i = ox + col, j = oy + r;
-			if (i >= L.w || j >= L.h) continue;
-			const float zz = z[idx];
-			L.z[(size_t)j * L.w + i] = zz;
-			if (C.k[kb][c] & 1) acc += (double)C.b[kb][c] * zz;
+			for (int c = 0; c < 4; c++)   // 0 at inactive cells
+				acc += (double)C.b[kb][c] * z[cell_z(C, kb, c)];
 		}
 	}
-	const double sum = bsum<MG_T>(acc, red);
-	if (threadIdx.x == 0) tpart[blockIdx.x] = sum;
+	if (DOT) {
+		const double sum = bsum<MG_T>(acc, red);
+		if (threadIdx.x == 0) tpart[blockIdx.x] = sum;
+	}
 }
 
 // rho_new = r.z folded over the plane's level-0 tiles
@@ -786,6 +970,11 @@ __global__ __launch_bounds__(MG_TAIL_T) void k_mg_tail(const MgLevel *lvall,
 	}
 }
 
+static const char *const g_down_name[MG_FUSE_MAX + 1] = {
+	"mg_down0", "mg_down1", "mg_down2", "mg_down3"};
+static const char *const g_up_name[MG_FUSE_MAX + 1] = {
+	"mg_up0", "mg_up1", "mg_up2", "mg_up3"};
+
 static void vcycle(const Cg32Batch &B, hipStream_t s)
 {
 	const int L = B.nlev;
@@ -794,11 +983,20 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 	// z and a quarter coarse b (10 B); up reads z, b, the code and a quarter
 	// coarse z, writes z (14 B)
 	if (B.prof) B.prof("mg_down0", 1, 0, s);
-	k_mg_down0<<<B.ntiles0, MG_T, 0, s>>>(B.tiles0, B.mg[0].lv, B.mg[0].coarse, B.scal);
+	k_mg_down<Lev0><<<B.ntiles0, MG_T, 0, s>>>(B.tiles0, B.mg[0].lv, B.mg[0].coarse,
+			B.scal);
 	if (B.prof) B.prof("mg_down0", 0, 10.0 * B.px0, s);
-	// large levels >= 1 not in a tail: pre-smooth, restrict
+	// levels >= 1 outside the tail: fused tiles (packed coefficients; per
+	// cell down reads b and the code, writes z and a quarter coarse b and z,
+	// 14 B; up reads z, b, the code and a quarter coarse z, writes z, 17 B)
+	// or per-level kernels (pre-smooth, restrict)
 	for (int l = 1; l < L; l++) {
 		const MgLevelWork &W = B.mg[l];
+		if (W.ntiles) {
+			if (B.prof) B.prof(g_down_name[l], 1, 0, s);
+			k_mg_down<LevG><<<W.ntiles, MG_T, 0, s>>>(W.tiles, W.lv, W.coarse, B.scal);
+			if (B.prof) B.prof(g_down_name[l], 0, 14.0 * W.cells, s);
+		}
 		if (!W.nmid) continue;
 		for (int it = 0; it < B.nu; it++) {
 			k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 0, it == 0, 0);
@@ -807,21 +1005,30 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 		k_mg_restrict<<<W.ncells_c, NT, 0, s>>>(W.cells_c, W.lv, W.coarse, B.scal, 0);
 	}
 	// small levels of every plane in one work-group each
+	if (B.prof) B.prof("mg_tail", 1, 0, s);
 	k_mg_tail<<<B.nplanes, MG_TAIL_T, 0, s>>>(B.tail_lv, B.tail_rng, B.scal, B.nu,
 			B.coarse_sweeps, B.wc);
+	if (B.prof) B.prof("mg_tail", 0, 0.0, s);
 	// up: correct, post-smooth in the mirrored order
 	for (int l = L - 1; l >= 1; l--) {
 		const MgLevelWork &W = B.mg[l];
-		if (!W.nmid) continue;
-		k_mg_prolong<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, W.coarse, B.scal, B.wc, 0);
-		for (int it = 0; it < B.nu; it++) {
-			k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 1, 0, 0);
-			k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 0, 0, 0);
+		if (W.nmid) {
+			k_mg_prolong<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, W.coarse, B.scal, B.wc, 0);
+			for (int it = 0; it < B.nu; it++) {
+				k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 1, 0, 0);
+				k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 0, 0, 0);
+			}
+		}
+		if (W.ntiles) {
+			if (B.prof) B.prof(g_up_name[l], 1, 0, s);
+			k_mg_up<LevG, false><<<W.ntiles, MG_T, 0, s>>>(W.tiles, W.lv, W.coarse,
+					B.scal, B.wc, nullptr);
+			if (B.prof) B.prof(g_up_name[l], 0, 17.0 * W.cells, s);
 		}
 	}
 	if (B.prof) B.prof("mg_up0", 1, 0, s);
-	k_mg_up0<<<B.ntiles0, MG_T, 0, s>>>(B.tiles0, B.mg[0].lv, B.mg[0].coarse, B.scal,
-			B.wc, B.tpart);
+	k_mg_up<Lev0, true><<<B.ntiles0, MG_T, 0, s>>>(B.tiles0, B.mg[0].lv, B.mg[0].coarse,
+			B.scal, B.wc, B.tpart);
 	if (B.prof) B.prof("mg_up0", 0, 14.0 * B.px0, s);
 }
 
@@ -841,6 +1048,12 @@ void mg_build(const Cg32Batch &B, hipStream_t s)
 		const MgLevelWork &W = B.mg[l];
 		if (W.nbuild_c)
 			k_mg_coarsen<<<W.nbuild_c, NT, 0, s>>>(W.build_c, W.lv, W.coarse, l == 0);
+	}
+	// packed coefficients of the fused levels (their couplings are final)
+	for (int l = 0; l < MG_FUSE_MAX && l + 1 < B.nlev; l++) {
+		const MgLevelWork &W = B.mg[l];
+		if (W.nbuild_c && B.mg[l + 1].ntiles)
+			k_mg_gcode<<<W.nbuild_c, NT, 0, s>>>(W.build_c, W.coarse);
 	}
 }
 

@@ -297,6 +297,9 @@ struct Plan {
 	}
 };
 
+// per-plane arrays start on 256-byte boundaries (paired loads need 8)
+static size_t rup64(size_t n) { return (n + 63) & ~(size_t)63; }
+
 static int op_radius(float tstep, float *teff)
 {
 	if (!(tstep < 0)) { *teff = tstep; return 1; }
@@ -598,14 +601,22 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 	size_t cells = 0;
 	for (int z = 0; z < np; z++)
 		for (size_t l = 1; l < shp[z].size(); l++)
-			cells += (size_t)shp[z][l].first * shp[z][l].second;
+			cells += rup64((size_t)shp[z][l].first * shp[z][l].second);
 	float *pool = nullptr;
 	TRY(pl.alloc(&pool, 5 * std::max<size_t>(cells, 1)));
 	size_t px0 = 0;
-	for (int z = 0; z < np; z++) px0 += (size_t)cp[z].w * cp[z].h;
+	for (int z = 0; z < np; z++) px0 += rup64((size_t)cp[z].w * cp[z].h);
 	unsigned char *codes = nullptr;
 	TRY(pl.alloc(&codes, px0));
 	size_t coff = 0;
+	// packed coefficients of the levels that may run fused
+	size_t gcells = 0;
+	for (int z = 0; z < np; z++)
+		for (size_t l = 1; l < shp[z].size() && l <= (size_t)MG_FUSE_MAX; l++)
+			gcells += rup64((size_t)shp[z][l].first * shp[z][l].second);
+	unsigned *gcodes = nullptr;
+	TRY(pl.alloc(&gcodes, std::max<size_t>(gcells, 1)));
+	size_t goff = 0;
 	std::vector<int4> tiles0;
 	std::vector<std::vector<MgLevel>> lv(nlev, std::vector<MgLevel>(np));
 	size_t off = 0;
@@ -619,7 +630,7 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 				L.z = cp[z].z;
 				L.b = cp[z].r;
 				L.code = codes + coff;
-				coff += (size_t)L.w * L.h;
+				coff += rup64((size_t)L.w * L.h);
 				cp[z].t0 = (int)tiles0.size();
 				for (int y0 = 0; y0 < L.h; y0 += MG_TY)
 					for (int x0 = 0; x0 < L.w; x0 += MG_TX)
@@ -627,7 +638,8 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 				cp[z].nt = (int)tiles0.size() - cp[z].t0;
 				continue;
 			}
-			const size_t n = (size_t)L.w * L.h;
+			const size_t n = rup64((size_t)L.w * L.h);
+			if (l <= (size_t)MG_FUSE_MAX) { L.gcode = gcodes + goff; goff += n; }
 			L.D = pool + off; off += n;
 			L.wE = pool + off; off += n;
 			L.wS = pool + off; off += n;
@@ -646,13 +658,22 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 	mgw.assign(nlev, MgLevelWork{});
 	for (int l = 0; l < nlev; l++) {
 		std::vector<BlockWork> mid, cc, bc;
+		std::vector<int4> tl;
+		double tcells = 0;
 		for (int z = 0; z < np; z++) {
 			const int d = (int)shp[z].size();
 			if (d <= l) continue;
-			const size_t n = (size_t)lv[l][z].w * lv[l][z].h;
+			const MgLevel &Lz = lv[l][z];
+			const size_t n = (size_t)Lz.w * Lz.h;
 			const size_t nc = d > l + 1 ? (size_t)lv[l + 1][z].w * lv[l + 1][z].h : 0;
 			for (size_t st = 0; st < nc; st += EW_SPAN) bc.push_back({z, (int)st});
-			if (l >= 1 && l < rng[z].x) {   // a large level outside the tail
+			if (l < 1 || l >= rng[z].x) continue;   // level 0 or in the tail
+			if (l <= MG_FUSE_MAX) {   // fused tiles
+				for (int y0 = 0; y0 < Lz.h; y0 += MG_TY)
+					for (int x0 = 0; x0 < Lz.w; x0 += MG_TX)
+						tl.push_back(make_int4(z, x0, y0, 0));
+				tcells += (double)n;
+			} else {                  // per-level kernels
 				for (size_t st = 0; st < n; st += EW_SPAN) mid.push_back({z, (int)st});
 				for (size_t st = 0; st < nc; st += EW_SPAN) cc.push_back({z, (int)st});
 			}
@@ -663,6 +684,9 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 		TRY(pl.upload(&W.mid, mid));
 		TRY(pl.upload(&W.cells_c, cc));
 		TRY(pl.upload(&W.build_c, bc));
+		TRY(pl.upload(&W.tiles, tl));
+		W.ntiles = (int)tl.size();
+		W.cells = tcells;
 		W.nmid = (int)mid.size();
 		W.nlast = 0;
 		W.ncells_c = (int)cc.size();
@@ -696,7 +720,7 @@ static int cg32_build(Plan &pl, int k, int bilap, float eps)
 		if (L.cgit[m] != 0) C.ms.push_back((int)m);
 	if (C.ms.empty()) { C.built = true; return SP_OK; }
 	size_t need = 0;
-	for (int m : C.ms) need += (size_t)L.sp[m].w * L.sp[m].h;
+	for (int m : C.ms) need += rup64((size_t)L.sp[m].w * L.sp[m].h);
 	if (need > pl.cg32_cap) {
 		TRY(pl.alloc(&pl.cg32_vec, need * 4));
 		TRY(pl.alloc(&pl.cg32_x, need));
@@ -711,15 +735,15 @@ static int cg32_build(Plan &pl, int k, int bilap, float eps)
 	for (size_t z = 0; z < C.ms.size(); z++) {
 		const int m = C.ms[z];
 		const SpPlane &S = L.sp[m];
-		const size_t n = (size_t)S.w * S.h;
+		const size_t n = (size_t)S.w * S.h, na = rup64(n);
 		Cg32Plane c{};
 		c.g = S.g; c.f = S.f; c.u = S.u; c.cnt = S.cnt;
-		c.x = pl.cg32_x + offx; offx += n;
-		c.r = pl.cg32_vec + off; off += n;
-		c.p = pl.cg32_vec + off; off += n;
-		c.q = pl.cg32_vec + off; off += n;
+		c.x = pl.cg32_x + offx; offx += na;
+		c.r = pl.cg32_vec + off; off += na;
+		c.p = pl.cg32_vec + off; off += na;
+		c.q = pl.cg32_vec + off; off += na;
 		c.z = c.r;
-		if (pl.P.cg_mg) { c.z = pl.cg32_vec + off; off += n; }
+		if (pl.P.cg_mg) { c.z = pl.cg32_vec + off; off += na; }
 		c.w = S.w; c.h = S.h; c.bilap = bilap;
 		c.blk0 = (int)bw.size();
 		for (size_t st = 0; st < n; st += EW_SPAN) bw.push_back({(int)z, (int)st});
