@@ -421,6 +421,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-side", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="built-in Gauss-Seidel tile shapes instead of the "
+                         "autotuner's choice (tuned during the warmup solve)")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
                     help="c2 (default) is the headline; the others print their "
                          "own JSON line")
@@ -446,7 +449,8 @@ def main():
         a.niter = {"cg": 10, "gs": 256}.get(a.method, 0)
     params = dict(tstep=0.25, niter=a.niter, scale=1 if a.method == "mgpcg1" else 99,
                   cgit=0 if a.method == "gs" else -1, cg_fp32=1, cg_rel=a.cg_rel,
-                  trace=0, cg_chunk=32, cg_mg=0 if a.method == "cg" else 1)
+                  trace=0, cg_chunk=32, cg_mg=0 if a.method == "cg" else 1,
+                  autotune=0 if a.no_autotune else 1)
     torch.cuda.synchronize()
 
     def step():
@@ -552,20 +556,33 @@ def main():
         result["gs_tile"] = {"ms": round(gs_ms, 3), "launches": gs_launch,
                              "pass_GBps": round(gs_bytes / (gs_ms * 1e-3) / 1e9, 1)}
     if gs_ms > 0 and a.method == "gs":
-        # per-pass compulsory traffic: a launch group of BT fused sweeps reads
-        # u, f and writes u once per pixel (12 B); the effective bandwidth
-        # counts every sweep
+        # per-pass compulsory traffic: a pass of BT fused sweeps reads u and
+        # the coefficient plane and writes u once per pixel (12 B); the
+        # effective bandwidth counts those 12 B for every sweep of every
+        # level swept in tiles (levels whose u and coefficients exceed 64 KiB)
         achieved = gs_bytes / (gs_ms * 1e-3) / 1e9
-        bt = 16
+        tiled_px, w = 0, a.size
+        for _ in range(stats.levels):
+            if 8 * w * w > 65536:
+                tiled_px += w * w * a.channels
+            w = (w + 1) // 2
+        eff = 12.0 * a.niter * tiled_px / (gs_ms * 1e-3) / 1e9
+        tune = sp.autotune_report().strip().splitlines()
+        finest = max(tune, key=lambda ln: int(ln.split("px=")[1].split()[0])) if tune else ""
+        shape = ([c[1:].split(":")[0] for c in finest.split("choice=")[1].split(",")
+                  if c.startswith("*")] or ["32x128x16"])[0] if finest else "32x128x16"
         result["roofline"] = {
-            "kernel": f"k_gs_tile<1> ({bt} lexicographic sweeps fused per pass, "
-                      "wavefront of skewed tiles per launch)",
+            "kernel": f"k_gs_tile<1> (tile {shape} = BIxBJxBT: BT lexicographic "
+                      "sweeps fused per pass, one launch per wavefront of "
+                      "skewed tiles)",
             "bound": "hbm", "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBPS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
             "algorithmic_bytes": gs_bytes, "ms": round(gs_ms, 3),
             "launches": gs_launch,
-            "sweep_GBps_effective": round(achieved * bt, 1)}
+            "sweep_GBps_effective": round(eff, 1),
+            "sweep_frac_effective": round(eff / PEAK_HBM_GBPS, 4)}
+        result["autotune"] = tune
     if not a.no_cpu:
         result["cpu_baseline"] = cpu_baseline(g, f, a.cg_rel, a.cpu_side, params)
     print(json.dumps(result), flush=True)

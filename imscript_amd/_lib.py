@@ -16,7 +16,8 @@ class sp_params(ctypes.Structure):
                 ("pmsfac", ctypes.c_double), ("ponlit", ctypes.c_double),
                 ("cg_eps", ctypes.c_double), ("trace", ctypes.c_int),
                 ("cg_fp32", ctypes.c_int), ("cg_rel", ctypes.c_double),
-                ("cg_chunk", ctypes.c_int), ("cg_mg", ctypes.c_int)]
+                ("cg_chunk", ctypes.c_int), ("cg_mg", ctypes.c_int),
+                ("autotune", ctypes.c_int)]
 
 
 class sp_image(ctypes.Structure):
@@ -57,6 +58,7 @@ _SIGNATURES = {
     "sp_profile_read": (_I, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                              ctypes.POINTER(ctypes.c_long),
                              ctypes.POINTER(ctypes.c_double)]),
+    "sp_autotune_report": (_I, [ctypes.c_char_p, _I]),
 }
 
 

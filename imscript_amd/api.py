@@ -162,3 +162,12 @@ def profile_read(name):
     check(lib.sp_profile_read(name.encode(), ctypes.byref(ms), ctypes.byref(n),
                               ctypes.byref(by)))
     return ms.value, n.value, by.value
+
+
+def autotune_report():
+    """Tile shapes chosen by the autotuner in this process, one text line per
+    tuned level size (see sp_autotune_report in include/simpois_hip.h)."""
+    n = lib.sp_autotune_report(None, 0)
+    buf = ctypes.create_string_buffer(n)
+    lib.sp_autotune_report(buf, n)
+    return buf.value.decode()

@@ -45,11 +45,39 @@ struct GsInst {
 // Skewed temporal tile: (instance, I', J', T').
 struct GsTile { int inst, ti, tj, tt; };
 
-// Tile shapes (skewed extents and sweeps per tile) per stencil radius.
-template <int R> struct TileShape;
-template <> struct TileShape<1> { static constexpr int BI = 32, BJ = 128, BT = 16; };
-template <> struct TileShape<2> { static constexpr int BI = 32, BJ = 32, BT = 8; };
-template <> struct TileShape<3> { static constexpr int BI = 32, BJ = 32, BT = 4; };
+// Tile shape: skewed extents BI x BJ and BT sweeps per tile (the sweeps of
+// one pass over the image) for stencil radius R.
+template <int R_, int BI_, int BJ_, int BT_> struct Shape {
+	static constexpr int R = R_, BI = BI_, BJ = BJ_, BT = BT_;
+};
+
+// The compiled shapes (radius, variant, BI, BJ, BT); variant 0 of each
+// radius is the default, the others are candidates for the autotuner.
+#define SP_GS_SHAPES(X) \
+	X(1, 0, 32, 128, 16) X(1, 1, 32, 64, 16) X(1, 2, 64, 64, 16) \
+	X(1, 3, 16, 128, 32) X(1, 4, 32, 64, 8) X(1, 5, 64, 32, 16) \
+	X(2, 0, 32, 32, 8) X(2, 1, 32, 64, 8) X(2, 2, 32, 32, 16) \
+	X(2, 3, 64, 32, 8) \
+	X(3, 0, 32, 32, 4) X(3, 1, 32, 64, 4) X(3, 2, 32, 32, 8)
+
+// calls f(Shape<...>{}) for variant v of radius R; false if there is none
+template <class F> inline bool sp_with_shape(int R, int v, F &&f)
+{
+#define SP_SHAPE_CASE(r, var, bi, bj, bt) \
+	if (R == r && v == var) { f(Shape<r, bi, bj, bt>{}); return true; }
+	SP_GS_SHAPES(SP_SHAPE_CASE)
+#undef SP_SHAPE_CASE
+	return false;
+}
+
+inline int sp_shape_count(int R)
+{
+	int n = 0;
+#define SP_SHAPE_COUNT(r, var, bi, bj, bt) n += R == r;
+	SP_GS_SHAPES(SP_SHAPE_COUNT)
+#undef SP_SHAPE_COUNT
+	return n;
+}
 
 SP_HD uint32_t sp_bits(float x) { return __builtin_bit_cast(uint32_t, x); }
 SP_HD float sp_float(uint32_t b) { return __builtin_bit_cast(float, b); }
@@ -124,9 +152,8 @@ SP_HD void gs_update(float *U, float cv, int idx, int i, int j, int W, int H,
 
 // ------------------------------------------------------------- tiles
 
-template <int R> struct Geo {
-	static constexpr int BI = TileShape<R>::BI, BJ = TileShape<R>::BJ,
-		BT = TileShape<R>::BT;
+template <class S> struct Geo {
+	static constexpr int R = S::R, BI = S::BI, BJ = S::BJ, BT = S::BT;
 	static constexpr int SH = R - 1;                       // col shear per row
 	static constexpr int HC = R == 1 ? 1 : R * (R - 1);    // column halo
 	static constexpr int LW0 = R == 1 ? BI + BT + 1 : BI + R * (R - 1) * (BT - 1) + 2 * HC;
@@ -135,21 +162,24 @@ template <int R> struct Geo {
 	static constexpr int NT = BI * BT;                      // threads per tile
 	static constexpr int NSTEP = R == 1 ? BI + BJ - 1 : BI + BJ + BT - 2;
 	static constexpr int LDS_FLOATS = 2 * LW * LH;          // u and coefficient
+	static_assert(NT <= 1024 && NT % 64 == 0, "tile threads");
+	static_assert(LDS_FLOATS * 4 <= 65536, "tile footprint exceeds 64 KiB");
 };
 
 // placement of one tile: LDS cell (row, col) holds pixel
 // j = row + jlo, i = col + K - SH*j
-template <int R> struct TileCtx {
+template <class S> struct TileCtx {
 	int x0, y0, jlo, K, bte, W, H;
 	float t;
 	float *U, *C;
 };
 
-template <int R>
-SP_HD TileCtx<R> tile_ctx(const GsTile &tl, const GsInst &in, float *lds)
+template <class S>
+SP_HD TileCtx<S> tile_ctx(const GsTile &tl, const GsInst &in, float *lds)
 {
-	using G = Geo<R>;
-	TileCtx<R> c;
+	using G = Geo<S>;
+	constexpr int R = S::R;
+	TileCtx<S> c;
 	const int k0 = tl.tt * G::BT;
 	c.bte = in.nsweep - k0 < G::BT ? in.nsweep - k0 : G::BT;
 	if (R == 1) {
@@ -171,11 +201,16 @@ SP_HD TileCtx<R> tile_ctx(const GsTile &tl, const GsInst &in, float *lds)
 	return c;
 }
 
-// phase 1 (thread tid): copy the tile's footprint into LDS
-template <int R>
-SP_HD void tile_load(const TileCtx<R> &c, const GsInst &in, int tid)
+// phase 1 (thread tid): copy the tile's footprint into LDS.  For R = 1 a
+// cell outside the image holds a NaN value and the known marker: the
+// Laplacian skips non-finite neighbours, and leaving out a clamped
+// neighbour (the pixel itself, adding x - x = +0) does not change the sum,
+// so the update needs no image-bounds tests.
+template <class S>
+SP_HD void tile_load(const TileCtx<S> &c, const GsInst &in, int tid)
 {
-	using G = Geo<R>;
+	using G = Geo<S>;
+	constexpr int R = S::R;
 	for (int e = tid; e < G::LW * G::LH; e += G::NT) {
 		const int row = e / G::LW, col = e - row * G::LW;
 		const int j = row + c.jlo, i = col + c.K - G::SH * j;
@@ -183,20 +218,50 @@ SP_HD void tile_load(const TileCtx<R> &c, const GsInst &in, int tid)
 			const size_t q = (size_t)j * c.W + i;
 			c.U[e] = in.u[q];
 			c.C[e] = in.c[q];
+		} else if (R == 1) {
+			c.U[e] = sp_float(SP_QNAN_BITS);
+			c.C[e] = sp_float(SP_KNOWN_BITS);
 		}
 	}
 }
 
+// R = 1 update of the cell U[idx] inside a loaded tile (every neighbour is
+// a footprint cell; outside the image it is NaN), same terms and order as
+// gs_update<1>.  A known cell stores its own value back: no other thread
+// writes it on this hyperplane, so the store changes nothing, and the reads
+// need not wait for the known test.
+SP_HD void gs_update_tile1(float *U, const float *C, int idx, int RS, float t)
+{
+	const float cv = C[idx], x = U[idx];
+	const float e = U[idx + 1], so = U[idx + RS], w = U[idx - 1], n = U[idx - RS];
+	float r = 0.0f;
+	if (isfinite(e)) r = r + (e - x);
+	if (isfinite(so)) r = r + (so - x);
+	if (isfinite(w)) r = r + (w - x);
+	if (isfinite(n)) r = r + (n - x);
+	const float y = x + t * (r - cv);
+	U[idx] = sp_known(cv) ? x : y;
+}
+
 // phase 2 (thread tid, step sg): the update of thread (a, tau) on the
 // in-tile hyperplane sg; steps run in increasing sg with a barrier between
-template <int R>
-SP_HD void tile_step(const TileCtx<R> &c, int tid, int sg)
+template <class S>
+SP_HD void tile_step(const TileCtx<S> &c, int tid, int sg)
 {
-	using G = Geo<R>;
+	using G = Geo<S>;
+	constexpr int R = S::R;
 	const int a = tid % G::BI, tau = tid / G::BI;
 	if (tau >= c.bte) return;
 	const int b = R == 1 ? sg - a : sg - a - tau;
 	if (b < 0 || b >= G::BJ) return;
+	if (R == 1) {
+		// (i, j) = (x0 + a - tau, y0 + b - tau) sits at row j - jlo =
+		// b - tau + BT, column i - K = a - tau + BT; cells outside the
+		// image carry the known marker
+		const int idx = (b - tau + G::BT) * G::LW + (a - tau + G::BT);
+		gs_update_tile1(c.U, c.C, idx, G::LW, c.t);
+		return;
+	}
 	int i, j;
 	if (R == 1) { i = c.x0 + a - tau; j = c.y0 + b - tau; }
 	else { i = c.x0 + a - G::SH * b; j = c.y0 + b - R * tau; }
@@ -208,10 +273,19 @@ SP_HD void tile_step(const TileCtx<R> &c, int tid, int sg)
 }
 
 // does this tile update pixel (i, j) at some sweep?
-template <int R>
-SP_HD bool tile_owns(const TileCtx<R> &c, int i, int j)
+template <class S>
+SP_HD bool tile_owns(const TileCtx<S> &c, int i, int j)
 {
-	using G = Geo<R>;
+	using G = Geo<S>;
+	constexpr int R = S::R;
+	if (R == 1) {
+		// sweeps tt with 0 <= i - x0 + tt < BI and 0 <= j - y0 + tt < BJ
+		const int di = i - c.x0, dj = j - c.y0;
+		const int lo = di < dj ? -di : -dj;
+		int hi = G::BI - di < G::BJ - dj ? G::BI - di : G::BJ - dj;
+		if (hi > c.bte) hi = c.bte;
+		return (lo > 0 ? lo : 0) < hi;
+	}
 	for (int tt = 0; tt < c.bte; tt++) {
 		int aa, bb;
 		if (R == 1) { aa = i - c.x0 + tt; bb = j - c.y0 + tt; }
@@ -222,15 +296,16 @@ SP_HD bool tile_owns(const TileCtx<R> &c, int i, int j)
 }
 
 // phase 3 (thread tid): write back the hole pixels this tile updated
-template <int R>
-SP_HD void tile_store(const TileCtx<R> &c, const GsInst &in, int tid)
+template <class S>
+SP_HD void tile_store(const TileCtx<S> &c, const GsInst &in, int tid)
 {
-	using G = Geo<R>;
+	using G = Geo<S>;
+	constexpr int R = S::R;
 	for (int e = tid; e < G::LW * G::LH; e += G::NT) {
 		const int row = e / G::LW, col = e - row * G::LW;
 		const int j = row + c.jlo, i = col + c.K - G::SH * j;
 		if (i < 0 || i >= c.W || j < 0 || j >= c.H || sp_known(c.C[e])) continue;
-		if (tile_owns<R>(c, i, j)) in.u[(size_t)j * c.W + i] = c.U[e];
+		if (tile_owns<S>(c, i, j)) in.u[(size_t)j * c.W + i] = c.U[e];
 	}
 }
 

@@ -81,6 +81,8 @@ extern "C" void sp_params_default(sp_params *p)
 	p->cg_rel = 0;
 	p->cg_chunk = 32;
 	p->cg_mg = 0;
+	const char *at = getenv("SIMPOIS_HIP_AUTOTUNE");
+	p->autotune = at && *at && strcmp(at, "0");
 }
 
 // ------------------------------------------------------------- device
@@ -226,6 +228,10 @@ struct LevelPlan {
 	int *d_first = nullptr;          // first work block of each member, + end
 	int *d_bcnt = nullptr;           // holes per work block
 	GsInst *d_insts = nullptr;
+	std::vector<GsInst> insts;       // host copy of d_insts
+	std::vector<int> tiled;          // members swept in tiles (too big for LDS)
+	int gs_var = 0;                  // tile shape variant (sp_gs.h)
+	bool gs_tuned = false;
 	std::vector<int> wf_off;
 	GsTile *d_tiles = nullptr;
 	long tile_pass_px = 0;           // for the profiler
@@ -315,6 +321,183 @@ static int op_radius(float tstep, float *teff)
 }
 
 static constexpr int EW_SPAN = 256 * 4;   // pixels per elementwise work block
+
+// Tiles of the level's tiled members for shape variant `var`, merged by
+// wavefront index (tiles of one wavefront are independent across members)
+static int gs_tiles(const Plan &pl, const LevelPlan &L, int var,
+		std::vector<GsTile> &tiles, std::vector<int> &wf_off, long *pass_px)
+{
+	std::vector<std::vector<GsTile>> wf;
+	long px = 0;
+	for (int m : L.tiled) {
+		const int w = L.sp[m].w, h = L.sp[m].h, T = L.nsweep[m];
+		std::vector<GsTile> t;
+		std::vector<int> o;
+		const int bt = build_tiles_var(pl.R, var, w, h, T, 1, t, o);
+		if (!bt) return set_err(SP_ERR_ARG, "no tile shape %d for radius %d", var, pl.R);
+		if (wf.size() + 1 < o.size()) wf.resize(o.size() - 1);
+		for (size_t z = 0; z + 1 < o.size(); z++)
+			for (int e = o[z]; e < o[z + 1]; e++) {
+				GsTile tl = t[e];
+				tl.inst = m;
+				wf[z].push_back(tl);
+			}
+		px += (long)((T + bt - 1) / bt) * w * h;
+	}
+	tiles.clear();
+	wf_off.assign(1, 0);
+	for (auto &v : wf) {
+		tiles.insert(tiles.end(), v.begin(), v.end());
+		wf_off.push_back((int)tiles.size());
+	}
+	if (pass_px) *pass_px = px;
+	return SP_OK;
+}
+
+static int gs_build(Plan &pl, LevelPlan &L, int var)
+{
+	std::vector<GsTile> tiles;
+	TRY(gs_tiles(pl, L, var, tiles, L.wf_off, &L.tile_pass_px));
+	TRY(pl.upload(&L.d_tiles, tiles));
+	L.gs_var = var;
+	return SP_OK;
+}
+
+// ---------------------------------------------------------- autotuner
+//
+// With sp_params.autotune set, the first sweep of a level whose members run
+// in tiles times every compiled tile shape of its radius (sp_gs.h) on a
+// scratch copy of the level's current data, keeps the fastest, and caches
+// the choice for the process under (radius, sweeps, member sizes).  Every
+// shape runs the same lexicographic schedule, so the choice never changes
+// the result, only the time.
+
+struct TuneRec {
+	std::vector<long> key;
+	int var;
+	std::vector<float> ms;   // per variant
+};
+static std::mutex g_tune_mu;
+static std::vector<TuneRec> g_tune;
+
+static std::vector<long> gs_key(const Plan &pl, const LevelPlan &L)
+{
+	std::vector<long> k = {pl.R};
+	for (int m : L.tiled) {
+		k.push_back(L.sp[m].w);
+		k.push_back(L.sp[m].h);
+		k.push_back(L.nsweep[m]);
+	}
+	return k;
+}
+
+static int gs_launch_all(const Plan &pl, int var, const GsTile *d_tiles,
+		const std::vector<int> &wf_off, const GsInst *d_insts, hipStream_t s)
+{
+	for (size_t k = 0; k + 1 < wf_off.size(); k++)
+		spk::launch_gs_tiles(pl.R, var, d_tiles + wf_off[k], wf_off[k + 1] - wf_off[k],
+				d_insts, s);
+	HIPCHK(hipGetLastError());
+	return SP_OK;
+}
+
+static int gs_tune(Plan &pl, LevelPlan &L, hipStream_t s)
+{
+	L.gs_tuned = true;
+	const std::vector<long> key = gs_key(pl, L);
+	{
+		std::lock_guard<std::mutex> lk(g_tune_mu);
+		for (const TuneRec &r : g_tune)
+			if (r.key == key) {
+				if (r.var != L.gs_var) TRY(gs_build(pl, L, r.var));
+				return SP_OK;
+			}
+	}
+	// scratch copy of the tiled members' current values
+	size_t need = 0;
+	for (int m : L.tiled) need += rup64((size_t)L.sp[m].w * L.sp[m].h);
+	float *scratch = nullptr;
+	TRY(pl.alloc(&scratch, need));
+	std::vector<GsInst> gi = L.insts;
+	size_t off = 0;
+	for (int m : L.tiled) {
+		gi[m].u = scratch + off;
+		off += rup64((size_t)L.sp[m].w * L.sp[m].h);
+	}
+	GsInst *d_gi = nullptr;
+	TRY(pl.upload(&d_gi, gi));
+	hipEvent_t a, b;
+	HIPCHK(hipEventCreate(&a));
+	HIPCHK(hipEventCreate(&b));
+	const int nv = sp_shape_count(pl.R);
+	std::vector<float> ms(nv, 0.0f);
+	int best = 0;
+	int rc = SP_OK;
+	for (int v = 0; v < nv && rc == SP_OK; v++) {
+		std::vector<GsTile> tiles;
+		std::vector<int> wo;
+		rc = gs_tiles(pl, L, v, tiles, wo, nullptr);
+		GsTile *d_t = nullptr;
+		if (rc == SP_OK) rc = pl.upload(&d_t, tiles);
+		float t = 1e30f;
+		for (int rep = 0; rep < 2 && rc == SP_OK; rep++) {   // the first warms up
+			for (int m : L.tiled)
+				if (hipMemcpyAsync(gi[m].u, L.sp[m].u,
+						(size_t)L.sp[m].w * L.sp[m].h * 4,
+						hipMemcpyDeviceToDevice, s) != hipSuccess)
+					rc = set_err(SP_ERR_HIP, "autotune copy failed");
+			if (rc != SP_OK) break;
+			(void)hipEventRecord(a, s);
+			rc = gs_launch_all(pl, v, d_t, wo, d_gi, s);
+			(void)hipEventRecord(b, s);
+			if (hipEventSynchronize(b) != hipSuccess)
+				rc = set_err(SP_ERR_HIP, "autotune run failed");
+			float e = 0;
+			(void)hipEventElapsedTime(&e, a, b);
+			if (rep) t = e;
+		}
+		ms[v] = t;
+		if (t < ms[best]) best = v;
+	}
+	(void)hipEventDestroy(a);
+	(void)hipEventDestroy(b);
+	TRY(rc);
+	if (best != L.gs_var) TRY(gs_build(pl, L, best));
+	std::lock_guard<std::mutex> lk(g_tune_mu);
+	g_tune.push_back({key, best, ms});
+	return SP_OK;
+}
+
+extern "C" int sp_autotune_report(char *buf, int cap)
+{
+	std::string out;
+	std::lock_guard<std::mutex> lk(g_tune_mu);
+	for (const TuneRec &r : g_tune) {
+		const int R = (int)r.key[0];
+		char line[256];
+		long px = 0;
+		for (size_t z = 1; z + 2 < r.key.size(); z += 3) px += r.key[z] * r.key[z + 1];
+		snprintf(line, sizeof line, "gs R=%d T=%ld planes=%d px=%ld choice=", R,
+				r.key.size() > 3 ? r.key[3] : 0L, (int)(r.key.size() - 1) / 3, px);
+		out += line;
+		for (int v = 0; v < (int)r.ms.size(); v++) {
+			sp_with_shape(R, v, [&](auto sh) {
+				using S = decltype(sh);
+				snprintf(line, sizeof line, "%s%dx%dx%d:%.3f", v ? "," : "",
+						S::BI, S::BJ, S::BT, r.ms[v]);
+			});
+			if (v == r.var) out += "*";
+			out += line;
+		}
+		out += "\n";
+	}
+	if (buf && cap > 0) {
+		const size_t n = std::min(out.size(), (size_t)cap - 1);
+		memcpy(buf, out.data(), n);
+		buf[n] = 0;
+	}
+	return (int)out.size() + 1;
+}
 
 static int plan_create(Plan &pl, const std::vector<PlaneIn> &planes,
 		const sp_params &P, bool has_f)
@@ -411,11 +594,9 @@ static int plan_create(Plan &pl, const std::vector<PlaneIn> &planes,
 			gi[m] = {L.sp[m].u, L.sp[m].c, L.sp[m].w, L.sp[m].h, L.nsweep[m],
 				pl.teff, nullptr};
 		TRY(pl.upload(&L.d_insts, gi));
+		L.insts = gi;
 		// Gauss-Seidel: members that fit in LDS run whole; others in tiles
 		std::vector<int> small;
-		std::vector<std::vector<GsTile>> wf;
-		const int bt = pl.R == 1 ? TileShape<1>::BT : pl.R == 2
-			? TileShape<2>::BT : TileShape<3>::BT;
 		for (int m = 0; m < nm; m++) {
 			const int w = L.sp[m].w, h = L.sp[m].h, T = L.nsweep[m];
 			if (T <= 0) continue;
@@ -429,29 +610,11 @@ static int plan_create(Plan &pl, const std::vector<PlaneIn> &planes,
 				L.small_lds = std::max(L.small_lds, spk::small_lds_bytes(w, h));
 				continue;
 			}
-			std::vector<GsTile> t;
-			std::vector<int> o;
-			if (pl.R == 1) build_tiles<1>(w, h, T, 1, t, o);
-			else if (pl.R == 2) build_tiles<2>(w, h, T, 1, t, o);
-			else build_tiles<3>(w, h, T, 1, t, o);
-			if (wf.size() + 1 < o.size()) wf.resize(o.size() - 1);
-			for (size_t z = 0; z + 1 < o.size(); z++)
-				for (int e = o[z]; e < o[z + 1]; e++) {
-					GsTile tl = t[e];
-					tl.inst = m;
-					wf[z].push_back(tl);
-				}
-			L.tile_pass_px += (long)((T + bt - 1) / bt) * w * h;
+			L.tiled.push_back(m);
 		}
 		L.nsmall = (int)small.size();
 		TRY(pl.upload(&L.d_small, small));
-		std::vector<GsTile> tiles;
-		L.wf_off.assign(1, 0);
-		for (auto &v : wf) {
-			tiles.insert(tiles.end(), v.begin(), v.end());
-			L.wf_off.push_back((int)tiles.size());
-		}
-		TRY(pl.upload(&L.d_tiles, tiles));
+		TRY(gs_build(pl, L, 0));
 	}
 	// parent (finer) and coarse descriptors aligned with each level's members
 	for (int k = 0; k < nlev; k++) {
@@ -483,6 +646,7 @@ static int plan_create(Plan &pl, const std::vector<PlaneIn> &planes,
 
 static int run_gs(Plan &pl, LevelPlan &L, hipStream_t s)
 {
+	if (pl.P.autotune && !L.gs_tuned && L.wf_off.size() > 1) TRY(gs_tune(pl, L, s));
 	if (L.nsmall)
 		spk::launch_gs_small(pl.R, L.d_insts, L.d_small, L.nsmall, L.small_lds,
 				L.small_threads, s);
@@ -490,9 +654,7 @@ static int run_gs(Plan &pl, LevelPlan &L, hipStream_t s)
 		// algorithmic bytes: per pass of BT sweeps, read u and the
 		// coefficient plane and write u once (12 B per pixel)
 		ProfScope pr(pl.R == 1 ? "gs_tile" : "gs_tile_r23", s);
-		for (size_t k = 0; k + 1 < L.wf_off.size(); k++)
-			spk::launch_gs_tiles(pl.R, L.d_tiles + L.wf_off[k],
-					L.wf_off[k + 1] - L.wf_off[k], L.d_insts, s);
+		TRY(gs_launch_all(pl, L.gs_var, L.d_tiles, L.wf_off, L.d_insts, s));
 		pr.add((long)L.wf_off.size() - 1, 12.0 * L.tile_pass_px);
 	}
 	HIPCHK(hipGetLastError());

@@ -46,10 +46,11 @@ void launch_finalize(const SpPlane *lv, float *const *outs,
 		const BlockWork *bw, int nblocks, hipStream_t s);
 
 // Gauss-Seidel
-int tile_lds_bytes(int R);
+// tiles of shape variant `var` of radius R (sp_gs.h); one launch per wavefront
+int tile_lds_bytes(int R, int var);
 int small_lds_bytes(int w, int h);
-void launch_gs_tiles(int R, const GsTile *tiles, int ntiles, const GsInst *insts,
-		hipStream_t s);
+void launch_gs_tiles(int R, int var, const GsTile *tiles, int ntiles,
+		const GsInst *insts, hipStream_t s);
 void launch_gs_small(int R, const GsInst *insts, const int *ids, int nids,
 		int lds_bytes, int threads, hipStream_t s);
 

@@ -225,44 +225,40 @@ void launch_finalize(const SpPlane *lv, float *const *outs,
 
 // ------------------------------------------------------------ GS kernels
 
-template <int R>
-__global__ __launch_bounds__(Geo<R>::NT) void k_gs_tile(const GsTile *tiles,
+template <class S>
+__global__ __launch_bounds__(Geo<S>::NT) void k_gs_tile(const GsTile *tiles,
 		const GsInst *insts)
 {
-	using G = Geo<R>;
+	using G = Geo<S>;
 	extern __shared__ float smem[];
 	const GsTile tl = tiles[blockIdx.x];
 	const GsInst in = insts[tl.inst];
 	if (in.active && !*in.active) return;
-	const TileCtx<R> c = tile_ctx<R>(tl, in, smem);
-	tile_load<R>(c, in, threadIdx.x);
+	const TileCtx<S> c = tile_ctx<S>(tl, in, smem);
+	tile_load<S>(c, in, threadIdx.x);
 	__syncthreads();
 	for (int sg = 0; sg < G::NSTEP; sg++) {
-		tile_step<R>(c, threadIdx.x, sg);
+		tile_step<S>(c, threadIdx.x, sg);
 		__syncthreads();
 	}
-	tile_store<R>(c, in, threadIdx.x);
+	tile_store<S>(c, in, threadIdx.x);
 }
 
-int tile_lds_bytes(int R)
+int tile_lds_bytes(int R, int var)
 {
-	switch (R) {
-	case 1: return Geo<1>::LDS_FLOATS * 4;
-	case 2: return Geo<2>::LDS_FLOATS * 4;
-	default: return Geo<3>::LDS_FLOATS * 4;
-	}
+	int b = 0;
+	sp_with_shape(R, var, [&](auto sh) { b = Geo<decltype(sh)>::LDS_FLOATS * 4; });
+	return b;
 }
 
-void launch_gs_tiles(int R, const GsTile *tiles, int ntiles, const GsInst *insts,
-		hipStream_t s)
+void launch_gs_tiles(int R, int var, const GsTile *tiles, int ntiles,
+		const GsInst *insts, hipStream_t s)
 {
 	if (!ntiles) return;
-	const int lds = tile_lds_bytes(R);
-	switch (R) {
-	case 1: k_gs_tile<1><<<ntiles, Geo<1>::NT, lds, s>>>(tiles, insts); break;
-	case 2: k_gs_tile<2><<<ntiles, Geo<2>::NT, lds, s>>>(tiles, insts); break;
-	default: k_gs_tile<3><<<ntiles, Geo<3>::NT, lds, s>>>(tiles, insts); break;
-	}
+	sp_with_shape(R, var, [&](auto sh) {
+		using S = decltype(sh);
+		k_gs_tile<S><<<ntiles, Geo<S>::NT, Geo<S>::LDS_FLOATS * 4, s>>>(tiles, insts);
+	});
 }
 
 // whole level in one block (levels that fit in LDS)

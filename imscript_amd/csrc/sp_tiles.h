@@ -7,11 +7,11 @@
 #include "sp_gs.h"
 
 // skewed tiles of one level, grouped by wavefront I'+J'+T'
-template <int R>
+template <class S>
 inline void build_tiles(int W, int H, int T, int pd, std::vector<GsTile> &out,
 		std::vector<int> &off)
 {
-	using S = TileShape<R>;
+	constexpr int R = S::R;
 	const int SH = R - 1;
 	long ispan, jspan;
 	if (R == 1) { ispan = (long)W + T - 1; jspan = (long)H + T - 1; }
@@ -53,3 +53,16 @@ inline void build_tiles(int W, int H, int T, int pd, std::vector<GsTile> &out,
 	}
 }
 
+
+// the same for variant v of radius R; returns the variant's sweeps per tile
+inline int build_tiles_var(int R, int v, int W, int H, int T, int pd,
+		std::vector<GsTile> &out, std::vector<int> &off)
+{
+	int bt = 0;
+	sp_with_shape(R, v, [&](auto sh) {
+		using S = decltype(sh);
+		build_tiles<S>(W, H, T, pd, out, off);
+		bt = S::BT;
+	});
+	return bt;
+}

@@ -82,6 +82,10 @@ typedef struct sp_params {
 	int cg_chunk;    /* CG iterations per captured graph (FP32 CG) */
 	int cg_mg;       /* 1: precondition the FP32 CG with a multigrid V-cycle
 	                    (Poisson only); 0: plain CG as the reference */
+	int autotune;    /* 1: time every compiled Gauss-Seidel tile shape on the
+	                    first sweep of each level size and keep the fastest
+	                    (cached per process; results do not depend on it);
+	                    0: built-in shapes.  Default from SIMPOIS_HIP_AUTOTUNE */
 } sp_params;
 
 /* one image of a batch: planar w*h*pd device buffers, f may be NULL */
@@ -156,6 +160,13 @@ int sp_profile_read(const char *name, double *ms, long *launches, double *bytes)
  * array `rel`. */
 int sp_residual_device(const float *d_u, const float *d_g, const float *d_f,
 		int w, int h, int pd, int bilap, double *rel, void *stream);
+
+/* Choices of the tile-shape autotuner so far in this process, one text
+ * line per tuned level size: "gs R=<radius> T=<sweeps> planes=<n>
+ * px=<pixels> choice=<BIxBJxBT:ms>,..." with the chosen shape marked '*'.
+ * Writes at most cap bytes (NUL-terminated) and returns the size needed.
+ * No reference counterpart. */
+int sp_autotune_report(char *buf, int cap);
 
 #ifdef __cplusplus
 }

@@ -9,8 +9,8 @@
 // Built with AddressSanitizer by the tests, so any LDS or image access out
 // of bounds aborts the run.
 //
-// usage: gs_emulate W H T tstep seed holefrac withf mode
-//   mode: tiles | small
+// usage: gs_emulate W H T tstep seed holefrac withf mode [variant]
+//   mode: tiles | small; variant: tile shape variant of the radius (sp_gs.h)
 // prints "OK <n>" (n = holes) or "MISMATCH ..." / "CONFLICT ..." and exits 1
 #include <cstdio>
 #include <cstdlib>
@@ -29,37 +29,37 @@ static double urand()
 	return (rng_state >> 8) * (1.0 / 16777216.0);
 }
 
-template <int R>
+template <class S>
 static int run_tiles(float *u, const float *c, int W, int H, int T, float t)
 {
-	using G = Geo<R>;
+	using G = Geo<S>;
 	std::vector<GsTile> tiles;
 	std::vector<int> off;
-	build_tiles<R>(W, H, T, 1, tiles, off);
+	build_tiles<S>(W, H, T, 1, tiles, off);
 	GsInst in{u, c, W, H, T, t, nullptr};
 	std::vector<int> owner((size_t)W * H, -1);
 	for (size_t w = 0; w + 1 < off.size(); w++) {
 		const int n = off[w + 1] - off[w];
 		std::vector<float *> lds(n);
-		std::vector<TileCtx<R>> ctx(n);
+		std::vector<TileCtx<S>> ctx(n);
 		for (int k = 0; k < n; k++) {
 			lds[k] = new float[G::LDS_FLOATS];
 			for (int e = 0; e < G::LDS_FLOATS; e++) lds[k][e] = -12345.0f;
-			ctx[k] = tile_ctx<R>(tiles[off[w] + k], in, lds[k]);
+			ctx[k] = tile_ctx<S>(tiles[off[w] + k], in, lds[k]);
 		}
 		for (int k = 0; k < n; k++)
-			for (int tid = 0; tid < G::NT; tid++) tile_load<R>(ctx[k], in, tid);
+			for (int tid = 0; tid < G::NT; tid++) tile_load<S>(ctx[k], in, tid);
 		for (int k = 0; k < n; k++)
 			for (int sg = 0; sg < G::NSTEP; sg++)
-				for (int tid = 0; tid < G::NT; tid++) tile_step<R>(ctx[k], tid, sg);
+				for (int tid = 0; tid < G::NT; tid++) tile_step<S>(ctx[k], tid, sg);
 		// write-write conflicts between tiles of one wavefront
 		for (int k = 0; k < n; k++) {
-			const TileCtx<R> &cc = ctx[k];
+			const TileCtx<S> &cc = ctx[k];
 			for (int e = 0; e < G::LW * G::LH; e++) {
 				int row = e / G::LW, col = e - row * G::LW;
 				int j = row + cc.jlo, i = col + cc.K - G::SH * j;
 				if (i < 0 || i >= W || j < 0 || j >= H || sp_known(cc.C[e])) continue;
-				if (!tile_owns<R>(cc, i, j)) continue;
+				if (!tile_owns<S>(cc, i, j)) continue;
 				size_t q = (size_t)j * W + i;
 				if (owner[q] == (int)w * 1000000 + k) continue;
 				if (owner[q] >= (int)w * 1000000) {
@@ -70,7 +70,7 @@ static int run_tiles(float *u, const float *c, int W, int H, int T, float t)
 			}
 		}
 		for (int k = 0; k < n; k++)
-			for (int tid = 0; tid < G::NT; tid++) tile_store<R>(ctx[k], in, tid);
+			for (int tid = 0; tid < G::NT; tid++) tile_store<S>(ctx[k], in, tid);
 		for (int k = 0; k < n; k++) delete[] lds[k];
 	}
 	return 0;
@@ -98,10 +98,12 @@ static int run_small(float *u, const float *c, int W, int H, int T, float t)
 
 int main(int argc, char **argv)
 {
-	if (argc != 9) {
-		fprintf(stderr, "usage: gs_emulate W H T tstep seed holefrac withf tiles|small\n");
+	if (argc != 9 && argc != 10) {
+		fprintf(stderr, "usage: gs_emulate W H T tstep seed holefrac withf "
+				"tiles|small [variant]\n");
 		return 2;
 	}
+	const int var = argc == 10 ? atoi(argv[9]) : 0;
 	const int W = atoi(argv[1]), H = atoi(argv[2]), T = atoi(argv[3]);
 	const float tstep = (float)atof(argv[4]);
 	rng_state = (unsigned)atoi(argv[5]) * 2654435761u + 7;
@@ -130,8 +132,11 @@ int main(int argc, char **argv)
 	int rc;
 	if (small) rc = R == 1 ? run_small<1>(u, c, W, H, T, teff)
 		: R == 2 ? run_small<2>(u, c, W, H, T, teff) : run_small<3>(u, c, W, H, T, teff);
-	else rc = R == 1 ? run_tiles<1>(u, c, W, H, T, teff)
-		: R == 2 ? run_tiles<2>(u, c, W, H, T, teff) : run_tiles<3>(u, c, W, H, T, teff);
+	else if (!sp_with_shape(R, var, [&](auto sh) {
+			rc = run_tiles<decltype(sh)>(u, c, W, H, T, teff); })) {
+		printf("NO SHAPE %d for radius %d\n", var, R);
+		return 2;
+	}
 	if (rc) return 1;
 	for (size_t q = 0; q < n; q++)
 		if (memcmp(&u[q], &uref[q], 4)) {

@@ -256,3 +256,23 @@ def test_multigrid_batch_of_uneven_planes():
             if torch.isnan(g[l]).all():
                 continue
             assert rel[l] <= 2e-6, (tuple(g.shape), rel)
+
+
+@pytest.mark.parametrize("tstep,niter", [(0.25, 40), (-0.08, 12), (-1000.05, 6)])
+def test_autotuned_tile_shapes_match_oracle(tstep, niter):
+    """The autotuner times every compiled tile shape on a scratch copy and
+    keeps one: the solve stays bit-identical to the oracle, and the choice
+    is reported."""
+    import torch
+    m = _imscript()
+    g, f = random_case(31, 2, 300, 420, 0.4, True)
+    print(f"autotune t={tstep} n={niter} 2x300x420", flush=True)
+    ref, _ = oracle_solve(g, f, tstep, niter, 99, 0)
+    dg, df = torch.from_numpy(g).cuda(), torch.from_numpy(f).cuda()
+    for rep in range(2):   # tuned on the first call, cached for the second
+        out, _ = m.solve_device(dg, df, tstep=tstep, niter=niter, autotune=1)
+        torch.cuda.synchronize()
+        assert bits_equal(out.cpu().numpy(), ref)
+    rep = m.autotune_report()
+    R = 1 if tstep > 0 else (3 if tstep < -1000 else 2)
+    assert f"gs R={R} T={niter}" in rep and "*" in rep

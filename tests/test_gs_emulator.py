@@ -60,3 +60,29 @@ def test_kernel_phases_match_raster_gauss_seidel(emulator, args):
                        text=True, timeout=600, env=env)
     assert p.returncode == 0, p.stdout + p.stderr
     assert p.stdout.startswith("OK")
+
+
+# every compiled tile shape (sp_gs.h SP_GS_SHAPES), sizes spanning several
+# tiles in each skewed direction and a partial last pass in time
+SHAPE_CASES = (
+    [f"150 140 40 0.25 {11 + v} 0.45 1 tiles {v}" for v in range(6)]
+    + [f"200 3 37 0.25 {21 + v} 0.5 0 tiles {v}" for v in range(6)]
+    + [f"100 90 20 -0.08 {31 + v} 0.4 1 tiles {v}" for v in range(4)]
+    + [f"97 83 9 -1000.02 {41 + v} 0.4 1 tiles {v}" for v in range(3)]
+)
+
+
+@pytest.mark.parametrize("args", SHAPE_CASES)
+def test_every_tile_shape_matches_raster_gauss_seidel(emulator, args):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    p = subprocess.run([emulator, *args.split()], capture_output=True,
+                       text=True, timeout=600, env=env)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert p.stdout.startswith("OK")
+
+
+def test_unknown_tile_shape_is_refused(emulator):
+    p = subprocess.run([emulator, *"40 30 5 0.25 1 0.4 0 tiles 99".split()],
+                       capture_output=True, text=True, timeout=60,
+                       env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"))
+    assert p.returncode == 2 and "NO SHAPE" in p.stdout
