@@ -483,10 +483,9 @@ extern "C" int sp_autotune_report(char *buf, int cap)
 		for (int v = 0; v < (int)r.ms.size(); v++) {
 			sp_with_shape(R, v, [&](auto sh) {
 				using S = decltype(sh);
-				snprintf(line, sizeof line, "%s%dx%dx%d:%.3f", v ? "," : "",
-						S::BI, S::BJ, S::BT, r.ms[v]);
+				snprintf(line, sizeof line, "%s%s%dx%dx%d:%.3f", v ? "," : "",
+						v == r.var ? "*" : "", S::BI, S::BJ, S::BT, r.ms[v]);
 			});
-			if (v == r.var) out += "*";
 			out += line;
 		}
 		out += "\n";

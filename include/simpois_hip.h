@@ -163,7 +163,7 @@ int sp_residual_device(const float *d_u, const float *d_g, const float *d_f,
 
 /* Choices of the tile-shape autotuner so far in this process, one text
  * line per tuned level size: "gs R=<radius> T=<sweeps> planes=<n>
- * px=<pixels> choice=<BIxBJxBT:ms>,..." with the chosen shape marked '*'.
+ * px=<pixels> choice=<BIxBJxBT:ms>,..." with the chosen shape prefixed '*'.
  * Writes at most cap bytes (NUL-terminated) and returns the size needed.
  * No reference counterpart. */
 int sp_autotune_report(char *buf, int cap);
