@@ -59,6 +59,7 @@ _SIGNATURES = {
                              ctypes.POINTER(ctypes.c_long),
                              ctypes.POINTER(ctypes.c_double)]),
     "sp_autotune_report": (_I, [ctypes.c_char_p, _I]),
+    "sp_stream_copy": (_I, [_P, _P, ctypes.c_size_t, _P]),
 }
 
 

@@ -171,3 +171,16 @@ def autotune_report():
     buf = ctypes.create_string_buffer(n)
     lib.sp_autotune_report(buf, n)
     return buf.value.decode()
+
+
+def stream_copy(dst, src, stream=None):
+    """dst[:] = src with the library's streaming copy kernel (device float32
+    tensors of equal size, a multiple of 4 elements)."""
+    import torch
+    if (dst.dtype != torch.float32 or src.dtype != torch.float32
+            or dst.numel() != src.numel() or not dst.is_contiguous()
+            or not src.is_contiguous()):
+        raise TypeError("dst and src must be contiguous float32 tensors of one size")
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib.sp_stream_copy(dst.data_ptr(), src.data_ptr(), src.numel(),
+                             ctypes.c_void_p(s.cuda_stream)))

@@ -40,9 +40,9 @@ struct MgLevel {
 };
 
 // levels 1..MG_FUSE_MAX run the fused tile kernels: their coefficients are
-// integers small enough to pack (diagonal <= 4^(l+1) <= 511, couplings
-// <= 2^l <= 15)
-constexpr int MG_FUSE_MAX = 3;
+// integers small enough to pack in 32 bits (diagonal <= 4^(l+1) <= 1024 in
+// 11 bits, couplings <= 2^l <= 16 in 5 bits each)
+constexpr int MG_FUSE_MAX = 4;
 
 // work lists of one multigrid level (device arrays indexed by batch plane)
 struct MgLevelWork {

@@ -468,8 +468,8 @@ __global__ __launch_bounds__(NT) void k_mg_reduce(const Cg32Plane *pl, Cg32Scal 
 // bit 0 unknown, bits 1-4 the E, S, W, N neighbour is an in-image unknown,
 // bits 5-7 the number of in-image neighbours (the diagonal, 2..4).
 // Levels 1..MG_FUSE_MAX pack their integer Galerkin coefficients into 32
-// bits: the diagonal in bits 0-8, the E, S, W, N couplings in 4 bits each
-// from bit 9 (a cell is active when its diagonal is not 0).
+// bits: the diagonal in bits 0-10, the E, S, W, N couplings in 5 bits each
+// from bit 11 (a cell is active when its diagonal is not 0).
 // The kernels below run on tiles of TX x TY cells (even origin) with a halo
 // of H = 2*nu: every red-black half sweep shrinks the region where z is
 // exact by one cell, and the first half sweep from z = 0 is exact
@@ -525,7 +525,7 @@ __global__ __launch_bounds__(NT) void k_mg_gcode(const BlockWork *bw,
 		const unsigned e = (unsigned)L.wE[q], so = (unsigned)L.wS[q];
 		const unsigned we = i > 0 ? (unsigned)L.wE[q - 1] : 0u;
 		const unsigned no = j > 0 ? (unsigned)L.wS[q - L.w] : 0u;
-		L.gcode[q] = d | e << 9 | so << 13 | we << 17 | no << 21;
+		L.gcode[q] = d | e << 11 | so << 16 | we << 21 | no << 26;
 	}
 }
 
@@ -591,23 +591,23 @@ struct LevG {
 		return k;
 	}
 	static __device__ __forceinline__ float mask(unsigned k)
-	{ return (k & 511u) ? 1.0f : 0.0f; }
-	static __device__ __forceinline__ float diag(unsigned k) { return (float)(k & 511u); }
+	{ return (k & 2047u) ? 1.0f : 0.0f; }
+	static __device__ __forceinline__ float diag(unsigned k) { return (float)(k & 2047u); }
 	// 1/diagonal, 0 for an inactive cell (diagonal 0).  The hardware
 	// reciprocal (1 ulp) is the same in the pre- and post-smoother, so the
 	// V-cycle stays symmetric.
 	static __device__ __forceinline__ float rdiag(unsigned k)
 	{
-		const unsigned d = k & 511u;
+		const unsigned d = k & 2047u;
 		return d ? __builtin_amdgcn_rcpf((float)d) : 0.0f;
 	}
 	static __device__ __forceinline__ float gather(unsigned k, float s,
 			const float *z, int id)
 	{
-		s += (float)((k >> 9) & 15u) * z[id + 1];
-		s += (float)((k >> 13) & 15u) * z[id + ZS];
-		s += (float)((k >> 17) & 15u) * z[id - 1];
-		s += (float)((k >> 21) & 15u) * z[id - ZS];
+		s += (float)((k >> 11) & 31u) * z[id + 1];
+		s += (float)((k >> 16) & 31u) * z[id + ZS];
+		s += (float)((k >> 21) & 31u) * z[id - 1];
+		s += (float)(k >> 26) * z[id - ZS];
 		return s;
 	}
 };
@@ -971,9 +971,9 @@ __global__ __launch_bounds__(MG_TAIL_T) void k_mg_tail(const MgLevel *lvall,
 }
 
 static const char *const g_down_name[MG_FUSE_MAX + 1] = {
-	"mg_down0", "mg_down1", "mg_down2", "mg_down3"};
+	"mg_down0", "mg_down1", "mg_down2", "mg_down3", "mg_down4"};
 static const char *const g_up_name[MG_FUSE_MAX + 1] = {
-	"mg_up0", "mg_up1", "mg_up2", "mg_up3"};
+	"mg_up0", "mg_up1", "mg_up2", "mg_up3", "mg_up4"};
 
 static void vcycle(const Cg32Batch &B, hipStream_t s)
 {

@@ -302,7 +302,6 @@ template <class S>
 SP_HD void tile_store(const TileCtx<S> &c, const GsInst &in, int tid)
 {
 	using G = Geo<S>;
-	constexpr int R = S::R;
 	for (int e = tid; e < G::LW * G::LH; e += G::NT) {
 		const int row = e / G::LW, col = e - row * G::LW;
 		const int j = row + c.jlo, i = col + c.K - G::SH * j;

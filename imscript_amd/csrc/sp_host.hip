@@ -1225,6 +1225,19 @@ extern "C" int sp_adjoint_batch_device(int n, const sp_adjoint_image *imgs,
 	return SP_OK;
 }
 
+// ------------------------------------------------------------ stream copy
+
+extern "C" int sp_stream_copy(float *d_dst, const float *d_src, size_t n, void *stream)
+{
+	if (!d_dst || !d_src || (n & 3) || ((uintptr_t)d_dst & 15) || ((uintptr_t)d_src & 15))
+		return set_err(SP_ERR_ARG, "stream copy needs 16-byte aligned buffers "
+				"and a multiple of 4 floats");
+	TRY(ensure_device());
+	spk::launch_stream_copy(d_dst, d_src, n / 4, (hipStream_t)stream);
+	HIPCHK(hipGetLastError());
+	return SP_OK;
+}
+
 // ------------------------------------------------------------ residual
 
 extern "C" int sp_residual_device(const float *d_u, const float *d_g,

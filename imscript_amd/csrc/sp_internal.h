@@ -46,6 +46,9 @@ void launch_finalize(const SpPlane *lv, float *const *outs,
 		const BlockWork *bw, int nblocks, hipStream_t s);
 
 // Gauss-Seidel
+// dst[0..4*n4) = src[0..4*n4), 16-byte aligned
+void launch_stream_copy(float *dst, const float *src, size_t n4, hipStream_t s);
+
 // tiles of shape variant `var` of radius R (sp_gs.h); one launch per wavefront
 int tile_lds_bytes(int R, int var);
 int small_lds_bytes(int w, int h);

@@ -568,6 +568,37 @@ void launch_residual(const SpPlane *lv, const BlockWork *bw, int nblocks,
 	if (nblocks) k_residual<<<nblocks, EW_NT, 0, s>>>(lv, bw, bilap, part);
 }
 
+// plain streaming copy: each thread moves 4 x 16 bytes (all loads issued
+// before the stores), one pass over the buffer; the measured bandwidth
+// ceiling of the device and the calibration of the HBM counters
+constexpr int SC_T = 256, SC_U = 4;
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(SC_T) void k_stream_copy(v4f *__restrict__ dst,
+		const v4f *__restrict__ src, size_t n4)
+{
+	const size_t q0 = (size_t)blockIdx.x * (SC_T * SC_U) + threadIdx.x;
+	v4f v[SC_U];
+#pragma unroll
+	for (int u = 0; u < SC_U; u++) {
+		const size_t q = q0 + (size_t)u * SC_T;
+		if (q < n4) v[u] = __builtin_nontemporal_load(src + q);
+	}
+#pragma unroll
+	for (int u = 0; u < SC_U; u++) {
+		const size_t q = q0 + (size_t)u * SC_T;
+		if (q < n4) __builtin_nontemporal_store(v[u], dst + q);
+	}
+}
+
+void launch_stream_copy(float *dst, const float *src, size_t n4, hipStream_t s)
+{
+	const size_t blocks = (n4 + SC_T * SC_U - 1) / (SC_T * SC_U);
+	if (blocks)
+		k_stream_copy<<<(unsigned)blocks, SC_T, 0, s>>>((v4f *)dst,
+				(const v4f *)src, n4);
+}
+
 void launch_residual_reduce(const double *part, const int *plane_first,
 		int nplanes, double *out, hipStream_t s)
 {

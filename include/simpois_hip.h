@@ -161,6 +161,13 @@ int sp_profile_read(const char *name, double *ms, long *launches, double *bytes)
 int sp_residual_device(const float *d_u, const float *d_g, const float *d_f,
 		int w, int h, int pd, int bilap, double *rel, void *stream);
 
+/* dst[0..n) = src[0..n) on the device with a plain streaming kernel
+ * (k_stream_copy): the bandwidth ceiling the solver kernels are compared
+ * with, and the known-size reference of the HBM traffic counters.  n a
+ * multiple of 4, both buffers 16-byte aligned; asynchronous on `stream`.
+ * No reference counterpart. */
+int sp_stream_copy(float *d_dst, const float *d_src, size_t n, void *stream);
+
 /* Choices of the tile-shape autotuner so far in this process, one text
  * line per tuned level size: "gs R=<radius> T=<sweeps> planes=<n>
  * px=<pixels> choice=<BIxBJxBT:ms>,..." with the chosen shape prefixed '*'.
