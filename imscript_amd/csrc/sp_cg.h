@@ -11,6 +11,9 @@ struct Cg32Plane {
 	                   // would otherwise cost ~1e-5 of relative residual
 	float *r, *p, *q;
 	float *z;          // preconditioned residual (== r without multigrid)
+	unsigned char *code;   // per pixel: bit 0 unknown, bits 1-4 the E, S, W,
+	                       // N neighbour is an in-image unknown, bits 5-7 the
+	                       // number of in-image neighbours (set by cg32_setup)
 	const int *cnt;    // number of unknowns (device)
 	int w, h, bilap;
 	int blk0, nblk;    // this plane's work blocks / partial-sum slots

@@ -96,7 +96,8 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 			P.x[q] = 0; P.r[q] = 0; P.p[q] = 0; P.q[q] = 0;
 			continue;
 		}
-		const int i = (int)(q % P.w), j = (int)(q / P.w);
+		const unsigned jj = (unsigned)q / (unsigned)P.w;
+		const int i = (int)((unsigned)q - jj * (unsigned)P.w), j = (int)jj;
 		const float *g = P.g, *u = P.u;
 		const float a0 = op32(P.bilap, i, j, P.w, P.h, [&](size_t k) {
 			float v = g[k]; return isnan(v) ? 0.0f : v; });
@@ -122,10 +123,19 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 static __device__ void fold(const double *part, const Cg32Plane &P, int stride,
 		int off, double *red, double *out)
 {
-	double v = 0;
-	for (int k = P.blk0 + threadIdx.x; k < P.blk0 + P.nblk; k += NT)
-		v += part[stride * k + off];
-	const double s = bsum<NT>(v, red);
+	// eight loads in flight per thread; slot k of the plane always lands in
+	// accumulator ((k / NT) % 8) of thread k % NT, so the order is fixed
+	double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	const double *pp = part + (size_t)stride * P.blk0 + off;
+	int k = threadIdx.x;
+	for (; k + 7 * NT < P.nblk; k += 8 * NT)
+#pragma unroll
+		for (int u = 0; u < 8; u++) v[u] += pp[(size_t)stride * (k + u * NT)];
+#pragma unroll
+	for (int u = 0; u < 7; u++)
+		if (k + u * NT < P.nblk) v[u] += pp[(size_t)stride * (k + u * NT)];
+	const double t = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+	const double s = bsum<NT>(t, red);
 	if (threadIdx.x == 0) *out = s;
 }
 
@@ -162,15 +172,38 @@ __global__ __launch_bounds__(NT) void k32_spmv(const BlockWork *bw,
 	const Cg32Plane P = pl[b.plane];
 	const float *p = P.p;
 	double acc = 0;
-	for (int k_ = 0; k_ < PPT; k_++) {
-		const long qq = pix(b, k_, (size_t)P.w * P.h);
-		if (qq < 0) break;
-		const size_t q = (size_t)qq;
-		if (!isnan(P.g[q])) continue;
-		const int i = (int)(q % P.w), j = (int)(q / P.w);
-		const float y = op32(P.bilap, i, j, P.w, P.h, [&](size_t k) { return p[k]; });
-		P.q[q] = y;
-		acc += (double)y * p[q];
+	const unsigned n = (unsigned)P.w * (unsigned)P.h, w = (unsigned)P.w;
+	if (!P.bilap) {
+		// -Laplacian with clamped reads: a clamped neighbour is the pixel
+		// itself, p is 0 at known pixels, so A p = d p - (sum of the
+		// in-image unknown neighbours), d = in-image neighbours; the code
+		// byte gives both, no coordinates needed
+		for (int k_ = 0; k_ < PPT; k_++) {
+			const unsigned q = (unsigned)b.start + k_ * NT + threadIdx.x;
+			if (q >= n) break;
+			const unsigned c = P.code[q];
+			if (!(c & 1)) continue;
+			const float pc = p[q];
+			float nb = 0.0f;
+			if (c & 2) nb += p[q + 1];
+			if (c & 4) nb += p[q + w];
+			if (c & 8) nb += p[q - 1];
+			if (c & 16) nb += p[q - w];
+			const float y = (float)(c >> 5) * pc - nb;
+			P.q[q] = y;
+			acc += (double)y * pc;
+		}
+	} else {
+		for (int k_ = 0; k_ < PPT; k_++) {
+			const unsigned q = (unsigned)b.start + k_ * NT + threadIdx.x;
+			if (q >= n) break;
+			if (!(P.code[q] & 1)) continue;
+			const unsigned j = q / w, i = q - j * w;
+			const float y = op32(1, (int)i, (int)j, P.w, P.h,
+					[&](size_t k) { return p[k]; });
+			P.q[q] = y;
+			acc += (double)y * p[q];
+		}
 	}
 	const double s = bsum<NT>(acc, red);
 	if (threadIdx.x == 0) part[2 * blockIdx.x] = s;
@@ -213,7 +246,7 @@ __global__ __launch_bounds__(NT) void k32_update(const BlockWork *bw,
 		const long qq = pix(b, k_, (size_t)P.w * P.h);
 		if (qq < 0) break;
 		const size_t q = (size_t)qq;
-		if (!isnan(P.g[q])) continue;
+		if (!(P.code[q] & 1)) continue;
 		P.x[q] = P.x[q] + ad * (double)P.p[q];
 		const float r = P.r[q] - a * P.q[q];
 		P.r[q] = r;
@@ -260,7 +293,7 @@ __global__ __launch_bounds__(NT) void k32_dir(const BlockWork *bw,
 		const long qq = pix(b, k_, (size_t)P.w * P.h);
 		if (qq < 0) break;
 		const size_t q = (size_t)qq;
-		if (!isnan(P.g[q])) continue;
+		if (!(P.code[q] & 1)) continue;
 		P.p[q] = P.z[q] + be * P.p[q];
 	}
 }
@@ -274,7 +307,7 @@ __global__ __launch_bounds__(NT) void k32_finish(const BlockWork *bw,
 		const long qq = pix(b, k_, (size_t)P.w * P.h);
 		if (qq < 0) break;
 		const size_t q = (size_t)qq;
-		if (isnan(P.g[q])) P.u[q] = (float)P.x[q];
+		if (P.code[q] & 1) P.u[q] = (float)P.x[q];
 	}
 }
 
@@ -426,42 +459,6 @@ __global__ __launch_bounds__(NT) void k_mg_prolong(const BlockWork *bw,
 	}
 }
 
-// partial r.z over the unknowns
-__global__ __launch_bounds__(NT) void k_mg_dot(const BlockWork *bw,
-		const Cg32Plane *pl, const Cg32Scal *sc, double *part)
-{
-	__shared__ double red[NT / 64];
-	const BlockWork b = bw[blockIdx.x];
-	if (!sc[b.plane].active) return;
-	const Cg32Plane P = pl[b.plane];
-	double acc = 0;
-	for (int k_ = 0; k_ < PPT; k_++) {
-		const long qq = pix(b, k_, (size_t)P.w * P.h);
-		if (qq < 0) break;
-		const size_t q = (size_t)qq;
-		if (!isnan(P.g[q])) continue;
-		acc += (double)P.r[q] * P.z[q];
-	}
-	const double s = bsum<NT>(acc, red);
-	if (threadIdx.x == 0) part[2 * blockIdx.x] = s;
-}
-
-// rho_new = r.z; beta = rho_new / rho (iterations) or p = z (setup)
-__global__ __launch_bounds__(NT) void k_mg_reduce(const Cg32Plane *pl, Cg32Scal *sc,
-		const double *part, int setup)
-{
-	__shared__ double red[NT / 64];
-	__shared__ double rz;
-	if (!sc[blockIdx.x].active) return;
-	fold(part, pl[blockIdx.x], 2, 0, red, &rz);
-	if (threadIdx.x == 0) {
-		Cg32Scal &s = sc[blockIdx.x];
-		s.beta = setup ? 0.0 : rz / s.rz;
-		s.rz = rz;
-	}
-}
-
-
 // ------------------------------------------------------ fused V-cycle ends
 //
 // Level 0 is the masked -Laplacian itself.  A byte per pixel encodes it:
@@ -487,26 +484,27 @@ constexpr int MG_RG = MG_T / PAIRS_X, MG_K = ROWPAIRS / MG_RG;
 static_assert(PAIRS_X * MG_RG == MG_T && MG_RG * MG_K == ROWPAIRS,
 		"multigrid tile shape must divide the thread layout");
 
-__global__ __launch_bounds__(NT) void k_mg_code(const BlockWork *bw,
-		const MgLevel *lv)
+// level-0 code of every pixel of a CG plane (layout at Cg32Plane::code)
+__global__ __launch_bounds__(NT) void k32_code(const BlockWork *bw,
+		const Cg32Plane *pl)
 {
 	const BlockWork b = bw[blockIdx.x];
-	const MgLevel L = lv[b.plane];
+	const Cg32Plane P = pl[b.plane];
+	const unsigned n = (unsigned)P.w * (unsigned)P.h, w = (unsigned)P.w;
 	for (int k_ = 0; k_ < PPT; k_++) {
-		const long qq = pix(b, k_, (size_t)L.w * L.h);
-		if (qq < 0) break;
-		const size_t q = (size_t)qq;
-		const int i = (int)(q % L.w), j = (int)(q / L.w);
+		const unsigned q = (unsigned)b.start + k_ * NT + threadIdx.x;
+		if (q >= n) break;
+		const unsigned j = q / w, i = q - j * w;
 		unsigned c = 0;
-		if (isnan(L.g[q])) {
+		if (isnan(P.g[q])) {
 			c = 1;
-			if (i + 1 < L.w && isnan(L.g[q + 1])) c |= 2;
-			if (j + 1 < L.h && isnan(L.g[q + L.w])) c |= 4;
-			if (i > 0 && isnan(L.g[q - 1])) c |= 8;
-			if (j > 0 && isnan(L.g[q - L.w])) c |= 16;
-			c |= (unsigned)((i > 0) + (i + 1 < L.w) + (j > 0) + (j + 1 < L.h)) << 5;
+			if (i + 1 < w && isnan(P.g[q + 1])) c |= 2;
+			if (j + 1 < (unsigned)P.h && isnan(P.g[q + w])) c |= 4;
+			if (i > 0 && isnan(P.g[q - 1])) c |= 8;
+			if (j > 0 && isnan(P.g[q - w])) c |= 16;
+			c |= ((i > 0) + (i + 1 < w) + (j > 0) + (j + 1 < (unsigned)P.h)) << 5;
 		}
-		L.code[q] = (unsigned char)c;
+		P.code[q] = (unsigned char)c;
 	}
 }
 
@@ -1043,7 +1041,6 @@ static void precondition(const Cg32Batch &B, int setup, hipStream_t s)
 
 void mg_build(const Cg32Batch &B, hipStream_t s)
 {
-	k_mg_code<<<B.nblk, NT, 0, s>>>(B.bw, B.mg[0].lv);
 	for (int l = 0; l + 1 < B.nlev; l++) {
 		const MgLevelWork &W = B.mg[l];
 		if (W.nbuild_c)
@@ -1059,6 +1056,7 @@ void mg_build(const Cg32Batch &B, hipStream_t s)
 
 void cg32_setup(const Cg32Batch &B, hipStream_t s)
 {
+	k32_code<<<B.nblk, NT, 0, s>>>(B.bw, B.planes);
 	k32_setup<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.part);
 	k32_reduce0<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive);
 	if (B.nlev) {

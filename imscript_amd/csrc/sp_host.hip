@@ -271,6 +271,7 @@ struct Plan {
 	// FP32 batched CG scratch: r p q (float) and x (double) per pixel
 	float *cg32_vec = nullptr;
 	double *cg32_x = nullptr;
+	unsigned char *cg32_code = nullptr;
 	size_t cg32_cap = 0;
 	std::vector<std::vector<CgTrace>> traces;   // [plane][level]
 	std::vector<int> cg_iters_finest;
@@ -765,11 +766,6 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 			cells += rup64((size_t)shp[z][l].first * shp[z][l].second);
 	float *pool = nullptr;
 	TRY(pl.alloc(&pool, 5 * std::max<size_t>(cells, 1)));
-	size_t px0 = 0;
-	for (int z = 0; z < np; z++) px0 += rup64((size_t)cp[z].w * cp[z].h);
-	unsigned char *codes = nullptr;
-	TRY(pl.alloc(&codes, px0));
-	size_t coff = 0;
 	// packed coefficients of the levels that may run fused
 	size_t gcells = 0;
 	for (int z = 0; z < np; z++)
@@ -790,8 +786,7 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 				L.g = cp[z].g;
 				L.z = cp[z].z;
 				L.b = cp[z].r;
-				L.code = codes + coff;
-				coff += rup64((size_t)L.w * L.h);
+				L.code = cp[z].code;
 				cp[z].t0 = (int)tiles0.size();
 				for (int y0 = 0; y0 < L.h; y0 += MG_TY)
 					for (int x0 = 0; x0 < L.w; x0 += MG_TX)
@@ -885,6 +880,7 @@ static int cg32_build(Plan &pl, int k, int bilap, float eps)
 	if (need > pl.cg32_cap) {
 		TRY(pl.alloc(&pl.cg32_vec, need * 4));
 		TRY(pl.alloc(&pl.cg32_x, need));
+		TRY(pl.alloc(&pl.cg32_code, need));
 		pl.cg32_cap = need;
 	}
 	std::vector<Cg32Plane> cp;
@@ -899,6 +895,7 @@ static int cg32_build(Plan &pl, int k, int bilap, float eps)
 		const size_t n = (size_t)S.w * S.h, na = rup64(n);
 		Cg32Plane c{};
 		c.g = S.g; c.f = S.f; c.u = S.u; c.cnt = S.cnt;
+		c.code = pl.cg32_code + offx;
 		c.x = pl.cg32_x + offx; offx += na;
 		c.r = pl.cg32_vec + off; off += na;
 		c.p = pl.cg32_vec + off; off += na;
