@@ -35,6 +35,16 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBPS = 8000.0
 
+# HBM bytes per launch of the roofline kernels at the 4096x4096x3 workload,
+# from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over
+# `bench.py --steps 1 --warmup 0 --no-cpu`, each scaled by what the same
+# counter reads for k_stream_copy moving 1 GiB each way in that run (FETCH_SIZE
+# x2.000, WRITE_SIZE x1.000).  Summary and commands: profiles/hbm_traffic_*.md.
+TRAFFIC_PMC = {
+    "mg_up0": 910.1e6,
+    "mg_down0": 640.9e6,
+}
+
 
 def sinusoid_sum(torch, rng, n, dev, k=8):
     y = torch.arange(n, device=dev, dtype=torch.float32).view(n, 1)
@@ -638,7 +648,9 @@ def main():
                        "k32_spmv": "k32_spmv"}[dom],
             "bound": "hbm", "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
+            "frac": round(achieved / PEAK_HBM_GBPS, 4),
+            "traffic": (TRAFFIC_PMC.get(dom) if a.size == 4096 and a.channels == 3
+                        else None),
             "algorithmic_bytes_per_launch": by / max(n, 1),
             "avg_launch_us": round(ms * 1e3 / max(n, 1), 2), "launches": n}
         result["kernels"] = rep
@@ -674,6 +686,17 @@ def main():
             "sweep_frac_effective": round(eff / PEAK_HBM_GBPS, 4)}
         result["autotune"] = tune
     result["stream_copy"] = stream_copy_ceiling(torch, sp)
+    # the drop-in boundary takes host buffers: one solve through it, copies
+    # to and from the device included (reported beside `value`, never as it)
+    gh, fh = g.cpu().numpy(), f.cpu().numpy()
+    hp = {k: v for k, v in params.items()}
+    sp.simpois(gh, fh, **hp)
+    t0 = time.perf_counter()
+    sp.simpois(gh, fh, **hp)
+    th = time.perf_counter() - t0
+    result["host_buffers"] = {"Mpix_s": round(mpix / th, 3), "ms": round(th * 1e3, 2),
+                              "path": "sp_solve_host (the drop-in's host-buffer entry), "
+                                      "H2D and D2H copies included"}
     if not a.no_cpu:
         result["cpu_baseline"] = cpu_baseline(g, f, a.cg_rel, a.cpu_side, params)
         result["config"]["fp64_check"] = fp64_check(torch, sp, params, a.seed)

@@ -21,6 +21,19 @@ namespace spk {
 
 constexpr int NT = 256;
 constexpr int PPT = 4;
+static_assert(NT * PPT == 1024, "work blocks span 1024 pixels");
+
+// Vector kernels below give each thread the 4 consecutive pixels
+// start + 4 tid .. + 3 of its work block (plane vectors start 256-byte
+// aligned).  Known pixels hold x = r = p = q = z = 0 throughout, so a group
+// with at least one unknown is updated whole: the known entries stay 0.
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+static __device__ __forceinline__ unsigned code4(const unsigned char *code, unsigned q4)
+{
+	return *(const unsigned *)(code + q4);
+}
 
 static __device__ __forceinline__ int clampi(int v, int lo, int hi)
 {
@@ -178,20 +191,53 @@ __global__ __launch_bounds__(NT) void k32_spmv(const BlockWork *bw,
 		// itself, p is 0 at known pixels, so A p = d p - (sum of the
 		// in-image unknown neighbours), d = in-image neighbours; the code
 		// byte gives both, no coordinates needed
-		for (int k_ = 0; k_ < PPT; k_++) {
-			const unsigned q = (unsigned)b.start + k_ * NT + threadIdx.x;
-			if (q >= n) break;
-			const unsigned c = P.code[q];
-			if (!(c & 1)) continue;
-			const float pc = p[q];
-			float nb = 0.0f;
-			if (c & 2) nb += p[q + 1];
-			if (c & 4) nb += p[q + w];
-			if (c & 8) nb += p[q - 1];
-			if (c & 16) nb += p[q - w];
-			const float y = (float)(c >> 5) * pc - nb;
-			P.q[q] = y;
-			acc += (double)y * pc;
+		const unsigned q4 = (unsigned)b.start + 4 * threadIdx.x;
+		if (q4 + 3 < n) {
+			const unsigned cc = code4(P.code, q4);
+			if (cc & 0x01010101u) {
+				const f4 pc = *(const f4 *)(p + q4);
+				f4 pn, ps;
+				if (!(w & 3) && q4 >= w) pn = *(const f4 *)(p + q4 - w);
+				else
+#pragma unroll
+					for (int k = 0; k < 4; k++)
+						pn[k] = (cc >> (8 * k)) & 16 ? p[q4 + k - w] : 0.0f;
+				if (!(w & 3) && q4 + w + 3 < n) ps = *(const f4 *)(p + q4 + w);
+				else
+#pragma unroll
+					for (int k = 0; k < 4; k++)
+						ps[k] = (cc >> (8 * k)) & 4 ? p[q4 + k + w] : 0.0f;
+				const float pe = cc & (2u << 24) ? p[q4 + 4] : 0.0f;
+				const float pw = cc & 8u ? p[q4 - 1] : 0.0f;
+				f4 y;
+#pragma unroll
+				for (int k = 0; k < 4; k++) {
+					const unsigned c = (cc >> (8 * k)) & 255u;
+					const float e = k < 3 ? pc[k + 1] : pe, ww = k > 0 ? pc[k - 1] : pw;
+					float nb = 0.0f;
+					if (c & 2) nb += e;
+					if (c & 4) nb += ps[k];
+					if (c & 8) nb += ww;
+					if (c & 16) nb += pn[k];
+					y[k] = (float)(c >> 5) * pc[k] - nb;
+					acc += (double)y[k] * pc[k];
+				}
+				*(f4 *)(P.q + q4) = y;
+			}
+		} else {
+			for (unsigned q = q4; q < n; q++) {
+				const unsigned c = P.code[q];
+				if (!(c & 1)) continue;
+				const float pc = p[q];
+				float nb = 0.0f;
+				if (c & 2) nb += p[q + 1];
+				if (c & 4) nb += p[q + w];
+				if (c & 8) nb += p[q - 1];
+				if (c & 16) nb += p[q - w];
+				const float y = (float)(c >> 5) * pc - nb;
+				P.q[q] = y;
+				acc += (double)y * pc;
+			}
 		}
 	} else {
 		for (int k_ = 0; k_ < PPT; k_++) {
@@ -242,15 +288,30 @@ __global__ __launch_bounds__(NT) void k32_update(const BlockWork *bw,
 	const double ad = s.alpha;
 	const float a = (float)ad;
 	double acc = 0;
-	for (int k_ = 0; k_ < PPT; k_++) {
-		const long qq = pix(b, k_, (size_t)P.w * P.h);
-		if (qq < 0) break;
-		const size_t q = (size_t)qq;
-		if (!(P.code[q] & 1)) continue;
-		P.x[q] = P.x[q] + ad * (double)P.p[q];
-		const float r = P.r[q] - a * P.q[q];
-		P.r[q] = r;
-		acc += (double)r * r;
+	const unsigned n = (unsigned)P.w * (unsigned)P.h;
+	const unsigned q4 = (unsigned)b.start + 4 * threadIdx.x;
+	if (q4 + 3 < n) {
+		if (code4(P.code, q4) & 0x01010101u) {
+			d4 x = *(const d4 *)(P.x + q4);
+			const f4 pp = *(const f4 *)(P.p + q4), qq = *(const f4 *)(P.q + q4);
+			f4 r = *(const f4 *)(P.r + q4);
+#pragma unroll
+			for (int k = 0; k < 4; k++) {
+				x[k] = x[k] + ad * (double)pp[k];
+				r[k] = r[k] - a * qq[k];
+				acc += (double)r[k] * r[k];
+			}
+			*(d4 *)(P.x + q4) = x;
+			*(f4 *)(P.r + q4) = r;
+		}
+	} else {
+		for (unsigned q = q4; q < n; q++) {
+			if (!(P.code[q] & 1)) continue;
+			P.x[q] = P.x[q] + ad * (double)P.p[q];
+			const float r = P.r[q] - a * P.q[q];
+			P.r[q] = r;
+			acc += (double)r * r;
+		}
 	}
 	const double t = bsum<NT>(acc, red);
 	if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = t;
@@ -289,12 +350,19 @@ __global__ __launch_bounds__(NT) void k32_dir(const BlockWork *bw,
 	if (!s.active) return;
 	const Cg32Plane P = pl[b.plane];
 	const float be = (float)s.beta;
-	for (int k_ = 0; k_ < PPT; k_++) {
-		const long qq = pix(b, k_, (size_t)P.w * P.h);
-		if (qq < 0) break;
-		const size_t q = (size_t)qq;
-		if (!(P.code[q] & 1)) continue;
-		P.p[q] = P.z[q] + be * P.p[q];
+	const unsigned n = (unsigned)P.w * (unsigned)P.h;
+	const unsigned q4 = (unsigned)b.start + 4 * threadIdx.x;
+	if (q4 + 3 < n) {
+		if (code4(P.code, q4) & 0x01010101u) {
+			const f4 z = *(const f4 *)(P.z + q4);
+			f4 pp = *(const f4 *)(P.p + q4);
+#pragma unroll
+			for (int k = 0; k < 4; k++) pp[k] = z[k] + be * pp[k];
+			*(f4 *)(P.p + q4) = pp;
+		}
+	} else {
+		for (unsigned q = q4; q < n; q++)
+			if (P.code[q] & 1) P.p[q] = P.z[q] + be * P.p[q];
 	}
 }
 
