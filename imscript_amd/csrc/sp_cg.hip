@@ -541,16 +541,29 @@ __global__ __launch_bounds__(NT) void k_mg_prolong(const BlockWork *bw,
 // everywhere, so after 2*nu half sweeps z is exact on the tile plus one
 // cell (down: enough for the residual) or on the tile (up).
 
-constexpr int TX = MG_TX, TY = MG_TY, MG_NU = 2, HALO = 2 * MG_NU;
-constexpr int RW = TX + 2 * HALO, RH = TY + 2 * HALO;
 // Thread layout: a thread owns the 2x2 blocks (column pair pc, row pair m)
-// for m = m0, m0+4, ...; b and the codes of its cells stay in registers
+// for m = m0, m0+RG, ...; b and the codes of its cells stay in registers
 // across half sweeps, and a block is also one coarse cell (origins even),
 // so the restriction and the coarse correction are thread-local.
-constexpr int MG_T = 256, PAIRS_X = RW / 2, ROWPAIRS = RH / 2;
-constexpr int MG_RG = MG_T / PAIRS_X, MG_K = ROWPAIRS / MG_RG;
-static_assert(PAIRS_X * MG_RG == MG_T && MG_RG * MG_K == ROWPAIRS,
-		"multigrid tile shape must divide the thread layout");
+constexpr int MG_T = 256;
+
+template <int TX_, int TY_, int NU_> struct MgGeo {
+	static constexpr int TX = TX_, TY = TY_, NU = NU_, HALO = 2 * NU;
+	static constexpr int RW = TX + 2 * HALO, RH = TY + 2 * HALO;
+	static constexpr int PAIRS_X = RW / 2, ROWPAIRS = RH / 2;
+	static constexpr int RG = MG_T / PAIRS_X, K = ROWPAIRS / RG;
+	// z of the region in LDS with a zero frame: one row above and below and
+	// one column after each row (the column before a row is the previous
+	// row's frame), so the four neighbour reads of every region cell stay
+	// in bounds and read 0 outside the region, with no tests
+	static constexpr int ZS = RW + 1, ZOFF = ZS, ZN = (RH + 2) * ZS;
+	static_assert(TX % 2 == 0 && TY % 2 == 0, "tile origins must be even");
+	static_assert(PAIRS_X * RG == MG_T && RG * K == ROWPAIRS,
+			"multigrid tile shape must divide the thread layout");
+	static __device__ __forceinline__ int zid(int r, int col) { return ZOFF + r * ZS + col; }
+};
+using GeoFine = MgGeo<MG_TX_FINE, MG_TY, MG_NU_FINE>;
+using GeoCoarse = MgGeo<MG_TX_COARSE, MG_TY, MG_NU_COARSE>;
 
 // level-0 code of every pixel of a CG plane (layout at Cg32Plane::code)
 __global__ __launch_bounds__(NT) void k32_code(const BlockWork *bw,
@@ -595,24 +608,16 @@ __global__ __launch_bounds__(NT) void k_mg_gcode(const BlockWork *bw,
 	}
 }
 
-// The region's z sits in LDS with a zero frame: one row above and below
-// and one column after each row (the column before a row is the previous
-// row's frame), so the four neighbour reads of every region cell stay in
-// bounds and read 0 outside the region, with no tests.
-constexpr int ZS = RW + 1, ZOFF = ZS, ZN = (RH + 2) * ZS;
-
-static __device__ __forceinline__ int zid(int r, int col) { return ZOFF + r * ZS + col; }
-
-// level 0: byte codes, four to a register
+// level 0: byte codes, four to a register per 2x2 block
 struct Lev0 {
-	struct Codes { unsigned v[MG_K]; };
+	static constexpr int CW = 1;   // code words per block
 	typedef unsigned char Raw;
 	typedef uchar2 Raw2;
 	static __device__ __forceinline__ const Raw *codes(const MgLevel &L) { return L.code; }
-	static __device__ __forceinline__ void set(Codes &K, int kb, int c, unsigned x)
-	{ K.v[kb] = c ? K.v[kb] | x << (8 * c) : x; }
-	static __device__ __forceinline__ unsigned get(const Codes &K, int kb, int c)
-	{ return (K.v[kb] >> (8 * c)) & 255u; }
+	static __device__ __forceinline__ void set(unsigned (&k)[CW], int c, unsigned x)
+	{ k[0] = c ? k[0] | x << (8 * c) : x; }
+	static __device__ __forceinline__ unsigned get(const unsigned (&k)[CW], int c)
+	{ return (k[0] >> (8 * c)) & 255u; }
 	static __device__ __forceinline__ unsigned fresh(unsigned k)
 	{
 		asm volatile("" : "+v"(k));
@@ -626,11 +631,11 @@ struct Lev0 {
 		const unsigned d = k >> 5;
 		return d == 4 ? 0.25f : d == 3 ? (1.0f / 3.0f) : d == 2 ? 0.5f : 0.0f;
 	}
-	// s + the neighbours of the cell at z[id] (E, S, W, N)
+	// s + the neighbours of the cell at z[id] (E, S, W, N); zs = row pitch
 	static __device__ __forceinline__ float gather(unsigned k, float s,
-			const float *z, int id)
+			const float *z, int id, int zs)
 	{
-		const float e = z[id + 1], so = z[id + ZS], w = z[id - 1], n = z[id - ZS];
+		const float e = z[id + 1], so = z[id + zs], w = z[id - 1], n = z[id - zs];
 		s += (k & 2) ? e : 0.0f;
 		s += (k & 4) ? so : 0.0f;
 		s += (k & 8) ? w : 0.0f;
@@ -639,16 +644,16 @@ struct Lev0 {
 	}
 };
 
-// levels 1..MG_FUSE_MAX: packed integer coefficients
+// levels 1..MG_FUSE_MAX: packed integer coefficients, one word per cell
 struct LevG {
-	struct Codes { unsigned v[MG_K][4]; };
+	static constexpr int CW = 4;
 	typedef unsigned Raw;
 	typedef uint2 Raw2;
 	static __device__ __forceinline__ const Raw *codes(const MgLevel &L) { return L.gcode; }
-	static __device__ __forceinline__ void set(Codes &K, int kb, int c, unsigned x)
-	{ K.v[kb][c] = x; }
-	static __device__ __forceinline__ unsigned get(const Codes &K, int kb, int c)
-	{ return K.v[kb][c]; }
+	static __device__ __forceinline__ void set(unsigned (&k)[CW], int c, unsigned x)
+	{ k[c] = x; }
+	static __device__ __forceinline__ unsigned get(const unsigned (&k)[CW], int c)
+	{ return k[c]; }
 	// decode the coefficients again in every half sweep: hoisting the
 	// decoded floats out of the sweep loop costs 80 registers per thread
 	static __device__ __forceinline__ unsigned fresh(unsigned k)
@@ -668,31 +673,31 @@ struct LevG {
 		return d ? __builtin_amdgcn_rcpf((float)d) : 0.0f;
 	}
 	static __device__ __forceinline__ float gather(unsigned k, float s,
-			const float *z, int id)
+			const float *z, int id, int zs)
 	{
 		s += (float)((k >> 11) & 31u) * z[id + 1];
-		s += (float)((k >> 16) & 31u) * z[id + ZS];
+		s += (float)((k >> 16) & 31u) * z[id + zs];
 		s += (float)((k >> 21) & 31u) * z[id - 1];
-		s += (float)(k >> 26) * z[id - ZS];
+		s += (float)(k >> 26) * z[id - zs];
 		return s;
 	}
 };
 
 // per-thread cells: 2x2 blocks, cell c of block kb at (2m + (c>>1), 2pc + (c&1))
-template <class P> struct MgCells {
-	float b[MG_K][4];
-	typename P::Codes k;
+template <class P, class G> struct MgCells {
+	float b[G::K][4];
+	unsigned k[G::K][P::CW];
 	int pc, m0;
 };
 
-template <class P>
-static __device__ __forceinline__ int cell_z(const MgCells<P> &C, int kb, int c)
+template <class P, class G>
+static __device__ __forceinline__ int cell_z(const MgCells<P, G> &C, int kb, int c)
 {
-	return zid(2 * (C.m0 + MG_RG * kb) + (c >> 1), 2 * C.pc + (c & 1));
+	return G::zid(2 * (C.m0 + G::RG * kb) + (c >> 1), 2 * C.pc + (c & 1));
 }
 
 // Region geometry of one thread: its columns i0, i0+1 and the rows
-// j0 + 2*MG_RG*kb + (0, 1); q0 = j0 * w + i0 (may lie outside the image).
+// j0 + 2*RG*kb + (0, 1); q0 = j0 * w + i0 (may lie outside the image).
 struct MgSpan {
 	int i0, j0, w, h;
 	long q0;
@@ -713,23 +718,26 @@ static __device__ __forceinline__ MgSpan mg_span(const MgLevel &L, int ox, int o
 	return S;
 }
 
+template <class G>
 static __device__ __forceinline__ bool row_ok(const MgSpan &S, int kb, int hr)
 {
-	const int j = S.j0 + 2 * MG_RG * kb + hr;
+	const int j = S.j0 + 2 * G::RG * kb + hr;
 	return j >= 0 && j < S.h;
 }
 
+template <class G>
 static __device__ __forceinline__ long row_q(const MgSpan &S, int kb, int hr)
 {
-	return S.q0 + (long)(2 * MG_RG * kb + hr) * S.w;
+	return S.q0 + (long)(2 * G::RG * kb + hr) * S.w;
 }
 
 // zero frame of the region's z
+template <class G>
 static __device__ __forceinline__ void z_frame(float *z)
 {
-	for (int e = threadIdx.x; e < 2 * ZS + RH; e += MG_T) {
-		const int at = e < ZS ? e : e < 2 * ZS ? ZOFF + RH * ZS + (e - ZS)
-			: ZOFF + (e - 2 * ZS) * ZS + RW;
+	for (int e = threadIdx.x; e < 2 * G::ZS + G::RH; e += MG_T) {
+		const int at = e < G::ZS ? e : e < 2 * G::ZS ? G::ZOFF + G::RH * G::ZS + (e - G::ZS)
+			: G::ZOFF + (e - 2 * G::ZS) * G::ZS + G::RW;
 		z[at] = 0.0f;
 	}
 }
@@ -737,23 +745,23 @@ static __device__ __forceinline__ void z_frame(float *z)
 // load b and codes (zero outside the image); z of the region from zsrc or 0.
 // With an even width the two cells of a block row are one aligned pair
 // (region origins are even), loaded together.
-template <class P>
-static __device__ void mg_load(const MgLevel &L, int ox, int oy, MgCells<P> &C,
+template <class P, class G>
+static __device__ void mg_load(const MgLevel &L, int ox, int oy, MgCells<P, G> &C,
 		float *z, const float *zsrc)
 {
-	C.pc = threadIdx.x % PAIRS_X;
-	C.m0 = threadIdx.x / PAIRS_X;
+	C.pc = threadIdx.x % G::PAIRS_X;
+	C.m0 = threadIdx.x / G::PAIRS_X;
 	const MgSpan S = mg_span(L, ox, oy, C.pc, C.m0);
 	const typename P::Raw *code = P::codes(L);
 	const bool pair = !(L.w & 1);
 #pragma unroll
-	for (int kb = 0; kb < MG_K; kb++)
+	for (int kb = 0; kb < G::K; kb++)
 #pragma unroll
 		for (int hr = 0; hr < 2; hr++) {
 			float bv[2] = {0.0f, 0.0f}, zv[2] = {0.0f, 0.0f};
 			unsigned kv[2] = {0u, 0u};
-			if (row_ok(S, kb, hr)) {
-				const long q = row_q(S, kb, hr);
+			if (row_ok<G>(S, kb, hr)) {
+				const long q = row_q<G>(S, kb, hr);
 				if (pair) {
 					if (S.colok[0]) {
 						const float2 b2 = *(const float2 *)(L.b + q);
@@ -779,22 +787,22 @@ static __device__ void mg_load(const MgLevel &L, int ox, int oy, MgCells<P> &C,
 			for (int hc = 0; hc < 2; hc++) {
 				const int c = 2 * hr + hc;
 				C.b[kb][c] = bv[hc];
-				P::set(C.k, kb, c, kv[hc]);
+				P::set(C.k[kb], c, kv[hc]);
 				z[cell_z(C, kb, c)] = zv[hc];
 			}
 		}
 }
 
 // store z of the thread's tile cells (inside the image)
-template <class P>
+template <class P, class G>
 static __device__ __forceinline__ void mg_store_z(const MgLevel &L, int ox, int oy,
-		const MgCells<P> &C, const float *z, int kb)
+		const MgCells<P, G> &C, const float *z, int kb)
 {
 	const MgSpan S = mg_span(L, ox, oy, C.pc, C.m0);
 #pragma unroll
 	for (int hr = 0; hr < 2; hr++) {
-		if (!row_ok(S, kb, hr)) continue;
-		const long q = row_q(S, kb, hr);
+		if (!row_ok<G>(S, kb, hr)) continue;
+		const long q = row_q<G>(S, kb, hr);
 		const float z0 = z[cell_z(C, kb, 2 * hr)], z1 = z[cell_z(C, kb, 2 * hr + 1)];
 		if (!(L.w & 1)) {
 			if (S.colok[0]) *(float2 *)(L.z + q) = make_float2(z0, z1);
@@ -809,25 +817,25 @@ static __device__ __forceinline__ void mg_store_z(const MgLevel &L, int ox, int 
 // neighbour loads first (cells of one colour do not depend on each other).
 // An inactive cell (known pixel, empty aggregate, outside the image) has
 // b = 0, no couplings and 1/diagonal = 0, so it keeps z = 0 without a test.
-template <class P>
-static __device__ __forceinline__ void mg_half(const MgCells<P> &C, float *z,
+template <class P, class G>
+static __device__ __forceinline__ void mg_half(const MgCells<P, G> &C, float *z,
 		int color, bool first)
 {
-	float nz[MG_K][2];
+	float nz[G::K][2];
 #pragma unroll
-	for (int kb = 0; kb < MG_K; kb++)
+	for (int kb = 0; kb < G::K; kb++)
 #pragma unroll
 		for (int h = 0; h < 2; h++) {
 			// colour of cell c is ((c>>1) + (c&1) + pair origin parity) & 1;
 			// origins are even, so cells 0,3 have colour 0 and 1,2 colour 1
 			const int c = color == 0 ? (h ? 3 : 0) : (h ? 2 : 1);
-			const unsigned k = P::fresh(P::get(C.k, kb, c));
+			const unsigned k = P::fresh(P::get(C.k[kb], c));
 			float s = C.b[kb][c];
-			if (!first) s = P::gather(k, s, z, cell_z(C, kb, c));
+			if (!first) s = P::gather(k, s, z, cell_z(C, kb, c), G::ZS);
 			nz[kb][h] = s * P::rdiag(k);
 		}
 #pragma unroll
-	for (int kb = 0; kb < MG_K; kb++)
+	for (int kb = 0; kb < G::K; kb++)
 #pragma unroll
 		for (int h = 0; h < 2; h++) {
 			const int c = color == 0 ? (h ? 3 : 0) : (h ? 2 : 1);
@@ -835,47 +843,47 @@ static __device__ __forceinline__ void mg_half(const MgCells<P> &C, float *z,
 		}
 }
 
-template <class P>
-static __device__ __forceinline__ bool in_tile(const MgCells<P> &C, int kb)
+template <class P, class G>
+static __device__ __forceinline__ bool in_tile(const MgCells<P, G> &C, int kb)
 {
-	const int m = C.m0 + MG_RG * kb;
-	return C.pc >= HALO / 2 && C.pc < HALO / 2 + TX / 2
-		&& m >= HALO / 2 && m < HALO / 2 + TY / 2;
+	const int m = C.m0 + G::RG * kb;
+	return C.pc >= G::HALO / 2 && C.pc < G::HALO / 2 + G::TX / 2
+		&& m >= G::HALO / 2 && m < G::HALO / 2 + G::TY / 2;
 }
 
 // pre-smoothing from z = 0, residual, 2x2 restriction into the next level
-template <class P>
+template <class P, class G>
 __global__ __launch_bounds__(MG_T) __attribute__((amdgpu_waves_per_eu(4)))
 void k_mg_down(const int4 *tiles,
 		const MgLevel *lv, const MgLevel *coarse, const Cg32Scal *sc)
 {
-	__shared__ float z[ZN];
+	__shared__ float z[G::ZN];
 	const int4 t = tiles[blockIdx.x];
 	if (!sc[t.x].active) return;
 	const MgLevel L = lv[t.x], Cc = coarse[t.x];
-	const int ox = t.y - HALO, oy = t.z - HALO;
-	MgCells<P> C;
-	z_frame(z);
+	const int ox = t.y - G::HALO, oy = t.z - G::HALO;
+	MgCells<P, G> C;
+	z_frame<G>(z);
 	mg_load(L, ox, oy, C, z, nullptr);
 	__syncthreads();
-	for (int s = 0; s < 2 * MG_NU; s++) {
+	for (int s = 0; s < 2 * G::NU; s++) {
 		mg_half(C, z, s & 1, s == 0);
 		__syncthreads();
 	}
 #pragma unroll
-	for (int kb = 0; kb < MG_K; kb++) {
+	for (int kb = 0; kb < G::K; kb++) {
 		if (!in_tile(C, kb)) continue;
 		mg_store_z(L, ox, oy, C, z, kb);
 		float acc = 0.0f;
 #pragma unroll
 		for (int c = 0; c < 4; c++) {
-			const unsigned k = P::get(C.k, kb, c);
+			const unsigned k = P::get(C.k[kb], c);
 			const int id = cell_z(C, kb, c);
 			// inactive cells add b - A z = 0 - 0
-			const float az = P::diag(k) * z[id] - P::gather(k, 0.0f, z, id);
+			const float az = P::diag(k) * z[id] - P::gather(k, 0.0f, z, id, G::ZS);
 			acc += C.b[kb][c] - az;
 		}
-		const int I = (ox >> 1) + C.pc, J = (oy >> 1) + C.m0 + MG_RG * kb;
+		const int I = (ox >> 1) + C.pc, J = (oy >> 1) + C.m0 + G::RG * kb;
 		if (I < Cc.w && J < Cc.h) {
 			const size_t cq = (size_t)J * Cc.w + I;
 			Cc.b[cq] = acc;
@@ -886,38 +894,38 @@ void k_mg_down(const int4 *tiles,
 
 // coarse correction, post-smoothing; at level 0 also the partial r.z of
 // the tile
-template <class P, bool DOT>
+template <class P, class G, bool DOT>
 __global__ __launch_bounds__(MG_T) __attribute__((amdgpu_waves_per_eu(4)))
 void k_mg_up(const int4 *tiles, const MgLevel *lv,
 		const MgLevel *coarse, const Cg32Scal *sc, float wc, double *tpart)
 {
-	__shared__ float z[ZN];
+	__shared__ float z[G::ZN];
 	__shared__ double red[MG_T / 64];
 	const int4 t = tiles[blockIdx.x];
 	if (!sc[t.x].active) return;
 	const MgLevel L = lv[t.x], Cc = coarse[t.x];
-	const int ox = t.y - HALO, oy = t.z - HALO;
-	MgCells<P> C;
-	z_frame(z);
+	const int ox = t.y - G::HALO, oy = t.z - G::HALO;
+	MgCells<P, G> C;
+	z_frame<G>(z);
 	mg_load(L, ox, oy, C, z, L.z);
 	// every block of the region is one coarse cell: add its correction
 #pragma unroll
-	for (int kb = 0; kb < MG_K; kb++) {
-		const int I = (ox >> 1) + C.pc, J = (oy >> 1) + C.m0 + MG_RG * kb;
+	for (int kb = 0; kb < G::K; kb++) {
+		const int I = (ox >> 1) + C.pc, J = (oy >> 1) + C.m0 + G::RG * kb;
 		if (I < 0 || J < 0 || I >= Cc.w || J >= Cc.h) continue;
 		const float e = wc * Cc.z[(size_t)J * Cc.w + I];
 #pragma unroll
 		for (int c = 0; c < 4; c++)
-			z[cell_z(C, kb, c)] += e * P::mask(P::get(C.k, kb, c));
+			z[cell_z(C, kb, c)] += e * P::mask(P::get(C.k[kb], c));
 	}
 	__syncthreads();
-	for (int s = 0; s < 2 * MG_NU; s++) {
+	for (int s = 0; s < 2 * G::NU; s++) {
 		mg_half(C, z, (s & 1) ^ 1, false);
 		__syncthreads();
 	}
 	double acc = 0;
 #pragma unroll
-	for (int kb = 0; kb < MG_K; kb++) {
+	for (int kb = 0; kb < G::K; kb++) {
 		if (!in_tile(C, kb)) continue;
 		mg_store_z(L, ox, oy, C, z, kb);
 		if (DOT) {
@@ -1049,8 +1057,8 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 	// z and a quarter coarse b (10 B); up reads z, b, the code and a quarter
 	// coarse z, writes z (14 B)
 	if (B.prof) B.prof("mg_down0", 1, 0, s);
-	k_mg_down<Lev0><<<B.ntiles0, MG_T, 0, s>>>(B.tiles0, B.mg[0].lv, B.mg[0].coarse,
-			B.scal);
+	k_mg_down<Lev0, GeoFine><<<B.ntiles0, MG_T, 0, s>>>(B.tiles0, B.mg[0].lv,
+			B.mg[0].coarse, B.scal);
 	if (B.prof) B.prof("mg_down0", 0, 10.0 * B.px0, s);
 	// levels >= 1 outside the tail: fused tiles (packed coefficients; per
 	// cell down reads b and the code, writes z and a quarter coarse b and z,
@@ -1060,11 +1068,17 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 		const MgLevelWork &W = B.mg[l];
 		if (W.ntiles) {
 			if (B.prof) B.prof(g_down_name[l], 1, 0, s);
-			k_mg_down<LevG><<<W.ntiles, MG_T, 0, s>>>(W.tiles, W.lv, W.coarse, B.scal);
+			if (l == 1)
+				k_mg_down<LevG, GeoFine><<<W.ntiles, MG_T, 0, s>>>(W.tiles, W.lv,
+						W.coarse, B.scal);
+			else
+				k_mg_down<LevG, GeoCoarse><<<W.ntiles, MG_T, 0, s>>>(W.tiles, W.lv,
+						W.coarse, B.scal);
 			if (B.prof) B.prof(g_down_name[l], 0, 14.0 * W.cells, s);
 		}
 		if (!W.nmid) continue;
-		for (int it = 0; it < B.nu; it++) {
+		const int nu = l >= 2 ? B.nu_c : B.nu;
+		for (int it = 0; it < nu; it++) {
 			k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 0, it == 0, 0);
 			k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 1, 0, 0);
 		}
@@ -1072,7 +1086,7 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 	}
 	// small levels of every plane in one work-group each
 	if (B.prof) B.prof("mg_tail", 1, 0, s);
-	k_mg_tail<<<B.nplanes, MG_TAIL_T, 0, s>>>(B.tail_lv, B.tail_rng, B.scal, B.nu,
+	k_mg_tail<<<B.nplanes, MG_TAIL_T, 0, s>>>(B.tail_lv, B.tail_rng, B.scal, B.nu_c,
 			B.coarse_sweeps, B.wc);
 	if (B.prof) B.prof("mg_tail", 0, 0.0, s);
 	// up: correct, post-smooth in the mirrored order
@@ -1080,21 +1094,26 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 		const MgLevelWork &W = B.mg[l];
 		if (W.nmid) {
 			k_mg_prolong<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, W.coarse, B.scal, B.wc, 0);
-			for (int it = 0; it < B.nu; it++) {
+			const int nu = l >= 2 ? B.nu_c : B.nu;
+			for (int it = 0; it < nu; it++) {
 				k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 1, 0, 0);
 				k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 0, 0, 0);
 			}
 		}
 		if (W.ntiles) {
 			if (B.prof) B.prof(g_up_name[l], 1, 0, s);
-			k_mg_up<LevG, false><<<W.ntiles, MG_T, 0, s>>>(W.tiles, W.lv, W.coarse,
-					B.scal, B.wc, nullptr);
+			if (l == 1)
+				k_mg_up<LevG, GeoFine, false><<<W.ntiles, MG_T, 0, s>>>(W.tiles,
+						W.lv, W.coarse, B.scal, B.wc, nullptr);
+			else
+				k_mg_up<LevG, GeoCoarse, false><<<W.ntiles, MG_T, 0, s>>>(W.tiles,
+						W.lv, W.coarse, B.scal, B.wc, nullptr);
 			if (B.prof) B.prof(g_up_name[l], 0, 17.0 * W.cells, s);
 		}
 	}
 	if (B.prof) B.prof("mg_up0", 1, 0, s);
-	k_mg_up<Lev0, true><<<B.ntiles0, MG_T, 0, s>>>(B.tiles0, B.mg[0].lv, B.mg[0].coarse,
-			B.scal, B.wc, B.tpart);
+	k_mg_up<Lev0, GeoFine, true><<<B.ntiles0, MG_T, 0, s>>>(B.tiles0, B.mg[0].lv,
+			B.mg[0].coarse, B.scal, B.wc, B.tpart);
 	if (B.prof) B.prof("mg_up0", 0, 14.0 * B.px0, s);
 }
 

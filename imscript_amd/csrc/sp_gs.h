@@ -313,28 +313,97 @@ SP_HD void tile_store(const TileCtx<S> &c, const GsInst &in, int tid)
 // ------------------------------------------ whole level in one work-group
 
 // For levels that fit in LDS: one block runs every sweep of one plane,
-// hyperplane by hyperplane.  Step s, thread slot e: (row j, k-offset qk).
+// hyperplane by hyperplane.  Step s, thread slot e: (row j, k-offset qk);
+// a thread's slots are fixed for the whole run.  For R = 1 the level sits
+// in LDS inside a one-cell frame of NaN / known cells (same argument as the
+// tiles), so the update needs no bounds tests.
+constexpr int SMALL_MAXSLOT = 8;
+struct SmallSlots { int n; int j[SMALL_MAXSLOT], qk[SMALL_MAXSLOT]; };
+
+SP_HD int small_slot_count(int R, int W, int H)
+{
+	return H * ((W - 1) / (R * R + 1) + 1);
+}
+
+// LDS cells of one plane (u and the coefficient take this many floats each)
+SP_HD int small_cells(int R, int W, int H)
+{
+	return R == 1 ? (W + 2) * (H + 2) : W * H;
+}
+
+template <int R>
+SP_HD void small_slots(int W, int H, int tid, int nthreads, SmallSlots &o)
+{
+	const int slots = small_slot_count(R, W, H);
+	o.n = tid < slots ? (slots - tid + nthreads - 1) / nthreads : 0;
+	if (o.n > SMALL_MAXSLOT) o.n = SMALL_MAXSLOT;
+#pragma unroll
+	for (int m = 0; m < SMALL_MAXSLOT; m++) {
+		const int e = tid + m * nthreads;
+		o.j[m] = e % H;
+		o.qk[m] = e / H;
+	}
+}
+
+template <int R>
+SP_HD void small_load(float *U, float *C, const GsInst &in, int tid, int nthreads)
+{
+	const int W = in.w, H = in.h;
+	if (R == 1) {
+		const int P = W + 2, n = P * (H + 2);
+		for (int e = tid; e < n; e += nthreads) {
+			const int r = e / P, col = e - r * P, i = col - 1, j = r - 1;
+			if (i >= 0 && i < W && j >= 0 && j < H) {
+				U[e] = in.u[j * W + i];
+				C[e] = in.c[j * W + i];
+			} else {
+				U[e] = sp_float(SP_QNAN_BITS);
+				C[e] = sp_float(SP_KNOWN_BITS);
+			}
+		}
+	} else {
+		for (int e = tid; e < W * H; e += nthreads) {
+			U[e] = in.u[e];
+			C[e] = in.c[e];
+		}
+	}
+}
+
+template <int R>
+SP_HD void small_store(const float *U, const float *C, const GsInst &in, int tid,
+		int nthreads)
+{
+	const int W = in.w, H = in.h;
+	for (int q = tid; q < W * H; q += nthreads) {
+		const int e = R == 1 ? (q / W + 1) * (W + 2) + q % W + 1 : q;
+		if (!sp_known(C[e])) in.u[q] = U[e];
+	}
+}
+
 template <int R>
 SP_HD void small_step(float *U, const float *C, int W, int H, int T, float t,
-		long s, int tid, int nthreads)
+		int s, const SmallSlots &sl)
 {
 	constexpr int CT = R * R + 1;
-	const int KT = (W - 1) / CT + 1;     // sweeps meeting one row on a plane
-	const int slots = H * KT;
-	for (int e = tid; e < slots; e += nthreads) {
-		const int j = e % H, qk = e / H;
-		const long base = s - (long)R * j;   // = i + CT*k
+#pragma unroll
+	for (int m = 0; m < SMALL_MAXSLOT; m++) {
+		if (m >= sl.n) break;
+		const int j = sl.j[m];
+		const int base = s - R * j;   // = i + CT*k
 		if (base < 0) continue;
-		long k = base / CT;
+		int k = base / CT;
 		if (k > T - 1) k = T - 1;
-		k -= qk;
+		k -= sl.qk[m];
 		if (k < 0) continue;
-		const long i = base - CT * k;
+		const int i = base - CT * k;
 		if (i >= W) continue;
-		const int idx = j * W + (int)i;
-		const float cv = C[idx];
-		if (!sp_known(cv))
-			gs_update<R>(U, cv, idx, (int)i, j, W, H, W, t);
+		if (R == 1) {
+			gs_update_tile1(U, C, (j + 1) * (W + 2) + i + 1, W + 2, t);
+		} else {
+			const int idx = j * W + i;
+			const float cv = C[idx];
+			if (!sp_known(cv)) gs_update<R>(U, cv, idx, i, j, W, H, W, t);
+		}
 	}
 }
 

@@ -600,14 +600,17 @@ static int plan_create(Plan &pl, const std::vector<PlaneIn> &planes,
 		for (int m = 0; m < nm; m++) {
 			const int w = L.sp[m].w, h = L.sp[m].h, T = L.nsweep[m];
 			if (T <= 0) continue;
-			if ((long)8 * w * h <= g_max_lds) {
+			// whole level in one work-group: it fits in LDS, every thread
+			// holds at most SMALL_MAXSLOT slots, and the step count fits int
+			const long slots = (long)h * ((w - 1) / (pl.R * pl.R + 1) + 1);
+			if ((long)8 * small_cells(pl.R, w, h) <= g_max_lds
+					&& slots <= 1024L * SMALL_MAXSLOT
+					&& small_nsteps(pl.R, w, h, T) < 0x7fffffffL) {
 				small.push_back(m);
-				const int ct = pl.R * pl.R + 1;
-				const long slots = (long)h * ((w - 1) / ct + 1);
 				const long th = (slots + 63) / 64 * 64;
 				L.small_threads = (int)std::max<long>(L.small_threads,
 						std::min<long>(1024, th));
-				L.small_lds = std::max(L.small_lds, spk::small_lds_bytes(w, h));
+				L.small_lds = std::max(L.small_lds, spk::small_lds_bytes(pl.R, w, h));
 				continue;
 			}
 			L.tiled.push_back(m);
@@ -789,7 +792,7 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 				L.code = cp[z].code;
 				cp[z].t0 = (int)tiles0.size();
 				for (int y0 = 0; y0 < L.h; y0 += MG_TY)
-					for (int x0 = 0; x0 < L.w; x0 += MG_TX)
+					for (int x0 = 0; x0 < L.w; x0 += MG_TX_FINE)
 						tiles0.push_back(make_int4(z, x0, y0, 0));
 				cp[z].nt = (int)tiles0.size() - cp[z].t0;
 				continue;
@@ -825,8 +828,9 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 			for (size_t st = 0; st < nc; st += EW_SPAN) bc.push_back({z, (int)st});
 			if (l < 1 || l >= rng[z].x) continue;   // level 0 or in the tail
 			if (l <= MG_FUSE_MAX) {   // fused tiles
+				const int tx = l == 1 ? MG_TX_FINE : MG_TX_COARSE;
 				for (int y0 = 0; y0 < Lz.h; y0 += MG_TY)
-					for (int x0 = 0; x0 < Lz.w; x0 += MG_TX)
+					for (int x0 = 0; x0 < Lz.w; x0 += tx)
 						tl.push_back(make_int4(z, x0, y0, 0));
 				tcells += (double)n;
 			} else {                  // per-level kernels
@@ -858,9 +862,10 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 	TRY(pl.alloc(&B.tpart, tiles0.size()));
 	B.mg = mgw.data();
 	B.nlev = nlev;
-	B.nu = 2;
-	B.coarse_sweeps = 8;
-	B.wc = 1.6f;
+	B.nu = MG_NU_FINE;
+	B.nu_c = MG_NU_COARSE;
+	B.coarse_sweeps = 4;
+	B.wc = 1.8f;
 	return SP_OK;
 }
 

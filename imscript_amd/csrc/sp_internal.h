@@ -51,7 +51,7 @@ void launch_stream_copy(float *dst, const float *src, size_t n4, hipStream_t s);
 
 // tiles of shape variant `var` of radius R (sp_gs.h); one launch per wavefront
 int tile_lds_bytes(int R, int var);
-int small_lds_bytes(int w, int h);
+int small_lds_bytes(int R, int w, int h);
 void launch_gs_tiles(int R, int var, const GsTile *tiles, int ntiles,
 		const GsInst *insts, hipStream_t s);
 void launch_gs_small(int R, const GsInst *insts, const int *ids, int nids,

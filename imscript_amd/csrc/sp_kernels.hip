@@ -263,29 +263,27 @@ void launch_gs_tiles(int R, int var, const GsTile *tiles, int ntiles,
 
 // whole level in one block (levels that fit in LDS)
 template <int R>
-__global__ void k_gs_small(const GsInst *insts, const int *ids)
+__global__ __launch_bounds__(1024) void k_gs_small(const GsInst *insts, const int *ids)
 {
 	extern __shared__ float smem[];
 	const GsInst in = insts[ids[blockIdx.x]];
 	if (in.active && !*in.active) return;
-	const int W = in.w, H = in.h, n = W * H, T = in.nsweep;
+	const int W = in.w, H = in.h, T = in.nsweep;
 	if (T <= 0) return;
-	float *U = smem, *C = smem + n;
-	for (int e = threadIdx.x; e < n; e += blockDim.x) {
-		U[e] = in.u[e];
-		C[e] = in.c[e];
-	}
+	float *U = smem, *C = smem + small_cells(R, W, H);
+	small_load<R>(U, C, in, threadIdx.x, blockDim.x);
+	SmallSlots sl;
+	small_slots<R>(W, H, threadIdx.x, blockDim.x, sl);
 	__syncthreads();
-	const long ns = small_nsteps(R, W, H, T);
-	for (long s = 0; s < ns; s++) {
-		small_step<R>(U, C, W, H, T, in.t, s, threadIdx.x, blockDim.x);
+	const int ns = (int)small_nsteps(R, W, H, T);
+	for (int s = 0; s < ns; s++) {
+		small_step<R>(U, C, W, H, T, in.t, s, sl);
 		__syncthreads();
 	}
-	for (int e = threadIdx.x; e < n; e += blockDim.x)
-		if (!sp_known(C[e])) in.u[e] = U[e];
+	small_store<R>(U, C, in, threadIdx.x, blockDim.x);
 }
 
-int small_lds_bytes(int w, int h) { return 8 * w * h; }
+int small_lds_bytes(int R, int w, int h) { return 8 * small_cells(R, w, h); }
 
 void launch_gs_small(int R, const GsInst *insts, const int *ids, int nids,
 		int lds_bytes, int threads, hipStream_t s)

@@ -79,19 +79,27 @@ static int run_tiles(float *u, const float *c, int W, int H, int T, float t)
 template <int R>
 static int run_small(float *u, const float *c, int W, int H, int T, float t)
 {
-	const int n = W * H;
-	float *lds = new float[2 * n];
-	float *U = lds, *C = lds + n;
-	for (int e = 0; e < n; e++) { U[e] = u[e]; C[e] = c[e]; }
-	const int ct = R * R + 1;
-	long slots = (long)H * ((W - 1) / ct + 1);
+	const int cells = small_cells(R, W, H);
+	float *lds = new float[2 * cells];
+	for (int e = 0; e < 2 * cells; e++) lds[e] = -12345.0f;
+	float *U = lds, *C = lds + cells;
+	GsInst in{u, c, W, H, T, t, nullptr};
+	const long slots = small_slot_count(R, W, H);
 	int nt = (int)std::min<long>(1024, std::max<long>(64, (slots + 63) / 64 * 64));
+	if (slots > (long)nt * SMALL_MAXSLOT) {
+		printf("TOO MANY SLOTS %ld\n", slots);
+		return 1;
+	}
+	std::vector<SmallSlots> sl(nt);
+	for (int tid = 0; tid < nt; tid++) {
+		small_load<R>(U, C, in, tid, nt);
+		small_slots<R>(W, H, tid, nt, sl[tid]);
+	}
 	const long ns = small_nsteps(R, W, H, T);
 	for (long s = 0; s < ns; s++)
 		for (int tid = 0; tid < nt; tid++)
-			small_step<R>(U, C, W, H, T, t, s, tid, nt);
-	for (int e = 0; e < n; e++)
-		if (!sp_known(C[e])) u[e] = U[e];
+			small_step<R>(U, C, W, H, T, t, (int)s, sl[tid]);
+	for (int tid = 0; tid < nt; tid++) small_store<R>(U, C, in, tid, nt);
 	delete[] lds;
 	return 0;
 }

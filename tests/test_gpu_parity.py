@@ -276,3 +276,21 @@ def test_autotuned_tile_shapes_match_oracle(tstep, niter):
     rep = m.autotune_report()
     R = 1 if tstep > 0 else (3 if tstep < -1000 else 2)
     assert f"gs R={R} T={niter}" in rep and "*" in rep
+
+
+def test_stream_copy_copies_and_checks_its_arguments():
+    """The measurement copy kernel moves every element (tail of a size that is
+    not a multiple of its 4096-float step included) and refuses misaligned or
+    ragged input with a status code instead of launching."""
+    import torch
+    m = _imscript()
+    print("stream copy 1000004 floats", flush=True)
+    src = torch.arange(1000004, dtype=torch.float32, device="cuda")
+    dst = torch.zeros_like(src)
+    m.stream_copy(dst, src)
+    torch.cuda.synchronize()
+    assert torch.equal(dst, src)
+    with pytest.raises(m.SimpoisError):
+        m.stream_copy(dst[:1000002], src[:1000002])        # not a multiple of 4
+    with pytest.raises(m.SimpoisError):
+        m.stream_copy(dst[1:1000001], src[1:1000001])      # misaligned

@@ -86,3 +86,22 @@ def test_unknown_tile_shape_is_refused(emulator):
                        capture_output=True, text=True, timeout=60,
                        env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"))
     assert p.returncode == 2 and "NO SHAPE" in p.stdout
+
+
+# whole-level kernel at the limits of its slot table and LDS budget
+SMALL_LIMIT_CASES = [
+    "1 8000 9 0.25 51 0.5 1 small",      # W = 1: 8000 slots, 8 per thread
+    "8000 1 9 0.25 52 0.5 1 small",
+    "89 89 30 0.25 53 0.4 1 small",      # framed 91x91 cells = 66 KB of u + c
+    "63 61 25 -0.08 54 0.4 1 small",
+    "40 40 7 -1000.02 55 0.4 1 small",
+]
+
+
+@pytest.mark.parametrize("args", SMALL_LIMIT_CASES)
+def test_whole_level_kernel_limits(emulator, args):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    p = subprocess.run([emulator, *args.split()], capture_output=True,
+                       text=True, timeout=600, env=env)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert p.stdout.startswith("OK")
