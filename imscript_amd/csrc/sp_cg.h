@@ -31,9 +31,11 @@ struct Cg32Scal {
 
 // Fused multigrid tile kernels: levels 0 and 1 smooth nu = 2 times on
 // 120x32 tiles, levels 2..MG_FUSE_MAX nu = 4 times on 112x32 tiles (the halo
-// is 2 nu cells; the region is 128 cells wide either way).  Levels from 2
-// on are cheap (a sixteenth of the pixels), and the stronger smoothing there
-// saves V-cycles (14 -> 11 CG iterations to 5e-6 on 50% rectangle holes).
+// is 2 nu cells; the region is 128 cells wide either way).  Those levels are
+// cheap (a sixteenth of the pixels and less), and the stronger smoothing
+// there saves V-cycles (14 -> 11 CG iterations to 5e-6 on 50% rectangle
+// holes); the levels below them (per-level kernels, tail) smooth nu = 2
+// times, where more sweeps change nothing.
 constexpr int MG_NU_FINE = 2, MG_NU_COARSE = 4;
 constexpr int MG_TX_FINE = 120, MG_TX_COARSE = 112, MG_TY = 32;
 
@@ -85,7 +87,7 @@ struct Cg32Batch {
 	int nplanes, nblk;
 	// multigrid preconditioner (nlev == 0: plain CG)
 	MgLevelWork *mg;   // host array of nlev levels
-	int nlev, nu, nu_c, coarse_sweeps;   // nu: levels 0-1, nu_c: levels >= 2
+	int nlev, nu, coarse_sweeps;   // nu: per-level kernels and the tail
 	float wc;          // coarse-correction factor
 	MgLevel *tail_lv;  // [plane * MG_MAXLEV + level]
 	int2 *tail_rng;    // per plane: (first tail level, depth)

@@ -100,30 +100,43 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 	const BlockWork b = bw[blockIdx.x];
 	const Cg32Plane P = pl[b.plane];
 	double rr = 0, bb = 0;
-	for (int k_ = 0; k_ < PPT; k_++) {
-		const long qq = pix(b, k_, (size_t)P.w * P.h);
-		if (qq < 0) break;
-		const size_t q = (size_t)qq;
-		const float gq = P.g[q];
-		if (!isnan(gq)) {
-			P.x[q] = 0; P.r[q] = 0; P.p[q] = 0; P.q[q] = 0;
-			continue;
+	const unsigned n = (unsigned)P.w * (unsigned)P.h, w = (unsigned)P.w;
+	const unsigned q4 = (unsigned)b.start + 4 * threadIdx.x;
+	const unsigned q1 = q4 + 4 < n ? q4 + 4 : n;
+	const bool whole = q4 + 3 < n;
+	const unsigned cc = whole ? code4(P.code, q4) : 0u;
+	f4 rv = {0.0f, 0.0f, 0.0f, 0.0f};
+	d4 xv = {0.0, 0.0, 0.0, 0.0};
+	if (!whole || (cc & 0x01010101u)) {
+		// coordinates of the group's first pixel; the others follow along
+		// the row (a group may wrap onto the next row)
+		unsigned j = q4 / w, i = q4 - j * w;
+		for (unsigned q = q4; q < q1; q++) {
+			const unsigned k = q - q4;
+			const bool hole = whole ? ((cc >> (8 * k)) & 1u) : (P.code[q] & 1u);
+			if (hole) {
+				const float *g = P.g, *u = P.u;
+				const float a0 = op32(P.bilap, (int)i, (int)j, P.w, P.h, [&](size_t t) {
+					const float v = g[t]; return isnan(v) ? 0.0f : v; });
+				const float ax = op32(P.bilap, (int)i, (int)j, P.w, P.h, [&](size_t t) {
+					return isnan(g[t]) ? u[t] : 0.0f; });
+				const float bv = (P.f ? -P.f[q] : 0.0f) - a0;
+				const float r = bv - ax;
+				rr += (double)r * r;
+				bb += (double)bv * bv;
+				if (whole) { rv[k] = r; xv[k] = (double)u[q]; }
+				else { P.r[q] = r; P.p[q] = r; P.q[q] = 0.0f; P.x[q] = (double)u[q]; }
+			} else if (!whole) {
+				P.r[q] = 0.0f; P.p[q] = 0.0f; P.q[q] = 0.0f; P.x[q] = 0.0;
+			}
+			if (++i == w) { i = 0; j++; }
 		}
-		const unsigned jj = (unsigned)q / (unsigned)P.w;
-		const int i = (int)((unsigned)q - jj * (unsigned)P.w), j = (int)jj;
-		const float *g = P.g, *u = P.u;
-		const float a0 = op32(P.bilap, i, j, P.w, P.h, [&](size_t k) {
-			float v = g[k]; return isnan(v) ? 0.0f : v; });
-		const float ax = op32(P.bilap, i, j, P.w, P.h, [&](size_t k) {
-			return isnan(g[k]) ? u[k] : 0.0f; });
-		const float bv = (P.f ? -P.f[q] : 0.0f) - a0;
-		const float r = bv - ax;
-		P.x[q] = (double)u[q];
-		P.r[q] = r;
-		P.p[q] = r;
-		P.q[q] = 0;
-		rr += (double)r * r;
-		bb += (double)bv * bv;
+	}
+	if (whole) {   // known entries are 0 in every vector
+		*(f4 *)(P.r + q4) = rv;
+		*(f4 *)(P.p + q4) = rv;
+		*(f4 *)(P.q + q4) = f4{0.0f, 0.0f, 0.0f, 0.0f};
+		*(d4 *)(P.x + q4) = xv;
 	}
 	const double s1 = bsum<NT>(rr, red), s2 = bsum<NT>(bb, red);
 	if (threadIdx.x == 0) {
@@ -1077,8 +1090,7 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 			if (B.prof) B.prof(g_down_name[l], 0, 14.0 * W.cells, s);
 		}
 		if (!W.nmid) continue;
-		const int nu = l >= 2 ? B.nu_c : B.nu;
-		for (int it = 0; it < nu; it++) {
+		for (int it = 0; it < B.nu; it++) {
 			k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 0, it == 0, 0);
 			k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 1, 0, 0);
 		}
@@ -1086,7 +1098,7 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 	}
 	// small levels of every plane in one work-group each
 	if (B.prof) B.prof("mg_tail", 1, 0, s);
-	k_mg_tail<<<B.nplanes, MG_TAIL_T, 0, s>>>(B.tail_lv, B.tail_rng, B.scal, B.nu_c,
+	k_mg_tail<<<B.nplanes, MG_TAIL_T, 0, s>>>(B.tail_lv, B.tail_rng, B.scal, B.nu,
 			B.coarse_sweeps, B.wc);
 	if (B.prof) B.prof("mg_tail", 0, 0.0, s);
 	// up: correct, post-smooth in the mirrored order
@@ -1094,8 +1106,7 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 		const MgLevelWork &W = B.mg[l];
 		if (W.nmid) {
 			k_mg_prolong<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, W.coarse, B.scal, B.wc, 0);
-			const int nu = l >= 2 ? B.nu_c : B.nu;
-			for (int it = 0; it < nu; it++) {
+			for (int it = 0; it < B.nu; it++) {
 				k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 1, 0, 0);
 				k_mg_smooth<<<W.nmid, NT, 0, s>>>(W.mid, W.lv, B.scal, 0, 0, 0);
 			}

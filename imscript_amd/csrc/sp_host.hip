@@ -863,7 +863,6 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 	B.mg = mgw.data();
 	B.nlev = nlev;
 	B.nu = MG_NU_FINE;
-	B.nu_c = MG_NU_COARSE;
 	B.coarse_sweeps = 4;
 	B.wc = 1.8f;
 	return SP_OK;
