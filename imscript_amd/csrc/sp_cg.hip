@@ -116,10 +116,31 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 			const bool hole = whole ? ((cc >> (8 * k)) & 1u) : (P.code[q] & 1u);
 			if (hole) {
 				const float *g = P.g, *u = P.u;
-				const float a0 = op32(P.bilap, (int)i, (int)j, P.w, P.h, [&](size_t t) {
-					const float v = g[t]; return isnan(v) ? 0.0f : v; });
-				const float ax = op32(P.bilap, (int)i, (int)j, P.w, P.h, [&](size_t t) {
-					return isnan(g[t]) ? u[t] : 0.0f; });
+				float a0, ax;
+				if (!P.bilap) {
+					// op32's clamped 5-point sum in its term order, both fields
+					// (known values, unknowns) from one set of loads
+					const unsigned t[4] = {i + 1 < w ? q + 1 : q,
+						j + 1 < (unsigned)P.h ? q + w : q, i > 0 ? q - 1 : q,
+						j > 0 ? q - w : q};
+					float gv[4], uv[4];
+#pragma unroll
+					for (int d = 0; d < 4; d++) { gv[d] = g[t[d]]; uv[d] = u[t[d]]; }
+					float r0 = -4.0f * 0.0f, rx = -4.0f * u[q];
+#pragma unroll
+					for (int d = 0; d < 4; d++) {
+						const bool kn = !isnan(gv[d]);
+						r0 = r0 + (kn ? gv[d] : 0.0f);
+						rx = rx + (kn ? 0.0f : uv[d]);
+					}
+					a0 = -r0;
+					ax = -rx;
+				} else {
+					a0 = op32(1, (int)i, (int)j, P.w, P.h, [&](size_t tt) {
+						const float v = g[tt]; return isnan(v) ? 0.0f : v; });
+					ax = op32(1, (int)i, (int)j, P.w, P.h, [&](size_t tt) {
+						return isnan(g[tt]) ? u[tt] : 0.0f; });
+				}
 				const float bv = (P.f ? -P.f[q] : 0.0f) - a0;
 				const float r = bv - ax;
 				rr += (double)r * r;
@@ -384,11 +405,18 @@ __global__ __launch_bounds__(NT) void k32_finish(const BlockWork *bw,
 {
 	const BlockWork b = bw[blockIdx.x];
 	const Cg32Plane P = pl[b.plane];
-	for (int k_ = 0; k_ < PPT; k_++) {
-		const long qq = pix(b, k_, (size_t)P.w * P.h);
-		if (qq < 0) break;
-		const size_t q = (size_t)qq;
-		if (P.code[q] & 1) P.u[q] = (float)P.x[q];
+	const unsigned n = (unsigned)P.w * (unsigned)P.h;
+	const unsigned q4 = (unsigned)b.start + 4 * threadIdx.x;
+	if (q4 + 3 < n) {
+		const unsigned cc = code4(P.code, q4);
+		if (!(cc & 0x01010101u)) return;
+		// u may be the caller's buffer: 4-byte accesses, known pixels kept
+#pragma unroll
+		for (int k = 0; k < 4; k++)
+			if ((cc >> (8 * k)) & 1u) P.u[q4 + k] = (float)P.x[q4 + k];
+	} else {
+		for (unsigned q = q4; q < n; q++)
+			if (P.code[q] & 1) P.u[q] = (float)P.x[q];
 	}
 }
 
@@ -404,33 +432,27 @@ __global__ __launch_bounds__(NT) void k32_finish(const BlockWork *bw,
 // black then red after it, so that M is symmetric; the coarse correction is
 // scaled by wc.  The coarsest level runs many such sweeps.
 
-// level-0 coefficients from the mask: D = in-image neighbours, w = 1
-// between two unknowns
-static __device__ __forceinline__ bool unk0(const MgLevel &L, int i, int j)
-{
-	return isnan(L.g[(size_t)j * L.w + i]);
-}
-
+// level-0 coefficients from the code byte: D = in-image neighbours, w = 1
+// between two unknowns; levels > 0 from their stored Galerkin coefficients
 static __device__ __forceinline__ float diag_at(const MgLevel &L, int i, int j,
 		bool l0)
 {
-	if (!l0) return L.D[(size_t)j * L.w + i];
-	if (!unk0(L, i, j)) return 0.0f;
-	return (float)((i > 0) + (i + 1 < L.w) + (j > 0) + (j + 1 < L.h));
+	const size_t q = (size_t)j * L.w + i;
+	return l0 ? (float)(L.code[q] >> 5) : L.D[q];   // code 0 at known pixels
 }
 
 static __device__ __forceinline__ float we_at(const MgLevel &L, int i, int j, bool l0)
 {   // coupling between (i,j) and (i+1,j)
 	if (i + 1 >= L.w) return 0.0f;
-	if (!l0) return L.wE[(size_t)j * L.w + i];
-	return (unk0(L, i, j) && unk0(L, i + 1, j)) ? 1.0f : 0.0f;
+	const size_t q = (size_t)j * L.w + i;
+	return l0 ? (float)((L.code[q] >> 1) & 1u) : L.wE[q];
 }
 
 static __device__ __forceinline__ float ws_at(const MgLevel &L, int i, int j, bool l0)
 {   // coupling between (i,j) and (i,j+1)
 	if (j + 1 >= L.h) return 0.0f;
-	if (!l0) return L.wS[(size_t)j * L.w + i];
-	return (unk0(L, i, j) && unk0(L, i, j + 1)) ? 1.0f : 0.0f;
+	const size_t q = (size_t)j * L.w + i;
+	return l0 ? (float)((L.code[q] >> 2) & 1u) : L.wS[q];
 }
 
 // sum of couplings times z over the four neighbours of (i,j)
@@ -456,7 +478,8 @@ __global__ __launch_bounds__(NT) void k_mg_coarsen(const BlockWork *bw,
 		const long qq = pix(b, k_, (size_t)C.w * C.h);
 		if (qq < 0) break;
 		const size_t q = (size_t)qq;
-		const int I = (int)(q % C.w), J = (int)(q / C.w);
+		const unsigned Ju = (unsigned)q / (unsigned)C.w;
+		const int I = (int)((unsigned)q - Ju * (unsigned)C.w), J = (int)Ju;
 		float d = 0, inner = 0, e = 0, sth = 0;
 		for (int c = 0; c < 4; c++) {
 			const int i = 2 * I + (c & 1), j = 2 * J + (c >> 1);
@@ -584,22 +607,27 @@ __global__ __launch_bounds__(NT) void k32_code(const BlockWork *bw,
 {
 	const BlockWork b = bw[blockIdx.x];
 	const Cg32Plane P = pl[b.plane];
-	const unsigned n = (unsigned)P.w * (unsigned)P.h, w = (unsigned)P.w;
-	for (int k_ = 0; k_ < PPT; k_++) {
-		const unsigned q = (unsigned)b.start + k_ * NT + threadIdx.x;
-		if (q >= n) break;
-		const unsigned j = q / w, i = q - j * w;
+	const unsigned n = (unsigned)P.w * (unsigned)P.h, w = (unsigned)P.w,
+		h = (unsigned)P.h;
+	const unsigned q4 = (unsigned)b.start + 4 * threadIdx.x;
+	if (q4 >= n) return;
+	const unsigned q1 = q4 + 4 < n ? q4 + 4 : n;
+	unsigned j = q4 / w, i = q4 - j * w, word = 0;
+	for (unsigned q = q4; q < q1; q++) {
 		unsigned c = 0;
 		if (isnan(P.g[q])) {
 			c = 1;
 			if (i + 1 < w && isnan(P.g[q + 1])) c |= 2;
-			if (j + 1 < (unsigned)P.h && isnan(P.g[q + w])) c |= 4;
+			if (j + 1 < h && isnan(P.g[q + w])) c |= 4;
 			if (i > 0 && isnan(P.g[q - 1])) c |= 8;
 			if (j > 0 && isnan(P.g[q - w])) c |= 16;
-			c |= ((i > 0) + (i + 1 < w) + (j > 0) + (j + 1 < (unsigned)P.h)) << 5;
+			c |= ((i > 0) + (i + 1 < w) + (j > 0) + (j + 1 < h)) << 5;
 		}
-		P.code[q] = (unsigned char)c;
+		if (q1 - q4 == 4) word |= c << (8 * (q - q4));
+		else P.code[q] = (unsigned char)c;
+		if (++i == w) { i = 0; j++; }
 	}
+	if (q1 - q4 == 4) *(unsigned *)(P.code + q4) = word;
 }
 
 // packed coefficients of a level 1..MG_FUSE_MAX (one thread per cell)
