@@ -30,14 +30,15 @@ struct Cg32Scal {
 };
 
 // Fused multigrid tile kernels: levels 0 and 1 smooth nu = 2 times on
-// 120x32 tiles, levels 2..MG_FUSE_MAX nu = 4 times on 112x32 tiles (the halo
-// is 2 nu cells; the region is 128 cells wide either way).  Those levels are
+// 120x56 tiles (512 threads), levels 2..MG_FUSE_MAX nu = 4 times on 112x32
+// tiles (256 threads); the halo is 2 nu cells, the region 128 cells wide.  Those levels are
 // cheap (a sixteenth of the pixels and less), and the stronger smoothing
 // there saves V-cycles (14 -> 11 CG iterations to 5e-6 on 50% rectangle
 // holes); the levels below them (per-level kernels, tail) smooth nu = 2
 // times, where more sweeps change nothing.
 constexpr int MG_NU_FINE = 2, MG_NU_COARSE = 4;
-constexpr int MG_TX_FINE = 120, MG_TX_COARSE = 112, MG_TY = 32;
+constexpr int MG_TX_FINE = 120, MG_TY_FINE = 56, MG_T_FINE = 512;
+constexpr int MG_TX_COARSE = 112, MG_TY_COARSE = 32, MG_T_COARSE = 256;
 
 // one plane at one level of the multigrid preconditioner
 struct MgLevel {

@@ -581,25 +581,23 @@ __global__ __launch_bounds__(NT) void k_mg_prolong(const BlockWork *bw,
 // for m = m0, m0+RG, ...; b and the codes of its cells stay in registers
 // across half sweeps, and a block is also one coarse cell (origins even),
 // so the restriction and the coarse correction are thread-local.
-constexpr int MG_T = 256;
-
-template <int TX_, int TY_, int NU_> struct MgGeo {
-	static constexpr int TX = TX_, TY = TY_, NU = NU_, HALO = 2 * NU;
+template <int TX_, int TY_, int NU_, int T_> struct MgGeo {
+	static constexpr int TX = TX_, TY = TY_, NU = NU_, HALO = 2 * NU, T = T_;
 	static constexpr int RW = TX + 2 * HALO, RH = TY + 2 * HALO;
 	static constexpr int PAIRS_X = RW / 2, ROWPAIRS = RH / 2;
-	static constexpr int RG = MG_T / PAIRS_X, K = ROWPAIRS / RG;
+	static constexpr int RG = T / PAIRS_X, K = ROWPAIRS / RG;
 	// z of the region in LDS with a zero frame: one row above and below and
 	// one column after each row (the column before a row is the previous
 	// row's frame), so the four neighbour reads of every region cell stay
 	// in bounds and read 0 outside the region, with no tests
 	static constexpr int ZS = RW + 1, ZOFF = ZS, ZN = (RH + 2) * ZS;
 	static_assert(TX % 2 == 0 && TY % 2 == 0, "tile origins must be even");
-	static_assert(PAIRS_X * RG == MG_T && RG * K == ROWPAIRS,
+	static_assert(PAIRS_X * RG == T && RG * K == ROWPAIRS,
 			"multigrid tile shape must divide the thread layout");
 	static __device__ __forceinline__ int zid(int r, int col) { return ZOFF + r * ZS + col; }
 };
-using GeoFine = MgGeo<MG_TX_FINE, MG_TY, MG_NU_FINE>;
-using GeoCoarse = MgGeo<MG_TX_COARSE, MG_TY, MG_NU_COARSE>;
+using GeoFine = MgGeo<MG_TX_FINE, MG_TY_FINE, MG_NU_FINE, MG_T_FINE>;
+using GeoCoarse = MgGeo<MG_TX_COARSE, MG_TY_COARSE, MG_NU_COARSE, MG_T_COARSE>;
 
 // level-0 code of every pixel of a CG plane (layout at Cg32Plane::code)
 __global__ __launch_bounds__(NT) void k32_code(const BlockWork *bw,
@@ -672,15 +670,18 @@ struct Lev0 {
 		const unsigned d = k >> 5;
 		return d == 4 ? 0.25f : d == 3 ? (1.0f / 3.0f) : d == 2 ? 0.5f : 0.0f;
 	}
-	// s + the neighbours of the cell at z[id] (E, S, W, N); zs = row pitch
-	static __device__ __forceinline__ float gather(unsigned k, float s,
+	// s + the neighbours of the cell at z[id] (E, S, W, N); zs = row pitch.
+	// A neighbour that is not an in-image unknown (known pixel, outside
+	// the image, the region's frame) holds z = 0 at all times, so adding
+	// all four equals adding the unknown ones (x + 0 = x): no masks needed.
+	static __device__ __forceinline__ float gather(unsigned, float s,
 			const float *z, int id, int zs)
 	{
 		const float e = z[id + 1], so = z[id + zs], w = z[id - 1], n = z[id - zs];
-		s += (k & 2) ? e : 0.0f;
-		s += (k & 4) ? so : 0.0f;
-		s += (k & 8) ? w : 0.0f;
-		s += (k & 16) ? n : 0.0f;
+		s += e;
+		s += so;
+		s += w;
+		s += n;
 		return s;
 	}
 };
@@ -776,7 +777,7 @@ static __device__ __forceinline__ long row_q(const MgSpan &S, int kb, int hr)
 template <class G>
 static __device__ __forceinline__ void z_frame(float *z)
 {
-	for (int e = threadIdx.x; e < 2 * G::ZS + G::RH; e += MG_T) {
+	for (int e = threadIdx.x; e < 2 * G::ZS + G::RH; e += G::T) {
 		const int at = e < G::ZS ? e : e < 2 * G::ZS ? G::ZOFF + G::RH * G::ZS + (e - G::ZS)
 			: G::ZOFF + (e - 2 * G::ZS) * G::ZS + G::RW;
 		z[at] = 0.0f;
@@ -894,7 +895,7 @@ static __device__ __forceinline__ bool in_tile(const MgCells<P, G> &C, int kb)
 
 // pre-smoothing from z = 0, residual, 2x2 restriction into the next level
 template <class P, class G>
-__global__ __launch_bounds__(MG_T) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(4)))
 void k_mg_down(const int4 *tiles,
 		const MgLevel *lv, const MgLevel *coarse, const Cg32Scal *sc)
 {
@@ -936,12 +937,12 @@ void k_mg_down(const int4 *tiles,
 // coarse correction, post-smoothing; at level 0 also the partial r.z of
 // the tile
 template <class P, class G, bool DOT>
-__global__ __launch_bounds__(MG_T) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(4)))
 void k_mg_up(const int4 *tiles, const MgLevel *lv,
 		const MgLevel *coarse, const Cg32Scal *sc, float wc, double *tpart)
 {
 	__shared__ float z[G::ZN];
-	__shared__ double red[MG_T / 64];
+	__shared__ double red[G::T / 64];
 	const int4 t = tiles[blockIdx.x];
 	if (!sc[t.x].active) return;
 	const MgLevel L = lv[t.x], Cc = coarse[t.x];
@@ -976,7 +977,7 @@ void k_mg_up(const int4 *tiles, const MgLevel *lv,
 		}
 	}
 	if (DOT) {
-		const double sum = bsum<MG_T>(acc, red);
+		const double sum = bsum<G::T>(acc, red);
 		if (threadIdx.x == 0) tpart[blockIdx.x] = sum;
 	}
 }
@@ -1098,7 +1099,7 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 	// z and a quarter coarse b (10 B); up reads z, b, the code and a quarter
 	// coarse z, writes z (14 B)
 	if (B.prof) B.prof("mg_down0", 1, 0, s);
-	k_mg_down<Lev0, GeoFine><<<B.ntiles0, MG_T, 0, s>>>(B.tiles0, B.mg[0].lv,
+	k_mg_down<Lev0, GeoFine><<<B.ntiles0, GeoFine::T, 0, s>>>(B.tiles0, B.mg[0].lv,
 			B.mg[0].coarse, B.scal);
 	if (B.prof) B.prof("mg_down0", 0, 10.0 * B.px0, s);
 	// levels >= 1 outside the tail: fused tiles (packed coefficients; per
@@ -1110,10 +1111,10 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 		if (W.ntiles) {
 			if (B.prof) B.prof(g_down_name[l], 1, 0, s);
 			if (l == 1)
-				k_mg_down<LevG, GeoFine><<<W.ntiles, MG_T, 0, s>>>(W.tiles, W.lv,
+				k_mg_down<LevG, GeoFine><<<W.ntiles, GeoFine::T, 0, s>>>(W.tiles, W.lv,
 						W.coarse, B.scal);
 			else
-				k_mg_down<LevG, GeoCoarse><<<W.ntiles, MG_T, 0, s>>>(W.tiles, W.lv,
+				k_mg_down<LevG, GeoCoarse><<<W.ntiles, GeoCoarse::T, 0, s>>>(W.tiles, W.lv,
 						W.coarse, B.scal);
 			if (B.prof) B.prof(g_down_name[l], 0, 14.0 * W.cells, s);
 		}
@@ -1142,16 +1143,16 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 		if (W.ntiles) {
 			if (B.prof) B.prof(g_up_name[l], 1, 0, s);
 			if (l == 1)
-				k_mg_up<LevG, GeoFine, false><<<W.ntiles, MG_T, 0, s>>>(W.tiles,
+				k_mg_up<LevG, GeoFine, false><<<W.ntiles, GeoFine::T, 0, s>>>(W.tiles,
 						W.lv, W.coarse, B.scal, B.wc, nullptr);
 			else
-				k_mg_up<LevG, GeoCoarse, false><<<W.ntiles, MG_T, 0, s>>>(W.tiles,
+				k_mg_up<LevG, GeoCoarse, false><<<W.ntiles, GeoCoarse::T, 0, s>>>(W.tiles,
 						W.lv, W.coarse, B.scal, B.wc, nullptr);
 			if (B.prof) B.prof(g_up_name[l], 0, 17.0 * W.cells, s);
 		}
 	}
 	if (B.prof) B.prof("mg_up0", 1, 0, s);
-	k_mg_up<Lev0, GeoFine, true><<<B.ntiles0, MG_T, 0, s>>>(B.tiles0, B.mg[0].lv,
+	k_mg_up<Lev0, GeoFine, true><<<B.ntiles0, GeoFine::T, 0, s>>>(B.tiles0, B.mg[0].lv,
 			B.mg[0].coarse, B.scal, B.wc, B.tpart);
 	if (B.prof) B.prof("mg_up0", 0, 14.0 * B.px0, s);
 }

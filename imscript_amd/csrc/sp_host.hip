@@ -791,7 +791,7 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 				L.b = cp[z].r;
 				L.code = cp[z].code;
 				cp[z].t0 = (int)tiles0.size();
-				for (int y0 = 0; y0 < L.h; y0 += MG_TY)
+				for (int y0 = 0; y0 < L.h; y0 += MG_TY_FINE)
 					for (int x0 = 0; x0 < L.w; x0 += MG_TX_FINE)
 						tiles0.push_back(make_int4(z, x0, y0, 0));
 				cp[z].nt = (int)tiles0.size() - cp[z].t0;
@@ -829,7 +829,8 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 			if (l < 1 || l >= rng[z].x) continue;   // level 0 or in the tail
 			if (l <= MG_FUSE_MAX) {   // fused tiles
 				const int tx = l == 1 ? MG_TX_FINE : MG_TX_COARSE;
-				for (int y0 = 0; y0 < Lz.h; y0 += MG_TY)
+				const int ty = l == 1 ? MG_TY_FINE : MG_TY_COARSE;
+				for (int y0 = 0; y0 < Lz.h; y0 += ty)
 					for (int x0 = 0; x0 < Lz.w; x0 += tx)
 						tl.push_back(make_int4(z, x0, y0, 0));
 				tcells += (double)n;
