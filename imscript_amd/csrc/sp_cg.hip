@@ -921,9 +921,10 @@ void k_mg_down(const int4 *tiles,
 		for (int c = 0; c < 4; c++) {
 			const unsigned k = P::get(C.k[kb], c);
 			const int id = cell_z(C, kb, c);
-			// inactive cells add b - A z = 0 - 0
+			// an inactive cell adds nothing (its unmasked neighbour sum need
+			// not be 0)
 			const float az = P::diag(k) * z[id] - P::gather(k, 0.0f, z, id, G::ZS);
-			acc += C.b[kb][c] - az;
+			acc += P::mask(k) * (C.b[kb][c] - az);
 		}
 		const int I = (ox >> 1) + C.pc, J = (oy >> 1) + C.m0 + G::RG * kb;
 		if (I < Cc.w && J < Cc.h) {
