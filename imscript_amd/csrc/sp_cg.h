@@ -73,6 +73,8 @@ struct MgLevelWork {
 constexpr int MG_MAXLEV = 16;
 constexpr int MG_TAIL_CELLS = 4096;    // first level at or below this size
 constexpr int MG_TAIL_T = 1024;
+// LDS budget of the tail kernel: z, b, D, wE, wS of every tail level
+constexpr int MG_TAIL_LDS_MAX = 155000;
 
 // optional timer around individual launches (profiling runs only; never
 // set while a graph is being captured)
@@ -91,7 +93,9 @@ struct Cg32Batch {
 	int nlev, nu, coarse_sweeps;   // nu: per-level kernels and the tail
 	float wc;          // coarse-correction factor
 	MgLevel *tail_lv;  // [plane * MG_MAXLEV + level]
-	int2 *tail_rng;    // per plane: (first tail level, depth)
+	int4 *tail_rng;    // per plane: (first tail level, depth, runs in LDS,
+	                   // tail cells of all its levels)
+	int tail_lds;      // dynamic LDS bytes of the LDS tail kernel (0: none)
 	int4 *tiles0;      // level-0 tiles (plane, x0, y0, -), contiguous per plane
 	int ntiles0;
 	double *tpart;     // one partial per level-0 tile
@@ -104,4 +108,6 @@ void cg32_setup(const Cg32Batch &B, hipStream_t s);
 void cg32_iter(const Cg32Batch &B, hipStream_t s);
 void cg32_finish(const Cg32Batch &B, hipStream_t s);
 void mg_build(const Cg32Batch &B, hipStream_t s);
+// allow `bytes` of dynamic LDS for the LDS tail kernel; 0 on success
+int mg_tail_lds_enable(int bytes);
 }

@@ -1057,10 +1057,10 @@ static __device__ void tail_prolong(const MgLevel &F, const MgLevel &C, float wc
 }
 
 __global__ __launch_bounds__(MG_TAIL_T) void k_mg_tail(const MgLevel *lvall,
-		const int2 *rng, const Cg32Scal *sc, int nu, int csw, float wc)
+		const int4 *rng, const Cg32Scal *sc, int nu, int csw, float wc)
 {
 	const int z = blockIdx.x;
-	if (!sc[z].active) return;
+	if (!sc[z].active || rng[z].z) return;   // planes of the LDS tail skip
 	const int l0 = rng[z].x, d = rng[z].y;
 	const MgLevel *lv = lvall + (size_t)z * MG_MAXLEV;
 	for (int l = l0; l + 1 < d; l++) {
@@ -1085,6 +1085,135 @@ __global__ __launch_bounds__(MG_TAIL_T) void k_mg_tail(const MgLevel *lvall,
 			tail_smooth(lv[l], 0, false);
 		}
 	}
+}
+
+// The same tail with every level's z, b, D, wE, wS in LDS (planes whose tail
+// fits MG_TAIL_LDS_MAX): one LDS round trip and a barrier per phase instead
+// of dependent L2 accesses.  Same operations in the same order as above.
+struct TailLv {
+	float *z, *b, *D, *wE, *wS;
+	int w, h;
+};
+
+static __device__ float tl_off(const TailLv &L, int q, int i, int j)
+{
+	float s = 0.0f;
+	if (i + 1 < L.w) s += L.wE[q] * L.z[q + 1];
+	if (i > 0) s += L.wE[q - 1] * L.z[q - 1];
+	if (j + 1 < L.h) s += L.wS[q] * L.z[q + L.w];
+	if (j > 0) s += L.wS[q - L.w] * L.z[q - L.w];
+	return s;
+}
+
+static __device__ void tl_smooth(const TailLv &L, int color, bool first)
+{
+	const int n = L.w * L.h;
+	for (int q = threadIdx.x; q < n; q += blockDim.x) {
+		const int j = q / L.w, i = q - j * L.w;
+		if (((i + j) & 1) != color) {
+			if (first) L.z[q] = 0.0f;
+			continue;
+		}
+		const float d = L.D[q];
+		if (d == 0.0f) continue;
+		L.z[q] = (first ? L.b[q] : L.b[q] + tl_off(L, q, i, j)) / d;
+	}
+	__syncthreads();
+}
+
+static __device__ void tl_restrict(const TailLv &F, const TailLv &C)
+{
+	const int n = C.w * C.h;
+	for (int q = threadIdx.x; q < n; q += blockDim.x) {
+		const int J = q / C.w, I = q - J * C.w;
+		float r = 0;
+		for (int c = 0; c < 4; c++) {
+			const int i = 2 * I + (c & 1), j = 2 * J + (c >> 1);
+			if (i >= F.w || j >= F.h) continue;
+			const int f = j * F.w + i;
+			const float d = F.D[f];
+			if (d == 0.0f) continue;
+			r += F.b[f] - (d * F.z[f] - tl_off(F, f, i, j));
+		}
+		C.b[q] = r;
+		C.z[q] = 0.0f;
+	}
+	__syncthreads();
+}
+
+static __device__ void tl_prolong(const TailLv &F, const TailLv &C, float wc)
+{
+	const int n = F.w * F.h;
+	for (int q = threadIdx.x; q < n; q += blockDim.x) {
+		if (F.D[q] == 0.0f) continue;
+		const int j = q / F.w, i = q - j * F.w;
+		F.z[q] += wc * C.z[(j >> 1) * C.w + (i >> 1)];
+	}
+	__syncthreads();
+}
+
+__global__ __launch_bounds__(MG_TAIL_T) void k_mg_tail_lds(const MgLevel *lvall,
+		const int4 *rng, const Cg32Scal *sc, int nu, int csw, float wc)
+{
+	extern __shared__ float tsm[];
+	__shared__ int toff[MG_MAXLEV + 1], tw[MG_MAXLEV], th[MG_MAXLEV];
+	const int z = blockIdx.x;
+	if (!sc[z].active || !rng[z].z) return;
+	const int l0 = rng[z].x, d = rng[z].y;
+	const MgLevel *lv = lvall + (size_t)z * MG_MAXLEV;
+	if (threadIdx.x == 0) {
+		int o = 0;
+		for (int l = l0; l < d; l++) {
+			toff[l] = o;
+			tw[l] = lv[l].w;
+			th[l] = lv[l].h;
+			o += 5 * lv[l].w * lv[l].h;
+		}
+	}
+	__syncthreads();
+	// level l's arrays (z, b, D, wE, wS) from the offsets, no per-lane copies
+	auto T = [&](int l) {
+		const int n = tw[l] * th[l];
+		float *p = tsm + toff[l];
+		return TailLv{p, p + n, p + 2 * n, p + 3 * n, p + 4 * n, tw[l], th[l]};
+	};
+	for (int l = l0; l < d; l++) {
+		const TailLv L = T(l);
+		const int n = L.w * L.h;
+		for (int q = threadIdx.x; q < n; q += blockDim.x) {
+			L.D[q] = lv[l].D[q];
+			L.wE[q] = lv[l].wE[q];
+			L.wS[q] = lv[l].wS[q];
+			if (l == l0) L.b[q] = lv[l].b[q];
+		}
+	}
+	__syncthreads();
+	for (int l = l0; l + 1 < d; l++) {
+		for (int it = 0; it < nu; it++) {
+			tl_smooth(T(l), 0, it == 0);
+			tl_smooth(T(l), 1, false);
+		}
+		tl_restrict(T(l), T(l + 1));
+	}
+	for (int it = 0; it < csw; it++) {
+		tl_smooth(T(d - 1), 0, it == 0);
+		tl_smooth(T(d - 1), 1, false);
+	}
+	for (int it = 0; it < csw; it++) {
+		tl_smooth(T(d - 1), 1, false);
+		tl_smooth(T(d - 1), 0, false);
+	}
+	for (int l = d - 2; l >= l0; l--) {
+		tl_prolong(T(l), T(l + 1), wc);
+		for (int it = 0; it < nu; it++) {
+			tl_smooth(T(l), 1, false);
+			tl_smooth(T(l), 0, false);
+		}
+	}
+	// the first tail level's correction is read by the next finer level
+	const TailLv L0 = T(l0);
+	const int n0 = L0.w * L0.h;
+	for (int q = threadIdx.x; q < n0; q += blockDim.x) lv[l0].z[q] = L0.z[q];
 }
 
 static const char *const g_down_name[MG_FUSE_MAX + 1] = {
@@ -1130,6 +1259,9 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 	if (B.prof) B.prof("mg_tail", 1, 0, s);
 	k_mg_tail<<<B.nplanes, MG_TAIL_T, 0, s>>>(B.tail_lv, B.tail_rng, B.scal, B.nu,
 			B.coarse_sweeps, B.wc);
+	if (B.tail_lds)
+		k_mg_tail_lds<<<B.nplanes, MG_TAIL_T, B.tail_lds, s>>>(B.tail_lv, B.tail_rng,
+				B.scal, B.nu, B.coarse_sweeps, B.wc);
 	if (B.prof) B.prof("mg_tail", 0, 0.0, s);
 	// up: correct, post-smooth in the mirrored order
 	for (int l = L - 1; l >= 1; l--) {
@@ -1165,6 +1297,12 @@ static void precondition(const Cg32Batch &B, int setup, hipStream_t s)
 	k_mg_reduce_t<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.tpart, setup);
 	if (setup)   // p = z + 0 * p
 		k32_dir<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal);
+}
+
+int mg_tail_lds_enable(int bytes)
+{
+	return hipFuncSetAttribute((const void *)k_mg_tail_lds,
+			hipFuncAttributeMaxDynamicSharedMemorySize, bytes) == hipSuccess ? 0 : -1;
 }
 
 void mg_build(const Cg32Batch &B, hipStream_t s)

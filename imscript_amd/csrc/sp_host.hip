@@ -88,6 +88,11 @@ extern "C" void sp_params_default(sp_params *p)
 // ------------------------------------------------------------- device
 
 static int g_dev_checked = 0, g_dev_ok = 0, g_max_lds = 65536;
+// SIMPOIS_HIP_TAIL_LDS=0 keeps every multigrid tail in global memory
+static const int g_tail_lds_off = [] {
+	const char *e = getenv("SIMPOIS_HIP_TAIL_LDS");
+	return e && !strcmp(e, "0") ? 1 : 0;
+}();
 
 static int ensure_device()
 {
@@ -806,13 +811,26 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 			L.b = pool + off; off += n;
 		}
 	// first level of each plane handled by the single-work-group tail
-	std::vector<int2> rng(np);
+	// (planes whose tail levels fit in LDS run the LDS version of it)
+	std::vector<int4> rng(np);
+	int tail_lds = 0;
 	for (int z = 0; z < np; z++) {
 		const int d = (int)shp[z].size();
 		int t = d - 1;
 		for (int l = 1; l < d; l++)
 			if ((long)shp[z][l].first * shp[z][l].second <= MG_TAIL_CELLS) { t = l; break; }
-		rng[z] = make_int2(std::max(1, std::min(t, d - 1)), d);
+		t = std::max(1, std::min(t, d - 1));
+		long cells = 0;
+		for (int l = t; l < d; l++) cells += (long)shp[z][l].first * shp[z][l].second;
+		const long bytes = 20 * cells;
+		const int in_lds = bytes <= MG_TAIL_LDS_MAX && !g_tail_lds_off;
+		if (in_lds) tail_lds = std::max<int>(tail_lds, (int)bytes);
+		rng[z] = make_int4(t, d, in_lds, (int)cells);
+	}
+	if (tail_lds > 65536 && spk::mg_tail_lds_enable(tail_lds)) {
+		// the device refused the LDS size: every plane takes the global tail
+		for (auto &r : rng) r.z = 0;
+		tail_lds = 0;
 	}
 	mgw.assign(nlev, MgLevelWork{});
 	for (int l = 0; l < nlev; l++) {
@@ -858,6 +876,7 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 		for (size_t l = 0; l < shp[z].size(); l++) all[(size_t)z * MG_MAXLEV + l] = lv[l][z];
 	TRY(pl.upload(&B.tail_lv, all));
 	TRY(pl.upload(&B.tail_rng, rng));
+	B.tail_lds = tail_lds;
 	TRY(pl.upload(&B.tiles0, tiles0));
 	B.ntiles0 = (int)tiles0.size();
 	TRY(pl.alloc(&B.tpart, tiles0.size()));

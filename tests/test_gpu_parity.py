@@ -258,6 +258,32 @@ def test_multigrid_batch_of_uneven_planes():
             assert rel[l] <= 2e-6, (tuple(g.shape), rel)
 
 
+def test_multigrid_tail_in_lds_and_in_global_memory():
+    """Square-ish planes run their coarsest multigrid levels in one LDS-
+    resident work-group; a 2x16384 plane's tail (8192 cells, over the LDS
+    budget) runs from global memory.  Both meet the residual target in one
+    batch, alone or together."""
+    import torch
+    m = _imscript()
+    shapes = [(1, 2, 16384), (1, 300, 257), (2, 97, 61)]
+    gs, fs = [], []
+    for k, (pd, h, w) in enumerate(shapes):
+        g, f = random_case(70 + k, pd, h, w, 0.4, True)
+        gs.append(torch.from_numpy(g).cuda())
+        fs.append(torch.from_numpy(f).cuda())
+    print("mg tail lds/global", shapes, flush=True)
+    kw = dict(niter=0, scale=1, cgit=-1, cg_fp32=1, cg_rel=1e-6, cg_mg=1)
+    outs, _ = m.solve_batch_device(gs, fs, **kw)
+    for g, f, o in zip(gs, fs, outs):
+        rel = m.residual_device(o, g, f)
+        assert max(rel) <= 2e-6, (tuple(g.shape), rel)
+    # planes of a batch do not interact: each alone gives the same bits
+    for g, f, o in zip(gs, fs, outs):
+        alone, _ = m.solve_batch_device([g], [f], **kw)
+        torch.cuda.synchronize()
+        assert torch.equal(alone[0].view(torch.int32), o.view(torch.int32)), tuple(g.shape)
+
+
 @pytest.mark.parametrize("tstep,niter", [(0.25, 40), (-0.08, 12), (-1000.05, 6)])
 def test_autotuned_tile_shapes_match_oracle(tstep, niter):
     """The autotuner times every compiled tile shape on a scratch copy and
