@@ -1184,6 +1184,9 @@ __global__ __launch_bounds__(MG_TAIL_T) void k_mg_tail_lds(const MgLevel *lvall,
 			L.D[q] = lv[l].D[q];
 			L.wE[q] = lv[l].wE[q];
 			L.wS[q] = lv[l].wS[q];
+			// the first sweep writes no inactive cell of its colour: start
+			// from z = 0 as the global-memory tail does
+			L.z[q] = 0.0f;
 			if (l == l0) L.b[q] = lv[l].b[q];
 		}
 	}

@@ -276,7 +276,10 @@ def test_multigrid_tail_in_lds_and_in_global_memory():
     outs, _ = m.solve_batch_device(gs, fs, **kw)
     for g, f, o in zip(gs, fs, outs):
         rel = m.residual_device(o, g, f)
-        assert max(rel) <= 2e-6, (tuple(g.shape), rel)
+        # 1e-5, the solver's target: rounding the solution to float32 alone
+        # leaves up to ~4e-6 here (|u| ~ 100 against ||b|| ~ 2 per pixel),
+        # whatever the CG tolerance
+        assert max(rel) <= 1e-5, (tuple(g.shape), rel)
     # planes of a batch do not interact: each alone gives the same bits
     for g, f, o in zip(gs, fs, outs):
         alone, _ = m.solve_batch_device([g], [f], **kw)
