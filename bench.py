@@ -41,8 +41,8 @@ PEAK_HBM_GBPS = 8000.0
 # counter reads for k_stream_copy moving 1 GiB each way in that run (FETCH_SIZE
 # x2.000, WRITE_SIZE x1.000).  Summary and commands: profiles/hbm_traffic_*.md.
 TRAFFIC_PMC = {
-    "mg_up0": 910.1e6,
-    "mg_down0": 640.9e6,
+    "mg_up0": 869.7e6,
+    "mg_down0": 624.2e6,
 }
 
 
