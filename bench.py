@@ -179,6 +179,104 @@ def fp64_check(torch, sp, params, seed, side=1024):
             "rel_residual": sp.residual_device(out, g, f)[0], "fp64_solve_s": round(dt, 2)}
 
 
+# --------------------------------------------------- CPU legs of C1, C3-C5
+#
+# The oracle, the reference binary and the FP64 scipy solves are used here,
+# on the host, as checkers and timed baselines only; --no-cpu skips them.
+
+def _oracle_imports():
+    for d in ("tests", "oracle"):
+        if os.path.join(ROOT, d) not in sys.path:
+            sys.path.insert(0, os.path.join(ROOT, d))
+    from helpers import oracle_solve, bits_equal
+    import adjoint_ref as ar
+    return oracle_solve, bits_equal, ar
+
+
+def cpu_leg_c1(g, got):
+    """C1 against the oracle port (bit-exact, timed), the reference binary
+    (bit-exact, timed) and an FP64 direct solve (error)."""
+    oracle_solve, bits_equal, ar = _oracle_imports()
+    t1 = time.perf_counter()
+    want, _ = oracle_solve(g, niter=2000)
+    res = {"bit_exact_vs_oracle": bits_equal(got, want),
+           "oracle_port_s": round(time.perf_counter() - t1, 3),
+           "max_abs_err_vs_fp64_solve": float(np.max(np.abs(got - ar.solve(g))))}
+    ref = os.path.join(ROOT, "oracle", "_ref", "simpois")
+    if os.path.exists(ref):
+        with tempfile.TemporaryDirectory() as td:
+            np.save(os.path.join(td, "g.npy"), g)
+            t1 = time.perf_counter()
+            subprocess.run([ref, "-i", os.path.join(td, "g.npy"), "-n", "2000", "-o",
+                            os.path.join(td, "o.npy")], check=True)
+            res["reference_cpu_s"] = round(time.perf_counter() - t1, 3)
+            res["reference_bitexact"] = bits_equal(
+                np.load(os.path.join(td, "o.npy"))[..., 0], got)
+    return res
+
+
+def cpu_leg_c3(torch, sp, a, side=512):
+    """Error of the C3 method against an FP64 direct solve of the 13-point
+    system on a side x side instance of the same generator."""
+    _, _, ar = _oracle_imports()
+    rng = np.random.default_rng(a.seed + 33)
+    g = grf(rng, side)
+    g[stripes(rng, side, 0.2)] = np.nan
+    dg = torch.from_numpy(g).cuda()
+    p = dict(tstep=-0.08, niter=0, scale=99, cgit=-1, cg_eps=0.9e-6 * bnorm_bilap(dg),
+             trace=0)
+    out, _ = sp.solve_device(dg, **p)
+    u = out.cpu().numpy().astype(np.float64)
+    t0 = time.perf_counter()
+    ex = ar.solve_bilap(g)
+    hole = np.isnan(g)
+    err = np.abs(u - ex)[hole]
+    return {"instance": f"{side}x{side} of the same generator (seed {a.seed + 33})",
+            "max_abs_err": float(err.max()), "rel_max_err": float(err.max() / np.abs(ex).max()),
+            "rel_residual": sp.residual_device(out, dg, bilap=True)[0],
+            "fp64_solve_s": round(time.perf_counter() - t0, 2)}
+
+
+def cpu_leg_c4(gs, outs, k=8):
+    """Error of the batched results against FP64 direct solves of the k
+    smallest images."""
+    _, _, ar = _oracle_imports()
+    order = sorted(range(len(gs)), key=lambda i: gs[i].numel())[:k]
+    errs = []
+    for i in order:
+        g = gs[i].cpu().numpy()
+        ex = ar.solve(g)
+        hole = np.isnan(g)
+        e = np.abs(outs[i].cpu().numpy().astype(np.float64) - ex)[hole]
+        errs.append(float(e.max() / np.abs(ex).max()) if e.size else 0.0)
+    return {"images": f"the {k} smallest of the batch", "rel_max_err_max": max(errs),
+            "rel_max_err_each": [round(e, 9) for e in errs]}
+
+
+def cpu_leg_c5(torch, poisson_fill, dev):
+    """Forward fill and gradient on a small instance against the FP64
+    reference (direct solve and adjoint formulas)."""
+    _, _, ar = _oracle_imports()
+    rs = np.random.default_rng(9)
+    gs = sinusoids_np(rs, 48, 56).astype(np.float64)
+    gs[10:30, 12:40] = np.nan
+    fs, ts = rs.standard_normal(gs.shape), rs.random(gs.shape) * 255
+    dgs = torch.tensor(gs, dtype=torch.float32, device=dev).requires_grad_(True)
+    dfs = torch.tensor(fs, dtype=torch.float32, device=dev).requires_grad_(True)
+    u = poisson_fill(dgs, dfs, tol=1e-7)
+    hs = torch.isnan(dgs.detach())
+    ((u - torch.tensor(ts, dtype=torch.float32, device=dev)) ** 2)[hs].sum().backward()
+    uref = ar.solve(gs, fs)
+    v = np.where(np.isnan(gs), 2 * (uref - ts), 0)
+    rg, rf = ar.gradient(gs, fs, v)
+    uh = u.detach().cpu().numpy().astype(np.float64)
+    return {"fwd_rel_err_vs_fp64": float(np.max(np.abs(uh - uref)) / np.max(np.abs(uref))),
+            "grad_rel_err_vs_fp64_g": float(np.max(np.abs(dgs.grad.cpu().numpy() - rg))
+                                            / np.max(np.abs(rg))),
+            "grad_rel_err_vs_fp64_f": float(np.max(np.abs(dfs.grad.cpu().numpy() - rf))
+                                            / np.max(np.abs(rf)))}
+
+
 # ------------------------------------------------------------ other configs
 
 def emit(d):
@@ -200,10 +298,6 @@ def run_c1(a):
     Gauss-Seidel on every pyramid level)."""
     import torch
     import imscript_amd as sp
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from helpers import oracle_solve, bits_equal
-    import adjoint_ref as ar
     rng = np.random.default_rng(a.seed)
     n = 512
     g = sinusoids_np(rng, n, n)
@@ -235,27 +329,12 @@ def run_c1(a):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / a.steps
     got = out.cpu().numpy()
-    t1 = time.perf_counter()
-    want, _ = oracle_solve(g, niter=2000)
-    t_orc = time.perf_counter() - t1
-    exact = ar.solve(g)
     res = {"config": "c1", "workload": "512x512 Laplace, 30% disc holes, simpois -n 2000",
            "hole_fraction": round(float(m.mean()), 4), "gpu_ms": round(dt * 1e3, 2),
            "gpu_Mpix_s": round(n * n / dt / 1e6, 3),
-           "bit_exact_vs_oracle": bits_equal(got, want),
-           "oracle_port_s": round(t_orc, 3),
-           "max_abs_err_vs_fp64_solve": float(np.max(np.abs(got - exact))),
            "rel_residual": sp.residual_device(out, dg)[0]}
-    ref = os.path.join(ROOT, "oracle", "_ref", "simpois")
-    if os.path.exists(ref):
-        with tempfile.TemporaryDirectory() as td:
-            np.save(os.path.join(td, "g.npy"), g)
-            t1 = time.perf_counter()
-            subprocess.run([ref, "-i", os.path.join(td, "g.npy"), "-n", "2000", "-o",
-                            os.path.join(td, "o.npy")], check=True)
-            res["reference_cpu_s"] = round(time.perf_counter() - t1, 3)
-            res["reference_bitexact"] = bits_equal(
-                np.load(os.path.join(td, "o.npy"))[..., 0], got)
+    if not a.no_cpu:
+        res.update(cpu_leg_c1(g, got))
     emit(res)
 
 
@@ -305,15 +384,20 @@ def run_c3(a):
     # the CG stops on its own recursive residual evaluated with the float
     # operator; a 10% margin keeps the FP64 residual under 1e-6
     p = dict(tstep=-0.08, niter=0, scale=99, cgit=-1, cg_eps=0.9e-6 * bn, trace=0)
-    t0 = time.perf_counter()
-    _, st = sp.solve_device(dg, out=out, **p)
+    sp.solve_device(dg, out=out, **p)   # warmup (plan)
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        _, st = sp.solve_device(dg, out=out, **p)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps
     rel = sp.residual_device(out, dg, bilap=True)[0]
     res = {"config": "c3", "workload": f"{n}x{n} biharmonic, GRF k^-3, 20% stripes, "
            "FP64 CG at every level to 1e-6*||b||", "hole_fraction": round(float(m.mean()), 4),
-           "gpu_s": round(dt, 3), "gpu_Mpix_s": round(n * n / dt / 1e6, 3),
+           "gpu_s": round(dt, 4), "gpu_Mpix_s": round(n * n / dt / 1e6, 3),
            "cg_iters_finest": st.cg_iters_finest, "rel_residual": rel}
+    if not a.no_cpu:
+        res["fp64_check"] = cpu_leg_c3(torch, sp, a)
     emit(res)
 
 
@@ -394,24 +478,26 @@ def run_c4(a):
     torch.cuda.synchronize()
     tl = time.perf_counter() - t0
     rell = [sp.residual_device(o, g)[0] for o, g in zip(outs, gs)]
-    emit({"config": "c4", "workload": f"{len(gs)} images, {mpix:.1f} Mpx total, "
-          "log-uniform 128^2-2048^2, aspect 1-4, 10-60% holes, rel residual 1e-5",
-          "method": "multigrid-preconditioned FP32 CG per image",
-          "batched_s": round(tb, 4), "batched_Mpix_s": round(mpix / tb, 1),
-          "loop_s": round(tl, 4), "loop_Mpix_s": round(mpix / tl, 1),
-          "speedup_batched_vs_loop": round(tl / tb, 2),
-          "max_rel_residual_batched": max(relb), "max_rel_residual_loop": max(rell),
-          "hole_fraction_mean": round(float(np.mean(fr)), 3)})
+    res = {"config": "c4", "workload": f"{len(gs)} images, {mpix:.1f} Mpx total, "
+           "log-uniform 128^2-2048^2, aspect 1-4, 10-60% holes, rel residual 1e-5",
+           "method": "multigrid-preconditioned FP32 CG per image",
+           "batched_s": round(tb, 4), "batched_Mpix_s": round(mpix / tb, 1),
+           "loop_s": round(tl, 4), "loop_Mpix_s": round(mpix / tl, 1),
+           "speedup_batched_vs_loop": round(tl / tb, 2),
+           "max_rel_residual_batched": max(relb), "max_rel_residual_loop": max(rell),
+           "hole_fraction_mean": round(float(np.mean(fr)), 3)}
+    if not a.no_cpu:
+        sp.solve_batch_device(gs, outs=outs, **p)   # the batched results again
+        torch.cuda.synchronize()
+        res["fp64_check"] = cpu_leg_c4(gs, outs)
+    emit(res)
 
 
 def run_c5(a):
     """Batch 64 x 1024x1024, 35% disc holes: forward Poisson fill, loss on
     the filled pixels, backward by the adjoint solve."""
     import torch
-    import imscript_amd as sp
     from imscript_amd.autograd import poisson_fill
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import adjoint_ref as ar
     rng = np.random.default_rng(a.seed + 5)
     B, n = a.c5_batch, a.c5_size
     dev = "cuda:0"
@@ -452,25 +538,12 @@ def run_c5(a):
         step()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / a.steps
-    # gradient check on a small instance against the FP64 reference
-    rs = np.random.default_rng(9)
-    gs = sinusoids_np(rs, 48, 56).astype(np.float64)
-    gs[10:30, 12:40] = np.nan
-    fs, ts = rs.standard_normal(gs.shape), rs.random(gs.shape) * 255
-    dgs = torch.tensor(gs, dtype=torch.float32, device=dev).requires_grad_(True)
-    dfs = torch.tensor(fs, dtype=torch.float32, device=dev).requires_grad_(True)
-    u = poisson_fill(dgs, dfs, tol=1e-7)
-    hs = torch.isnan(dgs.detach())
-    ((u - torch.tensor(ts, dtype=torch.float32, device=dev)) ** 2)[hs].sum().backward()
-    uref = ar.solve(gs, fs)
-    v = np.where(np.isnan(gs), 2 * (uref - ts), 0)
-    rg, rf = ar.gradient(gs, fs, v)
-    eg = np.max(np.abs(dgs.grad.cpu().numpy() - rg)) / np.max(np.abs(rg))
-    ef = np.max(np.abs(dfs.grad.cpu().numpy() - rf)) / np.max(np.abs(rf))
-    emit({"config": "c5", "workload": f"batch {B} x {n}x{n}, 35% disc holes, "
-          "forward fill + loss + adjoint backward",
-          "step_s": round(dt, 4), "Mpix_s_fwd_bwd": round(B * n * n / dt / 1e6, 1),
-          "grad_rel_err_vs_fp64_g": float(eg), "grad_rel_err_vs_fp64_f": float(ef)})
+    res = {"config": "c5", "workload": f"batch {B} x {n}x{n}, 35% disc holes, "
+           "forward fill + loss + adjoint backward",
+           "step_s": round(dt, 4), "Mpix_s_fwd_bwd": round(B * n * n / dt / 1e6, 1)}
+    if not a.no_cpu:
+        res.update(cpu_leg_c5(torch, poisson_fill, dev))
+    emit(res)
 
 
 def stream_copy_ceiling(torch, sp, nbytes=1 << 30, reps=10):

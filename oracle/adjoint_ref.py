@@ -15,6 +15,11 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 _ST5 = [(0, 0, -4.0), (1, 0, 1.0), (0, 1, 1.0), (-1, 0, 1.0), (0, -1, 1.0)]
+# the reference's 13-point bilaplacian (cleant_cgpois.c:91-121): 20 c - 8 (4
+# neighbours) + 2 (diagonals) + (distance-2 points), every read clamped
+_ST13 = [(0, 0, 20.0), (1, 0, -8.0), (0, 1, -8.0), (-1, 0, -8.0), (0, -1, -8.0),
+         (1, 1, 2.0), (1, -1, 2.0), (-1, 1, 2.0), (-1, -1, 2.0),
+         (2, 0, 1.0), (0, 2, 1.0), (-2, 0, 1.0), (0, -2, 1.0)]
 
 
 def masked_operator(g):
@@ -46,6 +51,48 @@ def solve(g, f=None):
     """Converged fill in FP64 (float64 array, known pixels copied)."""
     g = np.asarray(g, np.float64)
     Ahh, Ahk, ih, ik = masked_operator(g)
+    gf = g.ravel()
+    b = -Ahk @ gf[ik]
+    if f is not None:
+        b -= np.asarray(f, np.float64).ravel()[ih]
+    u = gf.copy()
+    if len(ih):
+        u[ih] = spla.spsolve(Ahh.tocsc(), b)
+    return u.reshape(g.shape)
+
+
+def masked_bilaplacian(g):
+    """(A_hh, A_hk, holes, known) of the 13-point clamped bilaplacian, the
+    operator of the reference's biharmonic CG (sign as there: +20 centre)."""
+    h, w = g.shape
+    hole = np.isnan(g).ravel()
+    idx_h = np.flatnonzero(hole)
+    idx_k = np.flatnonzero(~hole)
+    pos = -np.ones(h * w, int)
+    pos[idx_h] = np.arange(len(idx_h))
+    posk = -np.ones(h * w, int)
+    posk[idx_k] = np.arange(len(idx_k))
+    i, j = idx_h % w, idx_h // w
+    rh, ch, vh, rk, ck, vk = [], [], [], [], [], []
+    rows = np.arange(len(idx_h))
+    for di, dj, c in _ST13:
+        t = np.clip(j + dj, 0, h - 1) * w + np.clip(i + di, 0, w - 1)
+        th = hole[t]
+        rh.append(rows[th]); ch.append(pos[t[th]]); vh.append(np.full(th.sum(), c))
+        rk.append(rows[~th]); ck.append(posk[t[~th]]); vk.append(np.full((~th).sum(), c))
+    cat = np.concatenate
+    Ahh = sp.csr_matrix((cat(vh), (cat(rh), cat(ch))), shape=(len(idx_h), len(idx_h)))
+    Ahk = sp.csr_matrix((cat(vk), (cat(rk), cat(ck))), shape=(len(idx_h), len(idx_k)))
+    return Ahh, Ahk, idx_h, idx_k
+
+
+def solve_bilap(g, f=None):
+    """Converged biharmonic fill in FP64: the system of the reference's CG,
+    A_hh u = -f - A_hk g_k with the 13-point operator (Delta^2 u = -f;
+    cleant_cgpois.c:225-236, 273-278); a direct solve, since A is not
+    symmetric within two pixels of the border."""
+    g = np.asarray(g, np.float64)
+    Ahh, Ahk, ih, ik = masked_bilaplacian(g)
     gf = g.ravel()
     b = -Ahk @ gf[ik]
     if f is not None:

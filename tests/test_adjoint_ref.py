@@ -1,12 +1,14 @@
-"""The FP64 adjoint reference (oracle/adjoint_ref.py) pinned on the CPU:
-central finite differences of its own FP64 solve, the symmetry of the
-clamped 5-point operator, and agreement of its solve with the oracle's
-restatement run to convergence."""
+"""The FP64 references of oracle/adjoint_ref.py pinned on the CPU: central
+finite differences of its own FP64 solve, the symmetry of the clamped
+5-point operator, agreement of its solve with the oracle's restatement run
+to convergence; for the 13-point bilaplacian, exact refill of cubics and
+agreement with an independent per-pixel assembly."""
 import os
 import sys
 
 import numpy as np
 import pytest
+import scipy.sparse.linalg as spla
 
 from helpers import ROOT, oracle_solve
 
@@ -77,3 +79,35 @@ def test_fp64_solve_matches_converged_oracle():
     o, _ = oracle_solve(g.astype(np.float32), f.astype(np.float32), niter=0,
                         scale=1, cgit=-1, cg_eps=1e-9)
     assert np.max(np.abs(o - u)) < 1e-3
+
+
+def test_bilaplacian_fp64_solve_refills_cubics():
+    # the 13-point bilaplacian annihilates cubic polynomials away from the
+    # border, so holes at least two pixels inside refill the cubic exactly
+    h, w = 23, 29
+    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+    cub = 0.01 * x ** 3 - 0.02 * x * y * y + 0.5 * x * y - 3 * y + 7
+    g = cub.copy()
+    g[4:12, 5:20] = np.nan
+    g[15:19, 3:9] = np.nan
+    u = ar.solve_bilap(g)
+    assert np.max(np.abs(u - cub)) < 1e-8 * np.max(np.abs(cub))
+
+
+def test_bilaplacian_operator_matches_independent_assembly():
+    # the vectorised assembly equals the per-pixel one of the oracle tests,
+    # right-hand side included, on holes that touch the border
+    from test_oracle_math import _dense_system
+    rng = np.random.default_rng(8)
+    g = (rng.random((14, 16)) * 40).astype(np.float32)
+    g[3:10, 4:12] = np.nan
+    g[0:2, 13:16] = np.nan
+    f = rng.standard_normal((14, 16)).astype(np.float32)
+    A, b, unk = _dense_system(g, f, 1)
+    Ahh, Ahk, ih, ik = ar.masked_bilaplacian(g.astype(np.float64))
+    assert np.array_equal(ih, unk)
+    assert abs(A - Ahh).max() == 0
+    b2 = -Ahk @ g.astype(np.float64).ravel()[ik] - f.astype(np.float64).ravel()[ih]
+    assert np.max(np.abs(b - b2)) < 1e-9 * np.max(np.abs(b))
+    u = ar.solve_bilap(g, f)
+    assert np.max(np.abs(u.ravel()[ih] - spla.spsolve(A.tocsc(), b))) < 1e-8 * np.max(np.abs(u))
