@@ -17,7 +17,7 @@ class sp_params(ctypes.Structure):
                 ("cg_eps", ctypes.c_double), ("trace", ctypes.c_int),
                 ("cg_fp32", ctypes.c_int), ("cg_rel", ctypes.c_double),
                 ("cg_chunk", ctypes.c_int), ("cg_mg", ctypes.c_int),
-                ("autotune", ctypes.c_int)]
+                ("autotune", ctypes.c_int), ("guidance", ctypes.c_int)]
 
 
 class sp_image(ctypes.Structure):
@@ -60,6 +60,8 @@ _SIGNATURES = {
                              ctypes.POINTER(ctypes.c_double)]),
     "sp_autotune_report": (_I, [ctypes.c_char_p, _I]),
     "sp_stream_copy": (_I, [_P, _P, ctypes.c_size_t, _P]),
+    "sp_plambda_lap_device": (_I, [_P, _P, _I, _I, _I, _I, _I, _P]),
+    "sp_plambda_lap_host": (_I, [_P, _P, _I, _I, _I, _I, _I]),
 }
 
 

@@ -184,3 +184,27 @@ def stream_copy(dst, src, stream=None):
     s = stream if stream is not None else torch.cuda.current_stream()
     check(lib.sp_stream_copy(dst.data_ptr(), src.data_ptr(), src.numel(),
                              ctypes.c_void_p(s.cuda_stream)))
+
+
+def plambda_lap(x, variant=0, boundary=1, out=None, stream=None):
+    """plambda's "x,l" (variant 0) or "x,lr" (1) Laplacian of planar float32
+    planes (h, w) or (pd, h, w); boundary = the PLAMBDA_GETPIXEL rule (0 zero,
+    1 nearest, 2 reflection, 3 periodic, 4 NaN).  numpy arrays run through
+    the host entry, torch device tensors through the device entry."""
+    if isinstance(x, np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        x3 = x.reshape((1,) + x.shape) if x.ndim == 2 else x
+        pd, h, w = x3.shape
+        y = np.empty_like(x3)
+        check(lib.sp_plambda_lap_host(_ptr(y), _ptr(x3), w, h, pd, variant, boundary))
+        return y.reshape(x.shape)
+    import torch
+    if x.dtype != torch.float32 or not x.is_cuda or not x.is_contiguous():
+        raise TypeError("x must be a contiguous float32 device tensor or a numpy array")
+    pd, h, w = (1,) + tuple(x.shape) if x.dim() == 2 else tuple(x.shape)
+    if out is None:
+        out = torch.empty_like(x)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib.sp_plambda_lap_device(out.data_ptr(), x.data_ptr(), w, h, pd, variant,
+                                    boundary, ctypes.c_void_p(s.cuda_stream)))
+    return out

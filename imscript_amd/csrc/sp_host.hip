@@ -283,6 +283,10 @@ struct Plan {
 	hipStream_t cap_stream = nullptr;
 
 	std::vector<char> key;           // identity of the call this plan serves
+	// guidance from a source image: plambda's Laplacian of each plane's
+	// caller-side f (the image) into plan memory, which the solve uses as f
+	std::vector<std::pair<const float *, float *>> guide;
+	std::vector<std::pair<int, int>> guide_wh;
 
 	~Plan()
 	{
@@ -509,13 +513,25 @@ static int plan_create(Plan &pl, const std::vector<PlaneIn> &planes,
 {
 	pl.P = P;
 	pl.planes = planes;
+	if (P.guidance) {
+		if (P.guidance != 1 && P.guidance != 2)
+			return set_err(SP_ERR_ARG, "guidance must be 0, 1 (plambda x,l) or 2 (x,lr)");
+		for (PlaneIn &q : pl.planes) {
+			if (!q.f) continue;
+			float *d = nullptr;
+			TRY(pl.alloc(&d, (size_t)q.w * q.h));
+			pl.guide.push_back({q.f, d});
+			pl.guide_wh.push_back({q.w, q.h});
+			q.f = d;
+		}
+	}
 	pl.R = op_radius(P.tstep, &pl.teff);
 	const int np = (int)planes.size();
 	// pyramid depth of each plane, simpois.c:253-256
 	std::vector<std::vector<std::pair<int, int>>> shapes(np);
 	int nlev = 0;
 	for (int p = 0; p < np; p++) {
-		int cw = planes[p].w, ch = planes[p].h, sc = P.scale;
+		int cw = pl.planes[p].w, ch = pl.planes[p].h, sc = P.scale;
 		for (;;) {
 			shapes[p].push_back({cw, ch});
 			if (!(sc > 1 && (cw > 1 || ch > 1))) break;
@@ -1045,6 +1061,10 @@ static int plan_execute(Plan &pl, hipStream_t s, sp_stats *st)
 	HIPCHK(hipEventCreate(&e1));
 	HIPCHK(hipEventRecord(e0, s));
 	for (auto &v : pl.traces) for (auto &t : v) { t.n = 0; t.rr.clear(); }
+	// data term from the source image (plambda "x,l" / "x,lr", nearest rule)
+	for (size_t k = 0; k < pl.guide.size(); k++)
+		spk::launch_plambda_lap(pl.guide[k].second, pl.guide[k].first,
+				pl.guide_wh[k].first, pl.guide_wh[k].second, 1, pl.P.guidance - 1, 1, s);
 	// pyramid of data, simpois.c:260-262
 	{
 		ProfScope pr("pyramid", s);
@@ -1144,6 +1164,7 @@ extern "C" int sp_solve_batch_device(int n, const sp_image *imgs,
 	put(&P.cg_eps, sizeof P.cg_eps); put(&P.trace, sizeof P.trace);
 	put(&P.cg_fp32, sizeof P.cg_fp32); put(&P.cg_rel, sizeof P.cg_rel);
 	put(&P.cg_chunk, sizeof P.cg_chunk); put(&P.cg_mg, sizeof P.cg_mg);
+	put(&P.autotune, sizeof P.autotune); put(&P.guidance, sizeof P.guidance);
 	static std::mutex cache_mu;
 	static Plan *cache = nullptr;
 	std::lock_guard<std::mutex> lk(cache_mu);
@@ -1202,6 +1223,9 @@ extern "C" int sp_adjoint_batch_device(int n, const sp_adjoint_image *imgs,
 	if (!(p->tstep > 0))
 		return set_err(SP_ERR_ARG, "the adjoint covers the Poisson/Laplace fill "
 				"(tstep > 0): the clamped bilaplacian is not symmetric");
+	if (p->guidance)
+		return set_err(SP_ERR_ARG, "the adjoint takes the data term itself "
+				"(guidance = 0); differentiate plambda's Laplacian separately");
 	TRY(ensure_device());
 	hipStream_t s = (hipStream_t)stream;
 	Plan tmp;   // owns the scratch planes
@@ -1243,6 +1267,51 @@ extern "C" int sp_adjoint_batch_device(int n, const sp_adjoint_image *imgs,
 	spk::launch_adj_gather(d_ap, d_bw, (int)bw.size(), s);
 	HIPCHK(hipGetLastError());
 	HIPCHK(hipStreamSynchronize(s));
+	return SP_OK;
+}
+
+// ------------------------------------------------- plambda's Laplacian
+
+static int lap_args(const float *out, const float *in, int w, int h, int pd,
+		int variant, int boundary)
+{
+	if (!out || !in || w < 1 || h < 1 || pd < 1)
+		return set_err(SP_ERR_ARG, "bad image arguments");
+	if ((long)w * h * pd > 0x7fffffffL)
+		return set_err(SP_ERR_ARG, "image too large");
+	if (variant != 0 && variant != 1)
+		return set_err(SP_ERR_ARG, "variant must be 0 (x,l) or 1 (x,lr)");
+	if (boundary < 0 || boundary > 4)
+		return set_err(SP_ERR_ARG, "boundary rule must be 0..4");
+	if (out == in) return set_err(SP_ERR_ARG, "out must not alias in");
+	return SP_OK;
+}
+
+extern "C" int sp_plambda_lap_device(float *d_out, const float *d_in, int w, int h,
+		int pd, int variant, int boundary, void *stream)
+{
+	TRY(lap_args(d_out, d_in, w, h, pd, variant, boundary));
+	TRY(ensure_device());
+	spk::launch_plambda_lap(d_out, d_in, w, h, pd, variant, boundary,
+			(hipStream_t)stream);
+	HIPCHK(hipGetLastError());
+	return SP_OK;
+}
+
+extern "C" int sp_plambda_lap_host(float *out, const float *in, int w, int h, int pd,
+		int variant, int boundary)
+{
+	TRY(lap_args(out, in, w, h, pd, variant, boundary));
+	TRY(ensure_device());
+	const size_t n = (size_t)w * h * pd;
+	Plan tmp;   // owns the device buffers
+	float *d_in = nullptr, *d_out = nullptr;
+	TRY(tmp.alloc(&d_in, n));
+	TRY(tmp.alloc(&d_out, n));
+	HIPCHK(hipMemcpy(d_in, in, n * 4, hipMemcpyHostToDevice));
+	spk::launch_plambda_lap(d_out, d_in, w, h, pd, variant, boundary, nullptr);
+	HIPCHK(hipGetLastError());
+	HIPCHK(hipMemcpy(out, d_out, n * 4, hipMemcpyDeviceToHost));
 	return SP_OK;
 }
 

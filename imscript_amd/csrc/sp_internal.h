@@ -46,6 +46,11 @@ void launch_finalize(const SpPlane *lv, float *const *outs,
 		const BlockWork *bw, int nblocks, hipStream_t s);
 
 // Gauss-Seidel
+// plambda "x,l" (variant 0) / "x,lr" (1) of pd planar w x h planes under
+// boundary rule mode (0 zero, 1 nearest, 2 reflection, 3 periodic, 4 NaN)
+void launch_plambda_lap(float *out, const float *in, int w, int h, int pd,
+		int variant, int mode, hipStream_t s);
+
 // dst[0..4*n4) = src[0..4*n4), 16-byte aligned
 void launch_stream_copy(float *dst, const float *src, size_t n4, hipStream_t s);
 

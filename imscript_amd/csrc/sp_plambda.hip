@@ -1,0 +1,94 @@
+// plambda's 3x3 Laplacian image operators, the producer of simpois's
+// guidance field (plambda.c:2589-2605 stencils, :2680-2689 application,
+// getpixel.c boundary rules).  Planar float planes, bit-identical to the
+// reference: every output is r = 0; r += s[k] * P(i-1+k%3, j-1+k/3) for
+// k = 0..8 in that order, zero weights included (0 * NaN = NaN, 0 * -x =
+// -0 as there), float multiply and add without contraction.
+#include "sp_internal.h"
+
+namespace spk {
+
+constexpr int LP_T = 256, LP_PX = 4;   // threads per block, pixels per thread
+
+// stencils "l" and "lr" (plambda.c:2589-2605)
+template <int V> struct LapStencil;
+template <> struct LapStencil<0> {
+	static constexpr float s[9] = {0, 1, 0, 1, -4, 1, 0, 1, 0};
+};
+template <> struct LapStencil<1> {
+	static constexpr float s[9] = {0.5f, 0, 0.5f, 0, -2, 0, 0.5f, 0, 0.5f};
+};
+
+static __device__ __forceinline__ int lp_gmod(int x, int m)
+{
+	const int r = x % m;
+	return r < 0 ? r + m : r;
+}
+
+static __device__ __forceinline__ int lp_reflex(int n, int p)
+{   // getpixel.c:74-83
+	int r = lp_gmod(n, 2 * p);
+	if (r == p) r -= 1;
+	if (r > p) r = 2 * p - r;
+	return r;
+}
+
+// one sample under boundary rule `mode` (0 zero, 1 nearest, 2 reflection,
+// 3 periodic, 4 NaN; getpixel.c:10-99)
+static __device__ __forceinline__ float lp_sample(const float *x, int w, int h,
+		int i, int j, int mode)
+{
+	const bool in = i >= 0 && i < w && j >= 0 && j < h;
+	if (in) return x[(size_t)j * w + i];
+	switch (mode) {
+	case 0: return 0.0f;
+	case 2: i = lp_reflex(i, w); j = lp_reflex(j, h); break;
+	case 3: i = lp_gmod(i, w); j = lp_gmod(j, h); break;
+	case 4: return __builtin_nanf("");
+	default:
+		i = i < 0 ? 0 : (i >= w ? w - 1 : i);
+		j = j < 0 ? 0 : (j >= h ? h - 1 : j);
+	}
+	return x[(size_t)j * w + i];
+}
+
+// grid: x = column chunks of LP_T*LP_PX, y = rows of all planes
+template <int V>
+__global__ __launch_bounds__(LP_T) void k_plambda_lap(float *__restrict__ out,
+		const float *__restrict__ in, int w, int h, int mode)
+{
+	const int row = blockIdx.y, l = row / h, j = row - l * h;
+	const int i0 = (blockIdx.x * LP_T + threadIdx.x) * LP_PX;
+	if (i0 >= w) return;
+	const float *x = in + (size_t)l * w * h;
+	float *y = out + (size_t)l * w * h + (size_t)j * w;
+	constexpr const float *s = LapStencil<V>::s;
+	// the 3 x (LP_PX + 2) window around the thread's pixels
+	float v[3][LP_PX + 2];
+	const bool inner = i0 >= 1 && i0 + LP_PX < w && j >= 1 && j + 1 < h;
+#pragma unroll
+	for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+		for (int dx = 0; dx < LP_PX + 2; dx++) {
+			const int i = i0 - 1 + dx, jj = j - 1 + dy;
+			v[dy][dx] = inner ? x[(size_t)jj * w + i] : lp_sample(x, w, h, i, jj, mode);
+		}
+#pragma unroll
+	for (int p = 0; p < LP_PX; p++) {
+		if (i0 + p >= w) break;
+		float r = 0.0f;
+#pragma unroll
+		for (int k = 0; k < 9; k++) r = r + s[k] * v[k / 3][p + k % 3];
+		y[i0 + p] = r;
+	}
+}
+
+void launch_plambda_lap(float *out, const float *in, int w, int h, int pd,
+		int variant, int mode, hipStream_t s)
+{
+	const dim3 grid((w + LP_T * LP_PX - 1) / (LP_T * LP_PX), h * pd);
+	if (variant) k_plambda_lap<1><<<grid, LP_T, 0, s>>>(out, in, w, h, mode);
+	else k_plambda_lap<0><<<grid, LP_T, 0, s>>>(out, in, w, h, mode);
+}
+
+} // namespace spk

@@ -86,6 +86,11 @@ typedef struct sp_params {
 	                    first sweep of each level size and keep the fastest
 	                    (cached per process; results do not depend on it);
 	                    0: built-in shapes.  Default from SIMPOIS_HIP_AUTOTUNE */
+	int guidance;    /* 0: f is the data term, as in the reference; 1 or 2: f
+	                    holds a source image v and the data term is plambda's
+	                    "x,l" (1) or "x,lr" (2) Laplacian of v with the
+	                    nearest-pixel rule, computed inside the solve (the
+	                    Poisson-editing pipeline plambda v "x,l" | simpois -f) */
 } sp_params;
 
 /* one image of a batch: planar w*h*pd device buffers, f may be NULL */
@@ -160,6 +165,20 @@ int sp_profile_read(const char *name, double *ms, long *launches, double *bytes)
  * array `rel`. */
 int sp_residual_device(const float *d_u, const float *d_g, const float *d_f,
 		int w, int h, int pd, int bilap, double *rel, void *stream);
+
+/* plambda's 3x3 Laplacian image operator on pd planar w x h float planes
+ * (the producer of simpois's guidance field; replaces `plambda in "x,l"`,
+ * plambda.c:2589-2605 stencils and :2680-2689 apply_3x3_stencil, on one
+ * plane layout): out = sum over k = 0..8, in that order, of s[k] *
+ * P(in, i-1+k%3, j-1+k/3), bit-identical to the reference.  variant 0: "x,l"
+ * {0,1,0, 1,-4,1, 0,1,0}; 1: "x,lr" {.5,0,.5, 0,-2,0, .5,0,.5}.  boundary:
+ * the PLAMBDA_GETPIXEL rule (plambda.c:2573-2588, getpixel.c): 0 zero,
+ * 1 nearest (plambda's default), 2 reflection, 3 periodic, 4 NaN.  out must
+ * not alias in.  The device form is asynchronous on `stream`. */
+int sp_plambda_lap_device(float *d_out, const float *d_in, int w, int h, int pd,
+		int variant, int boundary, void *stream);
+int sp_plambda_lap_host(float *out, const float *in, int w, int h, int pd,
+		int variant, int boundary);
 
 /* dst[0..n) = src[0..n) on the device with a plain streaming kernel
  * (k_stream_copy): the bandwidth ceiling the solver kernels are compared

@@ -51,6 +51,7 @@ def oracle():
         o.orc_cg_extend.restype = I
         o.orc_rel_residual.argtypes = [V, V, V, I, I, I, V]
         o.orc_rel_residual.restype = ctypes.c_double
+        o.orc_plambda_lap.argtypes = [V, V, I, I, I, I, I]
         _oracle = o
     return _oracle
 
@@ -162,3 +163,26 @@ def gpu_available():
         return torch.cuda.is_available()
     except Exception:
         return False
+
+
+def oracle_plambda_lap(x, variant, mode):
+    """Oracle plambda "x,l" (variant 0) / "x,lr" (1) of a planar (pd,h,w) or
+    (h,w) float32 image under boundary rule ``mode``."""
+    x = np.ascontiguousarray(x, np.float32)
+    x3 = x.reshape((1,) + x.shape) if x.ndim == 2 else x
+    pd, h, w = x3.shape
+    out = np.empty_like(x3)
+    oracle().orc_plambda_lap(ptr(out), ptr(x3), w, h, pd, variant, mode)
+    return out.reshape(x.shape)
+
+
+def load_plambda_golden():
+    """[(input (pd,h,w), output (pd,h,w), variant, mode)] of the plambda
+    fixtures (stored interleaved (h,w,pd) as plambda keeps them)."""
+    z = np.load(os.path.join(ROOT, "tests", "golden", "plambda_lap.npz"))
+    out = []
+    for k, (variant, mode) in enumerate(z["meta"]):
+        x = np.ascontiguousarray(np.moveaxis(z[f"in{k}"], -1, 0))
+        y = np.ascontiguousarray(np.moveaxis(z[f"out{k}"], -1, 0))
+        out.append((x, y, int(variant), int(mode)))
+    return out

@@ -1,0 +1,71 @@
+"""GPU parity of plambda's 3x3 Laplacian operators (the producer of
+simpois's guidance field) through the C ABI: bit-identical to the golden
+vectors of the reference plambda and to the oracle restatement, every
+variant and boundary rule; the solver's fused guidance option equals the
+two-step pipeline bit for bit."""
+import numpy as np
+import pytest
+
+from helpers import bits_equal, load_plambda_golden, oracle_plambda_lap
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = load_plambda_golden()
+
+
+def _m():
+    import imscript_amd
+    return imscript_amd
+
+
+def test_golden_through_the_host_entry():
+    m = _m()
+    print(f"plambda golden {len(GOLDEN)} cases", flush=True)
+    for x, want, variant, mode in GOLDEN:
+        got = m.plambda_lap(x, variant, mode)
+        assert bits_equal(got, want), (x.shape, variant, mode)
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("mode", range(5))
+def test_device_entry_matches_oracle(variant, mode):
+    import torch
+    m = _m()
+    rng = np.random.default_rng(40 + 5 * variant + mode)
+    x = (rng.standard_normal((3, 517, 1031)) * 40).astype(np.float32)
+    x[0, 5, 7], x[1, 0, 0], x[2, -1, -1], x[0, 300, 1030] = np.nan, np.inf, -np.inf, -0.0
+    print(f"plambda device variant={variant} mode={mode} {x.shape}", flush=True)
+    got = m.plambda_lap(torch.from_numpy(x).cuda(), variant, mode)
+    torch.cuda.synchronize()
+    assert bits_equal(got.cpu().numpy(), oracle_plambda_lap(x, variant, mode))
+
+
+def test_bad_arguments_are_refused():
+    import torch
+    m = _m()
+    x = torch.zeros(8, 9, device="cuda")
+    with pytest.raises(m.SimpoisError):
+        m.plambda_lap(x, 2, 1)
+    with pytest.raises(m.SimpoisError):
+        m.plambda_lap(x, 0, 5)
+    with pytest.raises(m.SimpoisError):
+        m.plambda_lap(x, 0, 1, out=x)       # in place is not allowed
+
+
+@pytest.mark.parametrize("guid,kw", [
+    (1, dict(niter=12, scale=99)),                                  # GS pyramid
+    (2, dict(niter=0, scale=1, cgit=-1, cg_fp32=1, cg_rel=1e-6, cg_mg=1)),
+])
+def test_fused_guidance_equals_the_two_step_pipeline(guid, kw):
+    import torch
+    m = _m()
+    from test_gpu_parity import random_case
+    g, v = random_case(81, 2, 300, 413, 0.4, True)
+    dg, dv = torch.from_numpy(g).cuda(), torch.from_numpy(v).cuda()
+    print(f"fused guidance {guid} {kw}", flush=True)
+    f = m.plambda_lap(dv, guid - 1, 1)
+    two, _ = m.solve_device(dg, f, **kw)
+    two = two.clone()
+    one, _ = m.solve_device(dg, dv, guidance=guid, **kw)
+    torch.cuda.synchronize()
+    assert torch.equal(one.view(torch.int32), two.view(torch.int32))
