@@ -78,6 +78,7 @@ def make_inputs(torch, n, pd, seed, dev):
          - 4 * v).contiguous()
     holes = torch.from_numpy(rect_holes(rng, n)).to(dev)
     g[:, holes] = float("nan")
+    make_inputs.source = v.contiguous()   # the guidance source image itself
     return g, f, float(holes.float().mean())
 
 
@@ -165,7 +166,9 @@ def fp64_check(torch, sp, params, seed, side=1024):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import adjoint_ref as ar
     g, f, _ = make_inputs(torch, side, 1, seed + 100, "cuda:0")
-    out, _ = sp.solve_device(g, f, **params)
+    if params.get("guidance"):
+        f = sp.plambda_lap(make_inputs.source, params["guidance"] - 1, 1)
+    out, _ = sp.solve_device(g, f, **{k: v for k, v in params.items() if k != "guidance"})
     torch.cuda.synchronize()
     gh, fh, uh = g[0].cpu().numpy(), f[0].cpu().numpy(), out[0].cpu().numpy()
     t0 = time.perf_counter()
@@ -191,6 +194,32 @@ def _oracle_imports():
     from helpers import oracle_solve, bits_equal
     import adjoint_ref as ar
     return oracle_solve, bits_equal, ar
+
+
+def cpu_leg_lap(v, out, side=2048):
+    """The reference plambda on a side x side crop of channel 0 (file I/O
+    included, one thread), bit-compared with the GPU result on that crop."""
+    oracle_solve, bits_equal, ar = _oracle_imports()
+    ref = os.path.join(ROOT, "oracle", "_ref", "plambda")
+    if not os.path.exists(ref):
+        return None
+    x = np.ascontiguousarray(v[0, :side, :side].cpu().numpy())
+    with tempfile.TemporaryDirectory() as td:
+        np.save(os.path.join(td, "v.npy"), x)
+        t0 = time.perf_counter()
+        subprocess.run([ref, os.path.join(td, "v.npy"), "x,l", "-o",
+                        os.path.join(td, "o.npy")], check=True,
+                       env={k: e for k, e in os.environ.items()
+                            if k not in ("PLAMBDA_GETPIXEL", "GETPIXEL")})
+        dt = time.perf_counter() - t0
+        y = np.load(os.path.join(td, "o.npy")).reshape(side, side)
+    # the crop's own border is clamped by the reference: compare the interior
+    got = out[0, 1:side - 1, 1:side - 1].cpu().numpy()
+    return {"value": round(side * side / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1,
+            "kind": "reference", "seconds": round(dt, 3),
+            "bit_exact_interior": bits_equal(got, y[1:-1, 1:-1]),
+            "sample": f"plambda v.npy \"x,l\" on the top-left {side}x{side} crop "
+                      "of channel 0, file I/O included"}
 
 
 def cpu_leg_c1(g, got):
@@ -546,6 +575,46 @@ def run_c5(a):
     emit(res)
 
 
+def run_lap(a):
+    """plambda's x,l Laplacian (the guidance-field producer) on the C2 source
+    images, 4096x4096x3 planar float32: throughput, roofline of the kernel
+    (8 B per pixel: read the image once, write the result once) and the
+    reference plambda on a crop of one channel."""
+    import torch
+    import imscript_amd as sp
+    make_inputs(torch, a.size, a.channels, a.seed, "cuda:0")
+    v = make_inputs.source
+    out = torch.empty_like(v)
+    for _ in range(max(1, a.warmup)):
+        sp.plambda_lap(v, 0, 1, out=out)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = max(10, a.steps)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        sp.plambda_lap(v, 0, 1, out=out)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    px = v.numel()
+    gbps = 8.0 * px / (ms * 1e-3) / 1e9
+    res = {"metric": "plambda x,l Laplacian, Mpix/s (guidance-field producer)",
+           "value": round(px / 1e6 / (ms * 1e-3), 1), "unit": "Mpix/s", "n_gpus": 1,
+           "steps": reps, "warmup": a.warmup, "ms_per_step": round(ms, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"plambda \"x,l\" of {a.size}x{a.size}x{a.channels} "
+                                  "float32 planar source images, nearest-pixel rule"},
+           "roofline": {"kernel": "k_plambda_lap<0> (4 pixels per thread)", "bound": "hbm",
+                        "achieved": round(gbps, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                        "frac": round(gbps / PEAK_HBM_GBPS, 4), "traffic": None,
+                        "algorithmic_bytes_per_launch": 8.0 * px,
+                        "avg_launch_us": round(ms * 1e3, 2)}}
+    if not a.no_cpu:
+        res["cpu_baseline"] = cpu_leg_lap(v, out)
+    emit(res)
+
+
 def stream_copy_ceiling(torch, sp, nbytes=1 << 30, reps=10):
     """Bandwidth of the library's plain copy kernel over nbytes (read +
     write counted), timed with events on the current stream: the measured
@@ -595,7 +664,11 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="built-in Gauss-Seidel tile shapes instead of the "
                          "autotuner's choice (tuned during the warmup solve)")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+    ap.add_argument("--guidance", action="store_true",
+                    help="C2 from the source image: the solver computes the "
+                         "data term as plambda's x,l Laplacian of it inside "
+                         "the solve (plambda v x,l | simpois -f, fused)")
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "lap"],
                     help="c2 (default) is the headline; the others print their "
                          "own JSON line")
     ap.add_argument("--c3-size", type=int, default=2048)
@@ -608,7 +681,8 @@ def main():
               "supported (see DESIGN.md for how it would shard)", file=sys.stderr)
         sys.exit(2)
     if a.config != "c2":
-        {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5}[a.config](a)
+        {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5,
+         "lap": run_lap}[a.config](a)
         return
 
     import torch
@@ -624,6 +698,12 @@ def main():
                   autotune=0 if a.no_autotune else 1)
     torch.cuda.synchronize()
 
+    f_term = f
+    if a.guidance:
+        params["guidance"] = 1
+        f = make_inputs.source
+        f_term = sp.plambda_lap(f, 0, 1)   # the data term the solve computes
+
     def step():
         return sp.solve_device(g, f, out=out, **params)[1]
 
@@ -636,7 +716,7 @@ def main():
         stats = step()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / a.steps
-    rel = sp.residual_device(out, g, f)
+    rel = sp.residual_device(out, g, f_term)
     ok = max(rel) <= a.target
 
     # phase timers on one more solve of the same workload
@@ -683,6 +763,8 @@ def main():
                       "(bit-identical to the reference's default path)",
             }[a.method],
             "hole_fraction": round(hole_frac, 4),
+            "data_term": ("plambda x,l of the source image, computed inside the "
+                          "solve" if a.guidance else "f given"),
             "levels": stats.levels,
             "cg_ms": round(cg_ms, 3),
             "cg_iters_finest_max": stats.cg_iters_finest,
@@ -771,7 +853,7 @@ def main():
                               "path": "sp_solve_host (the drop-in's host-buffer entry), "
                                       "H2D and D2H copies included"}
     if not a.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(g, f, a.cg_rel, a.cpu_side, params)
+        result["cpu_baseline"] = cpu_baseline(g, f_term, a.cg_rel, a.cpu_side, params)
         result["config"]["fp64_check"] = fp64_check(torch, sp, params, a.seed)
     print(json.dumps(result), flush=True)
     if not ok:

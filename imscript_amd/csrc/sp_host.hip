@@ -553,7 +553,7 @@ static int plan_create(Plan &pl, const std::vector<PlaneIn> &planes,
 	pl.cg_iters_finest.assign(np, 0);
 	{
 		std::vector<float *> outs(np);
-		for (int p = 0; p < np; p++) outs[p] = planes[p].out;
+		for (int p = 0; p < np; p++) outs[p] = pl.planes[p].out;
 		TRY(pl.upload(&pl.d_outs, outs));
 	}
 	size_t cnt_off = 0;
@@ -576,9 +576,9 @@ static int plan_create(Plan &pl, const std::vector<PlaneIn> &planes,
 			S.w = w; S.h = h;
 			S.cnt = pl.d_cnt + cnt_off + m;
 			if (k == 0) {
-				S.g = const_cast<float *>(planes[p].g);
-				S.f = const_cast<float *>(planes[p].f);
-				S.u = planes[p].out;
+				S.g = const_cast<float *>(pl.planes[p].g);
+				S.f = const_cast<float *>(pl.planes[p].f);
+				S.u = pl.planes[p].out;
 				S.c = pool + off; off += n;
 			} else {
 				S.g = pool + off; off += n;
