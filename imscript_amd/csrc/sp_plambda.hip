@@ -4,6 +4,7 @@
 // reference: every output is r = 0; r += s[k] * P(i-1+k%3, j-1+k/3) for
 // k = 0..8 in that order, zero weights included (0 * NaN = NaN, 0 * -x =
 // -0 as there), float multiply and add without contraction.
+#include <stdint.h>
 #include "sp_internal.h"
 
 namespace spk {
@@ -53,7 +54,7 @@ static __device__ __forceinline__ float lp_sample(const float *x, int w, int h,
 }
 
 // grid: x = column chunks of LP_T*LP_PX, y = rows of all planes
-template <int V>
+template <int V, bool VEC>
 __global__ __launch_bounds__(LP_T) void k_plambda_lap(float *__restrict__ out,
 		const float *__restrict__ in, int w, int h, int mode)
 {
@@ -66,20 +67,41 @@ __global__ __launch_bounds__(LP_T) void k_plambda_lap(float *__restrict__ out,
 	// the 3 x (LP_PX + 2) window around the thread's pixels
 	float v[3][LP_PX + 2];
 	const bool inner = i0 >= 1 && i0 + LP_PX < w && j >= 1 && j + 1 < h;
+	// with 16-byte aligned planes (VEC: both base pointers aligned and the
+	// width a multiple of 4) every row starts aligned: one 16-byte load per row
+	const bool vec = VEC && inner;
+	if (vec) {
 #pragma unroll
-	for (int dy = 0; dy < 3; dy++)
-#pragma unroll
-		for (int dx = 0; dx < LP_PX + 2; dx++) {
-			const int i = i0 - 1 + dx, jj = j - 1 + dy;
-			v[dy][dx] = inner ? x[(size_t)jj * w + i] : lp_sample(x, w, h, i, jj, mode);
+		for (int dy = 0; dy < 3; dy++) {
+			const float *r = x + (size_t)(j - 1 + dy) * w + i0;
+			const float4 c = *(const float4 *)r;
+			v[dy][0] = r[-1];
+			v[dy][1] = c.x; v[dy][2] = c.y; v[dy][3] = c.z; v[dy][4] = c.w;
+			v[dy][5] = r[LP_PX];
 		}
+	} else {
+#pragma unroll
+		for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+			for (int dx = 0; dx < LP_PX + 2; dx++) {
+				const int i = i0 - 1 + dx, jj = j - 1 + dy;
+				v[dy][dx] = inner ? x[(size_t)jj * w + i] : lp_sample(x, w, h, i, jj, mode);
+			}
+	}
+	float o[LP_PX];
 #pragma unroll
 	for (int p = 0; p < LP_PX; p++) {
-		if (i0 + p >= w) break;
 		float r = 0.0f;
 #pragma unroll
 		for (int k = 0; k < 9; k++) r = r + s[k] * v[k / 3][p + k % 3];
-		y[i0 + p] = r;
+		o[p] = r;
+	}
+	if (VEC && i0 + LP_PX <= w) {
+		*(float4 *)(y + i0) = make_float4(o[0], o[1], o[2], o[3]);
+	} else {
+#pragma unroll
+		for (int p = 0; p < LP_PX; p++)
+			if (i0 + p < w) y[i0 + p] = o[p];
 	}
 }
 
@@ -87,8 +109,14 @@ void launch_plambda_lap(float *out, const float *in, int w, int h, int pd,
 		int variant, int mode, hipStream_t s)
 {
 	const dim3 grid((w + LP_T * LP_PX - 1) / (LP_T * LP_PX), h * pd);
-	if (variant) k_plambda_lap<1><<<grid, LP_T, 0, s>>>(out, in, w, h, mode);
-	else k_plambda_lap<0><<<grid, LP_T, 0, s>>>(out, in, w, h, mode);
+	const bool vec = !(w & 3) && !((uintptr_t)out & 15) && !((uintptr_t)in & 15);
+	if (variant) {
+		if (vec) k_plambda_lap<1, true><<<grid, LP_T, 0, s>>>(out, in, w, h, mode);
+		else k_plambda_lap<1, false><<<grid, LP_T, 0, s>>>(out, in, w, h, mode);
+	} else {
+		if (vec) k_plambda_lap<0, true><<<grid, LP_T, 0, s>>>(out, in, w, h, mode);
+		else k_plambda_lap<0, false><<<grid, LP_T, 0, s>>>(out, in, w, h, mode);
+	}
 }
 
 } // namespace spk

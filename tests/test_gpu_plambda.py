@@ -40,6 +40,24 @@ def test_device_entry_matches_oracle(variant, mode):
     assert bits_equal(got.cpu().numpy(), oracle_plambda_lap(x, variant, mode))
 
 
+def test_unaligned_planes_and_odd_widths():
+    # a view one float into its buffer is not 16-byte aligned, and odd
+    # widths leave partial groups: both take the element-wise path
+    import torch
+    m = _m()
+    rng = np.random.default_rng(47)
+    for h, w in [(33, 64), (31, 129), (7, 5)]:
+        x = (rng.standard_normal(h * w + 1) * 30).astype(np.float32)
+        base = torch.from_numpy(x).cuda()
+        view = base[1:].view(h, w)
+        out = torch.empty(h * w + 1, device="cuda")[1:].view(h, w)
+        print(f"plambda unaligned {h}x{w}", flush=True)
+        m.plambda_lap(view, 0, 1, out=out)
+        torch.cuda.synchronize()
+        want = oracle_plambda_lap(x[1:].reshape(h, w), 0, 1)
+        assert bits_equal(out.cpu().numpy(), want), (h, w)
+
+
 def test_bad_arguments_are_refused():
     import torch
     m = _m()
