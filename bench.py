@@ -615,6 +615,111 @@ def run_lap(a):
     emit(res)
 
 
+def cpu_leg_amle(x, sp, nn, niter, side):
+    """The reference amle on a side x side crop of plane 0 (file I/O
+    included, one thread), bit-compared with the GPU fill of the same crop."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "amle")
+    if not os.path.exists(ref):
+        return None
+    _, bits_equal, _ = _oracle_imports()
+    crop = np.ascontiguousarray(x[0, :side, :side].cpu().numpy())
+    with tempfile.TemporaryDirectory() as td:
+        np.save(os.path.join(td, "x.npy"), crop)
+        env = dict(os.environ, AMLE_NN=str(nn), AMLE_NITER=str(niter))
+        env.pop("AMLE_INIT", None)
+        t0 = time.perf_counter()
+        subprocess.run([ref, os.path.join(td, "x.npy"), "", os.path.join(td, "o.npy")],
+                       check=True, env=env, stderr=subprocess.DEVNULL)
+        dt = time.perf_counter() - t0
+        y = np.load(os.path.join(td, "o.npy")).reshape(side, side)
+    got = sp.amle(crop, nn, niter)
+    return {"value": round(side * side / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1,
+            "kind": "reference", "seconds": round(dt, 3),
+            "bit_exact": bits_equal(got, y),
+            "sample": f"amle x.npy on the top-left {side}x{side} crop of plane 0 "
+                      f"(AMLE_NN={nn}, AMLE_NITER={niter}), file I/O included; "
+                      "the reference also sweeps its lower bound"}
+
+
+def run_amle(a):
+    """AMLE hole filling (amle.c) of the C2 boundary images, 4096x4096 per
+    plane, the reference's defaults AMLE_NN = 4 and AMLE_NITER = 100: fill
+    throughput, the tile kernel's HBM roofline (12 B per pixel per pass of
+    BT fused sweeps) and the reference amle on a crop."""
+    import torch
+    import imscript_amd as sp
+    nn, niter = a.amle_nn, a.niter if a.niter is not None else 100
+    g, _, hole_frac = make_inputs(torch, a.size, a.channels, a.seed, "cuda:0")
+    g = g[:a.amle_planes].contiguous()
+    out = torch.empty_like(g)
+    for _ in range(max(1, a.warmup)):
+        sp.amle(g, nn, niter, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        sp.amle(g, nn, niter, out=out)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps
+    sp.profile(True)
+    sp.amle(g, nn, niter, out=out)
+    torch.cuda.synchronize()
+    t_ms, t_launch, t_bytes = sp.profile_read("amle_tile")
+    p_ms, _, _ = sp.profile_read("amle_prep")
+    sp.profile(False)
+    px = g.numel()
+    achieved = t_bytes / (t_ms * 1e-3) / 1e9
+    eff = 12.0 * niter * px / (t_ms * 1e-3) / 1e9
+    var = int(os.environ.get("SIMPOIS_HIP_AMLE_VARIANT", "0") or 0)
+    r1 = ["32x64x16", "32x128x16", "64x64x16", "16x128x32", "32x64x8",
+          "64x32x16", "32x32x16", "16x64x16", "32x64x32", "64x64x8"]
+    shape = (r1[var if 0 <= var < 10 else 0] if nn <= 4
+             else "32x32x8" if nn <= 8 else "32x32x4")
+    res = {"metric": "AMLE hole filling, Mpix/s (amle, AMLE_NN=%d, AMLE_NITER=%d)"
+                     % (nn, niter),
+           "value": round(px / 1e6 / dt, 1), "unit": "Mpix/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(dt * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"amle of {a.amle_planes} plane(s) of "
+                                  f"{a.size}x{a.size} float32, sum of sinusoids "
+                                  f"with {hole_frac:.0%} rectangle holes 8-512 px, "
+                                  f"{niter} raster sweeps of the {nn}-neighbour "
+                                  "update from the maximum",
+                      "output": "bit-identical to the reference amle (tsup); the "
+                                "lower bound tinf, which feeds only the stderr "
+                                "trace, is not swept"},
+           "roofline": {"kernel": f"k_amle_tile (tile {shape} = BIxBJxBT: BT "
+                                  "raster sweeps fused per pass, one launch per "
+                                  "wavefront of skewed tiles)",
+                        "bound": "hbm", "achieved": round(achieved, 1),
+                        "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                        "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
+                        "algorithmic_bytes": t_bytes, "ms": round(t_ms, 3),
+                        "launches": t_launch, "prep_ms": round(p_ms, 3),
+                        "sweep_GBps_effective": round(eff, 1),
+                        "sweep_frac_effective": round(eff / PEAK_HBM_GBPS, 4)}}
+    # with the reference's trace (tinf swept too, a line every tenth sweep)
+    import contextlib
+    with open(os.devnull, "w") as dn, contextlib.redirect_stderr(dn):
+        fd = os.dup(2)
+        os.dup2(dn.fileno(), 2)
+        try:
+            t0 = time.perf_counter()
+            sp.amle(g, nn, niter, out=out, trace=True)
+            torch.cuda.synchronize()
+            tt = time.perf_counter() - t0
+        finally:
+            os.dup2(fd, 2)
+            os.close(fd)
+    res["with_trace"] = {"ms": round(tt * 1e3, 3),
+                         "Mpix_s": round(px / 1e6 / tt, 1),
+                         "what": "tsup and tinf swept, diagnostics every tenth sweep "
+                                 "printed as the reference does"}
+    if not a.no_cpu:
+        res["cpu_baseline"] = cpu_leg_amle(g, sp, nn, niter, a.amle_cpu_side)
+    emit(res)
+
+
 def stream_copy_ceiling(torch, sp, nbytes=1 << 30, reps=10):
     """Bandwidth of the library's plain copy kernel over nbytes (read +
     write counted), timed with events on the current stream: the measured
@@ -668,9 +773,13 @@ def main():
                     help="C2 from the source image: the solver computes the "
                          "data term as plambda's x,l Laplacian of it inside "
                          "the solve (plambda v x,l | simpois -f, fused)")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "lap"],
+    ap.add_argument("--config", default="c2",
+                    choices=["c1", "c2", "c3", "c4", "c5", "lap", "amle"],
                     help="c2 (default) is the headline; the others print their "
                          "own JSON line")
+    ap.add_argument("--amle-nn", type=int, default=4)
+    ap.add_argument("--amle-planes", type=int, default=1)
+    ap.add_argument("--amle-cpu-side", type=int, default=2048)
     ap.add_argument("--c3-size", type=int, default=2048)
     ap.add_argument("--c4-count", type=int, default=256)
     ap.add_argument("--c5-batch", type=int, default=64)
@@ -682,7 +791,7 @@ def main():
         sys.exit(2)
     if a.config != "c2":
         {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5,
-         "lap": run_lap}[a.config](a)
+         "lap": run_lap, "amle": run_amle}[a.config](a)
         return
 
     import torch

@@ -42,6 +42,8 @@ _SIGNATURES = {
     "poisson_extension_by_cg": (None, [_P, _P, _P, _I, _I, _P, _I, _F]),
     "biharmonic_extension_by_cg": (None, [_P, _P, _P, _I, _I, _P, _I, _F]),
     "main_simpois": (_I, [_I, ctypes.POINTER(ctypes.c_char_p)]),
+    "amle": (None, [_P, _P, _I, _I]),
+    "main_amle": (_I, [_I, ctypes.POINTER(ctypes.c_char_p)]),
     "sp_params_default": (None, [ctypes.POINTER(sp_params)]),
     "sp_last_error": (ctypes.c_char_p, []),
     "sp_solve_host": (_I, [_P, _P, _P, _I, _I, _I, ctypes.POINTER(sp_params),
@@ -62,6 +64,8 @@ _SIGNATURES = {
     "sp_stream_copy": (_I, [_P, _P, ctypes.c_size_t, _P]),
     "sp_plambda_lap_device": (_I, [_P, _P, _I, _I, _I, _I, _I, _P]),
     "sp_plambda_lap_host": (_I, [_P, _P, _I, _I, _I, _I, _I]),
+    "sp_amle_device": (_I, [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P]),
+    "sp_amle_host": (_I, [_P, _P, _P, _I, _I, _I, _I, _I, _I]),
 }
 
 

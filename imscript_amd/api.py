@@ -208,3 +208,39 @@ def plambda_lap(x, variant=0, boundary=1, out=None, stream=None):
     check(lib.sp_plambda_lap_device(out.data_ptr(), x.data_ptr(), w, h, pd, variant,
                                     boundary, ctypes.c_void_p(s.cuda_stream)))
     return out
+
+
+def amle(x, nn=4, niter=100, init=None, trace=False, out=None, stream=None):
+    """AMLE hole filling of amle.c on planar float32 planes (h, w) or
+    (pd, h, w): the NaN pixels of each plane are filled by niter raster
+    sweeps of the AMLE_NN = nn neighbour scheme started from the plane's
+    maximum (or from `init` at the holes, as AMLE_INIT does).  trace=True
+    also sweeps the lower bound and prints the reference's diagnostic line
+    every tenth sweep on stderr.  numpy arrays run through the host entry,
+    torch device tensors through the device entry (out may alias x)."""
+    if isinstance(x, np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        x3 = x.reshape((1,) + x.shape) if x.ndim == 2 else x
+        pd, h, w = x3.shape
+        i3 = None
+        if init is not None:
+            i3 = np.ascontiguousarray(init, dtype=np.float32).reshape(x3.shape)
+        y = np.empty_like(x3)
+        check(lib.sp_amle_host(_ptr(y), _ptr(x3), _ptr(i3), w, h, pd, int(nn),
+                               int(niter), int(bool(trace))))
+        return y.reshape(x.shape)
+    import torch
+    if x.dtype != torch.float32 or not x.is_cuda or not x.is_contiguous():
+        raise TypeError("x must be a contiguous float32 device tensor or a numpy array")
+    pd, h, w = (1,) + tuple(x.shape) if x.dim() == 2 else tuple(x.shape)
+    if init is not None and (init.dtype != torch.float32 or not init.is_cuda or
+                             not init.is_contiguous() or init.numel() != x.numel()):
+        raise TypeError("init must be a contiguous float32 device tensor of x's size")
+    if out is None:
+        out = torch.empty_like(x)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib.sp_amle_device(out.data_ptr(), x.data_ptr(),
+                             None if init is None else init.data_ptr(), w, h, pd,
+                             int(nn), int(niter), int(bool(trace)),
+                             ctypes.c_void_p(s.cuda_stream)))
+    return out

@@ -1,9 +1,11 @@
 /*
- * sp_cli.c -- the simpois command line and the image I/O it needs.
+ * sp_cli.c -- the simpois and amle command lines and the image I/O they
+ * need.
  *
  * main_simpois keeps the reference's flags, defaults, stdin/stdout
  * handling, mask semantics and exit codes (simpois.c:351-439), and calls
- * the GPU solver through poisson_solver_separable.
+ * the GPU solver through poisson_solver_separable.  main_amle and amle do
+ * the same for amle.c:203-246 and :325-366 through sp_amle_host.
  *
  * Image I/O is the subset of iio that runs without external libraries:
  *   read : NPY 1.0 (iio.c:3216-3295), PFM (iio.c:2599-2616, see below)
@@ -509,5 +511,157 @@ int main_simpois(int argc, char *argv[])
 	free(img_i);
 	free(img_m);
 	free(img_f);
+	return 0;
+}
+
+/* ================================================================ amle */
+
+/* SMART_PARAMETER (smapa.h:19-42): getenv + sscanf("%lf") on first use,
+ * with the same message on stderr */
+static double amle_knob(const char *name, double dflt, int *known, double *val)
+{
+	if (*known) return *val;
+	fprintf(stderr, "scanning the environment for \"%s\"... ", name);
+	const char *sv = getenv(name);
+	double y;
+	if (sv && sscanf(sv, "%lf", &y) == 1) {
+		fprintf(stderr, "got value %g\n", y);
+		*val = y;
+	} else {
+		fprintf(stderr, "kept default value %g\n", dflt);
+		*val = dflt;
+	}
+	*known = 1;
+	return *val;
+}
+
+static int g_niter_known, g_nn_known;
+static double g_niter_val, g_nn_val;
+
+/* fail() of fail.c:62-80: message and exit(-1) */
+static void amle_fail(const char *msg)
+{
+	fprintf(stderr, "\nFAIL(\"%s\"): %s\n\n", g_cmdline, msg);
+	fflush(NULL);
+	exit(-1);
+}
+
+/* amle(y, x, w, h) of amle.c:203-246 on the GPU: AMLE_NITER and AMLE_NN
+ * (smart parameters, defaults 100 and 4), AMLE_INIT (an image file whose
+ * values start tsup at the holes), the diagnostic line every tenth sweep
+ * on stderr (SIMPOIS_HIP_QUIET=1 leaves it out and skips the tinf sweeps
+ * it needs).  AMLE_NN outside 1..32 is refused (the reference reads past
+ * its neighbour table there). */
+void amle(float *y, float *x, int w, int h)
+{
+	const size_t n = (size_t)w * h;
+	int holes = 0;
+	for (size_t k = 0; k < n && !holes; k++) holes = isnan(x[k]);
+	const double vit = amle_knob("AMLE_NITER", 100, &g_niter_known, &g_niter_val);
+	const int niter = vit > 0 ? (int)ceil(vit) : 0;
+	int nn = 4;
+	if (niter > 0 && holes)
+		nn = (int)amle_knob("AMLE_NN", 4, &g_nn_known, &g_nn_val);
+	float *init = NULL;
+	const char *fi = getenv("AMLE_INIT");
+	if (fi) {
+		int ww, hh;
+		init = read_mask(fi, &ww, &hh);
+		if ((long)ww * hh != (long)w * h) amle_fail("init size mismatch");
+	}
+	const char *q = getenv("SIMPOIS_HIP_QUIET");
+	const int trace = !(q && atoi(q));
+	if (sp_amle_host(y, x, init, w, h, 1, nn, niter, trace)) {
+		fprintf(stderr, "amle: %s\n", sp_last_error());
+		exit(1);
+	}
+	free(init);
+}
+
+/* help texts of amle.c:292-323, kept verbatim as part of the interface */
+static const char *amle_help_usage = "usage:\n\t"
+"simpois [in [mask [out]]]";
+static const char *amle_help_oneliner = "Absolutely Minimizing Lipschitz Extension";
+static const char *amle_help_version = "amle 1.0\n\nWritten by mnhrdt";
+static const char *amle_help_long =
+"Interpolate by AMLE (Absolutely Minimizing Lipschitz Extension).\n"
+"\n"
+"The pixels to fill-in are specified by NAN values in the input image\n"
+"or by a separate user-provided mask.  The interpolated is defined\n"
+"as that whose maximal slope is the lowest possible, recursively in all\n"
+"subsets on the region of interest.\n"
+"\n"
+"Usage: amle in.npy mask.png out.npy\n"
+"   or: amle in.npy mask.png > out.npy\n"
+"   or: amle in.npy > out.npy\n"
+"   or: cat in.npy | amle > out.npy\n"
+"\n"
+"Options:\n"
+" -h\tdisplay short help message\n"
+" --help\tdisplay longer help message\n"
+"\n"
+"Environement:\n"
+" AMLE_NN\tNeigborhood size (4, 6, 8, 16, 26 or 32, default=4)\n"
+" AMLE_TAU\tTimestep for the iterations (default=0.25)\n"
+" AMLE_NITER\tNumber of iterations at each scale (default=100)\n"
+" AMLE_INIT\tInitial image for the iterations (default=zeros)\n"
+"\n"
+"Examples:\n"
+" amle in.npy > out.npy               Fill-in the holes in a DSM\n"
+" AMLE_NN=32 amle in.npy > out.npy    More precise, slower solver\n"
+"\n"
+"Report bugs to <enric.meinhardt@ens-paris-saclay.fr>.";
+
+static void amle_maybe_help(const char *s)
+{
+	if (!s || !strcmp(s, "-h")) { puts(amle_help_usage); exit(0); }
+	if (!strcmp(s, "-?")) { puts(amle_help_oneliner); exit(0); }
+	if (!strcmp(s, "--help")) { puts(amle_help_long); exit(0); }
+	if (!strcmp(s, "--version")) { puts(amle_help_version); exit(0); }
+	if (!strncmp(s, "--man", 5)) {
+		fprintf(stderr, "amle: man pages are not available\n");
+		exit(1);
+	}
+	if (!strcmp(s, "--help-oneliner")) puts(amle_help_oneliner);
+}
+
+/* The amle command line (amle.c:325-366): amle [in [mask [out]]], stdin
+ * and stdout by default, mask > 0 makes a hole, scalar images (colour
+ * inputs are converted to gray as iio_read_image_float does). */
+int main_amle(int c, char *v[])
+{
+	static char cmd[1024];
+	cmd[0] = 0;
+	for (int i = 0; i < c; i++) {
+		if (i) strncat(cmd, " ", sizeof cmd - strlen(cmd) - 1);
+		strncat(cmd, v[i], sizeof cmd - strlen(cmd) - 1);
+	}
+	g_cmdline = cmd;
+	if (c == 2) amle_maybe_help(v[1]);
+	if (c > 4 || (c > 1 && v[1][0] == '-' && v[1][1] == 'h')) {
+		fprintf(stderr, "usage:\n\t%s [in [mask [out]]]\n", *v);
+		return 1;
+	}
+	const char *fin = c > 1 ? v[1] : "-";
+	const char *fmask = c > 2 ? v[2] : "";
+	const char *fout = c > 3 ? v[3] : "-";
+	int w[2], h[2];
+	float *in = read_mask(fin, w, h);
+	float *mask = NULL;
+	if (fmask[0]) {
+		mask = read_mask(fmask, w + 1, h + 1);
+		if (w[0] != w[1] || h[0] != h[1])
+			amle_fail("image and mask file size mismatch");
+	}
+	const size_t n = (size_t)w[0] * h[0];
+	float *out = xalloc(n * sizeof(float));
+	if (mask)
+		for (size_t k = 0; k < n; k++)
+			if (mask[k] > 0) in[k] = NAN;
+	amle(out, in, w[0], h[0]);
+	write_split(fout, out, w[0], h[0], 1);
+	free(out);
+	free(mask);
+	free(in);
 	return 0;
 }

@@ -12,6 +12,7 @@
 #include "sp_internal.h"
 #include "sp_cg.h"
 #include "sp_tiles.h"
+#include "sp_amle.h"
 #include "../../include/simpois_hip.h"
 
 #include <algorithm>
@@ -1267,6 +1268,210 @@ extern "C" int sp_adjoint_batch_device(int n, const sp_adjoint_image *imgs,
 	spk::launch_adj_gather(d_ap, d_bw, (int)bw.size(), s);
 	HIPCHK(hipGetLastError());
 	HIPCHK(hipStreamSynchronize(s));
+	return SP_OK;
+}
+
+// ------------------------------------------------------------------ AMLE
+//
+// amle (amle.c:203-246) on pd planes: the bounds of each plane, then niter
+// raster-order sweeps of tsup (and of tinf when the trace is on) on the
+// skewed tiles of sp_amle.h; the output is tsup (amle.c:244).  tinf only
+// feeds the diagnostic line (amle.c:226-233), so it is swept only with the
+// trace.  The schedules and scratch of the last call are kept for the next
+// call of the same size and parameters.
+
+struct AmleSched {
+	int T = 0;
+	GsTile *d_tiles = nullptr;
+	GsInst *d_insts = nullptr;
+	std::vector<int> off;
+	long px = 0;                      // pixel-sweep updates per run
+};
+
+struct AmlePlan {
+	int w = 0, h = 0, pd = 0, nn = 0, niter = 0, trace = 0, R = 1, var = 0;
+	Plan mem;                         // owns the device buffers
+	float *c = nullptr, *tinf = nullptr, *snap = nullptr;
+	float2 *mm = nullptr;
+	void *part = nullptr;
+	const float *u_for = nullptr;     // output buffer the instances point at
+	std::vector<AmleSched> sched;
+};
+
+static AmleSched *amle_sched(AmlePlan &A, int T)
+{
+	for (auto &sc : A.sched)
+		if (sc.T == T) return &sc;
+	return nullptr;
+}
+
+static int amle_build(AmlePlan &A, int T)
+{
+	if (T <= 0 || amle_sched(A, T)) return SP_OK;
+	const int ninst = A.trace ? 2 * A.pd : A.pd;
+	std::vector<GsTile> tiles;
+	AmleSched sc;
+	sc.T = T;
+	int bt = 0;
+	if (!amle_with_shape(A.R, A.var, [&](auto sh) {
+			using S = decltype(sh);
+			build_tiles<S>(A.w, A.h, T, ninst, tiles, sc.off);
+			bt = S::BT;
+		}))
+		return set_err(SP_ERR_ARG, "no AMLE tile shape %d for radius %d", A.var, A.R);
+	sc.px = (long)ninst * ((T + bt - 1) / bt) * A.w * A.h;
+	TRY(A.mem.upload(&sc.d_tiles, tiles));
+	TRY(A.mem.alloc(&sc.d_insts, ninst));
+	A.sched.push_back(sc);
+	return SP_OK;
+}
+
+// point the instances of every schedule at the current output buffer
+static int amle_bind(AmlePlan &A, float *d_out)
+{
+	if (A.u_for == d_out) return SP_OK;
+	const size_t n = (size_t)A.w * A.h;
+	for (auto &sc : A.sched) {
+		std::vector<GsInst> in;
+		for (int l = 0; l < A.pd; l++)
+			in.push_back({d_out + n * l, A.c + n * l, A.w, A.h, sc.T, 0.0f, nullptr});
+		if (A.trace)
+			for (int l = 0; l < A.pd; l++)
+				in.push_back({A.tinf + n * l, A.c + n * l, A.w, A.h, sc.T, 0.0f, nullptr});
+		HIPCHK(hipMemcpy(sc.d_insts, in.data(), in.size() * sizeof(GsInst),
+				hipMemcpyHostToDevice));
+	}
+	A.u_for = d_out;
+	return SP_OK;
+}
+
+static int amle_sweeps(AmlePlan &A, int T, hipStream_t s)
+{
+	if (T <= 0) return SP_OK;
+	AmleSched *sc = amle_sched(A, T);
+	if (!sc) return set_err(SP_ERR_ARG, "AMLE schedule for %d sweeps missing", T);
+	ProfScope ps("amle_tile", s);
+	for (size_t k = 0; k + 1 < sc->off.size(); k++)
+		spk::launch_amle_tiles(A.R, A.var, sc->d_tiles + sc->off[k],
+				sc->off[k + 1] - sc->off[k], sc->d_insts, A.nn, s);
+	// algorithmic bytes: u and c read and u written once per pixel per pass
+	ps.add((long)sc->off.size() - 1, 12.0 * (double)sc->px);
+	HIPCHK(hipGetLastError());
+	return SP_OK;
+}
+
+static int amle_args(const float *out, const float *in, int w, int h, int pd,
+		int nn, int niter)
+{
+	if (!out || !in || w < 1 || h < 1 || pd < 1)
+		return set_err(SP_ERR_ARG, "bad image arguments");
+	if ((long)w * h > (1L << 30) || (long)w * h * pd > 0x7fffffffL)
+		return set_err(SP_ERR_ARG, "image too large");
+	if (nn < 1 || nn > AMLE_NN_MAX)
+		return set_err(SP_ERR_ARG, "AMLE neighbourhood size must be 1..32");
+	if (niter < 0) return set_err(SP_ERR_ARG, "negative AMLE iteration count");
+	return SP_OK;
+}
+
+static std::mutex g_amle_mu;
+static AmlePlan *g_amle = nullptr;
+
+// tile shape of the 4-neighbour sweeps: SIMPOIS_HIP_AMLE_VARIANT (read at
+// each call; 0 = the default 32x64x16), variant 0 for larger neighbourhoods
+static int amle_variant(int R)
+{
+	const char *e = getenv("SIMPOIS_HIP_AMLE_VARIANT");
+	const int v = e ? atoi(e) : 0;
+	return R == 1 && v > 0 && v < amle_shape_count(1) ? v : 0;
+}
+
+extern "C" int sp_amle_device(float *d_out, const float *d_in, const float *d_init,
+		int w, int h, int pd, int nn, int niter, int trace, void *stream)
+{
+	TRY(amle_args(d_out, d_in, w, h, pd, nn, niter));
+	TRY(ensure_device());
+	hipStream_t s = (hipStream_t)stream;
+	std::lock_guard<std::mutex> lk(g_amle_mu);
+	const int R = amle_radius(nn);
+	const int var = amle_variant(R);
+	trace = trace != 0;
+	AmlePlan *A = g_amle;
+	if (!A || A->w != w || A->h != h || A->pd != pd || A->nn != nn ||
+			A->niter != niter || A->trace != trace || A->var != var) {
+		delete g_amle;
+		g_amle = A = new AmlePlan;
+		A->w = w; A->h = h; A->pd = pd; A->nn = nn; A->niter = niter;
+		A->trace = trace; A->R = R; A->var = var;
+		const size_t n = (size_t)w * h * pd;
+		int rc = A->mem.alloc(&A->c, n);
+		if (!rc && trace) rc = A->mem.alloc(&A->tinf, n);
+		if (!rc && trace) rc = A->mem.alloc(&A->snap, 2 * n);
+		if (!rc) rc = A->mem.alloc(&A->mm, pd);
+		if (!rc) rc = A->mem.alloc((char **)&A->part, spk::amle_part_bytes(w * h, pd));
+		if (!rc && !trace) rc = amle_build(*A, niter);
+		for (int it = 0; !rc && trace && it < niter; it += 10) {
+			rc = amle_build(*A, 1);
+			if (!rc) rc = amle_build(*A, std::min(niter - it - 1, 9));
+		}
+		if (rc) { delete g_amle; g_amle = nullptr; return rc; }
+	}
+	TRY(amle_bind(*A, d_out));
+	const int n = w * h;
+	{
+		ProfScope ps("amle_prep", s);
+		spk::launch_amle_prep(d_in, d_init, d_out, A->c, A->tinf, A->part, A->mm,
+				n, pd, s);
+		ps.add(3, 4.0 * n * pd * (trace ? 5 : 4));
+	}
+	HIPCHK(hipGetLastError());
+	if (!trace) return amle_sweeps(*A, niter, s);
+	// trace: every tenth sweep alone, between snapshots, then the diagnostic
+	// line of each plane as amle.c:230-233 prints it
+	const size_t np = (size_t)n * pd;
+	const int nb = spk::amle_part_blocks(n);
+	std::vector<double> hp((size_t)nb * pd * 5);
+	for (int it = 0; it < niter; it += 10) {
+		HIPCHK(hipMemcpyAsync(A->snap, d_out, np * 4, hipMemcpyDeviceToDevice, s));
+		HIPCHK(hipMemcpyAsync(A->snap + np, A->tinf, np * 4, hipMemcpyDeviceToDevice, s));
+		TRY(amle_sweeps(*A, 1, s));
+		spk::launch_amle_diag(A->c, A->tinf, d_out, A->snap + np, A->snap, n, pd,
+				(double *)A->part, s);
+		HIPCHK(hipGetLastError());
+		HIPCHK(hipMemcpyAsync(hp.data(), A->part, hp.size() * 8, hipMemcpyDeviceToHost, s));
+		HIPCHK(hipStreamSynchronize(s));
+		for (int l = 0; l < pd; l++) {
+			double cnt = 0, e = 0, ea = 0, ai = 0, as = 0;
+			for (int b = 0; b < nb; b++) {
+				const double *q = &hp[((size_t)l * nb + b) * 5];
+				cnt += q[0];
+				if (q[1] > e) e = q[1];
+				ea += q[2]; ai += q[3]; as += q[4];
+			}
+			fprintf(stderr, "iter %d, e = {%g %g}, eactus = {%g %g}\n", it,
+					(float)e, (float)((float)ea / (float)cnt), (float)ai, (float)as);
+		}
+		TRY(amle_sweeps(*A, std::min(niter - it - 1, 9), s));
+	}
+	return SP_OK;
+}
+
+extern "C" int sp_amle_host(float *out, const float *in, const float *init,
+		int w, int h, int pd, int nn, int niter, int trace)
+{
+	TRY(amle_args(out, in, w, h, pd, nn, niter));
+	TRY(ensure_device());
+	const size_t n = (size_t)w * h * pd;
+	Plan tmp;   // owns the device buffers
+	float *d_in = nullptr, *d_out = nullptr, *d_init = nullptr;
+	TRY(tmp.alloc(&d_in, n));
+	TRY(tmp.alloc(&d_out, n));
+	HIPCHK(hipMemcpy(d_in, in, n * 4, hipMemcpyHostToDevice));
+	if (init) {
+		TRY(tmp.alloc(&d_init, n));
+		HIPCHK(hipMemcpy(d_init, init, n * 4, hipMemcpyHostToDevice));
+	}
+	TRY(sp_amle_device(d_out, d_in, d_init, w, h, pd, nn, niter, trace, nullptr));
+	HIPCHK(hipMemcpy(out, d_out, n * 4, hipMemcpyDeviceToHost));
 	return SP_OK;
 }
 

@@ -51,6 +51,22 @@ void launch_finalize(const SpPlane *lv, float *const *outs,
 void launch_plambda_lap(float *out, const float *in, int w, int h, int pd,
 		int variant, int mode, hipStream_t s);
 
+// AMLE (sp_amle.h): bounds of each plane into mm and the initial u / c
+// (and tinf when non-null); part is scratch of amle_part_bytes
+size_t amle_part_bytes(int n, int pd);
+int amle_part_blocks(int n);
+void launch_amle_prep(const float *in, const float *init, float *u, float *c,
+		float *tinf, void *part, float2 *mm, int n, int pd, hipStream_t s);
+// one launch of the AMLE tiles of one wavefront (shape variant var of radius R)
+int amle_tile_lds_bytes(int R, int var);
+void launch_amle_tiles(int R, int var, const GsTile *tiles, int ntiles,
+		const GsInst *insts, int nn, hipStream_t s);
+// per-block partials (holes, max, sums) of the trace diagnostics: 5 doubles
+// per block, amle_part_blocks(n) blocks per plane
+void launch_amle_diag(const float *c, const float *tinf, const float *tsup,
+		const float *tinf0, const float *tsup0, int n, int pd, double *part,
+		hipStream_t s);
+
 // dst[0..4*n4) = src[0..4*n4), 16-byte aligned
 void launch_stream_copy(float *dst, const float *src, size_t n4, hipStream_t s);
 

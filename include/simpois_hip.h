@@ -55,6 +55,19 @@ void biharmonic_extension_by_cg(float *out, float *in, float *dat, int w, int h,
  * Replaces simpois.c:351-439 (main_simpois). */
 int main_simpois(int argc, char *argv[]);
 
+/* AMLE hole filling of one w x h plane: y = x with the NaN pixels filled by
+ * AMLE_NITER (default 100) raster-order sweeps of the AMLE_NN (default 4)
+ * neighbour scheme from the plane's maximum, or from the AMLE_INIT image at
+ * the holes; the diagnostic line every tenth sweep on stderr
+ * (SIMPOIS_HIP_QUIET=1 omits it).  Synchronous; failures print a message
+ * and exit.  AMLE_NN outside 1..32 is refused.
+ * Replaces amle.c:203-246 (amle). */
+void amle(float *y, float *x, int w, int h);
+
+/* The amle command line: amle [in [mask [out]]] (NPY/PFM I/O).
+ * Replaces amle.c:325-366 (main_amle). */
+int main_amle(int argc, char *argv[]);
+
 /* ------------------------------------------------------- status API */
 
 enum {
@@ -179,6 +192,21 @@ int sp_plambda_lap_device(float *d_out, const float *d_in, int w, int h, int pd,
 		int variant, int boundary, void *stream);
 int sp_plambda_lap_host(float *out, const float *in, int w, int h, int pd,
 		int variant, int boundary);
+
+/* AMLE hole filling (amle.c:203-246) of pd planar w x h planes, each on its
+ * own: the NaN pixels of d_in get niter raster-order sweeps of the
+ * nn-neighbour AMLE update (nn = AMLE_NN, 1..32; the neighbour table of
+ * amle.c:16-29), started from the plane's maximum or, where d_init is not
+ * NULL, from d_init at the holes (AMLE_INIT).  Bit-identical to the
+ * reference's output.  trace = 1 also sweeps the lower bound tinf and prints
+ * the reference's diagnostic line every tenth sweep (amle.c:230-233, one
+ * line per plane); the output does not depend on it.  d_out may alias d_in.
+ * Device pointers; asynchronous on `stream` without the trace.  The host
+ * form copies through device memory and returns when done. */
+int sp_amle_device(float *d_out, const float *d_in, const float *d_init,
+		int w, int h, int pd, int nn, int niter, int trace, void *stream);
+int sp_amle_host(float *out, const float *in, const float *init,
+		int w, int h, int pd, int nn, int niter, int trace);
 
 /* dst[0..n) = src[0..n) on the device with a plain streaming kernel
  * (k_stream_copy): the bandwidth ceiling the solver kernels are compared

@@ -11,6 +11,9 @@
 //
 // usage: gs_emulate W H T tstep seed holefrac withf mode [variant]
 //   mode: tiles | small; variant: tile shape variant of the radius (sp_gs.h)
+//        gs_emulate amle W H T nn seed holefrac [variant]
+//   the AMLE sweeps (sp_amle.h) against the oracle's raster-order sweeps
+//   (oracle/amle_oracle.c); variant: AMLE tile shape of radius amle_radius(nn)
 // prints "OK <n>" (n = holes) or "MISMATCH ..." / "CONFLICT ..." and exits 1
 #include <cstdio>
 #include <cstdlib>
@@ -18,9 +21,12 @@
 #include <vector>
 #include "../../imscript_amd/csrc/sp_gs.h"
 #include "../../imscript_amd/csrc/sp_tiles.h"
+#include "../../imscript_amd/csrc/sp_amle.h"
 
 extern "C" void orc_gs_sweeps(float *u, const float *g, const float *f, int w,
 		int h, float tstep, int niter);
+extern "C" void orc_amle_sweeps(float *x, const unsigned char *hole, int w, int h,
+		int nn, int nsweep);
 
 static unsigned rng_state;
 static double urand()
@@ -29,8 +35,9 @@ static double urand()
 	return (rng_state >> 8) * (1.0 / 16777216.0);
 }
 
-template <class S>
-static int run_tiles(float *u, const float *c, int W, int H, int T, float t)
+template <class S, class Step>
+static int run_tiles(float *u, const float *c, int W, int H, int T, float t,
+		Step step)
 {
 	using G = Geo<S>;
 	std::vector<GsTile> tiles;
@@ -51,7 +58,7 @@ static int run_tiles(float *u, const float *c, int W, int H, int T, float t)
 			for (int tid = 0; tid < G::NT; tid++) tile_load<S>(ctx[k], in, tid);
 		for (int k = 0; k < n; k++)
 			for (int sg = 0; sg < G::NSTEP; sg++)
-				for (int tid = 0; tid < G::NT; tid++) tile_step<S>(ctx[k], tid, sg);
+				for (int tid = 0; tid < G::NT; tid++) step(ctx[k], tid, sg);
 		// write-write conflicts between tiles of one wavefront
 		for (int k = 0; k < n; k++) {
 			const TileCtx<S> &cc = ctx[k];
@@ -104,8 +111,58 @@ static int run_small(float *u, const float *c, int W, int H, int T, float t)
 	return 0;
 }
 
+// AMLE: random values with holes, a random start at the holes (as the
+// bounds or an initial image would give), nn neighbours; seeds from 100
+// put +-Inf among the values, whose NaN offspring the scans must order as
+// the reference does
+static int main_amle(int argc, char **argv)
+{
+	if (argc != 8 && argc != 9) {
+		fprintf(stderr, "usage: gs_emulate amle W H T nn seed holefrac [variant]\n");
+		return 2;
+	}
+	const int W = atoi(argv[2]), H = atoi(argv[3]), T = atoi(argv[4]);
+	const int nn = atoi(argv[5]);
+	rng_state = (unsigned)atoi(argv[6]) * 2654435761u + 11;
+	const double frac = atof(argv[7]);
+	const int var = argc == 9 ? atoi(argv[8]) : 0;
+	const size_t n = (size_t)W * H;
+	std::vector<float> u(n), uref(n), c(n);
+	std::vector<unsigned char> hole(n);
+	int holes = 0;
+	const bool infs = atoi(argv[6]) >= 100;   // seeds from 100: some +-Inf values
+	for (size_t q = 0; q < n; q++) {
+		u[q] = (float)(urand() * 100 - 50);
+		if (urand() < 0.2) u[q] = (float)(int)u[q];     // ties
+		if (infs && urand() < 0.01) u[q] = urand() < 0.5 ? INFINITY : -INFINITY;
+		hole[q] = urand() < frac;
+		holes += hole[q];
+		c[q] = hole[q] ? 0.0f : sp_float(SP_KNOWN_BITS);
+		uref[q] = u[q];
+	}
+	orc_amle_sweeps(uref.data(), hole.data(), W, H, nn, T);
+	int rc = 0;
+	if (!amle_with_shape(amle_radius(nn), var, [&](auto sh) {
+			using S = decltype(sh);
+			rc = run_tiles<S>(u.data(), c.data(), W, H, T, 0.0f,
+				[nn](const TileCtx<S> &cc, int tid, int sg) {
+					amle_tile_step<S>(cc, tid, sg, nn); }); })) {
+		printf("NO SHAPE %d for nn %d\n", var, nn);
+		return 2;
+	}
+	if (rc) return 1;
+	for (size_t q = 0; q < n; q++)
+		if (memcmp(&u[q], &uref[q], 4)) {
+			printf("MISMATCH at %zu,%zu: %.9g vs %.9g\n", q % W, q / W, u[q], uref[q]);
+			return 1;
+		}
+	printf("OK %d\n", holes);
+	return 0;
+}
+
 int main(int argc, char **argv)
 {
+	if (argc > 1 && !strcmp(argv[1], "amle")) return main_amle(argc, argv);
 	if (argc != 9 && argc != 10) {
 		fprintf(stderr, "usage: gs_emulate W H T tstep seed holefrac withf "
 				"tiles|small [variant]\n");
@@ -141,7 +198,9 @@ int main(int argc, char **argv)
 	if (small) rc = R == 1 ? run_small<1>(u, c, W, H, T, teff)
 		: R == 2 ? run_small<2>(u, c, W, H, T, teff) : run_small<3>(u, c, W, H, T, teff);
 	else if (!sp_with_shape(R, var, [&](auto sh) {
-			rc = run_tiles<decltype(sh)>(u, c, W, H, T, teff); })) {
+			using S = decltype(sh);
+			rc = run_tiles<S>(u, c, W, H, T, teff,
+				[](const TileCtx<S> &cc, int tid, int sg) { tile_step<S>(cc, tid, sg); }); })) {
 		printf("NO SHAPE %d for radius %d\n", var, R);
 		return 2;
 	}

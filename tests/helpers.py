@@ -52,6 +52,9 @@ def oracle():
         o.orc_rel_residual.argtypes = [V, V, V, I, I, I, V]
         o.orc_rel_residual.restype = ctypes.c_double
         o.orc_plambda_lap.argtypes = [V, V, I, I, I, I, I]
+        o.orc_amle.argtypes = [V, V, I, I, I, I, V, V]
+        o.orc_amle.restype = I
+        o.orc_amle_sweeps.argtypes = [V, V, I, I, I, I]
         _oracle = o
     return _oracle
 
@@ -186,3 +189,43 @@ def load_plambda_golden():
         y = np.ascontiguousarray(np.moveaxis(z[f"out{k}"], -1, 0))
         out.append((x, y, int(variant), int(mode)))
     return out
+
+
+AMLE_GOLDEN = os.path.join(ROOT, "tests", "golden", "amle.npz")
+AMLE_BIN = os.path.join(ROOT, "imscript_amd", "lib", "amle")
+
+
+def oracle_amle(x, nn=4, niter=100, init=None, diag=False):
+    """Oracle amle of one (h, w) plane; with diag=True also the per-sweep
+    (e, ea, actus_inf, actus_sup) rows of the trace."""
+    x = np.ascontiguousarray(x, np.float32)
+    h, w = x.shape
+    y = np.empty_like(x)
+    i = None if init is None else np.ascontiguousarray(init, np.float32)
+    d = np.zeros((max(niter, 1), 4), np.float32) if diag else None
+    oracle().orc_amle(ptr(y), ptr(x), w, h, int(nn), int(niter), ptr(i), ptr(d))
+    return (y, d[:niter]) if diag else y
+
+
+def load_amle_golden():
+    """Cases of tests/golden/amle.npz: dicts with x (mask applied), nn,
+    niter, init, out, trace (the reference's stderr lines), mask, raw x."""
+    z = np.load(AMLE_GOLDEN)
+    cases = []
+    for k, (nn, niter) in enumerate(z["meta"]):
+        raw = z[f"in{k}"]
+        mask = z[f"mask{k}"] if f"mask{k}" in z else None
+        x = raw.copy()
+        if mask is not None:
+            x[mask > 0] = np.nan
+        cases.append(dict(x=x, raw=raw, mask=mask, nn=int(nn), niter=int(niter),
+                          init=z[f"init{k}"] if f"init{k}" in z else None,
+                          out=z[f"out{k}"], trace=str(z[f"trace{k}"])))
+    return cases
+
+
+def amle_trace_lines(diag):
+    """The reference's trace lines (amle.c:230-233) from per-sweep rows."""
+    return "\n".join("iter %d, e = {%g %g}, eactus = {%g %g}" %
+                     (it, *(float(v) for v in diag[it]))
+                     for it in range(0, len(diag), 10))

@@ -75,3 +75,25 @@ def test_cli_size_mismatch_returns_fprintf_count(tmp_path):
     p = run_cli(["-i", str(tmp_path / "g.npy"), "-m", str(tmp_path / "m.npy"),
                  "-o", str(tmp_path / "o.npy")])
     assert p.returncode == len("input sizes mismatch (i,m)")
+
+
+AMLE_BIN = os.path.join(ROOT, "imscript_amd", "lib", "amle")
+
+
+@pytest.mark.parametrize("flag,start", [("-h", b"usage:"), ("--help", b"Interpolate by AMLE"),
+                                        ("--version", b"amle 1.0"),
+                                        ("-?", b"Absolutely Minimizing")])
+def test_amle_cli_help_flags(flag, start):
+    p = subprocess.run([AMLE_BIN, flag], capture_output=True, timeout=60)
+    assert p.returncode == 0 and p.stdout.startswith(start)
+
+
+def test_amle_cli_too_many_arguments_exits_1():
+    p = subprocess.run([AMLE_BIN, "a", "b", "c", "d"], capture_output=True, timeout=60)
+    assert p.returncode == 1 and b"[in [mask [out]]]" in p.stderr
+
+
+def test_amle_cli_unreadable_input_exits_255(tmp_path):
+    p = subprocess.run([AMLE_BIN, str(tmp_path / "missing.npy")], capture_output=True,
+                       timeout=60)
+    assert p.returncode == 255 and b"IIO_ERROR" in p.stderr

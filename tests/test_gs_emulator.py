@@ -22,14 +22,17 @@ def emulator():
     if shutil.which("g++") is None:
         pytest.skip("g++ not available")
     os.makedirs(BUILD, exist_ok=True)
-    obj = os.path.join(BUILD, "oracle_asan.o")
     san = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all"]
-    subprocess.run(["gcc", "-O1", "-g", "-std=gnu99", "-ffp-contract=off", *san,
-                    "-c", os.path.join(ROOT, "oracle", "simpois_oracle.c"),
-                    "-o", obj], check=True)
+    objs = []
+    for src in ("simpois_oracle.c", "amle_oracle.c"):
+        obj = os.path.join(BUILD, src.replace(".c", "_asan.o"))
+        subprocess.run(["gcc", "-O1", "-g", "-std=gnu99", "-ffp-contract=off", *san,
+                        "-c", os.path.join(ROOT, "oracle", src), "-o", obj],
+                       check=True)
+        objs.append(obj)
     subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-ffp-contract=off", *san,
                     "-o", EXE, os.path.join(ROOT, "tests", "cpu", "gs_emulate.cpp"),
-                    obj, "-lm"], check=True)
+                    *objs, "-lm"], check=True)
     return EXE
 
 
@@ -105,3 +108,44 @@ def test_whole_level_kernel_limits(emulator, args):
                        text=True, timeout=600, env=env)
     assert p.returncode == 0, p.stdout + p.stderr
     assert p.stdout.startswith("OK")
+
+
+AMLE_CASES = [
+    # W H T nn seed holefrac [variant]
+    "40 30 5 4 1 0.5",
+    "150 140 40 4 2 0.7",
+    "200 3 33 4 3 0.5",
+    "3 170 21 4 4 0.6",
+    "1 1 7 4 1 1",
+    "1 9 5 4 2 0.6",
+    "61 47 9 1 5 0.5",
+    "61 47 9 2 6 0.5",
+    "61 47 9 3 7 0.5",
+    "70 45 17 6 4 0.6",
+    "70 45 17 8 8 0.6",
+    "2 2 9 8 3 0.5",
+    "66 59 11 16 5 0.6",
+    "63 71 10 24 6 0.6",
+    "80 77 12 32 9 0.7",
+    "5 4 6 32 2 0.8",
+    "90 70 23 4 101 0.5",
+    "47 33 9 4 103 0.3",
+    "61 58 12 8 105 0.5",
+    "55 51 9 32 107 0.5",
+]
+
+
+@pytest.mark.parametrize("args", AMLE_CASES)
+def test_amle_tiles_match_raster_sweeps(emulator, args):
+    """AMLE tile steps (sp_amle.h) on every neighbourhood size, against
+    amle.c's raster-order sweeps restated in oracle/amle_oracle.c."""
+    r = subprocess.run([emulator, "amle", *args.split()], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.startswith("OK"), r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("var", range(15))
+def test_every_amle_tile_shape_matches_raster_sweeps(emulator, var):
+    r = subprocess.run([emulator, "amle", "150", "140", "40", "4", str(var), "0.6",
+                        str(var)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.startswith("OK"), r.stdout + r.stderr
