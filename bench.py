@@ -720,6 +720,87 @@ def run_amle(a):
     emit(res)
 
 
+def cpu_leg_heatd(x, sp, n, t, side):
+    """The reference heatd on a side x side crop of plane 0 (file I/O
+    included, one thread), bit-compared with the GPU run on the same crop."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "heatd")
+    if not os.path.exists(ref):
+        return None
+    _, bits_equal, _ = _oracle_imports()
+    crop = np.ascontiguousarray(x[0, :side, :side].cpu().numpy())
+    with tempfile.TemporaryDirectory() as td:
+        np.save(os.path.join(td, "x.npy"), crop)
+        t0 = time.perf_counter()
+        subprocess.run([ref, "-n", str(n), "-t", repr(t), os.path.join(td, "x.npy"),
+                        os.path.join(td, "o.npy")], check=True)
+        dt = time.perf_counter() - t0
+        y = np.load(os.path.join(td, "o.npy")).reshape(side, side)
+    got = sp.heatd(crop, n, t)
+    return {"value": round(side * side / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1,
+            "kind": "reference", "seconds": round(dt, 3), "bit_exact": bits_equal(got, y),
+            "sample": f"heatd -n {n} -t {t} on the top-left {side}x{side} crop of "
+                      "plane 0, file I/O included"}
+
+
+def run_heatd(a):
+    """heatd's explicit diffusion (heatd.c) of the C2 source images,
+    4096x4096x3 planar float32, n steps: throughput of the whole diffusion,
+    the step kernel's HBM roofline (8 B per pixel per step: read the plane,
+    write the next; 12 B more with a diffusion-matrix field) and the
+    reference heatd on a crop."""
+    import torch
+    import imscript_amd as sp
+    n = a.niter if a.niter is not None else 100
+    t = 0.25
+    make_inputs(torch, a.size, a.channels, a.seed, "cuda:0")
+    v = make_inputs.source
+    A = None
+    if a.heatd_field:
+        rng = torch.Generator(device="cuda")
+        rng.manual_seed(a.seed)
+        A = torch.stack([0.2 + torch.rand(a.size, a.size, device="cuda", generator=rng),
+                         torch.rand(a.size, a.size, device="cuda", generator=rng) - 0.5,
+                         0.2 + torch.rand(a.size, a.size, device="cuda", generator=rng)],
+                        -1).contiguous()
+    out = torch.empty_like(v)
+    for _ in range(max(1, a.warmup)):
+        sp.heatd(v, n, t, A, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        sp.heatd(v, n, t, A, out=out)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps
+    sp.profile(True)
+    sp.heatd(v, n, t, A, out=out)
+    torch.cuda.synchronize()
+    k_ms, k_launch, k_bytes = sp.profile_read("heatd_step")
+    sp.profile(False)
+    px = v.numel()
+    per = k_bytes / max(k_launch, 1)
+    achieved = per / (k_ms * 1e-3 / max(k_launch, 1)) / 1e9
+    res = {"metric": "heatd diffusion, Mpix/s (n steps per pixel)",
+           "value": round(px / 1e6 / dt, 1), "unit": "Mpix/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(dt * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"heatd -n {n} -t {t}{' -A field' if A is not None else ''} "
+                                  f"of {a.size}x{a.size}x{a.channels} float32 planar "
+                                  "source images (sum of sinusoids)",
+                      "Mpix_steps_per_s": round(px * n / 1e6 / dt, 1)},
+           "roofline": {"kernel": "k_heatd_step<%s> (4 pixels per thread, one launch "
+                                  "per step)" % ("field" if A is not None else "isotropic"),
+                        "bound": "hbm", "achieved": round(achieved, 1),
+                        "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                        "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
+                        "algorithmic_bytes_per_launch": per,
+                        "avg_launch_us": round(k_ms * 1e3 / max(k_launch, 1), 2),
+                        "launches": k_launch}}
+    if not a.no_cpu:
+        res["cpu_baseline"] = cpu_leg_heatd(v, sp, n, t, a.heatd_cpu_side)
+    emit(res)
+
+
 def stream_copy_ceiling(torch, sp, nbytes=1 << 30, reps=10):
     """Bandwidth of the library's plain copy kernel over nbytes (read +
     write counted), timed with events on the current stream: the measured
@@ -774,9 +855,12 @@ def main():
                          "data term as plambda's x,l Laplacian of it inside "
                          "the solve (plambda v x,l | simpois -f, fused)")
     ap.add_argument("--config", default="c2",
-                    choices=["c1", "c2", "c3", "c4", "c5", "lap", "amle"],
+                    choices=["c1", "c2", "c3", "c4", "c5", "lap", "amle", "heatd"],
                     help="c2 (default) is the headline; the others print their "
                          "own JSON line")
+    ap.add_argument("--heatd-field", action="store_true",
+                    help="heatd with a per-pixel diffusion-matrix field (-A)")
+    ap.add_argument("--heatd-cpu-side", type=int, default=2048)
     ap.add_argument("--amle-nn", type=int, default=4)
     ap.add_argument("--amle-planes", type=int, default=1)
     ap.add_argument("--amle-cpu-side", type=int, default=2048)
@@ -791,7 +875,7 @@ def main():
         sys.exit(2)
     if a.config != "c2":
         {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5,
-         "lap": run_lap, "amle": run_amle}[a.config](a)
+         "lap": run_lap, "amle": run_amle, "heatd": run_heatd}[a.config](a)
         return
 
     import torch

@@ -19,11 +19,11 @@ from .api import (poisson_solver_separable, poisson_rec,
                   poisson_extension_by_cg, biharmonic_extension_by_cg,
                   simpois, solve_device, solve_batch_device, residual_device,
                   profile, profile_read, autotune_report, stream_copy,
-                  plambda_lap, amle)
+                  plambda_lap, amle, heatd)
 
 __all__ = ["lib", "lib_path", "SimpoisError", "sp_params", "sp_stats",
            "sp_image", "default_params", "exported_symbols",
            "poisson_solver_separable", "poisson_rec",
            "poisson_extension_by_cg", "biharmonic_extension_by_cg", "simpois",
            "solve_device", "solve_batch_device", "residual_device", "profile",
-           "profile_read", "autotune_report", "stream_copy", "plambda_lap", "amle"]
+           "profile_read", "autotune_report", "stream_copy", "plambda_lap", "amle", "heatd"]

@@ -44,6 +44,10 @@ _SIGNATURES = {
     "main_simpois": (_I, [_I, ctypes.POINTER(ctypes.c_char_p)]),
     "amle": (None, [_P, _P, _I, _I]),
     "main_amle": (_I, [_I, ctypes.POINTER(ctypes.c_char_p)]),
+    "linear_diffusion_separable": (None, [_P, _P, _I, _I, _I, _P, _I, _F]),
+    "linear_diffusion": (None, [_P, _P, _I, _I, _P, _I, _F]),
+    "linear_diffusion_step": (None, [_P, _P, _I, _I, _P, _F]),
+    "main_heatd": (_I, [_I, ctypes.POINTER(ctypes.c_char_p)]),
     "sp_params_default": (None, [ctypes.POINTER(sp_params)]),
     "sp_last_error": (ctypes.c_char_p, []),
     "sp_solve_host": (_I, [_P, _P, _P, _I, _I, _I, ctypes.POINTER(sp_params),
@@ -66,6 +70,8 @@ _SIGNATURES = {
     "sp_plambda_lap_host": (_I, [_P, _P, _I, _I, _I, _I, _I]),
     "sp_amle_device": (_I, [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P]),
     "sp_amle_host": (_I, [_P, _P, _P, _I, _I, _I, _I, _I, _I]),
+    "sp_heatd_device": (_I, [_P, _P, _P, _I, _I, _I, _I, _F, _P]),
+    "sp_heatd_host": (_I, [_P, _P, _P, _I, _I, _I, _I, _F]),
 }
 
 

@@ -244,3 +244,34 @@ def amle(x, nn=4, niter=100, init=None, trace=False, out=None, stream=None):
                              int(nn), int(niter), int(bool(trace)),
                              ctypes.c_void_p(s.cuda_stream)))
     return out
+
+
+def heatd(x, n=1, t=0.25, A=None, out=None, stream=None):
+    """heatd's explicit linear diffusion of planar float32 planes (h, w) or
+    (pd, h, w): n steps of x + t * L(x) with the 9-point scheme of the
+    diffusion-matrix field A ((h, w, 3) interleaved a, b, c; None: the
+    isotropic Laplacian), clamped borders.  numpy arrays run through the
+    host entry, torch device tensors through the device entry (x is not
+    written; out must not alias it)."""
+    if isinstance(x, np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        x3 = x.reshape((1,) + x.shape) if x.ndim == 2 else x
+        pd, h, w = x3.shape
+        a = None if A is None else np.ascontiguousarray(A, np.float32).reshape(h, w, 3)
+        y = np.empty_like(x3)
+        check(lib.sp_heatd_host(_ptr(y), _ptr(x3), _ptr(a), w, h, pd, int(n), float(t)))
+        return y.reshape(x.shape)
+    import torch
+    if x.dtype != torch.float32 or not x.is_cuda or not x.is_contiguous():
+        raise TypeError("x must be a contiguous float32 device tensor or a numpy array")
+    pd, h, w = (1,) + tuple(x.shape) if x.dim() == 2 else tuple(x.shape)
+    if A is not None and (A.dtype != torch.float32 or not A.is_cuda or
+                          not A.is_contiguous() or A.numel() != 3 * w * h):
+        raise TypeError("A must be a contiguous float32 device tensor of h*w*3 values")
+    if out is None:
+        out = torch.empty_like(x)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib.sp_heatd_device(out.data_ptr(), x.data_ptr(),
+                              None if A is None else A.data_ptr(), w, h, pd, int(n),
+                              float(t), ctypes.c_void_p(s.cuda_stream)))
+    return out

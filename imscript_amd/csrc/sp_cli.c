@@ -1,11 +1,12 @@
 /*
- * sp_cli.c -- the simpois and amle command lines and the image I/O they
- * need.
+ * sp_cli.c -- the simpois, amle and heatd command lines and the image I/O
+ * they need.
  *
  * main_simpois keeps the reference's flags, defaults, stdin/stdout
  * handling, mask semantics and exit codes (simpois.c:351-439), and calls
  * the GPU solver through poisson_solver_separable.  main_amle and amle do
- * the same for amle.c:203-246 and :325-366 through sp_amle_host.
+ * the same for amle.c:203-246 and :325-366 through sp_amle_host, main_heatd
+ * for heatd.c:109-150 through linear_diffusion_separable.
  *
  * Image I/O is the subset of iio that runs without external libraries:
  *   read : NPY 1.0 (iio.c:3216-3295), PFM (iio.c:2599-2616, see below)
@@ -663,5 +664,52 @@ int main_amle(int c, char *v[])
 	free(out);
 	free(mask);
 	free(in);
+	return 0;
+}
+
+/* =============================================================== heatd */
+
+/* The heatd command line (heatd.c:109-150): heatd [-n 1] [-t 0.25]
+ * [-A field] [in [out]]; -g, -m and -D are accepted and ignored as there.
+ * The field is an interleaved w x h x 3 image (a, b, c).  -n 0 writes the
+ * input unchanged (the reference writes an unset buffer). */
+int main_heatd(int c, char *v[])
+{
+	static char cmd[1024];
+	cmd[0] = 0;
+	for (int i = 0; i < c; i++) {
+		if (i) strncat(cmd, " ", sizeof cmd - strlen(cmd) - 1);
+		strncat(cmd, v[i], sizeof cmd - strlen(cmd) - 1);
+	}
+	g_cmdline = cmd;
+	int nsteps = atoi(take_option(&c, &v, "n", "1"));
+	float tstep = atof(take_option(&c, &v, "t", "0.25"));
+	(void)take_option(&c, &v, "g", "");
+	char *fa = take_option(&c, &v, "A", "");
+	(void)take_option(&c, &v, "m", "");
+	(void)take_option(&c, &v, "D", "");
+	if (c != 1 && c != 2 && c != 3)
+		return fprintf(stderr, "usage:\n\t%s [in [out]]\n", *v);
+	const char *fin = c > 1 ? v[1] : "-";
+	const char *fout = c > 2 ? v[2] : "-";
+	int w, h, pd;
+	float *x = read_split(fin, &w, &h, &pd);
+	float *A = NULL;
+	if (*fa) {
+		raw_image im;
+		read_any(fa, &im, NULL, NULL);
+		if (im.w != w || im.h != h || im.pd != 3)
+			return fprintf(stderr, "ERROR: bad A dimensions"
+				"(%d %d %d) ((%d %d))\n", im.w, im.h, im.pd, w, h);
+		A = im.px;
+	}
+	const size_t n = (size_t)w * h * pd;
+	float *y = xalloc(n * sizeof(float));
+	if (nsteps > 0) linear_diffusion_separable(y, x, w, h, pd, A, nsteps, tstep);
+	else memcpy(y, x, n * sizeof(float));
+	write_split(fout, y, w, h, pd);
+	free(y);
+	free(A);
+	free(x);
 	return 0;
 }

@@ -1475,6 +1475,101 @@ extern "C" int sp_amle_host(float *out, const float *in, const float *init,
 	return SP_OK;
 }
 
+// ------------------------------------------------------------------ heatd
+//
+// n explicit diffusion steps (heatd.c:58-103), one launch each, ping-pong
+// between the output and a stream-ordered scratch plane set, so that the
+// last step lands in the output.
+
+static int heatd_args(const float *out, const float *in, int w, int h, int pd, int n)
+{
+	if (!out || !in || w < 1 || h < 1 || pd < 1)
+		return set_err(SP_ERR_ARG, "bad image arguments");
+	if ((long)w * h * pd > 0x7fffffffL) return set_err(SP_ERR_ARG, "image too large");
+	if (n < 0) return set_err(SP_ERR_ARG, "negative number of diffusion steps");
+	if (out == in) return set_err(SP_ERR_ARG, "out must not alias in");
+	return SP_OK;
+}
+
+extern "C" int sp_heatd_device(float *d_out, const float *d_in, const float *d_A,
+		int w, int h, int pd, int n, float t, void *stream)
+{
+	TRY(heatd_args(d_out, d_in, w, h, pd, n));
+	TRY(ensure_device());
+	hipStream_t s = (hipStream_t)stream;
+	const size_t np = (size_t)w * h * pd;
+	if (n == 0) {
+		HIPCHK(hipMemcpyAsync(d_out, d_in, np * 4, hipMemcpyDeviceToDevice, s));
+		return SP_OK;
+	}
+	float *tmp = nullptr;
+	if (n > 1) HIPCHK(hipMallocAsync((void **)&tmp, np * 4, s));
+	ProfScope ps("heatd_step", s);
+	const float *src = d_in;
+	for (int k = 1; k <= n; k++) {
+		float *dst = (n - k) & 1 ? tmp : d_out;
+		spk::launch_heatd_step(dst, src, d_A, w, h, pd, t, s);
+		src = dst;
+	}
+	// algorithmic bytes: the plane read and written once per step, the
+	// field (3 floats per pixel) read once per step and plane
+	ps.add(n, (double)n * np * (8.0 + (d_A ? 12.0 : 0.0)));
+	const hipError_t e = hipGetLastError();
+	if (tmp) HIPCHK(hipFreeAsync(tmp, s));
+	HIPCHK(e);
+	return SP_OK;
+}
+
+extern "C" int sp_heatd_host(float *out, const float *in, const float *A,
+		int w, int h, int pd, int n, float t)
+{
+	TRY(heatd_args(out, in, w, h, pd, n));
+	TRY(ensure_device());
+	const size_t np = (size_t)w * h * pd;
+	Plan tmp;   // owns the device buffers
+	float *d_in = nullptr, *d_out = nullptr, *d_A = nullptr;
+	TRY(tmp.alloc(&d_in, np));
+	TRY(tmp.alloc(&d_out, np));
+	HIPCHK(hipMemcpy(d_in, in, np * 4, hipMemcpyHostToDevice));
+	if (A) {
+		TRY(tmp.alloc(&d_A, (size_t)w * h * 3));
+		HIPCHK(hipMemcpy(d_A, A, (size_t)w * h * 12, hipMemcpyHostToDevice));
+	}
+	TRY(sp_heatd_device(d_out, d_in, d_A, w, h, pd, n, t, nullptr));
+	HIPCHK(hipMemcpy(out, d_out, np * 4, hipMemcpyDeviceToHost));
+	return SP_OK;
+}
+
+// drop-ins of heatd.c:58-103 (host buffers, synchronous; failures print a
+// message and exit as the reference's allocation failures do)
+static void heatd_or_die(int rc)
+{
+	if (rc) {
+		fprintf(stderr, "heatd: %s\n", sp_last_error());
+		exit(1);
+	}
+}
+
+extern "C" void linear_diffusion_separable(float *y, float *x, int w, int h, int pd,
+		float *A, int n, float t)
+{
+	if (n <= 0) return;   // the reference writes neither y nor x
+	heatd_or_die(sp_heatd_host(y, x, A, w, h, pd, n, t));
+	memcpy(x, y, (size_t)w * h * pd * sizeof(float));   // x ends as y (:84-85)
+}
+
+extern "C" void linear_diffusion(float *y, float *x, int w, int h, float *A, int n,
+		float t)
+{
+	linear_diffusion_separable(y, x, w, h, 1, A, n, t);
+}
+
+extern "C" void linear_diffusion_step(float *y, float *x, int w, int h, float *A,
+		float t)
+{
+	heatd_or_die(sp_heatd_host(y, x, A, w, h, 1, 1, t));
+}
+
 // ------------------------------------------------- plambda's Laplacian
 
 static int lap_args(const float *out, const float *in, int w, int h, int pd,

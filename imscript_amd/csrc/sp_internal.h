@@ -67,6 +67,10 @@ void launch_amle_diag(const float *c, const float *tinf, const float *tsup,
 		const float *tinf0, const float *tsup0, int n, int pd, double *part,
 		hipStream_t s);
 
+// one explicit diffusion step of heatd (A: w*h*3 interleaved field or null)
+void launch_heatd_step(float *out, const float *in, const float *A, int w, int h,
+		int pd, float t, hipStream_t s);
+
 // dst[0..4*n4) = src[0..4*n4), 16-byte aligned
 void launch_stream_copy(float *dst, const float *src, size_t n4, hipStream_t s);
 

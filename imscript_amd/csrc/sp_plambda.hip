@@ -53,13 +53,13 @@ static __device__ __forceinline__ float lp_sample(const float *x, int w, int h,
 	return x[(size_t)j * w + i];
 }
 
-// grid: x = column chunks of LP_T*LP_PX, y = rows of all planes
+// grid: x = rows of all planes, y = column chunks of LP_T*LP_PX
 template <int V, bool VEC>
 __global__ __launch_bounds__(LP_T) void k_plambda_lap(float *__restrict__ out,
 		const float *__restrict__ in, int w, int h, int mode)
 {
-	const int row = blockIdx.y, l = row / h, j = row - l * h;
-	const int i0 = (blockIdx.x * LP_T + threadIdx.x) * LP_PX;
+	const int row = blockIdx.x, l = row / h, j = row - l * h;
+	const int i0 = (blockIdx.y * LP_T + threadIdx.x) * LP_PX;
 	if (i0 >= w) return;
 	const float *x = in + (size_t)l * w * h;
 	float *y = out + (size_t)l * w * h + (size_t)j * w;
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(LP_T) void k_plambda_lap(float *__restrict__ out,
 void launch_plambda_lap(float *out, const float *in, int w, int h, int pd,
 		int variant, int mode, hipStream_t s)
 {
-	const dim3 grid((w + LP_T * LP_PX - 1) / (LP_T * LP_PX), h * pd);
+	const dim3 grid(h * pd, (w + LP_T * LP_PX - 1) / (LP_T * LP_PX));
 	const bool vec = !(w & 3) && !((uintptr_t)out & 15) && !((uintptr_t)in & 15);
 	if (variant) {
 		if (vec) k_plambda_lap<1, true><<<grid, LP_T, 0, s>>>(out, in, w, h, mode);

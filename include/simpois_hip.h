@@ -68,6 +68,23 @@ void amle(float *y, float *x, int w, int h);
  * Replaces amle.c:325-366 (main_amle). */
 int main_amle(int argc, char *argv[]);
 
+/* Explicit linear diffusion of heatd: n steps of y = x + t * L(x) on each
+ * of pd planar w x h planes, L the 9-point scheme of the diffusion matrix
+ * (a, b, c) of each pixel divided by 4 (A: w*h*3 interleaved, shared by
+ * all planes; NULL: the identity, the 5-point Laplacian), clamped borders.
+ * Bit-identical to the reference.  As there, x ends equal to y; with
+ * n <= 0 neither is written.  Synchronous; failures print a message and
+ * exit.  Replace heatd.c:90-103 (linear_diffusion_separable), :72-88
+ * (linear_diffusion) and :58-70 (linear_diffusion_step, x unchanged). */
+void linear_diffusion_separable(float *y, float *x, int w, int h, int pd,
+		float *A, int n, float t);
+void linear_diffusion(float *y, float *x, int w, int h, float *A, int n, float t);
+void linear_diffusion_step(float *y, float *x, int w, int h, float *A, float t);
+
+/* The heatd command line: heatd [-n 1] [-t 0.25] [-A field] [in [out]].
+ * Replaces heatd.c:109-150 (main). */
+int main_heatd(int argc, char *argv[]);
+
 /* ------------------------------------------------------- status API */
 
 enum {
@@ -207,6 +224,16 @@ int sp_amle_device(float *d_out, const float *d_in, const float *d_init,
 		int w, int h, int pd, int nn, int niter, int trace, void *stream);
 int sp_amle_host(float *out, const float *in, const float *init,
 		int w, int h, int pd, int nn, int niter, int trace);
+
+/* heatd's diffusion (as linear_diffusion_separable) with a status code:
+ * d_out receives n steps from d_in, which is not written (n = 0 copies
+ * it); d_A is the w*h*3 interleaved field or NULL.  Device pointers,
+ * asynchronous on `stream` (scratch for n > 1 comes from the stream-ordered
+ * allocator).  The host form copies through device memory. */
+int sp_heatd_device(float *d_out, const float *d_in, const float *d_A,
+		int w, int h, int pd, int n, float t, void *stream);
+int sp_heatd_host(float *out, const float *in, const float *A,
+		int w, int h, int pd, int n, float t);
 
 /* dst[0..n) = src[0..n) on the device with a plain streaming kernel
  * (k_stream_copy): the bandwidth ceiling the solver kernels are compared

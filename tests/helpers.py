@@ -55,6 +55,8 @@ def oracle():
         o.orc_amle.argtypes = [V, V, I, I, I, I, V, V]
         o.orc_amle.restype = I
         o.orc_amle_sweeps.argtypes = [V, V, I, I, I, I]
+        o.orc_heatd.argtypes = [V, V, I, I, I, V, I, F]
+        o.orc_heatd_step.argtypes = [V, V, I, I, V, F]
         _oracle = o
     return _oracle
 
@@ -229,3 +231,33 @@ def amle_trace_lines(diag):
     return "\n".join("iter %d, e = {%g %g}, eactus = {%g %g}" %
                      (it, *(float(v) for v in diag[it]))
                      for it in range(0, len(diag), 10))
+
+
+HEATD_GOLDEN = os.path.join(ROOT, "tests", "golden", "heatd.npz")
+HEATD_BIN = os.path.join(ROOT, "imscript_amd", "lib", "heatd")
+
+
+def oracle_heatd(x, n=1, t=0.25, A=None):
+    """Oracle heatd of planar (pd, h, w) or (h, w) planes; A (h, w, 3) or
+    None.  Returns (y, x after the call) as the reference leaves them."""
+    x = np.array(x, np.float32, order="C", copy=True)
+    x3 = x.reshape((1,) + x.shape) if x.ndim == 2 else x
+    pd, h, w = x3.shape
+    a = None if A is None else np.ascontiguousarray(A, np.float32)
+    y = np.full_like(x3, np.nan)
+    oracle().orc_heatd(ptr(y), ptr(x3), w, h, pd, ptr(a), int(n), float(t))
+    return y.reshape(x.shape), x3.reshape(x.shape)
+
+
+def load_heatd_golden():
+    """Cases of tests/golden/heatd.npz as dicts: x planar (pd, h, w), the
+    interleaved input, A (h, w, 3) or None, n, t, out planar."""
+    z = np.load(HEATD_GOLDEN)
+    cases = []
+    for k, (n, t) in enumerate(z["meta"]):
+        xi = z[f"in{k}"]
+        cases.append(dict(xi=xi, x=np.ascontiguousarray(xi.transpose(2, 0, 1)),
+                          A=z[f"A{k}"] if f"A{k}" in z else None, n=int(n),
+                          t=float(t),
+                          out=np.ascontiguousarray(z[f"out{k}"].transpose(2, 0, 1))))
+    return cases

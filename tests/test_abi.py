@@ -97,3 +97,19 @@ def test_amle_cli_unreadable_input_exits_255(tmp_path):
     p = subprocess.run([AMLE_BIN, str(tmp_path / "missing.npy")], capture_output=True,
                        timeout=60)
     assert p.returncode == 255 and b"IIO_ERROR" in p.stderr
+
+
+HEATD_BIN = os.path.join(ROOT, "imscript_amd", "lib", "heatd")
+
+
+def test_heatd_cli_bad_positional_count_prints_usage():
+    p = subprocess.run([HEATD_BIN, "a", "b", "c"], capture_output=True, timeout=60)
+    assert p.returncode != 0 and b"[in [out]]" in p.stderr
+
+
+def test_heatd_cli_bad_field_dimensions(tmp_path):
+    np.save(tmp_path / "x.npy", np.zeros((4, 5, 1), np.float32))
+    np.save(tmp_path / "A.npy", np.zeros((4, 5, 2), np.float32))
+    p = subprocess.run([HEATD_BIN, "-A", str(tmp_path / "A.npy"), str(tmp_path / "x.npy"),
+                        str(tmp_path / "y.npy")], capture_output=True, timeout=60)
+    assert p.returncode != 0 and b"ERROR: bad A dimensions(5 4 2) ((5 4))" in p.stderr
