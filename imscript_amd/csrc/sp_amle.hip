@@ -149,9 +149,9 @@ __global__ __launch_bounds__(AM_T) void k_amle_diag(const float *c,
 
 static int am_chunks(int n, int *chunk)
 {
-	// about 64 blocks per plane of a large image, at least AM_T pixels each
-	int nb = (n + 65535) / 65536;
-	nb = nb < 1 ? 1 : nb > 1024 ? 1024 : nb;
+	// blocks of 16K pixels (64 per thread), at most 2048 per plane
+	int nb = (n + 16383) / 16384;
+	nb = nb < 1 ? 1 : nb > 2048 ? 2048 : nb;
 	*chunk = (n + nb - 1) / nb;
 	return (n + *chunk - 1) / *chunk;
 }

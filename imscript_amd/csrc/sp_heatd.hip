@@ -25,15 +25,19 @@ __global__ __launch_bounds__(HD_T) void k_heatd_step(float *__restrict__ out,
 	float *y = out + (size_t)l * w * h + (size_t)j * w;
 	// the 3 x (HD_PX + 2) window, rows and columns clamped (heatd.c:1-8)
 	float v[3][HD_PX + 2];
-	const bool inner = i0 >= 1 && i0 + HD_PX < w && j >= 1 && j + 1 < h;
-	if (VEC && inner) {
+	if (VEC) {
+		// w is a multiple of 4, so the thread's 4 pixels are in the row:
+		// one 16-byte load per clamped row, the two side columns clamped
+		const bool lt = i0 > 0, rt = i0 + HD_PX < w;
 #pragma unroll
 		for (int dy = 0; dy < 3; dy++) {
-			const float *r = x + (size_t)(j - 1 + dy) * w + i0;
+			int jj = j - 1 + dy;
+			jj = jj < 0 ? 0 : jj >= h ? h - 1 : jj;
+			const float *r = x + (size_t)jj * w + i0;
 			const float4 c = *(const float4 *)r;
-			v[dy][0] = r[-1];
+			v[dy][0] = lt ? r[-1] : c.x;
 			v[dy][1] = c.x; v[dy][2] = c.y; v[dy][3] = c.z; v[dy][4] = c.w;
-			v[dy][5] = r[HD_PX];
+			v[dy][5] = rt ? r[HD_PX] : c.w;
 		}
 	} else {
 #pragma unroll
