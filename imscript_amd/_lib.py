@@ -48,6 +48,12 @@ _SIGNATURES = {
     "linear_diffusion": (None, [_P, _P, _I, _I, _P, _I, _F]),
     "linear_diffusion_step": (None, [_P, _P, _I, _I, _P, _F]),
     "main_heatd": (_I, [_I, ctypes.POINTER(ctypes.c_char_p)]),
+    "iio_read_image_float_vec": (ctypes.POINTER(_F), [ctypes.c_char_p, _P, _P, _P]),
+    "iio_read_image_float_split": (ctypes.POINTER(_F), [ctypes.c_char_p, _P, _P, _P]),
+    "iio_read_image_float": (ctypes.POINTER(_F), [ctypes.c_char_p, _P, _P]),
+    "iio_write_image_float_vec": (None, [ctypes.c_char_p, _P, _I, _I, _I]),
+    "iio_write_image_float_split": (None, [ctypes.c_char_p, _P, _I, _I, _I]),
+    "iio_write_image_float": (None, [ctypes.c_char_p, _P, _I, _I]),
     "sp_params_default": (None, [ctypes.POINTER(sp_params)]),
     "sp_last_error": (ctypes.c_char_p, []),
     "sp_solve_host": (_I, [_P, _P, _P, _I, _I, _I, ctypes.POINTER(sp_params),

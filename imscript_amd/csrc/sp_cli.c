@@ -27,6 +27,8 @@
 #include <unistd.h>
 
 #include "../../include/simpois_hip.h"
+#include "../../include/simpois_iio.h"
+#include "sp_codecs.h"
 
 /* ------------------------------------------------------------ failures */
 
@@ -96,8 +98,8 @@ static void read_exact(FILE *f, void *buf, size_t n)
 	if (fread(buf, 1, n, f) != n) io_fail("input file ended before expected");
 }
 
-/* sample types of NPY descr strings (iio.c:3266-3283) */
-enum { T_F4, T_F8, T_U1, T_U2, T_U4, T_U8, T_I1, T_I2, T_I4, T_I8 };
+/* sample types (T_*) of NPY descr strings (iio.c:3266-3283) are those of
+ * sp_codecs.h */
 
 static int npy_type(const char *d, int *bytes, int *cplx)
 {
@@ -222,20 +224,72 @@ static void read_pfm(FILE *f, int upper, raw_image *im)
 	read_exact(f, im->px, n * sizeof(float));
 }
 
+/* the rest of a stream after the n bytes already read into head */
+static unsigned char *slurp(FILE *f, const unsigned char *head, size_t n, size_t *len)
+{
+	size_t cap = 1 << 16, k = n;
+	unsigned char *b = xalloc(cap);
+	memcpy(b, head, n);
+	for (;;) {
+		if (k == cap) {
+			unsigned char *t = realloc(b, cap *= 2);
+			if (!t) io_fail("out of memory reading an image");
+			b = t;
+		}
+		const size_t r = fread(b + k, 1, cap - k, f);
+		k += r;
+		if (!r) break;
+	}
+	*len = k;
+	return b;
+}
+
+/* PNG or TIFF through sp_codecs.c, samples converted as
+ * iio_convert_samples does (iio.c:980-1090: plain casts to float) */
+static void read_coded(FILE *f, const unsigned char *head, int png, raw_image *im,
+		unsigned char **keep_raw, int *raw_type)
+{
+	size_t n;
+	unsigned char *b = slurp(f, head, 4, &n);
+	codec_image c;
+	char err[256];
+	if ((png ? codec_read_png : codec_read_tiff)(b, n, &c, err, sizeof err))
+		io_fail("%s", err);
+	free(b);
+	const size_t ns = (size_t)c.w * c.h * c.pd;
+	const int sb = codec_type_bytes(c.type);
+	im->w = c.w; im->h = c.h; im->pd = c.pd;
+	im->px = xalloc(ns * sizeof(float));
+	for (size_t k = 0; k < ns; k++)
+		im->px[k] = sample_to_float(c.data + k * sb, c.type);
+	if (keep_raw) {
+		*raw_type = c.type;
+		*keep_raw = c.data;
+	} else {
+		free(c.data);
+	}
+}
+
 static void read_any(const char *name, raw_image *im, unsigned char **raw,
 		int *raw_type)
 {
 	FILE *f = open_in(name);
 	unsigned char m[4];
 	read_exact(f, m, 2);
+	if (raw) *raw = NULL;
 	if (m[0] == 'P' && (m[1] == 'f' || m[1] == 'F')) {
 		read_pfm(f, m[1] == 'F', im);
-		if (raw) *raw = NULL;
 	} else {
 		read_exact(f, m + 2, 2);
-		if (m[0] != 0x93 || m[1] != 'N' || m[2] != 'U' || m[3] != 'M')
-			io_fail("unsupported image format (NPY and PFM are supported)");
-		read_npy(f, im, raw, raw_type);
+		if (m[0] == 0x93 && m[1] == 'N' && m[2] == 'U' && m[3] == 'M')
+			read_npy(f, im, raw, raw_type);
+		else if ((m[0] == 'I' && m[1] == 'I' && m[2] == 42 && m[3] == 0) ||
+				(m[0] == 'M' && m[1] == 'M' && m[2] == 0 && m[3] == 42))
+			read_coded(f, m, 0, im, raw, raw_type);
+		else if (m[0] == 0x89 && m[1] == 'P' && m[2] == 'N' && m[3] == 'G')
+			read_coded(f, m, 1, im, raw, raw_type);
+		else
+			io_fail("unsupported image format (NPY, PFM, TIFF and PNG are supported)");
 	}
 	if (f != stdin) fclose(f);
 }
@@ -349,23 +403,83 @@ static void write_pfm(const char *name, const float *px, int w, int h, int pd)
 	if (f != stdout) fclose(f); else fflush(f);
 }
 
-/* iio_write_image_float_split (iio.c:6525-6531) and the writer dispatch of
- * iio_write_image_default (iio.c:6166-6497) for the formats kept here */
-static void write_split(const char *name, const float *planar, int w, int h, int pd)
+/* float to 8 / 16 bits as convert_datum does (iio.c:1036-1039): 0.5 added
+ * in double, clamped, truncated; NaN becomes 0 */
+static unsigned char to_u8(float v)
 {
-	size_t n = (size_t)w * h;
+	const double d = 0.5 + v;
+	return d > 0 ? (d < 0xff ? (unsigned char)d : 0xff) : 0;
+}
+
+static unsigned short to_u16(float v)
+{
+	const double d = 0.5 + v;
+	return d > 0 ? (d < 0xffff ? (unsigned short)d : 0xffff) : 0;
+}
+
+static void write_png(const char *name, const float *px, int w, int h, int pd, int b16)
+{
+	if (pd < 1 || pd > 4) io_fail("can not save %d-dimensional samples as PNG", pd);
+	const size_t ns = (size_t)w * h * pd;
+	unsigned char *p8 = b16 ? NULL : xalloc(ns);
+	unsigned short *p16 = b16 ? xalloc(ns * 2) : NULL;
+	for (size_t k = 0; k < ns; k++) {
+		if (b16) p16[k] = to_u16(px[k]);
+		else p8[k] = to_u8(px[k]);
+	}
+	FILE *f = open_out(name);
+	if (codec_write_png(f, p8, p16, w, h, pd)) io_fail("png write error");
+	if (f != stdout) fclose(f); else fflush(f);
+	free(p8);
+	free(p16);
+}
+
+/* float TIFF, LZW below 2000 x 2000 pixels unless IIOTIFF_PLAIN is set
+ * (iio.c:4043-4047) */
+static void write_tiff(const char *name, const float *px, int w, int h, int pd)
+{
+	const int lzw = (long)w * h < 2000L * 2000 && !getenv("IIOTIFF_PLAIN");
+	FILE *f = open_out(name);
+	if (codec_write_tiff(f, px, w, h, pd, lzw)) io_fail("error writing TIFF file");
+	if (f != stdout) fclose(f); else fflush(f);
+}
+
+/* the writer dispatch of iio_write_image_default (iio.c:6166-6497) for the
+ * formats kept here, on interleaved floats */
+static void write_vec(const char *name, const float *px, int w, int h, int pd)
+{
 	if (!strcmp(name, "-") && isatty(1)) {
 		printf("IMAGE %dx%d,%d %s\n", w, h, pd, "FLOAT");
 		return;
 	}
+	if (has_suffix(name, ".pfm") && (pd == 1 || pd == 3))
+		write_pfm(name, px, w, h, pd);
+	else if (has_suffix(name, ".npy"))
+		write_npy(name, px, w, h, pd);
+	else if (has_suffix(name, ".tiff") || has_suffix(name, ".tif") ||
+			has_suffix(name, ".TIFF") || has_suffix(name, ".TIF"))
+		write_tiff(name, px, w, h, pd);
+	else if (!strncmp(name, "TIFF:", 5))
+		write_tiff(name + 5, px, w, h, pd);
+	else if (!strncmp(name, "PNG:", 4))
+		write_png(name + 4, px, w, h, pd, 0);
+	else if (!strncmp(name, "PNG16:", 6))
+		write_png(name + 6, px, w, h, pd, 1);
+	else if (has_suffix(name, ".png") || has_suffix(name, ".PNG"))
+		write_png(name, px, w, h, pd, 0);
+	else
+		write_npy(name, px, w, h, pd);
+}
+
+/* iio_write_image_float_split (iio.c:6525-6531): planar to interleaved */
+static void write_split(const char *name, const float *planar, int w, int h, int pd)
+{
+	size_t n = (size_t)w * h;
 	float *px = xalloc(n * pd * sizeof(float));
 	for (int l = 0; l < pd; l++)
 		for (size_t i = 0; i < n; i++)
 			px[pd * i + l] = planar[n * l + i];
-	if (has_suffix(name, ".pfm") && (pd == 1 || pd == 3))
-		write_pfm(name, px, w, h, pd);
-	else
-		write_npy(name, px, w, h, pd);
+	write_vec(name, px, w, h, pd);
 	free(px);
 }
 
@@ -712,4 +826,45 @@ int main_heatd(int c, char *v[])
 	free(A);
 	free(x);
 	return 0;
+}
+
+/* ===================================================== iio entry points */
+
+/* iio.c:5747-5760 */
+float *iio_read_image_float_vec(const char *fname, int *w, int *h, int *pd)
+{
+	raw_image im;
+	read_any(fname, &im, NULL, NULL);
+	*w = im.w; *h = im.h; *pd = im.pd;
+	return im.px;
+}
+
+/* iio.c:5763-5771 */
+float *iio_read_image_float_split(const char *fname, int *w, int *h, int *pd)
+{
+	return read_split(fname, w, h, pd);
+}
+
+/* iio.c:6029-6048 */
+float *iio_read_image_float(const char *fname, int *w, int *h)
+{
+	return read_mask(fname, w, h);
+}
+
+/* iio.c:6516-6523 */
+void iio_write_image_float_vec(char *filename, float *x, int w, int h, int pd)
+{
+	write_vec(filename, x, w, h, pd);
+}
+
+/* iio.c:6525-6531 */
+void iio_write_image_float_split(char *filename, float *x, int w, int h, int pd)
+{
+	write_split(filename, x, w, h, pd);
+}
+
+/* iio.c:6570-6576 */
+void iio_write_image_float(char *filename, float *x, int w, int h)
+{
+	write_vec(filename, x, w, h, 1);
 }

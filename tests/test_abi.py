@@ -10,13 +10,16 @@ import pytest
 import imscript_amd
 from helpers import ROOT, SIMPOIS_BIN
 
-HEADER = os.path.join(ROOT, "include", "simpois_hip.h")
+HEADERS = [os.path.join(ROOT, "include", f) for f in sorted(os.listdir(
+    os.path.join(ROOT, "include"))) if f.endswith(".h")]
 
 
 def header_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\(", text, flags=re.M)
+    names = []
+    for hdr in HEADERS:
+        text = open(hdr).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\(", text, flags=re.M)
     return sorted(set(n for n in names if n not in ("if", "while")))
 
 
