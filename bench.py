@@ -774,11 +774,14 @@ def run_heatd(a):
     sp.profile(True)
     sp.heatd(v, n, t, A, out=out)
     torch.cuda.synchronize()
-    k_ms, k_launch, k_bytes = sp.profile_read("heatd_step")
+    multi = A is None and n > 1
+    k_ms, k_launch, k_bytes = sp.profile_read("heatd_multi" if multi else "heatd_step")
     sp.profile(False)
     px = v.numel()
     per = k_bytes / max(k_launch, 1)
     achieved = per / (k_ms * 1e-3 / max(k_launch, 1)) / 1e9
+    steps_per_launch = n / max(k_launch, 1)
+    eff = 8.0 * px * n / (k_ms * 1e-3) / 1e9
     res = {"metric": "heatd diffusion, Mpix/s (n steps per pixel)",
            "value": round(px / 1e6 / dt, 1), "unit": "Mpix/s", "n_gpus": 1,
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(dt * 1e3, 3),
@@ -788,14 +791,18 @@ def run_heatd(a):
                                   f"of {a.size}x{a.size}x{a.channels} float32 planar "
                                   "source images (sum of sinusoids)",
                       "Mpix_steps_per_s": round(px * n / 1e6 / dt, 1)},
-           "roofline": {"kernel": "k_heatd_step<%s> (4 pixels per thread, one launch "
-                                  "per step)" % ("field" if A is not None else "isotropic"),
+           "roofline": {"kernel": ("k_heatd_multi<T> (%.3g steps per launch on 64 x 48 "
+                                   "register tiles per wave)" % steps_per_launch) if multi
+                        else "k_heatd_step<%s> (4 pixels per thread, one launch per step)"
+                        % ("field" if A is not None else "isotropic"),
                         "bound": "hbm", "achieved": round(achieved, 1),
                         "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                         "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
                         "algorithmic_bytes_per_launch": per,
                         "avg_launch_us": round(k_ms * 1e3 / max(k_launch, 1), 2),
-                        "launches": k_launch}}
+                        "launches": k_launch,
+                        "step_GBps_effective": round(eff, 1),
+                        "step_frac_effective": round(eff / PEAK_HBM_GBPS, 4)}}
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_leg_heatd(v, sp, n, t, a.heatd_cpu_side)
     emit(res)

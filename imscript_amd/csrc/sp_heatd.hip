@@ -5,6 +5,7 @@
 // the nine products s[k] * v[k] are summed for k = 0..8 in order, zero
 // weights included (0 * Inf is NaN there too), without contraction.
 #include <stdint.h>
+#include <type_traits>
 #include "sp_internal.h"
 
 namespace spk {
@@ -93,6 +94,137 @@ __global__ __launch_bounds__(HD_T) void k_heatd_step(float *__restrict__ out,
 		for (int p = 0; p < HD_PX; p++)
 			if (i0 + p < w) y[i0 + p] = o[p];
 	}
+}
+
+// ------------------------------------------------ several steps per launch
+//
+// Isotropic diffusion only.  Each wave owns a tile of 64 columns (one per
+// lane) by HT_R rows held in registers, loaded with a halo of T pixels, and
+// runs T steps on it: after step k the values are exact everywhere but
+// within k pixels of the tile's edge, so the central OW x OH = (64 - 2T) x
+// (HT_R - 2T) pixels are written back.  Horizontal neighbours come from
+// the adjacent lanes, vertical ones from the lane's own registers; no LDS,
+// no barrier.  The last tile of a row or column is moved back to end at the
+// image's edge (its overlap with the previous tile is computed twice, with
+// the same result), so that the image's first and last rows sit at tile
+// rows T and HT_R - 1 - T, fixed when compiling; there the clamped borders
+// of heatd.c:1-8 are applied, and per lane at the first and last columns.
+// The planes must be at least OW x OH (the host's choice).
+//
+// Update: with the isotropic weights {0, 1, -0, 1, -4, 1, -0, 1, 0} the
+// reference's ordered sum r = 0 + 0*NW + N + (-0)*NE + W + (-4)*C + E +
+// (-0)*SW + S + 0*SE equals ((((0 + N) + W) + (-4)*C) + E) + S whenever the
+// four corners are finite: r starts at +0 and a sum is -0 only if both terms
+// are, so r is never -0 and adding a zero product leaves it unchanged.  A
+// non-finite corner makes its product, and r, NaN.
+
+constexpr int HT_R = 48;          // rows per wave tile
+constexpr int HT_WAVES = 4;       // wave tiles per block, stacked vertically
+
+// grid: x = tile columns of all planes (tiles_x per plane), y = groups of
+// HT_WAVES tile rows
+template <int T>
+__global__ __launch_bounds__(64 * HT_WAVES) void k_heatd_multi(float *__restrict__ out,
+		const float *__restrict__ in, int w, int h, int tiles_x, int tiles_y, float t)
+{
+	constexpr int OW = 64 - 2 * T, OH = HT_R - 2 * T;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int ty = blockIdx.y * HT_WAVES + wave;
+	if (ty >= tiles_y) return;
+	const int l = blockIdx.x / tiles_x, tx = blockIdx.x - l * tiles_x;
+	const float *x = in + (size_t)l * w * h;
+	float *y = out + (size_t)l * w * h;
+	const int x0 = min(tx * OW, w - OW), y0 = min(ty * OH, h - OH);
+	const int gi = x0 - T + lane, gj0 = y0 - T;
+	const int ci = gi < 0 ? 0 : gi >= w ? w - 1 : gi;
+	float v[HT_R];
+#pragma unroll
+	for (int r = 0; r < HT_R; r++) {
+		int gj = gj0 + r;
+		gj = gj < 0 ? 0 : gj >= h ? h - 1 : gj;
+		v[r] = x[(size_t)gj * w + ci];
+	}
+	// per-lane clamps wherever the tile's columns reach an image border
+	const bool atl = gi == 0, atr = gi == w - 1;
+	const bool edge_cols = x0 - T <= 0 || x0 + OW + T - 1 >= w - 1;
+	// clamped rows only in the first and the last tile of a column (image
+	// rows 0 and h-1 at tile rows T and HT_R-1-T); another tile whose rows
+	// reach past the image does not clamp, so rows within T-1 of such a
+	// border are wrong there and are left to the first / last tile
+	const bool top = y0 == 0, bot = y0 == h - OH;
+	for (int k = 0; k < T; k++) {
+		// window of old values: rows r-1 (u), r (c), r+1 (d), each with its
+		// west / east neighbours
+		// the corners of row r are the west / east neighbours of rows r-1
+		// and r+1: whether a row's pair is finite is kept with the row
+		float c = v[0];
+		float cw = __shfl_up(c, 1, 64), ce = __shfl_down(c, 1, 64);
+		if (edge_cols) { cw = atl ? c : cw; ce = atr ? c : ce; }
+		bool cf = isfinite(cw) && isfinite(ce);
+		float u = c;
+		bool uf = cf;
+#pragma unroll
+		for (int r = 0; r < HT_R; r++) {
+			const float d = r + 1 < HT_R ? v[r + 1] : c;
+			float dw = __shfl_up(d, 1, 64), de = __shfl_down(d, 1, 64);
+			if (edge_cols) { dw = atl ? d : dw; de = atr ? d : de; }
+			const bool df = isfinite(dw) && isfinite(de);
+			float n = u, so = d;
+			bool nf = uf, sf = df;
+			if (r == T && top) { n = c; nf = cf; }
+			if (r == HT_R - 1 - T && bot) { so = c; sf = cf; }
+			float q = 0.0f;
+			q = q + n;
+			q = q + cw;
+			q = q + -4.0f * c;
+			q = q + ce;
+			q = q + so;
+			q = nf && sf ? q : __builtin_nanf("");
+			const float nv = c + t * (q * 0.25f);
+			u = c; uf = cf;
+			c = d; cw = dw; ce = de; cf = df;
+			v[r] = nv;
+		}
+	}
+	if (lane < T || lane >= 64 - T) return;
+	const int jlo = top ? 0 : T - 1, jhi = bot ? h : h - T + 1;
+#pragma unroll
+	for (int r = T; r < HT_R - T; r++) {
+		const int gj = gj0 + r;
+		if (gj >= jlo && gj < jhi) y[(size_t)gj * w + gi] = v[r];
+	}
+}
+
+int heatd_multi_max() { return 8; }
+
+bool heatd_multi_fits(int w, int h, int T)
+{
+	return w >= 64 - 2 * T && h >= HT_R - 2 * T;
+}
+
+bool launch_heatd_multi(float *out, const float *in, int w, int h, int pd, int T,
+		float t, hipStream_t s)
+{
+	if (T < 1 || T > heatd_multi_max() || !heatd_multi_fits(w, h, T)) return false;
+	auto go = [&](auto tc) {
+		constexpr int TT = decltype(tc)::value;
+		constexpr int OW = 64 - 2 * TT, OH = HT_R - 2 * TT;
+		const int tiles_x = (w + OW - 1) / OW, tiles_y = (h + OH - 1) / OH;
+		const dim3 grid(tiles_x * pd, (tiles_y + HT_WAVES - 1) / HT_WAVES);
+		k_heatd_multi<TT><<<grid, 64 * HT_WAVES, 0, s>>>(out, in, w, h, tiles_x,
+				tiles_y, t);
+	};
+	switch (T) {
+	case 1: go(std::integral_constant<int, 1>{}); break;
+	case 2: go(std::integral_constant<int, 2>{}); break;
+	case 3: go(std::integral_constant<int, 3>{}); break;
+	case 4: go(std::integral_constant<int, 4>{}); break;
+	case 5: go(std::integral_constant<int, 5>{}); break;
+	case 6: go(std::integral_constant<int, 6>{}); break;
+	case 7: go(std::integral_constant<int, 7>{}); break;
+	default: go(std::integral_constant<int, 8>{}); break;
+	}
+	return true;
 }
 
 void launch_heatd_step(float *out, const float *in, const float *A, int w, int h,

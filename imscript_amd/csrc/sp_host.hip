@@ -1477,9 +1477,12 @@ extern "C" int sp_amle_host(float *out, const float *in, const float *init,
 
 // ------------------------------------------------------------------ heatd
 //
-// n explicit diffusion steps (heatd.c:58-103), one launch each, ping-pong
-// between the output and a stream-ordered scratch plane set, so that the
-// last step lands in the output.
+// n explicit diffusion steps (heatd.c:58-103).  With a diffusion-matrix
+// field, one launch per step; isotropic, T steps per launch on register
+// tiles (k_heatd_multi, T = SIMPOIS_HIP_HEATD_T, default 6, 1..8; a single
+// remaining step uses the one-step kernel).  Launches ping-pong between the
+// output and a stream-ordered scratch plane set, so that the last one
+// lands in the output.
 
 static int heatd_args(const float *out, const float *in, int w, int h, int pd, int n)
 {
@@ -1502,18 +1505,33 @@ extern "C" int sp_heatd_device(float *d_out, const float *d_in, const float *d_A
 		HIPCHK(hipMemcpyAsync(d_out, d_in, np * 4, hipMemcpyDeviceToDevice, s));
 		return SP_OK;
 	}
+	const char *tv = getenv("SIMPOIS_HIP_HEATD_T");
+	int T = tv ? atoi(tv) : 6;
+	T = T < 1 ? 1 : T > spk::heatd_multi_max() ? spk::heatd_multi_max() : T;
+	if (d_A) T = 1;
+	// T steps per launch; a remainder, or a chunk the planes are too small
+	// for (smaller T needs wider and taller planes), as single steps
+	std::vector<int> chunks;
+	for (int rem = n; rem > 0; rem -= chunks.back()) {
+		const int k = std::min(rem, T);
+		chunks.push_back(k > 1 && spk::heatd_multi_fits(w, h, k) ? k : 1);
+	}
+	const int L = (int)chunks.size();
 	float *tmp = nullptr;
-	if (n > 1) HIPCHK(hipMallocAsync((void **)&tmp, np * 4, s));
-	ProfScope ps("heatd_step", s);
+	if (L > 1) HIPCHK(hipMallocAsync((void **)&tmp, np * 4, s));
+	ProfScope ps(T > 1 ? "heatd_multi" : "heatd_step", s);
 	const float *src = d_in;
-	for (int k = 1; k <= n; k++) {
-		float *dst = (n - k) & 1 ? tmp : d_out;
-		spk::launch_heatd_step(dst, src, d_A, w, h, pd, t, s);
+	for (int k = 0; k < L; k++) {
+		float *dst = (L - 1 - k) & 1 ? tmp : d_out;
+		if (chunks[k] == 1) spk::launch_heatd_step(dst, src, d_A, w, h, pd, t, s);
+		else if (!spk::launch_heatd_multi(dst, src, w, h, pd, chunks[k], t, s))
+			return set_err(SP_ERR_ARG, "heatd: planes too small for %d steps per launch",
+					chunks[k]);
 		src = dst;
 	}
-	// algorithmic bytes: the plane read and written once per step, the
-	// field (3 floats per pixel) read once per step and plane
-	ps.add(n, (double)n * np * (8.0 + (d_A ? 12.0 : 0.0)));
+	// algorithmic bytes: the planes read and written once per launch, the
+	// field (3 floats per pixel) read once per launch and plane
+	ps.add(L, (double)L * np * (8.0 + (d_A ? 12.0 : 0.0)));
 	const hipError_t e = hipGetLastError();
 	if (tmp) HIPCHK(hipFreeAsync(tmp, s));
 	HIPCHK(e);

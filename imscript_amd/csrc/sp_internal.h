@@ -70,6 +70,12 @@ void launch_amle_diag(const float *c, const float *tinf, const float *tsup,
 // one explicit diffusion step of heatd (A: w*h*3 interleaved field or null)
 void launch_heatd_step(float *out, const float *in, const float *A, int w, int h,
 		int pd, float t, hipStream_t s);
+// T isotropic steps (1..heatd_multi_max()) in one launch on register tiles
+int heatd_multi_max();
+bool heatd_multi_fits(int w, int h, int T);   // planes large enough for T
+// (false, nothing launched, when the planes are too small for T)
+bool launch_heatd_multi(float *out, const float *in, int w, int h, int pd, int T,
+		float t, hipStream_t s);
 
 // dst[0..4*n4) = src[0..4*n4), 16-byte aligned
 void launch_stream_copy(float *dst, const float *src, size_t n4, hipStream_t s);

@@ -52,6 +52,27 @@ def test_device_matches_oracle(shape, aniso):
     assert bits_equal(y, oracle_heatd(x, 4, np.float32(0.23), A)[0])
 
 
+@pytest.mark.parametrize("T", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_multi_step_register_tiles_match_oracle(T, monkeypatch):
+    # several steps per launch (isotropic): tiles clamped at every image
+    # border, the last tile column / row moved back, a remainder launch;
+    # non-finite values on the borders and corners of the image
+    monkeypatch.setenv("SIMPOIS_HIP_HEATD_T", str(T))
+    rng = np.random.default_rng(T)
+    OW, OH = 64 - 2 * T, 48 - 2 * T
+    for shape in [(2, 97, 131), (1, OH, OW), (1, 211, 70), (1, 2 * OH + T, 2 * OW + T),
+                  (1, OH + T - 1, 3 * OW + 2), (1, 3 * OH - 1, OW + T + 1)]:
+        x = (rng.standard_normal(shape) * 10).astype(np.float32)
+        p, h, w = shape
+        x[0, 0, 0], x[0, -1, -1], x[0, 0, w // 2] = np.inf, -np.inf, np.nan
+        x[-1, h // 2, 0], x[-1, 3, -1] = -0.0, 1e30
+        x[-1, rng.integers(0, h, 5), rng.integers(0, w, 5)] = np.nan
+        n = 2 * T + 3
+        print(f"heatd T={T} {shape} n={n}", flush=True)
+        y = host(sp.heatd(dev(x), n, 0.21))
+        assert bits_equal(y, oracle_heatd(x, n, np.float32(0.21))[0]), shape
+
+
 def test_unaligned_views_use_the_element_path():
     import torch
     rng = np.random.default_rng(5)
