@@ -15,41 +15,41 @@ from helpers import ROOT
 
 TUT = os.path.join(ROOT, "tests", "golden", "tutorial")
 
-lib = ctypes.CDLL(imscript_amd.lib_path)
-_FP = ctypes.POINTER(ctypes.c_float)
-_IP = ctypes.POINTER(ctypes.c_int)
-lib.iio_read_image_float_vec.restype = _FP
-lib.iio_read_image_float_vec.argtypes = [ctypes.c_char_p, _IP, _IP, _IP]
-lib.iio_read_image_float.restype = _FP
-lib.iio_read_image_float.argtypes = [ctypes.c_char_p, _IP, _IP]
-for name in ("iio_write_image_float_vec", "iio_write_image_float_split"):
-    getattr(lib, name).argtypes = [ctypes.c_char_p, _FP, ctypes.c_int, ctypes.c_int,
-                                   ctypes.c_int]
-libc = ctypes.CDLL(None)
-libc.free.argtypes = [ctypes.c_void_p]
+# the package's loader binds the library to the process's HIP runtime (the
+# one torch brings); nothing is loaded before a test runs
+lib = imscript_amd.lib
+_libc = []
+
+
+def _free(p):
+    if not _libc:
+        _libc.append(ctypes.CDLL(None))
+        _libc[0].free.argtypes = [ctypes.c_void_p]
+    _libc[0].free(ctypes.cast(p, ctypes.c_void_p))
 
 
 def read_vec(path):
     w, h, pd = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-    p = lib.iio_read_image_float_vec(str(path).encode(), ctypes.byref(w), ctypes.byref(h),
-                                     ctypes.byref(pd))
+    p = lib.iio_read_image_float_vec(str(path).encode(), ctypes.addressof(w),
+                                     ctypes.addressof(h), ctypes.addressof(pd))
     a = np.ctypeslib.as_array(p, (h.value, w.value, pd.value)).copy()
-    libc.free(p)
+    _free(p)
     return a
 
 
 def read_gray(path):
     w, h = ctypes.c_int(), ctypes.c_int()
-    p = lib.iio_read_image_float(str(path).encode(), ctypes.byref(w), ctypes.byref(h))
+    p = lib.iio_read_image_float(str(path).encode(), ctypes.addressof(w),
+                                 ctypes.addressof(h))
     a = np.ctypeslib.as_array(p, (h.value, w.value)).copy()
-    libc.free(p)
+    _free(p)
     return a
 
 
 def write_vec(path, a):
     a = np.ascontiguousarray(a, np.float32)
     h, w, pd = a.shape
-    lib.iio_write_image_float_vec(str(path).encode(), a.ctypes.data_as(_FP), w, h, pd)
+    lib.iio_write_image_float_vec(str(path).encode(), a.ctypes.data, w, h, pd)
 
 
 def same(a, b):
