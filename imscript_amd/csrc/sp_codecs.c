@@ -171,7 +171,10 @@ int codec_read_png(const unsigned char *buf, size_t n, codec_image *im,
 		free(idat);
 		return cfail(err, errn, "png error (header)");
 	}
-	if ((uint64_t)w * h > (1u << 30)) { free(idat); return cfail(err, errn, "png too large"); }
+	if ((uint64_t)w * h > (1u << 30) || (uint64_t)w * h * ch * (depth > 8 ? 2 : 1) > (1ull << 31)) {
+		free(idat);
+		return cfail(err, errn, "png too large");
+	}
 	const int bits = depth * ch, bpp = bits < 8 ? 1 : bits / 8;
 	/* passes 0-6 of Adam7, or the whole image as pass 7 */
 	const int p0 = inter ? 0 : 7, p1 = inter ? 7 : 8;
@@ -605,6 +608,12 @@ int codec_read_tiff(const unsigned char *buf, size_t n, codec_image *im,
 		TFAIL("unrecognized tiff sample format %u (see tiff.h)", fmt);
 	}
 	if (spp < 1 || spp > 64) TFAIL("bad TIFF samples per pixel %u", spp);
+	/* sizes a damaged file could claim are refused before allocating */
+	if ((uint64_t)w * h * spp > (1ull << 31)) TFAIL("TIFF too large");
+	if (tw || tl) {
+		if (!tw || !tl || tw > 65536 || tl > 65536 || (uint64_t)tw * tl > (1ull << 26))
+			TFAIL("bad TIFF tile size %ux%u", tw, tl);
+	}
 	if (comp != 1 && comp != 5 && comp != 8 && comp != 32946 && comp != 32773)
 		TFAIL("unsupported TIFF compression %u", comp);
 	if (pred != 1 && pred != 2 && pred != 3) TFAIL("unsupported TIFF predictor %u", pred);
@@ -618,6 +627,7 @@ int codec_read_tiff(const unsigned char *buf, size_t n, codec_image *im,
 	uint32_t ch = tiled ? tl : (rps == 0 || rps > h ? h : rps);
 	const size_t rowb = ((size_t)cw * cspp * bps + 7) / 8;
 	const size_t need = rowb * ch;
+	if (need > (1ull << 31)) TFAIL("TIFF strip or tile too large");
 	const uint32_t across = tiled ? (w + tw - 1) / tw : 1;
 	const uint32_t down = (h + ch - 1) / ch;
 	const uint32_t per_plane = across * down;

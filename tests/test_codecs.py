@@ -241,3 +241,36 @@ def test_unknown_format_fails_like_iio(tmp_path):
                         str(tmp_path / "x.bin"), "-o", str(tmp_path / "o.npy")],
                        capture_output=True, timeout=60)
     assert p.returncode == 255 and b"unsupported image format" in p.stderr
+
+
+def test_decoders_survive_damaged_files(tmp_path):
+    """The decoders built with AddressSanitizer on damaged copies of the
+    tutorial files and of the variant files: refused or decoded, never an
+    access out of bounds (tests/cpu/codec_fuzz.c)."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    exe = tmp_path / "codec_fuzz"
+    san = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all"]
+    subprocess.run(["gcc", "-O1", "-g", "-std=gnu99", *san, "-o", str(exe),
+                    os.path.join(ROOT, "tests", "cpu", "codec_fuzz.c"),
+                    os.path.join(ROOT, "imscript_amd", "csrc", "sp_codecs.c"), "-lz"],
+                   check=True)
+    rng = np.random.default_rng(9)
+    files = [os.path.join(TUT, n) for n in ("X.tif", "mask.png", "lenac.png")]
+    a = _arr(rng, "f4", 20, 24, 3)
+    for k, kw in enumerate([dict(compression=5, predictor=3, tiles=(16, 16)),
+                            dict(compression=32773, planar=2, rps=3),
+                            dict(compression=8, predictor=2, be=True, rps=7)]):
+        p = tmp_path / f"v{k}.tif"
+        p.write_bytes(tiff_encode(a if k != 2 else a.view(np.uint32).astype(np.uint16), **kw))
+        files.append(str(p))
+    s = rng.integers(0, 4, (11, 13, 1))
+    (tmp_path / "i.png").write_bytes(png_encode(s, 3, 2, rng.integers(0, 256, (4, 3)),
+                                                bytes([7, 9]), interlace=True))
+    files.append(str(tmp_path / "i.png"))
+    env = dict(os.environ, ASAN_OPTIONS="allocator_may_return_null=1")
+    r = subprocess.run([str(exe), "7", "300", *files], capture_output=True, text=True,
+                       timeout=600, env=env)
+    assert r.returncode == 0 and r.stdout.startswith("OK"), r.stdout + r.stderr[-2000:]
