@@ -669,7 +669,7 @@ def run_amle(a):
     px = g.numel()
     achieved = t_bytes / (t_ms * 1e-3) / 1e9
     eff = 12.0 * niter * px / (t_ms * 1e-3) / 1e9
-    var = int(os.environ.get("SIMPOIS_HIP_AMLE_VARIANT", "0") or 0)
+    var = int(os.environ.get("SIMPOIS_HIP_AMLE_VARIANT", "6" if a.amle_planes == 1 else "0"))
     r1 = ["32x64x16", "32x128x16", "64x64x16", "16x128x32", "32x64x8",
           "64x32x16", "32x32x16", "16x64x16", "32x64x32", "64x64x8"]
     shape = (r1[var if 0 <= var < 10 else 0] if nn <= 4
@@ -688,7 +688,7 @@ def run_amle(a):
                       "output": "bit-identical to the reference amle (tsup); the "
                                 "lower bound tinf, which feeds only the stderr "
                                 "trace, is not swept"},
-           "roofline": {"kernel": f"k_amle_tile (tile {shape} = BIxBJxBT: BT "
+           "roofline": {"kernel": f"k_amle4_tile (tile {shape} = BIxBJxBT: BT "
                                   "raster sweeps fused per pass, one launch per "
                                   "wavefront of skewed tiles)",
                         "bound": "hbm", "achieved": round(achieved, 1),

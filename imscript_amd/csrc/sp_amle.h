@@ -144,3 +144,50 @@ SP_HD void amle_tile_step(const TileCtx<S> &c, int tid, int sg, int nn)
 	if (!sp_known(c.C[idx]))
 		c.U[idx] = amle_value(c.U, idx, i, j, c.W, c.H, G::LW + G::SH, nn);
 }
+
+// The 4-neighbour sweep of a radius-1 tile with its per-thread state kept
+// across steps: thread (a, tau) updates the cell at in-tile row b = sg - a,
+// whose LDS index and image row advance by one row per step, so a step is
+// the update alone.  Same cells, order and results as amle_tile_step with
+// nn = 4 (planes larger than 1 x 1).
+struct Amle4Thread { int idx, j, i, a; bool on, eb; };
+
+template <class S>
+SP_HD Amle4Thread amle4_thread(const TileCtx<S> &c, int tid)
+{
+	using G = Geo<S>;
+	Amle4Thread t;
+	const int a = tid % G::BI, tau = tid / G::BI;
+	t.a = a;
+	t.on = tau < c.bte;
+	t.i = c.x0 + a - tau;
+	t.j = c.y0 - a - tau;                              // at sg = 0, b = -a
+	t.idx = (G::BT - a - tau) * G::LW + (a - tau + G::BT);
+	t.eb = t.i + 1 >= c.W;                             // east neighbour outside
+	return t;
+}
+
+template <class S>
+SP_HD void amle4_step(const TileCtx<S> &c, Amle4Thread &t, int sg)
+{
+	using G = Geo<S>;
+	const int b = sg - t.a;
+	if (t.on && b >= 0 && b < G::BJ) {
+		const float *U = c.U;
+		const int q = t.idx;
+		const float cv = c.C[q], x = U[q];
+		const float e = U[q + 1], so = U[q + G::LW], w = U[q - 1], n = U[q - G::LW];
+		float f = e;
+		if (t.eb) f = t.j + 1 < c.H ? so : t.i > 0 ? w : n;
+		float mn = f, mx = f;
+		mn = so < mn ? so : mn;
+		mx = so > mx ? so : mx;
+		mn = w < mn ? w : mn;
+		mx = w > mx ? w : mx;
+		mn = n < mn ? n : mn;
+		mx = n > mx ? n : mx;
+		c.U[q] = sp_known(cv) ? x : (mn + mx) * 0.5f;
+	}
+	t.idx += G::LW;
+	t.j += 1;
+}

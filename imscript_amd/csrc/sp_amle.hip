@@ -105,6 +105,26 @@ __global__ __launch_bounds__(Geo<S>::NT) void k_amle_tile(const GsTile *tiles,
 	tile_store<S>(c, in, threadIdx.x);
 }
 
+// nn = 4 on planes larger than 1 x 1: per-thread state across the steps
+template <class S>
+__global__ __launch_bounds__(Geo<S>::NT) void k_amle4_tile(const GsTile *tiles,
+		const GsInst *insts)
+{
+	using G = Geo<S>;
+	extern __shared__ float smem[];
+	const GsTile tl = tiles[blockIdx.x];
+	const GsInst in = insts[tl.inst];
+	const TileCtx<S> c = tile_ctx<S>(tl, in, smem);
+	tile_load<S>(c, in, threadIdx.x);
+	Amle4Thread t = amle4_thread<S>(c, threadIdx.x);
+	__syncthreads();
+	for (int sg = 0; sg < G::NSTEP; sg++) {
+		amle4_step<S>(c, t, sg);
+		__syncthreads();
+	}
+	tile_store<S>(c, in, threadIdx.x);
+}
+
 // trace diagnostics over the holes of plane blockIdx.y, block partials:
 // holes, max |tinf - tsup| (NaN never wins, amle.c:120-121), sum of it,
 // sums of |old - new| of tinf and of tsup over the last sweep
@@ -199,11 +219,18 @@ int amle_tile_lds_bytes(int R, int var)
 }
 
 void launch_amle_tiles(int R, int var, const GsTile *tiles, int ntiles,
-		const GsInst *insts, int nn, hipStream_t s)
+		const GsInst *insts, int nn, bool fast4, hipStream_t s)
 {
 	if (!ntiles) return;
 	amle_with_shape(R, var, [&](auto sh) {
 		using S = decltype(sh);
+		if constexpr (S::R == 1) {
+			if (fast4) {
+				k_amle4_tile<S><<<ntiles, Geo<S>::NT, Geo<S>::LDS_FLOATS * 4, s>>>(
+						tiles, insts);
+				return;
+			}
+		}
 		k_amle_tile<S><<<ntiles, Geo<S>::NT, Geo<S>::LDS_FLOATS * 4, s>>>(tiles,
 				insts, nn);
 	});

@@ -1353,7 +1353,8 @@ static int amle_sweeps(AmlePlan &A, int T, hipStream_t s)
 	ProfScope ps("amle_tile", s);
 	for (size_t k = 0; k + 1 < sc->off.size(); k++)
 		spk::launch_amle_tiles(A.R, A.var, sc->d_tiles + sc->off[k],
-				sc->off[k + 1] - sc->off[k], sc->d_insts, A.nn, s);
+				sc->off[k + 1] - sc->off[k], sc->d_insts, A.nn,
+				A.nn == 4 && (A.w > 1 || A.h > 1), s);
 	// algorithmic bytes: u and c read and u written once per pixel per pass
 	ps.add((long)sc->off.size() - 1, 12.0 * (double)sc->px);
 	HIPCHK(hipGetLastError());
@@ -1377,11 +1378,13 @@ static std::mutex g_amle_mu;
 static AmlePlan *g_amle = nullptr;
 
 // tile shape of the 4-neighbour sweeps: SIMPOIS_HIP_AMLE_VARIANT (read at
-// each call; 0 = the default 32x64x16), variant 0 for larger neighbourhoods
-static int amle_variant(int R)
+// each call), else 32x32x16 (variant 6) for one plane, whose wavefronts hold
+// few tiles, and 32x64x16 (variant 0) for several (measured on 4096^2
+// planes); variant 0 for the larger neighbourhoods
+static int amle_variant(int R, int pd)
 {
 	const char *e = getenv("SIMPOIS_HIP_AMLE_VARIANT");
-	const int v = e ? atoi(e) : 0;
+	const int v = e ? atoi(e) : pd == 1 ? 6 : 0;
 	return R == 1 && v > 0 && v < amle_shape_count(1) ? v : 0;
 }
 
@@ -1393,7 +1396,7 @@ extern "C" int sp_amle_device(float *d_out, const float *d_in, const float *d_in
 	hipStream_t s = (hipStream_t)stream;
 	std::lock_guard<std::mutex> lk(g_amle_mu);
 	const int R = amle_radius(nn);
-	const int var = amle_variant(R);
+	const int var = amle_variant(R, pd);
 	trace = trace != 0;
 	AmlePlan *A = g_amle;
 	if (!A || A->w != w || A->h != h || A->pd != pd || A->nn != nn ||

@@ -59,8 +59,9 @@ void launch_amle_prep(const float *in, const float *init, float *u, float *c,
 		float *tinf, void *part, float2 *mm, int n, int pd, hipStream_t s);
 // one launch of the AMLE tiles of one wavefront (shape variant var of radius R)
 int amle_tile_lds_bytes(int R, int var);
+// fast4: nn = 4 on planes larger than 1 x 1 (k_amle4_tile)
 void launch_amle_tiles(int R, int var, const GsTile *tiles, int ntiles,
-		const GsInst *insts, int nn, hipStream_t s);
+		const GsInst *insts, int nn, bool fast4, hipStream_t s);
 // per-block partials (holes, max, sums) of the trace diagnostics: 5 doubles
 // per block, amle_part_blocks(n) blocks per plane
 void launch_amle_diag(const float *c, const float *tinf, const float *tsup,

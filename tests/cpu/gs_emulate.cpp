@@ -142,8 +142,23 @@ static int main_amle(int argc, char **argv)
 	}
 	orc_amle_sweeps(uref.data(), hole.data(), W, H, nn, T);
 	int rc = 0;
+	const bool fast4 = nn == 4 && (W > 1 || H > 1);   // the device's dispatch
 	if (!amle_with_shape(amle_radius(nn), var, [&](auto sh) {
 			using S = decltype(sh);
+			if constexpr (S::R == 1) {
+				if (fast4) {
+					// per-thread state of every tile, made at step 0
+					std::vector<std::vector<Amle4Thread>> st;
+					rc = run_tiles<S>(u.data(), c.data(), W, H, T, 0.0f,
+						[&st](const TileCtx<S> &cc, int tid, int sg) {
+							if (sg == 0 && tid == 0) st.emplace_back(Geo<S>::NT);
+							auto &v = st.back();
+							(void)cc;
+							if (sg == 0) v[tid] = amle4_thread<S>(cc, tid);
+							amle4_step<S>(cc, v[tid], sg); });
+					return;
+				}
+			}
 			rc = run_tiles<S>(u.data(), c.data(), W, H, T, 0.0f,
 				[nn](const TileCtx<S> &cc, int tid, int sg) {
 					amle_tile_step<S>(cc, tid, sg, nn); }); })) {
