@@ -688,7 +688,8 @@ def run_amle(a):
                       "output": "bit-identical to the reference amle (tsup); the "
                                 "lower bound tinf, which feeds only the stderr "
                                 "trace, is not swept"},
-           "roofline": {"kernel": f"k_amle4_tile (tile {shape} = BIxBJxBT: BT "
+           "roofline": {"kernel": ("k_amle4_tile" if nn == 4 else "k_amle_tile") +
+                                  f" (tile {shape} = BIxBJxBT: BT "
                                   "raster sweeps fused per pass, one launch per "
                                   "wavefront of skewed tiles)",
                         "bound": "hbm", "achieved": round(achieved, 1),

@@ -274,6 +274,33 @@ SP_HD void tile_step(const TileCtx<S> &c, int tid, int sg)
 		gs_update<R>(c.U, cv, idx, i, j, c.W, c.H, G::LW + G::SH, c.t);
 }
 
+// The R = 1 sweep with its per-thread state kept across steps: thread
+// (a, tau) updates the cell at in-tile row b = sg - a, whose LDS index
+// advances by one row per step, so a step is the update alone.  Same
+// cells, order and results as tile_step for R = 1.
+struct Gs1Thread { int idx, a; bool on; };
+
+template <class S>
+SP_HD Gs1Thread gs1_thread(const TileCtx<S> &c, int tid)
+{
+	using G = Geo<S>;
+	Gs1Thread t;
+	const int a = tid % G::BI, tau = tid / G::BI;
+	t.a = a;
+	t.on = tau < c.bte;
+	t.idx = (G::BT - a - tau) * G::LW + (a - tau + G::BT);   // b = -a at sg = 0
+	return t;
+}
+
+template <class S>
+SP_HD void gs1_step(const TileCtx<S> &c, Gs1Thread &t, int sg)
+{
+	using G = Geo<S>;
+	const int b = sg - t.a;
+	if (t.on && b >= 0 && b < G::BJ) gs_update_tile1(c.U, c.C, t.idx, G::LW, c.t);
+	t.idx += G::LW;
+}
+
 // does this tile update pixel (i, j) at some sweep?
 template <class S>
 SP_HD bool tile_owns(const TileCtx<S> &c, int i, int j)

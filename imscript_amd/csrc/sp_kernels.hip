@@ -237,9 +237,17 @@ __global__ __launch_bounds__(Geo<S>::NT) void k_gs_tile(const GsTile *tiles,
 	const TileCtx<S> c = tile_ctx<S>(tl, in, smem);
 	tile_load<S>(c, in, threadIdx.x);
 	__syncthreads();
-	for (int sg = 0; sg < G::NSTEP; sg++) {
-		tile_step<S>(c, threadIdx.x, sg);
-		__syncthreads();
+	if constexpr (S::R == 1) {
+		Gs1Thread t = gs1_thread<S>(c, threadIdx.x);
+		for (int sg = 0; sg < G::NSTEP; sg++) {
+			gs1_step<S>(c, t, sg);
+			__syncthreads();
+		}
+	} else {
+		for (int sg = 0; sg < G::NSTEP; sg++) {
+			tile_step<S>(c, threadIdx.x, sg);
+			__syncthreads();
+		}
 	}
 	tile_store<S>(c, in, threadIdx.x);
 }

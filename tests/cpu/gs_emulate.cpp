@@ -214,6 +214,17 @@ int main(int argc, char **argv)
 		: R == 2 ? run_small<2>(u, c, W, H, T, teff) : run_small<3>(u, c, W, H, T, teff);
 	else if (!sp_with_shape(R, var, [&](auto sh) {
 			using S = decltype(sh);
+			if constexpr (S::R == 1) {
+				// the device's R = 1 loop: per-thread state made at step 0
+				std::vector<std::vector<Gs1Thread>> st;
+				rc = run_tiles<S>(u, c, W, H, T, teff,
+					[&st](const TileCtx<S> &cc, int tid, int sg) {
+						if (sg == 0 && tid == 0) st.emplace_back(Geo<S>::NT);
+						auto &v = st.back();
+						if (sg == 0) v[tid] = gs1_thread<S>(cc, tid);
+						gs1_step<S>(cc, v[tid], sg); });
+				return;
+			}
 			rc = run_tiles<S>(u, c, W, H, T, teff,
 				[](const TileCtx<S> &cc, int tid, int sg) { tile_step<S>(cc, tid, sg); }); })) {
 		printf("NO SHAPE %d for radius %d\n", var, R);
