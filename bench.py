@@ -842,6 +842,10 @@ def main():
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--channels", type=int, default=3)
     ap.add_argument("--niter", type=int, default=None)
+    ap.add_argument("--cg-rel-coarse", type=float, default=0.0,
+                    help="CG tolerance of the pyramid levels above the finest")
+    ap.add_argument("--scale", type=int, default=None,
+                    help="pyramid levels (simpois -s); default 1 for mgpcg1, 99 otherwise")
     ap.add_argument("--method", default="mgpcg1", choices=["mgpcg", "mgpcg1", "cg", "gs"],
                     help="mgpcg: pyramid with multigrid-preconditioned CG at "
                          "every level; mgpcg1: the same on the finest level "
@@ -893,10 +897,11 @@ def main():
     out = torch.empty_like(g)
     if a.niter is None:
         a.niter = {"cg": 10, "gs": 256}.get(a.method, 0)
-    params = dict(tstep=0.25, niter=a.niter, scale=1 if a.method == "mgpcg1" else 99,
+    scale = a.scale if a.scale is not None else (1 if a.method == "mgpcg1" else 99)
+    params = dict(tstep=0.25, niter=a.niter, scale=scale,
                   cgit=0 if a.method == "gs" else -1, cg_fp32=1, cg_rel=a.cg_rel,
                   trace=0, cg_chunk=32, cg_mg=0 if a.method == "cg" else 1,
-                  autotune=0 if a.no_autotune else 1)
+                  autotune=0 if a.no_autotune else 1, cg_rel_coarse=a.cg_rel_coarse)
     torch.cuda.synchronize()
 
     f_term = f

@@ -17,7 +17,8 @@ class sp_params(ctypes.Structure):
                 ("cg_eps", ctypes.c_double), ("trace", ctypes.c_int),
                 ("cg_fp32", ctypes.c_int), ("cg_rel", ctypes.c_double),
                 ("cg_chunk", ctypes.c_int), ("cg_mg", ctypes.c_int),
-                ("autotune", ctypes.c_int), ("guidance", ctypes.c_int)]
+                ("autotune", ctypes.c_int), ("guidance", ctypes.c_int),
+                ("cg_rel_coarse", ctypes.c_double)]
 
 
 class sp_image(ctypes.Structure):

@@ -96,6 +96,7 @@ struct Cg32Batch {
 	int4 *tail_rng;    // per plane: (first tail level, depth, runs in LDS,
 	                   // tail cells of all its levels)
 	int tail_lds;      // dynamic LDS bytes of the LDS tail kernel (0: none)
+	int tail_global;   // some plane runs the global-memory tail
 	int4 *tiles0;      // level-0 tiles (plane, x0, y0, -), contiguous per plane
 	int ntiles0;
 	double *tpart;     // one partial per level-0 tile

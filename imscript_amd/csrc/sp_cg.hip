@@ -611,6 +611,30 @@ __global__ __launch_bounds__(NT) void k32_code(const BlockWork *bw,
 	if (q4 >= n) return;
 	const unsigned q1 = q4 + 4 < n ? q4 + 4 : n;
 	unsigned j = q4 / w, i = q4 - j * w, word = 0;
+	// a group inside one row of a 16-byte aligned plane: its row and the
+	// rows above and below as one 16-byte load each (w a multiple of 4)
+	if (q1 - q4 == 4 && !(w & 3) && !((uintptr_t)P.g & 15)) {
+		const f4 c = *(const f4 *)(P.g + q4);
+		const f4 up = j > 0 ? *(const f4 *)(P.g + q4 - w) : c;
+		const f4 dn = j + 1 < h ? *(const f4 *)(P.g + q4 + w) : c;
+		const float lf = i > 0 ? P.g[q4 - 1] : 0.0f;
+		const float rt = i + 4 < w ? P.g[q4 + 4] : 0.0f;
+		const unsigned nrow = (j > 0) + (j + 1 < h);
+#pragma unroll
+		for (int k = 0; k < 4; k++) {
+			if (!isnan(c[k])) continue;
+			const unsigned ii = i + k;
+			unsigned v = 1;
+			if (ii + 1 < w && isnan(k < 3 ? c[k + 1] : rt)) v |= 2;
+			if (j + 1 < h && isnan(dn[k])) v |= 4;
+			if (ii > 0 && isnan(k > 0 ? c[k - 1] : lf)) v |= 8;
+			if (j > 0 && isnan(up[k])) v |= 16;
+			v |= ((ii > 0) + (ii + 1 < w) + nrow) << 5;
+			word |= v << (8 * k);
+		}
+		*(unsigned *)(P.code + q4) = word;
+		return;
+	}
 	for (unsigned q = q4; q < q1; q++) {
 		unsigned c = 0;
 		if (isnan(P.g[q])) {
@@ -1008,19 +1032,44 @@ void cg32_setup(const Cg32Batch &B, hipStream_t s);
 // The small levels of a plane (from the first level of at most
 // MG_TAIL_CELLS cells down to the coarsest) run inside one work-group: the
 // same smoothing, restriction and correction as the per-level kernels, with
-// a barrier between phases instead of a launch.
+// a barrier between phases instead of a launch.  A thread's cells q =
+// threadIdx.x + k * blockDim.x are walked with their (i, j) advanced by a
+// constant step (no division per cell), and the smoother multiplies by the
+// hardware reciprocal of the diagonal (1 ulp, the same in the pre- and
+// post-smoother, so the V-cycle stays symmetric).
+struct TailIdx { int i, j, di, dj; };
+
+static __device__ __forceinline__ TailIdx tail_idx(int w)
+{
+	TailIdx t;
+	t.j = (int)threadIdx.x / w;
+	t.i = (int)threadIdx.x - t.j * w;
+	t.dj = (int)blockDim.x / w;
+	t.di = (int)blockDim.x - t.dj * w;
+	return t;
+}
+
+static __device__ __forceinline__ void tail_next(TailIdx &t, int w)
+{
+	t.i += t.di;
+	t.j += t.dj;
+	if (t.i >= w) { t.i -= w; t.j++; }
+}
+
 static __device__ void tail_smooth(const MgLevel &L, int color, bool first)
 {
 	const int n = L.w * L.h;
-	for (int q = threadIdx.x; q < n; q += blockDim.x) {
-		const int i = q % L.w, j = q / L.w;
+	TailIdx p = tail_idx(L.w);
+	for (int q = threadIdx.x; q < n; q += blockDim.x, tail_next(p, L.w)) {
+		const int i = p.i, j = p.j;
 		if (((i + j) & 1) != color) {
 			if (first) L.z[q] = 0.0f;
 			continue;
 		}
 		const float d = L.D[q];
 		if (d == 0.0f) continue;
-		L.z[q] = (first ? L.b[q] : L.b[q] + offdiag(L, i, j, false)) / d;
+		L.z[q] = (first ? L.b[q] : L.b[q] + offdiag(L, i, j, false)) *
+			__builtin_amdgcn_rcpf(d);
 	}
 	__syncthreads();
 }
@@ -1028,8 +1077,9 @@ static __device__ void tail_smooth(const MgLevel &L, int color, bool first)
 static __device__ void tail_restrict(const MgLevel &F, const MgLevel &C)
 {
 	const int n = C.w * C.h;
-	for (int q = threadIdx.x; q < n; q += blockDim.x) {
-		const int I = q % C.w, J = q / C.w;
+	TailIdx p = tail_idx(C.w);
+	for (int q = threadIdx.x; q < n; q += blockDim.x, tail_next(p, C.w)) {
+		const int I = p.i, J = p.j;
 		float r = 0;
 		for (int c = 0; c < 4; c++) {
 			const int i = 2 * I + (c & 1), j = 2 * J + (c >> 1);
@@ -1048,10 +1098,10 @@ static __device__ void tail_restrict(const MgLevel &F, const MgLevel &C)
 static __device__ void tail_prolong(const MgLevel &F, const MgLevel &C, float wc)
 {
 	const int n = F.w * F.h;
-	for (int q = threadIdx.x; q < n; q += blockDim.x) {
+	TailIdx p = tail_idx(F.w);
+	for (int q = threadIdx.x; q < n; q += blockDim.x, tail_next(p, F.w)) {
 		if (F.D[q] == 0.0f) continue;
-		const int i = q % F.w, j = q / F.w;
-		F.z[q] += wc * C.z[(j >> 1) * C.w + (i >> 1)];
+		F.z[q] += wc * C.z[(p.j >> 1) * C.w + (p.i >> 1)];
 	}
 	__syncthreads();
 }
@@ -1108,15 +1158,17 @@ static __device__ float tl_off(const TailLv &L, int q, int i, int j)
 static __device__ void tl_smooth(const TailLv &L, int color, bool first)
 {
 	const int n = L.w * L.h;
-	for (int q = threadIdx.x; q < n; q += blockDim.x) {
-		const int j = q / L.w, i = q - j * L.w;
+	TailIdx p = tail_idx(L.w);
+	for (int q = threadIdx.x; q < n; q += blockDim.x, tail_next(p, L.w)) {
+		const int i = p.i, j = p.j;
 		if (((i + j) & 1) != color) {
 			if (first) L.z[q] = 0.0f;
 			continue;
 		}
 		const float d = L.D[q];
 		if (d == 0.0f) continue;
-		L.z[q] = (first ? L.b[q] : L.b[q] + tl_off(L, q, i, j)) / d;
+		L.z[q] = (first ? L.b[q] : L.b[q] + tl_off(L, q, i, j)) *
+			__builtin_amdgcn_rcpf(d);
 	}
 	__syncthreads();
 }
@@ -1124,8 +1176,9 @@ static __device__ void tl_smooth(const TailLv &L, int color, bool first)
 static __device__ void tl_restrict(const TailLv &F, const TailLv &C)
 {
 	const int n = C.w * C.h;
-	for (int q = threadIdx.x; q < n; q += blockDim.x) {
-		const int J = q / C.w, I = q - J * C.w;
+	TailIdx p = tail_idx(C.w);
+	for (int q = threadIdx.x; q < n; q += blockDim.x, tail_next(p, C.w)) {
+		const int J = p.j, I = p.i;
 		float r = 0;
 		for (int c = 0; c < 4; c++) {
 			const int i = 2 * I + (c & 1), j = 2 * J + (c >> 1);
@@ -1144,10 +1197,10 @@ static __device__ void tl_restrict(const TailLv &F, const TailLv &C)
 static __device__ void tl_prolong(const TailLv &F, const TailLv &C, float wc)
 {
 	const int n = F.w * F.h;
-	for (int q = threadIdx.x; q < n; q += blockDim.x) {
+	TailIdx p = tail_idx(F.w);
+	for (int q = threadIdx.x; q < n; q += blockDim.x, tail_next(p, F.w)) {
 		if (F.D[q] == 0.0f) continue;
-		const int j = q / F.w, i = q - j * F.w;
-		F.z[q] += wc * C.z[(j >> 1) * C.w + (i >> 1)];
+		F.z[q] += wc * C.z[(p.j >> 1) * C.w + (p.i >> 1)];
 	}
 	__syncthreads();
 }
@@ -1260,8 +1313,9 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 	}
 	// small levels of every plane in one work-group each
 	if (B.prof) B.prof("mg_tail", 1, 0, s);
-	k_mg_tail<<<B.nplanes, MG_TAIL_T, 0, s>>>(B.tail_lv, B.tail_rng, B.scal, B.nu,
-			B.coarse_sweeps, B.wc);
+	if (B.tail_global)
+		k_mg_tail<<<B.nplanes, MG_TAIL_T, 0, s>>>(B.tail_lv, B.tail_rng, B.scal, B.nu,
+				B.coarse_sweeps, B.wc);
 	if (B.tail_lds)
 		k_mg_tail_lds<<<B.nplanes, MG_TAIL_T, B.tail_lds, s>>>(B.tail_lv, B.tail_rng,
 				B.scal, B.nu, B.coarse_sweeps, B.wc);

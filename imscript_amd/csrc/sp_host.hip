@@ -80,6 +80,7 @@ extern "C" void sp_params_default(sp_params *p)
 	p->trace = !(q && *q && strcmp(q, "0"));
 	p->cg_fp32 = 0;
 	p->cg_rel = 0;
+	p->cg_rel_coarse = 0;
 	p->cg_chunk = 32;
 	p->cg_mg = 0;
 	const char *at = getenv("SIMPOIS_HIP_AUTOTUNE");
@@ -894,6 +895,8 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 	TRY(pl.upload(&B.tail_lv, all));
 	TRY(pl.upload(&B.tail_rng, rng));
 	B.tail_lds = tail_lds;
+	B.tail_global = 0;
+	for (const int4 &r : rng) B.tail_global |= !r.z;
 	TRY(pl.upload(&B.tiles0, tiles0));
 	B.ntiles0 = (int)tiles0.size();
 	TRY(pl.alloc(&B.tpart, tiles0.size()));
@@ -950,8 +953,10 @@ static int cg32_build(Plan &pl, int k, int bilap, float eps)
 		cp.push_back(c);
 		Cg32Scal q{};
 		const int cg = L.cgit[m];
-		q.rel = pl.P.cg_rel > 0;
-		q.eps = q.rel ? pl.P.cg_rel : (pl.P.cg_eps >= 0 ? (double)(float)pl.P.cg_eps
+		const double rel = k > 0 && pl.P.cg_rel_coarse > 0 ? pl.P.cg_rel_coarse
+			: pl.P.cg_rel;
+		q.rel = rel > 0;
+		q.eps = q.rel ? rel : (pl.P.cg_eps >= 0 ? (double)(float)pl.P.cg_eps
 				: (double)eps);
 		q.maxit_from_count = cg == -1;
 		q.maxit = cg == -1 ? 0 : std::max(cg, 0);
@@ -1166,6 +1171,7 @@ extern "C" int sp_solve_batch_device(int n, const sp_image *imgs,
 	put(&P.cg_fp32, sizeof P.cg_fp32); put(&P.cg_rel, sizeof P.cg_rel);
 	put(&P.cg_chunk, sizeof P.cg_chunk); put(&P.cg_mg, sizeof P.cg_mg);
 	put(&P.autotune, sizeof P.autotune); put(&P.guidance, sizeof P.guidance);
+	put(&P.cg_rel_coarse, sizeof P.cg_rel_coarse);
 	static std::mutex cache_mu;
 	static Plan *cache = nullptr;
 	std::lock_guard<std::mutex> lk(cache_mu);

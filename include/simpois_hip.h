@@ -121,6 +121,9 @@ typedef struct sp_params {
 	                    "x,l" (1) or "x,lr" (2) Laplacian of v with the
 	                    nearest-pixel rule, computed inside the solve (the
 	                    Poisson-editing pipeline plambda v "x,l" | simpois -f) */
+	double cg_rel_coarse; /* > 0: relative CG tolerance of the pyramid levels
+	                    coarser than the finest, whose solutions only start
+	                    the next level (FP32 CG); 0: cg_rel at every level */
 } sp_params;
 
 /* one image of a batch: planar w*h*pd device buffers, f may be NULL */
