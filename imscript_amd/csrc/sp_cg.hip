@@ -586,15 +586,27 @@ template <int TX_, int TY_, int NU_, int T_> struct MgGeo {
 	static constexpr int RW = TX + 2 * HALO, RH = TY + 2 * HALO;
 	static constexpr int PAIRS_X = RW / 2, ROWPAIRS = RH / 2;
 	static constexpr int RG = T / PAIRS_X, K = ROWPAIRS / RG;
-	// z of the region in LDS with a zero frame: one row above and below and
-	// one column after each row (the column before a row is the previous
-	// row's frame), so the four neighbour reads of every region cell stay
-	// in bounds and read 0 outside the region, with no tests
+	// z of the region in LDS with a zero frame.  Each row stores its even
+	// columns, one frame cell, then its odd columns (ZS = RW + 1 per row),
+	// so the cells a wave touches at once (one colour, consecutive column
+	// pairs) are consecutive words: no bank conflicts.  The frame cell is
+	// the west neighbour of column 0 and the east neighbour of column RW-1;
+	// a frame row lies above and below the region.  Every neighbour read of
+	// a region cell stays in bounds and reads 0 outside the region, with no
+	// tests.
+	static constexpr int HALF = PAIRS_X + 1;   // even column c/2 -> odd column c/2
 	static constexpr int ZS = RW + 1, ZOFF = ZS, ZN = (RH + 2) * ZS;
 	static_assert(TX % 2 == 0 && TY % 2 == 0, "tile origins must be even");
 	static_assert(PAIRS_X * RG == T && RG * K == ROWPAIRS,
 			"multigrid tile shape must divide the thread layout");
-	static __device__ __forceinline__ int zid(int r, int col) { return ZOFF + r * ZS + col; }
+	static __device__ __forceinline__ int zid(int r, int col)
+	{
+		return ZOFF + r * ZS + (col & 1) * HALF + (col >> 1);
+	}
+	// offsets of the east / west neighbours of a cell in an even (par 0) or
+	// odd (par 1) column
+	static __device__ __forceinline__ int de(int par) { return par ? 1 - HALF : HALF; }
+	static __device__ __forceinline__ int dw(int par) { return par ? -HALF : HALF - 1; }
 };
 using GeoFine = MgGeo<MG_TX_FINE, MG_TY_FINE, MG_NU_FINE, MG_T_FINE>;
 using GeoCoarse = MgGeo<MG_TX_COARSE, MG_TY_COARSE, MG_NU_COARSE, MG_T_COARSE>;
@@ -694,14 +706,15 @@ struct Lev0 {
 		const unsigned d = k >> 5;
 		return d == 4 ? 0.25f : d == 3 ? (1.0f / 3.0f) : d == 2 ? 0.5f : 0.0f;
 	}
-	// s + the neighbours of the cell at z[id] (E, S, W, N); zs = row pitch.
-	// A neighbour that is not an in-image unknown (known pixel, outside
-	// the image, the region's frame) holds z = 0 at all times, so adding
-	// all four equals adding the unknown ones (x + 0 = x): no masks needed.
+	// s + the neighbours of the cell at z[id] (E, S, W, N); zs = row pitch,
+	// de / dw the east / west offsets.  A neighbour that is not an in-image
+	// unknown (known pixel, outside the image, the region's frame) holds
+	// z = 0 at all times, so adding all four equals adding the unknown ones
+	// (x + 0 = x): no masks needed.
 	static __device__ __forceinline__ float gather(unsigned, float s,
-			const float *z, int id, int zs)
+			const float *z, int id, int zs, int de, int dw)
 	{
-		const float e = z[id + 1], so = z[id + zs], w = z[id - 1], n = z[id - zs];
+		const float e = z[id + de], so = z[id + zs], w = z[id + dw], n = z[id - zs];
 		s += e;
 		s += so;
 		s += w;
@@ -739,11 +752,11 @@ struct LevG {
 		return d ? __builtin_amdgcn_rcpf((float)d) : 0.0f;
 	}
 	static __device__ __forceinline__ float gather(unsigned k, float s,
-			const float *z, int id, int zs)
+			const float *z, int id, int zs, int de, int dw)
 	{
-		s += (float)((k >> 11) & 31u) * z[id + 1];
+		s += (float)((k >> 11) & 31u) * z[id + de];
 		s += (float)((k >> 16) & 31u) * z[id + zs];
-		s += (float)((k >> 21) & 31u) * z[id - 1];
+		s += (float)((k >> 21) & 31u) * z[id + dw];
 		s += (float)(k >> 26) * z[id - zs];
 		return s;
 	}
@@ -797,13 +810,14 @@ static __device__ __forceinline__ long row_q(const MgSpan &S, int kb, int hr)
 	return S.q0 + (long)(2 * G::RG * kb + hr) * S.w;
 }
 
-// zero frame of the region's z
+// zero frame of the region's z: the rows above and below, and the frame
+// cell between the even and the odd columns of every row
 template <class G>
 static __device__ __forceinline__ void z_frame(float *z)
 {
 	for (int e = threadIdx.x; e < 2 * G::ZS + G::RH; e += G::T) {
 		const int at = e < G::ZS ? e : e < 2 * G::ZS ? G::ZOFF + G::RH * G::ZS + (e - G::ZS)
-			: G::ZOFF + (e - 2 * G::ZS) * G::ZS + G::RW;
+			: G::ZOFF + (e - 2 * G::ZS) * G::ZS + G::PAIRS_X;
 		z[at] = 0.0f;
 	}
 }
@@ -897,7 +911,8 @@ static __device__ __forceinline__ void mg_half(const MgCells<P, G> &C, float *z,
 			const int c = color == 0 ? (h ? 3 : 0) : (h ? 2 : 1);
 			const unsigned k = P::fresh(P::get(C.k[kb], c));
 			float s = C.b[kb][c];
-			if (!first) s = P::gather(k, s, z, cell_z(C, kb, c), G::ZS);
+			if (!first)
+				s = P::gather(k, s, z, cell_z(C, kb, c), G::ZS, G::de(c & 1), G::dw(c & 1));
 			nz[kb][h] = s * P::rdiag(k);
 		}
 #pragma unroll
@@ -947,7 +962,8 @@ void k_mg_down(const int4 *tiles,
 			const int id = cell_z(C, kb, c);
 			// an inactive cell adds nothing (its unmasked neighbour sum need
 			// not be 0)
-			const float az = P::diag(k) * z[id] - P::gather(k, 0.0f, z, id, G::ZS);
+			const float az = P::diag(k) * z[id] -
+				P::gather(k, 0.0f, z, id, G::ZS, G::de(c & 1), G::dw(c & 1));
 			acc += P::mask(k) * (C.b[kb][c] - az);
 		}
 		const int I = (ox >> 1) + C.pc, J = (oy >> 1) + C.m0 + G::RG * kb;
