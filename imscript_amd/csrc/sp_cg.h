@@ -29,16 +29,27 @@ struct Cg32Scal {
 	int trace0, trace_cap;  // slots of this plane in the rr trace
 };
 
-// Fused multigrid tile kernels: levels 0 and 1 smooth nu = 2 times on
-// 120x56 tiles (512 threads), levels 2..MG_FUSE_MAX nu = 4 times on 112x32
-// tiles (256 threads); the halo is 2 nu cells, the region 128 cells wide.  Those levels are
-// cheap (a sixteenth of the pixels and less), and the stronger smoothing
-// there saves V-cycles (14 -> 11 CG iterations to 5e-6 on 50% rectangle
-// holes); the levels below them (per-level kernels, tail) smooth nu = 2
-// times, where more sweeps change nothing.
+// Fused multigrid tile kernels: levels 0 and 1 smooth nu = 2 times on one
+// of the fine tile shapes, levels 2..MG_FUSE_MAX nu = 4 times on one of the
+// coarse tile shapes; the halo is 2 nu cells.  Those levels are cheap (a
+// sixteenth of the pixels and less), and the stronger smoothing there
+// saves V-cycles (14 -> 11 CG iterations to 5e-6 on 50% rectangle holes);
+// the levels below them (per-level kernels, tail) smooth nu = 2 times,
+// where more sweeps change nothing.
 constexpr int MG_NU_FINE = 2, MG_NU_COARSE = 4;
-constexpr int MG_TX_FINE = 120, MG_TY_FINE = 56, MG_T_FINE = 512;
-constexpr int MG_TX_COARSE = 112, MG_TY_COARSE = 32, MG_T_COARSE = 256;
+// tile shapes (width, height, threads).  The fewer cells a thread owns,
+// the shorter a tile's chain of dependent half sweeps, which is what bounds
+// the small levels (few tiles: latency); larger tiles recompute less halo.
+// A tile's interior is exact whatever its shape (it recomputes its halo),
+// so every shape gives the same bits.
+constexpr int MG_NFINE = 4;
+constexpr int MG_FINE_TX[MG_NFINE] = {120, 120, 120, 120};
+constexpr int MG_FINE_TY[MG_NFINE] = {56, 56, 88, 120};
+constexpr int MG_FINE_T[MG_NFINE] = {512, 1024, 1024, 1024};
+constexpr int MG_NGEO = 4;
+constexpr int MG_GEO_TX[MG_NGEO] = {112, 112, 112, 48};
+constexpr int MG_GEO_TY[MG_NGEO] = {32, 32, 48, 16};
+constexpr int MG_GEO_T[MG_NGEO] = {256, 512, 1024, 256};
 
 // one plane at one level of the multigrid preconditioner
 struct MgLevel {
@@ -46,14 +57,20 @@ struct MgLevel {
 	unsigned char *code;   // level 0: neighbour code per pixel (see sp_cg.hip)
 	float *D, *wE, *wS;    // levels > 0: Galerkin coefficients, D = 0 inactive
 	unsigned *gcode;       // levels 1..MG_FUSE_MAX: the same coefficients packed
+	                       // (one word per cell to MG_G32_MAX, then two)
 	float *z, *b;          // correction and right-hand side
+	float *zd;             // fused levels: z after the pre-smoothing
 	int w, h;
 };
 
-// levels 1..MG_FUSE_MAX run the fused tile kernels: their coefficients are
-// integers small enough to pack in 32 bits (diagonal <= 4^(l+1) <= 1024 in
-// 11 bits, couplings <= 2^l <= 16 in 5 bits each)
-constexpr int MG_FUSE_MAX = 4;
+// levels 1..MG_FUSE_MAX run the fused tile kernels.  Their coefficients
+// are integers: a coupling of level l is at most 2^l (the edges between two
+// 2^l x 2^l aggregates), the diagonal at most 4^(l+1).  Levels to
+// MG_G32_MAX pack them in 32 bits (diagonal in 11 bits, couplings in 5
+// bits each); the levels after them in 64 bits (couplings in a byte each,
+// <= 2^7, and the diagonal as a float).
+constexpr int MG_G32_MAX = 4;
+constexpr int MG_FUSE_MAX = 7;
 
 // work lists of one multigrid level (device arrays indexed by batch plane)
 struct MgLevelWork {
@@ -65,6 +82,8 @@ struct MgLevelWork {
 	BlockWork *build_c;    // coarse cells (next level) of every plane that has it
 	int4 *tiles;           // fused levels: tiles (plane, x0, y0, -) of the
 	                       // planes that run this level outside the tail
+	int geo;               // fused levels: tile shape (MG_FINE_* at level 1,
+	                       // MG_GEO_* from level 2)
 	int nmid, nlast, ncells_c, nbuild_c, ntiles;
 	double cells;          // cells covered by `tiles` (profiler bytes)
 };
@@ -99,6 +118,7 @@ struct Cg32Batch {
 	int tail_global;   // some plane runs the global-memory tail
 	int4 *tiles0;      // level-0 tiles (plane, x0, y0, -), contiguous per plane
 	int ntiles0;
+	int geo0;          // their shape (MG_FINE_*)
 	double *tpart;     // one partial per level-0 tile
 	double px0;        // level-0 pixels of the batch (profiler bytes)
 	ProfFn prof;

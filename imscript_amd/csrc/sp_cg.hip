@@ -568,9 +568,11 @@ __global__ __launch_bounds__(NT) void k_mg_prolong(const BlockWork *bw,
 // Level 0 is the masked -Laplacian itself.  A byte per pixel encodes it:
 // bit 0 unknown, bits 1-4 the E, S, W, N neighbour is an in-image unknown,
 // bits 5-7 the number of in-image neighbours (the diagonal, 2..4).
-// Levels 1..MG_FUSE_MAX pack their integer Galerkin coefficients into 32
+// Levels 1..MG_G32_MAX pack their integer Galerkin coefficients into 32
 // bits: the diagonal in bits 0-10, the E, S, W, N couplings in 5 bits each
-// from bit 11 (a cell is active when its diagonal is not 0).
+// from bit 11 (a cell is active when its diagonal is not 0); the levels
+// after them, to MG_FUSE_MAX, into two words: the couplings in a byte
+// each, and the diagonal's float bits.
 // The kernels below run on tiles of TX x TY cells (even origin) with a halo
 // of H = 2*nu: every red-black half sweep shrinks the region where z is
 // exact by one cell, and the first half sweep from z = 0 is exact
@@ -608,8 +610,10 @@ template <int TX_, int TY_, int NU_, int T_> struct MgGeo {
 	static __device__ __forceinline__ int de(int par) { return par ? 1 - HALF : HALF; }
 	static __device__ __forceinline__ int dw(int par) { return par ? -HALF : HALF - 1; }
 };
-using GeoFine = MgGeo<MG_TX_FINE, MG_TY_FINE, MG_NU_FINE, MG_T_FINE>;
-using GeoCoarse = MgGeo<MG_TX_COARSE, MG_TY_COARSE, MG_NU_COARSE, MG_T_COARSE>;
+template <int F>
+using GeoF = MgGeo<MG_FINE_TX[F], MG_FINE_TY[F], MG_NU_FINE, MG_FINE_T[F]>;
+template <int G>
+using GeoC = MgGeo<MG_GEO_TX[G], MG_GEO_TY[G], MG_NU_COARSE, MG_GEO_T[G]>;
 
 // level-0 code of every pixel of a CG plane (layout at Cg32Plane::code)
 __global__ __launch_bounds__(NT) void k32_code(const BlockWork *bw,
@@ -664,9 +668,11 @@ __global__ __launch_bounds__(NT) void k32_code(const BlockWork *bw,
 	if (q1 - q4 == 4) *(unsigned *)(P.code + q4) = word;
 }
 
-// packed coefficients of a level 1..MG_FUSE_MAX (one thread per cell)
+// packed coefficients of a level 1..MG_FUSE_MAX (one thread per cell);
+// wide: two words, the couplings E, S, W, N in a byte each and the
+// diagonal's float bits
 __global__ __launch_bounds__(NT) void k_mg_gcode(const BlockWork *bw,
-		const MgLevel *lv)
+		const MgLevel *lv, int wide)
 {
 	const BlockWork b = bw[blockIdx.x];
 	const MgLevel L = lv[b.plane];
@@ -679,21 +685,30 @@ __global__ __launch_bounds__(NT) void k_mg_gcode(const BlockWork *bw,
 		const unsigned e = (unsigned)L.wE[q], so = (unsigned)L.wS[q];
 		const unsigned we = i > 0 ? (unsigned)L.wE[q - 1] : 0u;
 		const unsigned no = j > 0 ? (unsigned)L.wS[q - L.w] : 0u;
-		L.gcode[q] = d | e << 11 | so << 16 | we << 21 | no << 26;
+		if (wide)
+			((uint2 *)L.gcode)[q] = make_uint2(e | so << 8 | we << 16 | no << 24,
+					__float_as_uint(L.D[q]));
+		else
+			L.gcode[q] = d | e << 11 | so << 16 | we << 21 | no << 26;
 	}
 }
 
 // level 0: byte codes, four to a register per 2x2 block
 struct Lev0 {
 	static constexpr int CW = 1;   // code words per block
-	typedef unsigned char Raw;
-	typedef uchar2 Raw2;
+	typedef unsigned char Raw;     // one cell's code in memory
+	typedef uchar2 Raw2;           // two cells'
+	typedef unsigned Code;         // one cell's code in registers
 	static __device__ __forceinline__ const Raw *codes(const MgLevel &L) { return L.code; }
-	static __device__ __forceinline__ void set(unsigned (&k)[CW], int c, unsigned x)
+	static __device__ __forceinline__ Code zero() { return 0u; }
+	static __device__ __forceinline__ Code one(Raw r) { return r; }
+	static __device__ __forceinline__ void split(Raw2 r, Code &a, Code &b)
+	{ a = r.x; b = r.y; }
+	static __device__ __forceinline__ void set(unsigned (&k)[CW], int c, Code x)
 	{ k[0] = c ? k[0] | x << (8 * c) : x; }
-	static __device__ __forceinline__ unsigned get(const unsigned (&k)[CW], int c)
+	static __device__ __forceinline__ Code get(const unsigned (&k)[CW], int c)
 	{ return (k[0] >> (8 * c)) & 255u; }
-	static __device__ __forceinline__ unsigned fresh(unsigned k)
+	static __device__ __forceinline__ Code fresh(Code k)
 	{
 		asm volatile("" : "+v"(k));
 		return k;
@@ -728,14 +743,19 @@ struct LevG {
 	static constexpr int CW = 4;
 	typedef unsigned Raw;
 	typedef uint2 Raw2;
+	typedef unsigned Code;
 	static __device__ __forceinline__ const Raw *codes(const MgLevel &L) { return L.gcode; }
-	static __device__ __forceinline__ void set(unsigned (&k)[CW], int c, unsigned x)
+	static __device__ __forceinline__ Code zero() { return 0u; }
+	static __device__ __forceinline__ Code one(Raw r) { return r; }
+	static __device__ __forceinline__ void split(Raw2 r, Code &a, Code &b)
+	{ a = r.x; b = r.y; }
+	static __device__ __forceinline__ void set(unsigned (&k)[CW], int c, Code x)
 	{ k[c] = x; }
-	static __device__ __forceinline__ unsigned get(const unsigned (&k)[CW], int c)
+	static __device__ __forceinline__ Code get(const unsigned (&k)[CW], int c)
 	{ return k[c]; }
 	// decode the coefficients again in every half sweep: hoisting the
 	// decoded floats out of the sweep loop costs 80 registers per thread
-	static __device__ __forceinline__ unsigned fresh(unsigned k)
+	static __device__ __forceinline__ Code fresh(Code k)
 	{
 		asm volatile("" : "+v"(k));
 		return k;
@@ -758,6 +778,46 @@ struct LevG {
 		s += (float)((k >> 16) & 31u) * z[id + zs];
 		s += (float)((k >> 21) & 31u) * z[id + dw];
 		s += (float)(k >> 26) * z[id - zs];
+		return s;
+	}
+};
+
+// levels MG_G32_MAX+1..MG_FUSE_MAX: couplings in a byte each, the diagonal
+// as a float (two words per cell)
+struct LevG64 {
+	static constexpr int CW = 8;
+	typedef uint2 Raw;
+	typedef uint4 Raw2;
+	typedef uint2 Code;
+	static __device__ __forceinline__ const Raw *codes(const MgLevel &L)
+	{ return (const uint2 *)L.gcode; }
+	static __device__ __forceinline__ Code zero() { return make_uint2(0u, 0u); }
+	static __device__ __forceinline__ Code one(Raw r) { return r; }
+	static __device__ __forceinline__ void split(Raw2 r, Code &a, Code &b)
+	{ a = make_uint2(r.x, r.y); b = make_uint2(r.z, r.w); }
+	static __device__ __forceinline__ void set(unsigned (&k)[CW], int c, Code x)
+	{ k[2 * c] = x.x; k[2 * c + 1] = x.y; }
+	static __device__ __forceinline__ Code get(const unsigned (&k)[CW], int c)
+	{ return make_uint2(k[2 * c], k[2 * c + 1]); }
+	static __device__ __forceinline__ Code fresh(Code k)
+	{
+		asm volatile("" : "+v"(k.x), "+v"(k.y));
+		return k;
+	}
+	static __device__ __forceinline__ float diag(Code k) { return __uint_as_float(k.y); }
+	static __device__ __forceinline__ float mask(Code k) { return diag(k) != 0.0f ? 1.0f : 0.0f; }
+	static __device__ __forceinline__ float rdiag(Code k)
+	{
+		const float d = diag(k);
+		return d != 0.0f ? __builtin_amdgcn_rcpf(d) : 0.0f;
+	}
+	static __device__ __forceinline__ float gather(Code k, float s,
+			const float *z, int id, int zs, int de, int dw)
+	{
+		s += (float)(k.x & 255u) * z[id + de];
+		s += (float)((k.x >> 8) & 255u) * z[id + zs];
+		s += (float)((k.x >> 16) & 255u) * z[id + dw];
+		s += (float)(k.x >> 24) * z[id - zs];
 		return s;
 	}
 };
@@ -839,7 +899,7 @@ static __device__ void mg_load(const MgLevel &L, int ox, int oy, MgCells<P, G> &
 #pragma unroll
 		for (int hr = 0; hr < 2; hr++) {
 			float bv[2] = {0.0f, 0.0f}, zv[2] = {0.0f, 0.0f};
-			unsigned kv[2] = {0u, 0u};
+			typename P::Code kv[2] = {P::zero(), P::zero()};
 			if (row_ok<G>(S, kb, hr)) {
 				const long q = row_q<G>(S, kb, hr);
 				if (pair) {
@@ -847,7 +907,7 @@ static __device__ void mg_load(const MgLevel &L, int ox, int oy, MgCells<P, G> &
 						const float2 b2 = *(const float2 *)(L.b + q);
 						const typename P::Raw2 k2 = *(const typename P::Raw2 *)(code + q);
 						bv[0] = b2.x; bv[1] = b2.y;
-						kv[0] = k2.x; kv[1] = k2.y;
+						P::split(k2, kv[0], kv[1]);
 						if (zsrc) {
 							const float2 z2 = *(const float2 *)(zsrc + q);
 							zv[0] = z2.x; zv[1] = z2.y;
@@ -858,7 +918,7 @@ static __device__ void mg_load(const MgLevel &L, int ox, int oy, MgCells<P, G> &
 					for (int hc = 0; hc < 2; hc++)
 						if (S.colok[hc]) {
 							bv[hc] = L.b[q + hc];
-							kv[hc] = code[q + hc];
+							kv[hc] = P::one(code[q + hc]);
 							if (zsrc) zv[hc] = zsrc[q + hc];
 						}
 				}
@@ -873,10 +933,10 @@ static __device__ void mg_load(const MgLevel &L, int ox, int oy, MgCells<P, G> &
 		}
 }
 
-// store z of the thread's tile cells (inside the image)
+// store z of the thread's tile cells (inside the image) into dst
 template <class P, class G>
-static __device__ __forceinline__ void mg_store_z(const MgLevel &L, int ox, int oy,
-		const MgCells<P, G> &C, const float *z, int kb)
+static __device__ __forceinline__ void mg_store_z(float *dst, const MgLevel &L,
+		int ox, int oy, const MgCells<P, G> &C, const float *z, int kb)
 {
 	const MgSpan S = mg_span(L, ox, oy, C.pc, C.m0);
 #pragma unroll
@@ -885,10 +945,10 @@ static __device__ __forceinline__ void mg_store_z(const MgLevel &L, int ox, int 
 		const long q = row_q<G>(S, kb, hr);
 		const float z0 = z[cell_z(C, kb, 2 * hr)], z1 = z[cell_z(C, kb, 2 * hr + 1)];
 		if (!(L.w & 1)) {
-			if (S.colok[0]) *(float2 *)(L.z + q) = make_float2(z0, z1);
+			if (S.colok[0]) *(float2 *)(dst + q) = make_float2(z0, z1);
 		} else {
-			if (S.colok[0]) L.z[q] = z0;
-			if (S.colok[1]) L.z[q + 1] = z1;
+			if (S.colok[0]) dst[q] = z0;
+			if (S.colok[1]) dst[q + 1] = z1;
 		}
 	}
 }
@@ -909,7 +969,7 @@ static __device__ __forceinline__ void mg_half(const MgCells<P, G> &C, float *z,
 			// colour of cell c is ((c>>1) + (c&1) + pair origin parity) & 1;
 			// origins are even, so cells 0,3 have colour 0 and 1,2 colour 1
 			const int c = color == 0 ? (h ? 3 : 0) : (h ? 2 : 1);
-			const unsigned k = P::fresh(P::get(C.k[kb], c));
+			const typename P::Code k = P::fresh(P::get(C.k[kb], c));
 			float s = C.b[kb][c];
 			if (!first)
 				s = P::gather(k, s, z, cell_z(C, kb, c), G::ZS, G::de(c & 1), G::dw(c & 1));
@@ -932,7 +992,8 @@ static __device__ __forceinline__ bool in_tile(const MgCells<P, G> &C, int kb)
 		&& m >= G::HALO / 2 && m < G::HALO / 2 + G::TY / 2;
 }
 
-// pre-smoothing from z = 0, residual, 2x2 restriction into the next level
+// pre-smoothing from z = 0 (into zd), residual, 2x2 restriction into the
+// next level
 template <class P, class G>
 __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(4)))
 void k_mg_down(const int4 *tiles,
@@ -954,11 +1015,11 @@ void k_mg_down(const int4 *tiles,
 #pragma unroll
 	for (int kb = 0; kb < G::K; kb++) {
 		if (!in_tile(C, kb)) continue;
-		mg_store_z(L, ox, oy, C, z, kb);
+		mg_store_z(L.zd, L, ox, oy, C, z, kb);
 		float acc = 0.0f;
 #pragma unroll
 		for (int c = 0; c < 4; c++) {
-			const unsigned k = P::get(C.k[kb], c);
+			const typename P::Code k = P::get(C.k[kb], c);
 			const int id = cell_z(C, kb, c);
 			// an inactive cell adds nothing (its unmasked neighbour sum need
 			// not be 0)
@@ -975,7 +1036,9 @@ void k_mg_down(const int4 *tiles,
 	}
 }
 
-// coarse correction, post-smoothing; at level 0 also the partial r.z of
+// coarse correction, post-smoothing (zd in, z out: a tile reads its halo
+// from other tiles' cells, so it must not read the buffer that tiles of the
+// same launch write); at level 0 also the partial r.z of
 // the tile
 template <class P, class G, bool DOT>
 __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(4)))
@@ -990,7 +1053,7 @@ void k_mg_up(const int4 *tiles, const MgLevel *lv,
 	const int ox = t.y - G::HALO, oy = t.z - G::HALO;
 	MgCells<P, G> C;
 	z_frame<G>(z);
-	mg_load(L, ox, oy, C, z, L.z);
+	mg_load(L, ox, oy, C, z, L.zd);
 	// every block of the region is one coarse cell: add its correction
 #pragma unroll
 	for (int kb = 0; kb < G::K; kb++) {
@@ -1010,7 +1073,7 @@ void k_mg_up(const int4 *tiles, const MgLevel *lv,
 #pragma unroll
 	for (int kb = 0; kb < G::K; kb++) {
 		if (!in_tile(C, kb)) continue;
-		mg_store_z(L, ox, oy, C, z, kb);
+		mg_store_z(L.z, L, ox, oy, C, z, kb);
 		if (DOT) {
 #pragma unroll
 			for (int c = 0; c < 4; c++)   // 0 at inactive cells
@@ -1289,36 +1352,60 @@ __global__ __launch_bounds__(MG_TAIL_T) void k_mg_tail_lds(const MgLevel *lvall,
 }
 
 static const char *const g_down_name[MG_FUSE_MAX + 1] = {
-	"mg_down0", "mg_down1", "mg_down2", "mg_down3", "mg_down4"};
+	"mg_down0", "mg_down1", "mg_down2", "mg_down3", "mg_down4", "mg_down5",
+	"mg_down6", "mg_down7"};
 static const char *const g_up_name[MG_FUSE_MAX + 1] = {
-	"mg_up0", "mg_up1", "mg_up2", "mg_up3", "mg_up4"};
+	"mg_up0", "mg_up1", "mg_up2", "mg_up3", "mg_up4", "mg_up5", "mg_up6", "mg_up7"};
+
+// call f with a value of the tile shape type Geo<geo>
+template <template <int> class Geo, class F>
+static void with_geo(int geo, F &&f)
+{
+	static_assert(MG_NFINE == 4 && MG_NGEO == 4, "one case per tile shape");
+	switch (geo) {
+	case 0: f(Geo<0>{}); break;
+	case 1: f(Geo<1>{}); break;
+	case 2: f(Geo<2>{}); break;
+	default: f(Geo<3>{}); break;
+	}
+}
 
 static void vcycle(const Cg32Batch &B, hipStream_t s)
 {
 	const int L = B.nlev;
 	// level 0 (never in the tail): fused pre-smoothing + restriction
 	// algorithmic bytes per level-0 pixel: down reads b and the code, writes
-	// z and a quarter coarse b (10 B); up reads z, b, the code and a quarter
-	// coarse z, writes z (14 B)
+	// zd and a quarter coarse b (10 B); up reads zd, b, the code and a
+	// quarter coarse z, writes z (14 B)
 	if (B.prof) B.prof("mg_down0", 1, 0, s);
-	k_mg_down<Lev0, GeoFine><<<B.ntiles0, GeoFine::T, 0, s>>>(B.tiles0, B.mg[0].lv,
-			B.mg[0].coarse, B.scal);
+	with_geo<GeoF>(B.geo0, [&](auto g) {
+		using G = decltype(g);
+		k_mg_down<Lev0, G><<<B.ntiles0, G::T, 0, s>>>(B.tiles0, B.mg[0].lv,
+				B.mg[0].coarse, B.scal);
+	});
 	if (B.prof) B.prof("mg_down0", 0, 10.0 * B.px0, s);
 	// levels >= 1 outside the tail: fused tiles (packed coefficients; per
-	// cell down reads b and the code, writes z and a quarter coarse b and z,
-	// 14 B; up reads z, b, the code and a quarter coarse z, writes z, 17 B)
-	// or per-level kernels (pre-smooth, restrict)
+	// cell down reads b and the code, writes zd and a quarter coarse b and
+	// z, 14 B; up reads zd, b, the code and a quarter coarse z, writes z,
+	// 17 B; 4 B more per cell for the two-word codes) or per-level kernels
+	// (pre-smooth, restrict)
 	for (int l = 1; l < L; l++) {
 		const MgLevelWork &W = B.mg[l];
 		if (W.ntiles) {
 			if (B.prof) B.prof(g_down_name[l], 1, 0, s);
+			auto down = [&](auto p, auto g) {
+				using P = decltype(p);
+				using G = decltype(g);
+				k_mg_down<P, G><<<W.ntiles, G::T, 0, s>>>(W.tiles, W.lv, W.coarse, B.scal);
+			};
 			if (l == 1)
-				k_mg_down<LevG, GeoFine><<<W.ntiles, GeoFine::T, 0, s>>>(W.tiles, W.lv,
-						W.coarse, B.scal);
+				with_geo<GeoF>(W.geo, [&](auto g) { down(LevG{}, g); });
+			else if (l <= MG_G32_MAX)
+				with_geo<GeoC>(W.geo, [&](auto g) { down(LevG{}, g); });
 			else
-				k_mg_down<LevG, GeoCoarse><<<W.ntiles, GeoCoarse::T, 0, s>>>(W.tiles, W.lv,
-						W.coarse, B.scal);
-			if (B.prof) B.prof(g_down_name[l], 0, 14.0 * W.cells, s);
+				with_geo<GeoC>(W.geo, [&](auto g) { down(LevG64{}, g); });
+			if (B.prof)
+				B.prof(g_down_name[l], 0, (l > MG_G32_MAX ? 18.0 : 14.0) * W.cells, s);
 		}
 		if (!W.nmid) continue;
 		for (int it = 0; it < B.nu; it++) {
@@ -1348,18 +1435,28 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 		}
 		if (W.ntiles) {
 			if (B.prof) B.prof(g_up_name[l], 1, 0, s);
+			auto up = [&](auto p, auto g) {
+				using P = decltype(p);
+				using G = decltype(g);
+				k_mg_up<P, G, false><<<W.ntiles, G::T, 0, s>>>(W.tiles, W.lv, W.coarse,
+						B.scal, B.wc, nullptr);
+			};
 			if (l == 1)
-				k_mg_up<LevG, GeoFine, false><<<W.ntiles, GeoFine::T, 0, s>>>(W.tiles,
-						W.lv, W.coarse, B.scal, B.wc, nullptr);
+				with_geo<GeoF>(W.geo, [&](auto g) { up(LevG{}, g); });
+			else if (l <= MG_G32_MAX)
+				with_geo<GeoC>(W.geo, [&](auto g) { up(LevG{}, g); });
 			else
-				k_mg_up<LevG, GeoCoarse, false><<<W.ntiles, GeoCoarse::T, 0, s>>>(W.tiles,
-						W.lv, W.coarse, B.scal, B.wc, nullptr);
-			if (B.prof) B.prof(g_up_name[l], 0, 17.0 * W.cells, s);
+				with_geo<GeoC>(W.geo, [&](auto g) { up(LevG64{}, g); });
+			if (B.prof)
+				B.prof(g_up_name[l], 0, (l > MG_G32_MAX ? 21.0 : 17.0) * W.cells, s);
 		}
 	}
 	if (B.prof) B.prof("mg_up0", 1, 0, s);
-	k_mg_up<Lev0, GeoFine, true><<<B.ntiles0, GeoFine::T, 0, s>>>(B.tiles0, B.mg[0].lv,
-			B.mg[0].coarse, B.scal, B.wc, B.tpart);
+	with_geo<GeoF>(B.geo0, [&](auto g) {
+		using G = decltype(g);
+		k_mg_up<Lev0, G, true><<<B.ntiles0, G::T, 0, s>>>(B.tiles0, B.mg[0].lv,
+				B.mg[0].coarse, B.scal, B.wc, B.tpart);
+	});
 	if (B.prof) B.prof("mg_up0", 0, 14.0 * B.px0, s);
 }
 
@@ -1389,7 +1486,8 @@ void mg_build(const Cg32Batch &B, hipStream_t s)
 	for (int l = 0; l < MG_FUSE_MAX && l + 1 < B.nlev; l++) {
 		const MgLevelWork &W = B.mg[l];
 		if (W.nbuild_c && B.mg[l + 1].ntiles)
-			k_mg_gcode<<<W.nbuild_c, NT, 0, s>>>(W.build_c, W.coarse);
+			k_mg_gcode<<<W.nbuild_c, NT, 0, s>>>(W.build_c, W.coarse,
+					l + 1 > MG_G32_MAX);
 	}
 }
 

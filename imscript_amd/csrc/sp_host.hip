@@ -96,6 +96,36 @@ static const int g_tail_lds_off = [] {
 	return e && !strcmp(e, "0") ? 1 : 0;
 }();
 
+// tile shape of the fused levels 0..MG_FUSE_MAX (index into MG_FINE_* at
+// levels 0 and 1, into MG_GEO_* from level 2); -1: the default rule.
+// SIMPOIS_HIP_MG_GEO="g0,g1,g2,..." sets them level by level.
+static int g_mg_geo[MG_FUSE_MAX + 1] = {-1, -1, -1, -1, -1, -1, -1, -1};
+static const int g_mg_geo_env = [] {
+	const char *e = getenv("SIMPOIS_HIP_MG_GEO");
+	for (int l = 0; e && *e && l <= MG_FUSE_MAX; l++) {
+		char *end;
+		const long v = strtol(e, &end, 10);
+		if (end == e) break;
+		if (v >= 0 && v < (l < 2 ? MG_NFINE : MG_NGEO)) g_mg_geo[l] = (int)v;
+		e = *end == ',' ? end + 1 : end;
+	}
+	return 0;
+}();
+
+// default tile shapes: 120x56 fine tiles with 1024 threads (two 2x2
+// blocks per thread); at coarse levels 112x48 with 1024 threads while the
+// level (all planes of the batch) has 2^19 cells or more, below that 48x16
+// tiles (many short tiles: those levels are bound by a tile's latency, not
+// by halo bytes).  Measured on 4096x4096x3 (per V-cycle, down + up): level
+// 0 338 us against 440 with 512 threads, level 2 50 us against 107 with
+// 112x32 / 256 threads, levels 4 and 5 16 and 15 us against 39 and 57.
+static int mg_geo_of(int l, double cells)
+{
+	if (g_mg_geo[l] >= 0) return g_mg_geo[l];
+	if (l < 2) return 1;
+	return cells >= (double)(1 << 19) ? 2 : 3;
+}
+
 static int ensure_device()
 {
 	static std::mutex mu;
@@ -785,18 +815,22 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 		}
 		nlev = std::max(nlev, (int)shp[z].size());
 	}
-	// storage: level 0 uses the CG vectors; levels > 0 own D wE wS z b
+	// storage: level 0 uses the CG vectors; levels > 0 own D wE wS z b;
+	// levels 0..MG_FUSE_MAX also zd
 	size_t cells = 0;
 	for (int z = 0; z < np; z++)
-		for (size_t l = 1; l < shp[z].size(); l++)
-			cells += rup64((size_t)shp[z][l].first * shp[z][l].second);
+		for (size_t l = 0; l < shp[z].size(); l++) {
+			const size_t n = rup64((size_t)shp[z][l].first * shp[z][l].second);
+			cells += (l > 0 ? 5 : 0) * n + (l <= (size_t)MG_FUSE_MAX ? n : 0);
+		}
 	float *pool = nullptr;
-	TRY(pl.alloc(&pool, 5 * std::max<size_t>(cells, 1)));
+	TRY(pl.alloc(&pool, std::max<size_t>(cells, 1)));
 	// packed coefficients of the levels that may run fused
 	size_t gcells = 0;
 	for (int z = 0; z < np; z++)
 		for (size_t l = 1; l < shp[z].size() && l <= (size_t)MG_FUSE_MAX; l++)
-			gcells += rup64((size_t)shp[z][l].first * shp[z][l].second);
+			gcells += (l > (size_t)MG_G32_MAX ? 2 : 1) *
+				rup64((size_t)shp[z][l].first * shp[z][l].second);
 	unsigned *gcodes = nullptr;
 	TRY(pl.alloc(&gcodes, std::max<size_t>(gcells, 1)));
 	size_t goff = 0;
@@ -808,20 +842,25 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 			MgLevel &L = lv[l][z];
 			L.w = shp[z][l].first;
 			L.h = shp[z][l].second;
+			const size_t n = rup64((size_t)L.w * L.h);
+			if (l <= (size_t)MG_FUSE_MAX) { L.zd = pool + off; off += n; }
 			if (l == 0) {
 				L.g = cp[z].g;
 				L.z = cp[z].z;
 				L.b = cp[z].r;
 				L.code = cp[z].code;
 				cp[z].t0 = (int)tiles0.size();
-				for (int y0 = 0; y0 < L.h; y0 += MG_TY_FINE)
-					for (int x0 = 0; x0 < L.w; x0 += MG_TX_FINE)
+				const int g0 = mg_geo_of(0, 0.0);
+				for (int y0 = 0; y0 < L.h; y0 += MG_FINE_TY[g0])
+					for (int x0 = 0; x0 < L.w; x0 += MG_FINE_TX[g0])
 						tiles0.push_back(make_int4(z, x0, y0, 0));
 				cp[z].nt = (int)tiles0.size() - cp[z].t0;
 				continue;
 			}
-			const size_t n = rup64((size_t)L.w * L.h);
-			if (l <= (size_t)MG_FUSE_MAX) { L.gcode = gcodes + goff; goff += n; }
+			if (l <= (size_t)MG_FUSE_MAX) {   // 256-byte aligned: n is a multiple of 64
+				L.gcode = gcodes + goff;
+				goff += (l > (size_t)MG_G32_MAX ? 2 : 1) * n;
+			}
 			L.D = pool + off; off += n;
 			L.wE = pool + off; off += n;
 			L.wS = pool + off; off += n;
@@ -855,6 +894,11 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 		std::vector<BlockWork> mid, cc, bc;
 		std::vector<int4> tl;
 		double tcells = 0;
+		// the level's tile shape from the cells of its fused planes
+		for (int z = 0; z < np; z++)
+			if ((int)shp[z].size() > l && l >= 1 && l < rng[z].x && l <= MG_FUSE_MAX)
+				tcells += (double)lv[l][z].w * lv[l][z].h;
+		const int geo = l <= MG_FUSE_MAX ? mg_geo_of(l, tcells) : 0;
 		for (int z = 0; z < np; z++) {
 			const int d = (int)shp[z].size();
 			if (d <= l) continue;
@@ -864,12 +908,11 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 			for (size_t st = 0; st < nc; st += EW_SPAN) bc.push_back({z, (int)st});
 			if (l < 1 || l >= rng[z].x) continue;   // level 0 or in the tail
 			if (l <= MG_FUSE_MAX) {   // fused tiles
-				const int tx = l == 1 ? MG_TX_FINE : MG_TX_COARSE;
-				const int ty = l == 1 ? MG_TY_FINE : MG_TY_COARSE;
+				const int tx = l == 1 ? MG_FINE_TX[geo] : MG_GEO_TX[geo];
+				const int ty = l == 1 ? MG_FINE_TY[geo] : MG_GEO_TY[geo];
 				for (int y0 = 0; y0 < Lz.h; y0 += ty)
 					for (int x0 = 0; x0 < Lz.w; x0 += tx)
 						tl.push_back(make_int4(z, x0, y0, 0));
-				tcells += (double)n;
 			} else {                  // per-level kernels
 				for (size_t st = 0; st < n; st += EW_SPAN) mid.push_back({z, (int)st});
 				for (size_t st = 0; st < nc; st += EW_SPAN) cc.push_back({z, (int)st});
@@ -883,6 +926,7 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 		TRY(pl.upload(&W.build_c, bc));
 		TRY(pl.upload(&W.tiles, tl));
 		W.ntiles = (int)tl.size();
+		W.geo = geo;
 		W.cells = tcells;
 		W.nmid = (int)mid.size();
 		W.nlast = 0;
@@ -899,6 +943,7 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 	for (const int4 &r : rng) B.tail_global |= !r.z;
 	TRY(pl.upload(&B.tiles0, tiles0));
 	B.ntiles0 = (int)tiles0.size();
+	B.geo0 = mg_geo_of(0, 0.0);
 	TRY(pl.alloc(&B.tpart, tiles0.size()));
 	B.mg = mgw.data();
 	B.nlev = nlev;
