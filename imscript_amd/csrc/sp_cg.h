@@ -27,6 +27,8 @@ struct Cg32Scal {
 	int active, iters, maxit;
 	int maxit_from_count;   // maxit = number of unknowns (the -c -1 rule)
 	int trace0, trace_cap;  // slots of this plane in the rr trace
+	int upd;           // this iteration updated r: the direction kernel
+	                   // adds alpha p to x before it replaces p
 };
 
 // Fused multigrid tile kernels: levels 0 and 1 smooth nu = 2 times on one

@@ -107,7 +107,41 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 	const unsigned cc = whole ? code4(P.code, q4) : 0u;
 	f4 rv = {0.0f, 0.0f, 0.0f, 0.0f};
 	d4 xv = {0.0, 0.0, 0.0, 0.0};
-	if (!whole || (cc & 0x01010101u)) {
+	const float *g = P.g, *u = P.u;
+	if (whole && (cc & 0x01010101u) && !P.bilap && !(w & 3)
+			&& !(((uintptr_t)g | (uintptr_t)u | (uintptr_t)P.f) & 15)) {
+		// a group inside one row of 16-byte aligned planes: the clamped
+		// 5-point reads of g and u as one 16-byte load per row and a scalar
+		// on each side; the same sums in the same order as below
+		const unsigned j = q4 / w, i = q4 - j * w;
+		const f4 gc = *(const f4 *)(g + q4), uc = *(const f4 *)(u + q4);
+		const f4 gn = j > 0 ? *(const f4 *)(g + q4 - w) : gc;
+		const f4 un = j > 0 ? *(const f4 *)(u + q4 - w) : uc;
+		const f4 gs = j + 1 < (unsigned)P.h ? *(const f4 *)(g + q4 + w) : gc;
+		const f4 us = j + 1 < (unsigned)P.h ? *(const f4 *)(u + q4 + w) : uc;
+		const float gw = i > 0 ? g[q4 - 1] : gc[0], uw = i > 0 ? u[q4 - 1] : uc[0];
+		const float ge = i + 4 < w ? g[q4 + 4] : gc[3], ue = i + 4 < w ? u[q4 + 4] : uc[3];
+		const f4 fv = P.f ? *(const f4 *)(P.f + q4) : f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+		for (int k = 0; k < 4; k++) {
+			if (!((cc >> (8 * k)) & 1u)) continue;
+			const float gv[4] = {k < 3 ? gc[k + 1] : ge, gs[k], k > 0 ? gc[k - 1] : gw, gn[k]};
+			const float uv[4] = {k < 3 ? uc[k + 1] : ue, us[k], k > 0 ? uc[k - 1] : uw, un[k]};
+			float r0 = -4.0f * 0.0f, rx = -4.0f * uc[k];
+#pragma unroll
+			for (int d = 0; d < 4; d++) {
+				const bool kn = !isnan(gv[d]);
+				r0 = r0 + (kn ? gv[d] : 0.0f);
+				rx = rx + (kn ? 0.0f : uv[d]);
+			}
+			const float bv = (P.f ? -fv[k] : 0.0f) - -r0;
+			const float r = bv - -rx;
+			rr += (double)r * r;
+			bb += (double)bv * bv;
+			rv[k] = r;
+			xv[k] = (double)uc[k];
+		}
+	} else if (!whole || (cc & 0x01010101u)) {
 		// coordinates of the group's first pixel; the others follow along
 		// the row (a group may wrap onto the next row)
 		unsigned j = q4 / w, i = q4 - j * w;
@@ -115,7 +149,6 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 			const unsigned k = q - q4;
 			const bool hole = whole ? ((cc >> (8 * k)) & 1u) : (P.code[q] & 1u);
 			if (hole) {
-				const float *g = P.g, *u = P.u;
 				float a0, ax;
 				if (!P.bilap) {
 					// op32's clamped 5-point sum in its term order, both fields
@@ -153,11 +186,15 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 			if (++i == w) { i = 0; j++; }
 		}
 	}
-	if (whole) {   // known entries are 0 in every vector
+	if (whole) {
+		// known entries are 0.  r is read everywhere (the multigrid's
+		// right-hand side); p, q, x only in groups with an unknown
 		*(f4 *)(P.r + q4) = rv;
-		*(f4 *)(P.p + q4) = rv;
-		*(f4 *)(P.q + q4) = f4{0.0f, 0.0f, 0.0f, 0.0f};
-		*(d4 *)(P.x + q4) = xv;
+		if (cc & 0x01010101u) {
+			*(f4 *)(P.p + q4) = rv;
+			*(f4 *)(P.q + q4) = f4{0.0f, 0.0f, 0.0f, 0.0f};
+			*(d4 *)(P.x + q4) = xv;
+		}
 	}
 	const double s1 = bsum<NT>(rr, red), s2 = bsum<NT>(bb, red);
 	if (threadIdx.x == 0) {
@@ -294,7 +331,10 @@ __global__ __launch_bounds__(NT) void k32_reduce_a(const Cg32Plane *pl, Cg32Scal
 {
 	__shared__ double red[NT / 64];
 	__shared__ double pq;
-	if (!sc[blockIdx.x].active) return;
+	if (!sc[blockIdx.x].active) {
+		if (threadIdx.x == 0) sc[blockIdx.x].upd = 0;
+		return;
+	}
 	fold(part, pl[blockIdx.x], 2, 0, red, &pq);
 	if (threadIdx.x == 0) {
 		Cg32Scal &s = sc[blockIdx.x];
@@ -302,15 +342,18 @@ __global__ __launch_bounds__(NT) void k32_reduce_a(const Cg32Plane *pl, Cg32Scal
 		// over): keep the current x and stop this plane
 		if (!(pq > 0) || !isfinite(s.rz / pq)) {
 			s.alpha = 0.0;
+			s.upd = 0;
 			s.active = 0;
 			atomicSub(nactive, 1);
 		} else {
 			s.alpha = s.rz / pq;
+			s.upd = 1;
 		}
 	}
 }
 
-// x += alpha p ; r -= alpha q ; partial r.r
+// r -= alpha q ; partial r.r  (x += alpha p follows in k32_dir, which
+// reads p anyway)
 __global__ __launch_bounds__(NT) void k32_update(const BlockWork *bw,
 		const Cg32Plane *pl, const Cg32Scal *sc, double *part)
 {
@@ -319,29 +362,24 @@ __global__ __launch_bounds__(NT) void k32_update(const BlockWork *bw,
 	const Cg32Scal &s = sc[b.plane];
 	if (!s.active) return;
 	const Cg32Plane P = pl[b.plane];
-	const double ad = s.alpha;
-	const float a = (float)ad;
+	const float a = (float)s.alpha;
 	double acc = 0;
 	const unsigned n = (unsigned)P.w * (unsigned)P.h;
 	const unsigned q4 = (unsigned)b.start + 4 * threadIdx.x;
 	if (q4 + 3 < n) {
 		if (code4(P.code, q4) & 0x01010101u) {
-			d4 x = *(const d4 *)(P.x + q4);
-			const f4 pp = *(const f4 *)(P.p + q4), qq = *(const f4 *)(P.q + q4);
+			const f4 qq = *(const f4 *)(P.q + q4);
 			f4 r = *(const f4 *)(P.r + q4);
 #pragma unroll
 			for (int k = 0; k < 4; k++) {
-				x[k] = x[k] + ad * (double)pp[k];
 				r[k] = r[k] - a * qq[k];
 				acc += (double)r[k] * r[k];
 			}
-			*(d4 *)(P.x + q4) = x;
 			*(f4 *)(P.r + q4) = r;
 		}
 	} else {
 		for (unsigned q = q4; q < n; q++) {
 			if (!(P.code[q] & 1)) continue;
-			P.x[q] = P.x[q] + ad * (double)P.p[q];
 			const float r = P.r[q] - a * P.q[q];
 			P.r[q] = r;
 			acc += (double)r * r;
@@ -375,28 +413,43 @@ __global__ __launch_bounds__(NT) void k32_reduce_b(const Cg32Plane *pl, Cg32Scal
 	}
 }
 
-// p = z + beta p  (z = r without preconditioner)
+// x += alpha p when this iteration updated r (also in the iteration that
+// stops the plane); p = z + beta p while the plane goes on (z = r without
+// preconditioner)
 __global__ __launch_bounds__(NT) void k32_dir(const BlockWork *bw,
 		const Cg32Plane *pl, const Cg32Scal *sc)
 {
 	const BlockWork b = bw[blockIdx.x];
 	const Cg32Scal &s = sc[b.plane];
-	if (!s.active) return;
+	const bool upd = s.upd, act = s.active;
+	if (!upd && !act) return;
 	const Cg32Plane P = pl[b.plane];
+	const double ad = s.alpha;
 	const float be = (float)s.beta;
 	const unsigned n = (unsigned)P.w * (unsigned)P.h;
 	const unsigned q4 = (unsigned)b.start + 4 * threadIdx.x;
 	if (q4 + 3 < n) {
 		if (code4(P.code, q4) & 0x01010101u) {
-			const f4 z = *(const f4 *)(P.z + q4);
 			f4 pp = *(const f4 *)(P.p + q4);
+			if (upd) {
+				d4 x = *(const d4 *)(P.x + q4);
 #pragma unroll
-			for (int k = 0; k < 4; k++) pp[k] = z[k] + be * pp[k];
-			*(f4 *)(P.p + q4) = pp;
+				for (int k = 0; k < 4; k++) x[k] = x[k] + ad * (double)pp[k];
+				*(d4 *)(P.x + q4) = x;
+			}
+			if (act) {
+				const f4 z = *(const f4 *)(P.z + q4);
+#pragma unroll
+				for (int k = 0; k < 4; k++) pp[k] = z[k] + be * pp[k];
+				*(f4 *)(P.p + q4) = pp;
+			}
 		}
 	} else {
-		for (unsigned q = q4; q < n; q++)
-			if (P.code[q] & 1) P.p[q] = P.z[q] + be * P.p[q];
+		for (unsigned q = q4; q < n; q++) {
+			if (!(P.code[q] & 1)) continue;
+			if (upd) P.x[q] = P.x[q] + ad * (double)P.p[q];
+			if (act) P.p[q] = P.z[q] + be * P.p[q];
+		}
 	}
 }
 
@@ -696,6 +749,7 @@ __global__ __launch_bounds__(NT) void k_mg_gcode(const BlockWork *bw,
 // level 0: byte codes, four to a register per 2x2 block
 struct Lev0 {
 	static constexpr int CW = 1;   // code words per block
+	static constexpr bool NEEDK = false;   // gather ignores the code
 	typedef unsigned char Raw;     // one cell's code in memory
 	typedef uchar2 Raw2;           // two cells'
 	typedef unsigned Code;         // one cell's code in registers
@@ -715,11 +769,13 @@ struct Lev0 {
 	}
 	static __device__ __forceinline__ float mask(unsigned k) { return (float)(k & 1u); }
 	static __device__ __forceinline__ float diag(unsigned k) { return (float)(k >> 5); }
-	// 1/diagonal, 0 for an inactive cell (code 0)
+	// 1/diagonal, 0 for an inactive cell (code 0); selects, no branches
 	static __device__ __forceinline__ float rdiag(unsigned k)
 	{
 		const unsigned d = k >> 5;
-		return d == 4 ? 0.25f : d == 3 ? (1.0f / 3.0f) : d == 2 ? 0.5f : 0.0f;
+		float r = d == 2 ? 0.5f : 0.0f;
+		r = d == 3 ? (1.0f / 3.0f) : r;
+		return d == 4 ? 0.25f : r;
 	}
 	// s + the neighbours of the cell at z[id] (E, S, W, N); zs = row pitch,
 	// de / dw the east / west offsets.  A neighbour that is not an in-image
@@ -741,6 +797,7 @@ struct Lev0 {
 // levels 1..MG_FUSE_MAX: packed integer coefficients, one word per cell
 struct LevG {
 	static constexpr int CW = 4;
+	static constexpr bool NEEDK = true;
 	typedef unsigned Raw;
 	typedef uint2 Raw2;
 	typedef unsigned Code;
@@ -786,6 +843,7 @@ struct LevG {
 // as a float (two words per cell)
 struct LevG64 {
 	static constexpr int CW = 8;
+	static constexpr bool NEEDK = true;
 	typedef uint2 Raw;
 	typedef uint4 Raw2;
 	typedef uint2 Code;
@@ -822,12 +880,24 @@ struct LevG64 {
 	}
 };
 
-// per-thread cells: 2x2 blocks, cell c of block kb at (2m + (c>>1), 2pc + (c&1))
+// per-thread cells: 2x2 blocks, cell c of block kb at (2m + (c>>1), 2pc + (c&1)).
+// With few cells per thread, 1/diagonal stays in registers (rd) instead of
+// being decoded again in every half sweep.
 template <class P, class G> struct MgCells {
+	static constexpr bool RD = G::K <= 4;
 	float b[G::K][4];
+	float rd[RD ? G::K : 1][4];
 	unsigned k[G::K][P::CW];
 	int pc, m0;
 };
+
+template <class P, class G>
+static __device__ __forceinline__ float cell_rdiag(const MgCells<P, G> &C, int kb,
+		int c, typename P::Code k)
+{
+	if constexpr (MgCells<P, G>::RD) return C.rd[kb][c];
+	else return P::rdiag(k);
+}
 
 template <class P, class G>
 static __device__ __forceinline__ int cell_z(const MgCells<P, G> &C, int kb, int c)
@@ -928,6 +998,7 @@ static __device__ void mg_load(const MgLevel &L, int ox, int oy, MgCells<P, G> &
 				const int c = 2 * hr + hc;
 				C.b[kb][c] = bv[hc];
 				P::set(C.k[kb], c, kv[hc]);
+				if constexpr (MgCells<P, G>::RD) C.rd[kb][c] = P::rdiag(kv[hc]);
 				z[cell_z(C, kb, c)] = zv[hc];
 			}
 		}
@@ -969,11 +1040,12 @@ static __device__ __forceinline__ void mg_half(const MgCells<P, G> &C, float *z,
 			// colour of cell c is ((c>>1) + (c&1) + pair origin parity) & 1;
 			// origins are even, so cells 0,3 have colour 0 and 1,2 colour 1
 			const int c = color == 0 ? (h ? 3 : 0) : (h ? 2 : 1);
-			const typename P::Code k = P::fresh(P::get(C.k[kb], c));
+			typename P::Code k = P::zero();
+			if constexpr (P::NEEDK || !MgCells<P, G>::RD) k = P::fresh(P::get(C.k[kb], c));
 			float s = C.b[kb][c];
 			if (!first)
 				s = P::gather(k, s, z, cell_z(C, kb, c), G::ZS, G::de(c & 1), G::dw(c & 1));
-			nz[kb][h] = s * P::rdiag(k);
+			nz[kb][h] = s * cell_rdiag(C, kb, c, k);
 		}
 #pragma unroll
 	for (int kb = 0; kb < G::K; kb++)

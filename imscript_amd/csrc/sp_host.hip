@@ -1096,9 +1096,9 @@ static int run_cg32(Plan &pl, int k, int bilap, float eps, hipStream_t s)
 		unk += (double)it * cnts[z];
 	}
 	// algorithmic bytes per unknown and iteration: spmv reads p, writes q
-	// (8); update reads x (double) r p q, writes x r (32); direction reads
-	// z p, writes p (12): 52 B.  The multigrid V-cycle is not counted.
-	pr.add((pl.P.cg_mg ? 0 : 5) * launched, 52.0 * unk);
+	// (8); update reads r q, writes r (12); direction reads z p x (double),
+	// writes p x (28): 48 B.  The multigrid V-cycle is not counted.
+	pr.add((pl.P.cg_mg ? 0 : 5) * launched, 48.0 * unk);
 	return SP_OK;
 }
 
