@@ -931,8 +931,12 @@ def main():
     torch.cuda.synchronize()
     cg_ms, cg_launch, cg_bytes = sp.profile_read("cg32")
     gs_ms, gs_launch, gs_bytes = sp.profile_read("gs_tile")
-    kern = {k: sp.profile_read(k) for k in ("mg_up0", "mg_down0", "k32_spmv")}
+    kern = {k: sp.profile_read(k) for k in ("mg_up0", "mg_down0", "k32_dirspmv")}
     sp.profile(False)
+    # the fused direction + product kernel per unknown: reads z, p_old, x
+    # (double) and the code, writes p_new, q, x (33 B)
+    ms_, n_, _ = kern["k32_dirspmv"]
+    kern["k32_dirspmv"] = (ms_, n_, 33.0 * hole_frac * a.size * a.size * a.channels * n_)
 
     mpix = a.size * a.size * a.channels / 1e6
     result = {
@@ -1006,7 +1010,8 @@ def main():
                                  "4 red-black half sweeps + r.z)",
                        "mg_down0": "k_mg_down0 (fused level-0 4 red-black half "
                                    "sweeps + residual + restriction)",
-                       "k32_spmv": "k32_spmv"}[dom],
+                       "k32_dirspmv": "k32_dirspmv (fused CG direction + "
+                                      "operator product + p.q)"}[dom],
             "bound": "hbm", "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBPS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBPS, 4),

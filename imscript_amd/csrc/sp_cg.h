@@ -10,6 +10,9 @@ struct Cg32Plane {
 	double *x;         // solution accumulates in double: ~1e3 float updates
 	                   // would otherwise cost ~1e-5 of relative residual
 	float *r, *p, *q;
+	float *p2;         // second direction buffer: the fused direction +
+	                   // product kernel reads p_old from one, writes p_new
+	                   // to the other, swapping every iteration
 	float *z;          // preconditioned residual (== r without multigrid)
 	unsigned char *code;   // per pixel: bit 0 unknown, bits 1-4 the E, S, W,
 	                       // N neighbour is an in-image unknown, bits 5-7 the
@@ -109,6 +112,7 @@ struct Cg32Batch {
 	double *trace;     // optional rr_new trace
 	int *nactive;      // planes still iterating
 	int nplanes, nblk;
+	int bilap;         // biharmonic: separate direction and product kernels
 	// multigrid preconditioner (nlev == 0: plain CG)
 	MgLevelWork *mg;   // host array of nlev levels
 	int nlev, nu, coarse_sweeps;   // nu: per-level kernels and the tail
@@ -128,7 +132,9 @@ struct Cg32Batch {
 
 namespace spk {
 void cg32_setup(const Cg32Batch &B, hipStream_t s);
-void cg32_iter(const Cg32Batch &B, hipStream_t s);
+// one iteration; par: parity of the iteration in its captured chunk (the
+// direction buffer it reads), chunks are even
+void cg32_iter(const Cg32Batch &B, hipStream_t s, int par);
 void cg32_finish(const Cg32Batch &B, hipStream_t s);
 void mg_build(const Cg32Batch &B, hipStream_t s);
 // allow `bytes` of dynamic LDS for the LDS tail kernel; 0 on success

@@ -188,13 +188,16 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 	}
 	if (whole) {
 		// known entries are 0.  r is read everywhere (the multigrid's
-		// right-hand side); p, q, x only in groups with an unknown
+		// right-hand side), x only in groups with an unknown.  The
+		// biharmonic iteration starts from p = r, and its product reads p
+		// at every pixel of the 13-point stencil, unknown or not: p and q
+		// everywhere.  (The Poisson iteration writes p and q itself.)
 		*(f4 *)(P.r + q4) = rv;
-		if (cc & 0x01010101u) {
+		if (P.bilap) {
 			*(f4 *)(P.p + q4) = rv;
 			*(f4 *)(P.q + q4) = f4{0.0f, 0.0f, 0.0f, 0.0f};
-			*(d4 *)(P.x + q4) = xv;
 		}
+		if (cc & 0x01010101u) *(d4 *)(P.x + q4) = xv;
 	}
 	const double s1 = bsum<NT>(rr, red), s2 = bsum<NT>(bb, red);
 	if (threadIdx.x == 0) {
@@ -451,6 +454,127 @@ __global__ __launch_bounds__(NT) void k32_dir(const BlockWork *bw,
 			if (act) P.p[q] = P.z[q] + be * P.p[q];
 		}
 	}
+}
+
+// Direction and product in one pass (Poisson operator):
+//   x += alpha p_old            when this iteration updated r (upd)
+//   p_new = z + beta p_old      while the plane goes on (first: p_new = z)
+//   q = A p_new, partial p_new.q
+// p_new of the unknown neighbours is recomputed from their z and p_old
+// with the same operations as their owners, so it has the same bits.
+// Neighbours read p_old while owners write p_new: p_old and p_new are two
+// buffers, p (par 1) or p2 (par 0) receiving p_new.  Every value is taken
+// under the unknown bits, so known pixels and groups without unknowns may
+// hold anything.
+static __device__ __forceinline__ float pnew(float z, float po, float be, bool first)
+{
+	return first ? z : z + be * po;
+}
+
+__global__ __launch_bounds__(NT) void k32_dirspmv(const BlockWork *bw,
+		const Cg32Plane *pl, const Cg32Scal *sc, double *part, int par, int first)
+{
+	__shared__ double red[NT / 64];
+	const BlockWork b = bw[blockIdx.x];
+	const Cg32Scal &s = sc[b.plane];
+	const bool upd = !first && s.upd, act = s.active;
+	if (!upd && !act) return;
+	const Cg32Plane P = pl[b.plane];
+	const float *po = par ? P.p2 : P.p;
+	float *pw_ = par ? P.p : P.p2;
+	const float be = (float)s.beta;
+	const double ad = s.alpha;
+	const bool fst = first;
+	double acc = 0;
+	const unsigned n = (unsigned)P.w * (unsigned)P.h, w = (unsigned)P.w;
+	const unsigned q4 = (unsigned)b.start + 4 * threadIdx.x;
+	if (q4 + 3 < n) {
+		const unsigned cc = code4(P.code, q4);
+		if (cc & 0x01010101u) {
+			const f4 z0 = {0.0f, 0.0f, 0.0f, 0.0f};
+			const f4 poc = fst ? z0 : *(const f4 *)(po + q4);
+			if (upd) {
+				d4 x = *(const d4 *)(P.x + q4);
+#pragma unroll
+				for (int k = 0; k < 4; k++)
+					if ((cc >> (8 * k)) & 1u) x[k] = x[k] + ad * (double)poc[k];
+				*(d4 *)(P.x + q4) = x;
+			}
+			if (act) {
+				const f4 zc = *(const f4 *)(P.z + q4);
+				f4 pc, pn, ps;
+#pragma unroll
+				for (int k = 0; k < 4; k++)
+					pc[k] = (cc >> (8 * k)) & 1u ? pnew(zc[k], poc[k], be, fst) : 0.0f;
+				if (!(w & 3) && q4 >= w) {
+					const f4 zn = *(const f4 *)(P.z + q4 - w);
+					const f4 on = fst ? z0 : *(const f4 *)(po + q4 - w);
+#pragma unroll
+					for (int k = 0; k < 4; k++)
+						pn[k] = (cc >> (8 * k)) & 16u ? pnew(zn[k], on[k], be, fst) : 0.0f;
+				} else {
+#pragma unroll
+					for (int k = 0; k < 4; k++)
+						pn[k] = (cc >> (8 * k)) & 16u ? pnew(P.z[q4 + k - w],
+								fst ? 0.0f : po[q4 + k - w], be, fst) : 0.0f;
+				}
+				if (!(w & 3) && q4 + w + 3 < n) {
+					const f4 zs = *(const f4 *)(P.z + q4 + w);
+					const f4 os = fst ? z0 : *(const f4 *)(po + q4 + w);
+#pragma unroll
+					for (int k = 0; k < 4; k++)
+						ps[k] = (cc >> (8 * k)) & 4u ? pnew(zs[k], os[k], be, fst) : 0.0f;
+				} else {
+#pragma unroll
+					for (int k = 0; k < 4; k++)
+						ps[k] = (cc >> (8 * k)) & 4u ? pnew(P.z[q4 + k + w],
+								fst ? 0.0f : po[q4 + k + w], be, fst) : 0.0f;
+				}
+				const float pe = cc & (2u << 24) ? pnew(P.z[q4 + 4],
+						fst ? 0.0f : po[q4 + 4], be, fst) : 0.0f;
+				const float pw = cc & 8u ? pnew(P.z[q4 - 1],
+						fst ? 0.0f : po[q4 - 1], be, fst) : 0.0f;
+				f4 y;
+#pragma unroll
+				for (int k = 0; k < 4; k++) {
+					const unsigned c = (cc >> (8 * k)) & 255u;
+					const float e = k < 3 ? pc[k + 1] : pe, ww = k > 0 ? pc[k - 1] : pw;
+					float nb = 0.0f;
+					if (c & 2) nb += e;
+					if (c & 4) nb += ps[k];
+					if (c & 8) nb += ww;
+					if (c & 16) nb += pn[k];
+					y[k] = (float)(c >> 5) * pc[k] - nb;
+					acc += (double)y[k] * pc[k];
+				}
+				*(f4 *)(pw_ + q4) = pc;
+				*(f4 *)(P.q + q4) = y;
+			}
+		}
+	} else {
+		for (unsigned q = q4; q < n; q++) {
+			const unsigned c = P.code[q];
+			if (!(c & 1)) {
+				if (act) { pw_[q] = 0.0f; P.q[q] = 0.0f; }
+				continue;
+			}
+			const float o = fst ? 0.0f : po[q];
+			if (upd) P.x[q] = P.x[q] + ad * (double)o;
+			if (!act) continue;
+			const float pc = pnew(P.z[q], o, be, fst);
+			float nb = 0.0f;
+			if (c & 2) nb += pnew(P.z[q + 1], fst ? 0.0f : po[q + 1], be, fst);
+			if (c & 4) nb += pnew(P.z[q + w], fst ? 0.0f : po[q + w], be, fst);
+			if (c & 8) nb += pnew(P.z[q - 1], fst ? 0.0f : po[q - 1], be, fst);
+			if (c & 16) nb += pnew(P.z[q - w], fst ? 0.0f : po[q - w], be, fst);
+			const float y = (float)(c >> 5) * pc - nb;
+			pw_[q] = pc;
+			P.q[q] = y;
+			acc += (double)y * pc;
+		}
+	}
+	const double t = bsum<NT>(acc, red);
+	if (threadIdx.x == 0) part[2 * blockIdx.x] = t;
 }
 
 __global__ __launch_bounds__(NT) void k32_finish(const BlockWork *bw,
@@ -1532,12 +1656,12 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 	if (B.prof) B.prof("mg_up0", 0, 14.0 * B.px0, s);
 }
 
-// z = M r, rho = r.z (and p = z at setup)
+// z = M r, rho = r.z (and p = z at setup on the separate-kernel path)
 static void precondition(const Cg32Batch &B, int setup, hipStream_t s)
 {
 	vcycle(B, s);
 	k_mg_reduce_t<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.tpart, setup);
-	if (setup)   // p = z + 0 * p
+	if (setup && B.bilap)   // p = z + 0 * p
 		k32_dir<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal);
 }
 
@@ -1572,21 +1696,38 @@ void cg32_setup(const Cg32Batch &B, hipStream_t s)
 		mg_build(B, s);
 		precondition(B, 1, s);
 	}
+	// p = z, q = A p, p.q into the buffer iteration 0 reads (par 1 -> p)
+	if (!B.bilap)
+		k32_dirspmv<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part, 1, 1);
 }
 
-void cg32_iter(const Cg32Batch &B, hipStream_t s)
+// Poisson: alpha, r -= alpha q, beta, z = M r, then the fused direction +
+// product (x += alpha p_old, p_new, q = A p_new, p.q) reading p_old from the
+// buffer of parity par.  Biharmonic: product, alpha, update, beta, z,
+// direction (one p buffer).
+void cg32_iter(const Cg32Batch &B, hipStream_t s, int par)
 {
 	const int pc = B.nlev > 0;
-	// spmv: read p (stencil) and the mask, write q: 12 B per pixel
-	if (B.prof) B.prof("k32_spmv", 1, 0, s);
-	k32_spmv<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
-	if (B.prof) B.prof("k32_spmv", 0, 12.0 * B.px0, s);
+	if (B.bilap) {
+		k32_spmv<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
+		k32_reduce_a<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive);
+		k32_update<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
+		k32_reduce_b<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive,
+				B.trace, pc);
+		if (pc) precondition(B, 0, s);
+		k32_dir<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal);
+		return;
+	}
 	k32_reduce_a<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive);
 	k32_update<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
 	k32_reduce_b<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive,
 			B.trace, pc);
 	if (pc) precondition(B, 0, s);
-	k32_dir<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal);
+	// reads z, the code, p_old (with the four neighbours) and x, writes
+	// p_new, q and x: 29 B per unknown
+	if (B.prof) B.prof("k32_dirspmv", 1, 0, s);
+	k32_dirspmv<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part, par, 0);
+	if (B.prof) B.prof("k32_dirspmv", 0, 0.0, s);
 }
 
 void cg32_finish(const Cg32Batch &B, hipStream_t s)

@@ -967,7 +967,7 @@ static int cg32_build(Plan &pl, int k, int bilap, float eps)
 	size_t need = 0;
 	for (int m : C.ms) need += rup64((size_t)L.sp[m].w * L.sp[m].h);
 	if (need > pl.cg32_cap) {
-		TRY(pl.alloc(&pl.cg32_vec, need * 4));
+		TRY(pl.alloc(&pl.cg32_vec, need * 5));
 		TRY(pl.alloc(&pl.cg32_x, need));
 		TRY(pl.alloc(&pl.cg32_code, need));
 		pl.cg32_cap = need;
@@ -989,6 +989,7 @@ static int cg32_build(Plan &pl, int k, int bilap, float eps)
 		c.r = pl.cg32_vec + off; off += na;
 		c.p = pl.cg32_vec + off; off += na;
 		c.q = pl.cg32_vec + off; off += na;
+		c.p2 = pl.cg32_vec + off; off += na;
 		c.z = c.r;
 		if (pl.P.cg_mg) { c.z = pl.cg32_vec + off; off += na; }
 		c.w = S.w; c.h = S.h; c.bilap = bilap;
@@ -1026,11 +1027,13 @@ static int cg32_build(Plan &pl, int k, int bilap, float eps)
 	// iterations in chunks, captured once as a graph
 	C.chunk = pl.P.cg_mg ? std::max(1, std::min(pl.P.cg_chunk, 4))
 		: std::max(1, pl.P.cg_chunk);
+	C.chunk += C.chunk & 1;   // even: the direction buffers swap every iteration
+	B.bilap = bilap;
 	if (!pl.cap_stream)
 		HIPCHK(hipStreamCreateWithFlags(&pl.cap_stream, hipStreamNonBlocking));
 	hipGraph_t graph;
 	HIPCHK(hipStreamBeginCapture(pl.cap_stream, hipStreamCaptureModeThreadLocal));
-	for (int z = 0; z < C.chunk; z++) spk::cg32_iter(B, pl.cap_stream);
+	for (int z = 0; z < C.chunk; z++) spk::cg32_iter(B, pl.cap_stream, z & 1);
 	HIPCHK(hipStreamEndCapture(pl.cap_stream, &graph));
 	const hipError_t ie = hipGraphInstantiate(&C.exec, graph, nullptr, nullptr, 0);
 	(void)hipGraphDestroy(graph);
@@ -1061,7 +1064,7 @@ static int run_cg32(Plan &pl, int k, int bilap, float eps, hipStream_t s)
 	spk::cg32_setup(B, s);
 	long launched = 0;
 	for (long done = 0; done < C.maxit_host; done += C.chunk) {
-		if (prof) for (int z = 0; z < C.chunk; z++) spk::cg32_iter(B, s);
+		if (prof) for (int z = 0; z < C.chunk; z++) spk::cg32_iter(B, s, z & 1);
 		else HIPCHK(hipGraphLaunch(C.exec, s));
 		launched += C.chunk;
 		HIPCHK(hipMemcpyAsync(pl.h_done, B.nactive, sizeof(int), hipMemcpyDeviceToHost, s));
