@@ -34,27 +34,29 @@ struct Cg32Scal {
 	                   // adds alpha p to x before it replaces p
 };
 
-// Fused multigrid tile kernels: levels 0 and 1 smooth nu = 2 times on one
-// of the fine tile shapes, levels 2..MG_FUSE_MAX nu = 4 times on one of the
-// coarse tile shapes; the halo is 2 nu cells.  Those levels are cheap (a
-// sixteenth of the pixels and less), and the stronger smoothing there
-// saves V-cycles (14 -> 11 CG iterations to 5e-6 on 50% rectangle holes);
-// the levels below them (per-level kernels, tail) smooth nu = 2 times,
-// where more sweeps change nothing.
-constexpr int MG_NU_FINE = 2, MG_NU_COARSE = 4;
-// tile shapes (width, height, threads).  The fewer cells a thread owns,
+// Fused multigrid tile kernels: levels 0 and 1 run one of the fine tile
+// shapes, levels 2..MG_FUSE_MAX one of the coarse shapes; a shape fixes the
+// number nu of red-black sweeps before and after the coarse correction
+// (the halo is 2 nu cells).  A launch moves its region through HBM once
+// whatever nu is, so stronger smoothing there costs compute only and saves
+// V-cycles; the levels below them (per-level kernels, tail) smooth
+// MG_NU_TAIL times, where more sweeps change nothing.
+constexpr int MG_NU_TAIL = 2;
+// tile shapes (width, height, nu, threads).  The fewer cells a thread owns,
 // the shorter a tile's chain of dependent half sweeps, which is what bounds
 // the small levels (few tiles: latency); larger tiles recompute less halo.
-// A tile's interior is exact whatever its shape (it recomputes its halo),
-// so every shape gives the same bits.
-constexpr int MG_NFINE = 4;
-constexpr int MG_FINE_TX[MG_NFINE] = {120, 120, 120, 120};
-constexpr int MG_FINE_TY[MG_NFINE] = {56, 56, 88, 120};
-constexpr int MG_FINE_T[MG_NFINE] = {512, 1024, 1024, 1024};
-constexpr int MG_NGEO = 4;
-constexpr int MG_GEO_TX[MG_NGEO] = {112, 112, 112, 48};
-constexpr int MG_GEO_TY[MG_NGEO] = {32, 32, 48, 16};
-constexpr int MG_GEO_T[MG_NGEO] = {256, 512, 1024, 256};
+// A tile's interior is exact whatever its size (it recomputes its halo),
+// so shapes of the same nu give the same bits.
+constexpr int MG_NFINE = 6;
+constexpr int MG_FINE_TX[MG_NFINE] = {120, 120, 116, 112, 108, 104};
+constexpr int MG_FINE_TY[MG_NFINE] = {56, 56, 52, 48, 44, 40};
+constexpr int MG_FINE_NU[MG_NFINE] = {2, 2, 3, 4, 5, 6};
+constexpr int MG_FINE_T[MG_NFINE] = {512, 1024, 1024, 1024, 1024, 1024};
+constexpr int MG_NGEO = 6;
+constexpr int MG_GEO_TX[MG_NGEO] = {112, 112, 112, 48, 104, 40};
+constexpr int MG_GEO_TY[MG_NGEO] = {32, 32, 48, 16, 40, 8};
+constexpr int MG_GEO_NU[MG_NGEO] = {4, 4, 4, 4, 6, 6};
+constexpr int MG_GEO_T[MG_NGEO] = {256, 512, 1024, 256, 1024, 256};
 
 // one plane at one level of the multigrid preconditioner
 struct MgLevel {

@@ -788,9 +788,9 @@ template <int TX_, int TY_, int NU_, int T_> struct MgGeo {
 	static __device__ __forceinline__ int dw(int par) { return par ? -HALF : HALF - 1; }
 };
 template <int F>
-using GeoF = MgGeo<MG_FINE_TX[F], MG_FINE_TY[F], MG_NU_FINE, MG_FINE_T[F]>;
+using GeoF = MgGeo<MG_FINE_TX[F], MG_FINE_TY[F], MG_FINE_NU[F], MG_FINE_T[F]>;
 template <int G>
-using GeoC = MgGeo<MG_GEO_TX[G], MG_GEO_TY[G], MG_NU_COARSE, MG_GEO_T[G]>;
+using GeoC = MgGeo<MG_GEO_TX[G], MG_GEO_TY[G], MG_GEO_NU[G], MG_GEO_T[G]>;
 
 // level-0 code of every pixel of a CG plane (layout at Cg32Plane::code)
 __global__ __launch_bounds__(NT) void k32_code(const BlockWork *bw,
@@ -1553,17 +1553,20 @@ static const char *const g_down_name[MG_FUSE_MAX + 1] = {
 static const char *const g_up_name[MG_FUSE_MAX + 1] = {
 	"mg_up0", "mg_up1", "mg_up2", "mg_up3", "mg_up4", "mg_up5", "mg_up6", "mg_up7"};
 
-// call f with a value of the tile shape type Geo<geo>
-template <template <int> class Geo, class F>
+// call f with a value of the tile shape type Geo<geo>, geo in [I, N)
+template <template <int> class Geo, int I, int N, class F>
+static void with_geo_from(int geo, F &f)
+{
+	if constexpr (I + 1 < N) {
+		if (geo != I) { with_geo_from<Geo, I + 1, N>(geo, f); return; }
+	}
+	f(Geo<I>{});
+}
+
+template <template <int> class Geo, int N, class F>
 static void with_geo(int geo, F &&f)
 {
-	static_assert(MG_NFINE == 4 && MG_NGEO == 4, "one case per tile shape");
-	switch (geo) {
-	case 0: f(Geo<0>{}); break;
-	case 1: f(Geo<1>{}); break;
-	case 2: f(Geo<2>{}); break;
-	default: f(Geo<3>{}); break;
-	}
+	with_geo_from<Geo, 0, N>(geo, f);
 }
 
 static void vcycle(const Cg32Batch &B, hipStream_t s)
@@ -1574,7 +1577,7 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 	// zd and a quarter coarse b (10 B); up reads zd, b, the code and a
 	// quarter coarse z, writes z (14 B)
 	if (B.prof) B.prof("mg_down0", 1, 0, s);
-	with_geo<GeoF>(B.geo0, [&](auto g) {
+	with_geo<GeoF, MG_NFINE>(B.geo0, [&](auto g) {
 		using G = decltype(g);
 		k_mg_down<Lev0, G><<<B.ntiles0, G::T, 0, s>>>(B.tiles0, B.mg[0].lv,
 				B.mg[0].coarse, B.scal);
@@ -1595,11 +1598,11 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 				k_mg_down<P, G><<<W.ntiles, G::T, 0, s>>>(W.tiles, W.lv, W.coarse, B.scal);
 			};
 			if (l == 1)
-				with_geo<GeoF>(W.geo, [&](auto g) { down(LevG{}, g); });
+				with_geo<GeoF, MG_NFINE>(W.geo, [&](auto g) { down(LevG{}, g); });
 			else if (l <= MG_G32_MAX)
-				with_geo<GeoC>(W.geo, [&](auto g) { down(LevG{}, g); });
+				with_geo<GeoC, MG_NGEO>(W.geo, [&](auto g) { down(LevG{}, g); });
 			else
-				with_geo<GeoC>(W.geo, [&](auto g) { down(LevG64{}, g); });
+				with_geo<GeoC, MG_NGEO>(W.geo, [&](auto g) { down(LevG64{}, g); });
 			if (B.prof)
 				B.prof(g_down_name[l], 0, (l > MG_G32_MAX ? 18.0 : 14.0) * W.cells, s);
 		}
@@ -1638,17 +1641,17 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 						B.scal, B.wc, nullptr);
 			};
 			if (l == 1)
-				with_geo<GeoF>(W.geo, [&](auto g) { up(LevG{}, g); });
+				with_geo<GeoF, MG_NFINE>(W.geo, [&](auto g) { up(LevG{}, g); });
 			else if (l <= MG_G32_MAX)
-				with_geo<GeoC>(W.geo, [&](auto g) { up(LevG{}, g); });
+				with_geo<GeoC, MG_NGEO>(W.geo, [&](auto g) { up(LevG{}, g); });
 			else
-				with_geo<GeoC>(W.geo, [&](auto g) { up(LevG64{}, g); });
+				with_geo<GeoC, MG_NGEO>(W.geo, [&](auto g) { up(LevG64{}, g); });
 			if (B.prof)
 				B.prof(g_up_name[l], 0, (l > MG_G32_MAX ? 21.0 : 17.0) * W.cells, s);
 		}
 	}
 	if (B.prof) B.prof("mg_up0", 1, 0, s);
-	with_geo<GeoF>(B.geo0, [&](auto g) {
+	with_geo<GeoF, MG_NFINE>(B.geo0, [&](auto g) {
 		using G = decltype(g);
 		k_mg_up<Lev0, G, true><<<B.ntiles0, G::T, 0, s>>>(B.tiles0, B.mg[0].lv,
 				B.mg[0].coarse, B.scal, B.wc, B.tpart);

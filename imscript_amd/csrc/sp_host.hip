@@ -112,18 +112,37 @@ static const int g_mg_geo_env = [] {
 	return 0;
 }();
 
-// default tile shapes: 120x56 fine tiles with 1024 threads (two 2x2
-// blocks per thread); at coarse levels 112x48 with 1024 threads while the
-// level (all planes of the batch) has 2^19 cells or more, below that 48x16
-// tiles (many short tiles: those levels are bound by a tile's latency, not
-// by halo bytes).  Measured on 4096x4096x3 (per V-cycle, down + up): level
-// 0 338 us against 440 with 512 threads, level 2 50 us against 107 with
-// 112x32 / 256 threads, levels 4 and 5 16 and 15 us against 39 and 57.
+// smoothing of the per-level kernels and the tail, sweeps at the coarsest
+// level, coarse-correction factor; SIMPOIS_HIP_MG_TAIL="nu,sweeps,wc" sets them
+static int g_mg_nu = MG_NU_TAIL, g_mg_csw = 4;
+static float g_mg_wc = 1.8f;
+static const int g_mg_tail_env = [] {
+	const char *e = getenv("SIMPOIS_HIP_MG_TAIL");
+	int nu, csw;
+	float wc;
+	if (e && sscanf(e, "%d,%d,%f", &nu, &csw, &wc) == 3 && nu >= 1 && nu <= 16
+			&& csw >= 1 && csw <= 256 && wc > 0.0f && wc < 4.0f) {
+		g_mg_nu = nu;
+		g_mg_csw = csw;
+		g_mg_wc = wc;
+	}
+	return 0;
+}();
+
+// default tile shapes.  The fused kernels move their region through HBM
+// once per launch whatever the number of sweeps, so more smoothing there
+// costs compute only and saves V-cycles: level 0 smooths nu = 5 times
+// (108x44 tiles), level 1 nu = 4 (112x48), the coarse levels nu = 6, on
+// 104x40 tiles with 1024 threads while the level (all planes of the batch)
+// has 2^19 cells or more, below that on 40x8 tiles (many short tiles: those
+// levels are bound by a tile's latency, not by halo bytes).  Measured on
+// 4096x4096x3 to 5e-6: 8 CG iterations, 10.2 ms, against 11 iterations and
+// 11.6 ms with nu = 2, 2, 4.
 static int mg_geo_of(int l, double cells)
 {
 	if (g_mg_geo[l] >= 0) return g_mg_geo[l];
-	if (l < 2) return 1;
-	return cells >= (double)(1 << 19) ? 2 : 3;
+	if (l < 2) return l == 0 ? 4 : 3;
+	return cells >= (double)(1 << 19) ? 4 : 5;
 }
 
 static int ensure_device()
@@ -947,9 +966,9 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 	TRY(pl.alloc(&B.tpart, tiles0.size()));
 	B.mg = mgw.data();
 	B.nlev = nlev;
-	B.nu = MG_NU_FINE;
-	B.coarse_sweeps = 4;
-	B.wc = 1.8f;
+	B.nu = g_mg_nu;
+	B.coarse_sweeps = g_mg_csw;
+	B.wc = g_mg_wc;
 	return SP_OK;
 }
 
@@ -1063,10 +1082,14 @@ static int run_cg32(Plan &pl, int k, int bilap, float eps, hipStream_t s)
 	B.prof = prof ? prof_fn : nullptr;
 	spk::cg32_setup(B, s);
 	long launched = 0;
-	for (long done = 0; done < C.maxit_host; done += C.chunk) {
-		if (prof) for (int z = 0; z < C.chunk; z++) spk::cg32_iter(B, s, z & 1);
+	// profiling runs go one iteration at a time, so that no launch after
+	// convergence (an empty one) enters the per-kernel averages
+	for (long done = 0; done < C.maxit_host;) {
+		const int step = prof ? 1 : C.chunk;
+		if (prof) spk::cg32_iter(B, s, (int)(done & 1));
 		else HIPCHK(hipGraphLaunch(C.exec, s));
-		launched += C.chunk;
+		done += step;
+		launched += step;
 		HIPCHK(hipMemcpyAsync(pl.h_done, B.nactive, sizeof(int), hipMemcpyDeviceToHost, s));
 		HIPCHK(hipStreamSynchronize(s));
 		if (pl.h_done[0] == 0) break;

@@ -309,12 +309,16 @@ def test_multigrid_tail_in_lds_and_in_global_memory():
 
 
 def _mg_child(h, w, seed, geo, tmp_path):
+    """Solve in a child process with SIMPOIS_HIP_MG_GEO = geo (the tile
+    shape of levels 0..7, read once when the library loads; None: the
+    default rule)."""
     env = dict(os.environ)
     if geo is None:
         env.pop("SIMPOIS_HIP_MG_GEO", None)
     else:
-        env["SIMPOIS_HIP_MG_GEO"] = ",".join([str(geo)] * 8)
-    out = tmp_path / f"mg_{h}x{w}_{geo}.npz"
+        env["SIMPOIS_HIP_MG_GEO"] = geo
+    tag = "default" if geo is None else geo.replace(",", "")
+    out = tmp_path / f"mg_{h}x{w}_{tag}.npz"
     child = os.path.join(os.path.dirname(__file__), "mg_geo_child.py")
     print(f"mg child {h}x{w} geo={geo}", flush=True)
     p = subprocess.run([sys.executable, child, str(h), str(w), str(seed), str(out)],
@@ -338,15 +342,22 @@ def test_multigrid_fused_coarse_levels(h, w, tmp_path):
     assert int(r["iters"]) <= 20, int(r["iters"])
 
 
-def test_multigrid_tile_shapes_give_the_same_bits(tmp_path):
-    """Every tile shape computes the same red-black sweeps (a tile
+@pytest.mark.parametrize("shapes", [
+    # nu = 2 at levels 0-1, 4 below: 512- and 1024-thread fine tiles, all
+    # four coarse shapes
+    ("1,1,0,0,0,0,0,0", "0,0,1,1,1,1,1,1", "1,0,2,2,2,2,2,2", "0,1,3,3,3,3,3,3"),
+    # the default rule (nu = 5, 4, then 6) against each nu = 6 coarse shape
+    (None, "4,3,4,4,4,4,4,4", "4,3,5,5,5,5,5,5"),
+])
+def test_multigrid_tile_shapes_give_the_same_bits(shapes, tmp_path):
+    """Tile shapes of the same nu compute the same red-black sweeps (a tile
     recomputes its halo, so its interior is exact whatever its size): the
     solutions are bit-identical across shapes, at every fused level (0..5
     here), and across processes (a tile reads its halo from a buffer that
     no tile of the same launch writes)."""
-    base = _mg_child(2600, 2700, 9, None, tmp_path)
+    base = _mg_child(2600, 2700, 9, shapes[0], tmp_path)
     assert base["rel"].max() <= 1e-5, base["rel"]
-    for geo in (0, 1, 2, 3):
+    for geo in shapes[1:]:
         r = _mg_child(2600, 2700, 9, geo, tmp_path)
         assert int(r["iters"]) == int(base["iters"]), geo
         assert np.array_equal(r["u"].view(np.uint32), base["u"].view(np.uint32)), geo
