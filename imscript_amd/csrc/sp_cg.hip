@@ -40,6 +40,21 @@ static __device__ __forceinline__ int clampi(int v, int lo, int hi)
 	return v < lo ? lo : (v > hi ? hi : v);
 }
 
+// Work-groups go to the 8 XCDs in turn (work-group b to XCD b % 8), each
+// XCD with its own L2.  xcd_block() maps b to a logical block so that every
+// XCD takes a contiguous run of logical blocks, in order: neighbouring
+// tiles meet in the same L2 and share their halos (the multigrid tiles'
+// HBM traffic halves).  A bijection of [0, gridDim.x): results do not
+// depend on it.  Only for launches of uniform work per block: the CG
+// vector kernels' work follows the holes, and contiguous runs of it
+// unbalance the XCDs (measured slower there).
+static __device__ __forceinline__ int xcd_block()
+{
+	const int n = (int)gridDim.x, b = (int)blockIdx.x;
+	const int x = b & 7, q = n >> 3, r = n & 7;
+	return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 template <int N>
 static __device__ double bsum(double v, double *red)
 {
@@ -97,7 +112,8 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 		const Cg32Plane *pl, double *part)
 {
 	__shared__ double red[NT / 64];
-	const BlockWork b = bw[blockIdx.x];
+	const int blk = blockIdx.x;
+	const BlockWork b = bw[blk];
 	const Cg32Plane P = pl[b.plane];
 	double rr = 0, bb = 0;
 	const unsigned n = (unsigned)P.w * (unsigned)P.h, w = (unsigned)P.w;
@@ -201,8 +217,8 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 	}
 	const double s1 = bsum<NT>(rr, red), s2 = bsum<NT>(bb, red);
 	if (threadIdx.x == 0) {
-		part[2 * blockIdx.x] = s1;
-		part[2 * blockIdx.x + 1] = s2;
+		part[2 * blk] = s1;
+		part[2 * blk + 1] = s2;
 	}
 }
 
@@ -361,7 +377,8 @@ __global__ __launch_bounds__(NT) void k32_update(const BlockWork *bw,
 		const Cg32Plane *pl, const Cg32Scal *sc, double *part)
 {
 	__shared__ double red[NT / 64];
-	const BlockWork b = bw[blockIdx.x];
+	const int blk = blockIdx.x;
+	const BlockWork b = bw[blk];
 	const Cg32Scal &s = sc[b.plane];
 	if (!s.active) return;
 	const Cg32Plane P = pl[b.plane];
@@ -389,7 +406,7 @@ __global__ __launch_bounds__(NT) void k32_update(const BlockWork *bw,
 		}
 	}
 	const double t = bsum<NT>(acc, red);
-	if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = t;
+	if (threadIdx.x == 0) part[2 * blk + 1] = t;
 }
 
 // rr_new, absolute stop test on sqrt(rr_new), beta
@@ -475,7 +492,8 @@ __global__ __launch_bounds__(NT) void k32_dirspmv(const BlockWork *bw,
 		const Cg32Plane *pl, const Cg32Scal *sc, double *part, int par, int first)
 {
 	__shared__ double red[NT / 64];
-	const BlockWork b = bw[blockIdx.x];
+	const int blk = blockIdx.x;
+	const BlockWork b = bw[blk];
 	const Cg32Scal &s = sc[b.plane];
 	const bool upd = !first && s.upd, act = s.active;
 	if (!upd && !act) return;
@@ -574,7 +592,7 @@ __global__ __launch_bounds__(NT) void k32_dirspmv(const BlockWork *bw,
 		}
 	}
 	const double t = bsum<NT>(acc, red);
-	if (threadIdx.x == 0) part[2 * blockIdx.x] = t;
+	if (threadIdx.x == 0) part[2 * blk] = t;
 }
 
 __global__ __launch_bounds__(NT) void k32_finish(const BlockWork *bw,
@@ -906,15 +924,19 @@ struct Lev0 {
 	// unknown (known pixel, outside the image, the region's frame) holds
 	// z = 0 at all times, so adding all four equals adding the unknown ones
 	// (x + 0 = x): no masks needed.
-	static __device__ __forceinline__ float gather(unsigned, float s,
-			const float *z, int id, int zs, int de, int dw)
+	static __device__ __forceinline__ float gather4(unsigned, float s, float e,
+			float so, float w, float n)
 	{
-		const float e = z[id + de], so = z[id + zs], w = z[id + dw], n = z[id - zs];
 		s += e;
 		s += so;
 		s += w;
 		s += n;
 		return s;
+	}
+	static __device__ __forceinline__ float gather(unsigned k, float s,
+			const float *z, int id, int zs, int de, int dw)
+	{
+		return gather4(k, s, z[id + de], z[id + zs], z[id + dw], z[id - zs]);
 	}
 };
 
@@ -952,14 +974,19 @@ struct LevG {
 		const unsigned d = k & 2047u;
 		return d ? __builtin_amdgcn_rcpf((float)d) : 0.0f;
 	}
+	static __device__ __forceinline__ float gather4(unsigned k, float s, float e,
+			float so, float w, float n)
+	{
+		s += (float)((k >> 11) & 31u) * e;
+		s += (float)((k >> 16) & 31u) * so;
+		s += (float)((k >> 21) & 31u) * w;
+		s += (float)(k >> 26) * n;
+		return s;
+	}
 	static __device__ __forceinline__ float gather(unsigned k, float s,
 			const float *z, int id, int zs, int de, int dw)
 	{
-		s += (float)((k >> 11) & 31u) * z[id + de];
-		s += (float)((k >> 16) & 31u) * z[id + zs];
-		s += (float)((k >> 21) & 31u) * z[id + dw];
-		s += (float)(k >> 26) * z[id - zs];
-		return s;
+		return gather4(k, s, z[id + de], z[id + zs], z[id + dw], z[id - zs]);
 	}
 };
 
@@ -993,27 +1020,39 @@ struct LevG64 {
 		const float d = diag(k);
 		return d != 0.0f ? __builtin_amdgcn_rcpf(d) : 0.0f;
 	}
+	static __device__ __forceinline__ float gather4(Code k, float s, float e,
+			float so, float w, float n)
+	{
+		s += (float)(k.x & 255u) * e;
+		s += (float)((k.x >> 8) & 255u) * so;
+		s += (float)((k.x >> 16) & 255u) * w;
+		s += (float)(k.x >> 24) * n;
+		return s;
+	}
 	static __device__ __forceinline__ float gather(Code k, float s,
 			const float *z, int id, int zs, int de, int dw)
 	{
-		s += (float)(k.x & 255u) * z[id + de];
-		s += (float)((k.x >> 8) & 255u) * z[id + zs];
-		s += (float)((k.x >> 16) & 255u) * z[id + dw];
-		s += (float)(k.x >> 24) * z[id - zs];
-		return s;
+		return gather4(k, s, z[id + de], z[id + zs], z[id + dw], z[id - zs]);
 	}
 };
 
-// per-thread cells: 2x2 blocks, cell c of block kb at (2m + (c>>1), 2pc + (c&1)).
-// With few cells per thread, 1/diagonal stays in registers (rd) instead of
-// being decoded again in every half sweep.
+// per-thread cells: K 2x2 blocks stacked in a 2 x 2K strip, block kb at
+// block row m = m0 K + kb, cell c at (2m + (c>>1), 2pc + (c&1)).  With few
+// cells per thread (RD), 1/diagonal and the thread's own z stay in
+// registers: a half sweep then reads from LDS only the neighbours that
+// other threads own (1.5 per cell instead of 3) and writes its new values
+// to both.
 template <class P, class G> struct MgCells {
 	static constexpr bool RD = G::K <= 4;
 	float b[G::K][4];
 	float rd[RD ? G::K : 1][4];
+	float zr[RD ? G::K : 1][4];
 	unsigned k[G::K][P::CW];
 	int pc, m0;
 };
+
+template <class G>
+static __device__ __forceinline__ int brow(int m0, int kb) { return m0 * G::K + kb; }
 
 template <class P, class G>
 static __device__ __forceinline__ float cell_rdiag(const MgCells<P, G> &C, int kb,
@@ -1026,23 +1065,24 @@ static __device__ __forceinline__ float cell_rdiag(const MgCells<P, G> &C, int k
 template <class P, class G>
 static __device__ __forceinline__ int cell_z(const MgCells<P, G> &C, int kb, int c)
 {
-	return G::zid(2 * (C.m0 + G::RG * kb) + (c >> 1), 2 * C.pc + (c & 1));
+	return G::zid(2 * brow<G>(C.m0, kb) + (c >> 1), 2 * C.pc + (c & 1));
 }
 
 // Region geometry of one thread: its columns i0, i0+1 and the rows
-// j0 + 2*RG*kb + (0, 1); q0 = j0 * w + i0 (may lie outside the image).
+// j0 + 2*kb + (0, 1); q0 = j0 * w + i0 (may lie outside the image).
 struct MgSpan {
 	int i0, j0, w, h;
 	long q0;
 	bool colok[2];
 };
 
+template <class G>
 static __device__ __forceinline__ MgSpan mg_span(const MgLevel &L, int ox, int oy,
 		int pc, int m0)
 {
 	MgSpan S;
 	S.i0 = ox + 2 * pc;
-	S.j0 = oy + 2 * m0;
+	S.j0 = oy + 2 * brow<G>(m0, 0);
 	S.w = L.w;
 	S.h = L.h;
 	S.q0 = (long)S.j0 * L.w + S.i0;
@@ -1054,14 +1094,14 @@ static __device__ __forceinline__ MgSpan mg_span(const MgLevel &L, int ox, int o
 template <class G>
 static __device__ __forceinline__ bool row_ok(const MgSpan &S, int kb, int hr)
 {
-	const int j = S.j0 + 2 * G::RG * kb + hr;
+	const int j = S.j0 + 2 * kb + hr;
 	return j >= 0 && j < S.h;
 }
 
 template <class G>
 static __device__ __forceinline__ long row_q(const MgSpan &S, int kb, int hr)
 {
-	return S.q0 + (long)(2 * G::RG * kb + hr) * S.w;
+	return S.q0 + (long)(2 * kb + hr) * S.w;
 }
 
 // zero frame of the region's z: the rows above and below, and the frame
@@ -1085,7 +1125,7 @@ static __device__ void mg_load(const MgLevel &L, int ox, int oy, MgCells<P, G> &
 {
 	C.pc = threadIdx.x % G::PAIRS_X;
 	C.m0 = threadIdx.x / G::PAIRS_X;
-	const MgSpan S = mg_span(L, ox, oy, C.pc, C.m0);
+	const MgSpan S = mg_span<G>(L, ox, oy, C.pc, C.m0);
 	const typename P::Raw *code = P::codes(L);
 	const bool pair = !(L.w & 1);
 #pragma unroll
@@ -1122,7 +1162,10 @@ static __device__ void mg_load(const MgLevel &L, int ox, int oy, MgCells<P, G> &
 				const int c = 2 * hr + hc;
 				C.b[kb][c] = bv[hc];
 				P::set(C.k[kb], c, kv[hc]);
-				if constexpr (MgCells<P, G>::RD) C.rd[kb][c] = P::rdiag(kv[hc]);
+				if constexpr (MgCells<P, G>::RD) {
+					C.rd[kb][c] = P::rdiag(kv[hc]);
+					C.zr[kb][c] = zv[hc];
+				}
 				z[cell_z(C, kb, c)] = zv[hc];
 			}
 		}
@@ -1133,7 +1176,7 @@ template <class P, class G>
 static __device__ __forceinline__ void mg_store_z(float *dst, const MgLevel &L,
 		int ox, int oy, const MgCells<P, G> &C, const float *z, int kb)
 {
-	const MgSpan S = mg_span(L, ox, oy, C.pc, C.m0);
+	const MgSpan S = mg_span<G>(L, ox, oy, C.pc, C.m0);
 #pragma unroll
 	for (int hr = 0; hr < 2; hr++) {
 		if (!row_ok<G>(S, kb, hr)) continue;
@@ -1148,14 +1191,15 @@ static __device__ __forceinline__ void mg_store_z(float *dst, const MgLevel &L,
 	}
 }
 
-// red-black half sweep over the thread's cells of colour `color`: all
+// red-black half sweep over the thread's cells of colour COLOR: all
 // neighbour loads first (cells of one colour do not depend on each other).
 // An inactive cell (known pixel, empty aggregate, outside the image) has
 // b = 0, no couplings and 1/diagonal = 0, so it keeps z = 0 without a test.
-template <class P, class G>
-static __device__ __forceinline__ void mg_half(const MgCells<P, G> &C, float *z,
-		int color, bool first)
+// Neighbours are summed E, S, W, N whether they come from registers or LDS.
+template <int COLOR, class P, class G>
+static __device__ __forceinline__ void mg_half(MgCells<P, G> &C, float *z, bool first)
 {
+	constexpr bool RD = MgCells<P, G>::RD;
 	float nz[G::K][2];
 #pragma unroll
 	for (int kb = 0; kb < G::K; kb++)
@@ -1163,19 +1207,52 @@ static __device__ __forceinline__ void mg_half(const MgCells<P, G> &C, float *z,
 		for (int h = 0; h < 2; h++) {
 			// colour of cell c is ((c>>1) + (c&1) + pair origin parity) & 1;
 			// origins are even, so cells 0,3 have colour 0 and 1,2 colour 1
-			const int c = color == 0 ? (h ? 3 : 0) : (h ? 2 : 1);
+			const int c = COLOR == 0 ? (h ? 3 : 0) : (h ? 2 : 1);
 			typename P::Code k = P::zero();
-			if constexpr (P::NEEDK || !MgCells<P, G>::RD) k = P::fresh(P::get(C.k[kb], c));
+			if constexpr (P::NEEDK || !RD) k = P::fresh(P::get(C.k[kb], c));
 			float s = C.b[kb][c];
-			if (!first)
-				s = P::gather(k, s, z, cell_z(C, kb, c), G::ZS, G::de(c & 1), G::dw(c & 1));
+			if (!first) {
+				const int id = cell_z(C, kb, c);
+				if constexpr (RD) {
+					// own cells from registers: the block's other colour, and
+					// the blocks above (kb - 1) and below (kb + 1) in the strip
+					float e, so, w, n;
+					if (c == 0) {
+						e = C.zr[kb][1];
+						so = C.zr[kb][2];
+						w = z[id + G::dw(0)];
+						n = kb > 0 ? C.zr[kb > 0 ? kb - 1 : 0][2] : z[id - G::ZS];
+					} else if (c == 3) {
+						e = z[id + G::de(1)];
+						so = kb + 1 < G::K ? C.zr[kb + 1 < G::K ? kb + 1 : kb][1]
+							: z[id + G::ZS];
+						w = C.zr[kb][2];
+						n = C.zr[kb][1];
+					} else if (c == 1) {
+						e = z[id + G::de(1)];
+						so = C.zr[kb][3];
+						w = C.zr[kb][0];
+						n = kb > 0 ? C.zr[kb > 0 ? kb - 1 : 0][3] : z[id - G::ZS];
+					} else {
+						e = C.zr[kb][3];
+						so = kb + 1 < G::K ? C.zr[kb + 1 < G::K ? kb + 1 : kb][0]
+							: z[id + G::ZS];
+						w = z[id + G::dw(0)];
+						n = C.zr[kb][0];
+					}
+					s = P::gather4(k, s, e, so, w, n);
+				} else {
+					s = P::gather(k, s, z, id, G::ZS, G::de(c & 1), G::dw(c & 1));
+				}
+			}
 			nz[kb][h] = s * cell_rdiag(C, kb, c, k);
 		}
 #pragma unroll
 	for (int kb = 0; kb < G::K; kb++)
 #pragma unroll
 		for (int h = 0; h < 2; h++) {
-			const int c = color == 0 ? (h ? 3 : 0) : (h ? 2 : 1);
+			const int c = COLOR == 0 ? (h ? 3 : 0) : (h ? 2 : 1);
+			if constexpr (RD) C.zr[kb][c] = nz[kb][h];
 			z[cell_z(C, kb, c)] = nz[kb][h];
 		}
 }
@@ -1183,7 +1260,7 @@ static __device__ __forceinline__ void mg_half(const MgCells<P, G> &C, float *z,
 template <class P, class G>
 static __device__ __forceinline__ bool in_tile(const MgCells<P, G> &C, int kb)
 {
-	const int m = C.m0 + G::RG * kb;
+	const int m = brow<G>(C.m0, kb);
 	return C.pc >= G::HALO / 2 && C.pc < G::HALO / 2 + G::TX / 2
 		&& m >= G::HALO / 2 && m < G::HALO / 2 + G::TY / 2;
 }
@@ -1196,7 +1273,7 @@ void k_mg_down(const int4 *tiles,
 		const MgLevel *lv, const MgLevel *coarse, const Cg32Scal *sc)
 {
 	__shared__ float z[G::ZN];
-	const int4 t = tiles[blockIdx.x];
+	const int4 t = tiles[xcd_block()];
 	if (!sc[t.x].active) return;
 	const MgLevel L = lv[t.x], Cc = coarse[t.x];
 	const int ox = t.y - G::HALO, oy = t.z - G::HALO;
@@ -1204,8 +1281,10 @@ void k_mg_down(const int4 *tiles,
 	z_frame<G>(z);
 	mg_load(L, ox, oy, C, z, nullptr);
 	__syncthreads();
-	for (int s = 0; s < 2 * G::NU; s++) {
-		mg_half(C, z, s & 1, s == 0);
+	for (int s = 0; s < G::NU; s++) {
+		mg_half<0>(C, z, s == 0);
+		__syncthreads();
+		mg_half<1>(C, z, false);
 		__syncthreads();
 	}
 #pragma unroll
@@ -1223,7 +1302,7 @@ void k_mg_down(const int4 *tiles,
 				P::gather(k, 0.0f, z, id, G::ZS, G::de(c & 1), G::dw(c & 1));
 			acc += P::mask(k) * (C.b[kb][c] - az);
 		}
-		const int I = (ox >> 1) + C.pc, J = (oy >> 1) + C.m0 + G::RG * kb;
+		const int I = (ox >> 1) + C.pc, J = (oy >> 1) + brow<G>(C.m0, kb);
 		if (I < Cc.w && J < Cc.h) {
 			const size_t cq = (size_t)J * Cc.w + I;
 			Cc.b[cq] = acc;
@@ -1243,7 +1322,8 @@ void k_mg_up(const int4 *tiles, const MgLevel *lv,
 {
 	__shared__ float z[G::ZN];
 	__shared__ double red[G::T / 64];
-	const int4 t = tiles[blockIdx.x];
+	const int tb = xcd_block();
+	const int4 t = tiles[tb];
 	if (!sc[t.x].active) return;
 	const MgLevel L = lv[t.x], Cc = coarse[t.x];
 	const int ox = t.y - G::HALO, oy = t.z - G::HALO;
@@ -1253,16 +1333,22 @@ void k_mg_up(const int4 *tiles, const MgLevel *lv,
 	// every block of the region is one coarse cell: add its correction
 #pragma unroll
 	for (int kb = 0; kb < G::K; kb++) {
-		const int I = (ox >> 1) + C.pc, J = (oy >> 1) + C.m0 + G::RG * kb;
+		const int I = (ox >> 1) + C.pc, J = (oy >> 1) + brow<G>(C.m0, kb);
 		if (I < 0 || J < 0 || I >= Cc.w || J >= Cc.h) continue;
 		const float e = wc * Cc.z[(size_t)J * Cc.w + I];
 #pragma unroll
-		for (int c = 0; c < 4; c++)
-			z[cell_z(C, kb, c)] += e * P::mask(P::get(C.k[kb], c));
+		for (int c = 0; c < 4; c++) {
+			const int id = cell_z(C, kb, c);
+			const float v = z[id] + e * P::mask(P::get(C.k[kb], c));
+			if constexpr (MgCells<P, G>::RD) C.zr[kb][c] = v;
+			z[id] = v;
+		}
 	}
 	__syncthreads();
-	for (int s = 0; s < 2 * G::NU; s++) {
-		mg_half(C, z, (s & 1) ^ 1, false);
+	for (int s = 0; s < G::NU; s++) {
+		mg_half<1>(C, z, false);
+		__syncthreads();
+		mg_half<0>(C, z, false);
 		__syncthreads();
 	}
 	double acc = 0;
@@ -1278,7 +1364,7 @@ void k_mg_up(const int4 *tiles, const MgLevel *lv,
 	}
 	if (DOT) {
 		const double sum = bsum<G::T>(acc, red);
-		if (threadIdx.x == 0) tpart[blockIdx.x] = sum;
+		if (threadIdx.x == 0) tpart[tb] = sum;
 	}
 }
 
