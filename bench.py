@@ -41,8 +41,9 @@ PEAK_HBM_GBPS = 8000.0
 # counter reads for k_stream_copy moving 1 GiB each way in that run (FETCH_SIZE
 # x2.000, WRITE_SIZE x1.000).  Summary and commands: profiles/hbm_traffic_*.md.
 TRAFFIC_PMC = {
-    "mg_up0": 869.7e6,
-    "mg_down0": 624.2e6,
+    "mg_up0": 644.0e6,
+    "mg_down0": 495.5e6,
+    "k32_dirspmv": 1008.7e6,
 }
 
 
@@ -1007,8 +1008,8 @@ def main():
         achieved = by / (ms * 1e-3) / 1e9
         result["roofline"] = {
             "kernel": {"mg_up0": "k_mg_up0 (fused level-0 coarse correction + "
-                                 "4 red-black half sweeps + r.z)",
-                       "mg_down0": "k_mg_down0 (fused level-0 4 red-black half "
+                                 "10 red-black half sweeps + r.z)",
+                       "mg_down0": "k_mg_down0 (fused level-0 10 red-black half "
                                    "sweeps + residual + restriction)",
                        "k32_dirspmv": "k32_dirspmv (fused CG direction + "
                                       "operator product + p.q)"}[dom],
