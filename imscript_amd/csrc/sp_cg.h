@@ -32,6 +32,8 @@ struct Cg32Scal {
 	int trace0, trace_cap;  // slots of this plane in the rr trace
 	int upd;           // this iteration updated r: the direction kernel
 	                   // adds alpha p to x before it replaces p
+	int fin;           // this iteration stopped the plane: the fused
+	                   // direction kernel writes the solution to u
 };
 
 // Fused multigrid tile kernels: levels 0 and 1 run one of the fine tile

@@ -107,7 +107,54 @@ static __device__ __forceinline__ long pix(const BlockWork &b, int k, size_t n)
 	return q < n ? (long)q : -1;
 }
 
-// b, r0 = b - A x0 (x0 = u at holes), p = r, x = x0; partial r.r and b.b
+// Level-0 code bytes of the pixels q4 .. q1-1 (at most 4, from the NaNs of
+// g), packed in one word (pixel q4 + k in byte k).  A group inside one row
+// of a 16-byte aligned plane (w a multiple of 4) reads its row and the rows
+// above and below as one 16-byte load each.
+static __device__ __forceinline__ unsigned group_code(const Cg32Plane &P, unsigned q4,
+		unsigned q1)
+{
+	const unsigned w = (unsigned)P.w, h = (unsigned)P.h;
+	unsigned j = q4 / w, i = q4 - j * w, word = 0;
+	if (q1 - q4 == 4 && !(w & 3) && !((uintptr_t)P.g & 15)) {
+		const f4 c = *(const f4 *)(P.g + q4);
+		const f4 up = j > 0 ? *(const f4 *)(P.g + q4 - w) : c;
+		const f4 dn = j + 1 < h ? *(const f4 *)(P.g + q4 + w) : c;
+		const float lf = i > 0 ? P.g[q4 - 1] : 0.0f;
+		const float rt = i + 4 < w ? P.g[q4 + 4] : 0.0f;
+		const unsigned nrow = (j > 0) + (j + 1 < h);
+#pragma unroll
+		for (int k = 0; k < 4; k++) {
+			if (!isnan(c[k])) continue;
+			const unsigned ii = i + k;
+			unsigned v = 1;
+			if (ii + 1 < w && isnan(k < 3 ? c[k + 1] : rt)) v |= 2;
+			if (j + 1 < h && isnan(dn[k])) v |= 4;
+			if (ii > 0 && isnan(k > 0 ? c[k - 1] : lf)) v |= 8;
+			if (j > 0 && isnan(up[k])) v |= 16;
+			v |= ((ii > 0) + (ii + 1 < w) + nrow) << 5;
+			word |= v << (8 * k);
+		}
+		return word;
+	}
+	for (unsigned q = q4; q < q1; q++) {
+		unsigned c = 0;
+		if (isnan(P.g[q])) {
+			c = 1;
+			if (i + 1 < w && isnan(P.g[q + 1])) c |= 2;
+			if (j + 1 < h && isnan(P.g[q + w])) c |= 4;
+			if (i > 0 && isnan(P.g[q - 1])) c |= 8;
+			if (j > 0 && isnan(P.g[q - w])) c |= 16;
+			c |= ((i > 0) + (i + 1 < w) + (j > 0) + (j + 1 < h)) << 5;
+		}
+		word |= c << (8 * (q - q4));
+		if (++i == w) { i = 0; j++; }
+	}
+	return word;
+}
+
+// code bytes (above), b, r0 = b - A x0 (x0 = u at holes), x = x0; partial
+// r.r and b.b
 __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 		const Cg32Plane *pl, double *part)
 {
@@ -120,7 +167,7 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 	const unsigned q4 = (unsigned)b.start + 4 * threadIdx.x;
 	const unsigned q1 = q4 + 4 < n ? q4 + 4 : n;
 	const bool whole = q4 + 3 < n;
-	const unsigned cc = whole ? code4(P.code, q4) : 0u;
+	const unsigned cc = q4 < n ? group_code(P, q4, q1) : 0u;
 	f4 rv = {0.0f, 0.0f, 0.0f, 0.0f};
 	d4 xv = {0.0, 0.0, 0.0, 0.0};
 	const float *g = P.g, *u = P.u;
@@ -163,7 +210,8 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 		unsigned j = q4 / w, i = q4 - j * w;
 		for (unsigned q = q4; q < q1; q++) {
 			const unsigned k = q - q4;
-			const bool hole = whole ? ((cc >> (8 * k)) & 1u) : (P.code[q] & 1u);
+			const bool hole = (cc >> (8 * k)) & 1u;
+			if (!whole) P.code[q] = (unsigned char)(cc >> (8 * k));
 			if (hole) {
 				float a0, ax;
 				if (!P.bilap) {
@@ -208,6 +256,7 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 		// biharmonic iteration starts from p = r, and its product reads p
 		// at every pixel of the 13-point stencil, unknown or not: p and q
 		// everywhere.  (The Poisson iteration writes p and q itself.)
+		*(unsigned *)(P.code + q4) = cc;
 		*(f4 *)(P.r + q4) = rv;
 		if (P.bilap) {
 			*(f4 *)(P.p + q4) = rv;
@@ -351,7 +400,7 @@ __global__ __launch_bounds__(NT) void k32_reduce_a(const Cg32Plane *pl, Cg32Scal
 	__shared__ double red[NT / 64];
 	__shared__ double pq;
 	if (!sc[blockIdx.x].active) {
-		if (threadIdx.x == 0) sc[blockIdx.x].upd = 0;
+		if (threadIdx.x == 0) sc[blockIdx.x].upd = sc[blockIdx.x].fin = 0;
 		return;
 	}
 	fold(part, pl[blockIdx.x], 2, 0, red, &pq);
@@ -362,6 +411,7 @@ __global__ __launch_bounds__(NT) void k32_reduce_a(const Cg32Plane *pl, Cg32Scal
 		if (!(pq > 0) || !isfinite(s.rz / pq)) {
 			s.alpha = 0.0;
 			s.upd = 0;
+			s.fin = 1;
 			s.active = 0;
 			atomicSub(nactive, 1);
 		} else {
@@ -428,6 +478,7 @@ __global__ __launch_bounds__(NT) void k32_reduce_b(const Cg32Plane *pl, Cg32Scal
 		s.rr = rr1;
 		if (sqrt(rr1) < s.stop || s.iters >= s.maxit) {
 			s.active = 0;
+			s.fin = 1;
 			atomicSub(nactive, 1);
 		}
 	}
@@ -495,8 +546,8 @@ __global__ __launch_bounds__(NT) void k32_dirspmv(const BlockWork *bw,
 	const int blk = blockIdx.x;
 	const BlockWork b = bw[blk];
 	const Cg32Scal &s = sc[b.plane];
-	const bool upd = !first && s.upd, act = s.active;
-	if (!upd && !act) return;
+	const bool upd = !first && s.upd, act = s.active, fin = !first && s.fin;
+	if (!upd && !act && !fin) return;
 	const Cg32Plane P = pl[b.plane];
 	const float *po = par ? P.p2 : P.p;
 	float *pw_ = par ? P.p : P.p2;
@@ -511,12 +562,20 @@ __global__ __launch_bounds__(NT) void k32_dirspmv(const BlockWork *bw,
 		if (cc & 0x01010101u) {
 			const f4 z0 = {0.0f, 0.0f, 0.0f, 0.0f};
 			const f4 poc = fst ? z0 : *(const f4 *)(po + q4);
-			if (upd) {
+			if (upd || fin) {
 				d4 x = *(const d4 *)(P.x + q4);
+				if (upd) {
 #pragma unroll
-				for (int k = 0; k < 4; k++)
-					if ((cc >> (8 * k)) & 1u) x[k] = x[k] + ad * (double)poc[k];
-				*(d4 *)(P.x + q4) = x;
+					for (int k = 0; k < 4; k++)
+						if ((cc >> (8 * k)) & 1u) x[k] = x[k] + ad * (double)poc[k];
+					*(d4 *)(P.x + q4) = x;
+				}
+				// the solution: u may be the caller's buffer, 4-byte stores
+				// at the unknowns only
+				if (fin)
+#pragma unroll
+					for (int k = 0; k < 4; k++)
+						if ((cc >> (8 * k)) & 1u) P.u[q4 + k] = (float)x[k];
 			}
 			if (act) {
 				const f4 zc = *(const f4 *)(P.z + q4);
@@ -578,6 +637,7 @@ __global__ __launch_bounds__(NT) void k32_dirspmv(const BlockWork *bw,
 			}
 			const float o = fst ? 0.0f : po[q];
 			if (upd) P.x[q] = P.x[q] + ad * (double)o;
+			if (fin) P.u[q] = (float)P.x[q];
 			if (!act) continue;
 			const float pc = pnew(P.z[q], o, be, fst);
 			float nb = 0.0f;
@@ -688,6 +748,108 @@ __global__ __launch_bounds__(NT) void k_mg_coarsen(const BlockWork *bw,
 		C.wE[q] = I + 1 < C.w ? e : 0.0f;
 		C.wS[q] = J + 1 < C.h ? sth : 0.0f;
 		C.z[q] = 0.0f;
+	}
+}
+
+// Level 0 -> 1 in one pass: the Galerkin coefficients of level 1 straight
+// from the code bytes, and their packed words (what k_mg_coarsen and
+// k_mg_gcode give, in the same integers).  Coarse cell (I, J) aggregates
+// fine cells c0 = (2I, 2J), c1 = (2I+1, 2J), c2 = (2I, 2J+1), c3 = (2I+1,
+// 2J+1): D = sum of their diagonals - 2 (E(c0) + S(c0) + S(c1) + E(c2)),
+// wE = E(c1) + E(c3), wS = S(c2) + S(c3), with E, S the unknown-neighbour
+// bits (0 at the image border).
+struct Coarse0 { unsigned d, e, s; };
+
+static __device__ __forceinline__ Coarse0 coarse0_of(unsigned c0, unsigned c1,
+		unsigned c2, unsigned c3)
+{
+	auto E = [](unsigned c) { return (c >> 1) & 1u; };
+	auto S = [](unsigned c) { return (c >> 2) & 1u; };
+	Coarse0 r;
+	r.d = (c0 >> 5) + (c1 >> 5) + (c2 >> 5) + (c3 >> 5)
+		- 2u * (E(c0) + S(c0) + S(c1) + E(c2));
+	r.e = E(c1) + E(c3);
+	r.s = S(c2) + S(c3);
+	return r;
+}
+
+// fine code byte at (i, j), 0 outside the fine level
+static __device__ __forceinline__ unsigned code_at(const MgLevel &F, int i, int j)
+{
+	return i >= 0 && j >= 0 && i < F.w && j < F.h ? F.code[(size_t)j * F.w + i] : 0u;
+}
+
+static __device__ __forceinline__ Coarse0 coarse0_cell(const MgLevel &F, int I, int J)
+{
+	return coarse0_of(code_at(F, 2 * I, 2 * J), code_at(F, 2 * I + 1, 2 * J),
+			code_at(F, 2 * I, 2 * J + 1), code_at(F, 2 * I + 1, 2 * J + 1));
+}
+
+static __device__ __forceinline__ void coarse0_store(const MgLevel &C, size_t q,
+		int I, int J, Coarse0 v, unsigned we, unsigned no)
+{
+	const unsigned e = I + 1 < C.w ? v.e : 0u, so = J + 1 < C.h ? v.s : 0u;
+	C.D[q] = (float)v.d;
+	C.wE[q] = (float)e;
+	C.wS[q] = (float)so;
+	C.z[q] = 0.0f;
+	C.gcode[q] = v.d | e << 11 | so << 16 | we << 21 | no << 26;
+}
+
+__global__ __launch_bounds__(NT) void k_mg_coarsen0(const BlockWork *bw,
+		const MgLevel *fine, const MgLevel *coarse)
+{
+	const BlockWork b = bw[blockIdx.x];
+	const MgLevel F = fine[b.plane], C = coarse[b.plane];
+	const unsigned n = (unsigned)C.w * (unsigned)C.h, cw = (unsigned)C.w;
+	const unsigned q4 = (unsigned)b.start + 4 * threadIdx.x;
+	if (q4 >= n) return;
+	const int J = (int)(q4 / cw), I = (int)(q4 - (unsigned)J * cw);
+	if (!(cw & 3) && F.w == 2 * C.w) {
+		// four cells of one row: 8 fine columns of fine rows 2J - 1 .. 2J + 1
+		// as one 8-byte load each (the fine width is a multiple of 8)
+		const unsigned char *r0 = F.code + (size_t)(2 * J) * F.w + 2 * I;
+		const uint2 a = *(const uint2 *)r0;
+		const uint2 c = 2 * J + 1 < F.h ? *(const uint2 *)(r0 + F.w) : make_uint2(0u, 0u);
+		const uint2 u = J > 0 ? *(const uint2 *)(r0 - F.w) : make_uint2(0u, 0u);
+		auto byte = [](uint2 v, int k) { return ((k < 4 ? v.x : v.y) >> (8 * (k & 3))) & 255u; };
+		// east coupling of the west neighbour (I - 1, J): fine column 2I - 1
+		unsigned we = 0;
+		if (I > 0) {
+			const unsigned w0 = r0[-1], w1 = 2 * J + 1 < F.h ? r0[F.w - 1] : 0u;
+			we = ((w0 >> 1) & 1u) + ((w1 >> 1) & 1u);
+		}
+		float4 D, E, S;
+		uint4 G;
+#pragma unroll
+		for (int k = 0; k < 4; k++) {
+			const Coarse0 v = coarse0_of(byte(a, 2 * k), byte(a, 2 * k + 1), byte(c, 2 * k),
+					byte(c, 2 * k + 1));
+			const unsigned e = I + k + 1 < C.w ? v.e : 0u, so = J + 1 < C.h ? v.s : 0u;
+			// south coupling of the north neighbour: fine row 2J - 1
+			const unsigned no = ((byte(u, 2 * k) >> 2) & 1u) + ((byte(u, 2 * k + 1) >> 2) & 1u);
+			const unsigned g = v.d | e << 11 | so << 16 | we << 21 | no << 26;
+			(&D.x)[k] = (float)v.d;
+			(&E.x)[k] = (float)e;
+			(&S.x)[k] = (float)so;
+			(&G.x)[k] = g;
+			we = e;
+		}
+		*(float4 *)(C.D + q4) = D;
+		*(float4 *)(C.wE + q4) = E;
+		*(float4 *)(C.wS + q4) = S;
+		*(float4 *)(C.z + q4) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		*(uint4 *)(C.gcode + q4) = G;
+		return;
+	}
+	const unsigned q1 = q4 + 4 < n ? q4 + 4 : n;
+	int i = I, j = J;
+	for (unsigned q = q4; q < q1; q++) {
+		const Coarse0 v = coarse0_cell(F, i, j);
+		const unsigned we = i > 0 ? coarse0_cell(F, i - 1, j).e : 0u;
+		const unsigned no = j > 0 ? coarse0_cell(F, i, j - 1).s : 0u;
+		coarse0_store(C, q, i, j, v, we, no);
+		if (++i == C.w) { i = 0; j++; }
 	}
 }
 
@@ -809,59 +971,6 @@ template <int F>
 using GeoF = MgGeo<MG_FINE_TX[F], MG_FINE_TY[F], MG_FINE_NU[F], MG_FINE_T[F]>;
 template <int G>
 using GeoC = MgGeo<MG_GEO_TX[G], MG_GEO_TY[G], MG_GEO_NU[G], MG_GEO_T[G]>;
-
-// level-0 code of every pixel of a CG plane (layout at Cg32Plane::code)
-__global__ __launch_bounds__(NT) void k32_code(const BlockWork *bw,
-		const Cg32Plane *pl)
-{
-	const BlockWork b = bw[blockIdx.x];
-	const Cg32Plane P = pl[b.plane];
-	const unsigned n = (unsigned)P.w * (unsigned)P.h, w = (unsigned)P.w,
-		h = (unsigned)P.h;
-	const unsigned q4 = (unsigned)b.start + 4 * threadIdx.x;
-	if (q4 >= n) return;
-	const unsigned q1 = q4 + 4 < n ? q4 + 4 : n;
-	unsigned j = q4 / w, i = q4 - j * w, word = 0;
-	// a group inside one row of a 16-byte aligned plane: its row and the
-	// rows above and below as one 16-byte load each (w a multiple of 4)
-	if (q1 - q4 == 4 && !(w & 3) && !((uintptr_t)P.g & 15)) {
-		const f4 c = *(const f4 *)(P.g + q4);
-		const f4 up = j > 0 ? *(const f4 *)(P.g + q4 - w) : c;
-		const f4 dn = j + 1 < h ? *(const f4 *)(P.g + q4 + w) : c;
-		const float lf = i > 0 ? P.g[q4 - 1] : 0.0f;
-		const float rt = i + 4 < w ? P.g[q4 + 4] : 0.0f;
-		const unsigned nrow = (j > 0) + (j + 1 < h);
-#pragma unroll
-		for (int k = 0; k < 4; k++) {
-			if (!isnan(c[k])) continue;
-			const unsigned ii = i + k;
-			unsigned v = 1;
-			if (ii + 1 < w && isnan(k < 3 ? c[k + 1] : rt)) v |= 2;
-			if (j + 1 < h && isnan(dn[k])) v |= 4;
-			if (ii > 0 && isnan(k > 0 ? c[k - 1] : lf)) v |= 8;
-			if (j > 0 && isnan(up[k])) v |= 16;
-			v |= ((ii > 0) + (ii + 1 < w) + nrow) << 5;
-			word |= v << (8 * k);
-		}
-		*(unsigned *)(P.code + q4) = word;
-		return;
-	}
-	for (unsigned q = q4; q < q1; q++) {
-		unsigned c = 0;
-		if (isnan(P.g[q])) {
-			c = 1;
-			if (i + 1 < w && isnan(P.g[q + 1])) c |= 2;
-			if (j + 1 < h && isnan(P.g[q + w])) c |= 4;
-			if (i > 0 && isnan(P.g[q - 1])) c |= 8;
-			if (j > 0 && isnan(P.g[q - w])) c |= 16;
-			c |= ((i > 0) + (i + 1 < w) + (j > 0) + (j + 1 < h)) << 5;
-		}
-		if (q1 - q4 == 4) word |= c << (8 * (q - q4));
-		else P.code[q] = (unsigned char)c;
-		if (++i == w) { i = 0; j++; }
-	}
-	if (q1 - q4 == 4) *(unsigned *)(P.code + q4) = word;
-}
 
 // packed coefficients of a level 1..MG_FUSE_MAX (one thread per cell);
 // wide: two words, the couplings E, S, W, N in a byte each and the
@@ -1762,13 +1871,17 @@ int mg_tail_lds_enable(int bytes)
 
 void mg_build(const Cg32Batch &B, hipStream_t s)
 {
-	for (int l = 0; l + 1 < B.nlev; l++) {
+	// level 1 with its packed words in one pass from the code bytes
+	if (B.nlev > 1 && B.mg[0].nbuild_c)
+		k_mg_coarsen0<<<B.mg[0].nbuild_c, NT, 0, s>>>(B.mg[0].build_c, B.mg[0].lv,
+				B.mg[0].coarse);
+	for (int l = 1; l + 1 < B.nlev; l++) {
 		const MgLevelWork &W = B.mg[l];
 		if (W.nbuild_c)
-			k_mg_coarsen<<<W.nbuild_c, NT, 0, s>>>(W.build_c, W.lv, W.coarse, l == 0);
+			k_mg_coarsen<<<W.nbuild_c, NT, 0, s>>>(W.build_c, W.lv, W.coarse, 0);
 	}
-	// packed coefficients of the fused levels (their couplings are final)
-	for (int l = 0; l < MG_FUSE_MAX && l + 1 < B.nlev; l++) {
+	// packed coefficients of the fused levels from 2 (their couplings are final)
+	for (int l = 1; l < MG_FUSE_MAX && l + 1 < B.nlev; l++) {
 		const MgLevelWork &W = B.mg[l];
 		if (W.nbuild_c && B.mg[l + 1].ntiles)
 			k_mg_gcode<<<W.nbuild_c, NT, 0, s>>>(W.build_c, W.coarse,
@@ -1778,7 +1891,6 @@ void mg_build(const Cg32Batch &B, hipStream_t s)
 
 void cg32_setup(const Cg32Batch &B, hipStream_t s)
 {
-	k32_code<<<B.nblk, NT, 0, s>>>(B.bw, B.planes);
 	k32_setup<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.part);
 	k32_reduce0<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive);
 	if (B.nlev) {
@@ -1819,9 +1931,10 @@ void cg32_iter(const Cg32Batch &B, hipStream_t s, int par)
 	if (B.prof) B.prof("k32_dirspmv", 0, 0.0, s);
 }
 
+// the Poisson iteration writes the solution in the pass that stops a plane
 void cg32_finish(const Cg32Batch &B, hipStream_t s)
 {
-	k32_finish<<<B.nblk, NT, 0, s>>>(B.bw, B.planes);
+	if (B.bilap) k32_finish<<<B.nblk, NT, 0, s>>>(B.bw, B.planes);
 }
 
 } // namespace spk

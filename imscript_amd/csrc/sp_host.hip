@@ -638,6 +638,8 @@ static int plan_create(Plan &pl, const std::vector<PlaneIn> &planes,
 				S.f = nullptr;
 				if (has_f) { S.f = pool + off; off += n; }
 			}
+			// only the Gauss-Seidel sweeps read the coefficient plane
+			if (P.niter <= 0) S.c = nullptr;
 		}
 		cnt_off += nm;
 		TRY(pl.upload(&L.d_sp, L.sp));

@@ -76,7 +76,7 @@ __global__ __launch_bounds__(EW_NT) void k_prepare(const SpPlane *planes,
 		if (q >= n) break;
 		float g = P.g[q];
 		holes += isnan(g) ? 1 : 0;
-		P.c[q] = coef_of(g, P.f, q);
+		if (P.c) P.c[q] = coef_of(g, P.f, q);
 	}
 	int s = block_sum_int<EW_NT>(holes, red);
 	if (threadIdx.x == 0) bcnt[blockIdx.x] = s;
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(EW_NT) void k_restrict(const SpPlane *fine,
 			C.f[q] = (float)((double)fv * pmsfac);
 		}
 		holes += isnan(g) ? 1 : 0;
-		C.c[q] = coef_of(g, C.f, q);
+		if (C.c) C.c[q] = coef_of(g, C.f, q);
 	}
 	int s = block_sum_int<EW_NT>(holes, red);
 	if (threadIdx.x == 0) bcnt[blockIdx.x] = s;
@@ -186,6 +186,20 @@ __global__ __launch_bounds__(EW_NT) void k_finalize(const SpPlane *lv,
 	const SpPlane P = lv[b.plane];
 	float *out = outs[b.plane];
 	const size_t n = (size_t)P.w * P.h;
+	if (out == P.u) {
+		// in place (the finest level solves in the output buffer): u holds g
+		// at finite known pixels already (k_init, and the sweeps and the CG
+		// write holes only), so only the known +-Inf pixels, which the
+		// solve saw as holes' initial values, change
+		if (*P.cnt <= 0) return;
+		for (int k = 0; k < EW_PPT; k++) {
+			size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
+			if (q >= n) break;
+			const float g = P.g[q];
+			if (isinf(g)) out[q] = g;
+		}
+		return;
+	}
 	for (int k = 0; k < EW_PPT; k++) {
 		size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
 		if (q >= n) break;
