@@ -19,6 +19,8 @@ struct Cg32Plane {
 	                       // number of in-image neighbours (set by cg32_setup)
 	const int *cnt;    // number of unknowns (device)
 	int w, h, bilap;
+	int u0;            // the initial guess is 0 at every unknown (coarsest
+	                   // pyramid level, no sweeps before the CG): u is not read
 	int blk0, nblk;    // this plane's work blocks / partial-sum slots
 	int t0, nt;        // this plane's level-0 multigrid tiles / their slots
 };

@@ -175,15 +175,20 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 			&& !(((uintptr_t)g | (uintptr_t)u | (uintptr_t)P.f) & 15)) {
 		// a group inside one row of 16-byte aligned planes: the clamped
 		// 5-point reads of g and u as one 16-byte load per row and a scalar
-		// on each side; the same sums in the same order as below
+		// on each side; the same sums in the same order as below.  With u0
+		// only its values at unknowns enter, all 0: u is not read.
 		const unsigned j = q4 / w, i = q4 - j * w;
-		const f4 gc = *(const f4 *)(g + q4), uc = *(const f4 *)(u + q4);
+		const f4 z0 = {0.0f, 0.0f, 0.0f, 0.0f};
+		const bool ru = !P.u0;
+		const f4 gc = *(const f4 *)(g + q4), uc = ru ? *(const f4 *)(u + q4) : z0;
 		const f4 gn = j > 0 ? *(const f4 *)(g + q4 - w) : gc;
-		const f4 un = j > 0 ? *(const f4 *)(u + q4 - w) : uc;
+		const f4 un = !ru ? z0 : j > 0 ? *(const f4 *)(u + q4 - w) : uc;
 		const f4 gs = j + 1 < (unsigned)P.h ? *(const f4 *)(g + q4 + w) : gc;
-		const f4 us = j + 1 < (unsigned)P.h ? *(const f4 *)(u + q4 + w) : uc;
-		const float gw = i > 0 ? g[q4 - 1] : gc[0], uw = i > 0 ? u[q4 - 1] : uc[0];
-		const float ge = i + 4 < w ? g[q4 + 4] : gc[3], ue = i + 4 < w ? u[q4 + 4] : uc[3];
+		const f4 us = !ru ? z0 : j + 1 < (unsigned)P.h ? *(const f4 *)(u + q4 + w) : uc;
+		const float gw = i > 0 ? g[q4 - 1] : gc[0];
+		const float uw = !ru ? 0.0f : i > 0 ? u[q4 - 1] : uc[0];
+		const float ge = i + 4 < w ? g[q4 + 4] : gc[3];
+		const float ue = !ru ? 0.0f : i + 4 < w ? u[q4 + 4] : uc[3];
 		const f4 fv = P.f ? *(const f4 *)(P.f + q4) : f4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
 		for (int k = 0; k < 4; k++) {

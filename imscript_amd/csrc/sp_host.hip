@@ -303,6 +303,7 @@ struct LevelPlan {
 		std::vector<int> ms;            // members in the batch
 		long maxit_host = 0;
 		int trace_len = 0, chunk = 1;
+		int last_chunks = 1;            // chunks the last solve ran
 		hipGraphExec_t exec = nullptr;
 	} cg;
 };
@@ -324,6 +325,10 @@ struct Plan {
 		*cg_part = nullptr, *cg_rr = nullptr;
 	int cg_rr_cap = 0;
 	int *cg_done = nullptr, *h_done = nullptr;
+	// pinned staging of small per-solve copies (descriptors in, results out):
+	// copies to and from pageable memory each cost a host round trip
+	void *h_stage = nullptr;
+	size_t h_stage_cap = 0;
 	// FP32 batched CG scratch: r p q (float) and x (double) per pixel
 	float *cg32_vec = nullptr;
 	double *cg32_x = nullptr;
@@ -345,7 +350,20 @@ struct Plan {
 			if (L.cg.exec) (void)hipGraphExecDestroy(L.cg.exec);
 		for (void *p : allocs) (void)hipFree(p);
 		if (h_done) (void)hipHostFree(h_done);
+		if (h_stage) (void)hipHostFree(h_stage);
 		if (cap_stream) (void)hipStreamDestroy(cap_stream);
+	}
+	// pinned staging area of at least n bytes (no copy may be in flight)
+	int stage(size_t n)
+	{
+		if (n <= h_stage_cap) return SP_OK;
+		if (h_stage) (void)hipHostFree(h_stage);
+		h_stage = nullptr;
+		h_stage_cap = 0;
+		if (hipHostMalloc(&h_stage, n) != hipSuccess)
+			return set_err(SP_ERR_NOMEM, "pinned host allocation of %zu bytes failed", n);
+		h_stage_cap = n;
+		return SP_OK;
 	}
 	template <typename T> int alloc(T **p, size_t n)
 	{
@@ -1014,6 +1032,9 @@ static int cg32_build(Plan &pl, int k, int bilap, float eps)
 		c.z = c.r;
 		if (pl.P.cg_mg) { c.z = pl.cg32_vec + off; off += na; }
 		c.w = S.w; c.h = S.h; c.bilap = bilap;
+		// the coarsest level starts from 0 at the unknowns (k_init) unless
+		// sweeps ran before the CG
+		c.u0 = k == (int)pl.lv.size() - 1 && L.nsweep[m] <= 0;
 		c.blk0 = (int)bw.size();
 		for (size_t st = 0; st < n; st += EW_SPAN) bw.push_back({(int)z, (int)st});
 		c.nblk = (int)bw.size() - c.blk0;
@@ -1071,8 +1092,16 @@ static int run_cg32(Plan &pl, int k, int bilap, float eps, hipStream_t s)
 	if (C.ms.empty()) return SP_OK;
 	Cg32Batch &B = C.B;
 	B.mg = C.mgw.data();
-	HIPCHK(hipMemcpyAsync(B.scal, C.sc0.data(), C.sc0.size() * sizeof(Cg32Scal),
-			hipMemcpyHostToDevice, s));
+	// staging: per-plane scalars, the rr trace, the level's hole counts
+	const size_t nsc = C.sc0.size() * sizeof(Cg32Scal);
+	const size_t ntr = C.trace_len * sizeof(double);
+	const size_t ncn = L.members.size() * sizeof(int);
+	// no copy of the staging area is in flight here: every run ends with a
+	// stream synchronization after its last copy
+	TRY(pl.stage(nsc + ntr + ncn));
+	char *stg = (char *)pl.h_stage;
+	memcpy(stg, C.sc0.data(), nsc);
+	HIPCHK(hipMemcpyAsync(B.scal, stg, nsc, hipMemcpyHostToDevice, s));
 	HIPCHK(hipMemsetAsync(B.nactive, 0, sizeof(int), s));
 	ProfScope pr("cg32", s);
 	// profiling runs launch the iterations directly, with a timer around
@@ -1086,32 +1115,42 @@ static int run_cg32(Plan &pl, int k, int bilap, float eps, hipStream_t s)
 	long launched = 0;
 	// profiling runs go one iteration at a time, so that no launch after
 	// convergence (an empty one) enters the per-kernel averages
+	// Other runs launch at first as many chunks as the last solve of this
+	// plan needed, back to back, before the first look at the stop flag
+	// (repeated solves of one shape converge alike), then one at a time.
+	int ahead = prof ? 1 : std::max(1, C.last_chunks);
+	long chunks = 0;
 	for (long done = 0; done < C.maxit_host;) {
-		const int step = prof ? 1 : C.chunk;
-		if (prof) spk::cg32_iter(B, s, (int)(done & 1));
-		else HIPCHK(hipGraphLaunch(C.exec, s));
-		done += step;
-		launched += step;
+		for (int a = 0; a < ahead && done < C.maxit_host; a++) {
+			const int step = prof ? 1 : C.chunk;
+			if (prof) spk::cg32_iter(B, s, (int)(done & 1));
+			else HIPCHK(hipGraphLaunch(C.exec, s));
+			done += step;
+			launched += step;
+			chunks++;
+		}
+		ahead = 1;
 		HIPCHK(hipMemcpyAsync(pl.h_done, B.nactive, sizeof(int), hipMemcpyDeviceToHost, s));
 		HIPCHK(hipStreamSynchronize(s));
 		if (pl.h_done[0] == 0) break;
 	}
+	if (!prof) C.last_chunks = (int)chunks;
 	B.prof = nullptr;
 	spk::cg32_finish(B, s);
 	HIPCHK(hipGetLastError());
-	// per-plane results: iterations and trace
+	// per-plane results: iterations and trace (the members' hole counters
+	// are consecutive: one copy)
+	HIPCHK(hipMemcpyAsync(stg, B.scal, nsc, hipMemcpyDeviceToHost, s));
+	if (ntr) HIPCHK(hipMemcpyAsync(stg + nsc, B.trace, ntr, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipMemcpyAsync(stg + nsc + ntr, L.sp[0].cnt, ncn, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
 	std::vector<Cg32Scal> sc(C.sc0.size());
-	HIPCHK(hipMemcpyAsync(sc.data(), B.scal, sc.size() * sizeof(Cg32Scal),
-			hipMemcpyDeviceToHost, s));
+	memcpy(sc.data(), stg, nsc);
 	std::vector<double> trace(C.trace_len);
-	if (C.trace_len)
-		HIPCHK(hipMemcpyAsync(trace.data(), B.trace, C.trace_len * sizeof(double),
-				hipMemcpyDeviceToHost, s));
+	if (ntr) memcpy(trace.data(), stg + nsc, ntr);
 	std::vector<int> cnts(C.ms.size());
 	for (size_t z = 0; z < C.ms.size(); z++)
-		HIPCHK(hipMemcpyAsync(&cnts[z], L.sp[C.ms[z]].cnt, sizeof(int),
-				hipMemcpyDeviceToHost, s));
-	HIPCHK(hipStreamSynchronize(s));
+		memcpy(&cnts[z], stg + nsc + ntr + C.ms[z] * sizeof(int), sizeof(int));
 	double unk = 0;
 	for (size_t z = 0; z < C.ms.size(); z++) {
 		const int p = L.members[C.ms[z]];
