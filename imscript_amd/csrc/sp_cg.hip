@@ -1613,37 +1613,49 @@ __global__ __launch_bounds__(MG_TAIL_T) void k_mg_tail(const MgLevel *lvall,
 }
 
 // The same tail with every level's z, b, D, wE, wS in LDS (planes whose tail
-// fits MG_TAIL_LDS_MAX): one LDS round trip and a barrier per phase instead
-// of dependent L2 accesses.  Same operations in the same order as above.
+// fits MG_TAIL_LDS_MAX).  Each level is stored with a frame of zero cells
+// (width w + 2, height h + 2): the frame holds z = 0, b = 0, D = 0 and no
+// couplings, the last column has wE = 0 and the last row wS = 0, so every
+// neighbour term is read unconditionally and a cell's loads go out
+// together (a frame or border term adds 0).  Inactive cells (D = 0) get
+// z = s * 0 = 0 (s is 0 there: no right-hand side, no couplings).
 struct TailLv {
 	float *z, *b, *D, *wE, *wS;
-	int w, h;
+	int w, h, W;   // W = w + 2
 };
 
-static __device__ float tl_off(const TailLv &L, int q, int i, int j)
+static __device__ __forceinline__ int tl_at(const TailLv &L, int i, int j)
+{
+	return (j + 1) * L.W + i + 1;
+}
+
+static __device__ __forceinline__ float tl_off(const TailLv &L, int p)
 {
 	float s = 0.0f;
-	if (i + 1 < L.w) s += L.wE[q] * L.z[q + 1];
-	if (i > 0) s += L.wE[q - 1] * L.z[q - 1];
-	if (j + 1 < L.h) s += L.wS[q] * L.z[q + L.w];
-	if (j > 0) s += L.wS[q - L.w] * L.z[q - L.w];
+	s += L.wE[p] * L.z[p + 1];
+	s += L.wE[p - 1] * L.z[p - 1];
+	s += L.wS[p] * L.z[p + L.W];
+	s += L.wS[p - L.W] * L.z[p - L.W];
 	return s;
+}
+
+static __device__ __forceinline__ float tl_rdiag(float d)
+{
+	return d != 0.0f ? __builtin_amdgcn_rcpf(d) : 0.0f;
 }
 
 static __device__ void tl_smooth(const TailLv &L, int color, bool first)
 {
 	const int n = L.w * L.h;
-	TailIdx p = tail_idx(L.w);
-	for (int q = threadIdx.x; q < n; q += blockDim.x, tail_next(p, L.w)) {
-		const int i = p.i, j = p.j;
-		if (((i + j) & 1) != color) {
-			if (first) L.z[q] = 0.0f;
+	TailIdx t = tail_idx(L.w);
+	for (int q = threadIdx.x; q < n; q += blockDim.x, tail_next(t, L.w)) {
+		const int p = tl_at(L, t.i, t.j);
+		if (((t.i + t.j) & 1) != color) {
+			if (first) L.z[p] = 0.0f;
 			continue;
 		}
-		const float d = L.D[q];
-		if (d == 0.0f) continue;
-		L.z[q] = (first ? L.b[q] : L.b[q] + tl_off(L, q, i, j)) *
-			__builtin_amdgcn_rcpf(d);
+		const float s = first ? L.b[p] : L.b[p] + tl_off(L, p);
+		L.z[p] = s * tl_rdiag(L.D[p]);
 	}
 	__syncthreads();
 }
@@ -1651,20 +1663,20 @@ static __device__ void tl_smooth(const TailLv &L, int color, bool first)
 static __device__ void tl_restrict(const TailLv &F, const TailLv &C)
 {
 	const int n = C.w * C.h;
-	TailIdx p = tail_idx(C.w);
-	for (int q = threadIdx.x; q < n; q += blockDim.x, tail_next(p, C.w)) {
-		const int J = p.j, I = p.i;
+	TailIdx t = tail_idx(C.w);
+	for (int q = threadIdx.x; q < n; q += blockDim.x, tail_next(t, C.w)) {
 		float r = 0;
+#pragma unroll
 		for (int c = 0; c < 4; c++) {
-			const int i = 2 * I + (c & 1), j = 2 * J + (c >> 1);
-			if (i >= F.w || j >= F.h) continue;
-			const int f = j * F.w + i;
+			// fine cells past an odd edge are frame cells: D = 0
+			const int f = tl_at(F, 2 * t.i + (c & 1), 2 * t.j + (c >> 1));
 			const float d = F.D[f];
-			if (d == 0.0f) continue;
-			r += F.b[f] - (d * F.z[f] - tl_off(F, f, i, j));
+			const float v = F.b[f] - (d * F.z[f] - tl_off(F, f));
+			r += d != 0.0f ? v : 0.0f;
 		}
-		C.b[q] = r;
-		C.z[q] = 0.0f;
+		const int p = tl_at(C, t.i, t.j);
+		C.b[p] = r;
+		C.z[p] = 0.0f;
 	}
 	__syncthreads();
 }
@@ -1672,10 +1684,11 @@ static __device__ void tl_restrict(const TailLv &F, const TailLv &C)
 static __device__ void tl_prolong(const TailLv &F, const TailLv &C, float wc)
 {
 	const int n = F.w * F.h;
-	TailIdx p = tail_idx(F.w);
-	for (int q = threadIdx.x; q < n; q += blockDim.x, tail_next(p, F.w)) {
-		if (F.D[q] == 0.0f) continue;
-		F.z[q] += wc * C.z[(p.j >> 1) * C.w + (p.i >> 1)];
+	TailIdx t = tail_idx(F.w);
+	for (int q = threadIdx.x; q < n; q += blockDim.x, tail_next(t, F.w)) {
+		const int p = tl_at(F, t.i, t.j);
+		const float z = F.z[p], e = wc * C.z[tl_at(C, t.i >> 1, t.j >> 1)];
+		F.z[p] = F.D[p] != 0.0f ? z + e : z;
 	}
 	__syncthreads();
 }
@@ -1695,27 +1708,30 @@ __global__ __launch_bounds__(MG_TAIL_T) void k_mg_tail_lds(const MgLevel *lvall,
 			toff[l] = o;
 			tw[l] = lv[l].w;
 			th[l] = lv[l].h;
-			o += 5 * lv[l].w * lv[l].h;
+			o += 5 * (lv[l].w + 2) * (lv[l].h + 2);
 		}
+		toff[d] = o;
 	}
 	__syncthreads();
-	// level l's arrays (z, b, D, wE, wS) from the offsets, no per-lane copies
+	// level l's framed arrays (z, b, D, wE, wS) from the offsets
 	auto T = [&](int l) {
-		const int n = tw[l] * th[l];
+		const int n = (tw[l] + 2) * (th[l] + 2);
 		float *p = tsm + toff[l];
-		return TailLv{p, p + n, p + 2 * n, p + 3 * n, p + 4 * n, tw[l], th[l]};
+		return TailLv{p, p + n, p + 2 * n, p + 3 * n, p + 4 * n, tw[l], th[l], tw[l] + 2};
 	};
+	// zero everything (frames included), then the interiors
+	for (int q = threadIdx.x; q < toff[d]; q += blockDim.x) tsm[q] = 0.0f;
+	__syncthreads();
 	for (int l = l0; l < d; l++) {
 		const TailLv L = T(l);
 		const int n = L.w * L.h;
-		for (int q = threadIdx.x; q < n; q += blockDim.x) {
-			L.D[q] = lv[l].D[q];
-			L.wE[q] = lv[l].wE[q];
-			L.wS[q] = lv[l].wS[q];
-			// the first sweep writes no inactive cell of its colour: start
-			// from z = 0 as the global-memory tail does
-			L.z[q] = 0.0f;
-			if (l == l0) L.b[q] = lv[l].b[q];
+		TailIdx t = tail_idx(L.w);
+		for (int q = threadIdx.x; q < n; q += blockDim.x, tail_next(t, L.w)) {
+			const int p = tl_at(L, t.i, t.j);
+			L.D[p] = lv[l].D[q];
+			L.wE[p] = lv[l].wE[q];
+			L.wS[p] = lv[l].wS[q];
+			if (l == l0) L.b[p] = lv[l].b[q];
 		}
 	}
 	__syncthreads();
@@ -1744,7 +1760,9 @@ __global__ __launch_bounds__(MG_TAIL_T) void k_mg_tail_lds(const MgLevel *lvall,
 	// the first tail level's correction is read by the next finer level
 	const TailLv L0 = T(l0);
 	const int n0 = L0.w * L0.h;
-	for (int q = threadIdx.x; q < n0; q += blockDim.x) lv[l0].z[q] = L0.z[q];
+	TailIdx t = tail_idx(L0.w);
+	for (int q = threadIdx.x; q < n0; q += blockDim.x, tail_next(t, L0.w))
+		lv[l0].z[q] = L0.z[tl_at(L0, t.i, t.j)];
 }
 
 static const char *const g_down_name[MG_FUSE_MAX + 1] = {

@@ -916,9 +916,12 @@ static int mg_plan(Plan &pl, std::vector<Cg32Plane> &cp, int bilap,
 		for (int l = 1; l < d; l++)
 			if ((long)shp[z][l].first * shp[z][l].second <= MG_TAIL_CELLS) { t = l; break; }
 		t = std::max(1, std::min(t, d - 1));
-		long cells = 0;
-		for (int l = t; l < d; l++) cells += (long)shp[z][l].first * shp[z][l].second;
-		const long bytes = 20 * cells;
+		long cells = 0, framed = 0;
+		for (int l = t; l < d; l++) {
+			cells += (long)shp[z][l].first * shp[z][l].second;
+			framed += (long)(shp[z][l].first + 2) * (shp[z][l].second + 2);
+		}
+		const long bytes = 20 * framed;   // z b D wE wS with a zero frame
 		const int in_lds = bytes <= MG_TAIL_LDS_MAX && !g_tail_lds_off;
 		if (in_lds) tail_lds = std::max<int>(tail_lds, (int)bytes);
 		rng[z] = make_int4(t, d, in_lds, (int)cells);
