@@ -991,7 +991,7 @@ def main():
     if cg_ms > 0 and a.method == "cg":
         achieved = cg_bytes / (cg_ms * 1e-3) / 1e9
         result["roofline"] = {
-            "kernel": "CG iteration (k32_spmv, k32_update, k32_dir + 2 per-plane "
+            "kernel": "CG iteration (k32_update, k32_dirspmv + 2 per-plane "
                       "reductions)",
             "bound": "hbm", "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBPS, "unit": "GB/s",

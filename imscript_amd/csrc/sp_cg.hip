@@ -276,30 +276,35 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 	}
 }
 
+// Per-plane folds run one work-group of RT threads per plane: a plane has
+// up to ~16K partial sums (one per work block), and the fold is bound by
+// the latency of its loads.
+constexpr int RT = 1024;
+
 // one block per plane: fold the plane's slots in a fixed order
 static __device__ void fold(const double *part, const Cg32Plane &P, int stride,
 		int off, double *red, double *out)
 {
 	// eight loads in flight per thread; slot k of the plane always lands in
-	// accumulator ((k / NT) % 8) of thread k % NT, so the order is fixed
+	// accumulator ((k / RT) % 8) of thread k % RT, so the order is fixed
 	double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 	const double *pp = part + (size_t)stride * P.blk0 + off;
 	int k = threadIdx.x;
-	for (; k + 7 * NT < P.nblk; k += 8 * NT)
+	for (; k + 7 * RT < P.nblk; k += 8 * RT)
 #pragma unroll
-		for (int u = 0; u < 8; u++) v[u] += pp[(size_t)stride * (k + u * NT)];
+		for (int u = 0; u < 8; u++) v[u] += pp[(size_t)stride * (k + u * RT)];
 #pragma unroll
 	for (int u = 0; u < 7; u++)
-		if (k + u * NT < P.nblk) v[u] += pp[(size_t)stride * (k + u * NT)];
+		if (k + u * RT < P.nblk) v[u] += pp[(size_t)stride * (k + u * RT)];
 	const double t = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
-	const double s = bsum<NT>(t, red);
+	const double s = bsum<RT>(t, red);
 	if (threadIdx.x == 0) *out = s;
 }
 
-__global__ __launch_bounds__(NT) void k32_reduce0(const Cg32Plane *pl, Cg32Scal *sc,
+__global__ __launch_bounds__(RT) void k32_reduce0(const Cg32Plane *pl, Cg32Scal *sc,
 		const double *part, int *nactive)
 {
-	__shared__ double red[NT / 64];
+	__shared__ double red[RT / 64];
 	__shared__ double rr, bb;
 	const Cg32Plane P = pl[blockIdx.x];
 	fold(part, P, 2, 0, red, &rr);
@@ -399,10 +404,10 @@ __global__ __launch_bounds__(NT) void k32_spmv(const BlockWork *bw,
 	if (threadIdx.x == 0) part[2 * blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(NT) void k32_reduce_a(const Cg32Plane *pl, Cg32Scal *sc,
+__global__ __launch_bounds__(RT) void k32_reduce_a(const Cg32Plane *pl, Cg32Scal *sc,
 		const double *part, int *nactive)
 {
-	__shared__ double red[NT / 64];
+	__shared__ double red[RT / 64];
 	__shared__ double pq;
 	if (!sc[blockIdx.x].active) {
 		if (threadIdx.x == 0) sc[blockIdx.x].upd = sc[blockIdx.x].fin = 0;
@@ -465,10 +470,10 @@ __global__ __launch_bounds__(NT) void k32_update(const BlockWork *bw,
 }
 
 // rr_new, absolute stop test on sqrt(rr_new), beta
-__global__ __launch_bounds__(NT) void k32_reduce_b(const Cg32Plane *pl, Cg32Scal *sc,
+__global__ __launch_bounds__(RT) void k32_reduce_b(const Cg32Plane *pl, Cg32Scal *sc,
 		const double *part, int *nactive, double *trace, int precond)
 {
-	__shared__ double red[NT / 64];
+	__shared__ double red[RT / 64];
 	__shared__ double rr1;
 	if (!sc[blockIdx.x].active) return;
 	fold(part, pl[blockIdx.x], 2, 1, red, &rr1);
@@ -1483,15 +1488,15 @@ void k_mg_up(const int4 *tiles, const MgLevel *lv,
 }
 
 // rho_new = r.z folded over the plane's level-0 tiles
-__global__ __launch_bounds__(NT) void k_mg_reduce_t(const Cg32Plane *pl,
+__global__ __launch_bounds__(RT) void k_mg_reduce_t(const Cg32Plane *pl,
 		Cg32Scal *sc, const double *tpart, int setup)
 {
-	__shared__ double red[NT / 64];
+	__shared__ double red[RT / 64];
 	if (!sc[blockIdx.x].active) return;
 	const Cg32Plane P = pl[blockIdx.x];
 	double v = 0;
-	for (int k = P.t0 + threadIdx.x; k < P.t0 + P.nt; k += NT) v += tpart[k];
-	const double rz = bsum<NT>(v, red);
+	for (int k = P.t0 + threadIdx.x; k < P.t0 + P.nt; k += RT) v += tpart[k];
+	const double rz = bsum<RT>(v, red);
 	if (threadIdx.x == 0) {
 		Cg32Scal &s = sc[blockIdx.x];
 		s.beta = setup ? 0.0 : rz / s.rz;
@@ -1624,6 +1629,24 @@ struct TailLv {
 	int w, h, W;   // W = w + 2
 };
 
+// f(i, j) on this thread's cells of a w x h level.  Levels up to 1024 wide
+// map threads by shifts: thread t takes column t mod VW and rows t / VW +
+// k (T / VW), VW the power of two at or above w; wider ones step (i, j).
+template <class F>
+static __device__ __forceinline__ void tail_for(int w, int h, F &&f)
+{
+	if (w <= 1024) {
+		const int sh = w > 1 ? 32 - __clz(w - 1) : 0;
+		const int i = (int)threadIdx.x & ((1 << sh) - 1);
+		if (i >= w) return;
+		const int dj = (int)blockDim.x >> sh;
+		for (int j = (int)threadIdx.x >> sh; j < h; j += dj) f(i, j);
+		return;
+	}
+	TailIdx t = tail_idx(w);
+	for (int q = threadIdx.x; q < w * h; q += blockDim.x, tail_next(t, w)) f(t.i, t.j);
+}
+
 static __device__ __forceinline__ int tl_at(const TailLv &L, int i, int j)
 {
 	return (j + 1) * L.W + i + 1;
@@ -1646,50 +1669,44 @@ static __device__ __forceinline__ float tl_rdiag(float d)
 
 static __device__ void tl_smooth(const TailLv &L, int color, bool first)
 {
-	const int n = L.w * L.h;
-	TailIdx t = tail_idx(L.w);
-	for (int q = threadIdx.x; q < n; q += blockDim.x, tail_next(t, L.w)) {
-		const int p = tl_at(L, t.i, t.j);
-		if (((t.i + t.j) & 1) != color) {
+	tail_for(L.w, L.h, [&](int i, int j) {
+		const int p = tl_at(L, i, j);
+		if (((i + j) & 1) != color) {
 			if (first) L.z[p] = 0.0f;
-			continue;
+			return;
 		}
 		const float s = first ? L.b[p] : L.b[p] + tl_off(L, p);
 		L.z[p] = s * tl_rdiag(L.D[p]);
-	}
+	});
 	__syncthreads();
 }
 
 static __device__ void tl_restrict(const TailLv &F, const TailLv &C)
 {
-	const int n = C.w * C.h;
-	TailIdx t = tail_idx(C.w);
-	for (int q = threadIdx.x; q < n; q += blockDim.x, tail_next(t, C.w)) {
+	tail_for(C.w, C.h, [&](int i, int j) {
 		float r = 0;
 #pragma unroll
 		for (int c = 0; c < 4; c++) {
 			// fine cells past an odd edge are frame cells: D = 0
-			const int f = tl_at(F, 2 * t.i + (c & 1), 2 * t.j + (c >> 1));
+			const int f = tl_at(F, 2 * i + (c & 1), 2 * j + (c >> 1));
 			const float d = F.D[f];
 			const float v = F.b[f] - (d * F.z[f] - tl_off(F, f));
 			r += d != 0.0f ? v : 0.0f;
 		}
-		const int p = tl_at(C, t.i, t.j);
+		const int p = tl_at(C, i, j);
 		C.b[p] = r;
 		C.z[p] = 0.0f;
-	}
+	});
 	__syncthreads();
 }
 
 static __device__ void tl_prolong(const TailLv &F, const TailLv &C, float wc)
 {
-	const int n = F.w * F.h;
-	TailIdx t = tail_idx(F.w);
-	for (int q = threadIdx.x; q < n; q += blockDim.x, tail_next(t, F.w)) {
-		const int p = tl_at(F, t.i, t.j);
-		const float z = F.z[p], e = wc * C.z[tl_at(C, t.i >> 1, t.j >> 1)];
+	tail_for(F.w, F.h, [&](int i, int j) {
+		const int p = tl_at(F, i, j);
+		const float z = F.z[p], e = wc * C.z[tl_at(C, i >> 1, j >> 1)];
 		F.z[p] = F.D[p] != 0.0f ? z + e : z;
-	}
+	});
 	__syncthreads();
 }
 
@@ -1881,7 +1898,7 @@ static void vcycle(const Cg32Batch &B, hipStream_t s)
 static void precondition(const Cg32Batch &B, int setup, hipStream_t s)
 {
 	vcycle(B, s);
-	k_mg_reduce_t<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.tpart, setup);
+	k_mg_reduce_t<<<B.nplanes, RT, 0, s>>>(B.planes, B.scal, B.tpart, setup);
 	if (setup && B.bilap)   // p = z + 0 * p
 		k32_dir<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal);
 }
@@ -1915,7 +1932,7 @@ void mg_build(const Cg32Batch &B, hipStream_t s)
 void cg32_setup(const Cg32Batch &B, hipStream_t s)
 {
 	k32_setup<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.part);
-	k32_reduce0<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive);
+	k32_reduce0<<<B.nplanes, RT, 0, s>>>(B.planes, B.scal, B.part, B.nactive);
 	if (B.nlev) {
 		mg_build(B, s);
 		precondition(B, 1, s);
@@ -1934,17 +1951,17 @@ void cg32_iter(const Cg32Batch &B, hipStream_t s, int par)
 	const int pc = B.nlev > 0;
 	if (B.bilap) {
 		k32_spmv<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
-		k32_reduce_a<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive);
+		k32_reduce_a<<<B.nplanes, RT, 0, s>>>(B.planes, B.scal, B.part, B.nactive);
 		k32_update<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
-		k32_reduce_b<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive,
+		k32_reduce_b<<<B.nplanes, RT, 0, s>>>(B.planes, B.scal, B.part, B.nactive,
 				B.trace, pc);
 		if (pc) precondition(B, 0, s);
 		k32_dir<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal);
 		return;
 	}
-	k32_reduce_a<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive);
+	k32_reduce_a<<<B.nplanes, RT, 0, s>>>(B.planes, B.scal, B.part, B.nactive);
 	k32_update<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
-	k32_reduce_b<<<B.nplanes, NT, 0, s>>>(B.planes, B.scal, B.part, B.nactive,
+	k32_reduce_b<<<B.nplanes, RT, 0, s>>>(B.planes, B.scal, B.part, B.nactive,
 			B.trace, pc);
 	if (pc) precondition(B, 0, s);
 	// reads z, the code, p_old (with the four neighbours) and x, writes
