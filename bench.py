@@ -41,9 +41,9 @@ PEAK_HBM_GBPS = 8000.0
 # counter reads for k_stream_copy moving 1 GiB each way in that run (FETCH_SIZE
 # x2.000, WRITE_SIZE x1.000).  Summary and commands: profiles/hbm_traffic_*.md.
 TRAFFIC_PMC = {
-    "mg_up0": 644.0e6,
+    "mg_up0": 644.7e6,
     "mg_down0": 495.5e6,
-    "k32_dirspmv": 1008.7e6,
+    "k32_dirspmv": 1026.9e6,
 }
 
 
