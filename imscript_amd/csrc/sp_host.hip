@@ -1721,6 +1721,8 @@ static int lap_args(const float *out, const float *in, int w, int h, int pd,
 		return set_err(SP_ERR_ARG, "bad image arguments");
 	if ((long)w * h * pd > 0x7fffffffL)
 		return set_err(SP_ERR_ARG, "image too large");
+	if (w > 65535 * 1024)   // column chunks of 1024 pixels per grid row
+		return set_err(SP_ERR_ARG, "image too wide");
 	if (variant != 0 && variant != 1)
 		return set_err(SP_ERR_ARG, "variant must be 0 (x,l) or 1 (x,lr)");
 	if (boundary < 0 || boundary > 4)

@@ -53,62 +53,74 @@ static __device__ __forceinline__ float lp_sample(const float *x, int w, int h,
 	return x[(size_t)j * w + i];
 }
 
-// grid: x = rows of all planes, y = column chunks of LP_T*LP_PX
+// Grid: x = strips of LP_RS rows of all planes, y = column chunks of
+// LP_T*LP_PX.  A thread walks its strip down with a rolling 3-row window (one
+// new row per output row: a 16-byte load and a scalar on each side when
+// its columns are inside the plane and the planes are 16-byte aligned).
+constexpr int LP_RS = 16;
+
+template <bool VEC>
+static __device__ __forceinline__ void lp_row(float (&v)[LP_PX + 2], const float *x,
+		int w, int h, int i0, int jj, bool colin, int mode)
+{
+	if (VEC && colin && jj >= 0 && jj < h) {
+		const float *r = x + (size_t)jj * w + i0;
+		const float4 c = *(const float4 *)r;
+		v[0] = r[-1];
+		v[1] = c.x; v[2] = c.y; v[3] = c.z; v[4] = c.w;
+		v[5] = r[LP_PX];
+		return;
+	}
+#pragma unroll
+	for (int dx = 0; dx < LP_PX + 2; dx++) v[dx] = lp_sample(x, w, h, i0 - 1 + dx, jj, mode);
+}
+
 template <int V, bool VEC>
 __global__ __launch_bounds__(LP_T) void k_plambda_lap(float *__restrict__ out,
 		const float *__restrict__ in, int w, int h, int mode)
 {
-	const int row = blockIdx.x, l = row / h, j = row - l * h;
+	const int strips = (h + LP_RS - 1) / LP_RS;
+	const int l = blockIdx.x / strips, j0 = (blockIdx.x - l * strips) * LP_RS;
 	const int i0 = (blockIdx.y * LP_T + threadIdx.x) * LP_PX;
 	if (i0 >= w) return;
+	const int j1 = j0 + LP_RS < h ? j0 + LP_RS : h;
 	const float *x = in + (size_t)l * w * h;
-	float *y = out + (size_t)l * w * h + (size_t)j * w;
+	float *y = out + (size_t)l * w * h;
 	constexpr const float *s = LapStencil<V>::s;
-	// the 3 x (LP_PX + 2) window around the thread's pixels
+	const bool colin = i0 >= 1 && i0 + LP_PX < w;
 	float v[3][LP_PX + 2];
-	const bool inner = i0 >= 1 && i0 + LP_PX < w && j >= 1 && j + 1 < h;
-	// with 16-byte aligned planes (VEC: both base pointers aligned and the
-	// width a multiple of 4) every row starts aligned: one 16-byte load per row
-	const bool vec = VEC && inner;
-	if (vec) {
+	lp_row<VEC>(v[0], x, w, h, i0, j0 - 1, colin, mode);
+	lp_row<VEC>(v[1], x, w, h, i0, j0, colin, mode);
+	for (int j = j0; j < j1; j++) {
+		lp_row<VEC>(v[2], x, w, h, i0, j + 1, colin, mode);
+		float o[LP_PX];
 #pragma unroll
-		for (int dy = 0; dy < 3; dy++) {
-			const float *r = x + (size_t)(j - 1 + dy) * w + i0;
-			const float4 c = *(const float4 *)r;
-			v[dy][0] = r[-1];
-			v[dy][1] = c.x; v[dy][2] = c.y; v[dy][3] = c.z; v[dy][4] = c.w;
-			v[dy][5] = r[LP_PX];
+		for (int p = 0; p < LP_PX; p++) {
+			float r = 0.0f;
+#pragma unroll
+			for (int k = 0; k < 9; k++) r = r + s[k] * v[k / 3][p + k % 3];
+			o[p] = r;
 		}
-	} else {
+		float *yr = y + (size_t)j * w;
+		if (VEC && i0 + LP_PX <= w) {
+			*(float4 *)(yr + i0) = make_float4(o[0], o[1], o[2], o[3]);
+		} else {
 #pragma unroll
-		for (int dy = 0; dy < 3; dy++)
+			for (int p = 0; p < LP_PX; p++)
+				if (i0 + p < w) yr[i0 + p] = o[p];
+		}
 #pragma unroll
-			for (int dx = 0; dx < LP_PX + 2; dx++) {
-				const int i = i0 - 1 + dx, jj = j - 1 + dy;
-				v[dy][dx] = inner ? x[(size_t)jj * w + i] : lp_sample(x, w, h, i, jj, mode);
-			}
-	}
-	float o[LP_PX];
-#pragma unroll
-	for (int p = 0; p < LP_PX; p++) {
-		float r = 0.0f;
-#pragma unroll
-		for (int k = 0; k < 9; k++) r = r + s[k] * v[k / 3][p + k % 3];
-		o[p] = r;
-	}
-	if (VEC && i0 + LP_PX <= w) {
-		*(float4 *)(y + i0) = make_float4(o[0], o[1], o[2], o[3]);
-	} else {
-#pragma unroll
-		for (int p = 0; p < LP_PX; p++)
-			if (i0 + p < w) y[i0 + p] = o[p];
+		for (int dx = 0; dx < LP_PX + 2; dx++) {
+			v[0][dx] = v[1][dx];
+			v[1][dx] = v[2][dx];
+		}
 	}
 }
 
 void launch_plambda_lap(float *out, const float *in, int w, int h, int pd,
 		int variant, int mode, hipStream_t s)
 {
-	const dim3 grid(h * pd, (w + LP_T * LP_PX - 1) / (LP_T * LP_PX));
+	const dim3 grid((h + LP_RS - 1) / LP_RS * pd, (w + LP_T * LP_PX - 1) / (LP_T * LP_PX));
 	const bool vec = !(w & 3) && !((uintptr_t)out & 15) && !((uintptr_t)in & 15);
 	if (variant) {
 		if (vec) k_plambda_lap<1, true><<<grid, LP_T, 0, s>>>(out, in, w, h, mode);
