@@ -45,6 +45,11 @@ TRAFFIC_PMC = {
     "mg_down0": 495.5e6,
     "k32_dirspmv": 1026.9e6,
 }
+# k_plambda_lap on the 4096x4096x3 planes, from the same two passes over
+# `bench.py --config lap --no-cpu`, scaled by k_stream_copy moving the same
+# 201 MB each way in that run (x2.000, x1.000): 221.0 MB read (the halo rows
+# of 32-row strips) + 201.3 MB written.
+TRAFFIC_PMC_LAP = 422.3e6
 
 
 def sinusoid_sum(torch, rng, n, dev, k=8):
@@ -586,10 +591,11 @@ def run_lap(a):
     make_inputs(torch, a.size, a.channels, a.seed, "cuda:0")
     v = make_inputs.source
     out = torch.empty_like(v)
-    for _ in range(max(1, a.warmup)):
+    # one launch is about 0.1 ms: warm up and time many of them
+    for _ in range(max(20, a.warmup)):
         sp.plambda_lap(v, 0, 1, out=out)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    reps = max(10, a.steps)
+    reps = max(200, a.steps)
     torch.cuda.synchronize()
     e0.record()
     for _ in range(reps):
@@ -606,11 +612,17 @@ def run_lap(a):
            "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"plambda \"x,l\" of {a.size}x{a.size}x{a.channels} "
                                   "float32 planar source images, nearest-pixel rule"},
-           "roofline": {"kernel": "k_plambda_lap<0> (4 pixels per thread)", "bound": "hbm",
+           "roofline": {"kernel": "k_plambda_lap<0> (4 pixels per thread, 32-row strips)",
+                        "bound": "hbm",
                         "achieved": round(gbps, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                        "frac": round(gbps / PEAK_HBM_GBPS, 4), "traffic": None,
+                        "frac": round(gbps / PEAK_HBM_GBPS, 4),
+                        "traffic": (TRAFFIC_PMC_LAP if a.size == 4096 and a.channels == 3
+                                    else None),
                         "algorithmic_bytes_per_launch": 8.0 * px,
                         "avg_launch_us": round(ms * 1e3, 2)}}
+    # the plain copy kernel over the same bytes (one read and one write of
+    # the planes): the measured ceiling at this size
+    res["stream_copy_same_bytes"] = stream_copy_ceiling(torch, sp, nbytes=4 * px, reps=200)
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_leg_lap(v, out)
     emit(res)

@@ -54,10 +54,12 @@ static __device__ __forceinline__ float lp_sample(const float *x, int w, int h,
 }
 
 // Grid: x = strips of LP_RS rows of all planes, y = column chunks of
-// LP_T*LP_PX.  A thread walks its strip down with a rolling 3-row window (one
-// new row per output row: a 16-byte load and a scalar on each side when
-// its columns are inside the plane and the planes are 16-byte aligned).
-constexpr int LP_RS = 16;
+// LP_T*LP_PX.  A thread walks its strip down with a rolling window of rows:
+// each step loads the next LP_U rows (a 16-byte load and a scalar on each
+// side when its columns are inside the plane and the planes are 16-byte
+// aligned; the boundary rule otherwise), so LP_U rows of loads are in
+// flight at once, then writes LP_U output rows.
+constexpr int LP_RS = 32, LP_U = 2;
 
 template <bool VEC>
 static __device__ __forceinline__ void lp_row(float (&v)[LP_PX + 2], const float *x,
@@ -88,31 +90,39 @@ __global__ __launch_bounds__(LP_T) void k_plambda_lap(float *__restrict__ out,
 	float *y = out + (size_t)l * w * h;
 	constexpr const float *s = LapStencil<V>::s;
 	const bool colin = i0 >= 1 && i0 + LP_PX < w;
-	float v[3][LP_PX + 2];
+	// rows j-1 .. j+LP_U of the current step (rows past the plane take the
+	// boundary rule, rows past the strip are read and not written)
+	float v[LP_U + 2][LP_PX + 2];
 	lp_row<VEC>(v[0], x, w, h, i0, j0 - 1, colin, mode);
 	lp_row<VEC>(v[1], x, w, h, i0, j0, colin, mode);
-	for (int j = j0; j < j1; j++) {
-		lp_row<VEC>(v[2], x, w, h, i0, j + 1, colin, mode);
-		float o[LP_PX];
+	for (int j = j0; j < j1; j += LP_U) {
 #pragma unroll
-		for (int p = 0; p < LP_PX; p++) {
-			float r = 0.0f;
+		for (int t = 0; t < LP_U; t++)
+			lp_row<VEC>(v[2 + t], x, w, h, i0, j + 1 + t, colin, mode);
 #pragma unroll
-			for (int k = 0; k < 9; k++) r = r + s[k] * v[k / 3][p + k % 3];
-			o[p] = r;
-		}
-		float *yr = y + (size_t)j * w;
-		if (VEC && i0 + LP_PX <= w) {
-			*(float4 *)(yr + i0) = make_float4(o[0], o[1], o[2], o[3]);
-		} else {
+		for (int t = 0; t < LP_U; t++) {
+			if (j + t >= j1) break;
+			float o[LP_PX];
 #pragma unroll
-			for (int p = 0; p < LP_PX; p++)
-				if (i0 + p < w) yr[i0 + p] = o[p];
+			for (int p = 0; p < LP_PX; p++) {
+				float r = 0.0f;
+#pragma unroll
+				for (int k = 0; k < 9; k++) r = r + s[k] * v[t + k / 3][p + k % 3];
+				o[p] = r;
+			}
+			float *yr = y + (size_t)(j + t) * w;
+			if (VEC && i0 + LP_PX <= w) {
+				*(float4 *)(yr + i0) = make_float4(o[0], o[1], o[2], o[3]);
+			} else {
+#pragma unroll
+				for (int p = 0; p < LP_PX; p++)
+					if (i0 + p < w) yr[i0 + p] = o[p];
+			}
 		}
 #pragma unroll
 		for (int dx = 0; dx < LP_PX + 2; dx++) {
-			v[0][dx] = v[1][dx];
-			v[1][dx] = v[2][dx];
+			v[0][dx] = v[LP_U][dx];
+			v[1][dx] = v[LP_U + 1][dx];
 		}
 	}
 }
