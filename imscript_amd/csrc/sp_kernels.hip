@@ -364,6 +364,8 @@ static __device__ __forceinline__ double cg_stencil(int bilap, int i, int j,
 static __device__ double sum_partials(const double *part, int nblk, double *red)
 {
 	double v = 0;
+	// unrolled so that the loads issue together; the adds keep their order
+#pragma unroll 4
 	for (int k = threadIdx.x; k < nblk; k += CG_NT) v += part[k];
 	double s = block_sum_dbl<CG_NT>(v, red);
 	__shared__ double bc;
@@ -420,49 +422,135 @@ static __device__ __forceinline__ bool cg_stopped(const CgDev &d)
 	return __hip_atomic_load(d.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
+// The iteration kernels walk the same grid-stride sequence of pixels per
+// thread as the setup (pixel t, t + S, t + 2S, ... with S = nblk * CG_NT)
+// and accumulate in that order, in batches of U pixels whose loads are
+// issued before the first value is used.  Indices are 32-bit (the solver
+// takes at most 2^30 pixels per plane).
+
+// taps of A at (i, j) in the reference's term order (cleant_cgpois.c:91-121):
+// Poisson (i,j), (i+1,j), (i,j+1), (i-1,j), (i,j-1); the biharmonic adds the
+// diagonals and the distance-2 taps
+template <int BILAP>
+struct CgTaps {
+	static constexpr int N = BILAP ? 13 : 5;
+};
+static __device__ __forceinline__ void cg_tap_offset(int k, int &di, int &dj)
+{
+	constexpr int DI[13] = {0, 1, 0, -1, 0, 1, 1, -1, -1, 2, 0, -2, 0};
+	constexpr int DJ[13] = {0, 0, 1, 0, -1, 1, -1, 1, -1, 0, 2, 0, -2};
+	di = DI[k];
+	dj = DJ[k];
+}
+
+// the stencil over the taps' values, each rounded to float, summed in float
+// in the reference's order (the same operations as cg_stencil)
+template <int BILAP>
+static __device__ __forceinline__ double cg_apply_taps(const double (&t)[CgTaps<BILAP>::N])
+{
+	if (!BILAP) {
+		float r = -4.0f * (float)t[0];
+		r = r + (float)t[1];
+		r = r + (float)t[2];
+		r = r + (float)t[3];
+		r = r + (float)t[4];
+		return -(double)r;
+	}
+	float r = 20.0f * (float)t[0];
+#pragma unroll
+	for (int k = 1; k < 5; k++) r = r - 8.0f * (float)t[k];
+#pragma unroll
+	for (int k = 5; k < 9; k++) r = r + 2.0f * (float)t[k];
+#pragma unroll
+	for (int k = 9; k < 13; k++) r = r + (float)t[k];
+	return (double)r;
+}
+
 // q = A p, partial p.q
+template <int BILAP, int U>
 __global__ __launch_bounds__(CG_NT) void k_cg_spmv(CgDev d)
 {
 	__shared__ double red[CG_NT / 64];
 	if (cg_stopped(d)) return;
-	const size_t n = (size_t)d.w * d.h;
+	constexpr int NT = CgTaps<BILAP>::N;
+	const int w = d.w, h = d.h;
+	const unsigned n = (unsigned)w * (unsigned)h, S = (unsigned)d.nblk * CG_NT;
 	const double *p = d.p;
 	double acc = 0;
-	for (size_t q = (size_t)blockIdx.x * CG_NT + threadIdx.x; q < n;
-			q += (size_t)d.nblk * CG_NT) {
-		if (!isnan(d.g[q])) continue;
-		const int i = (int)(q % d.w), j = (int)(q / d.w);
-		double y = cg_stencil(d.bilap, i, j, d.w, d.h, [&](size_t k) {
-			return (float)p[k]; });
-		d.q[q] = y;
-		acc += y * p[q];
+	for (unsigned q0 = blockIdx.x * CG_NT + threadIdx.x; q0 < n; q0 += U * S) {
+		bool hole[U];
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			const unsigned q = q0 + u * S;
+			hole[u] = q < n && isnan(d.g[q]);
+		}
+		double t[U][NT];
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			if (!hole[u]) continue;
+			const unsigned q = q0 + u * S;
+			const int j = (int)(q / (unsigned)w), i = (int)(q - (unsigned)j * w);
+#pragma unroll
+			for (int k = 0; k < NT; k++) {
+				int di, dj;
+				cg_tap_offset(k, di, dj);
+				const int a = clampi(i + di, 0, w - 1), b = clampi(j + dj, 0, h - 1);
+				t[u][k] = p[(unsigned)b * (unsigned)w + (unsigned)a];
+			}
+		}
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			if (!hole[u]) continue;
+			const unsigned q = q0 + u * S;
+			const double y = cg_apply_taps<BILAP>(t[u]);
+			d.q[q] = y;
+			acc += y * t[u][0];
+		}
 	}
 	double s = block_sum_dbl<CG_NT>(acc, red);
 	if (threadIdx.x == 0) d.part[blockIdx.x] = s;
 }
 
 // alpha = rr_old / p.q ; x += alpha p ; r -= alpha q ; partial r.r
+template <int U>
 __global__ __launch_bounds__(CG_NT) void k_cg_update(CgDev d, int it)
 {
 	__shared__ double red[CG_NT / 64];
 	if (cg_stopped(d)) return;
 	const double pq = sum_partials(d.part, d.nblk, red);
 	const double alpha = d.rr[it] / pq;
-	const size_t n = (size_t)d.w * d.h;
+	const unsigned n = (unsigned)d.w * (unsigned)d.h, S = (unsigned)d.nblk * CG_NT;
 	double acc = 0;
-	for (size_t q = (size_t)blockIdx.x * CG_NT + threadIdx.x; q < n;
-			q += (size_t)d.nblk * CG_NT) {
-		if (!isnan(d.g[q])) continue;
-		d.x[q] = d.x[q] + alpha * d.p[q];
-		double r = d.r[q] - alpha * d.q[q];
-		d.r[q] = r;
-		acc += r * r;
+	for (unsigned q0 = blockIdx.x * CG_NT + threadIdx.x; q0 < n; q0 += U * S) {
+		bool hole[U];
+		double xv[U], pv[U], rv[U], qv[U];
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			const unsigned q = q0 + u * S;
+			hole[u] = q < n && isnan(d.g[q]);
+		}
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			if (!hole[u]) continue;
+			const unsigned q = q0 + u * S;
+			xv[u] = d.x[q]; pv[u] = d.p[q]; rv[u] = d.r[q]; qv[u] = d.q[q];
+		}
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			if (!hole[u]) continue;
+			const unsigned q = q0 + u * S;
+			d.x[q] = xv[u] + alpha * pv[u];
+			const double r = rv[u] - alpha * qv[u];
+			d.r[q] = r;
+			acc += r * r;
+		}
 	}
 	double s = block_sum_dbl<CG_NT>(acc, red);
 	if (threadIdx.x == 0) d.part[d.nblk + blockIdx.x] = s;
 }
 
 // rr_new; absolute stop test; beta; p = r + beta p
+template <int U>
 __global__ __launch_bounds__(CG_NT) void k_cg_dir(CgDev d, int it)
 {
 	__shared__ double red[CG_NT / 64];
@@ -478,11 +566,26 @@ __global__ __launch_bounds__(CG_NT) void k_cg_dir(CgDev d, int it)
 		return;
 	}
 	const double beta = rr1 / rr0;
-	const size_t n = (size_t)d.w * d.h;
-	for (size_t q = (size_t)blockIdx.x * CG_NT + threadIdx.x; q < n;
-			q += (size_t)d.nblk * CG_NT) {
-		if (!isnan(d.g[q])) continue;
-		d.p[q] = d.r[q] + beta * d.p[q];
+	const unsigned n = (unsigned)d.w * (unsigned)d.h, S = (unsigned)d.nblk * CG_NT;
+	for (unsigned q0 = blockIdx.x * CG_NT + threadIdx.x; q0 < n; q0 += U * S) {
+		bool hole[U];
+		double rv[U], pv[U];
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			const unsigned q = q0 + u * S;
+			hole[u] = q < n && isnan(d.g[q]);
+		}
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			if (!hole[u]) continue;
+			const unsigned q = q0 + u * S;
+			rv[u] = d.r[q]; pv[u] = d.p[q];
+		}
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			if (!hole[u]) continue;
+			d.p[q0 + u * S] = rv[u] + beta * pv[u];
+		}
 	}
 }
 
@@ -503,9 +606,10 @@ void launch_cg_setup(const CgDev &d, hipStream_t s)
 
 void launch_cg_iter(const CgDev &d, int it, hipStream_t s)
 {
-	k_cg_spmv<<<d.nblk, CG_NT, 0, s>>>(d);
-	k_cg_update<<<d.nblk, CG_NT, 0, s>>>(d, it);
-	k_cg_dir<<<d.nblk, CG_NT, 0, s>>>(d, it);
+	if (d.bilap) k_cg_spmv<1, 2><<<d.nblk, CG_NT, 0, s>>>(d);
+	else k_cg_spmv<0, 4><<<d.nblk, CG_NT, 0, s>>>(d);
+	k_cg_update<4><<<d.nblk, CG_NT, 0, s>>>(d, it);
+	k_cg_dir<4><<<d.nblk, CG_NT, 0, s>>>(d, it);
 }
 
 void launch_cg_finish(const CgDev &d, hipStream_t s)
