@@ -759,9 +759,10 @@ def cpu_leg_heatd(x, sp, n, t, side):
 def run_heatd(a):
     """heatd's explicit diffusion (heatd.c) of the C2 source images,
     4096x4096x3 planar float32, n steps: throughput of the whole diffusion,
-    the step kernel's HBM roofline (8 B per pixel per step: read the plane,
-    write the next; 12 B more with a diffusion-matrix field) and the
-    reference heatd on a crop."""
+    the HBM roofline of the kernel that runs (8 B per pixel per launch: read
+    the plane, write the next; 12 B more with a diffusion-matrix field; the
+    multi-step kernels run several steps per launch) and the reference
+    heatd on a crop."""
     import torch
     import imscript_amd as sp
     n = a.niter if a.niter is not None else 100
@@ -788,14 +789,17 @@ def run_heatd(a):
     sp.profile(True)
     sp.heatd(v, n, t, A, out=out)
     torch.cuda.synchronize()
-    multi = A is None and n > 1
-    k_ms, k_launch, k_bytes = sp.profile_read("heatd_multi" if multi else "heatd_step")
+    k_ms, k_launch, k_bytes = sp.profile_read("heatd_multi")
+    multi = k_launch > 0
+    if not multi:
+        k_ms, k_launch, k_bytes = sp.profile_read("heatd_step")
     sp.profile(False)
     px = v.numel()
     per = k_bytes / max(k_launch, 1)
     achieved = per / (k_ms * 1e-3 / max(k_launch, 1)) / 1e9
     steps_per_launch = n / max(k_launch, 1)
-    eff = 8.0 * px * n / (k_ms * 1e-3) / 1e9
+    # the bytes a one-step-per-launch kernel would move for the n steps
+    eff = (8.0 + (12.0 if A is not None else 0.0)) * px * n / (k_ms * 1e-3) / 1e9
     res = {"metric": "heatd diffusion, Mpix/s (n steps per pixel)",
            "value": round(px / 1e6 / dt, 1), "unit": "Mpix/s", "n_gpus": 1,
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(dt * 1e3, 3),
@@ -805,8 +809,10 @@ def run_heatd(a):
                                   f"of {a.size}x{a.size}x{a.channels} float32 planar "
                                   "source images (sum of sinusoids)",
                       "Mpix_steps_per_s": round(px * n / 1e6 / dt, 1)},
-           "roofline": {"kernel": ("k_heatd_multi<T> (%.3g steps per launch on 64 x 48 "
-                                   "register tiles per wave)" % steps_per_launch) if multi
+           "roofline": {"kernel": ("k_heatd_multi%s<T> (%.3g steps per launch on "
+                                   "register tiles of 64 columns per wave)"
+                                   % ("_field" if A is not None else "", steps_per_launch))
+                        if multi
                         else "k_heatd_step<%s> (4 pixels per thread, one launch per step)"
                         % ("field" if A is not None else "isotropic"),
                         "bound": "hbm", "achieved": round(achieved, 1),

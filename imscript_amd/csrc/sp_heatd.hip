@@ -195,17 +195,133 @@ __global__ __launch_bounds__(64 * HT_WAVES) void k_heatd_multi(float *__restrict
 	}
 }
 
+// With a diffusion-matrix field: the same register tiles, HR rows per wave,
+// each lane also holding the (a, b, c) of its column's pixels, and the
+// reference's nine ordered products per pixel (heatd.c:10-45 with
+// s = {b/2, c, -b/2, a, -2a-2c, a, -b/2, c, b/2}, r / 4), the corners being
+// the west / east neighbours of the rows above and below.  Borders as in
+// k_heatd_multi: a clamped north (south) row brings its corners with it, and
+// the per-row west / east clamps make the corners of the border columns.
+// The field is read once per launch instead of once per step.
+template <int T, int HR>
+__global__ __launch_bounds__(64 * HT_WAVES) void k_heatd_multi_field(float *__restrict__ out,
+		const float *__restrict__ in, const float *__restrict__ A, int w, int h,
+		int tiles_x, int tiles_y, float t)
+{
+	constexpr int OW = 64 - 2 * T, OH = HR - 2 * T;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int ty = blockIdx.y * HT_WAVES + wave;
+	if (ty >= tiles_y) return;
+	const int l = blockIdx.x / tiles_x, tx = blockIdx.x - l * tiles_x;
+	const float *x = in + (size_t)l * w * h;
+	float *y = out + (size_t)l * w * h;
+	const int x0 = min(tx * OW, w - OW), y0 = min(ty * OH, h - OH);
+	const int gi = x0 - T + lane, gj0 = y0 - T;
+	const int ci = gi < 0 ? 0 : gi >= w ? w - 1 : gi;
+	float v[HR], fa[HR], fb[HR], fc[HR];
+#pragma unroll
+	for (int r = 0; r < HR; r++) {
+		int gj = gj0 + r;
+		gj = gj < 0 ? 0 : gj >= h ? h - 1 : gj;
+		const size_t q = (size_t)gj * w + ci;
+		v[r] = x[q];
+		fa[r] = A[3 * q];
+		fb[r] = A[3 * q + 1];
+		fc[r] = A[3 * q + 2];
+	}
+	const bool atl = gi == 0, atr = gi == w - 1;
+	const bool edge_cols = x0 - T <= 0 || x0 + OW + T - 1 >= w - 1;
+	const bool top = y0 == 0, bot = y0 == h - OH;
+	for (int k = 0; k < T; k++) {
+		float c = v[0];
+		float cw = __shfl_up(c, 1, 64), ce = __shfl_down(c, 1, 64);
+		if (edge_cols) { cw = atl ? c : cw; ce = atr ? c : ce; }
+		float u = c, uw = cw, ue = ce;
+#pragma unroll
+		for (int r = 0; r < HR; r++) {
+			const float d = r + 1 < HR ? v[r + 1] : c;
+			float dw = __shfl_up(d, 1, 64), de = __shfl_down(d, 1, 64);
+			if (edge_cols) { dw = atl ? d : dw; de = atr ? d : de; }
+			float n = u, nw = uw, ne = ue, so = d, sw = dw, se = de;
+			if (r == T && top) { n = c; nw = cw; ne = ce; }
+			if (r == HR - 1 - T && bot) { so = c; sw = cw; se = ce; }
+			const float a = fa[r], b = fb[r], cc = fc[r];
+			const float hb = b / 2, nhb = -b / 2, m = -2 * a - 2 * cc;
+			float q = 0.0f;
+			q = q + hb * nw;
+			q = q + cc * n;
+			q = q + nhb * ne;
+			q = q + a * cw;
+			q = q + m * c;
+			q = q + a * ce;
+			q = q + nhb * sw;
+			q = q + cc * so;
+			q = q + hb * se;
+			const float nv = c + t * (q / 4);
+			u = c; uw = cw; ue = ce;
+			c = d; cw = dw; ce = de;
+			v[r] = nv;
+		}
+	}
+	if (lane < T || lane >= 64 - T) return;
+	const int jlo = top ? 0 : T - 1, jhi = bot ? h : h - T + 1;
+#pragma unroll
+	for (int r = T; r < HR - T; r++) {
+		const int gj = gj0 + r;
+		if (gj >= jlo && gj < jhi) y[(size_t)gj * w + gi] = v[r];
+	}
+}
+
 int heatd_multi_max() { return 8; }
 
-bool heatd_multi_fits(int w, int h, int T)
+bool heatd_multi_fits(int w, int h, int T, int rows)
 {
-	return w >= 64 - 2 * T && h >= HT_R - 2 * T;
+	return w >= 64 - 2 * T && h >= rows - 2 * T;
+}
+
+bool launch_heatd_multi_field(float *out, const float *in, const float *A, int w, int h,
+		int pd, int T, int rows, float t, hipStream_t s)
+{
+	// rows - 2T >= T - 1: the first and last tiles of a column write every
+	// row that the others leave (those within T - 1 of the image's edge)
+	if (T < 1 || T > heatd_multi_max() ||
+			(rows != 16 && rows != 24 && rows != 32 && rows != 48) ||
+			!heatd_multi_fits(w, h, T, rows) || rows - 2 * T < T - 1)
+		return false;
+	auto go = [&](auto tc, auto rc) {
+		constexpr int TT = decltype(tc)::value, RR = decltype(rc)::value;
+		constexpr int OW = 64 - 2 * TT, OH = RR - 2 * TT;
+		const int tiles_x = (w + OW - 1) / OW, tiles_y = (h + OH - 1) / OH;
+		const dim3 grid(tiles_x * pd, (tiles_y + HT_WAVES - 1) / HT_WAVES);
+		k_heatd_multi_field<TT, RR><<<grid, 64 * HT_WAVES, 0, s>>>(out, in, A, w, h,
+				tiles_x, tiles_y, t);
+	};
+	auto by_rows = [&](auto tc) {
+		constexpr int TT = decltype(tc)::value;
+		if constexpr (16 - 2 * TT >= TT - 1)
+			if (rows == 16) return go(tc, std::integral_constant<int, 16>{});
+		if constexpr (24 - 2 * TT >= TT - 1)
+			if (rows == 24) return go(tc, std::integral_constant<int, 24>{});
+		if (rows == 32) return go(tc, std::integral_constant<int, 32>{});
+		go(tc, std::integral_constant<int, 48>{});
+	};
+	switch (T) {
+	case 1: by_rows(std::integral_constant<int, 1>{}); break;
+	case 2: by_rows(std::integral_constant<int, 2>{}); break;
+	case 3: by_rows(std::integral_constant<int, 3>{}); break;
+	case 4: by_rows(std::integral_constant<int, 4>{}); break;
+	case 5: by_rows(std::integral_constant<int, 5>{}); break;
+	case 6: by_rows(std::integral_constant<int, 6>{}); break;
+	case 7: by_rows(std::integral_constant<int, 7>{}); break;
+	default: by_rows(std::integral_constant<int, 8>{}); break;
+	}
+	return true;
 }
 
 bool launch_heatd_multi(float *out, const float *in, int w, int h, int pd, int T,
 		float t, hipStream_t s)
 {
-	if (T < 1 || T > heatd_multi_max() || !heatd_multi_fits(w, h, T)) return false;
+	if (T < 1 || T > heatd_multi_max() || !heatd_multi_fits(w, h, T, HT_R)) return false;
 	auto go = [&](auto tc) {
 		constexpr int TT = decltype(tc)::value;
 		constexpr int OW = 64 - 2 * TT, OH = HT_R - 2 * TT;

@@ -1601,12 +1601,16 @@ extern "C" int sp_amle_host(float *out, const float *in, const float *init,
 
 // ------------------------------------------------------------------ heatd
 //
-// n explicit diffusion steps (heatd.c:58-103).  With a diffusion-matrix
-// field, one launch per step; isotropic, T steps per launch on register
-// tiles (k_heatd_multi, T = SIMPOIS_HIP_HEATD_T, default 6, 1..8; a single
-// remaining step uses the one-step kernel).  Launches ping-pong between the
+// n explicit diffusion steps (heatd.c:58-103), T steps per launch on
+// register tiles: isotropic k_heatd_multi (T = SIMPOIS_HIP_HEATD_T, default
+// 6, 1..8), with a diffusion-matrix field k_heatd_multi_field (T =
+// SIMPOIS_HIP_HEATD_FT, default HEATD_FIELD_T; tile rows
+// SIMPOIS_HIP_HEATD_FR, 16, 24, 32 or 48, default HEATD_FIELD_ROWS); a single
+// remaining step uses the one-step kernel.  Launches ping-pong between the
 // output and a stream-ordered scratch plane set, so that the last one
 // lands in the output.
+
+constexpr int HEATD_FIELD_T = 5, HEATD_FIELD_ROWS = 32;   // measured best at 4096x4096x3
 
 static int heatd_args(const float *out, const float *in, int w, int h, int pd, int n)
 {
@@ -1629,16 +1633,21 @@ extern "C" int sp_heatd_device(float *d_out, const float *d_in, const float *d_A
 		HIPCHK(hipMemcpyAsync(d_out, d_in, np * 4, hipMemcpyDeviceToDevice, s));
 		return SP_OK;
 	}
-	const char *tv = getenv("SIMPOIS_HIP_HEATD_T");
-	int T = tv ? atoi(tv) : 6;
+	// steps per launch and (with a field) tile rows, read at each call
+	const char *tv = getenv(d_A ? "SIMPOIS_HIP_HEATD_FT" : "SIMPOIS_HIP_HEATD_T");
+	int T = tv ? atoi(tv) : d_A ? HEATD_FIELD_T : 6;
 	T = T < 1 ? 1 : T > spk::heatd_multi_max() ? spk::heatd_multi_max() : T;
-	if (d_A) T = 1;
+	const char *rv = getenv("SIMPOIS_HIP_HEATD_FR");
+	const int rv_n = rv ? atoi(rv) : 0;
+	const int rows = !d_A ? 48 : (rv_n == 16 || rv_n == 24 || rv_n == 32 || rv_n == 48)
+		? rv_n : HEATD_FIELD_ROWS;
+	if (d_A && T > (rows + 1) / 3) T = (rows + 1) / 3;   // rows >= 3T - 1
 	// T steps per launch; a remainder, or a chunk the planes are too small
 	// for (smaller T needs wider and taller planes), as single steps
 	std::vector<int> chunks;
 	for (int rem = n; rem > 0; rem -= chunks.back()) {
 		const int k = std::min(rem, T);
-		chunks.push_back(k > 1 && spk::heatd_multi_fits(w, h, k) ? k : 1);
+		chunks.push_back(k > 1 && spk::heatd_multi_fits(w, h, k, rows) ? k : 1);
 	}
 	const int L = (int)chunks.size();
 	float *tmp = nullptr;
@@ -1647,8 +1656,12 @@ extern "C" int sp_heatd_device(float *d_out, const float *d_in, const float *d_A
 	const float *src = d_in;
 	for (int k = 0; k < L; k++) {
 		float *dst = (L - 1 - k) & 1 ? tmp : d_out;
-		if (chunks[k] == 1) spk::launch_heatd_step(dst, src, d_A, w, h, pd, t, s);
-		else if (!spk::launch_heatd_multi(dst, src, w, h, pd, chunks[k], t, s))
+		const bool ok = chunks[k] == 1
+			? (spk::launch_heatd_step(dst, src, d_A, w, h, pd, t, s), true)
+			: d_A ? spk::launch_heatd_multi_field(dst, src, d_A, w, h, pd, chunks[k], rows,
+					t, s)
+			: spk::launch_heatd_multi(dst, src, w, h, pd, chunks[k], t, s);
+		if (!ok)
 			return set_err(SP_ERR_ARG, "heatd: planes too small for %d steps per launch",
 					chunks[k]);
 		src = dst;

@@ -73,10 +73,16 @@ void launch_heatd_step(float *out, const float *in, const float *A, int w, int h
 		int pd, float t, hipStream_t s);
 // T isotropic steps (1..heatd_multi_max()) in one launch on register tiles
 int heatd_multi_max();
-bool heatd_multi_fits(int w, int h, int T);   // planes large enough for T
+// planes large enough for T steps on tiles of `rows` rows (48 isotropic)
+bool heatd_multi_fits(int w, int h, int T, int rows);
 // (false, nothing launched, when the planes are too small for T)
 bool launch_heatd_multi(float *out, const float *in, int w, int h, int pd, int T,
 		float t, hipStream_t s);
+// T steps with the field A (w*h*3 interleaved) on tiles of rows = 16, 24,
+// 32 or 48 rows (at least 3T - 1); false, nothing launched, for other rows
+// or planes too small
+bool launch_heatd_multi_field(float *out, const float *in, const float *A, int w, int h,
+		int pd, int T, int rows, float t, hipStream_t s);
 
 // dst[0..4*n4) = src[0..4*n4), 16-byte aligned
 void launch_stream_copy(float *dst, const float *src, size_t n4, hipStream_t s);

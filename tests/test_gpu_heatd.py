@@ -73,6 +73,45 @@ def test_multi_step_register_tiles_match_oracle(T, monkeypatch):
         assert bits_equal(y, oracle_heatd(x, n, np.float32(0.21))[0]), shape
 
 
+@pytest.mark.parametrize("T,rows", [(T, r) for r in (16, 24, 32, 48)
+                                    for T in range(1, 9) if r >= 3 * T - 1])
+def test_multi_step_field_register_tiles_match_oracle(T, rows, monkeypatch):
+    # several steps per launch with a diffusion-matrix field: the same tile
+    # geometry as the isotropic kernel with `rows` rows per tile (at least
+    # 3T - 1, so that the first and last tile of a column write the rows the
+    # others leave); non-finite image values on the borders and corners,
+    # zero and -0 shear terms
+    monkeypatch.setenv("SIMPOIS_HIP_HEATD_FT", str(T))
+    monkeypatch.setenv("SIMPOIS_HIP_HEATD_FR", str(rows))
+    rng = np.random.default_rng(100 + 10 * T + rows)
+    OW, OH = 64 - 2 * T, rows - 2 * T
+    for shape in [(2, 97, 131), (1, OH, OW), (1, 211, 70), (1, 2 * OH + T, 2 * OW + T),
+                  (1, OH + T - 1, 3 * OW + 2), (1, 3 * OH - 1, OW + T + 1)]:
+        x = (rng.standard_normal(shape) * 10).astype(np.float32)
+        p, h, w = shape
+        x[0, 0, 0], x[0, -1, -1], x[0, 0, w // 2] = np.inf, -np.inf, np.nan
+        x[-1, h // 2, 0], x[-1, 3, -1] = -0.0, 1e30
+        x[-1, rng.integers(0, h, 5), rng.integers(0, w, 5)] = np.nan
+        A = field(rng, h, w)
+        A[0, 0, 1], A[h - 1, w // 3, 1], A[h // 2, w - 1, 1] = 0.0, -0.0, 0.0
+        n = 2 * T + 3
+        print(f"heatd field T={T} rows={rows} {shape} n={n}", flush=True)
+        y = host(sp.heatd(dev(x), n, 0.21, dev(A)))
+        assert bits_equal(y, oracle_heatd(x, n, np.float32(0.21), A)[0]), shape
+
+
+def test_field_steps_per_launch_beyond_the_tile_limit_are_reduced(monkeypatch):
+    # 8 steps per launch asked on 16-row tiles (which hold at most 5): the
+    # library runs 5 per launch and the result stays the reference's
+    monkeypatch.setenv("SIMPOIS_HIP_HEATD_FT", "8")
+    monkeypatch.setenv("SIMPOIS_HIP_HEATD_FR", "16")
+    rng = np.random.default_rng(7)
+    x = (rng.standard_normal((2, 97, 131)) * 10).astype(np.float32)
+    A = field(rng, 97, 131)
+    y = host(sp.heatd(dev(x), 19, 0.21, dev(A)))
+    assert bits_equal(y, oracle_heatd(x, 19, np.float32(0.21), A)[0])
+
+
 def test_unaligned_views_use_the_element_path():
     import torch
     rng = np.random.default_rng(5)
