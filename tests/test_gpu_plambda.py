@@ -40,6 +40,27 @@ def test_device_entry_matches_oracle(variant, mode):
     assert bits_equal(got.cpu().numpy(), oracle_plambda_lap(x, variant, mode))
 
 
+def test_aligned_planes_take_the_vector_path_and_match_oracle():
+    # widths divisible by 4 and 16-byte aligned planes: 16-byte row loads in
+    # 32-row strips; heights that end a strip early, one- and two-row planes,
+    # every boundary rule and both stencils
+    import torch
+    m = _m()
+    rng = np.random.default_rng(48)
+    for shape in [(3, 517, 1032), (1, 1, 8), (2, 2, 4), (1, 31, 2048), (2, 33, 12),
+                  (1, 65, 1028)]:
+        x = (rng.standard_normal(shape) * 40).astype(np.float32)
+        x[0, 0, 0], x[-1, -1, -1], x[0, shape[1] // 2, shape[2] // 2] = np.inf, -np.inf, np.nan
+        dx = torch.from_numpy(x).cuda()
+        for variant in (0, 1):
+            for mode in range(5):
+                print(f"plambda vector {shape} variant={variant} mode={mode}", flush=True)
+                got = m.plambda_lap(dx, variant, mode)
+                torch.cuda.synchronize()
+                assert bits_equal(got.cpu().numpy(), oracle_plambda_lap(x, variant, mode)), \
+                    (shape, variant, mode)
+
+
 def test_unaligned_planes_and_odd_widths():
     # a view one float into its buffer is not 16-byte aligned, and odd
     # widths leave partial groups: both take the element-wise path
