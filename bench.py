@@ -50,6 +50,12 @@ TRAFFIC_PMC = {
 # 201 MB each way in that run (x2.000, x1.000): 221.0 MB read (the halo rows
 # of 32-row strips) + 201.3 MB written.
 TRAFFIC_PMC_LAP = 422.3e6
+# heatd -n 100 on the 4096x4096x3 planes, per launch of the default
+# multi-step kernels (k_heatd_multi<6>; with the field
+# k_heatd_multi_field<5, 32>), same passes over `bench.py --config heatd
+# [--heatd-field] --no-cpu --steps 1 --warmup 0` and the same scaling by the
+# copy in that run: 403.1 + 217.4 MB and 990.4 + 224.1 MB.
+TRAFFIC_PMC_HEATD = {"isotropic": 620.5e6, "field": 1214.5e6}
 
 
 def sinusoid_sum(torch, rng, n, dev, k=8):
@@ -817,12 +823,20 @@ def run_heatd(a):
                         % ("field" if A is not None else "isotropic"),
                         "bound": "hbm", "achieved": round(achieved, 1),
                         "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                        "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
+                        "frac": round(achieved / PEAK_HBM_GBPS, 4),
+                        "traffic": (TRAFFIC_PMC_HEATD["field" if A is not None else "isotropic"]
+                                    if multi and a.size == 4096 and a.channels == 3
+                                    and not os.environ.get("SIMPOIS_HIP_HEATD_T")
+                                    and not os.environ.get("SIMPOIS_HIP_HEATD_FT")
+                                    and not os.environ.get("SIMPOIS_HIP_HEATD_FR") else None),
                         "algorithmic_bytes_per_launch": per,
                         "avg_launch_us": round(k_ms * 1e3 / max(k_launch, 1), 2),
                         "launches": k_launch,
                         "step_GBps_effective": round(eff, 1),
                         "step_frac_effective": round(eff / PEAK_HBM_GBPS, 4)}}
+    # the plain copy kernel over the plane bytes (one read, one write): the
+    # measured ceiling at this size and the counters' known-size reference
+    res["stream_copy_same_bytes"] = stream_copy_ceiling(torch, sp, nbytes=4 * px, reps=20)
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_leg_heatd(v, sp, n, t, a.heatd_cpu_side)
     emit(res)
