@@ -383,6 +383,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_setup(CgDev d)
 	for (size_t q = (size_t)blockIdx.x * CG_NT + threadIdx.x; q < n;
 			q += (size_t)d.nblk * CG_NT) {
 		const float gq = d.g[q];
+		d.hole[q] = isnan(gq);
 		if (!isnan(gq)) {
 			d.x[q] = 0; d.r[q] = 0; d.p[q] = 0; d.q[q] = 0;
 			continue;
@@ -425,8 +426,9 @@ static __device__ __forceinline__ bool cg_stopped(const CgDev &d)
 // The iteration kernels walk the same grid-stride sequence of pixels per
 // thread as the setup (pixel t, t + S, t + 2S, ... with S = nblk * CG_NT)
 // and accumulate in that order, in batches of U pixels whose loads are
-// issued before the first value is used.  Indices are 32-bit (the solver
-// takes at most 2^30 pixels per plane).
+// issued before the first value is used.  They find the unknowns in the
+// setup's byte mask (1 B per pixel instead of the 4 B of g).  Indices are
+// 32-bit (the solver takes at most 2^30 pixels per plane).
 
 // taps of A at (i, j) in the reference's term order (cleant_cgpois.c:91-121):
 // Poisson (i,j), (i+1,j), (i,j+1), (i-1,j), (i,j-1); the biharmonic adds the
@@ -482,7 +484,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_spmv(CgDev d)
 #pragma unroll
 		for (int u = 0; u < U; u++) {
 			const unsigned q = q0 + u * S;
-			hole[u] = q < n && isnan(d.g[q]);
+			hole[u] = q < n && d.hole[q];
 		}
 		double t[U][NT];
 #pragma unroll
@@ -527,7 +529,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_update(CgDev d, int it)
 #pragma unroll
 		for (int u = 0; u < U; u++) {
 			const unsigned q = q0 + u * S;
-			hole[u] = q < n && isnan(d.g[q]);
+			hole[u] = q < n && d.hole[q];
 		}
 #pragma unroll
 		for (int u = 0; u < U; u++) {
@@ -573,7 +575,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_dir(CgDev d, int it)
 #pragma unroll
 		for (int u = 0; u < U; u++) {
 			const unsigned q = q0 + u * S;
-			hole[u] = q < n && isnan(d.g[q]);
+			hole[u] = q < n && d.hole[q];
 		}
 #pragma unroll
 		for (int u = 0; u < U; u++) {
