@@ -324,6 +324,7 @@ struct Plan {
 	double *cg_x = nullptr, *cg_r = nullptr, *cg_p = nullptr, *cg_q = nullptr,
 		*cg_part = nullptr, *cg_rr = nullptr;
 	unsigned char *cg_hole = nullptr;   // unknowns mask of the plane being solved
+	float *cg_pf = nullptr;             // direction rounded to float
 	int cg_rr_cap = 0;
 	int *cg_done = nullptr, *h_done = nullptr;
 	// pinned staging of small per-solve copies (descriptors in, results out):
@@ -769,6 +770,7 @@ static int cg64_ensure(Plan &pl, int maxit)
 		TRY(pl.alloc(&pl.cg_p, n0));
 		TRY(pl.alloc(&pl.cg_q, n0));
 		TRY(pl.alloc(&pl.cg_hole, n0));
+		TRY(pl.alloc(&pl.cg_pf, n0));
 		TRY(pl.alloc(&pl.cg_part, 2 * 1024));
 		TRY(pl.alloc(&pl.cg_done, 4));
 	}
@@ -800,6 +802,7 @@ static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 	spk::CgDev d;
 	d.g = S.g; d.f = S.f; d.u = S.u;
 	d.x = pl.cg_x; d.r = pl.cg_r; d.p = pl.cg_p; d.q = pl.cg_q; d.hole = pl.cg_hole;
+	d.pf = pl.cg_pf;
 	d.part = pl.cg_part; d.rr = pl.cg_rr; d.done = pl.cg_done;
 	d.w = S.w; d.h = S.h; d.bilap = bilap; d.nblk = nblk;
 	d.stop = (double)e;

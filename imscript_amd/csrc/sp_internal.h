@@ -101,6 +101,7 @@ struct CgDev {
 	float *u;             // solution plane (read for x0, written at the end)
 	double *x, *r, *p, *q;
 	unsigned char *hole;  // 1 at the unknowns (NaN in g), written by the setup
+	float *pf;            // p rounded to float (what the stencil reads), kept with p
 	double *part;         // per-block partial sums (2 * nblk)
 	double *rr;           // rr history, rr[0] = r0.r0, rr[k+1] = after iter k
 	int *done;            // [0] = stop flag, [1] = iterations run

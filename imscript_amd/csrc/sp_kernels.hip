@@ -385,7 +385,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_setup(CgDev d)
 		const float gq = d.g[q];
 		d.hole[q] = isnan(gq);
 		if (!isnan(gq)) {
-			d.x[q] = 0; d.r[q] = 0; d.p[q] = 0; d.q[q] = 0;
+			d.x[q] = 0; d.r[q] = 0; d.p[q] = 0; d.pf[q] = 0; d.q[q] = 0;
 			continue;
 		}
 		const int i = (int)(q % d.w), j = (int)(q / d.w);
@@ -400,6 +400,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_setup(CgDev d)
 		d.x[q] = (double)u[q];
 		d.r[q] = r;
 		d.p[q] = r;
+		d.pf[q] = (float)r;
 		d.q[q] = 0;
 		acc += r * r;
 	}
@@ -445,26 +446,26 @@ static __device__ __forceinline__ void cg_tap_offset(int k, int &di, int &dj)
 	dj = DJ[k];
 }
 
-// the stencil over the taps' values, each rounded to float, summed in float
+// the stencil over the taps' values (p rounded to float), summed in float
 // in the reference's order (the same operations as cg_stencil)
 template <int BILAP>
-static __device__ __forceinline__ double cg_apply_taps(const double (&t)[CgTaps<BILAP>::N])
+static __device__ __forceinline__ double cg_apply_taps(const float (&t)[CgTaps<BILAP>::N])
 {
 	if (!BILAP) {
-		float r = -4.0f * (float)t[0];
-		r = r + (float)t[1];
-		r = r + (float)t[2];
-		r = r + (float)t[3];
-		r = r + (float)t[4];
+		float r = -4.0f * t[0];
+		r = r + t[1];
+		r = r + t[2];
+		r = r + t[3];
+		r = r + t[4];
 		return -(double)r;
 	}
-	float r = 20.0f * (float)t[0];
+	float r = 20.0f * t[0];
 #pragma unroll
-	for (int k = 1; k < 5; k++) r = r - 8.0f * (float)t[k];
+	for (int k = 1; k < 5; k++) r = r - 8.0f * t[k];
 #pragma unroll
-	for (int k = 5; k < 9; k++) r = r + 2.0f * (float)t[k];
+	for (int k = 5; k < 9; k++) r = r + 2.0f * t[k];
 #pragma unroll
-	for (int k = 9; k < 13; k++) r = r + (float)t[k];
+	for (int k = 9; k < 13; k++) r = r + t[k];
 	return (double)r;
 }
 
@@ -477,7 +478,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_spmv(CgDev d)
 	constexpr int NT = CgTaps<BILAP>::N;
 	const int w = d.w, h = d.h;
 	const unsigned n = (unsigned)w * (unsigned)h, S = (unsigned)d.nblk * CG_NT;
-	const double *p = d.p;
+	const float *pf = d.pf;
 	double acc = 0;
 	for (unsigned q0 = blockIdx.x * CG_NT + threadIdx.x; q0 < n; q0 += U * S) {
 		bool hole[U];
@@ -486,18 +487,20 @@ __global__ __launch_bounds__(CG_NT) void k_cg_spmv(CgDev d)
 			const unsigned q = q0 + u * S;
 			hole[u] = q < n && d.hole[q];
 		}
-		double t[U][NT];
+		float t[U][NT];
+		double pc[U];
 #pragma unroll
 		for (int u = 0; u < U; u++) {
 			if (!hole[u]) continue;
 			const unsigned q = q0 + u * S;
 			const int j = (int)(q / (unsigned)w), i = (int)(q - (unsigned)j * w);
+			pc[u] = d.p[q];
 #pragma unroll
 			for (int k = 0; k < NT; k++) {
 				int di, dj;
 				cg_tap_offset(k, di, dj);
 				const int a = clampi(i + di, 0, w - 1), b = clampi(j + dj, 0, h - 1);
-				t[u][k] = p[(unsigned)b * (unsigned)w + (unsigned)a];
+				t[u][k] = pf[(unsigned)b * (unsigned)w + (unsigned)a];
 			}
 		}
 #pragma unroll
@@ -506,7 +509,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_spmv(CgDev d)
 			const unsigned q = q0 + u * S;
 			const double y = cg_apply_taps<BILAP>(t[u]);
 			d.q[q] = y;
-			acc += y * t[u][0];
+			acc += y * pc[u];
 		}
 	}
 	double s = block_sum_dbl<CG_NT>(acc, red);
@@ -586,7 +589,9 @@ __global__ __launch_bounds__(CG_NT) void k_cg_dir(CgDev d, int it)
 #pragma unroll
 		for (int u = 0; u < U; u++) {
 			if (!hole[u]) continue;
-			d.p[q0 + u * S] = rv[u] + beta * pv[u];
+			const double pn = rv[u] + beta * pv[u];
+			d.p[q0 + u * S] = pn;
+			d.pf[q0 + u * S] = (float)pn;
 		}
 	}
 }
@@ -608,7 +613,7 @@ void launch_cg_setup(const CgDev &d, hipStream_t s)
 
 void launch_cg_iter(const CgDev &d, int it, hipStream_t s)
 {
-	if (d.bilap) k_cg_spmv<1, 2><<<d.nblk, CG_NT, 0, s>>>(d);
+	if (d.bilap) k_cg_spmv<1, 4><<<d.nblk, CG_NT, 0, s>>>(d);
 	else k_cg_spmv<0, 4><<<d.nblk, CG_NT, 0, s>>>(d);
 	k_cg_update<4><<<d.nblk, CG_NT, 0, s>>>(d, it);
 	k_cg_dir<4><<<d.nblk, CG_NT, 0, s>>>(d, it);
