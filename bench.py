@@ -37,14 +37,24 @@ PEAK_HBM_GBPS = 8000.0
 
 # HBM bytes per launch of the roofline kernels at the 4096x4096x3 workload,
 # from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over
-# `bench.py --steps 1 --warmup 0 --no-cpu`, each scaled by what the same
-# counter reads for k_stream_copy moving 1 GiB each way in that run (FETCH_SIZE
-# x2.000, WRITE_SIZE x1.000).  Summary and commands: profiles/hbm_traffic_*.md.
+# `bench.py --steps 1 --warmup 0 --no-cpu --no-configs`, each scaled by what
+# the same counter reads for k_stream_copy moving 1 GiB each way in that run
+# (FETCH_SIZE x2.000, WRITE_SIZE x1.000).  Summary and commands:
+# profiles/hbm_traffic_*.md.  The figures hold for the kernel source they were
+# measured on: TRAFFIC_SRC is the SHA-256 prefix of imscript_amd/csrc/sp_cg.hip
+# then, and the line reports traffic null when the source differs.
 TRAFFIC_PMC = {
     "mg_up0": 644.7e6,
     "mg_down0": 495.5e6,
     "k32_dirspmv": 1026.9e6,
 }
+TRAFFIC_SRC = "unset"
+
+
+def kernel_source_id(name="sp_cg.hip"):
+    import hashlib
+    with open(os.path.join(ROOT, "imscript_amd", "csrc", name), "rb") as fh:
+        return hashlib.sha256(fh.read()).hexdigest()[:16]
 # k_plambda_lap on the 4096x4096x3 planes, from the same two passes over
 # `bench.py --config lap --no-cpu`, scaled by k_stream_copy moving the same
 # 201 MB each way in that run (x2.000, x1.000): 221.0 MB read (the halo rows
@@ -94,6 +104,16 @@ def make_inputs(torch, n, pd, seed, dev):
     return g, f, float(holes.float().mean())
 
 
+def cpu_model():
+    """The host CPU's model name and logical CPU count (/proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            names = [ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")]
+        return f"{names[0]} ({len(names)} logical CPUs)" if names else "unknown"
+    except OSError:
+        return "unknown"
+
+
 def cpu_baseline(g, f, rel, side, params):
     """Reference CPU path on a bounded sample: channel 0, top-left side^2
     crop, same flags and residual target, one core."""
@@ -131,7 +151,7 @@ def cpu_baseline(g, f, rel, side, params):
               "rel_residual": res}
     if kind != "reference":
         return {"value": single["Mpix_s"], "unit": "Mpix/s", "cores": 1,
-                "kind": kind, "single_thread": single,
+                "cpu": cpu_model(), "kind": kind, "single_thread": single,
                 "sample": f"channel 0, top-left {side}x{side} crop, CPU port of "
                           f"simpois -n 10 -c -1 with CG_EPS = {rel:g} * ||b|| of "
                           "the crop, single thread"}
@@ -163,7 +183,8 @@ def cpu_baseline(g, f, rel, side, params):
     if any(rcs):
         raise RuntimeError(f"reference processes failed: {rcs}")
     return {"value": round(len(crops) * side * side / dtp / 1e6, 4), "unit": "Mpix/s",
-            "cores": len(crops), "kind": kind, "seconds": round(dtp, 3),
+            "cores": len(crops), "cpu": cpu_model(), "kind": kind,
+            "seconds": round(dtp, 3),
             "single_thread": single,
             "sample": f"{len(crops)} concurrent single-threaded reference processes, "
                       f"each on a distinct {side}x{side} crop of the input, simpois "
@@ -334,13 +355,11 @@ def sinusoids_np(rng, h, w, k=8, pmin=16, pmax=256):
     return (255 * img / img.max()).astype(np.float32)
 
 
-def run_c1(a):
-    """512x512 Laplace fill, 30% disc holes, simpois -n 2000 (bit-exact
-    Gauss-Seidel on every pyramid level)."""
-    import torch
-    import imscript_amd as sp
-    rng = np.random.default_rng(a.seed)
-    n = 512
+def c1_input(seed, n=512):
+    """The 512x512 Laplace workload: a sum of 8 seeded sinusoids in [0, 255]
+    with 40 discs (radii 5-40 px, scaled together until 29-31 % of the
+    pixels are covered) set to NaN.  Returns (g, hole fraction)."""
+    rng = np.random.default_rng(seed)
     g = sinusoids_np(rng, n, n)
     cy, cx = rng.uniform(0, n, 40), rng.uniform(0, n, 40)
     r0 = rng.uniform(5, 40, 40)
@@ -358,6 +377,16 @@ def run_c1(a):
         else:
             break
     g[m] = np.nan
+    return g, float(m.mean())
+
+
+def run_c1(a):
+    """512x512 Laplace fill, 30% disc holes, simpois -n 2000 (bit-exact
+    Gauss-Seidel on every pyramid level)."""
+    import torch
+    import imscript_amd as sp
+    n = 512
+    g, frac = c1_input(a.seed, n)
     dg = torch.from_numpy(g).cuda()
     out = torch.empty_like(dg)
     p = dict(tstep=0.25, niter=2000, scale=99, cgit=0, trace=0,
@@ -371,12 +400,12 @@ def run_c1(a):
     dt = (time.perf_counter() - t0) / a.steps
     got = out.cpu().numpy()
     res = {"config": "c1", "workload": "512x512 Laplace, 30% disc holes, simpois -n 2000",
-           "hole_fraction": round(float(m.mean()), 4), "gpu_ms": round(dt * 1e3, 2),
+           "hole_fraction": round(frac, 4), "gpu_ms": round(dt * 1e3, 2),
            "gpu_Mpix_s": round(n * n / dt / 1e6, 3),
            "rel_residual": sp.residual_device(out, dg)[0]}
     if not a.no_cpu:
         res.update(cpu_leg_c1(g, got))
-    emit(res)
+    return res
 
 
 def grf(rng, n, power=3.0):
@@ -439,7 +468,7 @@ def run_c3(a):
            "cg_iters_finest": st.cg_iters_finest, "rel_residual": rel}
     if not a.no_cpu:
         res["fp64_check"] = cpu_leg_c3(torch, sp, a)
-    emit(res)
+    return res
 
 
 def bnorm_bilap(dg):
@@ -511,39 +540,53 @@ def run_c4(a):
         _, st = sp.solve_batch_device(gs, outs=outs, **p)
     torch.cuda.synchronize()
     tb = (time.perf_counter() - t0) / a.steps
-    relb = [sp.residual_device(o, g)[0] for o, g in zip(outs, gs)]
-    # plain loop: one solve call per image
-    t0 = time.perf_counter()
-    for g, o in zip(gs, outs):
+    _, _, ps = sp.solve_batch_device(gs, outs=outs, plane_stats=True, **p)
+    relb = [q.rel_residual for q in ps]
+    status = sorted(set(sp.PLANE_STATUS_NAMES[q.status] for q in ps))
+    # plain loop: one solve call per image, each with its own cached plan
+    # (the cache is widened to hold all of them), warmed once like the
+    # batch and timed over the same number of steps
+    sp.plan_cache_limit(4 * len(gs) + 8, 128 << 30)
+    outl = [torch.empty_like(g) for g in gs]
+    for g, o in zip(gs, outl):
         sp.solve_device(g, out=o, **p)
     torch.cuda.synchronize()
-    tl = time.perf_counter() - t0
-    rell = [sp.residual_device(o, g)[0] for o, g in zip(outs, gs)]
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        for g, o in zip(gs, outl):
+            sp.solve_device(g, out=o, **p)
+    torch.cuda.synchronize()
+    tl = (time.perf_counter() - t0) / a.steps
+    rell = [sp.residual_device(o, g)[0] for o, g in zip(outl, gs)]
+    same = all(torch.equal(x.view(torch.int32), y.view(torch.int32))
+               for x, y in zip(outs, outl))
+    del outl
+    sp.plan_cache_clear()
+    sp.plan_cache_limit()
     res = {"config": "c4", "workload": f"{len(gs)} images, {mpix:.1f} Mpx total, "
            "log-uniform 128^2-2048^2, aspect 1-4, 10-60% holes, rel residual 1e-5",
            "method": "multigrid-preconditioned FP32 CG per image",
            "batched_s": round(tb, 4), "batched_Mpix_s": round(mpix / tb, 1),
            "loop_s": round(tl, 4), "loop_Mpix_s": round(mpix / tl, 1),
            "speedup_batched_vs_loop": round(tl / tb, 2),
+           "loop": "one sp_solve_device call per image, plans cached (warm), "
+                   f"timed over {a.steps} passes like the batch",
+           "loop_bits_equal_batch": bool(same),
+           "plane_status": status,
            "max_rel_residual_batched": max(relb), "max_rel_residual_loop": max(rell),
            "hole_fraction_mean": round(float(np.mean(fr)), 3)}
     if not a.no_cpu:
-        sp.solve_batch_device(gs, outs=outs, **p)   # the batched results again
-        torch.cuda.synchronize()
         res["fp64_check"] = cpu_leg_c4(gs, outs)
-    emit(res)
+    return res
 
 
-def run_c5(a):
-    """Batch 64 x 1024x1024, 35% disc holes: forward Poisson fill, loss on
-    the filled pixels, backward by the adjoint solve."""
-    import torch
-    from imscript_amd.autograd import poisson_fill
-    rng = np.random.default_rng(a.seed + 5)
-    B, n = a.c5_batch, a.c5_size
-    dev = "cuda:0"
+def c5_inputs(torch, seed, B, n, dev):
+    """The differentiable-fill workload: B seeded sinusoid images n x n in
+    [0, 255] with random discs (radii 5-60 px) set to NaN until 35 % of each
+    image; guidance f ~ N(0, 1); target ~ U[0, 255].  Returns (g, f, target)."""
+    rng = np.random.default_rng(seed + 5)
     gen = torch.Generator(device=dev)
-    gen.manual_seed(a.seed)
+    gen.manual_seed(seed)
     yy = torch.arange(n, device=dev, dtype=torch.float32).view(n, 1)
     xx = torch.arange(n, device=dev, dtype=torch.float32).view(1, n)
     g = torch.empty(B, n, n, device=dev)
@@ -562,6 +605,17 @@ def run_c5(a):
         g[k] = img
     f = torch.randn(B, n, n, device=dev, generator=gen)
     target = torch.rand(B, n, n, device=dev, generator=gen) * 255
+    return g, f, target
+
+
+def run_c5(a):
+    """Batch 64 x 1024x1024, 35% disc holes: forward Poisson fill, loss on
+    the filled pixels, backward by the adjoint solve."""
+    import torch
+    from imscript_amd.autograd import poisson_fill
+    B, n = a.c5_batch, a.c5_size
+    dev = "cuda:0"
+    g, f, target = c5_inputs(torch, a.seed, B, n, dev)
     # loss on the filled pixels through a 0/1 weight (the same loss and
     # gradient as indexing by the mask, without its device-to-host sync)
     holef = torch.isnan(g).to(torch.float32)
@@ -586,7 +640,7 @@ def run_c5(a):
            "step_s": round(dt, 4), "Mpix_s_fwd_bwd": round(B * n * n / dt / 1e6, 1)}
     if not a.no_cpu:
         res.update(cpu_leg_c5(torch, poisson_fill, dev))
-    emit(res)
+    return res
 
 
 def run_lap(a):
@@ -915,18 +969,51 @@ def main():
     ap.add_argument("--c4-count", type=int, default=256)
     ap.add_argument("--c5-batch", type=int, default=64)
     ap.add_argument("--c5-size", type=int, default=1024)
+    ap.add_argument("--no-configs", action="store_true",
+                    help="the 4096x4096x3 line alone, without the summaries of "
+                         "the other workloads (c1, c3, c4, c5, gs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this path runs on one GPU; --gpus {a.gpus} is not "
               "supported (see DESIGN.md for how it would shard)", file=sys.stderr)
         sys.exit(2)
-    if a.config != "c2":
-        {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5,
-         "lap": run_lap, "amle": run_amle, "heatd": run_heatd}[a.config](a)
+    if a.config in ("c1", "c3", "c4", "c5"):
+        emit({"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5}[a.config](a))
         return
+    if a.config != "c2":
+        {"lap": run_lap, "amle": run_amle, "heatd": run_heatd}[a.config](a)
+        return
+    result, ok, rel = run_c2(a, a.method, full=True)
+    if not a.no_configs and a.method == "mgpcg1":
+        # the other workloads, each timed over the same steps after its own
+        # warm-up, summarized in this one line
+        cfg = {}
+        for name, fn in (("c1", run_c1), ("c3", run_c3), ("c4", run_c4), ("c5", run_c5)):
+            t0 = time.perf_counter()
+            cfg[name] = fn(a)
+            cfg[name]["wall_s"] = round(time.perf_counter() - t0, 1)
+            print(f"bench.py: {name} done in {cfg[name]['wall_s']} s", file=sys.stderr,
+                  flush=True)
+        t0 = time.perf_counter()
+        gs_line, _, _ = run_c2(a, "gs", full=False)
+        cfg["gs"] = {k: gs_line[k] for k in ("metric", "value", "unit", "ms_per_step")}
+        cfg["gs"]["roofline"] = gs_line.get("roofline")
+        cfg["gs"]["workload"] = "the 4096x4096x3 input, simpois -n 256 (parity Gauss-Seidel)"
+        cfg["gs"]["wall_s"] = round(time.perf_counter() - t0, 1)
+        result["configs"] = cfg
+    print(json.dumps(result), flush=True)
+    if not ok:
+        print(f"bench.py: residual target missed: {rel}", file=sys.stderr)
 
+
+def run_c2(a, method, full):
+    """The 4096x4096x3 Poisson fill with guidance by `method`; full adds the
+    copy ceiling, the host-buffer leg, the CPU baseline and the FP64 check.
+    Returns (JSON dict, residual target met, residuals)."""
     import torch
     import imscript_amd as sp
+    a = argparse.Namespace(**vars(a))
+    a.method = method
     dev = "cuda:0"
     g, f, hole_frac = make_inputs(torch, a.size, a.channels, a.seed, dev)
     out = torch.empty_like(g)
@@ -1051,7 +1138,9 @@ def main():
             "peak": PEAK_HBM_GBPS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBPS, 4),
             "traffic": (TRAFFIC_PMC.get(dom) if a.size == 4096 and a.channels == 3
-                        else None),
+                        and kernel_source_id() == TRAFFIC_SRC else None),
+            "traffic_source": f"rocprofv3 --pmc on sp_cg.hip {TRAFFIC_SRC} "
+                              f"(this build: {kernel_source_id()})",
             "algorithmic_bytes_per_launch": by / max(n, 1),
             "avg_launch_us": round(ms * 1e3 / max(n, 1), 2), "launches": n}
         result["kernels"] = rep
@@ -1086,6 +1175,9 @@ def main():
             "sweep_GBps_effective": round(eff, 1),
             "sweep_frac_effective": round(eff / PEAK_HBM_GBPS, 4)}
         result["autotune"] = tune
+    if not full:
+        del g, f, out
+        return result, ok, rel
     result["stream_copy"] = stream_copy_ceiling(torch, sp)
     # the drop-in boundary takes host buffers: one solve through it, copies
     # to and from the device included (reported beside `value`, never as it)
@@ -1101,9 +1193,8 @@ def main():
     if not a.no_cpu:
         result["cpu_baseline"] = cpu_baseline(g, f_term, a.cg_rel, a.cpu_side, params)
         result["config"]["fp64_check"] = fp64_check(torch, sp, params, a.seed)
-    print(json.dumps(result), flush=True)
-    if not ok:
-        print(f"bench.py: residual target missed: {rel}", file=sys.stderr)
+    del g, f, out
+    return result, ok, rel
 
 
 if __name__ == "__main__":

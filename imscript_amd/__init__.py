@@ -14,16 +14,23 @@ Entry points mirror the reference (``simpois.c:288``, ``:247``,
 plus array-level helpers (:func:`simpois`) and device-tensor helpers.
 """
 from ._lib import (lib, lib_path, SimpoisError, sp_params, sp_stats,
-                   sp_image, default_params, exported_symbols)
+                   sp_image, sp_plane_stats, default_params, exported_symbols,
+                   PLANE_SWEPT, PLANE_CONVERGED, PLANE_MAXIT, PLANE_BREAKDOWN,
+                   PLANE_NONFINITE, PLANE_STATUS_NAMES)
 from .api import (poisson_solver_separable, poisson_rec,
                   poisson_extension_by_cg, biharmonic_extension_by_cg,
                   simpois, solve_device, solve_batch_device, residual_device,
                   profile, profile_read, autotune_report, stream_copy,
-                  plambda_lap, amle, heatd)
+                  plambda_lap, amle, heatd, plan_cache_limit,
+                  plan_cache_clear, set_alloc_cap, device_bytes, last_cg_trace)
 
 __all__ = ["lib", "lib_path", "SimpoisError", "sp_params", "sp_stats",
            "sp_image", "default_params", "exported_symbols",
            "poisson_solver_separable", "poisson_rec",
            "poisson_extension_by_cg", "biharmonic_extension_by_cg", "simpois",
            "solve_device", "solve_batch_device", "residual_device", "profile",
-           "profile_read", "autotune_report", "stream_copy", "plambda_lap", "amle", "heatd"]
+           "profile_read", "autotune_report", "stream_copy", "plambda_lap", "amle", "heatd",
+           "sp_plane_stats", "PLANE_SWEPT", "PLANE_CONVERGED", "PLANE_MAXIT",
+           "PLANE_BREAKDOWN", "PLANE_NONFINITE", "PLANE_STATUS_NAMES",
+           "plan_cache_limit", "plan_cache_clear", "set_alloc_cap", "device_bytes",
+           "last_cg_trace"]

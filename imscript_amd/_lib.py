@@ -32,6 +32,18 @@ class sp_stats(ctypes.Structure):
                 ("elapsed_ms", ctypes.c_double)]
 
 
+class sp_plane_stats(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("levels", ctypes.c_int),
+                ("holes", ctypes.c_int), ("sweeps", ctypes.c_int),
+                ("cg_iters", ctypes.c_int), ("cg_rr", ctypes.c_double),
+                ("rel_residual", ctypes.c_double)]
+
+
+# sp_plane_stats.status
+PLANE_SWEPT, PLANE_CONVERGED, PLANE_MAXIT, PLANE_BREAKDOWN, PLANE_NONFINITE = range(5)
+PLANE_STATUS_NAMES = ("swept", "converged", "maxit", "breakdown", "nonfinite")
+
+
 _P = ctypes.c_void_p
 _F = ctypes.c_float
 _I = ctypes.c_int
@@ -65,6 +77,14 @@ _SIGNATURES = {
                                 ctypes.POINTER(ctypes.c_double), _P]),
     "sp_solve_batch_device": (_I, [_I, _P, ctypes.POINTER(sp_params), _P,
                                    ctypes.POINTER(sp_stats)]),
+    "sp_solve_batch_device_ex": (_I, [_I, _P, ctypes.POINTER(sp_params), _P,
+                                      ctypes.POINTER(sp_stats), _P]),
+    "sp_last_cg_trace": (_I, [_I, _I, ctypes.POINTER(_I),
+                              ctypes.POINTER(ctypes.c_double), _I]),
+    "sp_plan_cache_limit": (None, [_I, ctypes.c_size_t]),
+    "sp_plan_cache_clear": (None, []),
+    "sp_set_alloc_cap": (ctypes.c_size_t, [ctypes.c_size_t]),
+    "sp_device_bytes": (ctypes.c_size_t, []),
     "sp_adjoint_batch_device": (_I, [_I, _P, ctypes.POINTER(sp_params), _P,
                                      ctypes.POINTER(sp_stats)]),
     "sp_profile": (None, [_I]),

@@ -3,7 +3,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import lib, check, default_params, sp_stats, sp_image
+from ._lib import lib, check, default_params, sp_stats, sp_image, sp_plane_stats
 
 
 def _f32(a, name):
@@ -124,11 +124,13 @@ def residual_device(u, g, f=None, bilap=False, stream=None):
     return list(rel)
 
 
-def solve_batch_device(gs, fs=None, outs=None, stream=None, **params):
+def solve_batch_device(gs, fs=None, outs=None, stream=None, plane_stats=False,
+                       **params):
     """Batched solve of device tensors of uneven shapes ((h,w) or (pd,h,w)),
     one pyramid per plane, all planes advancing together.  ``params`` are
     sp_params fields (tstep, niter, scale, cgit, cg_fp32, cg_rel, ...).
-    Returns (outs, sp_stats)."""
+    Returns (outs, sp_stats), or (outs, sp_stats, [sp_plane_stats per
+    plane]) with ``plane_stats=True`` (sp_solve_batch_device_ex)."""
     import torch
     n = len(gs)
     fs = fs if fs is not None else [None] * n
@@ -145,10 +147,50 @@ def solve_batch_device(gs, fs=None, outs=None, stream=None, **params):
     params.setdefault("trace", 0)
     p = default_params(**params)
     st = sp_stats()
-    check(lib.sp_solve_batch_device(n, arr, ctypes.byref(p),
-                                    ctypes.c_void_p(s.cuda_stream),
-                                    ctypes.byref(st)))
-    return outs, st
+    if not plane_stats:
+        check(lib.sp_solve_batch_device(n, arr, ctypes.byref(p),
+                                        ctypes.c_void_p(s.cuda_stream),
+                                        ctypes.byref(st)))
+        return outs, st
+    npl = sum(1 if g.dim() == 2 else g.shape[0] for g in gs)
+    ps = (sp_plane_stats * npl)()
+    check(lib.sp_solve_batch_device_ex(n, arr, ctypes.byref(p),
+                                       ctypes.c_void_p(s.cuda_stream),
+                                       ctypes.byref(st), ps))
+    return outs, st, list(ps)
+
+
+def last_cg_trace(plane=0, level=0):
+    """(unknowns, [r.r after each CG iteration]) of the last solve at one
+    plane and pyramid level (0 = finest), or None if it had no such level."""
+    n = ctypes.c_int()
+    k = lib.sp_last_cg_trace(plane, level, ctypes.byref(n), None, 0)
+    if k < 0:
+        return None
+    buf = (ctypes.c_double * max(k, 1))()
+    lib.sp_last_cg_trace(plane, level, ctypes.byref(n), buf, k)
+    return n.value, list(buf[:k])
+
+
+def plan_cache_limit(max_plans=256, max_bytes=16 << 30):
+    """Bound the solver's plan cache (least recently used plans go first)."""
+    lib.sp_plan_cache_limit(int(max_plans), int(max_bytes))
+
+
+def plan_cache_clear():
+    """Free every cached solve plan."""
+    lib.sp_plan_cache_clear()
+
+
+def set_alloc_cap(nbytes):
+    """Cap the device bytes the library holds (0: none); returns the old cap.
+    Allocations beyond it fail with SP_ERR_NOMEM (SimpoisError)."""
+    return lib.sp_set_alloc_cap(int(nbytes))
+
+
+def device_bytes():
+    """Device bytes the library holds now (cached plans included)."""
+    return lib.sp_device_bytes()
 
 
 def profile(on=True):

@@ -45,11 +45,29 @@ def _params(tol, niter, scale):
                           cg_fp32=1, cg_rel=tol, cg_mg=1, trace=0)
 
 
+def _check(g, f):
+    """The checks of api.solve_device, before anything reaches a kernel: a
+    host tensor's address or a wrong dtype or shape would otherwise be read
+    as device float32 planes."""
+    if not isinstance(g, torch.Tensor) or g.dtype != torch.float32 or not g.is_cuda:
+        raise TypeError("g must be a float32 device tensor")
+    if g.dim() != 3:
+        raise TypeError("g must have shape (B, H, W)")
+    if f is not None:
+        if not isinstance(f, torch.Tensor) or f.dtype != torch.float32 or not f.is_cuda:
+            raise TypeError("f must be a float32 device tensor")
+        if f.device != g.device:
+            raise TypeError("f must be on g's device")
+        if f.shape != g.shape:
+            raise TypeError("f must have g's shape")
+
+
 class PoissonFill(torch.autograd.Function):
     """u = fill(g, f) for planar float32 device tensors of shape (B, H, W)."""
 
     @staticmethod
     def forward(ctx, g, f, tol, niter, scale):
+        _check(g, f)
         g = g.contiguous()
         f = None if f is None else f.contiguous()
         out = torch.empty_like(g)
@@ -82,9 +100,10 @@ def poisson_fill(g, f=None, tol=1e-6, niter=0, scale=1):
     """Differentiable fill of the NaN pixels of g ((H,W) or (B,H,W) float32
     on the GPU) with optional guidance f; gradients flow to the known pixels
     of g and to f."""
-    squeeze = g.dim() == 2
+    squeeze = isinstance(g, torch.Tensor) and g.dim() == 2
     if squeeze:
         g = g[None]
-        f = None if f is None else f[None]
+        f = f[None] if isinstance(f, torch.Tensor) else f
+    _check(g, f)
     u = PoissonFill.apply(g, f, tol, niter, scale)
     return u[0] if squeeze else u

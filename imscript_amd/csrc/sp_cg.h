@@ -36,6 +36,8 @@ struct Cg32Scal {
 	                   // adds alpha p to x before it replaces p
 	int fin;           // this iteration stopped the plane: the fused
 	                   // direction kernel writes the solution to u
+	int status;        // why the plane stopped: 0 still running, else
+	                   // SP_PLANE_CONVERGED / _MAXIT / _BREAKDOWN / _NONFINITE
 };
 
 // Fused multigrid tile kernels: levels 0 and 1 run one of the fine tile

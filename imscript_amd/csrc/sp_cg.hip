@@ -16,6 +16,7 @@
 // Planes that have converged are skipped by all later launches.
 #include "sp_internal.h"
 #include "sp_cg.h"
+#include "../../include/simpois_hip.h"
 
 namespace spk {
 
@@ -319,7 +320,14 @@ __global__ __launch_bounds__(RT) void k32_reduce0(const Cg32Plane *pl, Cg32Scal 
 		s.iters = 0;
 		const int n = *P.cnt;
 		if (s.maxit_from_count) s.maxit = (int)(float)n;
-		s.active = (s.maxit > 0 && n > 0) ? 1 : 0;
+		// a plane stops before its first iteration when it has nothing to
+		// solve, no iteration allowed, a residual that is not finite (NaN
+		// or Inf in the data its system reads) or exactly zero (x0 solves
+		// it; the reference's 0/0, minicg.c:40-42, is not reproduced here)
+		s.status = n <= 0 || rr == 0.0 ? SP_PLANE_CONVERGED
+			: !isfinite(rr) ? SP_PLANE_NONFINITE
+			: s.maxit <= 0 ? SP_PLANE_MAXIT : 0;
+		s.active = s.status == 0;
 		if (s.active) atomicAdd(nactive, 1);
 	}
 }
@@ -423,6 +431,7 @@ __global__ __launch_bounds__(RT) void k32_reduce_a(const Cg32Plane *pl, Cg32Scal
 			s.upd = 0;
 			s.fin = 1;
 			s.active = 0;
+			s.status = isfinite(pq) ? SP_PLANE_BREAKDOWN : SP_PLANE_NONFINITE;
 			atomicSub(nactive, 1);
 		} else {
 			s.alpha = s.rz / pq;
@@ -486,7 +495,11 @@ __global__ __launch_bounds__(RT) void k32_reduce_b(const Cg32Plane *pl, Cg32Scal
 			s.rz = rr1;
 		}
 		s.rr = rr1;
-		if (sqrt(rr1) < s.stop || s.iters >= s.maxit) {
+		const int st = sqrt(rr1) < s.stop ? SP_PLANE_CONVERGED
+			: !isfinite(rr1) ? SP_PLANE_NONFINITE
+			: s.iters >= s.maxit ? SP_PLANE_MAXIT : 0;
+		if (st) {
+			s.status = st;
 			s.active = 0;
 			s.fin = 1;
 			atomicSub(nactive, 1);

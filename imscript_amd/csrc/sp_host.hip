@@ -16,11 +16,13 @@
 #include "../../include/simpois_hip.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <list>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -264,6 +266,42 @@ extern "C" int sp_profile_read(const char *name, double *ms, long *launches,
 	return SP_OK;
 }
 
+// ------------------------------------------------ device memory account
+//
+// Every device allocation of the library goes through dev_alloc / dev_free,
+// which keep the bytes held (cached plans included) and refuse one that
+// would take them over the cap of sp_set_alloc_cap (0: no cap) with
+// SP_ERR_NOMEM, as a failed hipMalloc returns.
+
+static std::atomic<size_t> g_live_bytes{0};
+static std::atomic<size_t> g_alloc_cap{0};
+
+static int dev_alloc(void **p, size_t bytes)
+{
+	*p = nullptr;
+	const size_t cap = g_alloc_cap.load(), live = g_live_bytes.load();
+	if (cap && live + bytes > cap)
+		return set_err(SP_ERR_NOMEM, "device allocation of %zu bytes refused: the "
+				"library holds %zu of its %zu-byte cap", bytes, live, cap);
+	HIPCHK(hipMalloc(p, bytes));
+	g_live_bytes += bytes;
+	return SP_OK;
+}
+
+static void dev_free(void *p, size_t bytes)
+{
+	if (!p) return;
+	(void)hipFree(p);
+	g_live_bytes -= bytes;
+}
+
+extern "C" size_t sp_set_alloc_cap(size_t bytes)
+{
+	return g_alloc_cap.exchange(bytes);
+}
+
+extern "C" size_t sp_device_bytes(void) { return g_live_bytes.load(); }
+
 // --------------------------------------------------------------- plan
 
 struct PlaneIn {            // one plane of one image (device pointers)
@@ -317,7 +355,8 @@ struct Plan {
 	std::vector<PlaneIn> planes;
 	std::vector<int> depth;          // pyramid levels of each plane
 	std::vector<LevelPlan> lv;
-	std::vector<void *> allocs;
+	std::vector<std::pair<void *, size_t>> allocs;
+	size_t bytes = 0;                // device bytes held by allocs
 	int *d_cnt = nullptr;
 	float **d_outs = nullptr;
 	// FP64 CG scratch, sized for the largest plane
@@ -338,6 +377,10 @@ struct Plan {
 	size_t cg32_cap = 0;
 	std::vector<std::vector<CgTrace>> traces;   // [plane][level]
 	std::vector<int> cg_iters_finest;
+	std::vector<int> status_finest;     // SP_PLANE_* of each plane's finest level
+	std::vector<double> rr_finest;      // last r.r of that CG (-1: none)
+	// FP64 residual of the output, per plane (sp_solve_batch_device_ex)
+	double *res_part = nullptr, *res_out = nullptr;
 	hipStream_t cap_stream = nullptr;
 
 	std::vector<char> key;           // identity of the call this plan serves
@@ -350,7 +393,7 @@ struct Plan {
 	{
 		for (auto &L : lv)
 			if (L.cg.exec) (void)hipGraphExecDestroy(L.cg.exec);
-		for (void *p : allocs) (void)hipFree(p);
+		for (auto &a : allocs) dev_free(a.first, a.second);
 		if (h_done) (void)hipHostFree(h_done);
 		if (h_stage) (void)hipHostFree(h_stage);
 		if (cap_stream) (void)hipStreamDestroy(cap_stream);
@@ -370,8 +413,10 @@ struct Plan {
 	template <typename T> int alloc(T **p, size_t n)
 	{
 		void *q = nullptr;
-		HIPCHK(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
-		allocs.push_back(q);
+		const size_t by = std::max<size_t>(n, 1) * sizeof(T);
+		TRY(dev_alloc(&q, by));
+		allocs.push_back({q, by});
+		bytes += by;
 		*p = (T *)q;
 		return SP_OK;
 	}
@@ -622,6 +667,8 @@ static int plan_create(Plan &pl, const std::vector<PlaneIn> &planes,
 	HIPCHK(hipHostMalloc((void **)&pl.h_done, 4 * sizeof(int)));
 	pl.traces.assign(np, std::vector<CgTrace>(nlev));
 	pl.cg_iters_finest.assign(np, 0);
+	pl.status_finest.assign(np, SP_PLANE_SWEPT);
+	pl.rr_finest.assign(np, -1.0);
 	{
 		std::vector<float *> outs(np);
 		for (int p = 0; p < np; p++) outs[p] = pl.planes[p].out;
@@ -791,10 +838,13 @@ static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 	int nmask = 0;
 	HIPCHK(hipMemcpyAsync(&nmask, S.cnt, sizeof(int), hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
+	const int plane = L.members[m];
+	if (k == 0) { pl.status_finest[plane] = SP_PLANE_CONVERGED; pl.rr_finest[plane] = 0; }
 	if (nmask == 0) return SP_OK;   // :203-207, the level output rule covers it
 	const float e = pl.P.cg_eps >= 0 ? (float)pl.P.cg_eps : eps;
 	const float fm = cgit == -1 ? (float)nmask : (float)cgit;
 	const int maxit = (int)fm;
+	if (k == 0) pl.status_finest[plane] = SP_PLANE_MAXIT;
 	if (maxit <= 0) return SP_OK;   // x = x0: u is unchanged
 	const size_t n = (size_t)S.w * S.h;
 	const int nblk = (int)std::min<size_t>(1024, std::max<size_t>(1, (n + 2047) / 2048));
@@ -830,7 +880,14 @@ static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 	HIPCHK(hipMemcpyAsync(tr.rr.data(), pl.cg_rr + 1, ran * sizeof(double),
 			hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
-	if (k == 0) pl.cg_iters_finest[L.members[m]] = ran;
+	if (k == 0) {
+		pl.cg_iters_finest[plane] = ran;
+		const double rr = ran > 0 ? tr.rr[ran - 1] : 0.0;
+		pl.rr_finest[plane] = rr;
+		// the stop test of minicg.c:47 on the last r.r
+		pl.status_finest[plane] = !std::isfinite(rr) ? SP_PLANE_NONFINITE
+			: sqrt(rr) < (double)e ? SP_PLANE_CONVERGED : SP_PLANE_MAXIT;
+	}
 	return SP_OK;
 }
 
@@ -1167,7 +1224,11 @@ static int run_cg32(Plan &pl, int k, int bilap, float eps, hipStream_t s)
 		const int it = sc[z].iters;
 		tr.rr.assign(trace.begin() + sc[z].trace0,
 				trace.begin() + sc[z].trace0 + std::min(it, sc[z].trace_cap));
-		if (k == 0) pl.cg_iters_finest[p] = it;
+		if (k == 0) {
+			pl.cg_iters_finest[p] = it;
+			pl.status_finest[p] = sc[z].status;
+			pl.rr_finest[p] = sc[z].rr;
+		}
 		unk += (double)it * cnts[z];
 	}
 	// algorithmic bytes per unknown and iteration: spmv reads p, writes q
@@ -1178,6 +1239,23 @@ static int run_cg32(Plan &pl, int k, int bilap, float eps, hipStream_t s)
 }
 
 // -------------------------------------------------------------- solve
+
+// CG histories of the last solve in the process, [plane][level]
+static std::mutex g_trace_mu;
+static std::vector<std::vector<CgTrace>> g_last_traces;
+
+extern "C" int sp_last_cg_trace(int plane, int level, int *n, double *rr, int cap)
+{
+	std::lock_guard<std::mutex> lk(g_trace_mu);
+	if (plane < 0 || plane >= (int)g_last_traces.size() || level < 0 ||
+			level >= (int)g_last_traces[plane].size())
+		return -1;
+	const CgTrace &t = g_last_traces[plane][level];
+	if (n) *n = t.n;
+	const int k = (int)t.rr.size();
+	if (rr) for (int i = 0; i < std::min(k, cap); i++) rr[i] = t.rr[i];
+	return k;
+}
 
 static int plan_execute(Plan &pl, hipStream_t s, sp_stats *st)
 {
@@ -1237,6 +1315,10 @@ static int plan_execute(Plan &pl, hipStream_t s, sp_stats *st)
 		st->cg_iters_finest = mx;
 		st->elapsed_ms = ms;
 	}
+	{
+		std::lock_guard<std::mutex> lk(g_trace_mu);
+		g_last_traces = pl.traces;
+	}
 	if (pl.P.trace)   // reference order: plane by plane, coarse to fine
 		for (size_t p = 0; p < pl.planes.size(); p++)
 			for (int k = pl.depth[p] - 1; k >= 0; k--) {
@@ -1256,8 +1338,121 @@ static int check_plane_args(const void *out, const void *in, int w, int h, int p
 	return SP_OK;
 }
 
-extern "C" int sp_solve_batch_device(int n, const sp_image *imgs,
-		const sp_params *p, void *stream, sp_stats *st)
+// ------------------------------------------------------------ plan cache
+//
+// Plans are kept by call signature (plane sizes, buffer addresses,
+// parameters), most recently used first, within a count and a byte budget;
+// repeated solves of the same call skip allocation, tiling and graph
+// instantiation.  A call holds the cache's lock while it runs.
+
+static std::mutex g_cache_mu;
+static std::list<Plan *> g_cache;
+static int g_cache_max = 256;
+static size_t g_cache_max_bytes = (size_t)16 << 30;
+
+// drop least recently used plans (never `keep`) until the cache fits
+static void cache_trim(const Plan *keep, int max_plans, size_t max_bytes)
+{
+	size_t bytes = 0;
+	for (const Plan *q : g_cache) bytes += q->bytes;
+	for (auto it = g_cache.end(); it != g_cache.begin() &&
+			((int)g_cache.size() > max_plans || bytes > max_bytes);) {
+		--it;
+		if (*it == keep) continue;
+		bytes -= (*it)->bytes;
+		delete *it;
+		it = g_cache.erase(it);
+	}
+}
+
+extern "C" void sp_plan_cache_limit(int max_plans, size_t max_bytes)
+{
+	std::lock_guard<std::mutex> lk(g_cache_mu);
+	g_cache_max = std::max(max_plans, 1);
+	g_cache_max_bytes = max_bytes;
+	cache_trim(nullptr, g_cache_max, g_cache_max_bytes);
+}
+
+extern "C" void sp_plan_cache_clear(void)
+{
+	std::lock_guard<std::mutex> lk(g_cache_mu);
+	cache_trim(nullptr, 0, 0);
+}
+
+// the cached plan of a call (created on a miss); the caller holds g_cache_mu
+static int cache_get(const std::vector<PlaneIn> &planes, const sp_params &P,
+		bool has_f, Plan **out)
+{
+	std::vector<char> key;
+	auto put = [&](const void *v, size_t n) {
+		key.insert(key.end(), (const char *)v, (const char *)v + n);
+	};
+	for (const PlaneIn &q : planes) {
+		put(&q.w, sizeof q.w); put(&q.h, sizeof q.h);
+		put(&q.g, sizeof q.g); put(&q.f, sizeof q.f); put(&q.out, sizeof q.out);
+	}
+	put(&P.tstep, sizeof P.tstep); put(&P.niter, sizeof P.niter);
+	put(&P.scale, sizeof P.scale); put(&P.cgit, sizeof P.cgit);
+	put(&P.pmsfac, sizeof P.pmsfac); put(&P.ponlit, sizeof P.ponlit);
+	put(&P.cg_eps, sizeof P.cg_eps); put(&P.trace, sizeof P.trace);
+	put(&P.cg_fp32, sizeof P.cg_fp32); put(&P.cg_rel, sizeof P.cg_rel);
+	put(&P.cg_chunk, sizeof P.cg_chunk); put(&P.cg_mg, sizeof P.cg_mg);
+	put(&P.autotune, sizeof P.autotune); put(&P.guidance, sizeof P.guidance);
+	put(&P.cg_rel_coarse, sizeof P.cg_rel_coarse);
+	for (auto it = g_cache.begin(); it != g_cache.end(); ++it)
+		if ((*it)->key == key) {
+			g_cache.splice(g_cache.begin(), g_cache, it);
+			*out = g_cache.front();
+			return SP_OK;
+		}
+	for (int attempt = 0;; attempt++) {
+		Plan *pl = new Plan;
+		const int rc = plan_create(*pl, planes, P, has_f);
+		if (rc == SP_OK) {
+			pl->key = key;
+			g_cache.push_front(pl);
+			*out = pl;
+			return SP_OK;
+		}
+		delete pl;
+		// out of device memory with plans cached: free them, try once more
+		if (rc != SP_ERR_NOMEM || attempt || g_cache.empty()) return rc;
+		cache_trim(nullptr, 0, 0);
+	}
+}
+
+// FP64 residual of each plane's output over its holes and the holes
+// themselves (finest level; its members are the planes in order)
+static int plane_residuals(Plan &pl, hipStream_t s, std::vector<double> &rel,
+		std::vector<int> &holes)
+{
+	LevelPlan &L0 = pl.lv[0];
+	const int np = (int)L0.members.size();
+	rel.assign(np, -1.0);
+	holes.assign(np, 0);
+	HIPCHK(hipMemcpyAsync(holes.data(), L0.sp[0].cnt, np * sizeof(int),
+			hipMemcpyDeviceToHost, s));
+	if (pl.R != 3) {   // (no CG system of the triharmonic operator)
+		if (!pl.res_part) {
+			TRY(pl.alloc(&pl.res_part, 2 * (size_t)L0.nbw));
+			TRY(pl.alloc(&pl.res_out, 3 * (size_t)np));
+		}
+		spk::launch_residual(L0.d_sp, L0.d_bw, L0.nbw, pl.R == 2, pl.res_part, s);
+		spk::launch_residual_reduce(pl.res_part, L0.d_first, np, pl.res_out, s);
+		HIPCHK(hipGetLastError());
+	}
+	std::vector<double> h3(3 * (size_t)np);
+	if (pl.R != 3)
+		HIPCHK(hipMemcpyAsync(h3.data(), pl.res_out, h3.size() * sizeof(double),
+				hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
+	if (pl.R != 3)
+		for (int m = 0; m < np; m++) rel[m] = h3[3 * (size_t)m + 2];
+	return SP_OK;
+}
+
+extern "C" int sp_solve_batch_device_ex(int n, const sp_image *imgs,
+		const sp_params *p, void *stream, sp_stats *st, sp_plane_stats *ps)
 {
 	if (n < 1 || !imgs || !p) return set_err(SP_ERR_ARG, "empty batch");
 	TRY(ensure_device());
@@ -1272,37 +1467,35 @@ extern "C" int sp_solve_batch_device(int n, const sp_image *imgs,
 			planes.push_back({I.w, I.h, I.g + wh * l, I.f ? I.f + wh * l : nullptr,
 				I.out + wh * l});
 	}
-	// the last plan is kept and reused by a call with the same planes,
-	// pointers and parameters (repeated solves skip allocation, tiling and
-	// graph instantiation)
-	std::vector<char> key;
-	auto put = [&](const void *v, size_t n) {
-		key.insert(key.end(), (const char *)v, (const char *)v + n);
-	};
-	for (const PlaneIn &q : planes) {
-		put(&q.w, sizeof q.w); put(&q.h, sizeof q.h);
-		put(&q.g, sizeof q.g); put(&q.f, sizeof q.f); put(&q.out, sizeof q.out);
+	hipStream_t s = (hipStream_t)stream;
+	std::lock_guard<std::mutex> lk(g_cache_mu);
+	Plan *pl = nullptr;
+	TRY(cache_get(planes, *p, has_f, &pl));
+	const int rc = plan_execute(*pl, s, st);
+	if (rc == SP_OK && ps) {
+		std::vector<double> rel;
+		std::vector<int> holes;
+		TRY(plane_residuals(*pl, s, rel, holes));
+		for (size_t q = 0; q < planes.size(); q++) {
+			sp_plane_stats &o = ps[q];
+			o.status = pl->status_finest[q];
+			o.levels = pl->depth[q];
+			o.holes = holes[q];
+			o.sweeps = pl->lv[0].nsweep[q];
+			o.cg_iters = pl->cg_iters_finest[q];
+			o.cg_rr = pl->rr_finest[q];
+			o.rel_residual = rel[q];
+			if (pl->R != 3 && !std::isfinite(rel[q])) o.status = SP_PLANE_NONFINITE;
+		}
 	}
-	const sp_params &P = *p;
-	put(&P.tstep, sizeof P.tstep); put(&P.niter, sizeof P.niter);
-	put(&P.scale, sizeof P.scale); put(&P.cgit, sizeof P.cgit);
-	put(&P.pmsfac, sizeof P.pmsfac); put(&P.ponlit, sizeof P.ponlit);
-	put(&P.cg_eps, sizeof P.cg_eps); put(&P.trace, sizeof P.trace);
-	put(&P.cg_fp32, sizeof P.cg_fp32); put(&P.cg_rel, sizeof P.cg_rel);
-	put(&P.cg_chunk, sizeof P.cg_chunk); put(&P.cg_mg, sizeof P.cg_mg);
-	put(&P.autotune, sizeof P.autotune); put(&P.guidance, sizeof P.guidance);
-	put(&P.cg_rel_coarse, sizeof P.cg_rel_coarse);
-	static std::mutex cache_mu;
-	static Plan *cache = nullptr;
-	std::lock_guard<std::mutex> lk(cache_mu);
-	if (!cache || cache->key != key) {
-		delete cache;
-		cache = new Plan;
-		const int rc = plan_create(*cache, planes, *p, has_f);
-		if (rc) { delete cache; cache = nullptr; return rc; }
-		cache->key = key;
-	}
-	return plan_execute(*cache, (hipStream_t)stream, st);
+	cache_trim(pl, g_cache_max, g_cache_max_bytes);
+	return rc;
+}
+
+extern "C" int sp_solve_batch_device(int n, const sp_image *imgs,
+		const sp_params *p, void *stream, sp_stats *st)
+{
+	return sp_solve_batch_device_ex(n, imgs, p, stream, st, nullptr);
 }
 
 extern "C" int sp_solve_device(float *d_out, const float *d_in, const float *d_dat,
@@ -1343,6 +1536,21 @@ extern "C" int sp_solve_host(float *out, const float *in, const float *dat,
 
 // ------------------------------------------------------------- adjoint
 
+// The adjoint system's planes (g2, f2, lambda) live in scratch kept for the
+// last batch shape, so that repeated backward passes of one shape solve on
+// the same addresses and find their solve plan in the cache.
+struct AdjScratch {
+	std::vector<int> shape;          // w, h, pd of each image
+	Plan mem;                        // owns the scratch and the descriptors
+	float *buf = nullptr;
+	size_t total = 0;
+	AdjPlane *d_ap = nullptr;
+	BlockWork *d_bw = nullptr;
+	int nbw = 0;
+};
+static std::mutex g_adj_mu;
+static AdjScratch *g_adj = nullptr;
+
 extern "C" int sp_adjoint_batch_device(int n, const sp_adjoint_image *imgs,
 		const sp_params *p, void *stream, sp_stats *st)
 {
@@ -1355,43 +1563,61 @@ extern "C" int sp_adjoint_batch_device(int n, const sp_adjoint_image *imgs,
 				"(guidance = 0); differentiate plambda's Laplacian separately");
 	TRY(ensure_device());
 	hipStream_t s = (hipStream_t)stream;
-	Plan tmp;   // owns the scratch planes
+	std::vector<int> shape;
 	size_t total = 0;
 	for (int i = 0; i < n; i++) {
 		const sp_adjoint_image &I = imgs[i];
 		if (!I.g || !I.grad_u || !I.grad_g || I.w < 1 || I.h < 1 || I.pd < 1)
 			return set_err(SP_ERR_ARG, "bad adjoint image %d", i);
+		if ((long)I.w * I.h > (1L << 30))
+			return set_err(SP_ERR_ARG, "adjoint image %d too large", i);
 		total += (size_t)I.w * I.h * I.pd;
+		shape.insert(shape.end(), {I.w, I.h, I.pd});
 	}
-	float *scratch;
-	TRY(tmp.alloc(&scratch, 3 * total));
+	std::lock_guard<std::mutex> lk(g_adj_mu);
+	if (!g_adj || g_adj->shape != shape) {
+		delete g_adj;
+		g_adj = new AdjScratch;
+		g_adj->shape = shape;
+		g_adj->total = total;
+		std::vector<BlockWork> bw;
+		int id = 0;
+		for (int i = 0; i < n; i++)
+			for (int l = 0; l < imgs[i].pd; l++, id++)
+				for (size_t st2 = 0; st2 < (size_t)imgs[i].w * imgs[i].h; st2 += EW_SPAN)
+					bw.push_back({id, (int)st2});
+		int rc = g_adj->mem.alloc(&g_adj->buf, 3 * total);
+		if (!rc) rc = g_adj->mem.alloc(&g_adj->d_ap, (size_t)id);
+		if (!rc) rc = g_adj->mem.upload(&g_adj->d_bw, bw);
+		g_adj->nbw = (int)bw.size();
+		if (rc) { delete g_adj; g_adj = nullptr; return rc; }
+	}
+	AdjScratch &A = *g_adj;
 	std::vector<AdjPlane> ap;
-	std::vector<BlockWork> bw;
 	std::vector<sp_image> fwd(n);
 	size_t off = 0;
 	for (int i = 0; i < n; i++) {
 		const sp_adjoint_image &I = imgs[i];
 		const size_t wh = (size_t)I.w * I.h, m = wh * I.pd;
-		float *g2 = scratch + off, *f2 = scratch + total + off,
-			*lam = scratch + 2 * total + off;
+		float *g2 = A.buf + off, *f2 = A.buf + total + off, *lam = A.buf + 2 * total + off;
 		off += m;
 		fwd[i] = sp_image{I.w, I.h, I.pd, g2, f2, lam};
-		for (int l = 0; l < I.pd; l++) {
-			const int id = (int)ap.size();
+		for (int l = 0; l < I.pd; l++)
 			ap.push_back({I.g + wh * l, I.grad_u + wh * l, g2 + wh * l, f2 + wh * l,
 				lam + wh * l, I.grad_g + wh * l, I.grad_f ? I.grad_f + wh * l : nullptr,
 				I.w, I.h});
-			for (size_t st2 = 0; st2 < wh; st2 += EW_SPAN) bw.push_back({id, (int)st2});
-		}
 	}
-	AdjPlane *d_ap;
-	BlockWork *d_bw;
-	TRY(tmp.upload(&d_ap, ap));
-	TRY(tmp.upload(&d_bw, bw));
-	spk::launch_adj_prepare(d_ap, d_bw, (int)bw.size(), s);
+	// the descriptors are read by kernels of this stream only: copy in
+	// stream order (an earlier call's kernels may still read them)
+	TRY(A.mem.stage(ap.size() * sizeof(AdjPlane)));
+	HIPCHK(hipStreamSynchronize(s));
+	memcpy(A.mem.h_stage, ap.data(), ap.size() * sizeof(AdjPlane));
+	HIPCHK(hipMemcpyAsync(A.d_ap, A.mem.h_stage, ap.size() * sizeof(AdjPlane),
+			hipMemcpyHostToDevice, s));
+	spk::launch_adj_prepare(A.d_ap, A.d_bw, A.nbw, s);
 	HIPCHK(hipGetLastError());
 	TRY(sp_solve_batch_device(n, fwd.data(), p, stream, st));
-	spk::launch_adj_gather(d_ap, d_bw, (int)bw.size(), s);
+	spk::launch_adj_gather(A.d_ap, A.d_bw, A.nbw, s);
 	HIPCHK(hipGetLastError());
 	HIPCHK(hipStreamSynchronize(s));
 	return SP_OK;
@@ -1422,6 +1648,12 @@ struct AmlePlan {
 	void *part = nullptr;
 	const float *u_for = nullptr;     // output buffer the instances point at
 	std::vector<AmleSched> sched;
+	// the end of the last call's work on its stream: a call on another
+	// stream waits for it before it reuses the scratch
+	hipEvent_t done = nullptr;
+	hipStream_t last = nullptr;
+	bool used = false;
+	~AmlePlan() { if (done) (void)hipEventDestroy(done); }
 };
 
 static AmleSched *amle_sched(AmlePlan &A, int T)
@@ -1452,20 +1684,31 @@ static int amle_build(AmlePlan &A, int T)
 	return SP_OK;
 }
 
-// point the instances of every schedule at the current output buffer
-static int amle_bind(AmlePlan &A, float *d_out)
+// point the instances of every schedule at the current output buffer.  The
+// copies go on the call's stream, after the work queued there before (which
+// may still read the old instances), from pinned staging that no copy
+// still reads (the stream is drained first).
+static int amle_bind(AmlePlan &A, float *d_out, hipStream_t s)
 {
 	if (A.u_for == d_out) return SP_OK;
 	const size_t n = (size_t)A.w * A.h;
+	std::vector<GsInst> in;
 	for (auto &sc : A.sched) {
-		std::vector<GsInst> in;
 		for (int l = 0; l < A.pd; l++)
 			in.push_back({d_out + n * l, A.c + n * l, A.w, A.h, sc.T, 0.0f, nullptr});
 		if (A.trace)
 			for (int l = 0; l < A.pd; l++)
 				in.push_back({A.tinf + n * l, A.c + n * l, A.w, A.h, sc.T, 0.0f, nullptr});
-		HIPCHK(hipMemcpy(sc.d_insts, in.data(), in.size() * sizeof(GsInst),
-				hipMemcpyHostToDevice));
+	}
+	TRY(A.mem.stage(in.size() * sizeof(GsInst)));
+	HIPCHK(hipStreamSynchronize(s));
+	memcpy(A.mem.h_stage, in.data(), in.size() * sizeof(GsInst));
+	size_t off = 0;
+	for (auto &sc : A.sched) {
+		const size_t k = (size_t)(A.trace ? 2 : 1) * A.pd;
+		HIPCHK(hipMemcpyAsync(sc.d_insts, (GsInst *)A.mem.h_stage + off,
+				k * sizeof(GsInst), hipMemcpyHostToDevice, s));
+		off += k;
 	}
 	A.u_for = d_out;
 	return SP_OK;
@@ -1502,6 +1745,8 @@ static int amle_args(const float *out, const float *in, int w, int h, int pd,
 
 static std::mutex g_amle_mu;
 static AmlePlan *g_amle = nullptr;
+static int amle_run(AmlePlan &A, float *d_out, const float *d_in, const float *d_init,
+		hipStream_t s);
 
 // tile shape of the 4-neighbour sweeps: SIMPOIS_HIP_AMLE_VARIANT (read at
 // each call), else 32x32x16 (variant 6) for one plane, whose wavefronts hold
@@ -1527,6 +1772,8 @@ extern "C" int sp_amle_device(float *d_out, const float *d_in, const float *d_in
 	AmlePlan *A = g_amle;
 	if (!A || A->w != w || A->h != h || A->pd != pd || A->nn != nn ||
 			A->niter != niter || A->trace != trace || A->var != var) {
+		// the old plan's scratch may still be in use on its stream
+		if (A && A->used) HIPCHK(hipEventSynchronize(A->done));
 		delete g_amle;
 		g_amle = A = new AmlePlan;
 		A->w = w; A->h = h; A->pd = pd; A->nn = nn; A->niter = niter;
@@ -1544,7 +1791,22 @@ extern "C" int sp_amle_device(float *d_out, const float *d_in, const float *d_in
 		}
 		if (rc) { delete g_amle; g_amle = nullptr; return rc; }
 	}
-	TRY(amle_bind(*A, d_out));
+	// scratch reuse across streams: wait for the previous call's work
+	if (A->used && A->last != s) HIPCHK(hipStreamWaitEvent(s, A->done, 0));
+	if (!A->done) HIPCHK(hipEventCreateWithFlags(&A->done, hipEventDisableTiming));
+	TRY(amle_bind(*A, d_out, s));
+	const int rc = amle_run(*A, d_out, d_in, d_init, s);
+	HIPCHK(hipEventRecord(A->done, s));
+	A->last = s;
+	A->used = true;
+	return rc;
+}
+
+static int amle_run(AmlePlan &Ar, float *d_out, const float *d_in, const float *d_init,
+		hipStream_t s)
+{
+	AmlePlan *A = &Ar;
+	const int w = A->w, h = A->h, pd = A->pd, niter = A->niter, trace = A->trace;
 	const int n = w * h;
 	{
 		ProfScope ps("amle_prep", s);
@@ -1666,9 +1928,11 @@ extern "C" int sp_heatd_device(float *d_out, const float *d_in, const float *d_A
 			: d_A ? spk::launch_heatd_multi_field(dst, src, d_A, w, h, pd, chunks[k], rows,
 					t, s)
 			: spk::launch_heatd_multi(dst, src, w, h, pd, chunks[k], t, s);
-		if (!ok)
+		if (!ok) {
+			if (tmp) (void)hipFreeAsync(tmp, s);
 			return set_err(SP_ERR_ARG, "heatd: planes too small for %d steps per launch",
 					chunks[k]);
+		}
 		src = dst;
 	}
 	// algorithmic bytes: the planes read and written once per launch, the

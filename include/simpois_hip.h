@@ -20,6 +20,8 @@
 #ifndef SIMPOIS_HIP_H
 #define SIMPOIS_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -135,10 +137,44 @@ typedef struct sp_image {
 } sp_image;
 
 typedef struct sp_stats {
-	int levels;            /* pyramid levels of the solve */
-	int cg_iters_finest;   /* CG iterations at the finest level (plane 0) */
+	int levels;            /* pyramid levels of the solve (the deepest plane) */
+	int cg_iters_finest;   /* CG iterations at the finest level, the maximum
+	                          over the planes of the call */
 	double elapsed_ms;     /* device time of the solve */
 } sp_stats;
+
+/* Why a plane of a solve stopped (sp_plane_stats.status). */
+enum {
+	SP_PLANE_SWEPT = 0,      /* no CG asked (-c 0): the fixed sweeps ran */
+	SP_PLANE_CONVERGED = 1,  /* the CG met its stop test, or had nothing to
+	                            solve (no holes, or a zero initial residual) */
+	SP_PLANE_MAXIT = 2,      /* the CG stopped at its iteration limit */
+	SP_PLANE_BREAKDOWN = 3,  /* the CG stopped on p.Ap <= 0 (round-off); the
+	                            iterate of that point is kept */
+	SP_PLANE_NONFINITE = 4   /* the residual is not finite: NaN or Inf in the
+	                            data the plane's system reads.  The FP32 CG
+	                            stops the plane there and keeps its initial
+	                            guess; the parity CG (FP64 vectors) runs on
+	                            like the reference and writes what it gets */
+};
+
+/* Per-plane outcome of a batched solve (one entry per plane: the planes of
+ * image 0, then of image 1, ...).  Planes never interact: a plane that
+ * fails leaves the others' bits unchanged. */
+typedef struct sp_plane_stats {
+	int status;            /* SP_PLANE_* of the finest level; NONFINITE also
+	                          when the output's residual is not finite */
+	int levels;            /* pyramid levels of this plane */
+	int holes;             /* NaN pixels of the plane (unknowns) */
+	int sweeps;            /* Gauss-Seidel sweeps run at the finest level */
+	int cg_iters;          /* CG iterations at the finest level */
+	double cg_rr;          /* the CG's last r.r at the finest level (its own
+	                          recurrence); -1 without CG */
+	double rel_residual;   /* FP64 ||b - A u|| / ||b|| of the output over
+	                          the holes, the system of cleant_cgpois.c:225-236
+	                          (Poisson, or biharmonic for tstep < 0); -1 for
+	                          the triharmonic operator (tstep < -1000) */
+} sp_plane_stats;
 
 /* Defaults of the reference (simpois.c:357-360) with the env knobs. */
 void sp_params_default(sp_params *p);
@@ -166,6 +202,37 @@ int sp_solve_device(float *d_out, const float *d_in, const float *d_dat,
  * solves one image per process (doc/misc/optimization.txt:44-50). */
 int sp_solve_batch_device(int n, const sp_image *imgs, const sp_params *p,
 		void *stream, sp_stats *st);
+
+/* sp_solve_batch_device, also filling ps[0 .. total planes) with the
+ * outcome of each plane (ps may be NULL).  The residuals cost one more
+ * read of the finest level.  No reference counterpart. */
+int sp_solve_batch_device_ex(int n, const sp_image *imgs, const sp_params *p,
+		void *stream, sp_stats *st, sp_plane_stats *ps);
+
+/* The CG history of the last solve in the process at one plane (0-based,
+ * planes of image 0 first) and pyramid level (0 = finest): the r.r values
+ * after each iteration, as minicg.c:46 prints them, at most cap of them
+ * into rr, and the level's unknowns into *n.  Returns the number of
+ * iterations, -1 for a plane or level the last solve did not have.  The
+ * FP64 (parity) CG always records it; the FP32 CG when sp_params.trace is
+ * set.  No reference counterpart. */
+int sp_last_cg_trace(int plane, int level, int *n, double *rr, int cap);
+
+/* Solve plans (buffers, tile lists, captured graphs) are cached by call
+ * signature (sizes, buffer addresses, parameters), least recently used
+ * first out: at most max_plans plans and max_bytes device bytes (defaults
+ * 256 and 16 GiB).  A plan that cannot be allocated empties the cache and
+ * is tried once more.  sp_plan_cache_clear frees every cached plan.  No
+ * reference counterpart. */
+void sp_plan_cache_limit(int max_plans, size_t max_bytes);
+void sp_plan_cache_clear(void);
+
+/* Cap on the device bytes the library holds (plans in the cache included);
+ * an allocation beyond it fails with SP_ERR_NOMEM as a failed hipMalloc
+ * does.  0 (the default): no cap.  Returns the previous cap.
+ * sp_device_bytes: the bytes held now.  No reference counterpart. */
+size_t sp_set_alloc_cap(size_t bytes);
+size_t sp_device_bytes(void);
 
 /* one image of an adjoint batch: planar w*h*pd device buffers */
 typedef struct sp_adjoint_image {

@@ -162,6 +162,24 @@ def bits_equal(a, b):
     return bool(same.all())
 
 
+def close_cg(ours, ref, g, rel=1e-4):
+    """CG parity bar: known pixels bit-exact, filled pixels within rel
+    relative max-abs (relative to max(1, max|ref|)), NaN where ref is NaN."""
+    holes = np.isnan(g)
+    assert bits_equal(ours[~holes], ref[~holes])
+    a, b = ours[holes].astype(np.float64), ref[holes].astype(np.float64)
+    if np.isnan(b).any():
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        a, b = a[~np.isnan(b)], b[~np.isnan(b)]
+    if b.size:
+        scale = max(1.0, np.max(np.abs(b)))
+        err = np.max(np.abs(a - b))
+        assert err <= rel * scale, (err, scale)
+
+
+DROPIN_CALLER = os.path.join(ROOT, "tests", "dropin", "_build", "dropin_caller")
+
+
 def gpu_available():
     try:
         import torch

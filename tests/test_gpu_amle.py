@@ -155,3 +155,31 @@ def test_bad_arguments_are_refused(nn, niter, w):
     rc = sp.lib.sp_amle_host(y.ctypes.data, x.ctypes.data, None, w, 4, 1, nn, niter, 0)
     assert rc == 1
     assert sp.lib.sp_last_error()
+
+
+def test_back_to_back_calls_on_side_streams():
+    """Two calls of one shape into different outputs, queued without a host
+    synchronisation on a non-default stream, then a third on another
+    stream: the instance tables are rebound in stream order and the shared
+    scratch is not reused before the previous call's kernels are done.
+    Every output equals the oracle bit for bit."""
+    import torch
+    rng = np.random.default_rng(77)
+    xs = []
+    for _ in range(3):
+        x = (rng.standard_normal((257, 389)) * 30).astype(np.float32)
+        x[rng.random(x.shape) < 0.6] = np.nan
+        xs.append(x)
+    dx = [dev(x) for x in xs]
+    outs = [torch.empty_like(d) for d in dx]
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    print("amle on side streams", flush=True)
+    with torch.cuda.stream(s1):
+        sp.amle(dx[0], 4, 60, out=outs[0])
+        sp.amle(dx[1], 4, 60, out=outs[1])
+    with torch.cuda.stream(s2):
+        sp.amle(dx[2], 4, 60, out=outs[2])
+    torch.cuda.synchronize()
+    for x, o in zip(xs, outs):
+        assert bits_equal(o.cpu().numpy(), oracle_amle(x, 4, 60))

@@ -1,0 +1,271 @@
+"""The benchmark configurations at their full workloads, through the C ABI.
+
+Inputs come from the benchmark's own generators (bench.py), so these are
+the workloads it times:
+
+- 512x512 Laplace, 29-31 % disc holes, simpois -n 2000 (10 levels x 2000
+  Gauss-Seidel sweeps): bit-exact against the oracle.
+- 4096x4096x3 Poisson with guidance, 50 % rectangle holes, to relative
+  residual 1e-5 with the multigrid-preconditioned CG (the benchmark's
+  default method): FP64 residual per channel, known pixels bit-equal to the
+  input; a 1024x1024 instance of the generator against an FP64 direct
+  solve (1e-4 relative max-abs); the reference's method (pyramid, -n 10, CG
+  to the same target at every level) in FP64 vectors against the oracle on a
+  512x512 instance, same iteration counts.
+- 2048x2048 biharmonic, GRF k^-3 with 20 % stripes, FP64 CG to 1e-6 ||b||:
+  a 512x512 instance against the oracle (1e-4, known pixels bit-exact,
+  iteration count at the finest level within one), the full size to its
+  residual target.
+- 256 images of uneven sizes to 1e-5 each, in one batched call: every
+  plane converged with its FP64 residual under target; eight images solved
+  alone give the same bits.
+- 64 x 1024x1024 differentiable fill: forward residual, gradient structure
+  and batch invariance at full size; the gradient of a two-image 256x256
+  instance against the FP64 reference gradient (1e-3).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import ROOT, bits_equal, close_cg, oracle_solve
+
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import adjoint_ref as ar  # noqa: E402
+import bench  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+SEED = 2   # bench.py's default --seed
+
+
+def _m():
+    import imscript_amd
+    return imscript_amd
+
+
+def _equal_bits(a, b):
+    import torch
+    return bool(torch.equal(a.contiguous().view(torch.int32),
+                            b.contiguous().view(torch.int32)))
+
+
+# ------------------------------------------------------------------ C1
+
+def test_c1_full_size_bit_exact_against_oracle():
+    import torch
+    m = _m()
+    g, frac = bench.c1_input(SEED)
+    assert 0.29 <= frac <= 0.31
+    print(f"c1 512x512 -n 2000 holes={frac:.3f}", flush=True)
+    dg = torch.from_numpy(g).cuda()
+    for autotune in (0, 1):
+        out, st = m.solve_device(dg, tstep=0.25, niter=2000, scale=99, cgit=0,
+                                 autotune=autotune)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        if autotune == 0:
+            want, _ = oracle_solve(g, niter=2000)
+        assert st.levels == 10
+        assert bits_equal(got, want), f"autotune={autotune}"
+
+
+# ------------------------------------------------------------------ C2
+
+C2_PARAMS = dict(tstep=0.25, niter=0, scale=1, cgit=-1, cg_fp32=1, cg_rel=5e-6,
+                 trace=0, cg_chunk=32, cg_mg=1, autotune=1)
+
+
+def test_c2_full_size_residual_and_known_pixels():
+    import torch
+    m = _m()
+    g, f, frac = bench.make_inputs(torch, 4096, 3, SEED, "cuda:0")
+    print(f"c2 4096x4096x3 holes={frac:.3f}", flush=True)
+    out, st = m.solve_device(g, f, **C2_PARAMS)
+    rel = m.residual_device(out, g, f)
+    print("c2 residuals", rel, "iterations", st.cg_iters_finest, flush=True)
+    assert max(rel) <= 1e-5, rel
+    assert 0 < st.cg_iters_finest <= 12
+    known = ~torch.isnan(g)
+    assert _equal_bits(out[known], g[known])
+    assert bool(torch.isfinite(out).all())
+    # the same solve again (cached plan) gives the same bits
+    out2 = torch.empty_like(out)
+    m.solve_device(g, f, out=out2, **C2_PARAMS)
+    m.solve_device(g, f, out=out2, **C2_PARAMS)
+    assert _equal_bits(out, out2)
+
+
+def test_c2_instance_against_fp64_direct_solve():
+    import torch
+    m = _m()
+    g, f, _ = bench.make_inputs(torch, 1024, 1, SEED + 100, "cuda:0")
+    print("c2 1024x1024 instance vs FP64 direct", flush=True)
+    out, _ = m.solve_device(g, f, **C2_PARAMS)
+    gh, fh, uh = g[0].cpu().numpy(), f[0].cpu().numpy(), out[0].cpu().numpy()
+    ex = ar.solve(gh, fh)
+    hole = np.isnan(gh)
+    err = np.abs(uh.astype(np.float64) - ex)[hole].max() / np.abs(ex).max()
+    print("c2 rel max err vs FP64", err, flush=True)
+    assert err <= 1e-4
+    assert bits_equal(uh[~hole], gh[~hole])
+
+
+def test_c2_reference_method_against_oracle():
+    """simpois -n 10 -c -1 with CG_EPS = 5e-6 ||b|| (the reference's method
+    on this workload) through the parity CG (FP64 vectors), against the
+    oracle on a 512x512 instance: CG bar, and the finest level's iteration
+    count and r.r history."""
+    import ctypes
+    import torch
+    from helpers import oracle
+    m = _m()
+    g, f, _ = bench.make_inputs(torch, 512, 1, SEED + 7, "cuda:0")
+    gh = np.ascontiguousarray(g[0].cpu().numpy())
+    fh = np.ascontiguousarray(f[0].cpu().numpy())
+    bn = ctypes.c_double()
+    u0 = np.where(np.isnan(gh), 0, gh).astype(np.float32)
+    oracle().orc_rel_residual(u0.ctypes.data, gh.ctypes.data, fh.ctypes.data, 512, 512,
+                              0, ctypes.byref(bn))
+    eps = float(np.float32(5e-6 * bn.value))
+    print(f"c2 reference method 512x512 eps={eps:g}", flush=True)
+    out, st = m.solve_device(g, f, niter=10, scale=99, cgit=-1, cg_eps=eps)
+    got = out[0].cpu().numpy()
+    n_ours, rr_ours = m.last_cg_trace(0, 0)
+    want, trace = oracle_solve(gh, fh, niter=10, cgit=-1, cg_eps=eps, trace_cap=200000)
+    close_cg(got, want, gh)
+    fin = [t for t in trace if t[0] == n_ours]
+    assert len(fin) == len(rr_ours)
+    for k, (a, b) in enumerate(zip(rr_ours, [t[2] for t in fin])):
+        assert abs(a - b) <= 1e-6 * b, (k, a, b)
+    # the FP32 form of the method (float vectors, FP64 reductions) meets the
+    # same target
+    out32, _ = m.solve_device(g, f, niter=10, scale=99, cgit=-1, cg_fp32=1, cg_rel=5e-6)
+    assert m.residual_device(out32, g, f)[0] <= 1e-5
+
+
+# ------------------------------------------------------------------ C3
+
+def _c3_input(n, seed):
+    rng = np.random.default_rng(seed)
+    g = bench.grf(rng, n)
+    m = bench.stripes(rng, n, 0.2)
+    g[m] = np.nan
+    return g
+
+
+def test_c3_instance_against_oracle():
+    import torch
+    m = _m()
+    g = _c3_input(512, SEED + 33)
+    dg = torch.from_numpy(g).cuda()
+    eps = float(np.float32(0.9e-6 * bench.bnorm_bilap(dg)))
+    print(f"c3 512x512 biharmonic CG eps={eps:g}", flush=True)
+    out, _ = m.solve_device(dg, tstep=-0.08, niter=0, scale=99, cgit=-1, cg_eps=eps)
+    got = out.cpu().numpy()
+    n_ours, rr_ours = m.last_cg_trace(0, 0)
+    want, trace = oracle_solve(g, tstep=-0.08, niter=0, cgit=-1, cg_eps=eps,
+                               trace_cap=500000)
+    close_cg(got, want, g)
+    fin = [t[2] for t in trace if t[0] == n_ours]
+    print("c3 finest iterations ours", len(rr_ours), "oracle", len(fin), flush=True)
+    assert abs(len(fin) - len(rr_ours)) <= 1
+    assert m.residual_device(out, dg, bilap=True)[0] <= 1e-6
+
+
+def test_c3_full_size_residual():
+    import torch
+    m = _m()
+    g = _c3_input(2048, SEED + 3)
+    dg = torch.from_numpy(g).cuda()
+    bn = bench.bnorm_bilap(dg)
+    print("c3 2048x2048", flush=True)
+    out, st = m.solve_device(dg, tstep=-0.08, niter=0, scale=99, cgit=-1,
+                             cg_eps=0.9e-6 * bn)
+    rel = m.residual_device(out, dg, bilap=True)[0]
+    print("c3 residual", rel, "iterations", st.cg_iters_finest, flush=True)
+    assert rel <= 1e-6
+    known = ~torch.isnan(dg)
+    assert _equal_bits(out[known], dg[known])
+
+
+# ------------------------------------------------------------------ C4
+
+C4_PARAMS = dict(tstep=0.25, niter=0, scale=1, cgit=-1, cg_fp32=1, cg_rel=5e-6, cg_mg=1)
+
+
+def test_c4_full_batch():
+    import torch
+    m = _m()
+    rng = np.random.default_rng(SEED + 4)
+    gs, fr = bench.c4_images(torch, rng, 256, "cuda:0")
+    print(f"c4 256 images {sum(g.numel() for g in gs) / 1e6:.1f} Mpx", flush=True)
+    outs, st, ps = m.solve_batch_device(gs, plane_stats=True, **C4_PARAMS)
+    assert len(ps) == 256
+    for k, (g, o, s) in enumerate(zip(gs, outs, ps)):
+        assert s.status == m.PLANE_CONVERGED, (k, s.status)
+        assert s.rel_residual <= 1e-5, (k, s.rel_residual)
+        assert s.holes == int(torch.isnan(g).sum())
+    # the per-plane residuals are those of the standalone residual entry
+    for k in (0, 100, 255):
+        assert abs(m.residual_device(outs[k], gs[k])[0] - ps[k].rel_residual) \
+            <= 1e-12 + 1e-9 * ps[k].rel_residual
+    # planes do not interact: images solved alone give the same bits
+    order = sorted(range(256), key=lambda i: gs[i].numel())
+    for k in order[::37][:8]:
+        alone, _ = m.solve_batch_device([gs[k]], **C4_PARAMS)
+        torch.cuda.synchronize()
+        assert _equal_bits(alone[0], outs[k]), (k, tuple(gs[k].shape))
+
+
+# ------------------------------------------------------------------ C5
+
+def test_c5_full_size_forward_backward():
+    import torch
+    from imscript_amd.autograd import poisson_fill
+    m = _m()
+    g, f, target = bench.c5_inputs(torch, SEED, 64, 1024, "cuda:0")
+    holef = torch.isnan(g).to(torch.float32)
+    print("c5 64x1024x1024 forward + backward", flush=True)
+    gg = g.clone().requires_grad_(True)
+    ff = f.clone().requires_grad_(True)
+    u = poisson_fill(gg, ff, tol=5e-6)
+    (((u - target) ** 2) * holef).sum().backward()
+    rel = m.residual_device(u.detach(), g, f)
+    assert max(rel) <= 1e-5, max(rel)
+    hole = holef.bool()
+    assert bool(torch.isfinite(gg.grad).all()) and bool(torch.isfinite(ff.grad).all())
+    assert bool((gg.grad[hole] == 0).all())      # dL/dg lives on known pixels
+    assert bool((ff.grad[~hole] == 0).all())     # dL/df on the holes
+    # one image alone: same forward bits and gradient bits
+    k = 5
+    g1 = g[k:k + 1].clone().requires_grad_(True)
+    f1 = f[k:k + 1].clone().requires_grad_(True)
+    u1 = poisson_fill(g1, f1, tol=5e-6)
+    (((u1 - target[k:k + 1]) ** 2) * holef[k:k + 1]).sum().backward()
+    assert _equal_bits(u1.detach()[0], u.detach()[k])
+    assert _equal_bits(g1.grad[0], gg.grad[k])
+    assert _equal_bits(f1.grad[0], ff.grad[k])
+
+
+def test_c5_instance_gradient_against_fp64_reference():
+    import torch
+    from imscript_amd.autograd import poisson_fill
+    g, f, target = bench.c5_inputs(torch, SEED + 1, 2, 256, "cuda:0")
+    print("c5 2x256x256 gradient vs FP64", flush=True)
+    gg = g.clone().requires_grad_(True)
+    ff = f.clone().requires_grad_(True)
+    holef = torch.isnan(g).to(torch.float32)
+    u = poisson_fill(gg, ff, tol=1e-7)
+    (((u - target) ** 2) * holef).sum().backward()
+    for k in range(2):
+        gh, fh, th = (x[k].detach().cpu().numpy().astype(np.float64)
+                      for x in (g, f, target))
+        uref = ar.solve(gh, fh)
+        v = np.where(np.isnan(gh), 2 * (uref - th), 0)
+        rg, rf = ar.gradient(gh, fh, v)
+        eg = np.max(np.abs(gg.grad[k].cpu().numpy() - rg)) / np.max(np.abs(rg))
+        ef = np.max(np.abs(ff.grad[k].cpu().numpy() - rf)) / np.max(np.abs(rf))
+        print(f"c5 plane {k}: grad_g err {eg:.2e}, grad_f err {ef:.2e}", flush=True)
+        assert eg <= 1e-3 and ef <= 1e-3

@@ -14,26 +14,29 @@ import pytest
 from helpers import ROOT
 
 BUILD = os.path.join(ROOT, "tests", "cpu", "_build")
-EXE = os.path.join(BUILD, "gs_emulate")
 
 
 @pytest.fixture(scope="module")
 def emulator():
     if shutil.which("g++") is None:
         pytest.skip("g++ not available")
-    os.makedirs(BUILD, exist_ok=True)
+    # one build per process (pytest-xdist workers build side by side)
+    build = os.path.join(BUILD, f"p{os.getpid()}")
+    os.makedirs(build, exist_ok=True)
     san = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all"]
     objs = []
     for src in ("simpois_oracle.c", "amle_oracle.c"):
-        obj = os.path.join(BUILD, src.replace(".c", "_asan.o"))
+        obj = os.path.join(build, src.replace(".c", "_asan.o"))
         subprocess.run(["gcc", "-O1", "-g", "-std=gnu99", "-ffp-contract=off", *san,
                         "-c", os.path.join(ROOT, "oracle", src), "-o", obj],
                        check=True)
         objs.append(obj)
+    exe = os.path.join(build, "gs_emulate")
     subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-ffp-contract=off", *san,
-                    "-o", EXE, os.path.join(ROOT, "tests", "cpu", "gs_emulate.cpp"),
+                    "-o", exe, os.path.join(ROOT, "tests", "cpu", "gs_emulate.cpp"),
                     *objs, "-lm"], check=True)
-    return EXE
+    yield exe
+    shutil.rmtree(build, ignore_errors=True)
 
 
 CASES = [
