@@ -9,7 +9,11 @@
  * for heatd.c:109-150 through linear_diffusion_separable.
  *
  * Image I/O is the subset of iio that runs without external libraries:
- *   read : NPY 1.0 (iio.c:3216-3295), PFM (iio.c:2599-2616, see below)
+ *   read : NPY 1.0 (iio.c:3216-3295), PFM (iio.c:2599-2616, see below),
+ *          TIFF and PNG (sp_codecs.c); the names "zero:WxH[,pd]",
+ *          "one:WxH[,pd]", "constant:V:WxH" (iio.c:5603-5634),
+ *          "TRANS[list]:name" and the IIO_TRANS variable (flips, axis
+ *          permutations, crops, pipes; iio.c:3800-3884, 5677-5704)
  *   write: NPY '<f4' shape (h, w, pd) (iio.c:4337-4373), PFM for a ".pfm"
  *          suffix (iio.c:4151-4164); anything else, and pipes, get NPY
  *          (iio.c:6495); a terminal on stdout gets a one-line summary
@@ -80,9 +84,13 @@ static char *take_option(int *argc, char ***argv, const char *name, char *dflt)
 
 /* ------------------------------------------------------------ reading */
 
+/* An image as read, before the conversion to float: the transformations
+ * of IIO_TRANS and "TRANS[...]:" names and the mask's colour-to-gray step
+ * act on the samples in their file type, as iio's do. */
 typedef struct raw_image {
 	int w, h, pd;
-	float *px;      /* interleaved, w*h*pd */
+	int type;              /* sample type, T_* of sp_codecs.h */
+	unsigned char *data;   /* interleaved samples, w*h*pd of that type */
 } raw_image;
 
 static FILE *open_in(const char *name)
@@ -138,8 +146,7 @@ static float sample_to_float(const unsigned char *p, int type)
 
 /* NPY 1.0 after the 4-byte magic "\x93NUM" (iio.c:3216-3295).  Like iio,
  * the byte-order character is skipped, not honoured. */
-static int read_npy(FILE *f, raw_image *im, unsigned char **keep_raw,
-		int *raw_type)
+static void read_npy(FILE *f, raw_image *im)
 {
 	unsigned char s[6];
 	read_exact(f, s, 6);
@@ -192,16 +199,8 @@ static int read_npy(FILE *f, raw_image *im, unsigned char **keep_raw,
 		raw = t;
 	}
 	im->w = w; im->h = h; im->pd = pd;
-	im->px = xalloc(n * sizeof(float));
-	for (size_t k = 0; k < n; k++)
-		im->px[k] = sample_to_float(raw + k * bytes, type);
-	if (keep_raw) {   /* keep raw samples for the mask conversion */
-		*raw_type = type;
-		*keep_raw = raw;
-	} else {
-		free(raw);
-	}
-	return 0;
+	im->type = type;
+	im->data = raw;
 }
 
 /* PFM: "Pf" (1 channel) or "PF" (3 channels), "w h\nscale\n", float rows.
@@ -220,8 +219,9 @@ static void read_pfm(FILE *f, int upper, raw_image *im)
 	int pd = upper ? 3 : 1;
 	size_t n = (size_t)w * h * pd;
 	im->w = w; im->h = h; im->pd = pd;
-	im->px = xalloc(n * sizeof(float));
-	read_exact(f, im->px, n * sizeof(float));
+	im->type = T_F4;
+	im->data = xalloc(n * sizeof(float));
+	read_exact(f, im->data, n * sizeof(float));
 }
 
 /* the rest of a stream after the n bytes already read into head */
@@ -244,10 +244,8 @@ static unsigned char *slurp(FILE *f, const unsigned char *head, size_t n, size_t
 	return b;
 }
 
-/* PNG or TIFF through sp_codecs.c, samples converted as
- * iio_convert_samples does (iio.c:980-1090: plain casts to float) */
-static void read_coded(FILE *f, const unsigned char *head, int png, raw_image *im,
-		unsigned char **keep_raw, int *raw_type)
+/* PNG or TIFF through sp_codecs.c, samples kept in their file type */
+static void read_coded(FILE *f, const unsigned char *head, int png, raw_image *im)
 {
 	size_t n;
 	unsigned char *b = slurp(f, head, 4, &n);
@@ -256,104 +254,370 @@ static void read_coded(FILE *f, const unsigned char *head, int png, raw_image *i
 	if ((png ? codec_read_png : codec_read_tiff)(b, n, &c, err, sizeof err))
 		io_fail("%s", err);
 	free(b);
-	const size_t ns = (size_t)c.w * c.h * c.pd;
-	const int sb = codec_type_bytes(c.type);
 	im->w = c.w; im->h = c.h; im->pd = c.pd;
-	im->px = xalloc(ns * sizeof(float));
-	for (size_t k = 0; k < ns; k++)
-		im->px[k] = sample_to_float(c.data + k * sb, c.type);
-	if (keep_raw) {
-		*raw_type = c.type;
-		*keep_raw = c.data;
-	} else {
-		free(c.data);
-	}
+	im->type = c.type;
+	im->data = c.data;
 }
 
-static void read_any(const char *name, raw_image *im, unsigned char **raw,
-		int *raw_type)
+/* one file or stream, format by its first bytes (read_image_f) */
+static void read_core(const char *name, raw_image *im)
 {
 	FILE *f = open_in(name);
 	unsigned char m[4];
 	read_exact(f, m, 2);
-	if (raw) *raw = NULL;
 	if (m[0] == 'P' && (m[1] == 'f' || m[1] == 'F')) {
 		read_pfm(f, m[1] == 'F', im);
 	} else {
 		read_exact(f, m + 2, 2);
 		if (m[0] == 0x93 && m[1] == 'N' && m[2] == 'U' && m[3] == 'M')
-			read_npy(f, im, raw, raw_type);
+			read_npy(f, im);
 		else if ((m[0] == 'I' && m[1] == 'I' && m[2] == 42 && m[3] == 0) ||
 				(m[0] == 'M' && m[1] == 'M' && m[2] == 0 && m[3] == 42))
-			read_coded(f, m, 0, im, raw, raw_type);
+			read_coded(f, m, 0, im);
 		else if (m[0] == 0x89 && m[1] == 'P' && m[2] == 'N' && m[3] == 'G')
-			read_coded(f, m, 1, im, raw, raw_type);
+			read_coded(f, m, 1, im);
 		else
 			io_fail("unsupported image format (NPY, PFM, TIFF and PNG are supported)");
 	}
 	if (f != stdin) fclose(f);
 }
 
+/* ------------------------------------ transformations of a read image */
+
+/* Geometric transformations of an image as read (iio.c:757-960): they move
+ * whole pixels of pd samples, in the samples' file type. */
+
+static size_t px_bytes(const raw_image *im)
+{
+	return (size_t)im->pd * codec_type_bytes(im->type);
+}
+
+static void swap_px(raw_image *im, int i, int j, int a, int b)
+{
+	const size_t ps = px_bytes(im);
+	unsigned char *p = im->data + ((size_t)j * im->w + i) * ps;
+	unsigned char *q = im->data + ((size_t)b * im->w + a) * ps;
+	for (size_t k = 0; k < ps; k++) {
+		unsigned char t = p[k];
+		p[k] = q[k];
+		q[k] = t;
+	}
+}
+
+static void flip_leftright(raw_image *im)
+{
+	for (int j = 0; j < im->h; j++)
+		for (int i = 0; i < im->w / 2; i++)
+			swap_px(im, i, j, im->w - i - 1, j);
+}
+
+static void flip_topdown(raw_image *im)
+{
+	for (int j = 0; j < im->h / 2; j++)
+		for (int i = 0; i < im->w; i++)
+			swap_px(im, i, j, i, im->h - j - 1);
+}
+
+static void transpose_px(raw_image *im)
+{
+	const size_t ps = px_bytes(im);
+	const int w = im->w, h = im->h;
+	unsigned char *t = xalloc((size_t)w * h * ps);
+	for (int j = 0; j < h; j++)        /* new pixel (j, i) = old (i, j) */
+		for (int i = 0; i < w; i++)
+			memcpy(t + ((size_t)i * h + j) * ps,
+				im->data + ((size_t)j * w + i) * ps, ps);
+	free(im->data);
+	im->data = t;
+	im->w = h;
+	im->h = w;
+}
+
+/* two orientation letters (iio.c:811-821): an upper-case first letter flips
+ * left-right, an upper-case second one top-down, and the pair is transposed
+ * when the first letter comes after the second in the alphabet */
+static void reorient(raw_image *im, int ox, int oy)
+{
+	if (isupper(ox)) flip_leftright(im);
+	if (isupper(oy)) flip_topdown(im);
+	if (toupper(ox) > toupper(oy)) transpose_px(im);
+}
+
+/* a permutation of the axes (sample, column, row) named "x", "y", "z"
+ * (iio.c:886-960) */
+static void reorient3d(raw_image *im, const char *s)
+{
+	const size_t sb = codec_type_bytes(im->type);
+	const int a = im->pd, b = im->w, c = im->h;   /* sizes along x, y, z */
+	int v0, v1, v2;
+	if (!strcmp(s, "xyz"))      { v0 = a; v1 = b; v2 = c; }
+	else if (!strcmp(s, "yxz")) { v0 = b; v1 = a; v2 = c; }
+	else if (!strcmp(s, "xzy")) { v0 = a; v1 = c; v2 = b; }
+	else if (!strcmp(s, "zxy")) { v0 = c; v1 = a; v2 = b; }
+	else if (!strcmp(s, "zyx")) { v0 = c; v1 = b; v2 = a; }
+	else if (!strcmp(s, "yzx")) { v0 = b; v1 = c; v2 = a; }
+	else { io_fail("unrecognized trasnposition string \"%s\"", s); return; }
+	unsigned char *t = xalloc((size_t)a * b * c * sb);
+	for (int k = 0; k < c; k++)
+	for (int j = 0; j < b; j++)
+	for (int i = 0; i < a; i++) {
+		/* coordinate of each output axis: the input coordinate of the
+		 * axis its letter names */
+		const int co[3] = {i, j, k};
+		const int o0 = co[s[0] - 'x'], o1 = co[s[1] - 'x'], o2 = co[s[2] - 'x'];
+		memcpy(t + (((size_t)o2 * v1 + o1) * v0 + o0) * sb,
+			im->data + (((size_t)k * b + j) * a + i) * sb, sb);
+	}
+	free(im->data);
+	im->data = t;
+	im->pd = v0;
+	im->w = v1;
+	im->h = v2;
+}
+
+/* Crop (or pad, for negative amounts) by the given numbers of columns and
+ * rows (iio.c:832-870).  New samples are NaN for 4- and 8-byte types, zero
+ * bytes otherwise, as iio fills them whatever the type.  iio's NaN is
+ * 0.0f/0.0f evaluated at run time, the x86 default NaN: sign bit set. */
+static void trim_px(raw_image *im, int left, int bottom, int right, int top)
+{
+	const size_t sb = codec_type_bytes(im->type), ps = px_bytes(im);
+	const int w = im->w, h = im->h;
+	const int nw = w - left - right, nh = h - bottom - top;
+	if (nw <= 0 || nh <= 0)
+		io_fail("image transformation leaves a %dx%d image", nw, nh);
+	const size_t ns = (size_t)nw * nh * im->pd;
+	unsigned char *t = xalloc(ns * sb);
+	if (sb == 4) {
+		const uint32_t nan4 = 0xffc00000u;
+		for (size_t k = 0; k < ns; k++) memcpy(t + 4 * k, &nan4, 4);
+	} else if (sb == 8) {
+		const uint64_t nan8 = 0xfff8000000000000ull;
+		for (size_t k = 0; k < ns; k++) memcpy(t + 8 * k, &nan8, 8);
+	} else {
+		memset(t, 0, ns * sb);
+	}
+	for (int j = 0; j < nh; j++)
+	for (int i = 0; i < nw; i++) {
+		const int x = i + left, y = j + top;
+		if (x >= 0 && y >= 0 && x < w && y < h)
+			memcpy(t + ((size_t)j * nw + i) * ps,
+				im->data + ((size_t)y * w + x) * ps, ps);
+	}
+	free(im->data);
+	im->data = t;
+	im->w = nw;
+	im->h = nh;
+}
+
+/* "flip=..." (iio.c:3810-3823): every test applies in turn */
+static void trans_flip(raw_image *im, const char *s)
+{
+	if (!strcmp(s, "leftright")) flip_leftright(im);
+	if (!strcmp(s, "topdown")) flip_topdown(im);
+	if (!strcmp(s, "transpose")) transpose_px(im);
+	if (!strcmp(s, "r270")) reorient(im, 'y', 'X');
+	if (!strcmp(s, "r90")) reorient(im, 'Y', 'x');
+	if (!strcmp(s, "r180")) reorient(im, 'X', 'Y');
+	if (!strcmp(s, "posetrans")) reorient(im, 'Y', 'X');
+	const size_t n = strlen(s);
+	if (n == 2) reorient(im, (unsigned char)s[0], (unsigned char)s[1]);
+	if (n == 3 && 3 * 'y' == tolower((unsigned char)s[0])
+			+ tolower((unsigned char)s[1]) + tolower((unsigned char)s[2]))
+		reorient3d(im, s);
+}
+
+static const char *npy_descr(int type)
+{
+	switch (type) {
+	case T_F4: return "<f4";
+	case T_F8: return "<f8";
+	case T_U1: return "<u1";
+	case T_U2: return "<u2";
+	case T_U4: return "<u4";
+	case T_U8: return "<u8";
+	case T_I1: return "<i1";
+	case T_I2: return "<i2";
+	case T_I4: return "<i4";
+	default: return "<i8";
+	}
+}
+
+static void npy_header(FILE *f, const char *descr, int w, int h, int pd);
+
+/* "pipe=COMMAND" (iio.c:3829-3852): the image goes to the command's stdin
+ * as NPY in its sample type, and what the command writes on stdout is read
+ * back, if the command succeeds */
+static void trans_pipe(raw_image *im, const char *cmd)
+{
+	char in[] = "/tmp/iio_tmp_file_XXXXXX", out[] = "/tmp/iio_tmp_file_XXXXXX";
+	int fi = mkstemp(in), fo = mkstemp(out);
+	if (fi < 0 || fo < 0) io_fail("could not create tmp filename");
+	close(fo);
+	FILE *f = fdopen(fi, "wb");
+	if (!f) io_fail("could not create tmp filename");
+	npy_header(f, npy_descr(im->type), im->w, im->h, im->pd);
+	fwrite(im->data, 1, (size_t)im->w * im->h * px_bytes(im), f);
+	fclose(f);
+	const size_t len = strlen(cmd) + strlen(in) + strlen(out) + 40;
+	char *c = xalloc(len);
+	snprintf(c, len, "(unset IIO_TRANS; %s) <%s >%s", cmd, in, out);
+	fflush(NULL);
+	if (!system(c)) {
+		free(im->data);
+		read_core(out, im);
+	}
+	free(c);
+	remove(in);
+	remove(out);
+}
+
+/* "OP=VALUE,OP=VALUE..." up to the first ']' (iio.c:3854-3884); tokens
+ * without '=' are skipped */
+static void trans_apply(raw_image *im, const char *f)
+{
+	size_t n = strcspn(f, "]");
+	char *t = xalloc(n + 1), *o = xalloc(n + 1), *v = xalloc(n + 1);
+	memcpy(t, f, n);
+	t[n] = 0;
+	for (char *tok = strtok(t, ","); tok; tok = strtok(NULL, ",")) {
+		if (sscanf(tok, "%[^=]=%[^]]", o, v) != 2) continue;
+		const int V = atoi(v), w = im->w, h = im->h;
+		if (!strcmp(o, "flip")) trans_flip(im, v);
+		else if (!strcmp(o, "pipe")) trans_pipe(im, v);
+		else if (!strcmp(o, "x")) trim_px(im, V, 0, 0, 0);
+		else if (!strcmp(o, "y")) trim_px(im, 0, 0, 0, V);
+		else if (!strcmp(o, "w")) trim_px(im, 0, 0, w - V, 0);
+		else if (!strcmp(o, "h")) trim_px(im, 0, h - V, 0, 0);
+	}
+	free(t);
+	free(o);
+	free(v);
+}
+
+/* "TRANS[list]:name" gives the colon, or NULL (iio.c:3800-3808) */
+static const char *trans_prefix(const char *f)
+{
+	if (strncmp(f, "TRANS[", 6)) return NULL;
+	const char *colon = strchr(f, ':');
+	if (!colon || colon[-1] != ']') return NULL;
+	return colon;
+}
+
+/* "zero:WxH[,pd]", "one:WxH[,pd]" (bytes) and "constant:V:WxH" (float),
+ * iio.c:5603-5634; returns 0 for other names */
+static int read_semantic(const char *name, raw_image *im)
+{
+	int s[2], pd = 1, one = 0;
+	float value = 0;
+	if (!strncmp(name, "zero:", 5) || (one = !strncmp(name, "one:", 4))) {
+		const char *a = name + (one ? 4 : 5);
+		if (sscanf(a, "%dx%d,%d", s, s + 1, &pd) != 3 &&
+				sscanf(a, "%dx%d", s, s + 1) != 2)
+			io_fail("bad semantical name \"%s\"", name);
+		im->type = T_U1;
+	} else if (!strncmp(name, "constant:", 9)) {
+		if (sscanf(name + 9, "%g:%dx%d", &value, s, s + 1) != 3)
+			io_fail("bad semantical name \"%s\"", name);
+		im->type = T_F4;
+	} else {
+		return 0;
+	}
+	if (s[0] <= 0 || s[1] <= 0 || pd <= 0)
+		io_fail("bad semantical name \"%s\"", name);
+	const size_t ns = (size_t)s[0] * s[1] * pd;
+	im->w = s[0]; im->h = s[1]; im->pd = pd;
+	im->data = xalloc(ns * codec_type_bytes(im->type));
+	if (im->type == T_U1)
+		memset(im->data, one, ns);
+	else
+		for (size_t k = 0; k < ns; k++) memcpy(im->data + 4 * k, &value, 4);
+	return 1;
+}
+
+/* iio's read_image (iio.c:5585-5706): semantic names first (no IIO_TRANS
+ * for them), then "TRANS[...]:" names recursively, then files; IIO_TRANS
+ * from the environment applies last, so a "TRANS[...]:" name gets it twice
+ * (once inside), as in the reference */
+static void read_image(const char *name, raw_image *im)
+{
+	if (read_semantic(name, im)) return;
+	const char *colon = trans_prefix(name);
+	if (colon) {
+		read_image(colon + 1, im);
+		trans_apply(im, name + 6);
+	} else {
+		read_core(name, im);
+	}
+	const char *t = getenv("IIO_TRANS");
+	if (t && *t) trans_apply(im, t);
+}
+
+/* interleaved floats of an image as read (iio_convert_samples,
+ * iio.c:980-1090: plain casts); frees the samples */
+static float *to_float(raw_image *im)
+{
+	const size_t ns = (size_t)im->w * im->h * im->pd;
+	const int sb = codec_type_bytes(im->type);
+	float *px = xalloc(ns * sizeof(float));
+	for (size_t k = 0; k < ns; k++)
+		px[k] = sample_to_float(im->data + k * sb, im->type);
+	free(im->data);
+	im->data = NULL;
+	return px;
+}
+
 /* planar float image, iio_read_image_float_split (iio.c:5763-5771) */
 static float *read_split(const char *name, int *w, int *h, int *pd)
 {
 	raw_image im;
-	read_any(name, &im, NULL, NULL);
+	read_image(name, &im);
+	float *px = to_float(&im);
 	size_t n = (size_t)im.w * im.h;
 	float *out = xalloc(n * im.pd * sizeof(float));
 	for (size_t i = 0; i < n; i++)
 		for (int l = 0; l < im.pd; l++)
-			out[n * l + i] = im.px[im.pd * i + l];
-	free(im.px);
+			out[n * l + i] = px[im.pd * i + l];
+	free(px);
 	*w = im.w; *h = im.h; *pd = im.pd;
 	return out;
 }
 
-/* gray value of a colour sample triple, computed in double and stored in
- * the sample type, as iio_hacky_uncolorize does (iio.c:1230-1300) */
-static float gray_of(const unsigned char *raw, int type, size_t k, int pd)
+/* Gray of the first three samples of each pixel, computed in double and
+ * stored in the sample type (iio_hacky_uncolorize[a], iio.c:1230-1300):
+ * 8-bit, 16-bit (three samples only) and float images */
+static void uncolorize(raw_image *im)
 {
-	int bytes = type == T_F8 || type == T_U8 || type == T_I8 ? 8
-		: type == T_F4 || type == T_U4 || type == T_I4 ? 4
-		: type == T_U2 || type == T_I2 ? 2 : 1;
-	double a = sample_to_float(raw + (k * pd + 0) * bytes, type);
-	double b = sample_to_float(raw + (k * pd + 1) * bytes, type);
-	double c = sample_to_float(raw + (k * pd + 2) * bytes, type);
-	double v = .299 * a + .587 * b + .114 * c;
-	switch (type) {
-	case T_U1: return (float)(uint8_t)v;
-	case T_U2: return (float)(uint16_t)v;
-	case T_F4: return (float)v;
-	default: io_fail("uncolorize type not supported");
+	const int t = im->type, pd = im->pd;
+	if (!(t == T_U1 || t == T_F4 || (t == T_U2 && pd == 3)))
+		io_fail(pd == 3 ? "uncolorize type not supported"
+				: "uncolorizea type not supported");
+	const size_t n = (size_t)im->w * im->h;
+	const int sb = codec_type_bytes(t);
+	unsigned char *g = xalloc(n * sb);
+	for (size_t i = 0; i < n; i++) {
+		const unsigned char *p = im->data + i * pd * sb;
+		const double v = .299 * sample_to_float(p, t)
+			+ .587 * sample_to_float(p + sb, t)
+			+ .114 * sample_to_float(p + 2 * sb, t);
+		if (t == T_U1) { g[i] = (uint8_t)v; }
+		else if (t == T_U2) { uint16_t s = (uint16_t)v; memcpy(g + 2 * i, &s, 2); }
+		else { float s = (float)v; memcpy(g + 4 * i, &s, 4); }
 	}
-	return 0;
+	free(im->data);
+	im->data = g;
+	im->pd = 1;
 }
 
 /* scalar mask, iio_read_image_float (iio.c:6029-6048) */
 static float *read_mask(const char *name, int *w, int *h)
 {
 	raw_image im;
-	unsigned char *raw = NULL;
-	int type = T_F4;
-	read_any(name, &im, &raw, &type);
-	size_t n = (size_t)im.w * im.h;
-	float *out = xalloc(n * sizeof(float));
-	if (im.pd == 1) {
-		memcpy(out, im.px, n * sizeof(float));
-	} else if (im.pd == 3 || im.pd == 4) {
-		for (size_t i = 0; i < n; i++) {
-			if (raw) out[i] = gray_of(raw, type, i, im.pd);
-			else out[i] = (float)(.299 * im.px[im.pd * i]
-					+ .587 * im.px[im.pd * i + 1]
-					+ .114 * im.px[im.pd * i + 2]);
-		}
-	} else {
-		io_fail("could not read image");
-	}
-	free(raw);
-	free(im.px);
+	read_image(name, &im);
+	if (im.pd == 3 || im.pd == 4) uncolorize(&im);
+	if (im.pd != 1) io_fail("could not read image");
 	*w = im.w; *h = im.h;
-	return out;
+	return to_float(&im);
 }
 
 /* ------------------------------------------------------------ writing */
@@ -374,7 +638,7 @@ static FILE *open_out(const char *name)
 
 /* NPY header exactly as iio_write_image_as_npy lays it out
  * (iio.c:4337-4373): padded with spaces to a multiple of 16, ending "\n" */
-static void write_npy(const char *name, const float *px, int w, int h, int pd)
+static void npy_header(FILE *f, const char *descr, int w, int h, int pd)
 {
 	char buf[1000];
 	buf[0] = (char)0x93;
@@ -382,15 +646,20 @@ static void write_npy(const char *name, const float *px, int w, int h, int pd)
 	buf[6] = 1; buf[7] = 0;
 	int n = 10;
 	n += snprintf(buf + n, sizeof buf - n,
-		"{'descr': '<f4', 'fortran_order': False, 'shape': (%d, %d, %d)}",
-		h, w, pd);
+		"{'descr': '%s', 'fortran_order': False, 'shape': (%d, %d, %d)}",
+		descr, h, w, pd);
 	int m = ((n + 15 + 1) / 16) * 16;
 	buf[8] = (char)((m - 10) % 256);
 	buf[9] = (char)((m - 10) / 256);
 	for (int i = n; i < m - 1; i++) buf[i] = ' ';
 	buf[m - 1] = '\n';
-	FILE *f = open_out(name);
 	fwrite(buf, 1, m, f);
+}
+
+static void write_npy(const char *name, const float *px, int w, int h, int pd)
+{
+	FILE *f = open_out(name);
+	npy_header(f, "<f4", w, h, pd);
 	fwrite(px, sizeof(float), (size_t)w * h * pd, f);
 	if (f != stdout) fclose(f); else fflush(f);
 }
@@ -811,11 +1080,11 @@ int main_heatd(int c, char *v[])
 	float *A = NULL;
 	if (*fa) {
 		raw_image im;
-		read_any(fa, &im, NULL, NULL);
+		read_image(fa, &im);
+		A = to_float(&im);
 		if (im.w != w || im.h != h || im.pd != 3)
 			return fprintf(stderr, "ERROR: bad A dimensions"
 				"(%d %d %d) ((%d %d))\n", im.w, im.h, im.pd, w, h);
-		A = im.px;
 	}
 	const size_t n = (size_t)w * h * pd;
 	float *y = xalloc(n * sizeof(float));
@@ -834,9 +1103,10 @@ int main_heatd(int c, char *v[])
 float *iio_read_image_float_vec(const char *fname, int *w, int *h, int *pd)
 {
 	raw_image im;
-	read_any(fname, &im, NULL, NULL);
+	read_image(fname, &im);
+	float *px = to_float(&im);
 	*w = im.w; *h = im.h; *pd = im.pd;
-	return im.px;
+	return px;
 }
 
 /* iio.c:5763-5771 */

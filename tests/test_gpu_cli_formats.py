@@ -84,3 +84,30 @@ def test_heatd_on_a_colour_png(tmp_path):
     want, _ = oracle_heatd(np.ascontiguousarray(x.transpose(2, 0, 1)), 3, 0.25)
     got = tiff_decode((tmp_path / "o.tif").read_bytes())
     assert bits_equal(got.transpose(2, 0, 1), want)
+
+
+def test_simpois_with_transformed_names(tmp_path):
+    """IIO_TRANS applies to every image simpois reads, and to a
+    "TRANS[...]:" name twice (inside, before the name's own list, and after
+    it, as iio's recursive read does); "one:" / "zero:" names work as masks.
+    The fill equals the oracle's on the transformed arrays."""
+    rng = np.random.default_rng(11)
+    x = (rng.standard_normal((48, 48)) * 10).astype(np.float32)
+    m = (rng.random((48, 48)) < 0.3).astype(np.float32)
+    np.save(tmp_path / "x.npy", x)
+    np.save(tmp_path / "m.npy", m)
+    env = dict(os.environ, IIO_TRANS="flip=transpose")
+    run([SIMPOIS_BIN, "-i", str(tmp_path / "x.npy"),
+         "-m", "TRANS[flip=leftright]:" + str(tmp_path / "m.npy"),
+         "-o", str(tmp_path / "o.npy")], env=env)
+    g = x.T.copy()
+    g[m.T[:, ::-1].T > 0] = np.nan          # transposed, flipped, transposed
+    want, _ = oracle_solve(g)
+    assert bits_equal(npy(tmp_path / "o.npy")[..., 0], want.reshape(g.shape))
+    # a mask with no hole keeps the input; one with every pixel a hole gives 0
+    run([SIMPOIS_BIN, "-i", str(tmp_path / "x.npy"), "-m", "zero:48x48",
+         "-o", str(tmp_path / "z.npy")])
+    assert bits_equal(npy(tmp_path / "z.npy")[..., 0], x)
+    run([SIMPOIS_BIN, "-i", str(tmp_path / "x.npy"), "-m", "one:48x48",
+         "-o", str(tmp_path / "a.npy")])
+    assert not npy(tmp_path / "a.npy").any()
