@@ -362,8 +362,9 @@ struct Plan {
 	// FP64 CG scratch, sized for the largest plane
 	double *cg_x = nullptr, *cg_r = nullptr, *cg_p = nullptr, *cg_q = nullptr,
 		*cg_part = nullptr, *cg_rr = nullptr;
-	unsigned char *cg_hole = nullptr;   // unknowns mask of the plane being solved
-	float *cg_pf = nullptr;             // direction rounded to float
+	unsigned *cg_idx = nullptr;         // pixel of each unknown of the plane solved
+	int *cg_bcnt = nullptr;             // unknowns per setup block, prefix sums
+	float *cg_pf = nullptr;             // direction rounded to float (full grid)
 	int cg_rr_cap = 0;
 	int *cg_done = nullptr, *h_done = nullptr;
 	// pinned staging of small per-solve copies (descriptors in, results out):
@@ -816,9 +817,12 @@ static int cg64_ensure(Plan &pl, int maxit)
 		TRY(pl.alloc(&pl.cg_r, n0));
 		TRY(pl.alloc(&pl.cg_p, n0));
 		TRY(pl.alloc(&pl.cg_q, n0));
-		TRY(pl.alloc(&pl.cg_hole, n0));
+		int nb0 = 0;
+		for (auto &p : pl.planes) nb0 = std::max(nb0, spk::cg_setup_blocks(p.w, p.h));
+		TRY(pl.alloc(&pl.cg_idx, n0));
+		TRY(pl.alloc(&pl.cg_bcnt, (size_t)nb0 + 1));
 		TRY(pl.alloc(&pl.cg_pf, n0));
-		TRY(pl.alloc(&pl.cg_part, 2 * 1024));
+		TRY(pl.alloc(&pl.cg_part, std::max<size_t>(2 * 1024, nb0)));
 		TRY(pl.alloc(&pl.cg_done, 4));
 	}
 	if (maxit + 2 > pl.cg_rr_cap) {
@@ -846,15 +850,17 @@ static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 	const int maxit = (int)fm;
 	if (k == 0) pl.status_finest[plane] = SP_PLANE_MAXIT;
 	if (maxit <= 0) return SP_OK;   // x = x0: u is unchanged
-	const size_t n = (size_t)S.w * S.h;
-	const int nblk = (int)std::min<size_t>(1024, std::max<size_t>(1, (n + 2047) / 2048));
+	// iteration kernels: 4 unknowns per thread and pass, at most 1024 blocks
+	const int nblk = std::min(1024, std::max(1, (nmask + 1023) / 1024));
 	TRY(cg64_ensure(pl, maxit));
 	spk::CgDev d;
 	d.g = S.g; d.f = S.f; d.u = S.u;
-	d.x = pl.cg_x; d.r = pl.cg_r; d.p = pl.cg_p; d.q = pl.cg_q; d.hole = pl.cg_hole;
-	d.pf = pl.cg_pf;
+	d.x = pl.cg_x; d.r = pl.cg_r; d.p = pl.cg_p; d.q = pl.cg_q;
+	d.idx = pl.cg_idx; d.bcnt = pl.cg_bcnt; d.pf = pl.cg_pf;
 	d.part = pl.cg_part; d.rr = pl.cg_rr; d.done = pl.cg_done;
 	d.w = S.w; d.h = S.h; d.bilap = bilap; d.nblk = nblk;
+	d.nb = spk::cg_setup_blocks(S.w, S.h);
+	d.nc = nmask;
 	d.stop = (double)e;
 	ProfScope pr("cg64", s);
 	spk::launch_cg_setup(d, s);

@@ -95,19 +95,26 @@ void launch_gs_tiles(int R, int var, const GsTile *tiles, int ntiles,
 void launch_gs_small(int R, const GsInst *insts, const int *ids, int nids,
 		int lds_bytes, int threads, hipStream_t s);
 
-// CG (FP64 vectors).  Full-grid vectors, zero at known pixels.
+// CG (FP64 vectors).  Vectors are compact: entry k belongs to the k-th
+// unknown in raster order, at pixel idx[k].
 struct CgDev {
 	const float *g, *f;   // plane data
 	float *u;             // solution plane (read for x0, written at the end)
-	double *x, *r, *p, *q;
-	unsigned char *hole;  // 1 at the unknowns (NaN in g), written by the setup
-	float *pf;            // p rounded to float (what the stencil reads), kept with p
-	double *part;         // per-block partial sums (2 * nblk)
+	double *x, *r, *p, *q;   // nc entries each
+	unsigned *idx;        // pixel of each unknown, written by the setup
+	int *bcnt;            // unknowns per setup block, then their prefix sums (nb + 1)
+	float *pf;            // full grid: p rounded to float at the unknowns (what
+	                      // the stencil reads), 0 at known pixels
+	double *part;         // per-block partial sums (max(2 * nblk, nb))
 	double *rr;           // rr history, rr[0] = r0.r0, rr[k+1] = after iter k
 	int *done;            // [0] = stop flag, [1] = iterations run
-	int w, h, bilap, nblk;
+	int w, h, bilap;
+	int nblk;             // blocks of the iteration kernels
+	int nb;               // setup blocks: cg_setup_blocks(w, h)
+	int nc;               // unknowns (NaN pixels of g)
 	double stop;          // absolute stop threshold on sqrt(rr)
 };
+int cg_setup_blocks(int w, int h);
 void launch_cg_setup(const CgDev &d, hipStream_t s);
 void launch_cg_iter(const CgDev &d, int it, hipStream_t s);
 void launch_cg_finish(const CgDev &d, hipStream_t s);

@@ -374,35 +374,112 @@ static __device__ double sum_partials(const double *part, int nblk, double *red)
 	return bc;
 }
 
-// x0 = u at holes; b = -f - A(boundary); r = b - A x0; p = r; partial r.r
+// The FP64 CG keeps its vectors compact, one entry per unknown in raster
+// order (as the reference's mask3 list does, cleant_cgpois.c:6-25): the
+// setup lists the unknowns (idx), and the iteration kernels stream over that
+// list, so known pixels cost nothing.  The stencil reads p rounded to float
+// from a full-grid copy pf (0 at known pixels), at clamped taps.  Indices
+// are 32-bit (the solver takes at most 2^30 pixels per plane).
+constexpr int CG_SPAN = CG_NT * 4;   // pixels per setup block (4 per thread)
+
+// holes of each CG_SPAN-pixel block into bcnt; pf = 0 at known pixels
+__global__ __launch_bounds__(CG_NT) void k_cg_count(CgDev d)
+{
+	__shared__ int red[CG_NT / 64];
+	const unsigned n = (unsigned)d.w * (unsigned)d.h;
+	const unsigned q0 = blockIdx.x * CG_SPAN + threadIdx.x * 4;
+	int c = 0;
+#pragma unroll
+	for (int u = 0; u < 4; u++) {
+		const unsigned q = q0 + u;
+		if (q >= n) break;
+		if (isnan(d.g[q])) c++;
+		else d.pf[q] = 0.0f;
+	}
+	for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+	if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		int t = 0;
+		for (int k = 0; k < CG_NT / 64; k++) t += red[k];
+		d.bcnt[blockIdx.x] = t;
+	}
+}
+
+// exclusive prefix sum of bcnt[0..nb) in place, bcnt[nb] = total (one
+// work-group: each thread sums a contiguous run, the runs are scanned)
+constexpr int CG_SCAN_T = 1024;
+__global__ __launch_bounds__(CG_SCAN_T) void k_cg_scan(CgDev d)
+{
+	__shared__ int tot[CG_SCAN_T];
+	const int nb = d.nb, per = (nb + CG_SCAN_T - 1) / CG_SCAN_T;
+	const int a = threadIdx.x * per, e = min(a + per, nb);
+	int s = 0;
+	for (int k = a; k < e; k++) s += d.bcnt[k];
+	tot[threadIdx.x] = s;
+	__syncthreads();
+	for (int o = 1; o < CG_SCAN_T; o <<= 1) {   // inclusive scan of the runs
+		const int v = threadIdx.x >= o ? tot[threadIdx.x - o] : 0;
+		__syncthreads();
+		tot[threadIdx.x] += v;
+		__syncthreads();
+	}
+	int run = threadIdx.x ? tot[threadIdx.x - 1] : 0;
+	for (int k = a; k < e; k++) {
+		const int v = d.bcnt[k];
+		d.bcnt[k] = run;
+		run += v;
+	}
+	if (threadIdx.x == CG_SCAN_T - 1) d.bcnt[nb] = tot[CG_SCAN_T - 1];
+}
+
+// list the unknowns of the block (raster order) and set up each:
+// x0 = u; b = -f - A(boundary); r = b - A x0; p = r; partial r.r
 __global__ __launch_bounds__(CG_NT) void k_cg_setup(CgDev d)
 {
 	__shared__ double red[CG_NT / 64];
-	const size_t n = (size_t)d.w * d.h;
+	__shared__ int wsum[CG_NT / 64];
+	const unsigned n = (unsigned)d.w * (unsigned)d.h;
+	const unsigned q0 = blockIdx.x * CG_SPAN + threadIdx.x * 4;
+	bool hole[4];
+	int c = 0;
+#pragma unroll
+	for (int u = 0; u < 4; u++) {
+		hole[u] = q0 + u < n && isnan(d.g[q0 + u]);
+		c += hole[u];
+	}
+	// rank of the thread's first unknown within the block
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	int incl = c;
+	for (int o = 1; o < 64; o <<= 1) {
+		const int v = __shfl_up(incl, o, 64);
+		if (lane >= o) incl += v;
+	}
+	if (lane == 63) wsum[wv] = incl;
+	__syncthreads();
+	int k = d.bcnt[blockIdx.x] + incl - c;
+	for (int z = 0; z < wv; z++) k += wsum[z];
 	double acc = 0;
-	for (size_t q = (size_t)blockIdx.x * CG_NT + threadIdx.x; q < n;
-			q += (size_t)d.nblk * CG_NT) {
-		const float gq = d.g[q];
-		d.hole[q] = isnan(gq);
-		if (!isnan(gq)) {
-			d.x[q] = 0; d.r[q] = 0; d.p[q] = 0; d.pf[q] = 0; d.q[q] = 0;
-			continue;
-		}
-		const int i = (int)(q % d.w), j = (int)(q / d.w);
-		const float *g = d.g, *u = d.u;
-		double a0 = cg_stencil(d.bilap, i, j, d.w, d.h, [&](size_t k) {
-			float v = g[k]; return isnan(v) ? 0.0f : v; });
-		double ax = cg_stencil(d.bilap, i, j, d.w, d.h, [&](size_t k) {
-			return isnan(g[k]) ? u[k] : 0.0f; });
-		float dv = d.f ? -d.f[q] : 0.0f;
-		double b = (double)dv - a0;
-		double r = b - ax;
-		d.x[q] = (double)u[q];
-		d.r[q] = r;
-		d.p[q] = r;
+	const float *g = d.g, *uu = d.u;
+#pragma unroll
+	for (int u = 0; u < 4; u++) {
+		if (!hole[u]) continue;
+		const unsigned q = q0 + u;
+		const int j = (int)(q / (unsigned)d.w), i = (int)(q - (unsigned)j * d.w);
+		const double a0 = cg_stencil(d.bilap, i, j, d.w, d.h, [&](size_t t) {
+			const float v = g[t]; return isnan(v) ? 0.0f : v; });
+		const double ax = cg_stencil(d.bilap, i, j, d.w, d.h, [&](size_t t) {
+			return isnan(g[t]) ? uu[t] : 0.0f; });
+		const float dv = d.f ? -d.f[q] : 0.0f;
+		const double b = (double)dv - a0;
+		const double r = b - ax;
+		d.idx[k] = q;
+		d.x[k] = (double)uu[q];
+		d.r[k] = r;
+		d.p[k] = r;
 		d.pf[q] = (float)r;
-		d.q[q] = 0;
 		acc += r * r;
+		k++;
 	}
 	double s = block_sum_dbl<CG_NT>(acc, red);
 	if (threadIdx.x == 0) d.part[blockIdx.x] = s;
@@ -411,7 +488,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_setup(CgDev d)
 __global__ __launch_bounds__(CG_NT) void k_cg_rr0(CgDev d)
 {
 	__shared__ double red[CG_NT / 64];
-	double s = sum_partials(d.part, d.nblk, red);
+	double s = sum_partials(d.part, d.nb, red);
 	if (threadIdx.x == 0) {
 		d.rr[0] = s;
 		d.done[0] = 0;
@@ -424,12 +501,10 @@ static __device__ __forceinline__ bool cg_stopped(const CgDev &d)
 	return __hip_atomic_load(d.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
-// The iteration kernels walk the same grid-stride sequence of pixels per
-// thread as the setup (pixel t, t + S, t + 2S, ... with S = nblk * CG_NT)
-// and accumulate in that order, in batches of U pixels whose loads are
-// issued before the first value is used.  They find the unknowns in the
-// setup's byte mask (1 B per pixel instead of the 4 B of g).  Indices are
-// 32-bit (the solver takes at most 2^30 pixels per plane).
+// The iteration kernels walk the list with the same grid stride (entry t,
+// t + S, t + 2S, ... with S = nblk * CG_NT) and accumulate in that order, in
+// batches of U entries whose loads are issued before the first value is
+// used.
 
 // taps of A at (i, j) in the reference's term order (cleant_cgpois.c:91-121):
 // Poisson (i,j), (i+1,j), (i,j+1), (i-1,j), (i,j-1); the biharmonic adds the
@@ -477,38 +552,37 @@ __global__ __launch_bounds__(CG_NT) void k_cg_spmv(CgDev d)
 	if (cg_stopped(d)) return;
 	constexpr int NT = CgTaps<BILAP>::N;
 	const int w = d.w, h = d.h;
-	const unsigned n = (unsigned)w * (unsigned)h, S = (unsigned)d.nblk * CG_NT;
+	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_NT;
 	const float *pf = d.pf;
 	double acc = 0;
-	for (unsigned q0 = blockIdx.x * CG_NT + threadIdx.x; q0 < n; q0 += U * S) {
-		bool hole[U];
-#pragma unroll
-		for (int u = 0; u < U; u++) {
-			const unsigned q = q0 + u * S;
-			hole[u] = q < n && d.hole[q];
-		}
-		float t[U][NT];
+	for (unsigned k0 = blockIdx.x * CG_NT + threadIdx.x; k0 < nc; k0 += U * S) {
+		bool on[U];
+		unsigned qq[U];
 		double pc[U];
 #pragma unroll
 		for (int u = 0; u < U; u++) {
-			if (!hole[u]) continue;
-			const unsigned q = q0 + u * S;
-			const int j = (int)(q / (unsigned)w), i = (int)(q - (unsigned)j * w);
-			pc[u] = d.p[q];
+			const unsigned k = k0 + u * S;
+			on[u] = k < nc;
+			if (on[u]) { qq[u] = d.idx[k]; pc[u] = d.p[k]; }
+		}
+		float t[U][NT];
 #pragma unroll
-			for (int k = 0; k < NT; k++) {
+		for (int u = 0; u < U; u++) {
+			if (!on[u]) continue;
+			const int j = (int)(qq[u] / (unsigned)w), i = (int)(qq[u] - (unsigned)j * w);
+#pragma unroll
+			for (int e = 0; e < NT; e++) {
 				int di, dj;
-				cg_tap_offset(k, di, dj);
+				cg_tap_offset(e, di, dj);
 				const int a = clampi(i + di, 0, w - 1), b = clampi(j + dj, 0, h - 1);
-				t[u][k] = pf[(unsigned)b * (unsigned)w + (unsigned)a];
+				t[u][e] = pf[(unsigned)b * (unsigned)w + (unsigned)a];
 			}
 		}
 #pragma unroll
 		for (int u = 0; u < U; u++) {
-			if (!hole[u]) continue;
-			const unsigned q = q0 + u * S;
+			if (!on[u]) continue;
 			const double y = cg_apply_taps<BILAP>(t[u]);
-			d.q[q] = y;
+			d.q[k0 + u * S] = y;
 			acc += y * pc[u];
 		}
 	}
@@ -524,29 +598,22 @@ __global__ __launch_bounds__(CG_NT) void k_cg_update(CgDev d, int it)
 	if (cg_stopped(d)) return;
 	const double pq = sum_partials(d.part, d.nblk, red);
 	const double alpha = d.rr[it] / pq;
-	const unsigned n = (unsigned)d.w * (unsigned)d.h, S = (unsigned)d.nblk * CG_NT;
+	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_NT;
 	double acc = 0;
-	for (unsigned q0 = blockIdx.x * CG_NT + threadIdx.x; q0 < n; q0 += U * S) {
-		bool hole[U];
+	for (unsigned k0 = blockIdx.x * CG_NT + threadIdx.x; k0 < nc; k0 += U * S) {
 		double xv[U], pv[U], rv[U], qv[U];
 #pragma unroll
 		for (int u = 0; u < U; u++) {
-			const unsigned q = q0 + u * S;
-			hole[u] = q < n && d.hole[q];
+			const unsigned k = k0 + u * S;
+			if (k < nc) { xv[u] = d.x[k]; pv[u] = d.p[k]; rv[u] = d.r[k]; qv[u] = d.q[k]; }
 		}
 #pragma unroll
 		for (int u = 0; u < U; u++) {
-			if (!hole[u]) continue;
-			const unsigned q = q0 + u * S;
-			xv[u] = d.x[q]; pv[u] = d.p[q]; rv[u] = d.r[q]; qv[u] = d.q[q];
-		}
-#pragma unroll
-		for (int u = 0; u < U; u++) {
-			if (!hole[u]) continue;
-			const unsigned q = q0 + u * S;
-			d.x[q] = xv[u] + alpha * pv[u];
+			const unsigned k = k0 + u * S;
+			if (k >= nc) continue;
+			d.x[k] = xv[u] + alpha * pv[u];
 			const double r = rv[u] - alpha * qv[u];
-			d.r[q] = r;
+			d.r[k] = r;
 			acc += r * r;
 		}
 	}
@@ -554,7 +621,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_update(CgDev d, int it)
 	if (threadIdx.x == 0) d.part[d.nblk + blockIdx.x] = s;
 }
 
-// rr_new; absolute stop test; beta; p = r + beta p
+// rr_new; absolute stop test; beta; p = r + beta p (and its float copy)
 template <int U>
 __global__ __launch_bounds__(CG_NT) void k_cg_dir(CgDev d, int it)
 {
@@ -571,27 +638,22 @@ __global__ __launch_bounds__(CG_NT) void k_cg_dir(CgDev d, int it)
 		return;
 	}
 	const double beta = rr1 / rr0;
-	const unsigned n = (unsigned)d.w * (unsigned)d.h, S = (unsigned)d.nblk * CG_NT;
-	for (unsigned q0 = blockIdx.x * CG_NT + threadIdx.x; q0 < n; q0 += U * S) {
-		bool hole[U];
+	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_NT;
+	for (unsigned k0 = blockIdx.x * CG_NT + threadIdx.x; k0 < nc; k0 += U * S) {
 		double rv[U], pv[U];
+		unsigned qq[U];
 #pragma unroll
 		for (int u = 0; u < U; u++) {
-			const unsigned q = q0 + u * S;
-			hole[u] = q < n && d.hole[q];
+			const unsigned k = k0 + u * S;
+			if (k < nc) { rv[u] = d.r[k]; pv[u] = d.p[k]; qq[u] = d.idx[k]; }
 		}
 #pragma unroll
 		for (int u = 0; u < U; u++) {
-			if (!hole[u]) continue;
-			const unsigned q = q0 + u * S;
-			rv[u] = d.r[q]; pv[u] = d.p[q];
-		}
-#pragma unroll
-		for (int u = 0; u < U; u++) {
-			if (!hole[u]) continue;
+			const unsigned k = k0 + u * S;
+			if (k >= nc) continue;
 			const double pn = rv[u] + beta * pv[u];
-			d.p[q0 + u * S] = pn;
-			d.pf[q0 + u * S] = (float)pn;
+			d.p[k] = pn;
+			d.pf[qq[u]] = (float)pn;
 		}
 	}
 }
@@ -599,15 +661,21 @@ __global__ __launch_bounds__(CG_NT) void k_cg_dir(CgDev d, int it)
 // u[holes] = (float) x
 __global__ __launch_bounds__(CG_NT) void k_cg_finish(CgDev d)
 {
-	const size_t n = (size_t)d.w * d.h;
-	for (size_t q = (size_t)blockIdx.x * CG_NT + threadIdx.x; q < n;
-			q += (size_t)d.nblk * CG_NT)
-		if (isnan(d.g[q])) d.u[q] = (float)d.x[q];
+	const unsigned nc = (unsigned)d.nc;
+	for (unsigned k = blockIdx.x * CG_NT + threadIdx.x; k < nc; k += (unsigned)d.nblk * CG_NT)
+		d.u[d.idx[k]] = (float)d.x[k];
+}
+
+int cg_setup_blocks(int w, int h)
+{
+	return (int)(((size_t)w * h + CG_SPAN - 1) / CG_SPAN);
 }
 
 void launch_cg_setup(const CgDev &d, hipStream_t s)
 {
-	k_cg_setup<<<d.nblk, CG_NT, 0, s>>>(d);
+	k_cg_count<<<d.nb, CG_NT, 0, s>>>(d);
+	k_cg_scan<<<1, CG_SCAN_T, 0, s>>>(d);
+	k_cg_setup<<<d.nb, CG_NT, 0, s>>>(d);
 	k_cg_rr0<<<1, CG_NT, 0, s>>>(d);
 }
 
