@@ -75,6 +75,8 @@ _SIGNATURES = {
                              _P, ctypes.POINTER(sp_stats)]),
     "sp_residual_device": (_I, [_P, _P, _P, _I, _I, _I, _I,
                                 ctypes.POINTER(ctypes.c_double), _P]),
+    "sp_residual_device_noalloc": (_I, [_P, _P, _P, _I, _I, _I, _I,
+                                        ctypes.POINTER(ctypes.c_double), _P]),
     "sp_solve_batch_device": (_I, [_I, _P, ctypes.POINTER(sp_params), _P,
                                    ctypes.POINTER(sp_stats)]),
     "sp_solve_batch_device_ex": (_I, [_I, _P, ctypes.POINTER(sp_params), _P,

@@ -2099,6 +2099,75 @@ extern "C" int sp_residual_device(const float *d_u, const float *d_g,
 	return SP_OK;
 }
 
+// ------------------------------------------- residual without allocation
+//
+// Device memory of sp_residual_device_noalloc: one fixed array per GPU,
+// held until the process ends.  In host code `scratch` only names the
+// symbol; its device address comes from hipGetSymbolAddress, once per
+// device with that device current, and only that address is used.
+constexpr size_t SP_NOALLOC_SCRATCH_BYTES = (size_t)4 << 20;
+static __device__ __align__(256) unsigned char scratch[SP_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::vector<unsigned char *> g_scratch_dev;   // address on each device
+
+static int scratch_address(unsigned char **p)
+{
+	int dev = 0;
+	HIPCHK(hipGetDevice(&dev));
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	if ((int)g_scratch_dev.size() <= dev) g_scratch_dev.resize(dev + 1, nullptr);
+	if (!g_scratch_dev[dev]) {
+		void *q = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !q)
+			return set_err(SP_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s",
+					dev, hipGetErrorString(e));
+		g_scratch_dev[dev] = (unsigned char *)q;
+	}
+	*p = g_scratch_dev[dev];
+	return SP_OK;
+}
+
+extern "C" int sp_residual_device_noalloc(const float *d_u, const float *d_g,
+		const float *d_f, int w, int h, int pd, int bilap, double *rel, void *stream)
+{
+	if (!d_u || !d_g || !rel || w < 1 || h < 1 || pd < 1)
+		return set_err(SP_ERR_ARG, "bad residual arguments");
+	const size_t n = (size_t)w * h;
+	if (n > 0x7fffffffUL) return set_err(SP_ERR_ARG, "plane too large");
+	// scratch layout: descriptors, work blocks, first block of each plane,
+	// per-block partial sums, per-plane (rr, bb, rel); every entry a kernel
+	// reads is written earlier in the same call, so nothing is cleared
+	const size_t per = (n + EW_SPAN - 1) / EW_SPAN, nb = per * (size_t)pd;
+	size_t off = 0;
+	auto take = [&](size_t bytes) { const size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+	const size_t o_lv = take((size_t)pd * sizeof(SpPlane)),
+		o_bw = take(nb * sizeof(BlockWork)), o_first = take(((size_t)pd + 1) * sizeof(int)),
+		o_part = take(2 * nb * sizeof(double)), o_out = take(3 * (size_t)pd * sizeof(double));
+	if (off > SP_NOALLOC_SCRATCH_BYTES)
+		return set_err(SP_ERR_ARG, "residual of %d planes of %dx%d needs %zu bytes of "
+				"scratch, more than the %zu the allocation-free form holds", pd, w, h,
+				off, SP_NOALLOC_SCRATCH_BYTES);
+	TRY(ensure_device());
+	unsigned char *base = nullptr;
+	TRY(scratch_address(&base));
+	hipStream_t s = (hipStream_t)stream;
+	SpPlane *lv = (SpPlane *)(base + o_lv);
+	BlockWork *bw = (BlockWork *)(base + o_bw);
+	int *first = (int *)(base + o_first);
+	double *part = (double *)(base + o_part), *out = (double *)(base + o_out);
+	spk::launch_residual_plan(lv, bw, first, d_u, d_g, d_f, w, h, pd, (int)per, s);
+	spk::launch_residual(lv, bw, (int)nb, bilap, part, s);
+	spk::launch_residual_reduce(part, first, pd, out, s);
+	HIPCHK(hipGetLastError());
+	// rel[p] = out[3p + 2]: one strided copy, no host buffer
+	HIPCHK(hipMemcpy2DAsync(rel, sizeof(double), out + 2, 3 * sizeof(double),
+			sizeof(double), pd, hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
+	return SP_OK;
+}
+
 // --------------------------------------------------------- CG drop-ins
 
 // single-level masked CG from `init`, cleant_cgpois.c:197-264

@@ -130,5 +130,9 @@ void launch_residual(const SpPlane *lv, const BlockWork *bw, int nblocks,
 		int bilap, double *part, hipStream_t s);
 void launch_residual_reduce(const double *part, const int *plane_first,
 		int nplanes, double *out, hipStream_t s);
+// lv[pd], bw[pd * per] (per blocks of 1024 pixels per plane) and first[pd + 1]
+// of the residual of pd planar w x h planes, written by a kernel
+void launch_residual_plan(SpPlane *lv, BlockWork *bw, int *first, const float *u,
+		const float *g, const float *f, int w, int h, int pd, int per, hipStream_t s);
 
 } // namespace spk

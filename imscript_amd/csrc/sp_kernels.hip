@@ -767,6 +767,30 @@ void launch_residual(const SpPlane *lv, const BlockWork *bw, int nblocks,
 	if (nblocks) k_residual<<<nblocks, EW_NT, 0, s>>>(lv, bw, bilap, part);
 }
 
+// the descriptors, work blocks (per of EW_NT * EW_PPT pixels for each of pd
+// planar w x h planes) and first blocks of a residual, written on the device
+__global__ __launch_bounds__(EW_NT) void k_residual_plan(SpPlane *lv, BlockWork *bw,
+		int *first, const float *u, const float *g, const float *f, int w, int h,
+		int pd, int per)
+{
+	const int t = blockIdx.x * EW_NT + threadIdx.x;
+	const size_t n = (size_t)w * h;
+	if (t < pd * per) bw[t] = BlockWork{t / per, (t % per) * (EW_NT * EW_PPT)};
+	if (t < pd)
+		lv[t] = SpPlane{const_cast<float *>(g) + n * t,
+			f ? const_cast<float *>(f) + n * t : nullptr,
+			const_cast<float *>(u) + n * t, nullptr, nullptr, w, h};
+	if (t <= pd) first[t] = t * per;
+}
+
+void launch_residual_plan(SpPlane *lv, BlockWork *bw, int *first, const float *u,
+		const float *g, const float *f, int w, int h, int pd, int per, hipStream_t s)
+{
+	const int n = std::max(pd * per, pd + 1);
+	k_residual_plan<<<(n + EW_NT - 1) / EW_NT, EW_NT, 0, s>>>(lv, bw, first, u, g, f,
+			w, h, pd, per);
+}
+
 // plain streaming copy: each thread moves 4 x 16 bytes (all loads issued
 // before the stores), one pass over the buffer; the measured bandwidth
 // ceiling of the device and the calibration of the HBM counters

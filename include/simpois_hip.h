@@ -266,6 +266,20 @@ int sp_profile_read(const char *name, double *ms, long *launches, double *bytes)
 int sp_residual_device(const float *d_u, const float *d_g, const float *d_f,
 		int w, int h, int pd, int bilap, double *rel, void *stream);
 
+/* sp_residual_device without any allocation: the same kernels and the same
+ * results (bit for bit), with the work-block descriptors, per-block partial
+ * sums and per-plane totals in a fixed 4 MiB device array that each GPU the
+ * library runs on holds until the process ends (planes up to about 170 M
+ * pixels in total; larger calls fail with SP_ERR_ARG).  It calls no
+ * hipMalloc, hipFree or other allocating routine.  Every call on one GPU
+ * uses that GPU's array, so calls that overlap on one GPU (other threads,
+ * other streams) race.  Synchronises `stream` before returning.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under graph capture, serving loops, harnesses that forbid
+ * hipMalloc). */
+int sp_residual_device_noalloc(const float *d_u, const float *d_g, const float *d_f,
+		int w, int h, int pd, int bilap, double *rel, void *stream);
+
 /* plambda's 3x3 Laplacian image operator on pd planar w x h float planes
  * (the producer of simpois's guidance field; replaces `plambda in "x,l"`,
  * plambda.c:2589-2605 stencils and :2680-2689 apply_3x3_stencil, on one
