@@ -360,13 +360,19 @@ struct Plan {
 	int *d_cnt = nullptr;
 	float **d_outs = nullptr;
 	// FP64 CG scratch, sized for the largest plane
-	double *cg_x = nullptr, *cg_r = nullptr, *cg_p = nullptr, *cg_q = nullptr,
+	float *cg_q = nullptr;
+	double *cg_x = nullptr, *cg_r = nullptr, *cg_p = nullptr,
 		*cg_part = nullptr, *cg_rr = nullptr;
 	unsigned *cg_idx = nullptr;         // pixel of each unknown of the plane solved
 	int *cg_bcnt = nullptr;             // unknowns per setup block, prefix sums
 	float *cg_pf = nullptr;             // direction rounded to float (full grid)
 	int cg_rr_cap = 0;
 	int *cg_done = nullptr, *h_done = nullptr;
+	spk::CgDev *cg_dev = nullptr;       // descriptor the FP64 CG kernels read
+	// FP64 CG iterations captured in chunks of 16 << i (i = 0..2), shared by
+	// every level and plane (they read *cg_dev); built on first use
+	hipGraphExec_t cg64_exec[3] = {};
+	int cg64_bilap = -1, cg64_grid = 0;
 	// pinned staging of small per-solve copies (descriptors in, results out):
 	// copies to and from pageable memory each cost a host round trip
 	void *h_stage = nullptr;
@@ -394,6 +400,8 @@ struct Plan {
 	{
 		for (auto &L : lv)
 			if (L.cg.exec) (void)hipGraphExecDestroy(L.cg.exec);
+		for (auto &e : cg64_exec)
+			if (e) (void)hipGraphExecDestroy(e);
 		for (auto &a : allocs) dev_free(a.first, a.second);
 		if (h_done) (void)hipHostFree(h_done);
 		if (h_stage) (void)hipHostFree(h_stage);
@@ -824,12 +832,41 @@ static int cg64_ensure(Plan &pl, int maxit)
 		TRY(pl.alloc(&pl.cg_pf, n0));
 		TRY(pl.alloc(&pl.cg_part, std::max<size_t>(2 * 1024, nb0)));
 		TRY(pl.alloc(&pl.cg_done, 4));
+		TRY(pl.alloc(&pl.cg_dev, 1));
 	}
 	if (maxit + 2 > pl.cg_rr_cap) {
 		const int cap = std::max(maxit + 2, 1024);
 		TRY(pl.alloc(&pl.cg_rr, cap));
 		pl.cg_rr_cap = cap;
 	}
+	return SP_OK;
+}
+
+// The FP64 CG's captured chunks: 16, 32 and 64 iterations over a
+// grid that covers the plan's largest plane (a level with fewer blocks
+// leaves the rest idle)
+static int cg64_graphs(Plan &pl, int bilap)
+{
+	if (pl.cg64_bilap == bilap) return SP_OK;
+	for (auto &e : pl.cg64_exec)
+		if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
+	size_t n0 = 0;
+	for (auto &p : pl.planes) n0 = std::max(n0, (size_t)p.w * p.h);
+	pl.cg64_grid = (int)std::min<size_t>(1024, std::max<size_t>(1, (n0 + 1023) / 1024));
+	if (!pl.cap_stream)
+		HIPCHK(hipStreamCreateWithFlags(&pl.cap_stream, hipStreamNonBlocking));
+	for (int i = 0; i < 3; i++) {
+		hipGraph_t graph;
+		HIPCHK(hipStreamBeginCapture(pl.cap_stream, hipStreamCaptureModeThreadLocal));
+		for (int z = 0; z < (16 << i); z++)
+			spk::launch_cg_iter(pl.cg_dev, bilap, pl.cg64_grid, pl.cap_stream);
+		HIPCHK(hipStreamEndCapture(pl.cap_stream, &graph));
+		const hipError_t ie = hipGraphInstantiate(&pl.cg64_exec[i], graph, nullptr,
+				nullptr, 0);
+		(void)hipGraphDestroy(graph);
+		HIPCHK(ie);
+	}
+	pl.cg64_bilap = bilap;
 	return SP_OK;
 }
 
@@ -862,21 +899,40 @@ static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 	d.nb = spk::cg_setup_blocks(S.w, S.h);
 	d.nc = nmask;
 	d.stop = (double)e;
+	TRY(cg64_graphs(pl, bilap));
+	// the descriptor into device memory, in stream order (no copy in
+	// flight uses the staging area: the hole count above was waited for)
+	TRY(pl.stage(sizeof d));
+	memcpy(pl.h_stage, &d, sizeof d);
+	HIPCHK(hipMemcpyAsync(pl.cg_dev, pl.h_stage, sizeof d, hipMemcpyHostToDevice, s));
 	ProfScope pr("cg64", s);
-	spk::launch_cg_setup(d, s);
-	int it = 0, chunk = 16, ran = maxit;
+	spk::launch_cg_setup(pl.cg_dev, d.nb, s);
+	// chunks of 16, 32, then 64 iterations from the captured graphs while a
+	// whole chunk fits under maxit, the rest launched one iteration at a
+	// time; the stop flag is read after each chunk (iterations after the
+	// stop return at once: 64 keeps those few against one host round trip
+	// per chunk)
+	int it = 0, ci = 0, ran = maxit;
 	while (it < maxit) {
-		const int c = std::min(chunk, maxit - it);
-		for (int z = 0; z < c; z++) spk::launch_cg_iter(d, it + z, s);
+		while (ci > 0 && (16 << ci) > maxit - it) ci--;
+		int c;
+		if ((16 << ci) <= maxit - it) {
+			c = 16 << ci;
+			HIPCHK(hipGraphLaunch(pl.cg64_exec[ci], s));
+			if (ci < 2) ci++;
+		} else {
+			c = maxit - it;
+			for (int z = 0; z < c; z++)
+				spk::launch_cg_iter(pl.cg_dev, bilap, nblk, s);
+		}
 		it += c;
 		HIPCHK(hipGetLastError());
 		HIPCHK(hipMemcpyAsync(pl.h_done, pl.cg_done, 2 * sizeof(int),
 				hipMemcpyDeviceToHost, s));
 		HIPCHK(hipStreamSynchronize(s));
 		if (pl.h_done[0]) { ran = pl.h_done[1]; break; }
-		chunk = std::min(chunk * 2, 256);
 	}
-	spk::launch_cg_finish(d, s);
+	spk::launch_cg_finish(pl.cg_dev, nblk, s);
 	HIPCHK(hipGetLastError());
 	// FP64 vectors: 88 B per unknown and iteration (twice the FP32 vectors)
 	pr.add(3L * it, 88.0 * ran * nmask);

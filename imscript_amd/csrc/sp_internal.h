@@ -100,24 +100,30 @@ void launch_gs_small(int R, const GsInst *insts, const int *ids, int nids,
 struct CgDev {
 	const float *g, *f;   // plane data
 	float *u;             // solution plane (read for x0, written at the end)
-	double *x, *r, *p, *q;   // nc entries each
+	double *x, *r, *p;    // nc entries each
+	float *q;             // A p: the stencil sums in float, so q is a float
+	                      // (the reference widens it to double)
 	unsigned *idx;        // pixel of each unknown, written by the setup
 	int *bcnt;            // unknowns per setup block, then their prefix sums (nb + 1)
 	float *pf;            // full grid: p rounded to float at the unknowns (what
 	                      // the stencil reads), 0 at known pixels
 	double *part;         // per-block partial sums (max(2 * nblk, nb))
 	double *rr;           // rr history, rr[0] = r0.r0, rr[k+1] = after iter k
-	int *done;            // [0] = stop flag, [1] = iterations run
+	int *done;            // [0] = stop flag, [1] = iterations run, [2] = index
+	                      // of the iteration under way
 	int w, h, bilap;
 	int nblk;             // blocks of the iteration kernels
 	int nb;               // setup blocks: cg_setup_blocks(w, h)
 	int nc;               // unknowns (NaN pixels of g)
 	double stop;          // absolute stop threshold on sqrt(rr)
 };
+// The kernels read the descriptor from device memory (dp), so captured
+// iterations serve every level and plane: the host rewrites *dp between
+// them.  An iteration launch's grid may exceed d.nblk (extra blocks return).
 int cg_setup_blocks(int w, int h);
-void launch_cg_setup(const CgDev &d, hipStream_t s);
-void launch_cg_iter(const CgDev &d, int it, hipStream_t s);
-void launch_cg_finish(const CgDev &d, hipStream_t s);
+void launch_cg_setup(const CgDev *dp, int nb, hipStream_t s);
+void launch_cg_iter(const CgDev *dp, int bilap, int grid, hipStream_t s);
+void launch_cg_finish(const CgDev *dp, int nblk, hipStream_t s);
 
 // adjoint of the converged Poisson fill
 void launch_adj_prepare(const AdjPlane *ap, const BlockWork *bw, int nblocks,

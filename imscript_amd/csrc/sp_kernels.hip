@@ -374,6 +374,21 @@ static __device__ double sum_partials(const double *part, int nblk, double *red)
 	return bc;
 }
 
+// sums of a[0..n) and b[0..n), each in exactly the order sum_partials uses
+static __device__ double2 sum_partials2(const double *a, const double *b, int n,
+		double *red)
+{
+	double va = 0, vb = 0;
+#pragma unroll 4
+	for (int k = threadIdx.x; k < n; k += CG_NT) { va += a[k]; vb += b[k]; }
+	const double sa = block_sum_dbl<CG_NT>(va, red);
+	const double sb = block_sum_dbl<CG_NT>(vb, red);
+	__shared__ double2 bc;
+	if (threadIdx.x == 0) bc = make_double2(sa, sb);
+	__syncthreads();
+	return bc;
+}
+
 // The FP64 CG keeps its vectors compact, one entry per unknown in raster
 // order (as the reference's mask3 list does, cleant_cgpois.c:6-25): the
 // setup lists the unknowns (idx), and the iteration kernels stream over that
@@ -383,8 +398,9 @@ static __device__ double sum_partials(const double *part, int nblk, double *red)
 constexpr int CG_SPAN = CG_NT * 4;   // pixels per setup block (4 per thread)
 
 // holes of each CG_SPAN-pixel block into bcnt; pf = 0 at known pixels
-__global__ __launch_bounds__(CG_NT) void k_cg_count(CgDev d)
+__global__ __launch_bounds__(CG_NT) void k_cg_count(const CgDev *dp)
 {
+	const CgDev d = *dp;
 	__shared__ int red[CG_NT / 64];
 	const unsigned n = (unsigned)d.w * (unsigned)d.h;
 	const unsigned q0 = blockIdx.x * CG_SPAN + threadIdx.x * 4;
@@ -409,8 +425,9 @@ __global__ __launch_bounds__(CG_NT) void k_cg_count(CgDev d)
 // exclusive prefix sum of bcnt[0..nb) in place, bcnt[nb] = total (one
 // work-group: each thread sums a contiguous run, the runs are scanned)
 constexpr int CG_SCAN_T = 1024;
-__global__ __launch_bounds__(CG_SCAN_T) void k_cg_scan(CgDev d)
+__global__ __launch_bounds__(CG_SCAN_T) void k_cg_scan(const CgDev *dp)
 {
+	const CgDev d = *dp;
 	__shared__ int tot[CG_SCAN_T];
 	const int nb = d.nb, per = (nb + CG_SCAN_T - 1) / CG_SCAN_T;
 	const int a = threadIdx.x * per, e = min(a + per, nb);
@@ -435,8 +452,9 @@ __global__ __launch_bounds__(CG_SCAN_T) void k_cg_scan(CgDev d)
 
 // list the unknowns of the block (raster order) and set up each:
 // x0 = u; b = -f - A(boundary); r = b - A x0; p = r; partial r.r
-__global__ __launch_bounds__(CG_NT) void k_cg_setup(CgDev d)
+__global__ __launch_bounds__(CG_NT) void k_cg_setup(const CgDev *dp)
 {
+	const CgDev d = *dp;
 	__shared__ double red[CG_NT / 64];
 	__shared__ int wsum[CG_NT / 64];
 	const unsigned n = (unsigned)d.w * (unsigned)d.h;
@@ -485,14 +503,16 @@ __global__ __launch_bounds__(CG_NT) void k_cg_setup(CgDev d)
 	if (threadIdx.x == 0) d.part[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(CG_NT) void k_cg_rr0(CgDev d)
+__global__ __launch_bounds__(CG_NT) void k_cg_rr0(const CgDev *dp)
 {
+	const CgDev d = *dp;
 	__shared__ double red[CG_NT / 64];
 	double s = sum_partials(d.part, d.nb, red);
 	if (threadIdx.x == 0) {
 		d.rr[0] = s;
 		d.done[0] = 0;
 		d.done[1] = 0;
+		d.done[2] = -1;   // iteration index, advanced by each k_cg_spmv
 	}
 }
 
@@ -546,10 +566,15 @@ static __device__ __forceinline__ double cg_apply_taps(const float (&t)[CgTaps<B
 
 // q = A p, partial p.q
 template <int BILAP, int U>
-__global__ __launch_bounds__(CG_NT) void k_cg_spmv(CgDev d)
+__global__ __launch_bounds__(CG_NT) void k_cg_spmv(const CgDev *dp)
 {
+	const CgDev d = *dp;
 	__shared__ double red[CG_NT / 64];
-	if (cg_stopped(d)) return;
+	// the launch may have more blocks than this level uses (a graph's grid
+	// is fixed); a block's whole work is skipped, so no barrier is split
+	if (blockIdx.x >= (unsigned)d.nblk || cg_stopped(d)) return;
+	// the first kernel of an iteration advances the index the next two read
+	if (blockIdx.x == 0 && threadIdx.x == 0) d.done[2] += 1;
 	constexpr int NT = CgTaps<BILAP>::N;
 	const int w = d.w, h = d.h;
 	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_NT;
@@ -582,7 +607,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_spmv(CgDev d)
 		for (int u = 0; u < U; u++) {
 			if (!on[u]) continue;
 			const double y = cg_apply_taps<BILAP>(t[u]);
-			d.q[k0 + u * S] = y;
+			d.q[k0 + u * S] = (float)y;   // y is a float widened: exact
 			acc += y * pc[u];
 		}
 	}
@@ -590,77 +615,105 @@ __global__ __launch_bounds__(CG_NT) void k_cg_spmv(CgDev d)
 	if (threadIdx.x == 0) d.part[blockIdx.x] = s;
 }
 
-// alpha = rr_old / p.q ; x += alpha p ; r -= alpha q ; partial r.r
+// alpha = rr_old / p.q ; r -= alpha q ; partial r.r.  The vectors' loads
+// issue before the partials are summed, so their latency overlaps the sum.
 template <int U>
-__global__ __launch_bounds__(CG_NT) void k_cg_update(CgDev d, int it)
+__global__ __launch_bounds__(CG_NT) void k_cg_update(const CgDev *dp)
 {
+	const CgDev d = *dp;
 	__shared__ double red[CG_NT / 64];
-	if (cg_stopped(d)) return;
-	const double pq = sum_partials(d.part, d.nblk, red);
-	const double alpha = d.rr[it] / pq;
+	__shared__ double alpha_s;
+	if (blockIdx.x >= (unsigned)d.nblk || cg_stopped(d)) return;
+	const int it = d.done[2];
 	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_NT;
 	double acc = 0;
-	for (unsigned k0 = blockIdx.x * CG_NT + threadIdx.x; k0 < nc; k0 += U * S) {
-		double xv[U], pv[U], rv[U], qv[U];
+	for (unsigned k0 = blockIdx.x * CG_NT + threadIdx.x, first = 1; ; k0 += U * S, first = 0) {
+		// every thread of the block takes part in the first pass (the
+		// partials' sum has barriers), later passes only while in range
+		if (!first && k0 >= nc) break;
+		double rv[U], qv[U];
 #pragma unroll
 		for (int u = 0; u < U; u++) {
 			const unsigned k = k0 + u * S;
-			if (k < nc) { xv[u] = d.x[k]; pv[u] = d.p[k]; rv[u] = d.r[k]; qv[u] = d.q[k]; }
+			if (k < nc) { rv[u] = d.r[k]; qv[u] = (double)d.q[k]; }
 		}
+		if (first) {
+			const double pq = sum_partials(d.part, d.nblk, red);
+			if (threadIdx.x == 0) alpha_s = d.rr[it] / pq;
+			__syncthreads();
+		}
+		const double alpha = alpha_s;
 #pragma unroll
 		for (int u = 0; u < U; u++) {
 			const unsigned k = k0 + u * S;
 			if (k >= nc) continue;
-			d.x[k] = xv[u] + alpha * pv[u];
 			const double r = rv[u] - alpha * qv[u];
 			d.r[k] = r;
 			acc += r * r;
 		}
+		if (k0 + U * S >= nc) break;
 	}
 	double s = block_sum_dbl<CG_NT>(acc, red);
 	if (threadIdx.x == 0) d.part[d.nblk + blockIdx.x] = s;
 }
 
-// rr_new; absolute stop test; beta; p = r + beta p (and its float copy)
+// rr_new; x += alpha p (alpha recomputed from the same partials, in the same
+// order, as k_cg_update did); absolute stop test; beta; p = r + beta p and
+// its float copy.  x is updated here, where p is read anyway, also in the
+// iteration that stops.
 template <int U>
-__global__ __launch_bounds__(CG_NT) void k_cg_dir(CgDev d, int it)
+__global__ __launch_bounds__(CG_NT) void k_cg_dir(const CgDev *dp)
 {
+	const CgDev d = *dp;
 	__shared__ double red[CG_NT / 64];
-	if (cg_stopped(d)) return;
-	const double rr1 = sum_partials(d.part + d.nblk, d.nblk, red);
-	const double rr0 = d.rr[it];
-	if (blockIdx.x == 0 && threadIdx.x == 0) d.rr[it + 1] = rr1;
-	if (sqrt(rr1) < d.stop) {
-		if (blockIdx.x == 0 && threadIdx.x == 0) {
-			d.done[1] = it + 1;
-			__hip_atomic_store(d.done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-		}
-		return;
-	}
-	const double beta = rr1 / rr0;
+	__shared__ double alpha_s, rr1_s;
+	if (blockIdx.x >= (unsigned)d.nblk || cg_stopped(d)) return;
+	const int it = d.done[2];
 	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_NT;
-	for (unsigned k0 = blockIdx.x * CG_NT + threadIdx.x; k0 < nc; k0 += U * S) {
-		double rv[U], pv[U];
+	const double rr0 = d.rr[it];
+	for (unsigned k0 = blockIdx.x * CG_NT + threadIdx.x, first = 1; ; k0 += U * S, first = 0) {
+		if (!first && k0 >= nc) break;
+		double rv[U], pv[U], xv[U];
 		unsigned qq[U];
 #pragma unroll
 		for (int u = 0; u < U; u++) {
 			const unsigned k = k0 + u * S;
-			if (k < nc) { rv[u] = d.r[k]; pv[u] = d.p[k]; qq[u] = d.idx[k]; }
+			if (k < nc) { rv[u] = d.r[k]; pv[u] = d.p[k]; xv[u] = d.x[k]; qq[u] = d.idx[k]; }
 		}
+		if (first) {
+			const double2 t = sum_partials2(d.part, d.part + d.nblk, d.nblk, red);
+			if (threadIdx.x == 0) { alpha_s = rr0 / t.x; rr1_s = t.y; }
+			__syncthreads();
+		}
+		const double alpha = alpha_s, rr1 = rr1_s;
+		const bool stop = sqrt(rr1) < d.stop;
+		const double beta = rr1 / rr0;
 #pragma unroll
 		for (int u = 0; u < U; u++) {
 			const unsigned k = k0 + u * S;
 			if (k >= nc) continue;
+			d.x[k] = xv[u] + alpha * pv[u];
+			if (stop) continue;
 			const double pn = rv[u] + beta * pv[u];
 			d.p[k] = pn;
 			d.pf[qq[u]] = (float)pn;
+		}
+		if (k0 + U * S >= nc) break;
+	}
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		const double rr1 = rr1_s;
+		d.rr[it + 1] = rr1;
+		if (sqrt(rr1) < d.stop) {
+			d.done[1] = it + 1;
+			__hip_atomic_store(d.done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		}
 	}
 }
 
 // u[holes] = (float) x
-__global__ __launch_bounds__(CG_NT) void k_cg_finish(CgDev d)
+__global__ __launch_bounds__(CG_NT) void k_cg_finish(const CgDev *dp)
 {
+	const CgDev d = *dp;
 	const unsigned nc = (unsigned)d.nc;
 	for (unsigned k = blockIdx.x * CG_NT + threadIdx.x; k < nc; k += (unsigned)d.nblk * CG_NT)
 		d.u[d.idx[k]] = (float)d.x[k];
@@ -671,25 +724,25 @@ int cg_setup_blocks(int w, int h)
 	return (int)(((size_t)w * h + CG_SPAN - 1) / CG_SPAN);
 }
 
-void launch_cg_setup(const CgDev &d, hipStream_t s)
+void launch_cg_setup(const CgDev *dp, int nb, hipStream_t s)
 {
-	k_cg_count<<<d.nb, CG_NT, 0, s>>>(d);
-	k_cg_scan<<<1, CG_SCAN_T, 0, s>>>(d);
-	k_cg_setup<<<d.nb, CG_NT, 0, s>>>(d);
-	k_cg_rr0<<<1, CG_NT, 0, s>>>(d);
+	k_cg_count<<<nb, CG_NT, 0, s>>>(dp);
+	k_cg_scan<<<1, CG_SCAN_T, 0, s>>>(dp);
+	k_cg_setup<<<nb, CG_NT, 0, s>>>(dp);
+	k_cg_rr0<<<1, CG_NT, 0, s>>>(dp);
 }
 
-void launch_cg_iter(const CgDev &d, int it, hipStream_t s)
+void launch_cg_iter(const CgDev *dp, int bilap, int grid, hipStream_t s)
 {
-	if (d.bilap) k_cg_spmv<1, 4><<<d.nblk, CG_NT, 0, s>>>(d);
-	else k_cg_spmv<0, 4><<<d.nblk, CG_NT, 0, s>>>(d);
-	k_cg_update<4><<<d.nblk, CG_NT, 0, s>>>(d, it);
-	k_cg_dir<4><<<d.nblk, CG_NT, 0, s>>>(d, it);
+	if (bilap) k_cg_spmv<1, 4><<<grid, CG_NT, 0, s>>>(dp);
+	else k_cg_spmv<0, 4><<<grid, CG_NT, 0, s>>>(dp);
+	k_cg_update<4><<<grid, CG_NT, 0, s>>>(dp);
+	k_cg_dir<4><<<grid, CG_NT, 0, s>>>(dp);
 }
 
-void launch_cg_finish(const CgDev &d, hipStream_t s)
+void launch_cg_finish(const CgDev *dp, int nblk, hipStream_t s)
 {
-	k_cg_finish<<<d.nblk, CG_NT, 0, s>>>(d);
+	k_cg_finish<<<nblk, CG_NT, 0, s>>>(dp);
 }
 
 // ------------------------------------------------------------- residual
