@@ -44,11 +44,11 @@ PEAK_HBM_GBPS = 8000.0
 # measured on: TRAFFIC_SRC is the SHA-256 prefix of imscript_amd/csrc/sp_cg.hip
 # then, and the line reports traffic null when the source differs.
 TRAFFIC_PMC = {
-    "mg_up0": 644.7e6,
+    "mg_up0": 644.6e6,
     "mg_down0": 495.5e6,
-    "k32_dirspmv": 1026.9e6,
+    "k32_dirspmv": 1026.7e6,
 }
-TRAFFIC_SRC = "unset"
+TRAFFIC_SRC = "903f9a667bcaa9e3"
 
 
 def kernel_source_id(name="sp_cg.hip"):
