@@ -1,14 +1,18 @@
 """Differentiable Poisson hole filling (torch.autograd).
 
-Forward: the converged masked Poisson fill of simpois (simpois.c:247-285 with
-CG to a relative tolerance): holes u_h solve A u_h = -f_h - A_hk g_k with
-A = -Laplacian (clamped 5-point, cleant_cgpois.c:67-89), known pixels are
-copied from g.
-
-Backward: A is symmetric, so the adjoint is one more masked Laplacian solve,
-A lambda = dL/du on the holes, run on the GPU by the same pyramid and CG
+poisson_fill -- the converged masked Poisson fill of simpois (simpois.c:
+247-285 with CG to a relative tolerance): holes u_h solve
+A u_h = -f_h - A_hk g_k with A = -Laplacian (clamped 5-point,
+cleant_cgpois.c:67-89), known pixels are copied from g.  Backward: A is
+symmetric, so the adjoint is one more masked Laplacian solve, A lambda =
+dL/du on the holes, run on the GPU by the same pyramid and CG
 (sp_adjoint_batch_device); then dL/df = -lambda on the holes and
 dL/dg = dL/du + (sum of lambda over hole neighbours) on the known pixels.
+
+simpois_fill -- simpois's own output with -c 0: niter Gauss-Seidel sweeps
+at every level of the pyramid (bit-identical to the reference).  Backward:
+the exact gradient of that finite-sweep output (sp_adjoint_batch_device
+with cgit = 0: transposed sweeps, prolongation and restrictions on the GPU).
 """
 import ctypes
 
@@ -106,4 +110,56 @@ def poisson_fill(g, f=None, tol=1e-6, niter=0, scale=1):
         f = f[None] if isinstance(f, torch.Tensor) else f
     _check(g, f)
     u = PoissonFill.apply(g, f, tol, niter, scale)
+    return u[0] if squeeze else u
+
+
+class SimpoisFill(torch.autograd.Function):
+    """u = simpois(g, f) with -c 0 for planar float32 device tensors
+    (B, H, W): the output of niter Gauss-Seidel sweeps per pyramid level."""
+
+    @staticmethod
+    def forward(ctx, g, f, tstep, niter, scale):
+        _check(g, f)
+        g = g.contiguous()
+        f = None if f is None else f.contiguous()
+        out = torch.empty_like(g)
+        p = default_params(tstep=tstep, niter=niter, scale=scale, cgit=0, trace=0)
+        pd, h, w = g.shape
+        st = sp_stats()
+        s = torch.cuda.current_stream()
+        check(lib.sp_solve_device(out.data_ptr(), g.data_ptr(),
+                                  None if f is None else f.data_ptr(), w, h, pd,
+                                  ctypes.byref(p), ctypes.c_void_p(s.cuda_stream),
+                                  ctypes.byref(st)))
+        ctx.save_for_backward(g)
+        ctx.has_f = f is not None
+        ctx.cfg = (tstep, niter, scale)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_u):
+        (g,) = ctx.saved_tensors
+        grad_u = grad_u.contiguous()
+        grad_g = torch.empty_like(g)
+        need_f = ctx.has_f and ctx.needs_input_grad[1]
+        grad_f = torch.empty_like(g) if need_f else None
+        tstep, niter, scale = ctx.cfg
+        _adjoint(g, grad_u, grad_g, grad_f,
+                 default_params(tstep=tstep, niter=niter, scale=scale, cgit=0, trace=0))
+        return grad_g, grad_f, None, None, None
+
+
+def simpois_fill(g, f=None, niter=10, scale=99, tstep=0.25):
+    """Differentiable simpois (-c 0): the NaN pixels of g ((H,W) or (B,H,W)
+    float32 on the GPU) filled by niter Gauss-Seidel sweeps at every pyramid
+    level, with optional data term f (simpois -f).  Gradients flow to the
+    known pixels of g and to f; tstep > 0 (Laplace / Poisson)."""
+    if not tstep > 0:
+        raise ValueError("simpois_fill differentiates the Laplace / Poisson sweeps (tstep > 0)")
+    squeeze = isinstance(g, torch.Tensor) and g.dim() == 2
+    if squeeze:
+        g = g[None]
+        f = f[None] if isinstance(f, torch.Tensor) else f
+    _check(g, f)
+    u = SimpoisFill.apply(g, f, float(tstep), int(niter), int(scale))
     return u[0] if squeeze else u

@@ -1613,6 +1613,197 @@ struct AdjScratch {
 static std::mutex g_adj_mu;
 static AdjScratch *g_adj = nullptr;
 
+// ------------------------------------------ adjoint of the finite sweeps
+//
+// With -c 0 the output of simpois (simpois.c:247-285) is n Gauss-Seidel
+// sweeps at every pyramid level, an affine map of the known pixels and the
+// data term for a fixed mask.  Its gradient, level by level from the finest:
+//
+// * The level's output is g at its known pixels and u at its holes
+//   (cleant_cgpois.c:251-256): dL/dg gets v there, lambda = v at the holes.
+// * One sweep, on the holes (known neighbours enter through B g):
+//   u <- u + M (b - A u), A = -Laplacian, M = (I/t + L)^-1, b = -f + B g,
+//   L / U the couplings before / after a pixel in raster order.  With
+//   S = I - M A the adjoint is S^T, and S^T = (I/t + U) S_b (I/t + U)^-1
+//   with S_b = I - (I/t + U)^-1 A a backward sweep (A symmetric).  So with
+//   nu_n = (I/t + U)^-1 lambda and nu_{j-1} = S_b nu_j:
+//     dL/d(initial u) = (I/t + U) nu_0,  dL/db = nu_n + ... + nu_1,
+//   dL/df = -dL/db and dL/dg += B^T dL/db at the known pixels.  On the image
+//   turned by 180 degrees a backward sweep is a forward one, and
+//   (I/t + U)^-1 lambda is one forward sweep from 0 with data -lambda: the
+//   library's own Gauss-Seidel kernels run both (boundary 0 at the known
+//   pixels, data 0 after the first), one sweep at a time so that each
+//   nu_j is added up.
+// * The initial value at the holes is the bilinear x2 of the coarser
+//   level's output (simpois.c:199-239): its transpose gives the coarser
+//   level's v, and the recursion goes on there.
+// * The coarser g and f are 2x2 means of the finite samples, f times PMSFAC
+//   (simpois.c:172-196, 260-262): their transposes carry the coarser
+//   levels' dL/dg and dL/df down to the finest level, coarsest first.
+//
+// The sweeps are linear with coefficients fixed by the mask, so the
+// backward pass reads no forward iterate: nothing is stored during the
+// forward solve and nothing is recomputed.  Requires tstep > 0 (the
+// symmetric 5-point operator) and finite or NaN data.
+struct AsLevel {
+	int w = 0, h = 0, nsweep = 0;
+	size_t n = 0;               // pixels of one plane at this level
+	// pd planes each, plane l at l * n
+	unsigned char *hole = nullptr;
+	float *v = nullptr, *gg = nullptr, *gf = nullptr, *gi = nullptr;
+	float *g_rot = nullptr, *f_rot = nullptr, *u_rot = nullptr, *acc = nullptr;
+	Plan *sweep = nullptr;      // one sweep of the turned planes, this size
+};
+
+struct AsScratch {
+	std::vector<int> key;       // w, h, pd, levels, niter, PONLIT
+	Plan mem;                   // owns the level buffers
+	std::vector<AsLevel> lv;
+	~AsScratch() { for (auto &L : lv) delete L.sweep; }
+};
+static std::mutex g_as_mu;
+static AsScratch *g_as = nullptr;
+
+static int as_scratch(int w, int h, int pd, const sp_params &P, AsScratch **out)
+{
+	std::vector<std::pair<int, int>> shp;
+	for (int cw = w, ch = h, sc = P.scale;;) {
+		shp.push_back({cw, ch});
+		if (!(sc > 1 && (cw > 1 || ch > 1))) break;
+		cw = (int)ceil(cw / 2.0);
+		ch = (int)ceil(ch / 2.0);
+		sc--;
+	}
+	std::vector<int> key = {w, h, pd, (int)shp.size(), P.niter, (int)P.ponlit};
+	if (g_as && g_as->key == key) { *out = g_as; return SP_OK; }
+	delete g_as;
+	g_as = nullptr;
+	AsScratch *A = new AsScratch;
+	A->key = key;
+	A->lv.resize(shp.size());
+	sp_params P1 = P;
+	P1.niter = 1; P1.scale = 1; P1.cgit = 0; P1.ponlit = 0; P1.autotune = 0;
+	P1.guidance = 0; P1.trace = 0;
+	int rc = SP_OK;
+	for (size_t k = 0; k < shp.size() && !rc; k++) {
+		AsLevel &L = A->lv[k];
+		L.w = shp[k].first;
+		L.h = shp[k].second;
+		L.n = (size_t)L.w * L.h;
+		// PONLIT: only the level of that width sweeps (simpois.c:271-272)
+		L.nsweep = P.ponlit && P.ponlit != L.w ? 0 : std::max(P.niter, 0);
+		const size_t m = L.n * pd;
+		rc = A->mem.alloc(&L.hole, m);
+		if (!rc) rc = A->mem.alloc(&L.v, m);
+		if (!rc) rc = A->mem.alloc(&L.gg, m);
+		if (!rc) rc = A->mem.alloc(&L.gf, m);
+		if (!rc) rc = A->mem.alloc(&L.gi, m);
+		if (rc || !L.nsweep) continue;
+		rc = A->mem.alloc(&L.g_rot, m);
+		if (!rc) rc = A->mem.alloc(&L.f_rot, m);
+		if (!rc) rc = A->mem.alloc(&L.u_rot, m);
+		if (!rc) rc = A->mem.alloc(&L.acc, m);
+		if (rc) break;
+		std::vector<PlaneIn> planes;
+		for (int l = 0; l < pd; l++)
+			planes.push_back(PlaneIn{L.w, L.h, L.g_rot + L.n * l, L.f_rot + L.n * l,
+				L.u_rot + L.n * l});
+		L.sweep = new Plan;
+		rc = plan_create(*L.sweep, planes, P1, true);
+	}
+	if (rc) { delete A; return rc; }
+	g_as = A;
+	*out = A;
+	return SP_OK;
+}
+
+// one sweep of the turned planes held by the level's plan: `prepare`
+// recomputes the coefficient planes from g_rot / f_rot, `start` sets u = 0
+static int as_sweep(AsLevel &L, bool prepare, bool start, hipStream_t s)
+{
+	Plan &pl = *L.sweep;
+	LevelPlan &L0 = pl.lv[0];
+	if (prepare) spk::launch_prepare(L0.d_sp, L0.d_bw, L0.nbw, L0.d_bcnt, s);
+	if (start) spk::launch_init(L0.d_sp, L0.d_coarse, L0.d_bw, L0.nbw, s);
+	return run_gs(pl, L0, s);
+}
+
+// the pd planes of one image
+static int adjoint_sweeps_image(const sp_adjoint_image &I, const sp_params &P,
+		hipStream_t s, int *levels)
+{
+	AsScratch *A = nullptr;
+	TRY(as_scratch(I.w, I.h, I.pd, P, &A));
+	const int nl = (int)A->lv.size(), np = I.pd;
+	const float t = P.tstep;
+	*levels = std::max(*levels, nl);
+	// hole masks of every level
+	spk::as_mask0(I.g, A->lv[0].n, np, A->lv[0].hole, s);
+	for (int k = 1; k < nl; k++)
+		spk::as_mask(A->lv[k - 1].hole, A->lv[k - 1].w, A->lv[k - 1].h, A->lv[k].hole,
+				A->lv[k].w, A->lv[k].h, np, s);
+	HIPCHK(hipMemcpyAsync(A->lv[0].v, I.grad_u, A->lv[0].n * np * sizeof(float),
+			hipMemcpyDeviceToDevice, s));
+	for (int k = 0; k < nl; k++) {
+		AsLevel &L = A->lv[k];
+		if (!L.nsweep) {
+			spk::as_nosweep(L.hole, L.v, L.gg, L.gi, L.gf, L.w, L.h, np, s);
+		} else {
+			spk::as_setup(L.hole, L.v, L.gg, L.g_rot, L.f_rot, L.w, L.h, np, s);
+			TRY(as_sweep(L, true, true, s));            // nu_n = (I/t + U)^-1 lambda
+			spk::as_acc(L.acc, L.u_rot, L.n, np, 1, s);
+			spk::as_fill(L.f_rot, L.n, np, 0.0f, s);    // data 0 from here on
+			for (int j = 1; j < L.nsweep; j++) {
+				TRY(as_sweep(L, j == 1, false, s));
+				spk::as_acc(L.acc, L.u_rot, L.n, np, 0, s);
+			}
+			TRY(as_sweep(L, L.nsweep == 1, false, s)); // nu_0
+			spk::as_finish(L.hole, L.u_rot, L.acc, L.w, L.h, np, t, L.gi, L.gf, L.gg, s);
+		}
+		if (k + 1 < nl)
+			spk::as_zoom_in_t(L.gi, L.hole, L.w, L.h, A->lv[k + 1].v, A->lv[k + 1].w,
+					A->lv[k + 1].h, np, s);
+	}
+	// coarser levels' dL/dg and dL/df down to the finest, coarsest first
+	for (int k = nl - 2; k >= 0; k--) {
+		const AsLevel &C = A->lv[k + 1], &F = A->lv[k];
+		spk::as_zoom_out_t(C.gg, C.w, C.h, F.hole, 1, F.w, F.h, np, 1.0f, F.gg, s);
+		spk::as_zoom_out_t(C.gf, C.w, C.h, F.hole, 0, F.w, F.h, np, (float)P.pmsfac, F.gf, s);
+	}
+	HIPCHK(hipGetLastError());
+	const size_t bytes = A->lv[0].n * np * sizeof(float);
+	HIPCHK(hipMemcpyAsync(I.grad_g, A->lv[0].gg, bytes, hipMemcpyDeviceToDevice, s));
+	if (I.grad_f)
+		HIPCHK(hipMemcpyAsync(I.grad_f, A->lv[0].gf, bytes, hipMemcpyDeviceToDevice, s));
+	return SP_OK;
+}
+
+static int adjoint_sweeps(int n, const sp_adjoint_image *imgs, const sp_params &P,
+		hipStream_t s, sp_stats *st)
+{
+	if (P.guidance)
+		return set_err(SP_ERR_ARG, "the adjoint takes the data term itself "
+				"(guidance = 0); differentiate plambda's Laplacian separately");
+	std::lock_guard<std::mutex> lk(g_as_mu);
+	hipEvent_t e0, e1;
+	HIPCHK(hipEventCreate(&e0));
+	HIPCHK(hipEventCreate(&e1));
+	HIPCHK(hipEventRecord(e0, s));
+	int levels = 0;
+	for (int i = 0; i < n; i++) {
+		const int rc = adjoint_sweeps_image(imgs[i], P, s, &levels);
+		if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return rc; }
+	}
+	HIPCHK(hipEventRecord(e1, s));
+	HIPCHK(hipEventSynchronize(e1));
+	float ms = 0;
+	(void)hipEventElapsedTime(&ms, e0, e1);
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (st) { st->levels = levels; st->cg_iters_finest = 0; st->elapsed_ms = ms; }
+	return SP_OK;
+}
+
 extern "C" int sp_adjoint_batch_device(int n, const sp_adjoint_image *imgs,
 		const sp_params *p, void *stream, sp_stats *st)
 {
@@ -1636,6 +1827,8 @@ extern "C" int sp_adjoint_batch_device(int n, const sp_adjoint_image *imgs,
 		total += (size_t)I.w * I.h * I.pd;
 		shape.insert(shape.end(), {I.w, I.h, I.pd});
 	}
+	// -c 0: the gradient of the sweeps' own output
+	if (p->cgit == 0) return adjoint_sweeps(n, imgs, *p, s, st);
 	std::lock_guard<std::mutex> lk(g_adj_mu);
 	if (!g_adj || g_adj->shape != shape) {
 		delete g_adj;

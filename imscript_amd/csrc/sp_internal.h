@@ -131,6 +131,24 @@ void launch_adj_prepare(const AdjPlane *ap, const BlockWork *bw, int nblocks,
 void launch_adj_gather(const AdjPlane *ap, const BlockWork *bw, int nblocks,
 		hipStream_t s);
 
+// adjoint of the finite sweeps (sp_kernels.hip): np planes of one level,
+// plane l's arrays at offset l * (pixels of the level)
+void as_mask0(const float *g, size_t n, int np, unsigned char *hole, hipStream_t s);
+void as_mask(const unsigned char *hf, int wf, int hf_, unsigned char *hc, int wc, int hc_,
+		int np, hipStream_t s);
+void as_setup(const unsigned char *hole, const float *v, float *gg, float *g_rot,
+		float *f_rot, int w, int h, int np, hipStream_t s);
+void as_fill(float *a, size_t n, int np, float v, hipStream_t s);
+void as_acc(float *acc, const float *u, size_t n, int np, int first, hipStream_t s);
+void as_finish(const unsigned char *hole, const float *nu0, const float *acc, int w, int h,
+		int np, float t, float *gi, float *gf, float *gg, hipStream_t s);
+void as_nosweep(const unsigned char *hole, const float *v, float *gg, float *gi, float *gf,
+		int w, int h, int np, hipStream_t s);
+void as_zoom_in_t(const float *gi, const unsigned char *hole, int wf, int hf, float *vc,
+		int wc, int hc, int np, hipStream_t s);
+void as_zoom_out_t(const float *gc, int wc, int hc, const unsigned char *hole, int masked,
+		int wf, int hf, int np, float scale, float *fine, hipStream_t s);
+
 // FP64 relative residual of the Poisson (bilap=0) or biharmonic system
 void launch_residual(const SpPlane *lv, const BlockWork *bw, int nblocks,
 		int bilap, double *part, hipStream_t s);

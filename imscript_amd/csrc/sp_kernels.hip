@@ -948,3 +948,263 @@ void launch_adj_gather(const AdjPlane *ap, const BlockWork *bw, int nblocks,
 }
 
 } // namespace spk
+
+// ------------------------------------------- adjoint of the finite sweeps
+//
+// Gradient of simpois's own output (simpois.c:247-285 with -c 0: niter
+// Gauss-Seidel sweeps at every pyramid level); the host driver (sp_host.hip,
+// adjoint_sweeps) explains the algebra.  Data are finite or NaN (holes).
+// Each launch covers the planes of one image at one level: plane
+// blockIdx.y, its arrays at offset blockIdx.y * (pixels of the level); one
+// pixel per thread, sums in a fixed order.
+namespace spk {
+
+constexpr int AS_NT = 256;
+
+static __device__ __forceinline__ bool as_hole(const unsigned char *hole, int w, int h,
+		int i, int j)
+{
+	return i >= 0 && j >= 0 && i < w && j < h && hole[(size_t)j * w + i];
+}
+
+// pixel index of this thread in its plane, or -1; *pl = plane offset
+static __device__ __forceinline__ long as_pix(size_t n, size_t *pl)
+{
+	*pl = (size_t)blockIdx.y * n;
+	const size_t q = (size_t)blockIdx.x * AS_NT + threadIdx.x;
+	return q < n ? (long)q : -1;
+}
+
+// holes of the finest level from g
+__global__ __launch_bounds__(AS_NT) void k_as_mask0(const float *g, size_t n,
+		unsigned char *hole)
+{
+	size_t o;
+	const long q = as_pix(n, &o);
+	if (q >= 0) hole[o + q] = isnan(g[o + q]);
+}
+
+// holes of the next coarser level: a coarse pixel is NaN when none of its
+// four clamped samples is finite (simpois.c:172-196)
+__global__ __launch_bounds__(AS_NT) void k_as_mask(const unsigned char *hf, int wf,
+		int hf_, unsigned char *hc, int wc, int hc_)
+{
+	const size_t nc = (size_t)wc * hc_, nf = (size_t)wf * hf_;
+	size_t oc;
+	const long q = as_pix(nc, &oc);
+	if (q < 0) return;
+	const unsigned char *F = hf + (size_t)blockIdx.y * nf;
+	const int I = (int)(q % wc), J = (int)(q / wc);
+	const int i0 = min(2 * I, wf - 1), i1 = min(2 * I + 1, wf - 1);
+	const int j0 = min(2 * J, hf_ - 1), j1 = min(2 * J + 1, hf_ - 1);
+	hc[oc + q] = F[(size_t)j0 * wf + i0] && F[(size_t)j0 * wf + i1] &&
+		F[(size_t)j1 * wf + i0] && F[(size_t)j1 * wf + i1];
+}
+
+// Level start.  The level's output is g at its known pixels and u at its
+// holes (cleant_cgpois.c:251-256), so dL/dg gets v at the known pixels and
+// the sweeps' adjoint starts from v at the holes.  The sweep problem is set
+// up on the image turned by 180 degrees: boundary 0 at the known pixels,
+// NaN at the holes, data -v at the holes.
+__global__ __launch_bounds__(AS_NT) void k_as_setup(const unsigned char *hole,
+		const float *v, float *gg, float *g_rot, float *f_rot, int w, int h)
+{
+	const size_t n = (size_t)w * h;
+	size_t o;
+	const long q = as_pix(n, &o);
+	if (q < 0) return;
+	const size_t r = o + (n - 1 - q);   // the pixel turned by 180 degrees
+	if (hole[o + q]) {
+		gg[o + q] = 0.0f;
+		g_rot[r] = __uint_as_float(SP_QNAN_BITS);
+		f_rot[r] = -v[o + q];
+	} else {
+		gg[o + q] = v[o + q];
+		g_rot[r] = 0.0f;
+		f_rot[r] = 0.0f;
+	}
+}
+
+__global__ __launch_bounds__(AS_NT) void k_as_fill(float *a, size_t n, float v)
+{
+	size_t o;
+	const long q = as_pix(n, &o);
+	if (q >= 0) a[o + q] = v;
+}
+
+// acc = u (first) or acc + u
+__global__ __launch_bounds__(AS_NT) void k_as_acc(float *acc, const float *u, size_t n,
+		int first)
+{
+	size_t o;
+	const long q = as_pix(n, &o);
+	if (q >= 0) acc[o + q] = first ? u[o + q] : acc[o + q] + u[o + q];
+}
+
+// Level end, from the turned buffers nu0 (the sweeps' result) and acc (the
+// sum of the intermediate ones): at a hole, dL/d(init) = (I/t + U) nu0 (U:
+// the hole couplings to the east and south, later in raster order) and
+// dL/df = -acc; at a known pixel, dL/dg += acc over its hole neighbours
+// (the sweeps read it as a neighbour value).
+__global__ __launch_bounds__(AS_NT) void k_as_finish(const unsigned char *hole,
+		const float *nu0, const float *acc, int w, int h, float t, float *gi,
+		float *gf, float *gg)
+{
+	const size_t n = (size_t)w * h;
+	size_t o;
+	const long q = as_pix(n, &o);
+	if (q < 0) return;
+	const unsigned char *H = hole + o;
+	const float *N0 = nu0 + o, *AC = acc + o;
+	const int i = (int)(q % w), j = (int)(q / w);
+	auto rot = [&](int a, int b) { return n - 1 - ((size_t)b * w + a); };
+	if (H[q]) {
+		float s = N0[rot(i, j)] / t;
+		if (as_hole(H, w, h, i + 1, j)) s -= N0[rot(i + 1, j)];
+		if (as_hole(H, w, h, i, j + 1)) s -= N0[rot(i, j + 1)];
+		gi[o + q] = s;
+		gf[o + q] = -AC[rot(i, j)];
+	} else {
+		float s = gg[o + q];
+		if (as_hole(H, w, h, i + 1, j)) s += AC[rot(i + 1, j)];
+		if (as_hole(H, w, h, i, j + 1)) s += AC[rot(i, j + 1)];
+		if (as_hole(H, w, h, i - 1, j)) s += AC[rot(i - 1, j)];
+		if (as_hole(H, w, h, i, j - 1)) s += AC[rot(i, j - 1)];
+		gg[o + q] = s;
+		gi[o + q] = 0.0f;
+		gf[o + q] = 0.0f;
+	}
+}
+
+// a level without sweeps: the output at the holes is the initial value
+__global__ __launch_bounds__(AS_NT) void k_as_nosweep(const unsigned char *hole,
+		const float *v, float *gg, float *gi, float *gf, int w, int h)
+{
+	size_t o;
+	const long q = as_pix((size_t)w * h, &o);
+	if (q < 0) return;
+	const bool hl = hole[o + q];
+	gg[o + q] = hl ? 0.0f : v[o + q];
+	gi[o + q] = hl ? v[o + q] : 0.0f;
+	gf[o + q] = 0.0f;
+}
+
+// bilinear x2 of simpois.c:199-239 at fine coordinate i: corners floor(x),
+// floor(x) + 1 (clamped to n) with weights 1 - (x - floor(x)) and
+// x - floor(x), x = (i - 0.5) / 2
+static __device__ __forceinline__ void as_bilin(int i, int n, int &a, int &b, float &wa,
+		float &wb)
+{
+	const float x = (float)((i - 0.5) / 2);
+	const int ip = (int)floorf(x);
+	const float px = x - (float)ip;
+	a = clampi(ip, 0, n - 1);
+	b = clampi(ip + 1, 0, n - 1);
+	wa = 1 - px;
+	wb = px;
+}
+
+// v of the coarse level = the transpose of the prolongation applied to
+// dL/d(init) at the fine holes (the initial value is read at holes only)
+__global__ __launch_bounds__(AS_NT) void k_as_zoom_in_t(const float *gi,
+		const unsigned char *hole, int wf, int hf, float *vc, int wc, int hc)
+{
+	size_t oc;
+	const long q = as_pix((size_t)wc * hc, &oc);
+	if (q < 0) return;
+	const size_t of = (size_t)blockIdx.y * wf * hf;
+	const int I = (int)(q % wc), J = (int)(q / wc);
+	float s = 0.0f;
+	for (int j = max(2 * J - 1, 0); j <= min(2 * J + 2, hf - 1); j++) {
+		int ja, jb;
+		float wja, wjb;
+		as_bilin(j, hc, ja, jb, wja, wjb);
+		const float wy = (ja == J ? wja : 0.0f) + (jb == J ? wjb : 0.0f);
+		if (wy == 0.0f) continue;
+		for (int i = max(2 * I - 1, 0); i <= min(2 * I + 2, wf - 1); i++) {
+			const size_t p = of + (size_t)j * wf + i;
+			if (!hole[p]) continue;
+			int ia, ib;
+			float wia, wib;
+			as_bilin(i, wc, ia, ib, wia, wib);
+			const float wx = (ia == I ? wia : 0.0f) + (ib == I ? wib : 0.0f);
+			s += gi[p] * wx * wy;
+		}
+	}
+	vc[oc + q] = s;
+}
+
+// fine += scale * (transpose of the NaN-aware 2x2 mean) applied to the
+// coarse gradient gc: a fine pixel is sampled only by its coarse pixel
+// (twice per axis on the clamped last row / column of an odd size), which
+// divides by its number of finite samples.  masked: g (only known pixels
+// are finite samples); otherwise f (every sample finite).
+__global__ __launch_bounds__(AS_NT) void k_as_zoom_out_t(const float *gc, int wc, int hc,
+		const unsigned char *hole, int masked, int wf, int hf, float scale, float *fine)
+{
+	size_t of;
+	const long q = as_pix((size_t)wf * hf, &of);
+	if (q < 0) return;
+	const unsigned char *H = hole + of;
+	if (masked && H[q]) return;
+	const int i = (int)(q % wf), j = (int)(q / wf), I = i >> 1, J = j >> 1;
+	const int i0 = min(2 * I, wf - 1), i1 = min(2 * I + 1, wf - 1);
+	const int j0 = min(2 * J, hf - 1), j1 = min(2 * J + 1, hf - 1);
+	int cnt = 4;
+	if (masked)
+		cnt = !H[(size_t)j0 * wf + i0] + !H[(size_t)j0 * wf + i1] +
+			!H[(size_t)j1 * wf + i0] + !H[(size_t)j1 * wf + i1];
+	const int mult = ((i0 == i) + (i1 == i)) * ((j0 == j) + (j1 == j));
+	fine[of + q] += scale * gc[(size_t)blockIdx.y * wc * hc + (size_t)J * wc + I] *
+		(float)mult / (float)cnt;
+}
+
+static dim3 as_grid(size_t n, int np) { return dim3((unsigned)((n + AS_NT - 1) / AS_NT), np); }
+
+void as_mask0(const float *g, size_t n, int np, unsigned char *hole, hipStream_t s)
+{
+	k_as_mask0<<<as_grid(n, np), AS_NT, 0, s>>>(g, n, hole);
+}
+void as_mask(const unsigned char *hf, int wf, int hf_, unsigned char *hc, int wc, int hc_,
+		int np, hipStream_t s)
+{
+	k_as_mask<<<as_grid((size_t)wc * hc_, np), AS_NT, 0, s>>>(hf, wf, hf_, hc, wc, hc_);
+}
+void as_setup(const unsigned char *hole, const float *v, float *gg, float *g_rot,
+		float *f_rot, int w, int h, int np, hipStream_t s)
+{
+	k_as_setup<<<as_grid((size_t)w * h, np), AS_NT, 0, s>>>(hole, v, gg, g_rot, f_rot, w, h);
+}
+void as_fill(float *a, size_t n, int np, float v, hipStream_t s)
+{
+	k_as_fill<<<as_grid(n, np), AS_NT, 0, s>>>(a, n, v);
+}
+void as_acc(float *acc, const float *u, size_t n, int np, int first, hipStream_t s)
+{
+	k_as_acc<<<as_grid(n, np), AS_NT, 0, s>>>(acc, u, n, first);
+}
+void as_finish(const unsigned char *hole, const float *nu0, const float *acc, int w, int h,
+		int np, float t, float *gi, float *gf, float *gg, hipStream_t s)
+{
+	k_as_finish<<<as_grid((size_t)w * h, np), AS_NT, 0, s>>>(hole, nu0, acc, w, h, t, gi,
+			gf, gg);
+}
+void as_nosweep(const unsigned char *hole, const float *v, float *gg, float *gi, float *gf,
+		int w, int h, int np, hipStream_t s)
+{
+	k_as_nosweep<<<as_grid((size_t)w * h, np), AS_NT, 0, s>>>(hole, v, gg, gi, gf, w, h);
+}
+void as_zoom_in_t(const float *gi, const unsigned char *hole, int wf, int hf, float *vc,
+		int wc, int hc, int np, hipStream_t s)
+{
+	k_as_zoom_in_t<<<as_grid((size_t)wc * hc, np), AS_NT, 0, s>>>(gi, hole, wf, hf, vc, wc,
+			hc);
+}
+void as_zoom_out_t(const float *gc, int wc, int hc, const unsigned char *hole, int masked,
+		int wf, int hf, int np, float scale, float *fine, hipStream_t s)
+{
+	k_as_zoom_out_t<<<as_grid((size_t)wf * hf, np), AS_NT, 0, s>>>(gc, wc, hc, hole,
+			masked, wf, hf, scale, fine);
+}
+
+} // namespace spk

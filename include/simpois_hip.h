@@ -244,11 +244,18 @@ typedef struct sp_adjoint_image {
 	float *grad_f;         /* dL/df: -lambda at holes, 0 elsewhere; may be NULL */
 } sp_adjoint_image;
 
-/* Gradient of the converged Poisson fill (tstep > 0 systems only; the
- * clamped 5-point operator is symmetric, so the adjoint is the masked
- * Laplacian solve A lambda = grad_u on the holes, run by the same pyramid
- * and CG as the forward solve with the parameters in p).  No reference
- * counterpart. */
+/* Gradient of the fill made with the parameters in p (tstep > 0 systems
+ * only: the clamped 5-point operator is symmetric).
+ * p->cgit != 0: the converged fill; the adjoint is the masked Laplacian
+ *   solve A lambda = grad_u on the holes, run by the same pyramid and CG as
+ *   the forward solve.
+ * p->cgit == 0: simpois's own output (niter Gauss-Seidel sweeps at each of
+ *   at most `scale` pyramid levels, PMSFAC, PONLIT), differentiated exactly:
+ *   transposed sweeps (run as forward sweeps of the image turned by 180
+ *   degrees), transposed prolongation and restrictions.  The sweeps are
+ *   linear with coefficients fixed by the mask, so no forward iterate is
+ *   stored or recomputed.  Data finite or NaN.
+ * No reference counterpart. */
 int sp_adjoint_batch_device(int n, const sp_adjoint_image *imgs,
 		const sp_params *p, void *stream, sp_stats *st);
 

@@ -114,3 +114,104 @@ def gradient(g, f, v):
     gg[ik] = v[ik] - Ahk.T @ lam
     gf[ih] = -lam
     return gg.reshape(g.shape), gf.reshape(g.shape)
+
+
+# ------------------------------------------------- simpois's finite sweeps
+#
+# simpois with -c 0 (simpois.c:247-285): niter raster-order Gauss-Seidel
+# sweeps at every level of the 2x pyramid, restated in FP64 for the finite
+# differences that check the gradient of the sweeps' own output.  Data are
+# finite or NaN (holes).
+
+def _zoom_out(a):
+    """NaN-aware 2x2 mean with clamped reads (simpois.c:172-196)."""
+    h, w = a.shape
+    hs, ws = (h + 1) // 2, (w + 1) // 2
+    out = np.empty((hs, ws))
+    for J in range(hs):
+        for I in range(ws):
+            s, c = 0.0, 0
+            for j, i in ((2 * J, 2 * I), (2 * J, 2 * I + 1), (2 * J + 1, 2 * I),
+                         (2 * J + 1, 2 * I + 1)):
+                v = a[min(j, h - 1), min(i, w - 1)]
+                if np.isfinite(v):
+                    s += v
+                    c += 1
+            out[J, I] = s / c if c else np.nan
+    return out
+
+
+def _zoom_in(a, w, h):
+    """Cell-centred bilinear x2 with clamped reads (simpois.c:199-239)."""
+    hs, ws = a.shape
+    out = np.empty((h, w))
+    for j in range(h):
+        y = (j - 0.5) / 2
+        iq = int(np.floor(y))
+        qy = y - iq
+        for i in range(w):
+            x = (i - 0.5) / 2
+            ip = int(np.floor(x))
+            px = x - ip
+            at = lambda ii, jj: a[min(max(jj, 0), hs - 1), min(max(ii, 0), ws - 1)]
+            out[j, i] = (at(ip, iq) * (1 - px) * (1 - qy) + at(ip + 1, iq) * px * (1 - qy)
+                         + at(ip, iq + 1) * (1 - px) * qy + at(ip + 1, iq + 1) * px * qy)
+    return out
+
+
+def _sweep(u, f, hole, t):
+    """One raster-order Gauss-Seidel sweep over the holes, in place
+    (simpois.c:72-119: the Laplacian skips non-finite and clamped
+    neighbours)."""
+    h, w = u.shape
+    for j, i in zip(*np.nonzero(hole)):
+        x = u[j, i]
+        r = 0.0
+        for a, b in ((i + 1, j), (i, j + 1), (i - 1, j), (i, j - 1)):
+            if 0 <= a < w and 0 <= b < h and np.isfinite(u[b, a]):
+                r += u[b, a] - x
+        u[j, i] = x + t * (r - (f[j, i] if f is not None else 0.0))
+
+
+def simpois64(g, f=None, tstep=0.25, niter=10, scale=99, pmsfac=3.0, ponlit=0):
+    """poisson_rec (simpois.c:247-285) with cgit = 0, in FP64."""
+    g = np.asarray(g, np.float64)
+    f = None if f is None else np.asarray(f, np.float64)
+    h, w = g.shape
+    if scale > 1 and (w > 1 or h > 1):
+        gs = _zoom_out(g)
+        fs = None if f is None else _zoom_out(f) * pmsfac
+        us = simpois64(gs, fs, tstep, niter, scale - 1, pmsfac, ponlit)
+        init = _zoom_in(us, w, h)
+    else:
+        init = np.zeros((h, w))
+    n = 0 if (ponlit and ponlit != w) else niter
+    u = np.where(np.isfinite(g), g, init)
+    hole = np.isnan(g)
+    for _ in range(n):
+        _sweep(u, f, hole, tstep)
+    # the level's output (cleant_cgpois.c:203-207, 251-256)
+    if hole.any():
+        return np.where(hole, u, g)
+    return np.where(np.isfinite(g), g, u)
+
+
+def sweeps_gradient_fd(g, f, v, eps=1e-3, **kw):
+    """(dL/dg at the known pixels, dL/df) of L = sum(v * simpois64(g, f)) by
+    FP64 central differences; the map is affine in (g, f), so they are exact
+    up to rounding."""
+    g = np.asarray(g, np.float64)
+    f = np.zeros_like(g) if f is None else np.asarray(f, np.float64)
+    v = np.asarray(v, np.float64)
+    loss = lambda gg, ff: float(np.sum(v * simpois64(gg, ff, **kw)))
+    gg = np.zeros_like(g)
+    gf = np.zeros_like(g)
+    for j, i in zip(*np.nonzero(~np.isnan(g))):
+        e = np.zeros_like(g)
+        e[j, i] = eps
+        gg[j, i] = (loss(g + e, f) - loss(g - e, f)) / (2 * eps)
+    for j, i in np.ndindex(*g.shape):
+        e = np.zeros_like(g)
+        e[j, i] = eps
+        gf[j, i] = (loss(g, f + e) - loss(g, f - e)) / (2 * eps)
+    return gg, gf

@@ -111,3 +111,44 @@ def test_bilaplacian_operator_matches_independent_assembly():
     assert np.max(np.abs(b - b2)) < 1e-9 * np.max(np.abs(b))
     u = ar.solve_bilap(g, f)
     assert np.max(np.abs(u.ravel()[ih] - spla.spsolve(A.tocsc(), b))) < 1e-8 * np.max(np.abs(u))
+
+
+# ------------------------------------------------- simpois's finite sweeps
+
+@pytest.mark.parametrize("h,w,niter,scale,withf,tstep", [
+    (12, 14, 3, 99, True, 0.25), (9, 13, 5, 2, True, 0.25), (17, 10, 4, 99, False, 0.25),
+    (7, 5, 1, 99, True, 0.25), (16, 16, 6, 3, True, 0.4), (1, 9, 3, 99, True, 0.25)])
+def test_simpois64_matches_the_oracle(h, w, niter, scale, withf, tstep):
+    """The FP64 restatement of -c 0 simpois (sweeps, pyramid, level output)
+    against the oracle (bit-exact with the reference) within float rounding:
+    a wrong weight, index or term would differ at the data's scale."""
+    from helpers import oracle_solve
+    rng = np.random.default_rng(h * 7 + w)
+    g = (rng.standard_normal((h, w)) * 10).astype(np.float32)
+    g[rng.random((h, w)) < 0.4] = np.nan
+    f = rng.standard_normal((h, w)).astype(np.float32) if withf else None
+    u64 = ar.simpois64(g, f, tstep, niter, scale)
+    u32, _ = oracle_solve(g, f, tstep, niter, scale, 0)
+    assert np.max(np.abs(u64 - u32.reshape(h, w))) <= 1e-6 * np.max(np.abs(u64))
+
+
+def test_sweeps_fd_gradient_is_the_transpose():
+    """sweeps_gradient_fd against an independent check: the map is affine,
+    so for random directions <grad, d> equals the change of the loss, and
+    the loss is linear in v (the gradient of a sum is the sum)."""
+    rng = np.random.default_rng(9)
+    g = rng.standard_normal((8, 11)) * 5
+    g[rng.random(g.shape) < 0.4] = np.nan
+    f = rng.standard_normal(g.shape)
+    v1, v2 = rng.standard_normal(g.shape), rng.standard_normal(g.shape)
+    kw = dict(tstep=0.25, niter=3, scale=99)
+    gg1, gf1 = ar.sweeps_gradient_fd(g, f, v1, **kw)
+    gg2, gf2 = ar.sweeps_gradient_fd(g, f, v2, **kw)
+    gg12, gf12 = ar.sweeps_gradient_fd(g, f, v1 + v2, **kw)
+    assert np.allclose(gg12, gg1 + gg2, atol=1e-9) and np.allclose(gf12, gf1 + gf2, atol=1e-9)
+    dg = np.where(np.isnan(g), 0, rng.standard_normal(g.shape))
+    df = rng.standard_normal(g.shape)
+    u0 = ar.simpois64(g, f, **kw)
+    u1 = ar.simpois64(g + dg, f + df, **kw)
+    assert np.isclose(np.sum(gg1 * dg) + np.sum(gf1 * df), np.sum(v1 * (u1 - u0)),
+                      rtol=1e-9)

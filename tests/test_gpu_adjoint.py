@@ -95,3 +95,78 @@ def test_gradient_matches_fp64_finite_differences():
         f2[a, b] -= eps
         fd = (L(g64, f1) - L(g64, f2)) / (2 * eps)
         assert abs(gf[0, a, b] - fd) <= 1e-3 * max(abs(fd), 1.0)
+
+
+# ------------------------------------------------- simpois's finite sweeps
+
+def sweep_problem(seed, b, h, w, frac=0.4):
+    rng = np.random.default_rng(seed)
+    g = (rng.standard_normal((b, h, w)) * 10).astype(np.float32)
+    g[rng.random((b, h, w)) < frac] = np.nan
+    f = rng.standard_normal((b, h, w)).astype(np.float32)
+    v = rng.standard_normal((b, h, w)).astype(np.float32)
+    return g, f, v
+
+
+def sweep_grads(g, f, v, niter, scale, tstep=0.25):
+    """(u, dL/dg, dL/df) of L = sum(v * simpois_fill(g, f)) on the GPU"""
+    import torch
+    from imscript_amd.autograd import simpois_fill
+    dg = torch.from_numpy(g).cuda().requires_grad_(True)
+    df = None if f is None else torch.from_numpy(f).cuda().requires_grad_(True)
+    u = simpois_fill(dg, df, niter=niter, scale=scale, tstep=tstep)
+    (u * torch.from_numpy(v).cuda()).sum().backward()
+    return (u.detach().cpu().numpy(), dg.grad.cpu().numpy(),
+            None if f is None else df.grad.cpu().numpy())
+
+
+@pytest.mark.parametrize("h,w,niter,scale,withf,tstep", [
+    (12, 14, 3, 99, True, 0.25), (9, 13, 5, 2, True, 0.25), (17, 10, 4, 99, False, 0.25),
+    (7, 5, 1, 99, True, 0.25), (16, 16, 6, 3, True, 0.4), (11, 20, 2, 1, True, 0.25)])
+def test_sweep_gradient_matches_fp64_finite_differences(h, w, niter, scale, withf, tstep):
+    """The gradient of simpois's own -c 0 output (niter sweeps at every
+    pyramid level) against FP64 central differences (eps 1e-3) of the FP64
+    restatement of the same map (oracle/adjoint_ref.simpois64, itself within
+    float rounding of the bit-exact oracle), two planes, bar 1e-3 relative;
+    the forward output is the reference's bit for bit (same float32 kernels
+    as simpois)."""
+    from helpers import bits_equal, oracle_solve
+    g, f, v = sweep_problem(h * 31 + w, 2, h, w)
+    if not withf:
+        f = None
+    print(f"sweep adjoint {h}x{w} n={niter} s={scale} f={withf} t={tstep}", flush=True)
+    u, gg, gf = sweep_grads(g, f, v, niter, scale, tstep)
+    for k in range(2):
+        fk = None if f is None else f[k]
+        want, _ = oracle_solve(g[k], fk, tstep, niter, scale, 0)
+        assert bits_equal(u[k], want.reshape(h, w))
+        rg, rf = ar.sweeps_gradient_fd(g[k], fk, v[k], tstep=tstep, niter=niter,
+                                       scale=scale)
+        assert rel_err(gg[k], rg) < 1e-3, k
+        assert not np.any(gg[k][np.isnan(g[k])])    # 0 at the holes
+        if f is not None:
+            assert rel_err(gf[k], rf) < 1e-3, k
+
+
+def test_sweep_gradient_dot_product_at_scale():
+    """The map is affine in (g, f), so for any direction (dg on the known
+    pixels, df everywhere) <grad, d> = sum(v * (u(g + dg, f + df) - u(g, f))):
+    checked at 2 x 600 x 700 with 30 sweeps on 10 levels, and the gradient
+    is the same on a second call (cached scratch)."""
+    import torch
+    from imscript_amd.autograd import simpois_fill
+    g, f, v = sweep_problem(5, 2, 600, 700, 0.5)
+    rng = np.random.default_rng(6)
+    dg = np.where(np.isnan(g), 0, rng.standard_normal(g.shape)).astype(np.float32)
+    df = rng.standard_normal(g.shape).astype(np.float32)
+    print("sweep adjoint dot product 2x600x700 n=30", flush=True)
+    u, gg, gf = sweep_grads(g, f, v, 30, 99)
+    u1 = simpois_fill(torch.from_numpy(g + dg).cuda(), torch.from_numpy(f + df).cuda(),
+                      niter=30, scale=99).cpu().numpy()
+    lhs = float(np.sum(gg.astype(np.float64) * dg) + np.sum(gf.astype(np.float64) * df))
+    rhs = float(np.sum(v.astype(np.float64) * (u1.astype(np.float64) - u)))
+    print("<grad, d>", lhs, "finite difference", rhs, flush=True)
+    assert abs(lhs - rhs) <= 1e-3 * abs(rhs)
+    _, gg2, gf2 = sweep_grads(g, f, v, 30, 99)
+    assert np.array_equal(gg2.view(np.uint32), gg.view(np.uint32))
+    assert np.array_equal(gf2.view(np.uint32), gf.view(np.uint32))
