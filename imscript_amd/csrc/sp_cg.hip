@@ -41,6 +41,25 @@ static __device__ __forceinline__ int clampi(int v, int lo, int hi)
 	return v < lo ? lo : (v > hi ? hi : v);
 }
 
+// Loads and stores through the global address space.  A pointer read from
+// a descriptor in memory is generic, and a flat access also counts against
+// the LDS counter: every wait for an LDS read then waits for it too.  The
+// type must be a scalar or an ext_vector type (a struct loaded through a
+// reference falls back to a flat access).
+typedef float gf2 __attribute__((ext_vector_type(2)));
+typedef unsigned gu2 __attribute__((ext_vector_type(2)));
+typedef unsigned gu4 __attribute__((ext_vector_type(4)));
+template <class T>
+static __device__ __forceinline__ T gload(const void *p)
+{
+	return *(const __attribute__((address_space(1))) T *)p;
+}
+template <class T>
+static __device__ __forceinline__ void gstore(void *p, T v)
+{
+	*(__attribute__((address_space(1))) T *)p = v;
+}
+
 // Work-groups go to the 8 XCDs in turn (work-group b to XCD b % 8), each
 // XCD with its own L2.  xcd_block() maps b to a logical block so that every
 // XCD takes a contiguous run of logical blocks, in order: neighbouring
@@ -54,6 +73,24 @@ static __device__ __forceinline__ int xcd_block()
 	const int n = (int)gridDim.x, b = (int)blockIdx.x;
 	const int x = b & 7, q = n >> 3, r = n & 7;
 	return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+// For the CG vector kernels: every XCD takes runs of CH consecutive work
+// blocks, the runs dealt to the XCDs in turn, so a block and the blocks of
+// the rows above and below it mostly share one L2 while work that follows
+// the holes still spreads over all XCDs.  A bijection of [0, gridDim.x)
+// (the grid's last partial round of runs keeps its order); the partial-sum
+// slot of a block follows the logical index, so sums do not depend on it.
+// n = gridDim.x, passed as an argument: it then arrives with the other
+// kernel arguments instead of one more dependent load.
+constexpr int CG_XCD_RUN = 32;
+static __device__ __forceinline__ int xcd_run_block(int n)
+{
+	const int b = (int)blockIdx.x;
+	const int full = n / (8 * CG_XCD_RUN) * (8 * CG_XCD_RUN);
+	if (b >= full) return b;
+	const int x = b & 7, i = b >> 3;
+	return ((i / CG_XCD_RUN) * 8 + x) * CG_XCD_RUN + i % CG_XCD_RUN;
 }
 
 template <int N>
@@ -332,12 +369,22 @@ __global__ __launch_bounds__(RT) void k32_reduce0(const Cg32Plane *pl, Cg32Scal 
 	}
 }
 
+// The loads of a plane's descriptor go out with those of its scalars: used
+// only past a kernel's early exit, the compiler would sink them there and
+// the work-group would wait for one more dependent load.
+static __device__ __forceinline__ void cg32_early(const Cg32Plane &P)
+{
+	asm volatile("" :: "s"(P.code), "s"(P.x), "s"(P.r), "s"(P.p), "s"(P.q), "s"(P.p2),
+			"s"(P.z), "s"(P.u), "s"(P.w), "s"(P.h));
+}
+
 // q = A p, partial p.q
 __global__ __launch_bounds__(NT) void k32_spmv(const BlockWork *bw,
-		const Cg32Plane *pl, const Cg32Scal *sc, double *part)
+		const Cg32Plane *pl, const Cg32Scal *sc, double *part, int nblk)
 {
 	__shared__ double red[NT / 64];
-	const BlockWork b = bw[blockIdx.x];
+	const int blk = xcd_run_block(nblk);
+	const BlockWork b = bw[blk];
 	if (!sc[b.plane].active) return;
 	const Cg32Plane P = pl[b.plane];
 	const float *p = P.p;
@@ -409,7 +456,7 @@ __global__ __launch_bounds__(NT) void k32_spmv(const BlockWork *bw,
 		}
 	}
 	const double s = bsum<NT>(acc, red);
-	if (threadIdx.x == 0) part[2 * blockIdx.x] = s;
+	if (threadIdx.x == 0) part[2 * blk] = s;
 }
 
 __global__ __launch_bounds__(RT) void k32_reduce_a(const Cg32Plane *pl, Cg32Scal *sc,
@@ -443,28 +490,32 @@ __global__ __launch_bounds__(RT) void k32_reduce_a(const Cg32Plane *pl, Cg32Scal
 // r -= alpha q ; partial r.r  (x += alpha p follows in k32_dir, which
 // reads p anyway)
 __global__ __launch_bounds__(NT) void k32_update(const BlockWork *bw,
-		const Cg32Plane *pl, const Cg32Scal *sc, double *part)
+		const Cg32Plane *pl, const Cg32Scal *sc, double *part, int nblk)
 {
 	__shared__ double red[NT / 64];
-	const int blk = blockIdx.x;
+	const int blk = xcd_run_block(nblk);
 	const BlockWork b = bw[blk];
-	const Cg32Scal &s = sc[b.plane];
-	if (!s.active) return;
 	const Cg32Plane P = pl[b.plane];
-	const float a = (float)s.alpha;
+	const Cg32Scal &s = sc[b.plane];
+	const bool act = s.active;
+	const double ad = s.alpha;
+	cg32_early(P);
+	asm volatile("" :: "s"(ad));
+	if (!act) return;
+	const float a = (float)ad;
 	double acc = 0;
 	const unsigned n = (unsigned)P.w * (unsigned)P.h;
 	const unsigned q4 = (unsigned)b.start + 4 * threadIdx.x;
 	if (q4 + 3 < n) {
-		if (code4(P.code, q4) & 0x01010101u) {
-			const f4 qq = *(const f4 *)(P.q + q4);
-			f4 r = *(const f4 *)(P.r + q4);
+		if (gload<unsigned>(P.code + q4) & 0x01010101u) {
+			const f4 qq = gload<f4>(P.q + q4);
+			f4 r = gload<f4>(P.r + q4);
 #pragma unroll
 			for (int k = 0; k < 4; k++) {
 				r[k] = r[k] - a * qq[k];
 				acc += (double)r[k] * r[k];
 			}
-			*(f4 *)(P.r + q4) = r;
+			gstore<f4>(P.r + q4, r);
 		}
 	} else {
 		for (unsigned q = q4; q < n; q++) {
@@ -511,9 +562,10 @@ __global__ __launch_bounds__(RT) void k32_reduce_b(const Cg32Plane *pl, Cg32Scal
 // stops the plane); p = z + beta p while the plane goes on (z = r without
 // preconditioner)
 __global__ __launch_bounds__(NT) void k32_dir(const BlockWork *bw,
-		const Cg32Plane *pl, const Cg32Scal *sc)
+		const Cg32Plane *pl, const Cg32Scal *sc, int nblk)
 {
-	const BlockWork b = bw[blockIdx.x];
+	const int blk = xcd_run_block(nblk);
+	const BlockWork b = bw[blk];
 	const Cg32Scal &s = sc[b.plane];
 	const bool upd = s.upd, act = s.active;
 	if (!upd && !act) return;
@@ -563,15 +615,19 @@ static __device__ __forceinline__ float pnew(float z, float po, float be, bool f
 }
 
 __global__ __launch_bounds__(NT) void k32_dirspmv(const BlockWork *bw,
-		const Cg32Plane *pl, const Cg32Scal *sc, double *part, int par, int first)
+		const Cg32Plane *pl, const Cg32Scal *sc, double *part, int par, int first,
+		int nblk)
 {
 	__shared__ double red[NT / 64];
-	const int blk = blockIdx.x;
+	const int blk = xcd_run_block(nblk);
 	const BlockWork b = bw[blk];
-	const Cg32Scal &s = sc[b.plane];
-	const bool upd = !first && s.upd, act = s.active, fin = !first && s.fin;
-	if (!upd && !act && !fin) return;
+	// the plane's descriptor loads alongside its scalars
 	const Cg32Plane P = pl[b.plane];
+	const Cg32Scal &s = sc[b.plane];
+	const int su = s.upd, sa = s.active, sf = s.fin;
+	const bool upd = !first && su, act = sa, fin = !first && sf;
+	cg32_early(P);
+	if (!upd && !act && !fin) return;
 	const float *po = par ? P.p2 : P.p;
 	float *pw_ = par ? P.p : P.p2;
 	const float be = (float)s.beta;
@@ -580,10 +636,81 @@ __global__ __launch_bounds__(NT) void k32_dirspmv(const BlockWork *bw,
 	double acc = 0;
 	const unsigned n = (unsigned)P.w * (unsigned)P.h, w = (unsigned)P.w;
 	const unsigned q4 = (unsigned)b.start + 4 * threadIdx.x;
-	if (q4 + 3 < n) {
+	const f4 z0 = {0.0f, 0.0f, 0.0f, 0.0f};
+	if (q4 + 3 < n && !(w & 3)) {
+		// rows of 16-byte groups: every load of the group goes out before
+		// the first use.  The rows above and below and the east / west
+		// pixels are clamped into the plane; their values are taken under
+		// the code bits only, which never name a pixel outside it.
+		const unsigned cc = gload<unsigned>(P.code + q4);
+		if (cc & 0x01010101u) {
+			const unsigned qn = q4 >= w ? q4 - w : q4, qs = q4 + w + 3 < n ? q4 + w : q4;
+			const unsigned qe = q4 + 4 < n ? q4 + 4 : q4, qw = q4 > 0 ? q4 - 1 : q4;
+			f4 poc = z0, on = z0, os = z0, zc = z0, zn = z0, zs = z0;
+			float oe = 0.0f, ow = 0.0f, ze = 0.0f, zw = 0.0f;
+			d4 x = {0.0, 0.0, 0.0, 0.0};
+			if (!fst) {
+				poc = gload<f4>(po + q4);
+				if (act) {
+					on = gload<f4>(po + qn);
+					os = gload<f4>(po + qs);
+					oe = gload<float>(po + qe);
+					ow = gload<float>(po + qw);
+				}
+			}
+			if (upd || fin) x = gload<d4>(P.x + q4);
+			if (act) {
+				zc = gload<f4>(P.z + q4);
+				zn = gload<f4>(P.z + qn);
+				zs = gload<f4>(P.z + qs);
+				ze = gload<float>(P.z + qe);
+				zw = gload<float>(P.z + qw);
+			}
+			// one straight stretch of arithmetic (the flags select), then
+			// the stores: a use of loaded values after a branch would make
+			// the compiler wait for every access in flight, stores included
+#pragma unroll
+			for (int k = 0; k < 4; k++)
+				if (upd && ((cc >> (8 * k)) & 1u)) x[k] = x[k] + ad * (double)poc[k];
+			f4 pc, pn, ps;
+#pragma unroll
+			for (int k = 0; k < 4; k++) {
+				const unsigned c = act ? (cc >> (8 * k)) & 255u : 0u;
+				pc[k] = c & 1u ? pnew(zc[k], poc[k], be, fst) : 0.0f;
+				pn[k] = c & 16u ? pnew(zn[k], on[k], be, fst) : 0.0f;
+				ps[k] = c & 4u ? pnew(zs[k], os[k], be, fst) : 0.0f;
+			}
+			const float pe = act && (cc & (2u << 24)) ? pnew(ze, oe, be, fst) : 0.0f;
+			const float pw = act && (cc & 8u) ? pnew(zw, ow, be, fst) : 0.0f;
+			f4 y;
+#pragma unroll
+			for (int k = 0; k < 4; k++) {
+				const unsigned c = (cc >> (8 * k)) & 255u;
+				const float e = k < 3 ? pc[k + 1] : pe, ww = k > 0 ? pc[k - 1] : pw;
+				float nb = 0.0f;
+				if (c & 2) nb += e;
+				if (c & 4) nb += ps[k];
+				if (c & 8) nb += ww;
+				if (c & 16) nb += pn[k];
+				y[k] = (float)(c >> 5) * pc[k] - nb;
+				if (act) acc += (double)y[k] * pc[k];
+			}
+			if (act) {
+				gstore<f4>(pw_ + q4, pc);
+				gstore<f4>(P.q + q4, y);
+			}
+			if (upd) gstore<d4>(P.x + q4, x);
+			// the solution: u may be the caller's buffer, 4-byte stores at
+			// the unknowns only
+			if (fin)
+#pragma unroll
+				for (int k = 0; k < 4; k++)
+					if ((cc >> (8 * k)) & 1u) gstore<float>(P.u + q4 + k, (float)x[k]);
+		}
+	} else if (q4 + 3 < n) {
+		// rows that do not start 16-byte aligned: neighbours one by one
 		const unsigned cc = code4(P.code, q4);
 		if (cc & 0x01010101u) {
-			const f4 z0 = {0.0f, 0.0f, 0.0f, 0.0f};
 			const f4 poc = fst ? z0 : *(const f4 *)(po + q4);
 			if (upd || fin) {
 				d4 x = *(const d4 *)(P.x + q4);
@@ -593,8 +720,6 @@ __global__ __launch_bounds__(NT) void k32_dirspmv(const BlockWork *bw,
 						if ((cc >> (8 * k)) & 1u) x[k] = x[k] + ad * (double)poc[k];
 					*(d4 *)(P.x + q4) = x;
 				}
-				// the solution: u may be the caller's buffer, 4-byte stores
-				// at the unknowns only
 				if (fin)
 #pragma unroll
 					for (int k = 0; k < 4; k++)
@@ -604,31 +729,12 @@ __global__ __launch_bounds__(NT) void k32_dirspmv(const BlockWork *bw,
 				const f4 zc = *(const f4 *)(P.z + q4);
 				f4 pc, pn, ps;
 #pragma unroll
-				for (int k = 0; k < 4; k++)
+				for (int k = 0; k < 4; k++) {
 					pc[k] = (cc >> (8 * k)) & 1u ? pnew(zc[k], poc[k], be, fst) : 0.0f;
-				if (!(w & 3) && q4 >= w) {
-					const f4 zn = *(const f4 *)(P.z + q4 - w);
-					const f4 on = fst ? z0 : *(const f4 *)(po + q4 - w);
-#pragma unroll
-					for (int k = 0; k < 4; k++)
-						pn[k] = (cc >> (8 * k)) & 16u ? pnew(zn[k], on[k], be, fst) : 0.0f;
-				} else {
-#pragma unroll
-					for (int k = 0; k < 4; k++)
-						pn[k] = (cc >> (8 * k)) & 16u ? pnew(P.z[q4 + k - w],
-								fst ? 0.0f : po[q4 + k - w], be, fst) : 0.0f;
-				}
-				if (!(w & 3) && q4 + w + 3 < n) {
-					const f4 zs = *(const f4 *)(P.z + q4 + w);
-					const f4 os = fst ? z0 : *(const f4 *)(po + q4 + w);
-#pragma unroll
-					for (int k = 0; k < 4; k++)
-						ps[k] = (cc >> (8 * k)) & 4u ? pnew(zs[k], os[k], be, fst) : 0.0f;
-				} else {
-#pragma unroll
-					for (int k = 0; k < 4; k++)
-						ps[k] = (cc >> (8 * k)) & 4u ? pnew(P.z[q4 + k + w],
-								fst ? 0.0f : po[q4 + k + w], be, fst) : 0.0f;
+					pn[k] = (cc >> (8 * k)) & 16u ? pnew(P.z[q4 + k - w],
+							fst ? 0.0f : po[q4 + k - w], be, fst) : 0.0f;
+					ps[k] = (cc >> (8 * k)) & 4u ? pnew(P.z[q4 + k + w],
+							fst ? 0.0f : po[q4 + k + w], be, fst) : 0.0f;
 				}
 				const float pe = cc & (2u << 24) ? pnew(P.z[q4 + 4],
 						fst ? 0.0f : po[q4 + 4], be, fst) : 0.0f;
@@ -1025,13 +1131,13 @@ struct Lev0 {
 	static constexpr int CW = 1;   // code words per block
 	static constexpr bool NEEDK = false;   // gather ignores the code
 	typedef unsigned char Raw;     // one cell's code in memory
-	typedef uchar2 Raw2;           // two cells'
+	typedef unsigned short Raw2;   // two cells' (first in the low byte)
 	typedef unsigned Code;         // one cell's code in registers
 	static __device__ __forceinline__ const Raw *codes(const MgLevel &L) { return L.code; }
 	static __device__ __forceinline__ Code zero() { return 0u; }
 	static __device__ __forceinline__ Code one(Raw r) { return r; }
 	static __device__ __forceinline__ void split(Raw2 r, Code &a, Code &b)
-	{ a = r.x; b = r.y; }
+	{ a = r & 255u; b = (unsigned)r >> 8; }
 	static __device__ __forceinline__ void set(unsigned (&k)[CW], int c, Code x)
 	{ k[0] = c ? k[0] | x << (8 * c) : x; }
 	static __device__ __forceinline__ Code get(const unsigned (&k)[CW], int c)
@@ -1077,7 +1183,7 @@ struct LevG {
 	static constexpr int CW = 4;
 	static constexpr bool NEEDK = true;
 	typedef unsigned Raw;
-	typedef uint2 Raw2;
+	typedef gu2 Raw2;
 	typedef unsigned Code;
 	static __device__ __forceinline__ const Raw *codes(const MgLevel &L) { return L.gcode; }
 	static __device__ __forceinline__ Code zero() { return 0u; }
@@ -1127,13 +1233,13 @@ struct LevG {
 struct LevG64 {
 	static constexpr int CW = 8;
 	static constexpr bool NEEDK = true;
-	typedef uint2 Raw;
-	typedef uint4 Raw2;
+	typedef gu2 Raw;
+	typedef gu4 Raw2;
 	typedef uint2 Code;
 	static __device__ __forceinline__ const Raw *codes(const MgLevel &L)
-	{ return (const uint2 *)L.gcode; }
+	{ return (const Raw *)L.gcode; }
 	static __device__ __forceinline__ Code zero() { return make_uint2(0u, 0u); }
-	static __device__ __forceinline__ Code one(Raw r) { return r; }
+	static __device__ __forceinline__ Code one(Raw r) { return make_uint2(r.x, r.y); }
 	static __device__ __forceinline__ void split(Raw2 r, Code &a, Code &b)
 	{ a = make_uint2(r.x, r.y); b = make_uint2(r.z, r.w); }
 	static __device__ __forceinline__ void set(unsigned (&k)[CW], int c, Code x)
@@ -1248,58 +1354,84 @@ static __device__ __forceinline__ void z_frame(float *z)
 	}
 }
 
-// load b and codes (zero outside the image); z of the region from zsrc or 0.
-// With an even width the two cells of a block row are one aligned pair
-// (region origins are even), loaded together.
-template <class P, class G>
-static __device__ void mg_load(const MgLevel &L, int ox, int oy, MgCells<P, G> &C,
-		float *z, const float *zsrc)
+// load b and codes (zero outside the image); z of the region from zsrc
+// (LOADZ) or 0.  Every load of the thread goes out before the first is
+// used: addresses are clamped into the image, and cells outside it take 0
+// by a select afterwards.  With an even width the two cells of a block row
+// are one aligned pair (region origins are even), loaded together.
+template <bool LOADZ, class P, class G>
+static __device__ __forceinline__ void mg_load(const MgLevel &L, int ox, int oy,
+		MgCells<P, G> &C, float *z, const float *zsrc)
 {
 	C.pc = threadIdx.x % G::PAIRS_X;
 	C.m0 = threadIdx.x / G::PAIRS_X;
 	const MgSpan S = mg_span<G>(L, ox, oy, C.pc, C.m0);
 	const typename P::Raw *code = P::codes(L);
-	const bool pair = !(L.w & 1);
+	float bv[G::K][4], zv[G::K][4];
+	typename P::Code kv[G::K][4];
+	if (!(L.w & 1)) {
+		const int ic = clampi(S.i0, 0, L.w - 2);
+		gf2 br[G::K][2], zr[G::K][2];
+		typename P::Raw2 kr[G::K][2];
+#pragma unroll
+		for (int kb = 0; kb < G::K; kb++)
+#pragma unroll
+			for (int hr = 0; hr < 2; hr++) {
+				const int j = clampi(S.j0 + 2 * kb + hr, 0, L.h - 1);
+				const size_t q = (size_t)j * L.w + ic;
+				br[kb][hr] = gload<gf2>(L.b + q);
+				kr[kb][hr] = gload<typename P::Raw2>(code + q);
+				if (LOADZ) zr[kb][hr] = gload<gf2>(zsrc + q);
+			}
+#pragma unroll
+		for (int kb = 0; kb < G::K; kb++)
+#pragma unroll
+			for (int hr = 0; hr < 2; hr++) {
+				const bool ok = S.colok[0] && row_ok<G>(S, kb, hr);
+				typename P::Code k0, k1;
+				P::split(kr[kb][hr], k0, k1);
+				bv[kb][2 * hr] = ok ? br[kb][hr].x : 0.0f;
+				bv[kb][2 * hr + 1] = ok ? br[kb][hr].y : 0.0f;
+				kv[kb][2 * hr] = ok ? k0 : P::zero();
+				kv[kb][2 * hr + 1] = ok ? k1 : P::zero();
+				zv[kb][2 * hr] = LOADZ && ok ? zr[kb][hr].x : 0.0f;
+				zv[kb][2 * hr + 1] = LOADZ && ok ? zr[kb][hr].y : 0.0f;
+			}
+	} else {
+		float br[G::K][4], zr[G::K][4];
+		typename P::Raw kr[G::K][4];
+#pragma unroll
+		for (int kb = 0; kb < G::K; kb++)
+#pragma unroll
+			for (int c = 0; c < 4; c++) {
+				const int j = clampi(S.j0 + 2 * kb + (c >> 1), 0, L.h - 1);
+				const int i = clampi(S.i0 + (c & 1), 0, L.w - 1);
+				const size_t q = (size_t)j * L.w + i;
+				br[kb][c] = gload<float>(L.b + q);
+				kr[kb][c] = gload<typename P::Raw>(code + q);
+				if (LOADZ) zr[kb][c] = gload<float>(zsrc + q);
+			}
+#pragma unroll
+		for (int kb = 0; kb < G::K; kb++)
+#pragma unroll
+			for (int c = 0; c < 4; c++) {
+				const bool ok = S.colok[c & 1] && row_ok<G>(S, kb, c >> 1);
+				bv[kb][c] = ok ? br[kb][c] : 0.0f;
+				kv[kb][c] = ok ? P::one(kr[kb][c]) : P::zero();
+				zv[kb][c] = LOADZ && ok ? zr[kb][c] : 0.0f;
+			}
+	}
 #pragma unroll
 	for (int kb = 0; kb < G::K; kb++)
 #pragma unroll
-		for (int hr = 0; hr < 2; hr++) {
-			float bv[2] = {0.0f, 0.0f}, zv[2] = {0.0f, 0.0f};
-			typename P::Code kv[2] = {P::zero(), P::zero()};
-			if (row_ok<G>(S, kb, hr)) {
-				const long q = row_q<G>(S, kb, hr);
-				if (pair) {
-					if (S.colok[0]) {
-						const float2 b2 = *(const float2 *)(L.b + q);
-						const typename P::Raw2 k2 = *(const typename P::Raw2 *)(code + q);
-						bv[0] = b2.x; bv[1] = b2.y;
-						P::split(k2, kv[0], kv[1]);
-						if (zsrc) {
-							const float2 z2 = *(const float2 *)(zsrc + q);
-							zv[0] = z2.x; zv[1] = z2.y;
-						}
-					}
-				} else {
-#pragma unroll
-					for (int hc = 0; hc < 2; hc++)
-						if (S.colok[hc]) {
-							bv[hc] = L.b[q + hc];
-							kv[hc] = P::one(code[q + hc]);
-							if (zsrc) zv[hc] = zsrc[q + hc];
-						}
-				}
+		for (int c = 0; c < 4; c++) {
+			C.b[kb][c] = bv[kb][c];
+			P::set(C.k[kb], c, kv[kb][c]);
+			if constexpr (MgCells<P, G>::RD) {
+				C.rd[kb][c] = P::rdiag(kv[kb][c]);
+				C.zr[kb][c] = zv[kb][c];
 			}
-#pragma unroll
-			for (int hc = 0; hc < 2; hc++) {
-				const int c = 2 * hr + hc;
-				C.b[kb][c] = bv[hc];
-				P::set(C.k[kb], c, kv[hc]);
-				if constexpr (MgCells<P, G>::RD) {
-					C.rd[kb][c] = P::rdiag(kv[hc]);
-					C.zr[kb][c] = zv[hc];
-				}
-				z[cell_z(C, kb, c)] = zv[hc];
-			}
+			z[cell_z(C, kb, c)] = zv[kb][c];
 		}
 }
 
@@ -1315,10 +1447,10 @@ static __device__ __forceinline__ void mg_store_z(float *dst, const MgLevel &L,
 		const long q = row_q<G>(S, kb, hr);
 		const float z0 = z[cell_z(C, kb, 2 * hr)], z1 = z[cell_z(C, kb, 2 * hr + 1)];
 		if (!(L.w & 1)) {
-			if (S.colok[0]) *(float2 *)(dst + q) = make_float2(z0, z1);
+			if (S.colok[0]) gstore<gf2>(dst + q, gf2{z0, z1});
 		} else {
-			if (S.colok[0]) dst[q] = z0;
-			if (S.colok[1]) dst[q + 1] = z1;
+			if (S.colok[0]) gstore<float>(dst + q, z0);
+			if (S.colok[1]) gstore<float>(dst + q + 1, z1);
 		}
 	}
 }
@@ -1399,19 +1531,21 @@ static __device__ __forceinline__ bool in_tile(const MgCells<P, G> &C, int kb)
 
 // pre-smoothing from z = 0 (into zd), residual, 2x2 restriction into the
 // next level
+// (two 1024-thread work-groups per CU need at most 64 registers a thread)
 template <class P, class G>
-__global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(G::T == 1024 ? 8 : 4)))
 void k_mg_down(const int4 *tiles,
 		const MgLevel *lv, const MgLevel *coarse, const Cg32Scal *sc)
 {
 	__shared__ float z[G::ZN];
 	const int4 t = tiles[xcd_block()];
-	if (!sc[t.x].active) return;
+	// the descriptors load alongside the activity flag
 	const MgLevel L = lv[t.x], Cc = coarse[t.x];
+	if (!sc[t.x].active) return;
 	const int ox = t.y - G::HALO, oy = t.z - G::HALO;
 	MgCells<P, G> C;
 	z_frame<G>(z);
-	mg_load(L, ox, oy, C, z, nullptr);
+	mg_load<false>(L, ox, oy, C, z, nullptr);
 	__syncthreads();
 	for (int s = 0; s < G::NU; s++) {
 		mg_half<0>(C, z, s == 0);
@@ -1437,8 +1571,8 @@ void k_mg_down(const int4 *tiles,
 		const int I = (ox >> 1) + C.pc, J = (oy >> 1) + brow<G>(C.m0, kb);
 		if (I < Cc.w && J < Cc.h) {
 			const size_t cq = (size_t)J * Cc.w + I;
-			Cc.b[cq] = acc;
-			Cc.z[cq] = 0.0f;
+			gstore<float>(Cc.b + cq, acc);
+			gstore<float>(Cc.z + cq, 0.0f);
 		}
 	}
 }
@@ -1447,8 +1581,9 @@ void k_mg_down(const int4 *tiles,
 // from other tiles' cells, so it must not read the buffer that tiles of the
 // same launch write); at level 0 also the partial r.z of
 // the tile
+// (two 1024-thread work-groups per CU need at most 64 registers a thread)
 template <class P, class G, bool DOT>
-__global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(G::T == 1024 ? 8 : 4)))
 void k_mg_up(const int4 *tiles, const MgLevel *lv,
 		const MgLevel *coarse, const Cg32Scal *sc, float wc, double *tpart)
 {
@@ -1456,18 +1591,28 @@ void k_mg_up(const int4 *tiles, const MgLevel *lv,
 	__shared__ double red[G::T / 64];
 	const int tb = xcd_block();
 	const int4 t = tiles[tb];
-	if (!sc[t.x].active) return;
 	const MgLevel L = lv[t.x], Cc = coarse[t.x];
+	if (!sc[t.x].active) return;
 	const int ox = t.y - G::HALO, oy = t.z - G::HALO;
-	MgCells<P, G> C;
-	z_frame<G>(z);
-	mg_load(L, ox, oy, C, z, L.zd);
-	// every block of the region is one coarse cell: add its correction
+	// every block of the region is one coarse cell (levels are at least
+	// 1 x 1): its correction loads with the region
+	float ce[G::K];
+	bool cin[G::K];
 #pragma unroll
 	for (int kb = 0; kb < G::K; kb++) {
-		const int I = (ox >> 1) + C.pc, J = (oy >> 1) + brow<G>(C.m0, kb);
-		if (I < 0 || J < 0 || I >= Cc.w || J >= Cc.h) continue;
-		const float e = wc * Cc.z[(size_t)J * Cc.w + I];
+		const int I = (ox >> 1) + (int)threadIdx.x % G::PAIRS_X;
+		const int J = (oy >> 1) + brow<G>((int)threadIdx.x / G::PAIRS_X, kb);
+		cin[kb] = I >= 0 && J >= 0 && I < Cc.w && J < Cc.h;
+		ce[kb] = gload<float>(Cc.z + (size_t)clampi(J, 0, Cc.h - 1) * Cc.w
+				+ clampi(I, 0, Cc.w - 1));
+	}
+	MgCells<P, G> C;
+	z_frame<G>(z);
+	mg_load<true>(L, ox, oy, C, z, L.zd);
+#pragma unroll
+	for (int kb = 0; kb < G::K; kb++) {
+		if (!cin[kb]) continue;
+		const float e = wc * ce[kb];
 #pragma unroll
 		for (int c = 0; c < 4; c++) {
 			const int id = cell_z(C, kb, c);
@@ -1913,7 +2058,7 @@ static void precondition(const Cg32Batch &B, int setup, hipStream_t s)
 	vcycle(B, s);
 	k_mg_reduce_t<<<B.nplanes, RT, 0, s>>>(B.planes, B.scal, B.tpart, setup);
 	if (setup && B.bilap)   // p = z + 0 * p
-		k32_dir<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal);
+		k32_dir<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.nblk);
 }
 
 int mg_tail_lds_enable(int bytes)
@@ -1952,7 +2097,7 @@ void cg32_setup(const Cg32Batch &B, hipStream_t s)
 	}
 	// p = z, q = A p, p.q into the buffer iteration 0 reads (par 1 -> p)
 	if (!B.bilap)
-		k32_dirspmv<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part, 1, 1);
+		k32_dirspmv<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part, 1, 1, B.nblk);
 }
 
 // Poisson: alpha, r -= alpha q, beta, z = M r, then the fused direction +
@@ -1963,24 +2108,24 @@ void cg32_iter(const Cg32Batch &B, hipStream_t s, int par)
 {
 	const int pc = B.nlev > 0;
 	if (B.bilap) {
-		k32_spmv<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
+		k32_spmv<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part, B.nblk);
 		k32_reduce_a<<<B.nplanes, RT, 0, s>>>(B.planes, B.scal, B.part, B.nactive);
-		k32_update<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
+		k32_update<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part, B.nblk);
 		k32_reduce_b<<<B.nplanes, RT, 0, s>>>(B.planes, B.scal, B.part, B.nactive,
 				B.trace, pc);
 		if (pc) precondition(B, 0, s);
-		k32_dir<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal);
+		k32_dir<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.nblk);
 		return;
 	}
 	k32_reduce_a<<<B.nplanes, RT, 0, s>>>(B.planes, B.scal, B.part, B.nactive);
-	k32_update<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part);
+	k32_update<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part, B.nblk);
 	k32_reduce_b<<<B.nplanes, RT, 0, s>>>(B.planes, B.scal, B.part, B.nactive,
 			B.trace, pc);
 	if (pc) precondition(B, 0, s);
 	// reads z, the code, p_old (with the four neighbours) and x, writes
 	// p_new, q and x: 29 B per unknown
 	if (B.prof) B.prof("k32_dirspmv", 1, 0, s);
-	k32_dirspmv<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part, par, 0);
+	k32_dirspmv<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part, par, 0, B.nblk);
 	if (B.prof) B.prof("k32_dirspmv", 0, 0.0, s);
 }
 
