@@ -41,25 +41,6 @@ static __device__ __forceinline__ int clampi(int v, int lo, int hi)
 	return v < lo ? lo : (v > hi ? hi : v);
 }
 
-// Loads and stores through the global address space.  A pointer read from
-// a descriptor in memory is generic, and a flat access also counts against
-// the LDS counter: every wait for an LDS read then waits for it too.  The
-// type must be a scalar or an ext_vector type (a struct loaded through a
-// reference falls back to a flat access).
-typedef float gf2 __attribute__((ext_vector_type(2)));
-typedef unsigned gu2 __attribute__((ext_vector_type(2)));
-typedef unsigned gu4 __attribute__((ext_vector_type(4)));
-template <class T>
-static __device__ __forceinline__ T gload(const void *p)
-{
-	return *(const __attribute__((address_space(1))) T *)p;
-}
-template <class T>
-static __device__ __forceinline__ void gstore(void *p, T v)
-{
-	*(__attribute__((address_space(1))) T *)p = v;
-}
-
 // Work-groups go to the 8 XCDs in turn (work-group b to XCD b % 8), each
 // XCD with its own L2.  xcd_block() maps b to a logical block so that every
 // XCD takes a contiguous run of logical blocks, in order: neighbouring

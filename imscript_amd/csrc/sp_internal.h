@@ -31,6 +31,25 @@ struct AdjPlane {
 
 namespace spk {
 
+// Loads and stores through the global address space.  A pointer read from
+// a descriptor in memory is generic, and a flat access also counts against
+// the LDS counter: every wait for an LDS read then waits for it too.  The
+// type must be a scalar or an ext_vector type (a struct loaded through a
+// reference falls back to a flat access).
+typedef float gf2 __attribute__((ext_vector_type(2)));
+typedef unsigned gu2 __attribute__((ext_vector_type(2)));
+typedef unsigned gu4 __attribute__((ext_vector_type(4)));
+template <class T>
+static __device__ __forceinline__ T gload(const void *p)
+{
+	return *(const __attribute__((address_space(1))) T *)p;
+}
+template <class T>
+static __device__ __forceinline__ void gstore(void *p, T v)
+{
+	*(__attribute__((address_space(1))) T *)p = v;
+}
+
 // elementwise / pyramid
 // prepare / restrict write the holes of each work block to bcnt;
 // launch_count folds them into each member's counter

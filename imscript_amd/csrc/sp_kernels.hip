@@ -366,7 +366,7 @@ static __device__ double sum_partials(const double *part, int nblk, double *red)
 	double v = 0;
 	// unrolled so that the loads issue together; the adds keep their order
 #pragma unroll 4
-	for (int k = threadIdx.x; k < nblk; k += CG_NT) v += part[k];
+	for (int k = threadIdx.x; k < nblk; k += CG_NT) v += gload<double>(part + k);
 	double s = block_sum_dbl<CG_NT>(v, red);
 	__shared__ double bc;
 	if (threadIdx.x == 0) bc = s;
@@ -380,7 +380,10 @@ static __device__ double2 sum_partials2(const double *a, const double *b, int n,
 {
 	double va = 0, vb = 0;
 #pragma unroll 4
-	for (int k = threadIdx.x; k < n; k += CG_NT) { va += a[k]; vb += b[k]; }
+	for (int k = threadIdx.x; k < n; k += CG_NT) {
+		va += gload<double>(a + k);
+		vb += gload<double>(b + k);
+	}
 	const double sa = block_sum_dbl<CG_NT>(va, red);
 	const double sb = block_sum_dbl<CG_NT>(vb, red);
 	__shared__ double2 bc;
@@ -518,13 +521,15 @@ __global__ __launch_bounds__(CG_NT) void k_cg_rr0(const CgDev *dp)
 
 static __device__ __forceinline__ bool cg_stopped(const CgDev &d)
 {
-	return __hip_atomic_load(d.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+	return __hip_atomic_load((__attribute__((address_space(1))) int *)d.done,
+			__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
 // The iteration kernels walk the list with the same grid stride (entry t,
 // t + S, t + 2S, ... with S = nblk * CG_NT) and accumulate in that order, in
 // batches of U entries whose loads are issued before the first value is
-// used.
+// used (through the global address space: flat loads under a branch made the
+// compiler wait for each batch member before the next).
 
 // taps of A at (i, j) in the reference's term order (cleant_cgpois.c:91-121):
 // Poisson (i,j), (i+1,j), (i,j+1), (i-1,j), (i,j-1); the biharmonic adds the
@@ -573,8 +578,6 @@ __global__ __launch_bounds__(CG_NT) void k_cg_spmv(const CgDev *dp)
 	// the launch may have more blocks than this level uses (a graph's grid
 	// is fixed); a block's whole work is skipped, so no barrier is split
 	if (blockIdx.x >= (unsigned)d.nblk || cg_stopped(d)) return;
-	// the first kernel of an iteration advances the index the next two read
-	if (blockIdx.x == 0 && threadIdx.x == 0) d.done[2] += 1;
 	constexpr int NT = CgTaps<BILAP>::N;
 	const int w = d.w, h = d.h;
 	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_NT;
@@ -588,7 +591,10 @@ __global__ __launch_bounds__(CG_NT) void k_cg_spmv(const CgDev *dp)
 		for (int u = 0; u < U; u++) {
 			const unsigned k = k0 + u * S;
 			on[u] = k < nc;
-			if (on[u]) { qq[u] = d.idx[k]; pc[u] = d.p[k]; }
+			if (on[u]) {
+				qq[u] = gload<unsigned>(d.idx + k);
+				pc[u] = gload<double>(d.p + k);
+			}
 		}
 		float t[U][NT];
 #pragma unroll
@@ -600,19 +606,21 @@ __global__ __launch_bounds__(CG_NT) void k_cg_spmv(const CgDev *dp)
 				int di, dj;
 				cg_tap_offset(e, di, dj);
 				const int a = clampi(i + di, 0, w - 1), b = clampi(j + dj, 0, h - 1);
-				t[u][e] = pf[(unsigned)b * (unsigned)w + (unsigned)a];
+				t[u][e] = gload<float>(pf + (unsigned)b * (unsigned)w + (unsigned)a);
 			}
 		}
 #pragma unroll
 		for (int u = 0; u < U; u++) {
 			if (!on[u]) continue;
 			const double y = cg_apply_taps<BILAP>(t[u]);
-			d.q[k0 + u * S] = (float)y;   // y is a float widened: exact
+			gstore<float>(d.q + k0 + u * S, (float)y);   // y is a float widened: exact
 			acc += y * pc[u];
 		}
 	}
 	double s = block_sum_dbl<CG_NT>(acc, red);
 	if (threadIdx.x == 0) d.part[blockIdx.x] = s;
+	// the first kernel of an iteration advances the index the next two read
+	if (blockIdx.x == 0 && threadIdx.x == 0) d.done[2] += 1;
 }
 
 // alpha = rr_old / p.q ; r -= alpha q ; partial r.r.  The vectors' loads
@@ -624,22 +632,27 @@ __global__ __launch_bounds__(CG_NT) void k_cg_update(const CgDev *dp)
 	__shared__ double red[CG_NT / 64];
 	__shared__ double alpha_s;
 	if (blockIdx.x >= (unsigned)d.nblk || cg_stopped(d)) return;
-	const int it = d.done[2];
+	const int it = gload<int>(d.done + 2);
+	const double rr0 = gload<double>(d.rr + it);
 	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_NT;
 	double acc = 0;
 	for (unsigned k0 = blockIdx.x * CG_NT + threadIdx.x, first = 1; ; k0 += U * S, first = 0) {
 		// every thread of the block takes part in the first pass (the
 		// partials' sum has barriers), later passes only while in range
 		if (!first && k0 >= nc) break;
-		double rv[U], qv[U];
+		double rv[U];
+		float qv[U];   // widened at the use: a conversion here would wait
 #pragma unroll
 		for (int u = 0; u < U; u++) {
 			const unsigned k = k0 + u * S;
-			if (k < nc) { rv[u] = d.r[k]; qv[u] = (double)d.q[k]; }
+			if (k < nc) {
+				rv[u] = gload<double>(d.r + k);
+				qv[u] = gload<float>(d.q + k);
+			}
 		}
 		if (first) {
 			const double pq = sum_partials(d.part, d.nblk, red);
-			if (threadIdx.x == 0) alpha_s = d.rr[it] / pq;
+			if (threadIdx.x == 0) alpha_s = rr0 / pq;
 			__syncthreads();
 		}
 		const double alpha = alpha_s;
@@ -647,8 +660,8 @@ __global__ __launch_bounds__(CG_NT) void k_cg_update(const CgDev *dp)
 		for (int u = 0; u < U; u++) {
 			const unsigned k = k0 + u * S;
 			if (k >= nc) continue;
-			const double r = rv[u] - alpha * qv[u];
-			d.r[k] = r;
+			const double r = rv[u] - alpha * (double)qv[u];
+			gstore<double>(d.r + k, r);
 			acc += r * r;
 		}
 		if (k0 + U * S >= nc) break;
@@ -668,9 +681,9 @@ __global__ __launch_bounds__(CG_NT) void k_cg_dir(const CgDev *dp)
 	__shared__ double red[CG_NT / 64];
 	__shared__ double alpha_s, rr1_s;
 	if (blockIdx.x >= (unsigned)d.nblk || cg_stopped(d)) return;
-	const int it = d.done[2];
+	const int it = gload<int>(d.done + 2);
 	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_NT;
-	const double rr0 = d.rr[it];
+	const double rr0 = gload<double>(d.rr + it);
 	for (unsigned k0 = blockIdx.x * CG_NT + threadIdx.x, first = 1; ; k0 += U * S, first = 0) {
 		if (!first && k0 >= nc) break;
 		double rv[U], pv[U], xv[U];
@@ -678,7 +691,12 @@ __global__ __launch_bounds__(CG_NT) void k_cg_dir(const CgDev *dp)
 #pragma unroll
 		for (int u = 0; u < U; u++) {
 			const unsigned k = k0 + u * S;
-			if (k < nc) { rv[u] = d.r[k]; pv[u] = d.p[k]; xv[u] = d.x[k]; qq[u] = d.idx[k]; }
+			if (k < nc) {
+				rv[u] = gload<double>(d.r + k);
+				pv[u] = gload<double>(d.p + k);
+				xv[u] = gload<double>(d.x + k);
+				qq[u] = gload<unsigned>(d.idx + k);
+			}
 		}
 		if (first) {
 			const double2 t = sum_partials2(d.part, d.part + d.nblk, d.nblk, red);
@@ -692,11 +710,11 @@ __global__ __launch_bounds__(CG_NT) void k_cg_dir(const CgDev *dp)
 		for (int u = 0; u < U; u++) {
 			const unsigned k = k0 + u * S;
 			if (k >= nc) continue;
-			d.x[k] = xv[u] + alpha * pv[u];
+			gstore<double>(d.x + k, xv[u] + alpha * pv[u]);
 			if (stop) continue;
 			const double pn = rv[u] + beta * pv[u];
-			d.p[k] = pn;
-			d.pf[qq[u]] = (float)pn;
+			gstore<double>(d.p + k, pn);
+			gstore<float>(d.pf + qq[u], (float)pn);
 		}
 		if (k0 + U * S >= nc) break;
 	}
