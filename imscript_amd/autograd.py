@@ -9,6 +9,14 @@ dL/du on the holes, run on the GPU by the same pyramid and CG
 (sp_adjoint_batch_device); then dL/df = -lambda on the holes and
 dL/dg = dL/du + (sum of lambda over hole neighbours) on the known pixels.
 
+biharmonic_fill -- the converged biharmonic fill (simpois -t negative, CG to
+a relative tolerance): the same system with the reference's 13-point
+clamped bilaplacian (cleant_cgpois.c:91-121).  Its clamped taps make A_hh
+non-symmetric within two pixels of the border, so the backward solves
+A_hh^T lambda = dL/du (the transposed operator in the FP32 CG), then
+dL/df = -lambda on the holes and dL/dg = dL/du - A_hk^T lambda on the known
+pixels.
+
 simpois_fill -- simpois's own output with -c 0: niter Gauss-Seidel sweeps
 at every level of the pyramid (bit-identical to the reference).  Backward:
 the exact gradient of that finite-sweep output (sp_adjoint_batch_device
@@ -162,4 +170,60 @@ def simpois_fill(g, f=None, niter=10, scale=99, tstep=0.25):
         f = f[None] if isinstance(f, torch.Tensor) else f
     _check(g, f)
     u = SimpoisFill.apply(g, f, float(tstep), int(niter), int(scale))
+    return u[0] if squeeze else u
+
+
+def _bih_params(tol, niter, scale):
+    # FP32 CG (x in FP64) without multigrid, which covers the Poisson system
+    return default_params(tstep=-0.08, niter=niter, scale=scale, cgit=-1,
+                          cg_fp32=1, cg_rel=tol, cg_mg=0, trace=0)
+
+
+class BiharmonicFill(torch.autograd.Function):
+    """u = biharmonic fill of (g, f) for planar float32 device tensors
+    (B, H, W)."""
+
+    @staticmethod
+    def forward(ctx, g, f, tol, niter, scale):
+        _check(g, f)
+        g = g.contiguous()
+        f = None if f is None else f.contiguous()
+        out = torch.empty_like(g)
+        p = _bih_params(tol, niter, scale)
+        pd, h, w = g.shape
+        st = sp_stats()
+        s = torch.cuda.current_stream()
+        check(lib.sp_solve_device(out.data_ptr(), g.data_ptr(),
+                                  None if f is None else f.data_ptr(), w, h, pd,
+                                  ctypes.byref(p), ctypes.c_void_p(s.cuda_stream),
+                                  ctypes.byref(st)))
+        ctx.save_for_backward(g)
+        ctx.has_f = f is not None
+        ctx.cfg = (tol, niter, scale)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_u):
+        (g,) = ctx.saved_tensors
+        grad_u = grad_u.contiguous()
+        grad_g = torch.empty_like(g)
+        need_f = ctx.has_f and ctx.needs_input_grad[1]
+        grad_f = torch.empty_like(g) if need_f else None
+        tol, _, scale = ctx.cfg
+        # the backward's own pyramid starts each level's CG; sweeps of the
+        # forward operator have no place in the transposed solve
+        _adjoint(g, grad_u, grad_g, grad_f, _bih_params(tol, 0, scale))
+        return grad_g, grad_f, None, None, None
+
+
+def biharmonic_fill(g, f=None, tol=1e-6, niter=0, scale=99):
+    """Differentiable biharmonic fill of the NaN pixels of g ((H,W) or
+    (B,H,W) float32 on the GPU) with optional data term f (Delta^2 u = -f
+    on the holes); gradients flow to the known pixels of g and to f."""
+    squeeze = isinstance(g, torch.Tensor) and g.dim() == 2
+    if squeeze:
+        g = g[None]
+        f = f[None] if isinstance(f, torch.Tensor) else f
+    _check(g, f)
+    u = BiharmonicFill.apply(g, f, float(tol), int(niter), int(scale))
     return u[0] if squeeze else u

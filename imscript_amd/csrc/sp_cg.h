@@ -18,7 +18,8 @@ struct Cg32Plane {
 	                       // N neighbour is an in-image unknown, bits 5-7 the
 	                       // number of in-image neighbours (set by cg32_setup)
 	const int *cnt;    // number of unknowns (device)
-	int w, h, bilap;
+	int w, h, bilap;   // 0 Poisson, 1 biharmonic, 2 the transposed biharmonic
+	                   // operator (the adjoint's solve)
 	int u0;            // the initial guess is 0 at every unknown (coarsest
 	                   // pyramid level, no sweeps before the CG): u is not read
 	int blk0, nblk;    // this plane's work blocks / partial-sum slots

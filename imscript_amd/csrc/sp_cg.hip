@@ -119,6 +119,40 @@ static __device__ __forceinline__ float op32(int bilap, int i, int j, int w, int
 	return r;
 }
 
+// The transposed 13-point operator at (i,j): the sum over the pixels
+// (m,n) whose clamped taps land on (i,j) of the tap's coefficient times
+// val(m,n), i.e. (A^T val)(i,j) over the whole plane.  Such pixels lie
+// within two of (i,j) in each direction.  Four or more pixels from the
+// border no tap of them clamps and the stencil is symmetric, so A^T = A
+// there (op32's sum).  The adjoint of the biharmonic fill solves with it.
+template <typename V>
+static __device__ float op32T(int i, int j, int w, int h, V val)
+{
+	if (i >= 4 && j >= 4 && i + 4 < w && j + 4 < h) return op32(1, i, j, w, h, val);
+	constexpr int DI[13] = {0, 1, 0, -1, 0, 1, 1, -1, -1, 2, 0, -2, 0};
+	constexpr int DJ[13] = {0, 0, 1, 0, -1, 1, -1, 1, -1, 0, 2, 0, -2};
+	constexpr float CF[13] = {20.0f, -8.0f, -8.0f, -8.0f, -8.0f, 2.0f, 2.0f, 2.0f, 2.0f,
+		1.0f, 1.0f, 1.0f, 1.0f};
+	float r = 0.0f;
+	for (int n = max(j - 2, 0); n <= min(j + 2, h - 1); n++)
+		for (int m = max(i - 2, 0); m <= min(i + 2, w - 1); m++) {
+			float c = 0.0f;   // the entry of A in row (m,n), column (i,j)
+#pragma unroll
+			for (int t = 0; t < 13; t++)
+				if (clampi(m + DI[t], 0, w - 1) == i && clampi(n + DJ[t], 0, h - 1) == j)
+					c += CF[t];
+			if (c != 0.0f) r += c * val((size_t)n * w + m);
+		}
+	return r;
+}
+
+// A (bilap 0 or 1) or A^T (bilap 2, the 13-point operator) at (i,j)
+template <typename V>
+static __device__ __forceinline__ float op32_any(int bilap, int i, int j, int w, int h, V val)
+{
+	return bilap == 2 ? op32T(i, j, w, h, val) : op32(bilap, i, j, w, h, val);
+}
+
 // pixel q of thread k-th visit inside work block b, or -1 past the plane
 static __device__ __forceinline__ long pix(const BlockWork &b, int k, size_t n)
 {
@@ -257,9 +291,9 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 					a0 = -r0;
 					ax = -rx;
 				} else {
-					a0 = op32(1, (int)i, (int)j, P.w, P.h, [&](size_t tt) {
+					a0 = op32_any(P.bilap, (int)i, (int)j, P.w, P.h, [&](size_t tt) {
 						const float v = g[tt]; return isnan(v) ? 0.0f : v; });
-					ax = op32(1, (int)i, (int)j, P.w, P.h, [&](size_t tt) {
+					ax = op32_any(P.bilap, (int)i, (int)j, P.w, P.h, [&](size_t tt) {
 						return isnan(g[tt]) ? u[tt] : 0.0f; });
 				}
 				const float bv = (P.f ? -P.f[q] : 0.0f) - a0;
@@ -430,7 +464,7 @@ __global__ __launch_bounds__(NT) void k32_spmv(const BlockWork *bw,
 			if (q >= n) break;
 			if (!(P.code[q] & 1)) continue;
 			const unsigned j = q / w, i = q - j * w;
-			const float y = op32(1, (int)i, (int)j, P.w, P.h,
+			const float y = op32_any(P.bilap, (int)i, (int)j, P.w, P.h,
 					[&](size_t k) { return p[k]; });
 			P.q[q] = y;
 			acc += (double)y * p[q];
@@ -2114,6 +2148,38 @@ void cg32_iter(const Cg32Batch &B, hipStream_t s, int par)
 void cg32_finish(const Cg32Batch &B, hipStream_t s)
 {
 	if (B.bilap) k32_finish<<<B.nblk, NT, 0, s>>>(B.bw, B.planes);
+}
+
+// Gradient of the converged biharmonic fill from lambda = A_hh^-T v (the
+// solve's output, 0 at known pixels): dL/df = -lambda on the holes, and on
+// the known pixels dL/dg = v - (A_hk^T lambda), the transposed operator
+// over lambda there.  Work blocks of 1024 pixels, 4 per thread.
+__global__ __launch_bounds__(NT) void k_adj_gather_bilap(const AdjPlane *ap,
+		const BlockWork *bw)
+{
+	const BlockWork b = bw[blockIdx.x];
+	const AdjPlane A = ap[b.plane];
+	const size_t n = (size_t)A.w * A.h;
+	for (int k = 0; k < PPT; k++) {
+		const size_t q = (size_t)b.start + k * NT + threadIdx.x;
+		if (q >= n) break;
+		if (isnan(A.g[q])) {
+			A.grad_g[q] = 0.0f;
+			if (A.grad_f) A.grad_f[q] = -A.lam[q];
+			continue;
+		}
+		const int i = (int)(q % A.w), j = (int)(q / A.w);
+		const float t = op32T(i, j, A.w, A.h, [&](size_t z) {
+			return isnan(A.g[z]) ? A.lam[z] : 0.0f; });
+		A.grad_g[q] = A.v[q] - t;
+		if (A.grad_f) A.grad_f[q] = 0.0f;
+	}
+}
+
+void launch_adj_gather_bilap(const AdjPlane *ap, const BlockWork *bw, int nblocks,
+		hipStream_t s)
+{
+	if (nblocks) k_adj_gather_bilap<<<nblocks, NT, 0, s>>>(ap, bw);
 }
 
 } // namespace spk

@@ -390,6 +390,8 @@ struct Plan {
 	double *res_part = nullptr, *res_out = nullptr;
 	hipStream_t cap_stream = nullptr;
 
+	int transpose = 0;               // the CG applies the transposed biharmonic
+	                                 // operator (the adjoint's solve)
 	std::vector<char> key;           // identity of the call this plan serves
 	// guidance from a source image: plambda's Laplacian of each plane's
 	// caller-side f (the image) into plan memory, which the solve uses as f
@@ -1158,7 +1160,7 @@ static int cg32_build(Plan &pl, int k, int bilap, float eps)
 		c.p2 = pl.cg32_vec + off; off += na;
 		c.z = c.r;
 		if (pl.P.cg_mg) { c.z = pl.cg32_vec + off; off += na; }
-		c.w = S.w; c.h = S.h; c.bilap = bilap;
+		c.w = S.w; c.h = S.h; c.bilap = bilap && pl.transpose ? 2 : bilap;
 		// the coarsest level starts from 0 at the unknowns (k_init) unless
 		// sweeps ran before the CG
 		c.u0 = k == (int)pl.lv.size() - 1 && L.nsweep[m] <= 0;
@@ -1443,12 +1445,13 @@ extern "C" void sp_plan_cache_clear(void)
 
 // the cached plan of a call (created on a miss); the caller holds g_cache_mu
 static int cache_get(const std::vector<PlaneIn> &planes, const sp_params &P,
-		bool has_f, Plan **out)
+		bool has_f, int transpose, Plan **out)
 {
 	std::vector<char> key;
 	auto put = [&](const void *v, size_t n) {
 		key.insert(key.end(), (const char *)v, (const char *)v + n);
 	};
+	put(&transpose, sizeof transpose);
 	for (const PlaneIn &q : planes) {
 		put(&q.w, sizeof q.w); put(&q.h, sizeof q.h);
 		put(&q.g, sizeof q.g); put(&q.f, sizeof q.f); put(&q.out, sizeof q.out);
@@ -1469,6 +1472,7 @@ static int cache_get(const std::vector<PlaneIn> &planes, const sp_params &P,
 		}
 	for (int attempt = 0;; attempt++) {
 		Plan *pl = new Plan;
+		pl->transpose = transpose;
 		const int rc = plan_create(*pl, planes, P, has_f);
 		if (rc == SP_OK) {
 			pl->key = key;
@@ -1513,10 +1517,15 @@ static int plane_residuals(Plan &pl, hipStream_t s, std::vector<double> &rel,
 	return SP_OK;
 }
 
-extern "C" int sp_solve_batch_device_ex(int n, const sp_image *imgs,
-		const sp_params *p, void *stream, sp_stats *st, sp_plane_stats *ps)
+// transpose: the CG solves with the transposed biharmonic operator (the
+// adjoint of the biharmonic fill; FP32 CG without multigrid or sweeps)
+static int solve_batch(int n, const sp_image *imgs, const sp_params *p, void *stream,
+		sp_stats *st, sp_plane_stats *ps, int transpose)
 {
 	if (n < 1 || !imgs || !p) return set_err(SP_ERR_ARG, "empty batch");
+	if (transpose && !(p->tstep < 0 && p->cg_fp32 && !p->cg_mg && p->niter == 0))
+		return set_err(SP_ERR_ARG, "transposed solves run the biharmonic FP32 CG "
+				"without multigrid or sweeps");
 	TRY(ensure_device());
 	std::vector<PlaneIn> planes;
 	bool has_f = false;
@@ -1532,7 +1541,7 @@ extern "C" int sp_solve_batch_device_ex(int n, const sp_image *imgs,
 	hipStream_t s = (hipStream_t)stream;
 	std::lock_guard<std::mutex> lk(g_cache_mu);
 	Plan *pl = nullptr;
-	TRY(cache_get(planes, *p, has_f, &pl));
+	TRY(cache_get(planes, *p, has_f, transpose, &pl));
 	const int rc = plan_execute(*pl, s, st);
 	if (rc == SP_OK && ps) {
 		std::vector<double> rel;
@@ -1552,6 +1561,12 @@ extern "C" int sp_solve_batch_device_ex(int n, const sp_image *imgs,
 	}
 	cache_trim(pl, g_cache_max, g_cache_max_bytes);
 	return rc;
+}
+
+extern "C" int sp_solve_batch_device_ex(int n, const sp_image *imgs,
+		const sp_params *p, void *stream, sp_stats *st, sp_plane_stats *ps)
+{
+	return solve_batch(n, imgs, p, stream, st, ps, 0);
 }
 
 extern "C" int sp_solve_batch_device(int n, const sp_image *imgs,
@@ -1808,9 +1823,16 @@ extern "C" int sp_adjoint_batch_device(int n, const sp_adjoint_image *imgs,
 		const sp_params *p, void *stream, sp_stats *st)
 {
 	if (n < 1 || !imgs || !p) return set_err(SP_ERR_ARG, "empty batch");
-	if (!(p->tstep > 0))
-		return set_err(SP_ERR_ARG, "the adjoint covers the Poisson/Laplace fill "
-				"(tstep > 0): the clamped bilaplacian is not symmetric");
+	// tstep > 0: the 5-point operator; tstep < 0: the 13-point one, whose
+	// clamped taps make it non-symmetric near the border (the adjoint solve
+	// then applies A^T)
+	const bool bih = p->tstep < 0;
+	if (!(p->tstep > 0) && !bih)
+		return set_err(SP_ERR_ARG, "tstep must be non-zero");
+	if (bih && p->cgit == 0)
+		return set_err(SP_ERR_ARG, "the gradient of the finite sweeps covers the "
+				"5-point operator (tstep > 0); the biharmonic one needs the "
+				"converged fill (cgit != 0)");
 	if (p->guidance)
 		return set_err(SP_ERR_ARG, "the adjoint takes the data term itself "
 				"(guidance = 0); differentiate plambda's Laplacian separately");
@@ -1871,8 +1893,20 @@ extern "C" int sp_adjoint_batch_device(int n, const sp_adjoint_image *imgs,
 			hipMemcpyHostToDevice, s));
 	spk::launch_adj_prepare(A.d_ap, A.d_bw, A.nbw, s);
 	HIPCHK(hipGetLastError());
-	TRY(sp_solve_batch_device(n, fwd.data(), p, stream, st));
-	spk::launch_adj_gather(A.d_ap, A.d_bw, A.nbw, s);
+	if (bih) {
+		// A^T lambda = v on the holes: the pyramid of the adjoint data gives
+		// each level's CG its start, the FP32 CG (x in FP64) solves with the
+		// transposed operator to the caller's tolerance
+		sp_params q = *p;
+		q.niter = 0;
+		q.cg_fp32 = 1;
+		q.cg_mg = 0;
+		TRY(solve_batch(n, fwd.data(), &q, stream, st, nullptr, 1));
+		spk::launch_adj_gather_bilap(A.d_ap, A.d_bw, A.nbw, s);
+	} else {
+		TRY(sp_solve_batch_device(n, fwd.data(), p, stream, st));
+		spk::launch_adj_gather(A.d_ap, A.d_bw, A.nbw, s);
+	}
 	HIPCHK(hipGetLastError());
 	HIPCHK(hipStreamSynchronize(s));
 	return SP_OK;

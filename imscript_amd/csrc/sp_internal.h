@@ -149,6 +149,10 @@ void launch_adj_prepare(const AdjPlane *ap, const BlockWork *bw, int nblocks,
 		hipStream_t s);
 void launch_adj_gather(const AdjPlane *ap, const BlockWork *bw, int nblocks,
 		hipStream_t s);
+// the same for the biharmonic fill (sp_cg.hip): the transposed 13-point
+// operator over lambda at the known pixels
+void launch_adj_gather_bilap(const AdjPlane *ap, const BlockWork *bw, int nblocks,
+		hipStream_t s);
 
 // adjoint of the finite sweeps (sp_kernels.hip): np planes of one level,
 // plane l's arrays at offset l * (pixels of the level)

@@ -240,17 +240,21 @@ typedef struct sp_adjoint_image {
 	const float *g;        /* forward input, NaN = hole */
 	const float *grad_u;   /* dL/du of the filled output */
 	float *grad_g;         /* dL/dg: v at known pixels plus the adjoint
-	                          flux from neighbouring holes; 0 at holes */
+	                          flux from nearby holes; 0 at holes */
 	float *grad_f;         /* dL/df: -lambda at holes, 0 elsewhere; may be NULL */
 } sp_adjoint_image;
 
-/* Gradient of the fill made with the parameters in p (tstep > 0 systems
- * only: the clamped 5-point operator is symmetric).
- * p->cgit != 0: the converged fill; the adjoint is the masked Laplacian
- *   solve A lambda = grad_u on the holes, run by the same pyramid and CG as
- *   the forward solve.
- * p->cgit == 0: simpois's own output (niter Gauss-Seidel sweeps at each of
- *   at most `scale` pyramid levels, PMSFAC, PONLIT), differentiated exactly:
+/* Gradient of the fill made with the parameters in p.
+ * p->cgit != 0: the converged fill.  tstep > 0 (clamped 5-point operator,
+ *   symmetric): the adjoint is the masked Laplacian solve A lambda = grad_u
+ *   on the holes, run by the same pyramid and CG as the forward solve.
+ *   tstep < 0 (the 13-point bilaplacian, not symmetric within two pixels of
+ *   the border): A^T lambda = grad_u with the transposed operator, by the
+ *   FP32 CG (x in FP64) at every pyramid level without sweeps, to the
+ *   tolerance in p; grad_g = grad_u - A_hk^T lambda on the known pixels.
+ * p->cgit == 0 (tstep > 0 only): simpois's own output (niter Gauss-Seidel
+ *   sweeps at each of at most `scale` pyramid levels, PMSFAC, PONLIT),
+ *   differentiated exactly:
  *   transposed sweeps (run as forward sweeps of the image turned by 180
  *   degrees), transposed prolongation and restrictions.  The sweeps are
  *   linear with coefficients fixed by the mask, so no forward iterate is

@@ -1,4 +1,5 @@
-"""FP64 CPU reference of the converged masked Poisson fill and its gradient.
+"""FP64 CPU reference of the converged masked Poisson and biharmonic fills
+and their gradients.
 
 TEST INFRASTRUCTURE: only tests/ and the benchmark's checks use it.
 
@@ -103,11 +104,13 @@ def solve_bilap(g, f=None):
     return u.reshape(g.shape)
 
 
-def gradient(g, f, v):
-    """(dL/dg, dL/df) for dL/du = v, FP64."""
+def gradient(g, f, v, bilap=False):
+    """(dL/dg, dL/df) for dL/du = v, FP64; bilap: the biharmonic fill of
+    solve_bilap (its A_hh is not symmetric near the border: the transpose
+    below matters there)."""
     g = np.asarray(g, np.float64)
     v = np.asarray(v, np.float64).ravel()
-    Ahh, Ahk, ih, ik = masked_operator(g)
+    Ahh, Ahk, ih, ik = (masked_bilaplacian if bilap else masked_operator)(g)
     lam = spla.spsolve(Ahh.T.tocsc(), v[ih]) if len(ih) else np.zeros(0)
     gg = np.zeros(g.size)
     gf = np.zeros(g.size)

@@ -1,8 +1,9 @@
 """The FP64 references of oracle/adjoint_ref.py pinned on the CPU: central
 finite differences of its own FP64 solve, the symmetry of the clamped
 5-point operator, agreement of its solve with the oracle's restatement run
-to convergence; for the 13-point bilaplacian, exact refill of cubics and
-agreement with an independent per-pixel assembly."""
+to convergence; for the 13-point bilaplacian, exact refill of cubics,
+agreement with an independent per-pixel assembly, and its gradient against
+the exact change of a linear loss."""
 import os
 import sys
 
@@ -111,6 +112,41 @@ def test_bilaplacian_operator_matches_independent_assembly():
     assert np.max(np.abs(b - b2)) < 1e-9 * np.max(np.abs(b))
     u = ar.solve_bilap(g, f)
     assert np.max(np.abs(u.ravel()[ih] - spla.spsolve(A.tocsc(), b))) < 1e-8 * np.max(np.abs(u))
+
+
+def test_bilaplacian_gradient_is_exact_and_needs_the_transpose():
+    """gradient(bilap=True) against the change of the loss: the fill is
+    affine in (g, f), so <grad, d> equals L(g + d) - L(g) for a linear loss,
+    on holes that touch the border.  The same formula with A_hh in place of
+    its transpose fails this check there (A_hh is not symmetric)."""
+    rng = np.random.default_rng(12)
+    g = rng.random((13, 15)) * 40
+    g[2:9, 3:11] = np.nan
+    g[0:2, 12:15] = np.nan
+    g[10:13, 0:2] = np.nan
+    f = rng.standard_normal(g.shape)
+    v = rng.standard_normal(g.shape)
+    hole = np.isnan(g)
+    gg, gf = ar.gradient(g, f, v, bilap=True)
+    assert np.all(gg[hole] == 0) and np.all(gf[~hole] == 0)
+    dg = np.where(hole, 0, rng.standard_normal(g.shape))
+    df = rng.standard_normal(g.shape)
+    du = ar.solve_bilap(g + dg, f + df) - ar.solve_bilap(g, f)
+    lhs = np.sum(gg * dg) + np.sum(gf * df)
+    assert np.isclose(lhs, np.sum(v * du), rtol=1e-9)
+    # the symmetric shortcut (lambda = A_hh^-1 v) gives a different gradient
+    Ahh, Ahk, ih, ik = ar.masked_bilaplacian(g)
+    assert (Ahh - Ahh.T).count_nonzero() > 0
+    lam_wrong = spla.spsolve(Ahh.tocsc(), v.ravel()[ih])
+    gf_wrong = np.zeros(g.size)
+    gf_wrong[ih] = -lam_wrong
+    lhs_wrong = np.sum(gg * dg) + np.sum(gf_wrong.reshape(g.shape) * df)
+    assert not np.isclose(lhs_wrong, np.sum(v * du), rtol=1e-6)
+    # interior holes only: the operator is symmetric there and both agree
+    g2 = rng.random((15, 15)) * 40
+    g2[5:10, 5:10] = np.nan
+    Ahh2, _, _, _ = ar.masked_bilaplacian(g2)
+    assert (Ahh2 - Ahh2.T).count_nonzero() == 0
 
 
 # ------------------------------------------------- simpois's finite sweeps

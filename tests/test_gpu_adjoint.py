@@ -170,3 +170,75 @@ def test_sweep_gradient_dot_product_at_scale():
     _, gg2, gf2 = sweep_grads(g, f, v, 30, 99)
     assert np.array_equal(gg2.view(np.uint32), gg.view(np.uint32))
     assert np.array_equal(gf2.view(np.uint32), gf.view(np.uint32))
+
+
+# ------------------------------------------- the converged biharmonic fill
+
+def bih_problem(seed, b=2, h=22, w=26):
+    """Smooth data with holes, some of them on the image border, where the
+    clamped 13-point operator is not symmetric."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:h, 0:w]
+    g = np.empty((b, h, w), np.float32)
+    for k in range(b):
+        g[k] = 30 * np.sin(x / rng.uniform(5, 9)) * np.cos(y / rng.uniform(5, 9)) + 50
+    g[:, 5:13, 6:17] = np.nan
+    g[:, 0:3, 19:26] = np.nan
+    g[1, 16:22, 0:4] = np.nan
+    f = rng.normal(0, 0.05, (b, h, w)).astype(np.float32)
+    v = rng.standard_normal((b, h, w)).astype(np.float32)
+    return g, f, v
+
+
+def bih_grads(g, f, v, tol):
+    import torch
+    from imscript_amd.autograd import biharmonic_fill
+    dg = torch.from_numpy(g).cuda().requires_grad_(True)
+    df = torch.from_numpy(f).cuda().requires_grad_(True)
+    u = biharmonic_fill(dg, df, tol=tol)
+    (u * torch.from_numpy(v).cuda()).sum().backward()
+    return u.detach().cpu().numpy(), dg.grad.cpu().numpy(), df.grad.cpu().numpy()
+
+
+def test_biharmonic_gradient_matches_fp64_reference():
+    """Forward against the FP64 direct solve and the gradient against the
+    FP64 adjoint (oracle/adjoint_ref.gradient with bilap, pinned by the exact
+    change of a linear loss on the CPU), bar 1e-3 relative.  The symmetric
+    shortcut (lambda = A_hh^-1 v) misses the reference by more than the bar
+    on these border holes, so the transposed operator is what passes."""
+    import scipy.sparse.linalg as spla
+    g, f, v = bih_problem(3)
+    print("biharmonic adjoint", g.shape, flush=True)
+    u, gg, gf = bih_grads(g, f, v, 1e-7)
+    for k in range(g.shape[0]):
+        hole = np.isnan(g[k])
+        uref = ar.solve_bilap(g[k], f[k])
+        assert rel_err(u[k], uref) < 1e-4, k
+        rg, rf = ar.gradient(g[k], f[k], v[k], bilap=True)
+        print(k, "grad_g", rel_err(gg[k], rg), "grad_f", rel_err(gf[k], rf), flush=True)
+        assert rel_err(gg[k], rg) < 1e-3, k
+        assert rel_err(gf[k], rf) < 1e-3, k
+        assert np.all(gg[k][hole] == 0) and np.all(gf[k][~hole] == 0)
+        Ahh, _, ih, _ = ar.masked_bilaplacian(g[k].astype(np.float64))
+        wrong = np.zeros(g[k].size)
+        wrong[ih] = -spla.spsolve(Ahh.tocsc(), v[k].astype(np.float64).ravel()[ih])
+        assert rel_err(wrong.reshape(g[k].shape), rf) > 1e-2, k
+
+
+def test_biharmonic_gradient_dot_product():
+    """At 2 x 96 x 120 against the change of the loss for a random direction
+    (the fill is affine in (g, f)), bar 1e-3 relative."""
+    import torch
+    from imscript_amd.autograd import biharmonic_fill
+    g, f, v = bih_problem(4, 2, 96, 120)
+    rng = np.random.default_rng(7)
+    dg = np.where(np.isnan(g), 0, rng.standard_normal(g.shape)).astype(np.float32)
+    df = (0.05 * rng.standard_normal(g.shape)).astype(np.float32)
+    print("biharmonic adjoint dot product 2x96x120", flush=True)
+    u, gg, gf = bih_grads(g, f, v, 1e-7)
+    u1 = biharmonic_fill(torch.from_numpy(g + dg).cuda(), torch.from_numpy(f + df).cuda(),
+                         tol=1e-7).cpu().numpy()
+    lhs = float(np.sum(gg.astype(np.float64) * dg) + np.sum(gf.astype(np.float64) * df))
+    rhs = float(np.sum(v.astype(np.float64) * (u1.astype(np.float64) - u)))
+    print("<grad, d>", lhs, "change of the loss", rhs, flush=True)
+    assert abs(lhs - rhs) <= 1e-3 * abs(rhs)
