@@ -96,7 +96,7 @@ __global__ __launch_bounds__(Geo<S>::NT) void k_amle_tile(const GsTile *tiles,
 	const GsTile tl = tiles[blockIdx.x];
 	const GsInst in = insts[tl.inst];
 	const TileCtx<S> c = tile_ctx<S>(tl, in, smem);
-	tile_load<S>(c, in, threadIdx.x);
+	tile_load_dev<S>(c, in);
 	__syncthreads();
 	for (int sg = 0; sg < G::NSTEP; sg++) {
 		amle_tile_step<S>(c, threadIdx.x, sg, nn);
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(Geo<S>::NT) void k_amle4_tile(const GsTile *tiles,
 	const GsTile tl = tiles[blockIdx.x];
 	const GsInst in = insts[tl.inst];
 	const TileCtx<S> c = tile_ctx<S>(tl, in, smem);
-	tile_load<S>(c, in, threadIdx.x);
+	tile_load_dev<S>(c, in);
 	Amle4Thread t = amle4_thread<S>(c, threadIdx.x);
 	__syncthreads();
 	for (int sg = 0; sg < G::NSTEP; sg++) {

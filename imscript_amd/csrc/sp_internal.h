@@ -50,6 +50,50 @@ static __device__ __forceinline__ void gstore(void *p, T v)
 	*(__attribute__((address_space(1))) T *)p = v;
 }
 
+// tile_load's footprint copy with a thread's loads issued in batches of up
+// to 8 cells before the batch's first LDS write: one at a time, each LDS
+// write waited for its own load.  Every load is of an address clamped into
+// the image (no branch around it: a value merged after a branch made the
+// compiler wait there), and a cell outside the image then takes, for R = 1,
+// the NaN value and the known marker by a select.  The same cells get the
+// same values as in tile_load.
+template <class S>
+static __device__ __forceinline__ void tile_load_dev(const TileCtx<S> &c, const GsInst &in)
+{
+	using G = Geo<S>;
+	constexpr int R = S::R;
+	constexpr int N = G::LW * G::LH;
+	constexpr int IT = (N + G::NT - 1) / G::NT;
+	constexpr int B = IT < 8 ? IT : 8;
+	const int tid = (int)threadIdx.x;
+	for (int e0 = 0; e0 < IT; e0 += B) {
+		float uv[B], cv[B];
+		bool ok[B];
+#pragma unroll
+		for (int u = 0; u < B; u++) {
+			const int e = tid + (e0 + u) * G::NT;
+			const int row = e / G::LW, col = e - row * G::LW;
+			const int j = row + c.jlo, i = col + c.K - G::SH * j;
+			ok[u] = e < N && i >= 0 && i < c.W && j >= 0 && j < c.H;
+			const size_t q = (size_t)sp_clamp(j, 0, c.H - 1) * c.W + sp_clamp(i, 0, c.W - 1);
+			uv[u] = gload<float>(in.u + q);
+			cv[u] = gload<float>(in.c + q);
+		}
+#pragma unroll
+		for (int u = 0; u < B; u++) {
+			const int e = tid + (e0 + u) * G::NT;
+			if (e >= N) continue;
+			if (ok[u]) {
+				c.U[e] = uv[u];
+				c.C[e] = cv[u];
+			} else if (R == 1) {
+				c.U[e] = sp_float(SP_QNAN_BITS);
+				c.C[e] = sp_float(SP_KNOWN_BITS);
+			}
+		}
+	}
+}
+
 // elementwise / pyramid
 // prepare / restrict write the holes of each work block to bcnt;
 // launch_count folds them into each member's counter
