@@ -71,10 +71,19 @@ __global__ __launch_bounds__(EW_NT) void k_prepare(const SpPlane *planes,
 	const SpPlane P = planes[b.plane];
 	const size_t n = (size_t)P.w * P.h;
 	int holes = 0;
+	// the thread's loads of g go out together (an index past the plane
+	// loads the plane's last pixel and is dropped)
+	float gv[EW_PPT];
+#pragma unroll
 	for (int k = 0; k < EW_PPT; k++) {
-		size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
+		const size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
+		gv[k] = gload<float>(P.g + (q < n ? q : n - 1));
+	}
+#pragma unroll
+	for (int k = 0; k < EW_PPT; k++) {
+		const size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
 		if (q >= n) break;
-		float g = P.g[q];
+		const float g = gv[k];
 		holes += isnan(g) ? 1 : 0;
 		if (P.c) P.c[q] = coef_of(g, P.f, q);
 	}
@@ -135,13 +144,9 @@ __global__ __launch_bounds__(EW_NT) void k_count(const SpPlane *planes,
 	if (threadIdx.x == 0) *planes[blockIdx.x].cnt = s;
 }
 
-// value of the coarse output at q: after a level, simpois writes the input
-// back at every non-NaN pixel when the level has holes (cleant_cgpois.c:251)
-static __device__ __forceinline__ float level_out(const SpPlane &P, size_t q)
-{
-	float g = P.g[q];
-	return (*P.cnt > 0 && !isnan(g)) ? g : P.u[q];
-}
+// The output of a level at q (level_out below): after a level, simpois
+// writes the input back at every non-NaN pixel when the level has holes
+// (cleant_cgpois.c:251), so the output is g there and u elsewhere.
 
 // u = isfinite(g) ? g : init, init = bilinear x2 of the coarse output
 // (simpois.c:159-160, 199-239) or 0 at the coarsest level
@@ -150,29 +155,56 @@ __global__ __launch_bounds__(EW_NT) void k_init(const SpPlane *lv,
 {
 	const BlockWork b = bw[blockIdx.x];
 	const SpPlane P = lv[b.plane];
+	const SpPlane C = coarse[b.plane];   // C.g == nullptr: coarsest level
 	const size_t n = (size_t)P.w * P.h;
+	// two rounds of loads for the thread's pixels, each issued whole before
+	// its values are used: g; then, at holes, the coarse level's output at
+	// the four corners (its g and u)
+	float gv[EW_PPT];
+#pragma unroll
 	for (int k = 0; k < EW_PPT; k++) {
-		size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
+		const size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
+		gv[k] = gload<float>(P.g + (q < n ? q : n - 1));
+	}
+	const bool cany = C.g && *C.cnt > 0;
+	float cg[EW_PPT][4], cu[EW_PPT][4], fx[EW_PPT], fy[EW_PPT];
+#pragma unroll
+	for (int k = 0; k < EW_PPT; k++) {
+		const size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
+		if (!(C.g && q < n && !isfinite(gv[k]))) continue;
+		const int i = (int)(q % P.w), j = (int)(q / P.w);
+		const float x = (float)((i - 0.5) / 2), y = (float)((j - 0.5) / 2);
+		const int ix = (int)floorf(x), iy = (int)floorf(y);
+		const int x0 = clampi(ix, 0, C.w - 1), x1 = clampi(ix + 1, 0, C.w - 1);
+		const int y0 = clampi(iy, 0, C.h - 1), y1 = clampi(iy + 1, 0, C.h - 1);
+		const size_t cq[4] = {(size_t)y0 * C.w + x0, (size_t)y0 * C.w + x1,
+			(size_t)y1 * C.w + x0, (size_t)y1 * C.w + x1};
+#pragma unroll
+		for (int c = 0; c < 4; c++) {
+			cg[k][c] = gload<float>(C.g + cq[c]);
+			cu[k][c] = gload<float>(C.u + cq[c]);
+		}
+		fx[k] = x - (float)ix;
+		fy[k] = y - (float)iy;
+	}
+#pragma unroll
+	for (int k = 0; k < EW_PPT; k++) {
+		const size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
 		if (q >= n) break;
-		float g = P.g[q];
+		const float g = gv[k];
 		float init = 0.0f;
-		const SpPlane C = coarse[b.plane];   // C.g == nullptr: coarsest level
 		if (C.g && !isfinite(g)) {
-			int i = (int)(q % P.w), j = (int)(q / P.w);
-			float x = (float)((i - 0.5) / 2), y = (float)((j - 0.5) / 2);
-			int ix = (int)floorf(x), iy = (int)floorf(y);
-			int x0 = clampi(ix, 0, C.w - 1), x1 = clampi(ix + 1, 0, C.w - 1);
-			int y0 = clampi(iy, 0, C.h - 1), y1 = clampi(iy + 1, 0, C.h - 1);
-			float a = level_out(C, (size_t)y0 * C.w + x0);
-			float bb = level_out(C, (size_t)y0 * C.w + x1);
-			float cc = level_out(C, (size_t)y1 * C.w + x0);
-			float d = level_out(C, (size_t)y1 * C.w + x1);
-			float fx = x - (float)ix, fy = y - (float)iy;
+			float v[4];
+#pragma unroll
+			for (int c = 0; c < 4; c++)
+				v[c] = cany && !isnan(cg[k][c]) ? cg[k][c] : cu[k][c];
+			const float a = v[0], bb = v[1], cc = v[2], d = v[3];
+			const float ffx = fx[k], ffy = fy[k];
 			float r = 0.0f;
-			r = r + a * (1 - fx) * (1 - fy);
-			r = r + bb * fx * (1 - fy);
-			r = r + cc * (1 - fx) * fy;
-			r = r + d * fx * fy;
+			r = r + a * (1 - ffx) * (1 - ffy);
+			r = r + bb * ffx * (1 - ffy);
+			r = r + cc * (1 - ffx) * ffy;
+			r = r + d * ffx * ffy;
 			init = r;
 		}
 		P.u[q] = isfinite(g) ? g : init;
@@ -192,18 +224,32 @@ __global__ __launch_bounds__(EW_NT) void k_finalize(const SpPlane *lv,
 		// write holes only), so only the known +-Inf pixels, which the
 		// solve saw as holes' initial values, change
 		if (*P.cnt <= 0) return;
+		float gv[EW_PPT];
+#pragma unroll
 		for (int k = 0; k < EW_PPT; k++) {
-			size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
-			if (q >= n) break;
-			const float g = P.g[q];
-			if (isinf(g)) out[q] = g;
+			const size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
+			gv[k] = gload<float>(P.g + (q < n ? q : n - 1));
+		}
+#pragma unroll
+		for (int k = 0; k < EW_PPT; k++) {
+			const size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
+			if (q < n && isinf(gv[k])) out[q] = gv[k];
 		}
 		return;
 	}
+	// the level's output at the thread's pixels, their loads issued together
+	const bool any = *P.cnt > 0;
+	float gv[EW_PPT], uv[EW_PPT];
+#pragma unroll
 	for (int k = 0; k < EW_PPT; k++) {
-		size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
-		if (q >= n) break;
-		out[q] = level_out(P, q);
+		const size_t q = (size_t)b.start + k * EW_NT + threadIdx.x, qc = q < n ? q : n - 1;
+		gv[k] = gload<float>(P.g + qc);
+		uv[k] = gload<float>(P.u + qc);
+	}
+#pragma unroll
+	for (int k = 0; k < EW_PPT; k++) {
+		const size_t q = (size_t)b.start + k * EW_NT + threadIdx.x;
+		if (q < n) out[q] = any && !isnan(gv[k]) ? gv[k] : uv[k];
 	}
 }
 
