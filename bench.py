@@ -44,11 +44,11 @@ PEAK_HBM_GBPS = 8000.0
 # measured on: TRAFFIC_SRC is the SHA-256 prefix of imscript_amd/csrc/sp_cg.hip
 # then, and the line reports traffic null when the source differs.
 TRAFFIC_PMC = {
-    "mg_up0": 634.0e6,
+    "mg_up0": 634.1e6,
     "mg_down0": 495.3e6,
-    "k32_dirspmv": 861.2e6,
+    "k32_dirspmv": 861.1e6,
 }
-TRAFFIC_SRC = "6c659cea2005750e"
+TRAFFIC_SRC = "4d2548068a087621"
 
 
 def kernel_source_id(name="sp_cg.hip"):
