@@ -975,8 +975,9 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this path runs on one GPU; --gpus {a.gpus} is not "
-              "supported (planes would shard across GPUs as whole images, one "
-              "process per GPU; nothing here measures more than one)", file=sys.stderr)
+              "supported (a batch would shard by image and a large image by "
+              "row slabs, one process per GPU; nothing here measures more than "
+              "one)", file=sys.stderr)
         sys.exit(2)
     if a.config in ("c1", "c3", "c4", "c5"):
         emit({"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5}[a.config](a))
