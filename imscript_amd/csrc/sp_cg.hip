@@ -164,6 +164,29 @@ static __device__ __forceinline__ long pix(const BlockWork &b, int k, size_t n)
 // g), packed in one word (pixel q4 + k in byte k).  A group inside one row
 // of a 16-byte aligned plane (w a multiple of 4) reads its row and the rows
 // above and below as one 16-byte load each.
+// the code word of the group at (i, j) from its row c, the rows above (up)
+// and below (dn) and the pixels left (lf) and right (rt) of it; a
+// neighbour outside the plane is never looked at, so those may hold any value
+static __device__ __forceinline__ unsigned row_code(f4 c, f4 up, f4 dn, float lf, float rt,
+		unsigned i, unsigned j, unsigned w, unsigned h)
+{
+	const unsigned nrow = (j > 0) + (j + 1 < h);
+	unsigned word = 0;
+#pragma unroll
+	for (int k = 0; k < 4; k++) {
+		if (!isnan(c[k])) continue;
+		const unsigned ii = i + k;
+		unsigned v = 1;
+		if (ii + 1 < w && isnan(k < 3 ? c[k + 1] : rt)) v |= 2;
+		if (j + 1 < h && isnan(dn[k])) v |= 4;
+		if (ii > 0 && isnan(k > 0 ? c[k - 1] : lf)) v |= 8;
+		if (j > 0 && isnan(up[k])) v |= 16;
+		v |= ((ii > 0) + (ii + 1 < w) + nrow) << 5;
+		word |= v << (8 * k);
+	}
+	return word;
+}
+
 static __device__ __forceinline__ unsigned group_code(const Cg32Plane &P, unsigned q4,
 		unsigned q1)
 {
@@ -175,20 +198,7 @@ static __device__ __forceinline__ unsigned group_code(const Cg32Plane &P, unsign
 		const f4 dn = j + 1 < h ? *(const f4 *)(P.g + q4 + w) : c;
 		const float lf = i > 0 ? P.g[q4 - 1] : 0.0f;
 		const float rt = i + 4 < w ? P.g[q4 + 4] : 0.0f;
-		const unsigned nrow = (j > 0) + (j + 1 < h);
-#pragma unroll
-		for (int k = 0; k < 4; k++) {
-			if (!isnan(c[k])) continue;
-			const unsigned ii = i + k;
-			unsigned v = 1;
-			if (ii + 1 < w && isnan(k < 3 ? c[k + 1] : rt)) v |= 2;
-			if (j + 1 < h && isnan(dn[k])) v |= 4;
-			if (ii > 0 && isnan(k > 0 ? c[k - 1] : lf)) v |= 8;
-			if (j > 0 && isnan(up[k])) v |= 16;
-			v |= ((ii > 0) + (ii + 1 < w) + nrow) << 5;
-			word |= v << (8 * k);
-		}
-		return word;
+		return row_code(c, up, dn, lf, rt, i, j, w, h);
 	}
 	for (unsigned q = q4; q < q1; q++) {
 		unsigned c = 0;
@@ -220,92 +230,104 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 	const unsigned q4 = (unsigned)b.start + 4 * threadIdx.x;
 	const unsigned q1 = q4 + 4 < n ? q4 + 4 : n;
 	const bool whole = q4 + 3 < n;
-	const unsigned cc = q4 < n ? group_code(P, q4, q1) : 0u;
 	f4 rv = {0.0f, 0.0f, 0.0f, 0.0f};
 	d4 xv = {0.0, 0.0, 0.0, 0.0};
 	const float *g = P.g, *u = P.u;
-	if (whole && (cc & 0x01010101u) && !P.bilap && !(w & 3)
+	unsigned cc = 0;
+	if (whole && !P.bilap && !(w & 3)
 			&& !(((uintptr_t)g | (uintptr_t)u | (uintptr_t)P.f) & 15)) {
 		// a group inside one row of 16-byte aligned planes: the clamped
-		// 5-point reads of g and u as one 16-byte load per row and a scalar
-		// on each side; the same sums in the same order as below.  With u0
-		// only its values at unknowns enter, all 0: u is not read.
-		const unsigned j = q4 / w, i = q4 - j * w;
+		// 5-point reads of g as one 16-byte load per row and a scalar on
+		// each side, all at once; the codes from them; then, for a group
+		// with an unknown, the same reads of u and f at once; the same sums
+		// in the same order as below.  With u0 only its values at unknowns
+		// enter, all 0: u is not read.
+		const unsigned h = (unsigned)P.h, j = q4 / w, i = q4 - j * w;
+		const unsigned qn = j > 0 ? q4 - w : q4, qs = j + 1 < h ? q4 + w : q4;
+		const unsigned qw = i > 0 ? q4 - 1 : q4, qe = i + 4 < w ? q4 + 4 : q4 + 3;
 		const f4 z0 = {0.0f, 0.0f, 0.0f, 0.0f};
 		const bool ru = !P.u0;
-		const f4 gc = *(const f4 *)(g + q4), uc = ru ? *(const f4 *)(u + q4) : z0;
-		const f4 gn = j > 0 ? *(const f4 *)(g + q4 - w) : gc;
-		const f4 un = !ru ? z0 : j > 0 ? *(const f4 *)(u + q4 - w) : uc;
-		const f4 gs = j + 1 < (unsigned)P.h ? *(const f4 *)(g + q4 + w) : gc;
-		const f4 us = !ru ? z0 : j + 1 < (unsigned)P.h ? *(const f4 *)(u + q4 + w) : uc;
-		const float gw = i > 0 ? g[q4 - 1] : gc[0];
-		const float uw = !ru ? 0.0f : i > 0 ? u[q4 - 1] : uc[0];
-		const float ge = i + 4 < w ? g[q4 + 4] : gc[3];
-		const float ue = !ru ? 0.0f : i + 4 < w ? u[q4 + 4] : uc[3];
-		const f4 fv = P.f ? *(const f4 *)(P.f + q4) : f4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-		for (int k = 0; k < 4; k++) {
-			if (!((cc >> (8 * k)) & 1u)) continue;
-			const float gv[4] = {k < 3 ? gc[k + 1] : ge, gs[k], k > 0 ? gc[k - 1] : gw, gn[k]};
-			const float uv[4] = {k < 3 ? uc[k + 1] : ue, us[k], k > 0 ? uc[k - 1] : uw, un[k]};
-			float r0 = -4.0f * 0.0f, rx = -4.0f * uc[k];
-#pragma unroll
-			for (int d = 0; d < 4; d++) {
-				const bool kn = !isnan(gv[d]);
-				r0 = r0 + (kn ? gv[d] : 0.0f);
-				rx = rx + (kn ? 0.0f : uv[d]);
+		const f4 gc = gload<f4>(g + q4), gn = gload<f4>(g + qn), gs = gload<f4>(g + qs);
+		const float gw = gload<float>(g + qw), ge = gload<float>(g + qe);
+		cc = row_code(gc, gn, gs, gw, ge, i, j, w, h);
+		if (cc & 0x01010101u) {
+			f4 uc = z0, un = z0, us = z0;
+			float uw = 0.0f, ue = 0.0f;
+			if (ru) {
+				uc = gload<f4>(u + q4);
+				un = gload<f4>(u + qn);
+				us = gload<f4>(u + qs);
+				uw = gload<float>(u + qw);
+				ue = gload<float>(u + qe);
 			}
-			const float bv = (P.f ? -fv[k] : 0.0f) - -r0;
-			const float r = bv - -rx;
-			rr += (double)r * r;
-			bb += (double)bv * bv;
-			rv[k] = r;
-			xv[k] = (double)uc[k];
-		}
-	} else if (!whole || (cc & 0x01010101u)) {
-		// coordinates of the group's first pixel; the others follow along
-		// the row (a group may wrap onto the next row)
-		unsigned j = q4 / w, i = q4 - j * w;
-		for (unsigned q = q4; q < q1; q++) {
-			const unsigned k = q - q4;
-			const bool hole = (cc >> (8 * k)) & 1u;
-			if (!whole) P.code[q] = (unsigned char)(cc >> (8 * k));
-			if (hole) {
-				float a0, ax;
-				if (!P.bilap) {
-					// op32's clamped 5-point sum in its term order, both fields
-					// (known values, unknowns) from one set of loads
-					const unsigned t[4] = {i + 1 < w ? q + 1 : q,
-						j + 1 < (unsigned)P.h ? q + w : q, i > 0 ? q - 1 : q,
-						j > 0 ? q - w : q};
-					float gv[4], uv[4];
+			const f4 fv = P.f ? gload<f4>(P.f + q4) : z0;
 #pragma unroll
-					for (int d = 0; d < 4; d++) { gv[d] = g[t[d]]; uv[d] = u[t[d]]; }
-					float r0 = -4.0f * 0.0f, rx = -4.0f * u[q];
+			for (int k = 0; k < 4; k++) {
+				if (!((cc >> (8 * k)) & 1u)) continue;
+				const float gv[4] = {k < 3 ? gc[k + 1] : ge, gs[k], k > 0 ? gc[k - 1] : gw, gn[k]};
+				const float uv[4] = {k < 3 ? uc[k + 1] : ue, us[k], k > 0 ? uc[k - 1] : uw, un[k]};
+				float r0 = -4.0f * 0.0f, rx = -4.0f * uc[k];
 #pragma unroll
-					for (int d = 0; d < 4; d++) {
-						const bool kn = !isnan(gv[d]);
-						r0 = r0 + (kn ? gv[d] : 0.0f);
-						rx = rx + (kn ? 0.0f : uv[d]);
-					}
-					a0 = -r0;
-					ax = -rx;
-				} else {
-					a0 = op32_any(P.bilap, (int)i, (int)j, P.w, P.h, [&](size_t tt) {
-						const float v = g[tt]; return isnan(v) ? 0.0f : v; });
-					ax = op32_any(P.bilap, (int)i, (int)j, P.w, P.h, [&](size_t tt) {
-						return isnan(g[tt]) ? u[tt] : 0.0f; });
+				for (int d = 0; d < 4; d++) {
+					const bool kn = !isnan(gv[d]);
+					r0 = r0 + (kn ? gv[d] : 0.0f);
+					rx = rx + (kn ? 0.0f : uv[d]);
 				}
-				const float bv = (P.f ? -P.f[q] : 0.0f) - a0;
-				const float r = bv - ax;
+				const float bv = (P.f ? -fv[k] : 0.0f) - -r0;
+				const float r = bv - -rx;
 				rr += (double)r * r;
 				bb += (double)bv * bv;
-				if (whole) { rv[k] = r; xv[k] = (double)u[q]; }
-				else { P.r[q] = r; P.p[q] = r; P.q[q] = 0.0f; P.x[q] = (double)u[q]; }
-			} else if (!whole) {
-				P.r[q] = 0.0f; P.p[q] = 0.0f; P.q[q] = 0.0f; P.x[q] = 0.0;
+				rv[k] = r;
+				xv[k] = (double)uc[k];
 			}
-			if (++i == w) { i = 0; j++; }
+		}
+	} else {
+		cc = q4 < n ? group_code(P, q4, q1) : 0u;
+		if (!whole || (cc & 0x01010101u)) {
+			// coordinates of the group's first pixel; the others follow along
+			// the row (a group may wrap onto the next row)
+			unsigned j = q4 / w, i = q4 - j * w;
+			for (unsigned q = q4; q < q1; q++) {
+				const unsigned k = q - q4;
+				const bool hole = (cc >> (8 * k)) & 1u;
+				if (!whole) P.code[q] = (unsigned char)(cc >> (8 * k));
+				if (hole) {
+					float a0, ax;
+					if (!P.bilap) {
+						// op32's clamped 5-point sum in its term order, both fields
+						// (known values, unknowns) from one set of loads
+						const unsigned t[4] = {i + 1 < w ? q + 1 : q,
+							j + 1 < (unsigned)P.h ? q + w : q, i > 0 ? q - 1 : q,
+							j > 0 ? q - w : q};
+						float gv[4], uv[4];
+#pragma unroll
+						for (int d = 0; d < 4; d++) { gv[d] = g[t[d]]; uv[d] = u[t[d]]; }
+						float r0 = -4.0f * 0.0f, rx = -4.0f * u[q];
+#pragma unroll
+						for (int d = 0; d < 4; d++) {
+							const bool kn = !isnan(gv[d]);
+							r0 = r0 + (kn ? gv[d] : 0.0f);
+							rx = rx + (kn ? 0.0f : uv[d]);
+						}
+						a0 = -r0;
+						ax = -rx;
+					} else {
+						a0 = op32_any(P.bilap, (int)i, (int)j, P.w, P.h, [&](size_t tt) {
+							const float v = g[tt]; return isnan(v) ? 0.0f : v; });
+						ax = op32_any(P.bilap, (int)i, (int)j, P.w, P.h, [&](size_t tt) {
+							return isnan(g[tt]) ? u[tt] : 0.0f; });
+					}
+					const float bv = (P.f ? -P.f[q] : 0.0f) - a0;
+					const float r = bv - ax;
+					rr += (double)r * r;
+					bb += (double)bv * bv;
+					if (whole) { rv[k] = r; xv[k] = (double)u[q]; }
+					else { P.r[q] = r; P.p[q] = r; P.q[q] = 0.0f; P.x[q] = (double)u[q]; }
+				} else if (!whole) {
+					P.r[q] = 0.0f; P.p[q] = 0.0f; P.q[q] = 0.0f; P.x[q] = 0.0;
+				}
+				if (++i == w) { i = 0; j++; }
+			}
 		}
 	}
 	if (whole) {
@@ -314,13 +336,13 @@ __global__ __launch_bounds__(NT) void k32_setup(const BlockWork *bw,
 		// biharmonic iteration starts from p = r, and its product reads p
 		// at every pixel of the 13-point stencil, unknown or not: p and q
 		// everywhere.  (The Poisson iteration writes p and q itself.)
-		*(unsigned *)(P.code + q4) = cc;
-		*(f4 *)(P.r + q4) = rv;
+		gstore<unsigned>(P.code + q4, cc);
+		gstore<f4>(P.r + q4, rv);
 		if (P.bilap) {
-			*(f4 *)(P.p + q4) = rv;
-			*(f4 *)(P.q + q4) = f4{0.0f, 0.0f, 0.0f, 0.0f};
+			gstore<f4>(P.p + q4, rv);
+			gstore<f4>(P.q + q4, f4{0.0f, 0.0f, 0.0f, 0.0f});
 		}
-		if (cc & 0x01010101u) *(d4 *)(P.x + q4) = xv;
+		if (cc & 0x01010101u) gstore<d4>(P.x + q4, xv);
 	}
 	const double s1 = bsum<NT>(rr, red), s2 = bsum<NT>(bb, red);
 	if (threadIdx.x == 0) {
@@ -952,17 +974,20 @@ __global__ __launch_bounds__(NT) void k_mg_coarsen0(const BlockWork *bw,
 	if (!(cw & 3) && F.w == 2 * C.w) {
 		// four cells of one row: 8 fine columns of fine rows 2J - 1 .. 2J + 1
 		// as one 8-byte load each (the fine width is a multiple of 8)
+		// (every load at once, rows and columns outside the level clamped to
+		// the group's own and their values dropped by selects)
 		const unsigned char *r0 = F.code + (size_t)(2 * J) * F.w + 2 * I;
-		const uint2 a = *(const uint2 *)r0;
-		const uint2 c = 2 * J + 1 < F.h ? *(const uint2 *)(r0 + F.w) : make_uint2(0u, 0u);
-		const uint2 u = J > 0 ? *(const uint2 *)(r0 - F.w) : make_uint2(0u, 0u);
-		auto byte = [](uint2 v, int k) { return ((k < 4 ? v.x : v.y) >> (8 * (k & 3))) & 255u; };
+		const bool hs = 2 * J + 1 < F.h, hn = J > 0, hw = I > 0;
+		const gu2 a = gload<gu2>(r0);
+		const gu2 cl = gload<gu2>(hs ? r0 + F.w : r0), ul = gload<gu2>(hn ? r0 - F.w : r0);
+		const unsigned w0 = gload<unsigned char>(hw ? r0 - 1 : r0);
+		const unsigned w1 = gload<unsigned char>(hw && hs ? r0 + F.w - 1 : r0);
+		const gu2 z2 = {0u, 0u};
+		const gu2 c = hs ? cl : z2, u = hn ? ul : z2;
+		auto byte = [](gu2 v, int k) { return ((k < 4 ? v.x : v.y) >> (8 * (k & 3))) & 255u; };
 		// east coupling of the west neighbour (I - 1, J): fine column 2I - 1
-		unsigned we = 0;
-		if (I > 0) {
-			const unsigned w0 = r0[-1], w1 = 2 * J + 1 < F.h ? r0[F.w - 1] : 0u;
-			we = ((w0 >> 1) & 1u) + ((w1 >> 1) & 1u);
-		}
+		const unsigned we0 = hw ? ((w0 >> 1) & 1u) + (hs ? (w1 >> 1) & 1u : 0u) : 0u;
+		unsigned we = we0;
 		float4 D, E, S;
 		uint4 G;
 #pragma unroll
@@ -979,11 +1004,11 @@ __global__ __launch_bounds__(NT) void k_mg_coarsen0(const BlockWork *bw,
 			(&G.x)[k] = g;
 			we = e;
 		}
-		*(float4 *)(C.D + q4) = D;
-		*(float4 *)(C.wE + q4) = E;
-		*(float4 *)(C.wS + q4) = S;
-		*(float4 *)(C.z + q4) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-		*(uint4 *)(C.gcode + q4) = G;
+		gstore<f4>(C.D + q4, f4{D.x, D.y, D.z, D.w});
+		gstore<f4>(C.wE + q4, f4{E.x, E.y, E.z, E.w});
+		gstore<f4>(C.wS + q4, f4{S.x, S.y, S.z, S.w});
+		gstore<f4>(C.z + q4, f4{0.0f, 0.0f, 0.0f, 0.0f});
+		gstore<gu4>(C.gcode + q4, gu4{G.x, G.y, G.z, G.w});
 		return;
 	}
 	const unsigned q1 = q4 + 4 < n ? q4 + 4 : n;
