@@ -1401,7 +1401,8 @@ static __device__ __forceinline__ void z_frame(float *z)
 // are one aligned pair (region origins are even), loaded together.
 template <bool LOADZ, class P, class G>
 static __device__ __forceinline__ void mg_load(const MgLevel &L, int ox, int oy,
-		MgCells<P, G> &C, float *z, const float *zsrc)
+		MgCells<P, G> &C, float *z, const float *zsrc, const float *ce = nullptr,
+		const bool *cin = nullptr, float wc = 0.0f)
 {
 	C.pc = threadIdx.x % G::PAIRS_X;
 	C.m0 = threadIdx.x / G::PAIRS_X;
@@ -1418,10 +1419,11 @@ static __device__ __forceinline__ void mg_load(const MgLevel &L, int ox, int oy,
 #pragma unroll
 			for (int hr = 0; hr < 2; hr++) {
 				const int j = clampi(S.j0 + 2 * kb + hr, 0, L.h - 1);
-				const size_t q = (size_t)j * L.w + ic;
-				br[kb][hr] = gload<gf2>(L.b + q);
-				kr[kb][hr] = gload<typename P::Raw2>(code + q);
-				if (LOADZ) zr[kb][hr] = gload<gf2>(zsrc + q);
+				const unsigned q = (unsigned)j * (unsigned)L.w + (unsigned)ic;
+				br[kb][hr] = gload_b<gf2>(L.b, 4 * q);
+				kr[kb][hr] = gload_b<typename P::Raw2>(code,
+						q * (unsigned)sizeof(typename P::Raw));
+				if (LOADZ) zr[kb][hr] = gload_b<gf2>(zsrc, 4 * q);
 			}
 #pragma unroll
 		for (int kb = 0; kb < G::K; kb++)
@@ -1446,10 +1448,11 @@ static __device__ __forceinline__ void mg_load(const MgLevel &L, int ox, int oy,
 			for (int c = 0; c < 4; c++) {
 				const int j = clampi(S.j0 + 2 * kb + (c >> 1), 0, L.h - 1);
 				const int i = clampi(S.i0 + (c & 1), 0, L.w - 1);
-				const size_t q = (size_t)j * L.w + i;
-				br[kb][c] = gload<float>(L.b + q);
-				kr[kb][c] = gload<typename P::Raw>(code + q);
-				if (LOADZ) zr[kb][c] = gload<float>(zsrc + q);
+				const unsigned q = (unsigned)j * (unsigned)L.w + (unsigned)i;
+				br[kb][c] = gload_b<float>(L.b, 4 * q);
+				kr[kb][c] = gload_b<typename P::Raw>(code,
+						q * (unsigned)sizeof(typename P::Raw));
+				if (LOADZ) zr[kb][c] = gload_b<float>(zsrc, 4 * q);
 			}
 #pragma unroll
 		for (int kb = 0; kb < G::K; kb++)
@@ -1465,13 +1468,17 @@ static __device__ __forceinline__ void mg_load(const MgLevel &L, int ox, int oy,
 	for (int kb = 0; kb < G::K; kb++)
 #pragma unroll
 		for (int c = 0; c < 4; c++) {
+			// with ce: a block (one coarse cell) inside the coarse level
+			// adds its coarse correction wc * ce at its unknowns
+			const float zc = ce && cin[kb] ? zv[kb][c] + wc * ce[kb] * P::mask(kv[kb][c])
+				: zv[kb][c];
 			C.b[kb][c] = bv[kb][c];
 			P::set(C.k[kb], c, kv[kb][c]);
 			if constexpr (MgCells<P, G>::RD) {
 				C.rd[kb][c] = P::rdiag(kv[kb][c]);
-				C.zr[kb][c] = zv[kb][c];
+				C.zr[kb][c] = zc;
 			}
-			z[cell_z(C, kb, c)] = zv[kb][c];
+			z[cell_z(C, kb, c)] = zc;
 		}
 }
 
@@ -1484,13 +1491,16 @@ static __device__ __forceinline__ void mg_store_z(float *dst, const MgLevel &L,
 #pragma unroll
 	for (int hr = 0; hr < 2; hr++) {
 		if (!row_ok<G>(S, kb, hr)) continue;
+		// pixel offsets of the pair's two cells (each stored only inside
+		// the level, so each is >= 0 when used)
 		const long q = row_q<G>(S, kb, hr);
+		const unsigned o0 = 4u * (unsigned)q, o1 = 4u * (unsigned)(q + 1);
 		const float z0 = z[cell_z(C, kb, 2 * hr)], z1 = z[cell_z(C, kb, 2 * hr + 1)];
 		if (!(L.w & 1)) {
-			if (S.colok[0]) gstore<gf2>(dst + q, gf2{z0, z1});
+			if (S.colok[0]) gstore_b<gf2>(dst, o0, gf2{z0, z1});
 		} else {
-			if (S.colok[0]) gstore<float>(dst + q, z0);
-			if (S.colok[1]) gstore<float>(dst + q + 1, z1);
+			if (S.colok[0]) gstore_b<float>(dst, o0, z0);
+			if (S.colok[1]) gstore_b<float>(dst, o1, z1);
 		}
 	}
 }
@@ -1648,19 +1658,7 @@ void k_mg_up(const int4 *tiles, const MgLevel *lv,
 	}
 	MgCells<P, G> C;
 	z_frame<G>(z);
-	mg_load<true>(L, ox, oy, C, z, L.zd);
-#pragma unroll
-	for (int kb = 0; kb < G::K; kb++) {
-		if (!cin[kb]) continue;
-		const float e = wc * ce[kb];
-#pragma unroll
-		for (int c = 0; c < 4; c++) {
-			const int id = cell_z(C, kb, c);
-			const float v = z[id] + e * P::mask(P::get(C.k[kb], c));
-			if constexpr (MgCells<P, G>::RD) C.zr[kb][c] = v;
-			z[id] = v;
-		}
-	}
+	mg_load<true>(L, ox, oy, C, z, L.zd, ce, cin, wc);
 	__syncthreads();
 	for (int s = 0; s < G::NU; s++) {
 		mg_half<1>(C, z, false);

@@ -49,6 +49,21 @@ static __device__ __forceinline__ void gstore(void *p, T v)
 {
 	*(__attribute__((address_space(1))) T *)p = v;
 }
+// the same at a 32-bit byte offset from a base held in scalar registers:
+// the loads' own address form, without 64-bit address arithmetic per lane
+// (a plane holds at most 2^30 pixels, so its byte offsets fit)
+template <class T>
+static __device__ __forceinline__ T gload_b(const void *base, unsigned off)
+{
+	return *(const __attribute__((address_space(1))) T *)(
+			(const __attribute__((address_space(1))) char *)base + off);
+}
+template <class T>
+static __device__ __forceinline__ void gstore_b(void *base, unsigned off, T v)
+{
+	*(__attribute__((address_space(1))) T *)((__attribute__((address_space(1))) char *)base
+			+ off) = v;
+}
 
 // tile_load's footprint copy with a thread's loads issued in batches of up
 // to 8 cells before the batch's first LDS write: one at a time, each LDS
