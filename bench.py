@@ -46,9 +46,9 @@ PEAK_HBM_GBPS = 8000.0
 TRAFFIC_PMC = {
     "mg_up0": 634.1e6,
     "mg_down0": 495.3e6,
-    "k32_dirspmv": 861.1e6,
+    "k32_dirspmv": 861.3e6,
 }
-TRAFFIC_SRC = "4d2548068a087621"
+TRAFFIC_SRC = "a6356c2e85292914"
 
 
 def kernel_source_id(name="sp_cg.hip"):
