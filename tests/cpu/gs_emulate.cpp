@@ -194,11 +194,16 @@ int main(int argc, char **argv)
 	float *g = new float[n], *f = withf ? new float[n] : nullptr;
 	float *u = new float[n], *uref = new float[n], *c = new float[n];
 	int holes = 0;
+	// seeds from 100: a few known pixels of +-Inf (neighbours the update
+	// skips) or +-3e37 (no overflow at t <= 0.25)
+	const bool wild = atoi(argv[5]) >= 100;
 	for (size_t q = 0; q < n; q++) {
 		g[q] = (float)(urand() * 100);
+		if (wild && urand() < 0.0002)
+			g[q] = (urand() < 0.5 ? INFINITY : 3e37f) * (urand() < 0.5 ? 1 : -1);
 		if (urand() < frac) { g[q] = NAN; holes++; }
 		if (f) f[q] = (float)(urand() * 4 - 2);
-		u[q] = isfinite(g[q]) ? g[q] : (float)(urand() * 100);
+		u[q] = !isnan(g[q]) ? g[q] : (float)(urand() * 100);
 		uref[q] = u[q];
 	}
 	for (size_t q = 0; q < n; q++) c[q] = sp_coef(g[q], f ? f[q] : 0.0f);

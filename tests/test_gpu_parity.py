@@ -137,6 +137,24 @@ def test_random_against_oracle(args):
             close_cg(out[l], ref[l], g[l])
 
 
+@pytest.mark.parametrize("seed,withf", [(41, True), (42, False)])
+def test_gs_tiles_with_infinite_and_huge_known_values(seed, withf):
+    """Known pixels of +-Inf (neighbours the update skips) and +-3e37 among
+    ordinary values: the tiled sweeps stay bit-identical to the oracle.
+    Single level, t = 0.25 (no value can overflow, so no NaN is made)."""
+    m = _imscript()
+    g, f = random_case(seed, 1, 600, 520, 0.4, withf)
+    rng = np.random.default_rng(seed + 1000)
+    known = np.flatnonzero(~np.isnan(g))
+    wild = rng.choice(known, 40, replace=False)
+    vals = np.array([np.inf, -np.inf, 3e37, -3e37], np.float32)
+    g.reshape(-1)[wild] = vals[rng.integers(0, 4, wild.size)]
+    print(f"wild known values seed={seed} 1x600x520 t=0.25 n=40 s=1", flush=True)
+    ref, _ = oracle_solve(g, f, 0.25, 40, 1, 0)
+    out = m.simpois(g, f, tstep=0.25, niter=40, scale=1, cgit=0, trace=False)
+    assert bits_equal(out, ref)
+
+
 def test_device_api_matches_host_api_and_is_deterministic():
     import torch
     m = _imscript()

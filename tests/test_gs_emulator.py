@@ -56,6 +56,10 @@ CASES = [
     ("70 45 17 -0.08 4 0.3 1 small"),
     ("50 37 6 -1000.01 5 0.3 1 small"),
     ("2 2 100 -0.08 3 0.5 1 small"),
+    # seeds from 100: a few infinite and 3e37 known values
+    ("300 260 40 0.25 105 0.4 1 tiles"),
+    ("260 250 70 0.25 107 0.5 0 tiles"),
+    ("60 50 30 0.25 108 0.4 1 small"),
 ]
 
 
