@@ -438,3 +438,173 @@ SP_HD long small_nsteps(int R, int W, int H, int T)
 {
 	return (long)(W - 1) + (long)R * (H - 1) + (long)(R * R + 1) * (T - 1) + 1;
 }
+
+// ------------------------------- whole R = 1 level by its unknown pixels
+//
+// The hyperplane schedule of k_gs_small with the work dealt by pixel: the
+// updates on hyperplane s = i + j + 2k all have i + j = s (mod 2), so a
+// level's unknown pixels are listed once per parity (raster order) and
+// dealt round-robin to the threads, and on step s a thread updates its
+// pixels of parity s & 1 whose sweep k = (s - i - j) / 2 lies in [0, T).
+// Known pixels take no thread.  A thread keeps its pixels' values and
+// coefficients in registers (no other thread writes them) and reads the
+// four neighbours from the level's values in LDS, inside a frame of NaN
+// (skipped as non-finite, as in the tiles).  Same updates, same order of
+// terms, same results as the raster sweeps.
+
+constexpr int GSC_MAXCELLS = 16384;   // pixels of a level: <= 8192 per parity
+constexpr int GSC_MAXNT = 1024;
+
+// LDS bytes: the framed values, the list of unknowns and 16 scan words
+SP_HD long gsc_lds_bytes(int W, int H)
+{
+	return 4L * ((long)(W + 2) * (H + 2) + (long)W * H + 16);
+}
+
+// threads for a level: one per pixel of a parity (64..1024)
+SP_HD int gsc_threads(int W, int H)
+{
+	const int half = (W * H + 1) / 2, th = (half + 63) / 64 * 64;
+	return th < 64 ? 64 : (th > GSC_MAXNT ? GSC_MAXNT : th);
+}
+
+// pixels per thread and parity the level can need with nt threads
+SP_HD int gsc_slots(int W, int H, int nt)
+{
+	return ((W * H + 1) / 2 + nt - 1) / nt;
+}
+
+template <int MAXS> struct GscThread {
+	int n[2];                 // pixels of each parity
+	// framed LDS index q = (j + 1) (W + 2) + i + 1 | (i + j) << 16 (both
+	// below 2^16 for GSC_MAXCELLS pixels)
+	unsigned qij[2][MAXS];
+	float x[2][MAXS], c[2][MAXS];
+};
+
+// phase 1 (thread tid of nt): the framed values into LDS
+SP_HD void gsc_load(float *U, const GsInst &in, int tid, int nt)
+{
+	const int W = in.w, H = in.h, P = W + 2, n = P * (H + 2);
+	for (int e = tid; e < n; e += nt) {
+		const int r = e / P, col = e - r * P, i = col - 1, j = r - 1;
+		U[e] = i >= 0 && i < W && j >= 0 && j < H ? in.u[j * W + i]
+			: sp_float(SP_QNAN_BITS);
+	}
+}
+
+// the raster chunk [q0, q1) of pixels a thread lists
+SP_HD void gsc_chunk(int W, int H, int tid, int nt, int &q0, int &q1)
+{
+	const int n = W * H, ch = (n + nt - 1) / nt;
+	q0 = tid * ch < n ? tid * ch : n;
+	q1 = q0 + ch < n ? q0 + ch : n;
+}
+
+// phase 2: the chunk's unknowns by parity, packed as even | odd << 16
+SP_HD int gsc_count(const GsInst &in, int tid, int nt)
+{
+	int q0, q1;
+	gsc_chunk(in.w, in.h, tid, nt, q0, q1);
+	int j = q0 / in.w, i = q0 - j * in.w, ce = 0, co = 0;
+	for (int q = q0; q < q1; q++) {
+		if (!sp_known(in.c[q])) {
+			if ((i + j) & 1) co++;
+			else ce++;
+		}
+		if (++i == in.w) { i = 0; j++; }
+	}
+	return ce | co << 16;
+}
+
+// phase 3: the chunk's unknowns into the list, the even parity's first;
+// pre = exclusive prefix of the packed counts over the threads, tot = sum
+SP_HD void gsc_fill(int *L, const GsInst &in, int tid, int nt, int pre, int tot)
+{
+	int q0, q1;
+	gsc_chunk(in.w, in.h, tid, nt, q0, q1);
+	const int P = in.w + 2;
+	int oe = pre & 0xffff, oo = (tot & 0xffff) + (pre >> 16);
+	int j = q0 / in.w, i = q0 - j * in.w;
+	for (int q = q0; q < q1; q++) {
+		if (!sp_known(in.c[q])) L[(i + j) & 1 ? oo++ : oe++] = (j + 1) * P + i + 1;
+		if (++i == in.w) { i = 0; j++; }
+	}
+}
+
+// phase 4: the thread's pixels, entries tid, tid + nt, ... of each list
+template <int MAXS>
+SP_HD void gsc_take(GscThread<MAXS> &t, const float *U, const int *L,
+		const GsInst &in, int tid, int nt, int tot)
+{
+	const int W = in.w, P = W + 2;
+	const int nn[2] = {tot & 0xffff, tot >> 16};
+#pragma unroll
+	for (int p = 0; p < 2; p++) {
+		t.n[p] = 0;
+#pragma unroll
+		for (int m = 0; m < MAXS; m++) {
+			const int r = tid + m * nt;
+			const bool on = r < nn[p];
+			const int q = on ? L[(p ? nn[0] : 0) + r] : P + 1;   // unused: pixel 0
+			const int j = q / P - 1, i = q - (j + 1) * P - 1;
+			t.qij[p][m] = (unsigned)q | (unsigned)(i + j) << 16;
+			t.x[p][m] = U[q];
+			t.c[p][m] = in.c[j * W + i];
+			t.n[p] += on;
+		}
+	}
+}
+
+// the update of gs_update_tile1 for one listed pixel (value x, coefficient
+// cv, never known), same terms in the same order
+SP_HD float gsc_update(const float *U, int q, int P, float x, float cv, float t)
+{
+	const float e = U[q + 1], so = U[q + P], w = U[q - 1], n = U[q - P];
+	float r = 0.0f;
+	if (isfinite(e)) r = r + (e - x);
+	if (isfinite(so)) r = r + (so - x);
+	if (isfinite(w)) r = r + (w - x);
+	if (isfinite(n)) r = r + (n - x);
+	return x + t * (r - cv);
+}
+
+template <int MAXS, int PAR>
+SP_HD void gsc_step_par(GscThread<MAXS> &th, float *U, int P, int s, int T2, float t)
+{
+#pragma unroll
+	for (int m = 0; m < MAXS; m++) {
+		if (m >= th.n[PAR]) break;
+		const int d = s - (int)(th.qij[PAR][m] >> 16);
+		if ((unsigned)d >= (unsigned)T2) continue;
+		const int q = (int)(th.qij[PAR][m] & 0xffffu);
+		const float y = gsc_update(U, q, P, th.x[PAR][m], th.c[PAR][m], t);
+		U[q] = y;
+		th.x[PAR][m] = y;
+	}
+}
+
+// phase 5, step s of small_nsteps(1, W, H, T): steps run in increasing s
+// with a barrier between them
+template <int MAXS>
+SP_HD void gsc_step(GscThread<MAXS> &th, float *U, int W, int T, float t, int s)
+{
+	if (s & 1) gsc_step_par<MAXS, 1>(th, U, W + 2, s, 2 * T, t);
+	else gsc_step_par<MAXS, 0>(th, U, W + 2, s, 2 * T, t);
+}
+
+// phase 6: the thread's pixels back to the image
+template <int MAXS>
+SP_HD void gsc_store(const GscThread<MAXS> &th, const GsInst &in)
+{
+	const int P = in.w + 2;
+#pragma unroll
+	for (int p = 0; p < 2; p++)
+#pragma unroll
+		for (int m = 0; m < MAXS; m++) {
+			if (m >= th.n[p]) break;
+			const int q = (int)(th.qij[p][m] & 0xffffu);
+			const int j = q / P - 1, i = q - (j + 1) * P - 1;
+			in.u[j * in.w + i] = th.x[p][m];
+		}
+}

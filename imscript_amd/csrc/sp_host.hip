@@ -92,6 +92,15 @@ extern "C" void sp_params_default(sp_params *p)
 // ------------------------------------------------------------- device
 
 static int g_dev_checked = 0, g_dev_ok = 0, g_max_lds = 65536;
+// LDS the whole-level R = 1 kernel by unknown pixel may use (0: path off);
+// SIMPOIS_HIP_GS_CELLS=0 turns it off (such levels then go to k_gs_small
+// or to tiles)
+static int g_gsc_lds = 0;
+static constexpr int GSC_LDS_MAX = 155000;
+static const int g_gsc_off = [] {
+	const char *e = getenv("SIMPOIS_HIP_GS_CELLS");
+	return e && !strcmp(e, "0") ? 1 : 0;
+}();
 // SIMPOIS_HIP_TAIL_LDS=0 keeps every multigrid tail in global memory
 static const int g_tail_lds_off = [] {
 	const char *e = getenv("SIMPOIS_HIP_TAIL_LDS");
@@ -169,6 +178,9 @@ static int ensure_device()
 	if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)
 			== hipSuccess && lds > 0)
 		g_max_lds = std::min(lds, 65536);
+	if (!g_gsc_off)
+		g_gsc_lds = spk::gs_cells_enable(GSC_LDS_MAX) == 0 ? GSC_LDS_MAX : g_max_lds;
+	(void)hipGetLastError();
 	g_dev_ok = 1;
 	return SP_OK;
 }
@@ -331,6 +343,7 @@ struct LevelPlan {
 	long tile_pass_px = 0;           // for the profiler
 	int *d_small = nullptr;
 	int nsmall = 0, small_lds = 0, small_threads = 64;
+	int cells_slots = 0;             // slots per thread of k_gs_cells (0: k_gs_small)
 	bool any_cg = false;
 	// FP32 CG of this level, built on first use and kept with the plan
 	struct {
@@ -747,9 +760,31 @@ static int plan_create(Plan &pl, const std::vector<PlaneIn> &planes,
 				pl.teff, nullptr};
 		TRY(pl.upload(&L.d_insts, gi));
 		L.insts = gi;
-		// Gauss-Seidel: members that fit in LDS run whole; others in tiles
+		// Gauss-Seidel: members that fit in LDS run whole; others in tiles.
+		// For R = 1 the whole-level kernel deals the unknown pixels to the
+		// threads (k_gs_cells), for levels of up to GSC_MAXCELLS pixels
 		std::vector<int> small;
-		for (int m = 0; m < nm; m++) {
+		const bool cells = pl.R == 1 && g_gsc_lds > 0;
+		if (cells) {
+			int th = 64, maxs = 1, lds = 0;
+			for (int m = 0; m < nm; m++) {
+				const int w = L.sp[m].w, h = L.sp[m].h, T = L.nsweep[m];
+				if (T <= 0) continue;
+				if ((long)w * h <= GSC_MAXCELLS && gsc_lds_bytes(w, h) <= g_gsc_lds
+						&& small_nsteps(1, w, h, T) < 0x7fffffffL) {
+					small.push_back(m);
+					th = std::max(th, gsc_threads(w, h));
+					lds = std::max<int>(lds, (int)gsc_lds_bytes(w, h));
+					continue;
+				}
+				L.tiled.push_back(m);
+			}
+			for (int m : small) maxs = std::max(maxs, gsc_slots(L.sp[m].w, L.sp[m].h, th));
+			L.small_threads = th;
+			L.small_lds = lds;
+			L.cells_slots = maxs;
+		}
+		for (int m = 0; m < nm && !cells; m++) {
 			const int w = L.sp[m].w, h = L.sp[m].h, T = L.nsweep[m];
 			if (T <= 0) continue;
 			// whole level in one work-group: it fits in LDS, every thread
@@ -802,7 +837,10 @@ static int plan_create(Plan &pl, const std::vector<PlaneIn> &planes,
 static int run_gs(Plan &pl, LevelPlan &L, hipStream_t s)
 {
 	if (pl.P.autotune && !L.gs_tuned && L.wf_off.size() > 1) TRY(gs_tune(pl, L, s));
-	if (L.nsmall)
+	if (L.nsmall && L.cells_slots)
+		spk::launch_gs_cells(L.cells_slots, L.d_insts, L.d_small, L.nsmall, L.small_lds,
+				L.small_threads, s);
+	else if (L.nsmall)
 		spk::launch_gs_small(pl.R, L.d_insts, L.d_small, L.nsmall, L.small_lds,
 				L.small_threads, s);
 	if (L.wf_off.size() > 1) {

@@ -168,6 +168,11 @@ void launch_stream_copy(float *dst, const float *src, size_t n4, hipStream_t s);
 // tiles of shape variant `var` of radius R (sp_gs.h); one launch per wavefront
 int tile_lds_bytes(int R, int var);
 int small_lds_bytes(int R, int w, int h);
+// whole R = 1 levels by unknown pixel (k_gs_cells): allow `bytes` of LDS
+// (0 on success), launch one work-group of `threads` per listed member
+int gs_cells_enable(int bytes);
+void launch_gs_cells(int maxs, const GsInst *insts, const int *ids, int nids,
+		int lds_bytes, int threads, hipStream_t s);
 void launch_gs_tiles(int R, int var, const GsTile *tiles, int ntiles,
 		const GsInst *insts, hipStream_t s);
 void launch_gs_small(int R, const GsInst *insts, const int *ids, int nids,

@@ -353,6 +353,93 @@ __global__ __launch_bounds__(1024) void k_gs_small(const GsInst *insts, const in
 
 int small_lds_bytes(int R, int w, int h) { return 8 * small_cells(R, w, h); }
 
+// exclusive prefix sum of v over the work-group (blockDim.x a multiple of
+// 64, at most 1024) and the sum into *tot; wsum: 16 words of LDS
+static __device__ int block_exscan(int v, int *wsum, int *tot)
+{
+	const int lane = (int)threadIdx.x & 63, wid = (int)threadIdx.x >> 6;
+	const int nw = (int)blockDim.x >> 6;
+	int x = v;
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1) {
+		const int y = __shfl_up(x, o, 64);
+		if (lane >= o) x += y;
+	}
+	if (lane == 63) wsum[wid] = x;
+	__syncthreads();
+	int pre = 0, sum = 0;
+	for (int k = 0; k < nw; k++) {
+		const int w = wsum[k];
+		if (k < wid) pre += w;
+		sum += w;
+	}
+	*tot = sum;
+	return pre + x - v;
+}
+
+// whole R = 1 level by its unknown pixels (sp_gs.h gsc_*), one plane per
+// work-group
+template <int MAXS>
+__global__ __launch_bounds__(GSC_MAXNT) void k_gs_cells(const GsInst *insts, const int *ids)
+{
+	extern __shared__ float smem[];
+	const GsInst in = insts[ids[blockIdx.x]];
+	if (in.active && !*in.active) return;
+	const int W = in.w, H = in.h, T = in.nsweep;
+	if (T <= 0) return;
+	const int nt = (int)blockDim.x, tid = (int)threadIdx.x;
+	float *U = smem;
+	int *L = (int *)(smem + (W + 2) * (H + 2));
+	int *wsum = L + W * H;
+	gsc_load(U, in, tid, nt);
+	int tot;
+	const int pre = block_exscan(gsc_count(in, tid, nt), wsum, &tot);
+	gsc_fill(L, in, tid, nt, pre, tot);
+	__syncthreads();
+	GscThread<MAXS> th;
+	gsc_take<MAXS>(th, U, L, in, tid, nt, tot);
+	// even and odd steps in turn, each with its parity known at compile
+	// time (a step of either parity behind one branch made the compiler
+	// index the values by parity, through scratch memory)
+	const int ns = (int)small_nsteps(1, W, H, T);
+	for (int s = 0; s < ns; s += 2) {
+		gsc_step_par<MAXS, 0>(th, U, W + 2, s, 2 * T, in.t);
+		__syncthreads();
+		if (s + 1 < ns) {
+			gsc_step_par<MAXS, 1>(th, U, W + 2, s + 1, 2 * T, in.t);
+			__syncthreads();
+		}
+	}
+	gsc_store<MAXS>(th, in);
+}
+
+// slots per thread and parity of the compiled instances
+static int gsc_inst(int maxs) { return maxs <= 1 ? 1 : maxs <= 2 ? 2 : maxs <= 4 ? 4 : 8; }
+
+int gs_cells_enable(int bytes)
+{
+	for (int m : {1, 2, 4, 8}) {
+		const void *f = m == 1 ? (const void *)k_gs_cells<1> : m == 2 ? (const void *)k_gs_cells<2>
+			: m == 4 ? (const void *)k_gs_cells<4> : (const void *)k_gs_cells<8>;
+		if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes)
+				!= hipSuccess)
+			return -1;
+	}
+	return 0;
+}
+
+void launch_gs_cells(int maxs, const GsInst *insts, const int *ids, int nids,
+		int lds_bytes, int threads, hipStream_t s)
+{
+	if (!nids) return;
+	switch (gsc_inst(maxs)) {
+	case 1: k_gs_cells<1><<<nids, threads, lds_bytes, s>>>(insts, ids); break;
+	case 2: k_gs_cells<2><<<nids, threads, lds_bytes, s>>>(insts, ids); break;
+	case 4: k_gs_cells<4><<<nids, threads, lds_bytes, s>>>(insts, ids); break;
+	default: k_gs_cells<8><<<nids, threads, lds_bytes, s>>>(insts, ids); break;
+	}
+}
+
 void launch_gs_small(int R, const GsInst *insts, const int *ids, int nids,
 		int lds_bytes, int threads, hipStream_t s)
 {

@@ -10,7 +10,7 @@
 // of bounds aborts the run.
 //
 // usage: gs_emulate W H T tstep seed holefrac withf mode [variant]
-//   mode: tiles | small; variant: tile shape variant of the radius (sp_gs.h)
+//   mode: tiles | small | cells; variant: tile shape variant of the radius (sp_gs.h)
 //        gs_emulate amle W H T nn seed holefrac [variant]
 //   the AMLE sweeps (sp_amle.h) against the oracle's raster-order sweeps
 //   (oracle/amle_oracle.c); variant: AMLE tile shape of radius amle_radius(nn)
@@ -111,6 +111,45 @@ static int run_small(float *u, const float *c, int W, int H, int T, float t)
 	return 0;
 }
 
+// the whole-level kernel by unknown pixels (R = 1): phases thread by
+// thread, the exclusive scan of the packed counts done here as the device
+// does it with a work-group scan
+template <int MAXS>
+static int run_cells(float *u, const float *c, int W, int H, int T, float t)
+{
+	if ((long)W * H > GSC_MAXCELLS) {
+		printf("TOO MANY CELLS %ld\n", (long)W * H);
+		return 1;
+	}
+	const int nt = gsc_threads(W, H);
+	if (gsc_slots(W, H, nt) > MAXS) {
+		printf("TOO MANY SLOTS %d\n", gsc_slots(W, H, nt));
+		return 1;
+	}
+	const long words = gsc_lds_bytes(W, H) / 4;
+	std::vector<float> lds(words, -12345.0f);
+	float *U = lds.data();
+	int *L = (int *)(U + (W + 2) * (H + 2));
+	for (long e = (W + 2) * (H + 2); e < words; e++) L[e - (W + 2) * (H + 2)] = -1;
+	GsInst in{u, c, W, H, T, t, nullptr};
+	for (int tid = 0; tid < nt; tid++) gsc_load(U, in, tid, nt);
+	std::vector<int> cnt(nt), pre(nt);
+	int tot = 0;
+	for (int tid = 0; tid < nt; tid++) {
+		cnt[tid] = gsc_count(in, tid, nt);
+		pre[tid] = tot;
+		tot += cnt[tid];
+	}
+	for (int tid = 0; tid < nt; tid++) gsc_fill(L, in, tid, nt, pre[tid], tot);
+	std::vector<GscThread<MAXS>> th(nt);
+	for (int tid = 0; tid < nt; tid++) gsc_take<MAXS>(th[tid], U, L, in, tid, nt, tot);
+	const long ns = small_nsteps(1, W, H, T);
+	for (long s = 0; s < ns; s++)
+		for (int tid = 0; tid < nt; tid++) gsc_step<MAXS>(th[tid], U, W, T, t, (int)s);
+	for (int tid = 0; tid < nt; tid++) gsc_store<MAXS>(th[tid], in);
+	return 0;
+}
+
 // AMLE: random values with holes, a random start at the holes (as the
 // bounds or an initial image would give), nn neighbours; seeds from 100
 // put +-Inf among the values, whose NaN offspring the scans must order as
@@ -180,7 +219,7 @@ int main(int argc, char **argv)
 	if (argc > 1 && !strcmp(argv[1], "amle")) return main_amle(argc, argv);
 	if (argc != 9 && argc != 10) {
 		fprintf(stderr, "usage: gs_emulate W H T tstep seed holefrac withf "
-				"tiles|small [variant]\n");
+				"tiles|small|cells [variant]\n");
 		return 2;
 	}
 	const int var = argc == 10 ? atoi(argv[9]) : 0;
@@ -189,7 +228,7 @@ int main(int argc, char **argv)
 	rng_state = (unsigned)atoi(argv[5]) * 2654435761u + 7;
 	const double frac = atof(argv[6]);
 	const bool withf = atoi(argv[7]) != 0;
-	const bool small = !strcmp(argv[8], "small");
+	const bool small = !strcmp(argv[8], "small"), cells = !strcmp(argv[8], "cells");
 	const size_t n = (size_t)W * H;
 	float *g = new float[n], *f = withf ? new float[n] : nullptr;
 	float *u = new float[n], *uref = new float[n], *c = new float[n];
@@ -215,7 +254,13 @@ int main(int argc, char **argv)
 		if (tstep < -1000) { R = 3; teff = tstep; teff += 1000; teff *= -1; }
 	}
 	int rc;
-	if (small) rc = R == 1 ? run_small<1>(u, c, W, H, T, teff)
+	if (cells) {
+		if (R != 1) {
+			printf("CELLS NEED R = 1\n");
+			return 2;
+		}
+		rc = run_cells<8>(u, c, W, H, T, teff);
+	} else if (small) rc = R == 1 ? run_small<1>(u, c, W, H, T, teff)
 		: R == 2 ? run_small<2>(u, c, W, H, T, teff) : run_small<3>(u, c, W, H, T, teff);
 	else if (!sp_with_shape(R, var, [&](auto sh) {
 			using S = decltype(sh);

@@ -155,6 +155,27 @@ def test_gs_tiles_with_infinite_and_huge_known_values(seed, withf):
     assert bits_equal(out, ref)
 
 
+@pytest.mark.parametrize("h,w,hole,n,seed", [
+    (128, 128, 0.99, 30, 51),    # 16384 pixels: 8 per thread and parity
+    (128, 128, 0.3, 200, 52),
+    (1, 16384, 0.5, 9, 53),
+    (163, 100, 0.5, 40, 54),
+    (3, 2, 0.5, 100, 55),
+])
+def test_whole_level_by_unknown_pixel_against_oracle(h, w, hole, n, seed):
+    """Single R = 1 levels of up to 16384 pixels run whole in one
+    work-group with the unknown pixels dealt to the threads: bit-identical
+    to the oracle's raster sweeps."""
+    m = _imscript()
+    g, f = random_case(seed, 1, h, w, 0.3, True)
+    rng = np.random.default_rng(seed)
+    g[rng.random(g.shape) < hole] = np.nan
+    print(f"whole level {h}x{w} holes {hole} n={n}", flush=True)
+    ref, _ = oracle_solve(g, f, 0.25, n, 1, 0)
+    out = m.simpois(g, f, tstep=0.25, niter=n, scale=1, cgit=0, trace=False)
+    assert bits_equal(out, ref)
+
+
 def test_device_api_matches_host_api_and_is_deterministic():
     import torch
     m = _imscript()

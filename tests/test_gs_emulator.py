@@ -108,6 +108,42 @@ SMALL_LIMIT_CASES = [
 ]
 
 
+# whole R = 1 level by unknown pixel (k_gs_cells): sizes up to its
+# 16384-pixel limit, all-hole planes (8 pixels per thread and parity),
+# single rows and columns, a 1x1 level, infinite and 3e37 known values
+CELLS_CASES = [
+    "40 30 5 0.25 1 0.4 0 cells",
+    "1 1 7 0.25 1 1 1 cells",
+    "2 3 100 0.25 4 0.5 1 cells",
+    "128 128 60 0.25 3 0.3 1 cells",
+    "128 128 30 0.25 8 1.0 1 cells",
+    "300 54 40 0.4 5 0.5 1 cells",
+    "16384 1 9 0.25 6 0.5 1 cells",
+    "1 16384 9 0.25 7 0.5 1 cells",
+    "61 67 45 0.25 108 0.4 1 cells",
+    "64 64 200 0.25 9 0.3 0 cells",
+]
+
+
+@pytest.mark.parametrize("args", CELLS_CASES)
+def test_whole_level_kernel_by_unknown_pixel(emulator, args):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    p = subprocess.run([emulator, *args.split()], capture_output=True,
+                       text=True, timeout=600, env=env)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert p.stdout.startswith("OK")
+
+
+def test_whole_level_kernel_by_unknown_pixel_limits(emulator):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    p = subprocess.run([emulator, *"129 128 5 0.25 1 0.4 1 cells".split()],
+                       capture_output=True, text=True, timeout=60, env=env)
+    assert p.returncode == 1 and "TOO MANY CELLS" in p.stdout
+    p = subprocess.run([emulator, *"40 30 5 -0.08 1 0.4 1 cells".split()],
+                       capture_output=True, text=True, timeout=60, env=env)
+    assert p.returncode == 2 and "CELLS NEED R = 1" in p.stdout
+
+
 @pytest.mark.parametrize("args", SMALL_LIMIT_CASES)
 def test_whole_level_kernel_limits(emulator, args):
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
