@@ -57,7 +57,7 @@ template <int R_, int BI_, int BJ_, int BT_> struct Shape {
 	X(1, 0, 32, 64, 16) X(1, 1, 32, 128, 16) X(1, 2, 64, 64, 16) \
 	X(1, 3, 16, 128, 32) X(1, 4, 32, 64, 8) X(1, 5, 64, 32, 16) \
 	X(1, 6, 32, 32, 16) X(1, 7, 16, 64, 16) X(1, 8, 32, 64, 32) \
-	X(1, 9, 64, 64, 8) \
+	X(1, 9, 64, 64, 8) X(1, 10, 16, 16, 32) X(1, 11, 16, 32, 32) \
 	X(2, 0, 32, 32, 8) X(2, 1, 32, 64, 8) X(2, 2, 32, 32, 16) \
 	X(2, 3, 64, 32, 8) \
 	X(3, 0, 32, 32, 4) X(3, 1, 32, 64, 4) X(3, 2, 32, 32, 8)

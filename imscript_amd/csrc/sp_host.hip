@@ -101,6 +101,19 @@ static const int g_gsc_off = [] {
 	const char *e = getenv("SIMPOIS_HIP_GS_CELLS");
 	return e && !strcmp(e, "0") ? 1 : 0;
 }();
+// Gauss-Seidel tile shape of a plan without the autotuner: variant 0 of
+// the radius, or SIMPOIS_HIP_GS_SHAPE=v (read when a plan is built) when v
+// is a compiled variant of it
+static int gs_default_var(int R)
+{
+	const char *e = getenv("SIMPOIS_HIP_GS_SHAPE");
+	if (!e || !*e) return 0;
+	char *end = nullptr;
+	const long v = strtol(e, &end, 10);
+	if (*end || v < 0 || v >= sp_shape_count(R)) return 0;
+	return (int)v;
+}
+
 // SIMPOIS_HIP_TAIL_LDS=0 keeps every multigrid tail in global memory
 static const int g_tail_lds_off = [] {
 	const char *e = getenv("SIMPOIS_HIP_TAIL_LDS");
@@ -804,7 +817,7 @@ static int plan_create(Plan &pl, const std::vector<PlaneIn> &planes,
 		}
 		L.nsmall = (int)small.size();
 		TRY(pl.upload(&L.d_small, small));
-		TRY(gs_build(pl, L, 0));
+		TRY(gs_build(pl, L, gs_default_var(pl.R)));
 	}
 	// parent (finer) and coarse descriptors aligned with each level's members
 	for (int k = 0; k < nlev; k++) {

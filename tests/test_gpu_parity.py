@@ -415,6 +415,25 @@ def test_autotuned_tile_shapes_match_oracle(tstep, niter):
     assert f"gs R={R} T={niter}" in rep and "*" in rep
 
 
+def test_every_tile_shape_on_the_device_matches_oracle():
+    """Each compiled R = 1 tile shape, chosen through SIMPOIS_HIP_GS_SHAPE
+    when the plan is built (a size of its own per shape, so no cached plan
+    is reused), gives the oracle's bits on a single 300x(260+v) level."""
+    import os
+    m = _imscript()
+    n_shapes = 12
+    try:
+        for v in range(n_shapes):
+            os.environ["SIMPOIS_HIP_GS_SHAPE"] = str(v)
+            g, f = random_case(70 + v, 1, 260 + v, 300, 0.4, True)
+            print(f"tile shape {v} 1x{260 + v}x300 n=40", flush=True)
+            ref, _ = oracle_solve(g, f, 0.25, 40, 1, 0)
+            out = m.simpois(g, f, tstep=0.25, niter=40, scale=1, cgit=0, trace=False)
+            assert bits_equal(out, ref), v
+    finally:
+        os.environ.pop("SIMPOIS_HIP_GS_SHAPE", None)
+
+
 def test_stream_copy_copies_and_checks_its_arguments():
     """The measurement copy kernel moves every element (tail of a size that is
     not a multiple of its 4096-float step included) and refuses misaligned or

@@ -75,8 +75,8 @@ def test_kernel_phases_match_raster_gauss_seidel(emulator, args):
 # every compiled tile shape (sp_gs.h SP_GS_SHAPES), sizes spanning several
 # tiles in each skewed direction and a partial last pass in time
 SHAPE_CASES = (
-    [f"150 140 40 0.25 {11 + v} 0.45 1 tiles {v}" for v in range(10)]
-    + [f"200 3 37 0.25 {21 + v} 0.5 0 tiles {v}" for v in range(10)]
+    [f"150 140 40 0.25 {11 + v} 0.45 1 tiles {v}" for v in range(12)]
+    + [f"200 3 37 0.25 {21 + v} 0.5 0 tiles {v}" for v in range(12)]
     + [f"100 90 20 -0.08 {31 + v} 0.4 1 tiles {v}" for v in range(4)]
     + [f"97 83 9 -1000.02 {41 + v} 0.4 1 tiles {v}" for v in range(3)]
 )
