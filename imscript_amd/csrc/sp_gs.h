@@ -45,6 +45,10 @@ struct GsInst {
 // Skewed temporal tile: (instance, I', J', T').
 struct GsTile { int inst, ti, tj, tt; };
 
+// a member's tile grid for the dependence-ordered launch: flag index of
+// tile (I', J', T') = base + (T' nJ + J') nI + I'
+struct GsFlowInst { int base, nI, nJ, nT; };
+
 // Tile shape: skewed extents BI x BJ and BT sweeps per tile (the sweeps of
 // one pass over the image) for stencil radius R.
 template <int R_, int BI_, int BJ_, int BT_> struct Shape {

@@ -114,6 +114,18 @@ static int gs_default_var(int R)
 	return (int)v;
 }
 
+// A level's Gauss-Seidel tiles run in one launch ordered by their
+// dependences (k_gs_flow); SIMPOIS_HIP_GS_FLOW=0 makes it one launch per
+// wavefront of tiles (k_gs_tile)
+static const int g_gs_flow = [] {
+	const char *e = getenv("SIMPOIS_HIP_GS_FLOW");
+	return e && !strcmp(e, "0") ? 0 : 1;
+}();
+// polls of one predecessor flag before k_gs_flow gives up (a poll is a
+// read from memory and a 128-cycle sleep, about 1 us: some 0.25 s in all,
+// far beyond the few tile run-times a predecessor can take)
+static constexpr int GS_FLOW_SPIN = 1 << 18;
+
 // SIMPOIS_HIP_TAIL_LDS=0 keeps every multigrid tail in global memory
 static const int g_tail_lds_off = [] {
 	const char *e = getenv("SIMPOIS_HIP_TAIL_LDS");
@@ -357,6 +369,11 @@ struct LevelPlan {
 	int *d_small = nullptr;
 	int nsmall = 0, small_lds = 0, small_threads = 64;
 	int cells_slots = 0;             // slots per thread of k_gs_cells (0: k_gs_small)
+	// dependence-ordered tiles (k_gs_flow): member tile grids, one flag per
+	// grid cell, resident work-groups, flag value of the last launch
+	GsFlowInst *d_flow = nullptr;
+	int *d_flags = nullptr;
+	int flow_blocks = 0, epoch = 0;
 	bool any_cg = false;
 	// FP32 CG of this level, built on first use and kept with the plan
 	struct {
@@ -416,6 +433,11 @@ struct Plan {
 	double *res_part = nullptr, *res_out = nullptr;
 	hipStream_t cap_stream = nullptr;
 
+	// claim counter of k_gs_flow and its error flag (read after a solve)
+	unsigned *gs_claim = nullptr;
+	int *gs_err = nullptr, *h_gs_err = nullptr;
+	bool gs_flow_used = false;
+
 	int transpose = 0;               // the CG applies the transposed biharmonic
 	                                 // operator (the adjoint's solve)
 	std::vector<char> key;           // identity of the call this plan serves
@@ -432,6 +454,7 @@ struct Plan {
 			if (e) (void)hipGraphExecDestroy(e);
 		for (auto &a : allocs) dev_free(a.first, a.second);
 		if (h_done) (void)hipHostFree(h_done);
+		if (h_gs_err) (void)hipHostFree(h_gs_err);
 		if (h_stage) (void)hipHostFree(h_stage);
 		if (cap_stream) (void)hipStreamDestroy(cap_stream);
 	}
@@ -523,6 +546,35 @@ static int gs_build(Plan &pl, LevelPlan &L, int var)
 	TRY(gs_tiles(pl, L, var, tiles, L.wf_off, &L.tile_pass_px));
 	TRY(pl.upload(&L.d_tiles, tiles));
 	L.gs_var = var;
+	L.d_flags = nullptr;
+	if (!g_gs_flow || tiles.empty()) return SP_OK;
+	// one flag per cell of each member's tile grid: 0 for a tile (below
+	// every epoch), INT_MAX where the grid has no tile (never waited for)
+	std::vector<GsFlowInst> fi(L.sp.size(), GsFlowInst{0, 0, 0, 0});
+	long total = 0;
+	for (int m : L.tiled) {
+		GsFlowInst &f = fi[m];
+		tile_grid_var(pl.R, var, L.sp[m].w, L.sp[m].h, L.nsweep[m], f.nI, f.nJ, f.nT);
+		f.base = (int)total;
+		total += (long)f.nI * f.nJ * f.nT;
+	}
+	if (total > 0x7fffffffL) return SP_OK;   // stays on the wavefront launches
+	std::vector<int> flags((size_t)total, 0x7fffffff);
+	for (const GsTile &t : tiles) {
+		const GsFlowInst &f = fi[t.inst];
+		flags[(size_t)f.base + ((size_t)t.tt * f.nJ + t.tj) * f.nI + t.ti] = 0;
+	}
+	TRY(pl.upload(&L.d_flow, fi));
+	TRY(pl.upload(&L.d_flags, flags));
+	L.flow_blocks = spk::gs_flow_blocks(pl.R, var);
+	L.epoch = 0;
+	if (!pl.gs_claim) {
+		TRY(pl.alloc(&pl.gs_claim, 1));
+		TRY(pl.alloc(&pl.gs_err, 1));
+		HIPCHK(hipMemset(pl.gs_err, 0, sizeof(int)));
+		if (hipHostMalloc(&pl.h_gs_err, sizeof(int)) != hipSuccess)
+			return set_err(SP_ERR_NOMEM, "pinned host allocation of 4 bytes failed");
+	}
 	return SP_OK;
 }
 
@@ -860,8 +912,19 @@ static int run_gs(Plan &pl, LevelPlan &L, hipStream_t s)
 		// algorithmic bytes: per pass of BT sweeps, read u and the
 		// coefficient plane and write u once (12 B per pixel)
 		ProfScope pr(pl.R == 1 ? "gs_tile" : "gs_tile_r23", s);
-		TRY(gs_launch_all(pl, L.gs_var, L.d_tiles, L.wf_off, L.d_insts, s));
-		pr.add((long)L.wf_off.size() - 1, 12.0 * L.tile_pass_px);
+		if (L.d_flags && L.epoch >= 0x7ffffffe)
+			L.d_flags = nullptr;   // flags exhausted: one launch per wavefront
+		if (L.d_flags) {
+			const int nt = L.wf_off.back(), nb = std::min(nt, L.flow_blocks);
+			spk::launch_gs_flow(pl.R, L.gs_var, L.d_tiles, nt, nb, L.d_insts, L.d_flow,
+					pl.gs_claim, L.d_flags, ++L.epoch, GS_FLOW_SPIN, pl.gs_err, s);
+			HIPCHK(hipGetLastError());
+			pl.gs_flow_used = true;
+			pr.add(1, 12.0 * L.tile_pass_px);
+		} else {
+			TRY(gs_launch_all(pl, L.gs_var, L.d_tiles, L.wf_off, L.d_insts, s));
+			pr.add((long)L.wf_off.size() - 1, 12.0 * L.tile_pass_px);
+		}
 	}
 	HIPCHK(hipGetLastError());
 	return SP_OK;
@@ -1417,8 +1480,15 @@ static int plan_execute(Plan &pl, hipStream_t s, sp_stats *st)
 	}
 	spk::launch_finalize(pl.lv[0].d_sp, pl.d_outs, pl.lv[0].d_bw, pl.lv[0].nbw, s);
 	HIPCHK(hipGetLastError());
+	if (pl.gs_flow_used)
+		HIPCHK(hipMemcpyAsync(pl.h_gs_err, pl.gs_err, sizeof(int), hipMemcpyDeviceToHost, s));
 	HIPCHK(hipEventRecord(e1, s));
 	HIPCHK(hipEventSynchronize(e1));
+	if (pl.gs_flow_used && *pl.h_gs_err) {
+		*pl.h_gs_err = 0;
+		(void)hipMemset(pl.gs_err, 0, sizeof(int));
+		return set_err(SP_ERR_HIP, "Gauss-Seidel tile dependence wait timed out");
+	}
 	float ms = 0;
 	(void)hipEventElapsedTime(&ms, e0, e1);
 	(void)hipEventDestroy(e0);

@@ -72,7 +72,11 @@ static __device__ __forceinline__ void gstore_b(void *base, unsigned off, T v)
 // compiler wait there), and a cell outside the image then takes, for R = 1,
 // the NaN value and the known marker by a select.  The same cells get the
 // same values as in tile_load.
-template <class S>
+//
+// COH: the values (not the coefficients) are read with agent-scope
+// coherent loads, which see stores other XCDs made with agent-scope
+// coherent stores without any cache invalidation.
+template <class S, bool COH = false>
 static __device__ __forceinline__ void tile_load_dev(const TileCtx<S> &c, const GsInst &in)
 {
 	using G = Geo<S>;
@@ -91,7 +95,11 @@ static __device__ __forceinline__ void tile_load_dev(const TileCtx<S> &c, const 
 			const int j = row + c.jlo, i = col + c.K - G::SH * j;
 			ok[u] = e < N && i >= 0 && i < c.W && j >= 0 && j < c.H;
 			const size_t q = (size_t)sp_clamp(j, 0, c.H - 1) * c.W + sp_clamp(i, 0, c.W - 1);
-			uv[u] = gload<float>(in.u + q);
+			if (COH)
+				uv[u] = __hip_atomic_load((__attribute__((address_space(1))) float *)(in.u + q),
+						__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			else
+				uv[u] = gload<float>(in.u + q);
 			cv[u] = gload<float>(in.c + q);
 		}
 #pragma unroll
@@ -177,6 +185,15 @@ void launch_gs_tiles(int R, int var, const GsTile *tiles, int ntiles,
 		const GsInst *insts, hipStream_t s);
 void launch_gs_small(int R, const GsInst *insts, const int *ids, int nids,
 		int lds_bytes, int threads, hipStream_t s);
+// the tiles of a level in one launch ordered by dependences (k_gs_flow):
+// nblocks work-groups (the resident count, gs_flow_blocks) claim the tiles
+// from *claim, which the launch first sets to 0; flags: one per cell of
+// each member's tile grid (GsFlowInst), INT_MAX for grid cells without a
+// tile, below epoch for a tile not yet run in this launch
+int gs_flow_blocks(int R, int var);
+void launch_gs_flow(int R, int var, const GsTile *tiles, int ntiles, int nblocks,
+		const GsInst *insts, const GsFlowInst *fi, unsigned *claim, int *flags,
+		int epoch, int spin_max, int *err, hipStream_t s);
 
 // CG (FP64 vectors).  Vectors are compact: entry k belongs to the k-th
 // unknown in raster order, at pixel idx[k].

@@ -6,10 +6,9 @@
 #include <vector>
 #include "sp_gs.h"
 
-// skewed tiles of one level, grouped by wavefront I'+J'+T'
+// extents of a level's skewed tile grid (tiles outside the image included)
 template <class S>
-inline void build_tiles(int W, int H, int T, int pd, std::vector<GsTile> &out,
-		std::vector<int> &off)
+inline void tile_grid(int W, int H, int T, int &nI, int &nJ, int &nT)
 {
 	constexpr int R = S::R;
 	const int SH = R - 1;
@@ -19,9 +18,20 @@ inline void build_tiles(int W, int H, int T, int pd, std::vector<GsTile> &out,
 		ispan = (long)(W - 1) + (long)SH * (H - 1) + (long)R * SH * (T - 1) + 1;
 		jspan = (long)(H - 1) + (long)R * (T - 1) + 1;
 	}
-	const int nI = (int)((ispan + S::BI - 1) / S::BI);
-	const int nJ = (int)((jspan + S::BJ - 1) / S::BJ);
-	const int nT = (T + S::BT - 1) / S::BT;
+	nI = (int)((ispan + S::BI - 1) / S::BI);
+	nJ = (int)((jspan + S::BJ - 1) / S::BJ);
+	nT = (T + S::BT - 1) / S::BT;
+}
+
+// skewed tiles of one level, grouped by wavefront I'+J'+T'
+template <class S>
+inline void build_tiles(int W, int H, int T, int pd, std::vector<GsTile> &out,
+		std::vector<int> &off)
+{
+	constexpr int R = S::R;
+	const int SH = R - 1;
+	int nI, nJ, nT;
+	tile_grid<S>(W, H, T, nI, nJ, nT);
 	std::vector<std::vector<GsTile>> wf(nI + nJ + nT);
 	for (int tt = 0; tt < nT; tt++) {
 		const long k0 = (long)tt * S::BT;
@@ -53,6 +63,13 @@ inline void build_tiles(int W, int H, int T, int pd, std::vector<GsTile> &out,
 	}
 }
 
+
+// tile_grid for variant v of radius R
+inline void tile_grid_var(int R, int v, int W, int H, int T, int &nI, int &nJ, int &nT)
+{
+	nI = nJ = nT = 0;
+	sp_with_shape(R, v, [&](auto sh) { tile_grid<decltype(sh)>(W, H, T, nI, nJ, nT); });
+}
 
 // the same for variant v of radius R; returns the variant's sweeps per tile
 inline int build_tiles_var(int R, int v, int W, int H, int T, int pd,

@@ -10,7 +10,7 @@
 // of bounds aborts the run.
 //
 // usage: gs_emulate W H T tstep seed holefrac withf mode [variant]
-//   mode: tiles | small | cells; variant: tile shape variant of the radius (sp_gs.h)
+//   mode: tiles | flow | small | cells; variant: tile shape variant of the radius (sp_gs.h)
 //        gs_emulate amle W H T nn seed holefrac [variant]
 //   the AMLE sweeps (sp_amle.h) against the oracle's raster-order sweeps
 //   (oracle/amle_oracle.c); variant: AMLE tile shape of radius amle_radius(nn)
@@ -108,6 +108,80 @@ static int run_small(float *u, const float *c, int W, int H, int T, float t)
 			small_step<R>(U, C, W, H, T, t, (int)s, sl[tid]);
 	for (int tid = 0; tid < nt; tid++) small_store<R>(U, C, in, tid, nt);
 	delete[] lds;
+	return 0;
+}
+
+// The dependence-ordered launch (k_gs_flow): tiles start in a random order
+// in which every existing predecessor (I'-a, J'-b, T'-c), a, b, c in
+// {0, 1}, has stored before a tile loads; up to 64 tiles are in flight
+// between their load and their store, finished in random order.
+template <class S>
+static int run_flow(float *u, const float *c, int W, int H, int T, float t, unsigned seed)
+{
+	using G = Geo<S>;
+	std::vector<GsTile> tiles;
+	std::vector<int> off;
+	build_tiles<S>(W, H, T, 1, tiles, off);
+	int nI, nJ, nT;
+	tile_grid<S>(W, H, T, nI, nJ, nT);
+	auto cell = [&](int ti, int tj, int tt) { return ((size_t)tt * nJ + tj) * nI + ti; };
+	// grid state: -1 no tile, 0 waiting, 1 in flight, 2 stored
+	std::vector<signed char> st((size_t)nI * nJ * nT, -1);
+	for (const GsTile &x : tiles) st[cell(x.ti, x.tj, x.tt)] = 0;
+	auto ready = [&](const GsTile &x) {
+		for (int q = 1; q < 8; q++) {
+			const int a = q & 1, b = q >> 1 & 1, cc = q >> 2;
+			if (x.ti < a || x.tj < b || x.tt < cc) continue;
+			const signed char v = st[cell(x.ti - a, x.tj - b, x.tt - cc)];
+			if (v == 0 || v == 1) return false;
+		}
+		return true;
+	};
+	GsInst in{u, c, W, H, T, t, nullptr};
+	struct Fly { TileCtx<S> ctx; std::vector<float> lds; GsTile tl; };
+	std::vector<Fly *> fly;
+	std::vector<char> started(tiles.size(), 0);
+	size_t done = 0;
+	unsigned r = seed * 2654435761u + 3;
+	auto rnd = [&](unsigned n) { r = r * 1664525u + 1013904223u; return (r >> 8) % n; };
+	while (done < tiles.size()) {
+		std::vector<size_t> rd;
+		for (size_t k = 0; k < tiles.size(); k++)
+			if (!started[k] && ready(tiles[k])) rd.push_back(k);
+		const bool start = !rd.empty() && (fly.empty() || (fly.size() < 64 && rnd(2)));
+		if (start) {
+			const size_t k = rd[rnd((unsigned)rd.size())];
+			Fly *f = new Fly;
+			f->lds.assign(G::LDS_FLOATS, -12345.0f);
+			f->tl = tiles[k];
+			f->ctx = tile_ctx<S>(f->tl, in, f->lds.data());
+			for (int tid = 0; tid < G::NT; tid++) tile_load<S>(f->ctx, in, tid);
+			started[k] = 1;
+			st[cell(f->tl.ti, f->tl.tj, f->tl.tt)] = 1;
+			fly.push_back(f);
+			continue;
+		}
+		if (fly.empty()) {
+			printf("STUCK with %zu of %zu tiles stored\n", done, tiles.size());
+			return 1;
+		}
+		const size_t z = rnd((unsigned)fly.size());
+		Fly *f = fly[z];
+		fly.erase(fly.begin() + z);
+		if (S::R == 1) {
+			std::vector<Gs1Thread> th(G::NT);
+			for (int tid = 0; tid < G::NT; tid++) th[tid] = gs1_thread<S>(f->ctx, tid);
+			for (int sg = 0; sg < G::NSTEP; sg++)
+				for (int tid = 0; tid < G::NT; tid++) gs1_step<S>(f->ctx, th[tid], sg);
+		} else {
+			for (int sg = 0; sg < G::NSTEP; sg++)
+				for (int tid = 0; tid < G::NT; tid++) tile_step<S>(f->ctx, tid, sg);
+		}
+		for (int tid = 0; tid < G::NT; tid++) tile_store<S>(f->ctx, in, tid);
+		st[cell(f->tl.ti, f->tl.tj, f->tl.tt)] = 2;
+		delete f;
+		done++;
+	}
 	return 0;
 }
 
@@ -219,7 +293,7 @@ int main(int argc, char **argv)
 	if (argc > 1 && !strcmp(argv[1], "amle")) return main_amle(argc, argv);
 	if (argc != 9 && argc != 10) {
 		fprintf(stderr, "usage: gs_emulate W H T tstep seed holefrac withf "
-				"tiles|small|cells [variant]\n");
+				"tiles|flow|small|cells [variant]\n");
 		return 2;
 	}
 	const int var = argc == 10 ? atoi(argv[9]) : 0;
@@ -229,6 +303,7 @@ int main(int argc, char **argv)
 	const double frac = atof(argv[6]);
 	const bool withf = atoi(argv[7]) != 0;
 	const bool small = !strcmp(argv[8], "small"), cells = !strcmp(argv[8], "cells");
+	const bool flow = !strcmp(argv[8], "flow");
 	const size_t n = (size_t)W * H;
 	float *g = new float[n], *f = withf ? new float[n] : nullptr;
 	float *u = new float[n], *uref = new float[n], *c = new float[n];
@@ -264,6 +339,10 @@ int main(int argc, char **argv)
 		: R == 2 ? run_small<2>(u, c, W, H, T, teff) : run_small<3>(u, c, W, H, T, teff);
 	else if (!sp_with_shape(R, var, [&](auto sh) {
 			using S = decltype(sh);
+			if (flow) {
+				rc = run_flow<S>(u, c, W, H, T, teff, (unsigned)atoi(argv[5]));
+				return;
+			}
 			if constexpr (S::R == 1) {
 				// the device's R = 1 loop: per-thread state made at step 0
 				std::vector<std::vector<Gs1Thread>> st;

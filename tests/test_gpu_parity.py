@@ -359,6 +359,22 @@ def _mg_child(h, w, seed, geo, tmp_path):
     return np.load(out)
 
 
+@pytest.mark.parametrize("flow", ["0", "1"])
+def test_tile_launch_forms_against_oracle(flow):
+    """The Gauss-Seidel tiles of a level as one launch per wavefront
+    (SIMPOIS_HIP_GS_FLOW=0) and as one launch ordered by the tiles'
+    dependences (the default), each in a child process (the variable is read
+    when the library loads): bit-identical to the oracle for every radius, a
+    multiscale pyramid and a batch of planes."""
+    env = dict(os.environ, SIMPOIS_HIP_GS_FLOW=flow)
+    child = os.path.join(os.path.dirname(__file__), "gs_flow_child.py")
+    print(f"gs child flow={flow}", flush=True)
+    p = subprocess.run([sys.executable, child], env=env, capture_output=True,
+                       timeout=300)
+    print(p.stdout.decode(), flush=True)
+    assert p.returncode == 0, p.stdout.decode()[-2000:] + p.stderr.decode()[-2000:]
+
+
 @pytest.mark.parametrize("h,w", [(2600, 2700), (1100, 40000)])
 def test_multigrid_fused_coarse_levels(h, w, tmp_path):
     """Planes whose multigrid levels 5 (2600x2700: 82x85) and 6 (1100x40000:

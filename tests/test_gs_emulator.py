@@ -108,6 +108,26 @@ SMALL_LIMIT_CASES = [
 ]
 
 
+# the dependence-ordered launch (k_gs_flow): random tile orders in which
+# each tile waits only for its existing predecessors at skewed distance
+# {0, 1}^3, up to 64 tiles between load and store
+FLOW_CASES = (
+    [f"150 140 40 0.25 {61 + v} 0.45 1 flow {v}" for v in (0, 3, 6, 10, 11)]
+    + ["200 3 37 0.25 21 0.5 0 flow 0", "300 260 40 0.25 105 0.4 1 flow 0",
+       "100 90 20 -0.08 31 0.4 1 flow 0", "100 90 20 -0.08 32 0.4 1 flow 3",
+       "97 83 9 -1000.02 41 0.4 1 flow 2", "1 9 7 -0.08 1 0.5 1 flow 0"]
+)
+
+
+@pytest.mark.parametrize("args", FLOW_CASES)
+def test_dependence_ordered_tiles_match_raster_gauss_seidel(emulator, args):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    p = subprocess.run([emulator, *args.split()], capture_output=True,
+                       text=True, timeout=600, env=env)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert p.stdout.startswith("OK")
+
+
 # whole R = 1 level by unknown pixel (k_gs_cells): sizes up to its
 # 16384-pixel limit, all-hole planes (8 pixels per thread and parity),
 # single rows and columns, a 1x1 level, infinite and 3e37 known values
