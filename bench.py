@@ -1153,11 +1153,12 @@ def run_c2(a, method, full):
         # per-pass compulsory traffic: a pass of BT fused sweeps reads u and
         # the coefficient plane and writes u once per pixel (12 B); the
         # effective bandwidth counts those 12 B for every sweep of every
-        # level swept in tiles (levels whose u and coefficients exceed 64 KiB)
+        # level swept in tiles (levels of more than 16384 pixels; smaller
+        # ones run whole in one work-group)
         achieved = gs_bytes / (gs_ms * 1e-3) / 1e9
         tiled_px, w = 0, a.size
         for _ in range(stats.levels):
-            if 8 * w * w > 65536:
+            if w * w > 16384:
                 tiled_px += w * w * a.channels
             w = (w + 1) // 2
         eff = 12.0 * a.niter * tiled_px / (gs_ms * 1e-3) / 1e9
@@ -1166,9 +1167,9 @@ def run_c2(a, method, full):
         shape = ([c[1:].split(":")[0] for c in finest.split("choice=")[1].split(",")
                   if c.startswith("*")] or ["32x128x16"])[0] if finest else "32x128x16"
         result["roofline"] = {
-            "kernel": f"k_gs_tile<1> (tile {shape} = BIxBJxBT: BT lexicographic "
-                      "sweeps fused per pass, one launch per wavefront of "
-                      "skewed tiles)",
+            "kernel": f"k_gs_flow<1> (tile {shape} = BIxBJxBT: BT lexicographic "
+                      "sweeps fused per pass; one launch per level, each skewed "
+                      "tile started once its predecessors have stored)",
             "bound": "hbm", "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBPS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,

@@ -540,6 +540,40 @@ static int gs_tiles(const Plan &pl, const LevelPlan &L, int var,
 	return SP_OK;
 }
 
+// the dependence-ordered launch's data for the tiles of variant var: each
+// member's tile grid and one flag per grid cell, 0 for a tile (below every
+// epoch), INT_MAX where the grid has no tile (never waited for); *d_flags
+// stays null when the grids exceed the flag index range
+static int gs_flow_build(Plan &pl, const LevelPlan &L, int var,
+		const std::vector<GsTile> &tiles, GsFlowInst **d_fi, int **d_flags)
+{
+	*d_flags = nullptr;
+	std::vector<GsFlowInst> fi(L.sp.size(), GsFlowInst{0, 0, 0, 0});
+	long total = 0;
+	for (int m : L.tiled) {
+		GsFlowInst &f = fi[m];
+		tile_grid_var(pl.R, var, L.sp[m].w, L.sp[m].h, L.nsweep[m], f.nI, f.nJ, f.nT);
+		f.base = (int)std::min<long>(total, 0x7fffffffL);
+		total += (long)f.nI * f.nJ * f.nT;
+	}
+	if (total > 0x7fffffffL) return SP_OK;
+	std::vector<int> flags((size_t)total, 0x7fffffff);
+	for (const GsTile &t : tiles) {
+		const GsFlowInst &f = fi[t.inst];
+		flags[(size_t)f.base + ((size_t)t.tt * f.nJ + t.tj) * f.nI + t.ti] = 0;
+	}
+	TRY(pl.upload(d_fi, fi));
+	TRY(pl.upload(d_flags, flags));
+	if (!pl.gs_claim) {
+		TRY(pl.alloc(&pl.gs_claim, 1));
+		TRY(pl.alloc(&pl.gs_err, 1));
+		HIPCHK(hipMemset(pl.gs_err, 0, sizeof(int)));
+		if (hipHostMalloc(&pl.h_gs_err, sizeof(int)) != hipSuccess)
+			return set_err(SP_ERR_NOMEM, "pinned host allocation of 4 bytes failed");
+	}
+	return SP_OK;
+}
+
 static int gs_build(Plan &pl, LevelPlan &L, int var)
 {
 	std::vector<GsTile> tiles;
@@ -548,33 +582,9 @@ static int gs_build(Plan &pl, LevelPlan &L, int var)
 	L.gs_var = var;
 	L.d_flags = nullptr;
 	if (!g_gs_flow || tiles.empty()) return SP_OK;
-	// one flag per cell of each member's tile grid: 0 for a tile (below
-	// every epoch), INT_MAX where the grid has no tile (never waited for)
-	std::vector<GsFlowInst> fi(L.sp.size(), GsFlowInst{0, 0, 0, 0});
-	long total = 0;
-	for (int m : L.tiled) {
-		GsFlowInst &f = fi[m];
-		tile_grid_var(pl.R, var, L.sp[m].w, L.sp[m].h, L.nsweep[m], f.nI, f.nJ, f.nT);
-		f.base = (int)total;
-		total += (long)f.nI * f.nJ * f.nT;
-	}
-	if (total > 0x7fffffffL) return SP_OK;   // stays on the wavefront launches
-	std::vector<int> flags((size_t)total, 0x7fffffff);
-	for (const GsTile &t : tiles) {
-		const GsFlowInst &f = fi[t.inst];
-		flags[(size_t)f.base + ((size_t)t.tt * f.nJ + t.tj) * f.nI + t.ti] = 0;
-	}
-	TRY(pl.upload(&L.d_flow, fi));
-	TRY(pl.upload(&L.d_flags, flags));
+	TRY(gs_flow_build(pl, L, var, tiles, &L.d_flow, &L.d_flags));
 	L.flow_blocks = spk::gs_flow_blocks(pl.R, var);
 	L.epoch = 0;
-	if (!pl.gs_claim) {
-		TRY(pl.alloc(&pl.gs_claim, 1));
-		TRY(pl.alloc(&pl.gs_err, 1));
-		HIPCHK(hipMemset(pl.gs_err, 0, sizeof(int)));
-		if (hipHostMalloc(&pl.h_gs_err, sizeof(int)) != hipSuccess)
-			return set_err(SP_ERR_NOMEM, "pinned host allocation of 4 bytes failed");
-	}
 	return SP_OK;
 }
 
@@ -654,6 +664,13 @@ static int gs_tune(Plan &pl, LevelPlan &L, hipStream_t s)
 		rc = gs_tiles(pl, L, v, tiles, wo, nullptr);
 		GsTile *d_t = nullptr;
 		if (rc == SP_OK) rc = pl.upload(&d_t, tiles);
+		// time the launch form the solve uses
+		GsFlowInst *d_fi = nullptr;
+		int *d_fl = nullptr, nb = 0;
+		if (rc == SP_OK && g_gs_flow && !tiles.empty()) {
+			rc = gs_flow_build(pl, L, v, tiles, &d_fi, &d_fl);
+			nb = std::min((int)tiles.size(), spk::gs_flow_blocks(pl.R, v));
+		}
 		float t = 1e30f;
 		for (int rep = 0; rep < 2 && rc == SP_OK; rep++) {   // the first warms up
 			for (int m : L.tiled)
@@ -663,7 +680,14 @@ static int gs_tune(Plan &pl, LevelPlan &L, hipStream_t s)
 					rc = set_err(SP_ERR_HIP, "autotune copy failed");
 			if (rc != SP_OK) break;
 			(void)hipEventRecord(a, s);
-			rc = gs_launch_all(pl, v, d_t, wo, d_gi, s);
+			if (d_fl) {
+				spk::launch_gs_flow(pl.R, v, d_t, (int)tiles.size(), nb, d_gi, d_fi,
+						pl.gs_claim, d_fl, rep + 1, GS_FLOW_SPIN, pl.gs_err, s);
+				if (hipGetLastError() != hipSuccess)
+					rc = set_err(SP_ERR_HIP, "autotune launch failed");
+			} else {
+				rc = gs_launch_all(pl, v, d_t, wo, d_gi, s);
+			}
 			(void)hipEventRecord(b, s);
 			if (hipEventSynchronize(b) != hipSuccess)
 				rc = set_err(SP_ERR_HIP, "autotune run failed");
