@@ -75,8 +75,9 @@ static __device__ __forceinline__ void gstore_b(void *base, unsigned off, T v)
 //
 // COH: the values (not the coefficients) are read with agent-scope
 // coherent loads, which see stores other XCDs made with agent-scope
-// coherent stores without any cache invalidation.
-template <class S, bool COH = false>
+// coherent stores without any cache invalidation.  PART: 0 values and
+// coefficients, 1 the coefficients only, 2 the values only.
+template <class S, bool COH = false, int PART = 0>
 static __device__ __forceinline__ void tile_load_dev(const TileCtx<S> &c, const GsInst &in)
 {
 	using G = Geo<S>;
@@ -95,23 +96,25 @@ static __device__ __forceinline__ void tile_load_dev(const TileCtx<S> &c, const 
 			const int j = row + c.jlo, i = col + c.K - G::SH * j;
 			ok[u] = e < N && i >= 0 && i < c.W && j >= 0 && j < c.H;
 			const size_t q = (size_t)sp_clamp(j, 0, c.H - 1) * c.W + sp_clamp(i, 0, c.W - 1);
-			if (COH)
-				uv[u] = __hip_atomic_load((__attribute__((address_space(1))) float *)(in.u + q),
-						__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-			else
-				uv[u] = gload<float>(in.u + q);
-			cv[u] = gload<float>(in.c + q);
+			if (PART != 1) {
+				if (COH)
+					uv[u] = __hip_atomic_load((__attribute__((address_space(1))) float *)
+							(in.u + q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				else
+					uv[u] = gload<float>(in.u + q);
+			}
+			if (PART != 2) cv[u] = gload<float>(in.c + q);
 		}
 #pragma unroll
 		for (int u = 0; u < B; u++) {
 			const int e = tid + (e0 + u) * G::NT;
 			if (e >= N) continue;
 			if (ok[u]) {
-				c.U[e] = uv[u];
-				c.C[e] = cv[u];
+				if (PART != 1) c.U[e] = uv[u];
+				if (PART != 2) c.C[e] = cv[u];
 			} else if (R == 1) {
-				c.U[e] = sp_float(SP_QNAN_BITS);
-				c.C[e] = sp_float(SP_KNOWN_BITS);
+				if (PART != 1) c.U[e] = sp_float(SP_QNAN_BITS);
+				if (PART != 2) c.C[e] = sp_float(SP_KNOWN_BITS);
 			}
 		}
 	}

@@ -288,13 +288,16 @@ void launch_finalize(const SpPlane *lv, float *const *outs,
 
 // one tile: its footprint into LDS, its BT sweeps hyperplane by hyperplane,
 // the pixels it updated back to the image
-template <class S, bool COH = false>
+// (FLOW: the form of k_gs_flow, whose coefficients are already in LDS and
+// whose values are read and written with agent-scope coherent accesses)
+template <class S, bool FLOW = false>
 static __device__ __forceinline__ void gs_tile_run(const GsTile &tl, const GsInst &in,
 		float *smem)
 {
 	using G = Geo<S>;
+	constexpr bool COH = FLOW;
 	const TileCtx<S> c = tile_ctx<S>(tl, in, smem);
-	tile_load_dev<S, COH>(c, in);
+	tile_load_dev<S, COH, FLOW ? 2 : 0>(c, in);
 	__syncthreads();
 	if constexpr (S::R == 1) {
 		Gs1Thread t = gs1_thread<S>(c, threadIdx.x);
@@ -365,6 +368,11 @@ __global__ __launch_bounds__(Geo<S>::NT) void k_gs_flow(const GsTile *tiles, int
 		const GsTile tl = tiles[k];
 		const GsFlowInst f = fi[tl.inst];
 		const int d = f.base + (tl.tt * f.nJ + tl.tj) * f.nI + tl.ti;
+		const GsInst in = insts[tl.inst];
+		const bool run = !in.active || *in.active;
+		// the coefficients do not change during a level: into LDS before the
+		// wait for the predecessors (the values follow after it)
+		if (run) tile_load_dev<S, false, 1>(tile_ctx<S>(tl, in, smem), in);
 		if (threadIdx.x < 7) {
 			const int a = (threadIdx.x + 1) & 1, b = (threadIdx.x + 1) >> 1 & 1;
 			const int cc = (threadIdx.x + 1) >> 2;
@@ -382,8 +390,7 @@ __global__ __launch_bounds__(Geo<S>::NT) void k_gs_flow(const GsTile *tiles, int
 			}
 		}
 		__syncthreads();
-		const GsInst in = insts[tl.inst];
-		if (!in.active || *in.active) gs_tile_run<S, true>(tl, in, smem);
+		if (run) gs_tile_run<S, true>(tl, in, smem);
 		__builtin_amdgcn_s_waitcnt(0);   // this wave's stores have completed
 		__syncthreads();
 		if (threadIdx.x == 0)
