@@ -125,6 +125,37 @@ __global__ __launch_bounds__(Geo<S>::NT) void k_amle4_tile(const GsTile *tiles,
 	tile_store<S>(c, in, threadIdx.x);
 }
 
+// the tiles of a pass in one launch ordered by their dependences
+// (tile_flow_loop), with the update of k_amle4_tile (FAST4) or k_amle_tile
+template <class S, bool FAST4>
+__global__ __launch_bounds__(Geo<S>::NT) void k_amle_flow(const GsTile *tiles, int ntiles,
+		const GsInst *insts, const GsFlowInst *fi, unsigned *claim, int *flags,
+		int epoch, int spin_max, int *err, int nn)
+{
+	using G = Geo<S>;
+	extern __shared__ float smem[];
+	tile_flow_loop<S>(tiles, ntiles, insts, fi, claim, flags, epoch, spin_max, err, smem,
+		[&](const GsTile &tl, const GsInst &in) {
+			const TileCtx<S> c = tile_ctx<S>(tl, in, smem);
+			tile_load_dev<S, true, 2>(c, in);
+			if constexpr (FAST4 && S::R == 1) {
+				Amle4Thread t = amle4_thread<S>(c, threadIdx.x);
+				__syncthreads();
+				for (int sg = 0; sg < G::NSTEP; sg++) {
+					amle4_step<S>(c, t, sg);
+					__syncthreads();
+				}
+			} else {
+				__syncthreads();
+				for (int sg = 0; sg < G::NSTEP; sg++) {
+					amle_tile_step<S>(c, threadIdx.x, sg, nn);
+					__syncthreads();
+				}
+			}
+			tile_store_coh<S>(c, in);
+		});
+}
+
 // trace diagnostics over the holes of plane blockIdx.y, block partials:
 // holes, max |tinf - tsup| (NaN never wins, amle.c:120-121), sum of it,
 // sums of |old - new| of tinf and of tsup over the last sweep
@@ -233,6 +264,45 @@ void launch_amle_tiles(int R, int var, const GsTile *tiles, int ntiles,
 		}
 		k_amle_tile<S><<<ntiles, Geo<S>::NT, Geo<S>::LDS_FLOATS * 4, s>>>(tiles,
 				insts, nn);
+	});
+}
+
+int amle_flow_blocks(int R, int var, bool fast4)
+{
+	int nb = 0;
+	amle_with_shape(R, var, [&](auto sh) {
+		using S = decltype(sh);
+		int per = 0;
+		const hipError_t e = fast4
+			? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_amle_flow<S, true>,
+					Geo<S>::NT, Geo<S>::LDS_FLOATS * 4)
+			: hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_amle_flow<S, false>,
+					Geo<S>::NT, Geo<S>::LDS_FLOATS * 4);
+		if (e != hipSuccess || per < 1) per = 1;
+		int dev = 0, ncu = 0;
+		(void)hipGetDevice(&dev);
+		if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)
+				!= hipSuccess || ncu <= 0)
+			ncu = 256;
+		nb = per * ncu;
+	});
+	return nb;
+}
+
+void launch_amle_flow(int R, int var, const GsTile *tiles, int ntiles, int nblocks,
+		const GsInst *insts, const GsFlowInst *fi, unsigned *claim, int *flags,
+		int epoch, int spin_max, int *err, int nn, bool fast4, hipStream_t s)
+{
+	if (!ntiles) return;
+	launch_flow_reset(claim, s);
+	amle_with_shape(R, var, [&](auto sh) {
+		using S = decltype(sh);
+		if (fast4)
+			k_amle_flow<S, true><<<nblocks, Geo<S>::NT, Geo<S>::LDS_FLOATS * 4, s>>>(
+					tiles, ntiles, insts, fi, claim, flags, epoch, spin_max, err, nn);
+		else
+			k_amle_flow<S, false><<<nblocks, Geo<S>::NT, Geo<S>::LDS_FLOATS * 4, s>>>(
+					tiles, ntiles, insts, fi, claim, flags, epoch, spin_max, err, nn);
 	});
 }
 

@@ -2072,6 +2072,12 @@ struct AmleSched {
 	GsInst *d_insts = nullptr;
 	std::vector<int> off;
 	long px = 0;                      // pixel-sweep updates per run
+	// the dependence-ordered launch (k_amle_flow): instance tile grids, one
+	// flag per grid cell, resident work-groups, flag value of the last run
+	GsFlowInst *d_fi = nullptr;
+	int *d_flags = nullptr;
+	int nblocks = 0, epoch = 0;
+	int ntiles = 0;
 };
 
 struct AmlePlan {
@@ -2114,6 +2120,34 @@ static int amle_build(AmlePlan &A, int T)
 	sc.px = (long)ninst * ((T + bt - 1) / bt) * A.w * A.h;
 	TRY(A.mem.upload(&sc.d_tiles, tiles));
 	TRY(A.mem.alloc(&sc.d_insts, ninst));
+	sc.ntiles = (int)tiles.size();
+	if (g_gs_flow && !tiles.empty()) {
+		// every instance has the same tile grid (flags as in gs_flow_build)
+		GsFlowInst g{0, 0, 0, 0};
+		amle_with_shape(A.R, A.var, [&](auto sh) {
+			tile_grid<decltype(sh)>(A.w, A.h, T, g.nI, g.nJ, g.nT);
+		});
+		const long per = (long)g.nI * g.nJ * g.nT;
+		if (per * ninst <= 0x7fffffffL) {
+			std::vector<GsFlowInst> fi(ninst, g);
+			for (int l = 0; l < ninst; l++) fi[l].base = (int)(per * l);
+			std::vector<int> flags((size_t)(per * ninst), 0x7fffffff);
+			for (const GsTile &t : tiles)
+				flags[(size_t)fi[t.inst].base + ((size_t)t.tt * g.nJ + t.tj) * g.nI + t.ti] = 0;
+			TRY(A.mem.upload(&sc.d_fi, fi));
+			TRY(A.mem.upload(&sc.d_flags, flags));
+			sc.nblocks = std::min(sc.ntiles, spk::amle_flow_blocks(A.R, A.var,
+					A.nn == 4 && (A.w > 1 || A.h > 1)));
+			if (!A.mem.gs_claim) {
+				TRY(A.mem.alloc(&A.mem.gs_claim, 1));
+				TRY(A.mem.alloc(&A.mem.gs_err, 1));
+				HIPCHK(hipMemset(A.mem.gs_err, 0, sizeof(int)));
+				if (hipHostMalloc(&A.mem.h_gs_err, sizeof(int)) != hipSuccess)
+					return set_err(SP_ERR_NOMEM, "pinned host allocation of 4 bytes failed");
+				*A.mem.h_gs_err = 0;
+			}
+		}
+	}
 	A.sched.push_back(sc);
 	return SP_OK;
 }
@@ -2154,12 +2188,21 @@ static int amle_sweeps(AmlePlan &A, int T, hipStream_t s)
 	AmleSched *sc = amle_sched(A, T);
 	if (!sc) return set_err(SP_ERR_ARG, "AMLE schedule for %d sweeps missing", T);
 	ProfScope ps("amle_tile", s);
-	for (size_t k = 0; k + 1 < sc->off.size(); k++)
-		spk::launch_amle_tiles(A.R, A.var, sc->d_tiles + sc->off[k],
-				sc->off[k + 1] - sc->off[k], sc->d_insts, A.nn,
-				A.nn == 4 && (A.w > 1 || A.h > 1), s);
-	// algorithmic bytes: u and c read and u written once per pixel per pass
-	ps.add((long)sc->off.size() - 1, 12.0 * (double)sc->px);
+	const bool fast4 = A.nn == 4 && (A.w > 1 || A.h > 1);
+	if (sc->d_flags && sc->epoch >= 0x7ffffffe) sc->d_flags = nullptr;   // flags exhausted
+	if (sc->d_flags) {
+		spk::launch_amle_flow(A.R, A.var, sc->d_tiles, sc->ntiles, sc->nblocks, sc->d_insts,
+				sc->d_fi, A.mem.gs_claim, sc->d_flags, ++sc->epoch, GS_FLOW_SPIN,
+				A.mem.gs_err, A.nn, fast4, s);
+		A.mem.gs_flow_used = true;
+		ps.add(1, 12.0 * (double)sc->px);
+	} else {
+		for (size_t k = 0; k + 1 < sc->off.size(); k++)
+			spk::launch_amle_tiles(A.R, A.var, sc->d_tiles + sc->off[k],
+					sc->off[k + 1] - sc->off[k], sc->d_insts, A.nn, fast4, s);
+		// algorithmic bytes: u and c read and u written once per pixel per pass
+		ps.add((long)sc->off.size() - 1, 12.0 * (double)sc->px);
+	}
 	HIPCHK(hipGetLastError());
 	return SP_OK;
 }
@@ -2204,6 +2247,14 @@ extern "C" int sp_amle_device(float *d_out, const float *d_in, const float *d_in
 	const int var = amle_variant(R, pd);
 	trace = trace != 0;
 	AmlePlan *A = g_amle;
+	// a dependence wait of an earlier call that timed out (the calls are
+	// asynchronous: it is reported by the next call, once that work is done)
+	if (A && A->used && A->mem.gs_flow_used && hipEventQuery(A->done) == hipSuccess
+			&& *A->mem.h_gs_err) {
+		*A->mem.h_gs_err = 0;
+		(void)hipMemset(A->mem.gs_err, 0, sizeof(int));
+		return set_err(SP_ERR_HIP, "AMLE tile dependence wait timed out in an earlier call");
+	}
 	if (!A || A->w != w || A->h != h || A->pd != pd || A->nn != nn ||
 			A->niter != niter || A->trace != trace || A->var != var) {
 		// the old plan's scratch may still be in use on its stream
@@ -2230,6 +2281,9 @@ extern "C" int sp_amle_device(float *d_out, const float *d_in, const float *d_in
 	if (!A->done) HIPCHK(hipEventCreateWithFlags(&A->done, hipEventDisableTiming));
 	TRY(amle_bind(*A, d_out, s));
 	const int rc = amle_run(*A, d_out, d_in, d_init, s);
+	if (A->mem.gs_flow_used)
+		HIPCHK(hipMemcpyAsync(A->mem.h_gs_err, A->mem.gs_err, sizeof(int),
+				hipMemcpyDeviceToHost, s));
 	HIPCHK(hipEventRecord(A->done, s));
 	A->last = s;
 	A->used = true;

@@ -120,6 +120,84 @@ static __device__ __forceinline__ void tile_load_dev(const TileCtx<S> &c, const 
 	}
 }
 
+// tile_store with agent-scope coherent stores (the values a later tile of
+// the same launch reads with coherent loads)
+template <class S>
+static __device__ __forceinline__ void tile_store_coh(const TileCtx<S> &c, const GsInst &in)
+{
+	using G = Geo<S>;
+	for (int e = threadIdx.x; e < G::LW * G::LH; e += G::NT) {
+		const int row = e / G::LW, col = e - row * G::LW;
+		const int j = row + c.jlo, i = col + c.K - G::SH * j;
+		if (i < 0 || i >= c.W || j < 0 || j >= c.H || sp_known(c.C[e])) continue;
+		if (tile_owns<S>(c, i, j))
+			__hip_atomic_store((__attribute__((address_space(1))) float *)(in.u
+					+ (size_t)j * c.W + i), c.U[e], __ATOMIC_RELAXED,
+					__HIP_MEMORY_SCOPE_AGENT);
+	}
+}
+
+// The skewed tiles of a level (or of a pass of AMLE sweeps) in one launch,
+// ordered by their dependences instead of by one launch per wavefront.
+// Work-groups claim tiles in wavefront order from *claim; a tile starts once
+// each existing predecessor (I'-a, J'-b, T'-c), a, b, c in {0, 1} not all 0,
+// has published its stores.  Every dependence between updates (true, anti
+// and output) has a skewed distance in {0, 1}^3, so these are the only
+// tiles whose order matters.  The values of the image move between tiles
+// only through agent-scope coherent loads and stores (coherent across the
+// XCDs' L2 caches without writing back or invalidating them); a tile
+// publishes (flag = epoch, a coherent store) once its stores have completed,
+// and a waiting tile polls the flags with coherent loads.  The predecessors
+// of a claimed tile were claimed before it by running work-groups, so every
+// wait ends; a wait past spin_max polls sets *err and lets the tile run (the
+// host reports the error).
+//
+// Before the wait the tile's coefficients go into LDS (they do not change
+// during the launch); body(tile, instance) then loads the values with
+// coherent loads, runs the sweeps and stores with tile_store_coh.
+template <class S, class Body>
+static __device__ __forceinline__ void tile_flow_loop(const GsTile *tiles, int ntiles,
+		const GsInst *insts, const GsFlowInst *fi, unsigned *claim, int *flags,
+		int epoch, int spin_max, int *err, float *smem, Body body)
+{
+	__shared__ int s_k;
+	for (;;) {
+		__syncthreads();   // the previous tile's LDS and s_k are free
+		if (threadIdx.x == 0) s_k = (int)min(atomicAdd(claim, 1u), (unsigned)ntiles);
+		__syncthreads();
+		const int k = s_k;
+		if (k >= ntiles) return;
+		const GsTile tl = tiles[k];
+		const GsFlowInst f = fi[tl.inst];
+		const int d = f.base + (tl.tt * f.nJ + tl.tj) * f.nI + tl.ti;
+		const GsInst in = insts[tl.inst];
+		const bool run = !in.active || *in.active;
+		if (run) tile_load_dev<S, false, 1>(tile_ctx<S>(tl, in, smem), in);
+		if (threadIdx.x < 7) {
+			const int a = (threadIdx.x + 1) & 1, b = (threadIdx.x + 1) >> 1 & 1;
+			const int cc = (threadIdx.x + 1) >> 2;
+			if (tl.ti >= a && tl.tj >= b && tl.tt >= cc) {
+				int *p = flags + d - a - b * f.nI - cc * f.nI * f.nJ;
+				int n = 0;
+				while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+						< epoch) {
+					if (++n > spin_max) {
+						atomicOr(err, 1);
+						break;
+					}
+					__builtin_amdgcn_s_sleep(2);
+				}
+			}
+		}
+		__syncthreads();
+		if (run) body(tl, in);
+		__builtin_amdgcn_s_waitcnt(0);   // this wave's stores have completed
+		__syncthreads();
+		if (threadIdx.x == 0)
+			__hip_atomic_store(flags + d, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	}
+}
+
 // elementwise / pyramid
 // prepare / restrict write the holes of each work block to bcnt;
 // launch_count folds them into each member's counter
@@ -151,6 +229,12 @@ int amle_tile_lds_bytes(int R, int var);
 // fast4: nn = 4 on planes larger than 1 x 1 (k_amle4_tile)
 void launch_amle_tiles(int R, int var, const GsTile *tiles, int ntiles,
 		const GsInst *insts, int nn, bool fast4, hipStream_t s);
+// the tiles of a pass in one launch ordered by dependences (as
+// launch_gs_flow; the claim counter is reset first)
+int amle_flow_blocks(int R, int var, bool fast4);
+void launch_amle_flow(int R, int var, const GsTile *tiles, int ntiles, int nblocks,
+		const GsInst *insts, const GsFlowInst *fi, unsigned *claim, int *flags,
+		int epoch, int spin_max, int *err, int nn, bool fast4, hipStream_t s);
 // per-block partials (holes, max, sums) of the trace diagnostics: 5 doubles
 // per block, amle_part_blocks(n) blocks per plane
 void launch_amle_diag(const float *c, const float *tinf, const float *tsup,
@@ -194,6 +278,7 @@ void launch_gs_small(int R, const GsInst *insts, const int *ids, int nids,
 // each member's tile grid (GsFlowInst), INT_MAX for grid cells without a
 // tile, below epoch for a tile not yet run in this launch
 int gs_flow_blocks(int R, int var);
+void launch_flow_reset(unsigned *claim, hipStream_t s);
 void launch_gs_flow(int R, int var, const GsTile *tiles, int ntiles, int nblocks,
 		const GsInst *insts, const GsFlowInst *fi, unsigned *claim, int *flags,
 		int epoch, int spin_max, int *err, hipStream_t s);

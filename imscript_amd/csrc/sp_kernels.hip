@@ -311,20 +311,8 @@ static __device__ __forceinline__ void gs_tile_run(const GsTile &tl, const GsIns
 			__syncthreads();
 		}
 	}
-	if (COH) {
-		// tile_store with agent-scope coherent stores
-		for (int e = threadIdx.x; e < G::LW * G::LH; e += G::NT) {
-			const int row = e / G::LW, col = e - row * G::LW;
-			const int j = row + c.jlo, i = col + c.K - G::SH * j;
-			if (i < 0 || i >= c.W || j < 0 || j >= c.H || sp_known(c.C[e])) continue;
-			if (tile_owns<S>(c, i, j))
-				__hip_atomic_store((__attribute__((address_space(1))) float *)(in.u
-						+ (size_t)j * c.W + i), c.U[e], __ATOMIC_RELAXED,
-						__HIP_MEMORY_SCOPE_AGENT);
-		}
-	} else {
-		tile_store<S>(c, in, threadIdx.x);
-	}
+	if (COH) tile_store_coh<S>(c, in);
+	else tile_store<S>(c, in, threadIdx.x);
 }
 
 template <class S>
@@ -338,64 +326,16 @@ __global__ __launch_bounds__(Geo<S>::NT) void k_gs_tile(const GsTile *tiles,
 	gs_tile_run<S>(tl, in, smem);
 }
 
-// The tiles of a level in one launch, ordered by their dependences instead
-// of by one launch per wavefront.  Work-groups claim tiles in wavefront
-// order from a counter; a tile starts once each existing predecessor
-// (I'-a, J'-b, T'-c), a, b, c in {0, 1} not all 0, has published its
-// stores.  Every dependence between updates (true, anti and output) has a
-// skewed distance in {0, 1}^3, so these are the only tiles whose order
-// matters.  The values of the image move between tiles only through
-// agent-scope coherent loads and stores (coherent across the XCDs' L2
-// caches without writing back or invalidating them); a tile publishes
-// (flag = epoch, a coherent store) once its stores have completed, and a
-// waiting tile polls the flags with coherent loads.
-// The predecessors of a claimed tile were claimed before it by running
-// work-groups, so every wait ends; a wait past spin_max polls sets *err and
-// lets the tile run (the host reports the error).
+// the tiles of a level in one launch ordered by their dependences
+// (tile_flow_loop)
 template <class S>
 __global__ __launch_bounds__(Geo<S>::NT) void k_gs_flow(const GsTile *tiles, int ntiles,
 		const GsInst *insts, const GsFlowInst *fi, unsigned *claim, int *flags,
 		int epoch, int spin_max, int *err)
 {
 	extern __shared__ float smem[];
-	__shared__ int s_k;
-	for (;;) {
-		__syncthreads();   // the previous tile's LDS and s_k are free
-		if (threadIdx.x == 0) s_k = (int)min(atomicAdd(claim, 1u), (unsigned)ntiles);
-		__syncthreads();
-		const int k = s_k;
-		if (k >= ntiles) return;
-		const GsTile tl = tiles[k];
-		const GsFlowInst f = fi[tl.inst];
-		const int d = f.base + (tl.tt * f.nJ + tl.tj) * f.nI + tl.ti;
-		const GsInst in = insts[tl.inst];
-		const bool run = !in.active || *in.active;
-		// the coefficients do not change during a level: into LDS before the
-		// wait for the predecessors (the values follow after it)
-		if (run) tile_load_dev<S, false, 1>(tile_ctx<S>(tl, in, smem), in);
-		if (threadIdx.x < 7) {
-			const int a = (threadIdx.x + 1) & 1, b = (threadIdx.x + 1) >> 1 & 1;
-			const int cc = (threadIdx.x + 1) >> 2;
-			if (tl.ti >= a && tl.tj >= b && tl.tt >= cc) {
-				int *p = flags + d - a - b * f.nI - cc * f.nI * f.nJ;
-				int n = 0;
-				while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-						< epoch) {
-					if (++n > spin_max) {
-						atomicOr(err, 1);
-						break;
-					}
-					__builtin_amdgcn_s_sleep(2);
-				}
-			}
-		}
-		__syncthreads();
-		if (run) gs_tile_run<S, true>(tl, in, smem);
-		__builtin_amdgcn_s_waitcnt(0);   // this wave's stores have completed
-		__syncthreads();
-		if (threadIdx.x == 0)
-			__hip_atomic_store(flags + d, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-	}
+	tile_flow_loop<S>(tiles, ntiles, insts, fi, claim, flags, epoch, spin_max, err, smem,
+			[&](const GsTile &tl, const GsInst &in) { gs_tile_run<S, true>(tl, in, smem); });
 }
 
 int gs_flow_blocks(int R, int var)
@@ -419,12 +359,17 @@ int gs_flow_blocks(int R, int var)
 
 __global__ void k_gs_flow_reset(unsigned *claim) { *claim = 0; }
 
+void launch_flow_reset(unsigned *claim, hipStream_t s)
+{
+	k_gs_flow_reset<<<1, 1, 0, s>>>(claim);
+}
+
 void launch_gs_flow(int R, int var, const GsTile *tiles, int ntiles, int nblocks,
 		const GsInst *insts, const GsFlowInst *fi, unsigned *claim, int *flags,
 		int epoch, int spin_max, int *err, hipStream_t s)
 {
 	if (!ntiles) return;
-	k_gs_flow_reset<<<1, 1, 0, s>>>(claim);
+	launch_flow_reset(claim, s);
 	sp_with_shape(R, var, [&](auto sh) {
 		using S = decltype(sh);
 		k_gs_flow<S><<<nblocks, Geo<S>::NT, Geo<S>::LDS_FLOATS * 4, s>>>(tiles, ntiles,

@@ -11,7 +11,7 @@
 //
 // usage: gs_emulate W H T tstep seed holefrac withf mode [variant]
 //   mode: tiles | flow | small | cells; variant: tile shape variant of the radius (sp_gs.h)
-//        gs_emulate amle W H T nn seed holefrac [variant]
+//        gs_emulate amle W H T nn seed holefrac [variant [flow]]
 //   the AMLE sweeps (sp_amle.h) against the oracle's raster-order sweeps
 //   (oracle/amle_oracle.c); variant: AMLE tile shape of radius amle_radius(nn)
 // prints "OK <n>" (n = holes) or "MISMATCH ..." / "CONFLICT ..." and exits 1
@@ -115,8 +115,9 @@ static int run_small(float *u, const float *c, int W, int H, int T, float t)
 // in which every existing predecessor (I'-a, J'-b, T'-c), a, b, c in
 // {0, 1}, has stored before a tile loads; up to 64 tiles are in flight
 // between their load and their store, finished in random order.
-template <class S>
-static int run_flow(float *u, const float *c, int W, int H, int T, float t, unsigned seed)
+template <class S, class Step>
+static int run_flow(float *u, const float *c, int W, int H, int T, float t, unsigned seed,
+		Step step)
 {
 	using G = Geo<S>;
 	std::vector<GsTile> tiles;
@@ -168,15 +169,8 @@ static int run_flow(float *u, const float *c, int W, int H, int T, float t, unsi
 		const size_t z = rnd((unsigned)fly.size());
 		Fly *f = fly[z];
 		fly.erase(fly.begin() + z);
-		if (S::R == 1) {
-			std::vector<Gs1Thread> th(G::NT);
-			for (int tid = 0; tid < G::NT; tid++) th[tid] = gs1_thread<S>(f->ctx, tid);
-			for (int sg = 0; sg < G::NSTEP; sg++)
-				for (int tid = 0; tid < G::NT; tid++) gs1_step<S>(f->ctx, th[tid], sg);
-		} else {
-			for (int sg = 0; sg < G::NSTEP; sg++)
-				for (int tid = 0; tid < G::NT; tid++) tile_step<S>(f->ctx, tid, sg);
-		}
+		for (int sg = 0; sg < G::NSTEP; sg++)
+			for (int tid = 0; tid < G::NT; tid++) step(f->ctx, tid, sg);
 		for (int tid = 0; tid < G::NT; tid++) tile_store<S>(f->ctx, in, tid);
 		st[cell(f->tl.ti, f->tl.tj, f->tl.tt)] = 2;
 		delete f;
@@ -230,15 +224,16 @@ static int run_cells(float *u, const float *c, int W, int H, int T, float t)
 // the reference does
 static int main_amle(int argc, char **argv)
 {
-	if (argc != 8 && argc != 9) {
-		fprintf(stderr, "usage: gs_emulate amle W H T nn seed holefrac [variant]\n");
+	if (argc < 8 || argc > 10) {
+		fprintf(stderr, "usage: gs_emulate amle W H T nn seed holefrac [variant [flow]]\n");
 		return 2;
 	}
+	const bool flow = argc == 10 && !strcmp(argv[9], "flow");
 	const int W = atoi(argv[2]), H = atoi(argv[3]), T = atoi(argv[4]);
 	const int nn = atoi(argv[5]);
 	rng_state = (unsigned)atoi(argv[6]) * 2654435761u + 11;
 	const double frac = atof(argv[7]);
-	const int var = argc == 9 ? atoi(argv[8]) : 0;
+	const int var = argc >= 9 ? atoi(argv[8]) : 0;
 	const size_t n = (size_t)W * H;
 	std::vector<float> u(n), uref(n), c(n);
 	std::vector<unsigned char> hole(n);
@@ -256,25 +251,27 @@ static int main_amle(int argc, char **argv)
 	orc_amle_sweeps(uref.data(), hole.data(), W, H, nn, T);
 	int rc = 0;
 	const bool fast4 = nn == 4 && (W > 1 || H > 1);   // the device's dispatch
+	const unsigned fseed = (unsigned)atoi(argv[6]);
 	if (!amle_with_shape(amle_radius(nn), var, [&](auto sh) {
 			using S = decltype(sh);
+			auto go = [&](auto step) {
+				rc = flow ? run_flow<S>(u.data(), c.data(), W, H, T, 0.0f, fseed, step)
+					: run_tiles<S>(u.data(), c.data(), W, H, T, 0.0f, step);
+			};
 			if constexpr (S::R == 1) {
 				if (fast4) {
 					// per-thread state of every tile, made at step 0
 					std::vector<std::vector<Amle4Thread>> st;
-					rc = run_tiles<S>(u.data(), c.data(), W, H, T, 0.0f,
-						[&st](const TileCtx<S> &cc, int tid, int sg) {
-							if (sg == 0 && tid == 0) st.emplace_back(Geo<S>::NT);
-							auto &v = st.back();
-							(void)cc;
-							if (sg == 0) v[tid] = amle4_thread<S>(cc, tid);
-							amle4_step<S>(cc, v[tid], sg); });
+					go([&st](const TileCtx<S> &cc, int tid, int sg) {
+						if (sg == 0 && tid == 0) st.emplace_back(Geo<S>::NT);
+						auto &v = st.back();
+						if (sg == 0) v[tid] = amle4_thread<S>(cc, tid);
+						amle4_step<S>(cc, v[tid], sg); });
 					return;
 				}
 			}
-			rc = run_tiles<S>(u.data(), c.data(), W, H, T, 0.0f,
-				[nn](const TileCtx<S> &cc, int tid, int sg) {
-					amle_tile_step<S>(cc, tid, sg, nn); }); })) {
+			go([nn](const TileCtx<S> &cc, int tid, int sg) {
+				amle_tile_step<S>(cc, tid, sg, nn); }); })) {
 		printf("NO SHAPE %d for nn %d\n", var, nn);
 		return 2;
 	}
@@ -339,23 +336,23 @@ int main(int argc, char **argv)
 		: R == 2 ? run_small<2>(u, c, W, H, T, teff) : run_small<3>(u, c, W, H, T, teff);
 	else if (!sp_with_shape(R, var, [&](auto sh) {
 			using S = decltype(sh);
-			if (flow) {
-				rc = run_flow<S>(u, c, W, H, T, teff, (unsigned)atoi(argv[5]));
-				return;
-			}
+			const unsigned fseed = (unsigned)atoi(argv[5]);
+			auto go = [&](auto step) {
+				rc = flow ? run_flow<S>(u, c, W, H, T, teff, fseed, step)
+					: run_tiles<S>(u, c, W, H, T, teff, step);
+			};
 			if constexpr (S::R == 1) {
 				// the device's R = 1 loop: per-thread state made at step 0
+				// (a tile's steps run together, tile after tile)
 				std::vector<std::vector<Gs1Thread>> st;
-				rc = run_tiles<S>(u, c, W, H, T, teff,
-					[&st](const TileCtx<S> &cc, int tid, int sg) {
-						if (sg == 0 && tid == 0) st.emplace_back(Geo<S>::NT);
-						auto &v = st.back();
-						if (sg == 0) v[tid] = gs1_thread<S>(cc, tid);
-						gs1_step<S>(cc, v[tid], sg); });
+				go([&st](const TileCtx<S> &cc, int tid, int sg) {
+					if (sg == 0 && tid == 0) st.emplace_back(Geo<S>::NT);
+					auto &v = st.back();
+					if (sg == 0) v[tid] = gs1_thread<S>(cc, tid);
+					gs1_step<S>(cc, v[tid], sg); });
 				return;
 			}
-			rc = run_tiles<S>(u, c, W, H, T, teff,
-				[](const TileCtx<S> &cc, int tid, int sg) { tile_step<S>(cc, tid, sg); }); })) {
+			go([](const TileCtx<S> &cc, int tid, int sg) { tile_step<S>(cc, tid, sg); }); })) {
 		printf("NO SHAPE %d for radius %d\n", var, R);
 		return 2;
 	}

@@ -2,14 +2,17 @@
 SIMPOIS_HIP_GS_FLOW (0: one launch per wavefront of tiles; otherwise one
 dependence-ordered launch per level) is read when the library loads, so the
 parent sets it in this process's environment.  Runs Gauss-Seidel solves of
-each radius, a batch of planes and a multiscale pyramid against the oracle
-and prints one line per case."""
+each radius, a batch of planes, a multiscale pyramid and AMLE fills (which
+take the same launch forms) against the oracle and prints one line per
+case."""
 import os
 import sys
 
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
-from helpers import bits_equal, oracle_solve
+import numpy as np
+
+from helpers import bits_equal, oracle_amle, oracle_solve
 from test_gpu_parity import random_case
 
 
@@ -25,6 +28,16 @@ def main():
         ok = bits_equal(out, ref)
         print(f"case {pd}x{h}x{w} t={t} n={n} s={s}: {'OK' if ok else 'MISMATCH'}",
               flush=True)
+        if not ok:
+            return 1
+    # AMLE on the same tile schedules (4- and 8-neighbour, one and two planes)
+    for k, (pd, nn) in enumerate([(1, 4), (2, 4), (1, 8)]):
+        rng = np.random.default_rng(90 + k)
+        x = (rng.standard_normal((pd, 260, 330)) * 20).astype(np.float32)
+        x[rng.random(x.shape) < 0.7] = np.nan
+        y = m.amle(x, nn, 30)
+        ok = all(bits_equal(y[l], oracle_amle(x[l], nn, 30)) for l in range(pd))
+        print(f"amle {pd}x260x330 nn={nn}: {'OK' if ok else 'MISMATCH'}", flush=True)
         if not ok:
             return 1
     return 0

@@ -207,6 +207,20 @@ def test_amle_tiles_match_raster_sweeps(emulator, args):
     assert r.returncode == 0 and r.stdout.startswith("OK"), r.stdout + r.stderr
 
 
+# the dependence-ordered launch of the AMLE sweeps: random tile orders
+# (each tile waits only for its existing predecessors) on every radius
+AMLE_FLOW_CASES = ["150 140 20 4 3 0.5 0", "150 140 20 4 103 0.5 6",
+                   "120 100 15 8 4 0.5 0", "100 90 12 16 5 0.5 0",
+                   "90 80 9 32 6 0.5 0", "97 83 9 2 7 0.5 0", "1 1 5 4 8 1 0"]
+
+
+@pytest.mark.parametrize("args", AMLE_FLOW_CASES)
+def test_dependence_ordered_amle_tiles_match_raster_sweeps(emulator, args):
+    r = subprocess.run([emulator, "amle", *args.split(), "flow"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.startswith("OK"), r.stdout + r.stderr
+
+
 @pytest.mark.parametrize("var", range(15))
 def test_every_amle_tile_shape_matches_raster_sweeps(emulator, var):
     r = subprocess.run([emulator, "amle", "150", "140", "40", "4", str(var), "0.6",
