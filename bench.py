@@ -763,10 +763,14 @@ def run_amle(a):
                       "output": "bit-identical to the reference amle (tsup); the "
                                 "lower bound tinf, which feeds only the stderr "
                                 "trace, is not swept"},
-           "roofline": {"kernel": ("k_amle4_tile" if nn == 4 else "k_amle_tile") +
-                                  f" (tile {shape} = BIxBJxBT: BT "
-                                  "raster sweeps fused per pass, one launch per "
-                                  "wavefront of skewed tiles)",
+           "roofline": {"kernel": (("k_amle4_tile" if nn == 4 else "k_amle_tile") +
+                                   f" (tile {shape} = BIxBJxBT: BT raster sweeps fused "
+                                   "per pass, one launch per wavefront of skewed tiles)"
+                                   if not gs_flow() else
+                                   f"k_amle_flow<{'4-neighbour' if nn == 4 else 'table'}> "
+                                   f"(tile {shape} = BIxBJxBT: BT raster sweeps fused per "
+                                   "pass; one launch per pass, each skewed tile started "
+                                   "once its predecessors have stored)"),
                         "bound": "hbm", "achieved": round(achieved, 1),
                         "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                         "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
@@ -921,6 +925,12 @@ def stream_copy_ceiling(torch, sp, nbytes=1 << 30, reps=10):
             "bytes_per_launch": 2 * nbytes, "avg_launch_us": round(ms * 1e3, 2),
             "GBps": round(gbps, 1), "frac_of_peak": round(gbps / PEAK_HBM_GBPS, 4),
             "copy_checked": ok}
+
+
+def gs_flow():
+    """Whether the library runs skewed tiles in dependence-ordered launches
+    (SIMPOIS_HIP_GS_FLOW, read when it loads; on unless set to 0)."""
+    return os.environ.get("SIMPOIS_HIP_GS_FLOW", "1") != "0"
 
 
 def main():
@@ -1167,9 +1177,12 @@ def run_c2(a, method, full):
         shape = ([c[1:].split(":")[0] for c in finest.split("choice=")[1].split(",")
                   if c.startswith("*")] or ["32x128x16"])[0] if finest else "32x128x16"
         result["roofline"] = {
-            "kernel": f"k_gs_flow<1> (tile {shape} = BIxBJxBT: BT lexicographic "
-                      "sweeps fused per pass; one launch per level, each skewed "
-                      "tile started once its predecessors have stored)",
+            "kernel": (f"k_gs_flow<1> (tile {shape} = BIxBJxBT: BT lexicographic "
+                       "sweeps fused per pass; one launch per level, each skewed "
+                       "tile started once its predecessors have stored)" if gs_flow() else
+                       f"k_gs_tile<1> (tile {shape} = BIxBJxBT: BT lexicographic "
+                       "sweeps fused per pass, one launch per wavefront of skewed "
+                       "tiles)"),
             "bound": "hbm", "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBPS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
