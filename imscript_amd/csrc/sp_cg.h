@@ -39,6 +39,8 @@ struct Cg32Scal {
 	                   // direction kernel writes the solution to u
 	int status;        // why the plane stopped: 0 still running, else
 	                   // SP_PLANE_CONVERGED / _MAXIT / _BREAKDOWN / _NONFINITE
+	double alpha_prev; // the alpha before the current one (the fused
+	                   // direction kernel adds alpha p to x in pairs)
 };
 
 // Fused multigrid tile kernels: levels 0 and 1 run one of the fine tile

@@ -508,6 +508,7 @@ __global__ __launch_bounds__(RT) void k32_reduce_a(const Cg32Plane *pl, Cg32Scal
 	fold(part, pl[blockIdx.x], 2, 0, red, &pq);
 	if (threadIdx.x == 0) {
 		Cg32Scal &s = sc[blockIdx.x];
+		s.alpha_prev = s.alpha;
 		// breakdown (p.Ap = 0 or not finite once float round-off has taken
 		// over): keep the current x and stop this plane
 		if (!(pq > 0) || !isfinite(s.rz / pq)) {
@@ -637,7 +638,8 @@ __global__ __launch_bounds__(NT) void k32_dir(const BlockWork *bw,
 }
 
 // Direction and product in one pass (Poisson operator):
-//   x += alpha p_old            when this iteration updated r (upd)
+//   x += alpha p_old            when this iteration updated r (upd), in
+//                               pairs: see below
 //   p_new = z + beta p_old      while the plane goes on (first: p_new = z)
 //   q = A p_new, partial p_new.q
 // p_new of the unknown neighbours is recomputed from their z and p_old
@@ -646,6 +648,14 @@ __global__ __launch_bounds__(NT) void k32_dir(const BlockWork *bw,
 // buffers, p (par 1) or p2 (par 0) receiving p_new.  Every value is taken
 // under the unknown bits, so known pixels and groups without unknowns may
 // hold anything.
+//
+// x is read and written every other pass only.  Pass T >= 1 (par 0 for T
+// odd, 1 for T even) owes x the term alpha_{T-1} p_{T-1} (p_old).  An odd
+// pass leaves it pending; an even pass adds the pending alpha_{T-2} p_{T-2}
+// (still in the buffer it is about to overwrite with p_new, read first)
+// and then its own term, the same two double additions in the same order
+// as one per pass, so x and u keep their bits.  The pass that stops a plane
+// adds what is owed and writes u (x is not read after that).
 static __device__ __forceinline__ float pnew(float z, float po, float be, bool first)
 {
 	return first ? z : z + be * po;
@@ -668,8 +678,12 @@ __global__ __launch_bounds__(NT) void k32_dirspmv(const BlockWork *bw,
 	const float *po = par ? P.p2 : P.p;
 	float *pw_ = par ? P.p : P.p2;
 	const float be = (float)s.beta;
-	const double ad = s.alpha;
+	const double ad = s.alpha, ad2 = s.alpha_prev;
 	const bool fst = first;
+	// even pass: the pending term comes in (pair), x is loaded, stored while
+	// the plane goes on; odd pass: x only for the stopping pass
+	const bool pair = par && !fst && (upd || fin);
+	const bool ldx = pair || fin, stx = pair && !fin;
 	double acc = 0;
 	const unsigned n = (unsigned)P.w * (unsigned)P.h, w = (unsigned)P.w;
 	const unsigned q4 = (unsigned)b.start + 4 * threadIdx.x;
@@ -683,7 +697,7 @@ __global__ __launch_bounds__(NT) void k32_dirspmv(const BlockWork *bw,
 		if (cc & 0x01010101u) {
 			const unsigned qn = q4 >= w ? q4 - w : q4, qs = q4 + w + 3 < n ? q4 + w : q4;
 			const unsigned qe = q4 + 4 < n ? q4 + 4 : q4, qw = q4 > 0 ? q4 - 1 : q4;
-			f4 poc = z0, on = z0, os = z0, zc = z0, zn = z0, zs = z0;
+			f4 poc = z0, on = z0, os = z0, zc = z0, zn = z0, zs = z0, pp2 = z0;
 			float oe = 0.0f, ow = 0.0f, ze = 0.0f, zw = 0.0f;
 			d4 x = {0.0, 0.0, 0.0, 0.0};
 			if (!fst) {
@@ -695,7 +709,8 @@ __global__ __launch_bounds__(NT) void k32_dirspmv(const BlockWork *bw,
 					ow = gload<float>(po + qw);
 				}
 			}
-			if (upd || fin) x = gload<d4>(P.x + q4);
+			if (pair) pp2 = gload<f4>(pw_ + q4);
+			if (ldx) x = gload<d4>(P.x + q4);
 			if (act) {
 				zc = gload<f4>(P.z + q4);
 				zn = gload<f4>(P.z + qn);
@@ -707,8 +722,10 @@ __global__ __launch_bounds__(NT) void k32_dirspmv(const BlockWork *bw,
 			// the stores: a use of loaded values after a branch would make
 			// the compiler wait for every access in flight, stores included
 #pragma unroll
-			for (int k = 0; k < 4; k++)
-				if (upd && ((cc >> (8 * k)) & 1u)) x[k] = x[k] + ad * (double)poc[k];
+			for (int k = 0; k < 4; k++) {
+				if (pair && ((cc >> (8 * k)) & 1u)) x[k] = x[k] + ad2 * (double)pp2[k];
+				if (upd && ldx && ((cc >> (8 * k)) & 1u)) x[k] = x[k] + ad * (double)poc[k];
+			}
 			f4 pc, pn, ps;
 #pragma unroll
 			for (int k = 0; k < 4; k++) {
@@ -736,7 +753,7 @@ __global__ __launch_bounds__(NT) void k32_dirspmv(const BlockWork *bw,
 				gstore<f4>(pw_ + q4, pc);
 				gstore<f4>(P.q + q4, y);
 			}
-			if (upd) gstore<d4>(P.x + q4, x);
+			if (stx) gstore<d4>(P.x + q4, x);
 			// the solution: u may be the caller's buffer, 4-byte stores at
 			// the unknowns only
 			if (fin)
@@ -749,14 +766,20 @@ __global__ __launch_bounds__(NT) void k32_dirspmv(const BlockWork *bw,
 		const unsigned cc = code4(P.code, q4);
 		if (cc & 0x01010101u) {
 			const f4 poc = fst ? z0 : *(const f4 *)(po + q4);
-			if (upd || fin) {
+			if (ldx) {
 				d4 x = *(const d4 *)(P.x + q4);
+				if (pair) {
+					const f4 pp2 = *(const f4 *)(pw_ + q4);   // before p_new replaces it
+#pragma unroll
+					for (int k = 0; k < 4; k++)
+						if ((cc >> (8 * k)) & 1u) x[k] = x[k] + ad2 * (double)pp2[k];
+				}
 				if (upd) {
 #pragma unroll
 					for (int k = 0; k < 4; k++)
 						if ((cc >> (8 * k)) & 1u) x[k] = x[k] + ad * (double)poc[k];
-					*(d4 *)(P.x + q4) = x;
 				}
+				if (stx) *(d4 *)(P.x + q4) = x;
 				if (fin)
 #pragma unroll
 					for (int k = 0; k < 4; k++)
@@ -802,8 +825,13 @@ __global__ __launch_bounds__(NT) void k32_dirspmv(const BlockWork *bw,
 				continue;
 			}
 			const float o = fst ? 0.0f : po[q];
-			if (upd) P.x[q] = P.x[q] + ad * (double)o;
-			if (fin) P.u[q] = (float)P.x[q];
+			if (ldx) {
+				double x = P.x[q];
+				if (pair) x = x + ad2 * (double)pw_[q];   // before p_new replaces it
+				if (upd) x = x + ad * (double)o;
+				if (stx) P.x[q] = x;
+				if (fin) P.u[q] = (float)x;
+			}
 			if (!act) continue;
 			const float pc = pnew(P.z[q], o, be, fst);
 			float nb = 0.0f;
