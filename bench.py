@@ -44,11 +44,11 @@ PEAK_HBM_GBPS = 8000.0
 # measured on: TRAFFIC_SRC is the SHA-256 prefix of imscript_amd/csrc/sp_cg.hip
 # then, and the line reports traffic null when the source differs.
 TRAFFIC_PMC = {
-    "mg_up0": 634.1e6,
+    "mg_up0": 633.9e6,
     "mg_down0": 495.3e6,
-    "k32_dirspmv": 861.3e6,
+    "k32_dirspmv": 703.3e6,
 }
-TRAFFIC_SRC = "a6356c2e85292914"
+TRAFFIC_SRC = "53d9b747b6fcb701"
 
 
 def kernel_source_id(name="sp_cg.hip"):
@@ -1067,10 +1067,12 @@ def run_c2(a, method, full):
     gs_ms, gs_launch, gs_bytes = sp.profile_read("gs_tile")
     kern = {k: sp.profile_read(k) for k in ("mg_up0", "mg_down0", "k32_dirspmv")}
     sp.profile(False)
-    # the fused direction + product kernel per unknown: reads z, p_old, x
-    # (double) and the code, writes p_new, q, x (33 B)
+    # the fused direction + product kernel per unknown: reads z, p_old and
+    # the code, writes p_new and q (17 B); every other pass also reads x
+    # (double) and the p of two passes before and writes x (20 B): 27 B on
+    # average
     ms_, n_, _ = kern["k32_dirspmv"]
-    kern["k32_dirspmv"] = (ms_, n_, 33.0 * hole_frac * a.size * a.size * a.channels * n_)
+    kern["k32_dirspmv"] = (ms_, n_, 27.0 * hole_frac * a.size * a.size * a.channels * n_)
 
     mpix = a.size * a.size * a.channels / 1e6
     result = {

@@ -2188,8 +2188,8 @@ void cg32_iter(const Cg32Batch &B, hipStream_t s, int par)
 	k32_reduce_b<<<B.nplanes, RT, 0, s>>>(B.planes, B.scal, B.part, B.nactive,
 			B.trace, pc);
 	if (pc) precondition(B, 0, s);
-	// reads z, the code, p_old (with the four neighbours) and x, writes
-	// p_new, q and x: 29 B per unknown
+	// reads z, the code and p_old (with the four neighbours), writes p_new
+	// and q; every other pass also x and the p of two passes before
 	if (B.prof) B.prof("k32_dirspmv", 1, 0, s);
 	k32_dirspmv<<<B.nblk, NT, 0, s>>>(B.bw, B.planes, B.scal, B.part, par, 0, B.nblk);
 	if (B.prof) B.prof("k32_dirspmv", 0, 0.0, s);
