@@ -123,8 +123,15 @@ static const int g_gs_flow = [] {
 }();
 // polls of one predecessor flag before k_gs_flow gives up (a poll is a
 // read from memory and a 128-cycle sleep, about 1 us: some 0.25 s in all,
-// far beyond the few tile run-times a predecessor can take)
-static constexpr int GS_FLOW_SPIN = 1 << 18;
+// far beyond the few tile run-times a predecessor can take).
+// SIMPOIS_HIP_GS_FLOW_SPIN=n (read once) sets it, for tests of the error
+// path: with n = 0 any wait that is not met at once gives up.
+static const int GS_FLOW_SPIN = [] {
+	const char *e = getenv("SIMPOIS_HIP_GS_FLOW_SPIN");
+	if (!e || !*e) return 1 << 18;
+	const long v = strtol(e, nullptr, 10);
+	return v < 0 ? 0 : v > (1L << 30) ? 1 << 30 : (int)v;
+}();
 
 // SIMPOIS_HIP_TAIL_LDS=0 keeps every multigrid tail in global memory
 static const int g_tail_lds_off = [] {

@@ -375,6 +375,21 @@ def test_tile_launch_forms_against_oracle(flow):
     assert p.returncode == 0, p.stdout.decode()[-2000:] + p.stderr.decode()[-2000:]
 
 
+def test_dependence_wait_timeout_is_reported():
+    """With the wait bound forced to 0 polls (SIMPOIS_HIP_GS_FLOW_SPIN, read
+    when the library loads, so in a child process) tiles give up on their
+    predecessors: the Gauss-Seidel solve returns an error and AMLE reports
+    one on a later call.  Every kernel still ends normally."""
+    env = dict(os.environ, SIMPOIS_HIP_GS_FLOW_SPIN="0")
+    env.pop("SIMPOIS_HIP_GS_FLOW", None)
+    child = os.path.join(os.path.dirname(__file__), "gs_flow_err_child.py")
+    print("gs flow error child", flush=True)
+    p = subprocess.run([sys.executable, child], env=env, capture_output=True,
+                       timeout=300)
+    print(p.stdout.decode(), flush=True)
+    assert p.returncode == 0, p.stdout.decode()[-2000:] + p.stderr.decode()[-2000:]
+
+
 @pytest.mark.parametrize("h,w", [(2600, 2700), (1100, 40000)])
 def test_multigrid_fused_coarse_levels(h, w, tmp_path):
     """Planes whose multigrid levels 5 (2600x2700: 82x85) and 6 (1100x40000:
