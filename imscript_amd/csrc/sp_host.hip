@@ -999,7 +999,8 @@ static int cg64_graphs(Plan &pl, int bilap)
 		if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
 	size_t n0 = 0;
 	for (auto &p : pl.planes) n0 = std::max(n0, (size_t)p.w * p.h);
-	pl.cg64_grid = (int)std::min<size_t>(1024, std::max<size_t>(1, (n0 + 1023) / 1024));
+	const size_t span = (size_t)spk::cg_iter_span();
+	pl.cg64_grid = (int)std::min<size_t>(1024, std::max<size_t>(1, (n0 + span - 1) / span));
 	if (!pl.cap_stream)
 		HIPCHK(hipStreamCreateWithFlags(&pl.cap_stream, hipStreamNonBlocking));
 	for (int i = 0; i < 3; i++) {
@@ -1035,7 +1036,8 @@ static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 	if (k == 0) pl.status_finest[plane] = SP_PLANE_MAXIT;
 	if (maxit <= 0) return SP_OK;   // x = x0: u is unchanged
 	// iteration kernels: 4 unknowns per thread and pass, at most 1024 blocks
-	const int nblk = std::min(1024, std::max(1, (nmask + 1023) / 1024));
+	const int span = spk::cg_iter_span();
+	const int nblk = std::min(1024, std::max(1, (nmask + span - 1) / span));
 	TRY(cg64_ensure(pl, maxit));
 	spk::CgDev d;
 	d.g = S.g; d.f = S.f; d.u = S.u;

@@ -310,6 +310,7 @@ struct CgDev {
 // them.  An iteration launch's grid may exceed d.nblk (extra blocks return).
 int cg_setup_blocks(int w, int h);
 void launch_cg_setup(const CgDev *dp, int nb, hipStream_t s);
+int cg_iter_span();   // unknowns per block of the iteration kernels
 void launch_cg_iter(const CgDev *dp, int bilap, int grid, hipStream_t s);
 void launch_cg_finish(const CgDev *dp, int nblk, hipStream_t s);
 

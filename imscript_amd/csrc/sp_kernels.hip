@@ -524,6 +524,11 @@ void launch_gs_small(int R, const GsInst *insts, const int *ids, int nids,
 // reference's float-returning getter does; dot products are double.
 
 constexpr int CG_NT = 256;
+// threads of the iteration kernels' blocks (4 unknowns each per pass):
+// 512 measured against 256 and 1024 on the 2048^2 biharmonic fill (18.7
+// against 19.7 ms per solve: half the partial sums of 256, more blocks
+// per CU than 1024)
+constexpr int CG_IT = 512;
 
 // stencil of A at (i,j) over values val(q) (float), Poisson = -(5-point),
 // biharmonic = 13-point, in the reference's term order
@@ -559,13 +564,14 @@ static __device__ __forceinline__ double cg_stencil(int bilap, int i, int j,
 }
 
 // sum of the nblk partials at part[0..nblk), same fixed tree in every block
+template <int NT = CG_NT>
 static __device__ double sum_partials(const double *part, int nblk, double *red)
 {
 	double v = 0;
 	// unrolled so that the loads issue together; the adds keep their order
 #pragma unroll 4
-	for (int k = threadIdx.x; k < nblk; k += CG_NT) v += gload<double>(part + k);
-	double s = block_sum_dbl<CG_NT>(v, red);
+	for (int k = threadIdx.x; k < nblk; k += NT) v += gload<double>(part + k);
+	double s = block_sum_dbl<NT>(v, red);
 	__shared__ double bc;
 	if (threadIdx.x == 0) bc = s;
 	__syncthreads();
@@ -573,17 +579,18 @@ static __device__ double sum_partials(const double *part, int nblk, double *red)
 }
 
 // sums of a[0..n) and b[0..n), each in exactly the order sum_partials uses
+template <int NT = CG_NT>
 static __device__ double2 sum_partials2(const double *a, const double *b, int n,
 		double *red)
 {
 	double va = 0, vb = 0;
 #pragma unroll 4
-	for (int k = threadIdx.x; k < n; k += CG_NT) {
+	for (int k = threadIdx.x; k < n; k += NT) {
 		va += gload<double>(a + k);
 		vb += gload<double>(b + k);
 	}
-	const double sa = block_sum_dbl<CG_NT>(va, red);
-	const double sb = block_sum_dbl<CG_NT>(vb, red);
+	const double sa = block_sum_dbl<NT>(va, red);
+	const double sb = block_sum_dbl<NT>(vb, red);
 	__shared__ double2 bc;
 	if (threadIdx.x == 0) bc = make_double2(sa, sb);
 	__syncthreads();
@@ -769,19 +776,19 @@ static __device__ __forceinline__ double cg_apply_taps(const float (&t)[CgTaps<B
 
 // q = A p, partial p.q
 template <int BILAP, int U>
-__global__ __launch_bounds__(CG_NT) void k_cg_spmv(const CgDev *dp)
+__global__ __launch_bounds__(CG_IT) void k_cg_spmv(const CgDev *dp)
 {
 	const CgDev d = *dp;
-	__shared__ double red[CG_NT / 64];
+	__shared__ double red[CG_IT / 64];
 	// the launch may have more blocks than this level uses (a graph's grid
 	// is fixed); a block's whole work is skipped, so no barrier is split
 	if (blockIdx.x >= (unsigned)d.nblk || cg_stopped(d)) return;
 	constexpr int NT = CgTaps<BILAP>::N;
 	const int w = d.w, h = d.h;
-	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_NT;
+	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_IT;
 	const float *pf = d.pf;
 	double acc = 0;
-	for (unsigned k0 = blockIdx.x * CG_NT + threadIdx.x; k0 < nc; k0 += U * S) {
+	for (unsigned k0 = blockIdx.x * CG_IT + threadIdx.x; k0 < nc; k0 += U * S) {
 		bool on[U];
 		unsigned qq[U];
 		double pc[U];
@@ -815,7 +822,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_spmv(const CgDev *dp)
 			acc += y * pc[u];
 		}
 	}
-	double s = block_sum_dbl<CG_NT>(acc, red);
+	double s = block_sum_dbl<CG_IT>(acc, red);
 	if (threadIdx.x == 0) d.part[blockIdx.x] = s;
 	// the first kernel of an iteration advances the index the next two read
 	if (blockIdx.x == 0 && threadIdx.x == 0) d.done[2] += 1;
@@ -824,17 +831,17 @@ __global__ __launch_bounds__(CG_NT) void k_cg_spmv(const CgDev *dp)
 // alpha = rr_old / p.q ; r -= alpha q ; partial r.r.  The vectors' loads
 // issue before the partials are summed, so their latency overlaps the sum.
 template <int U>
-__global__ __launch_bounds__(CG_NT) void k_cg_update(const CgDev *dp)
+__global__ __launch_bounds__(CG_IT) void k_cg_update(const CgDev *dp)
 {
 	const CgDev d = *dp;
-	__shared__ double red[CG_NT / 64];
+	__shared__ double red[CG_IT / 64];
 	__shared__ double alpha_s;
 	if (blockIdx.x >= (unsigned)d.nblk || cg_stopped(d)) return;
 	const int it = gload<int>(d.done + 2);
 	const double rr0 = gload<double>(d.rr + it);
-	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_NT;
+	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_IT;
 	double acc = 0;
-	for (unsigned k0 = blockIdx.x * CG_NT + threadIdx.x, first = 1; ; k0 += U * S, first = 0) {
+	for (unsigned k0 = blockIdx.x * CG_IT + threadIdx.x, first = 1; ; k0 += U * S, first = 0) {
 		// every thread of the block takes part in the first pass (the
 		// partials' sum has barriers), later passes only while in range
 		if (!first && k0 >= nc) break;
@@ -849,7 +856,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_update(const CgDev *dp)
 			}
 		}
 		if (first) {
-			const double pq = sum_partials(d.part, d.nblk, red);
+			const double pq = sum_partials<CG_IT>(d.part, d.nblk, red);
 			if (threadIdx.x == 0) alpha_s = rr0 / pq;
 			__syncthreads();
 		}
@@ -864,7 +871,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_update(const CgDev *dp)
 		}
 		if (k0 + U * S >= nc) break;
 	}
-	double s = block_sum_dbl<CG_NT>(acc, red);
+	double s = block_sum_dbl<CG_IT>(acc, red);
 	if (threadIdx.x == 0) d.part[d.nblk + blockIdx.x] = s;
 }
 
@@ -873,16 +880,16 @@ __global__ __launch_bounds__(CG_NT) void k_cg_update(const CgDev *dp)
 // its float copy.  x is updated here, where p is read anyway, also in the
 // iteration that stops.
 template <int U>
-__global__ __launch_bounds__(CG_NT) void k_cg_dir(const CgDev *dp)
+__global__ __launch_bounds__(CG_IT) void k_cg_dir(const CgDev *dp)
 {
 	const CgDev d = *dp;
-	__shared__ double red[CG_NT / 64];
+	__shared__ double red[CG_IT / 64];
 	__shared__ double alpha_s, rr1_s;
 	if (blockIdx.x >= (unsigned)d.nblk || cg_stopped(d)) return;
 	const int it = gload<int>(d.done + 2);
-	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_NT;
+	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_IT;
 	const double rr0 = gload<double>(d.rr + it);
-	for (unsigned k0 = blockIdx.x * CG_NT + threadIdx.x, first = 1; ; k0 += U * S, first = 0) {
+	for (unsigned k0 = blockIdx.x * CG_IT + threadIdx.x, first = 1; ; k0 += U * S, first = 0) {
 		if (!first && k0 >= nc) break;
 		double rv[U], pv[U], xv[U];
 		unsigned qq[U];
@@ -897,7 +904,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_dir(const CgDev *dp)
 			}
 		}
 		if (first) {
-			const double2 t = sum_partials2(d.part, d.part + d.nblk, d.nblk, red);
+			const double2 t = sum_partials2<CG_IT>(d.part, d.part + d.nblk, d.nblk, red);
 			if (threadIdx.x == 0) { alpha_s = rr0 / t.x; rr1_s = t.y; }
 			__syncthreads();
 		}
@@ -948,12 +955,14 @@ void launch_cg_setup(const CgDev *dp, int nb, hipStream_t s)
 	k_cg_rr0<<<1, CG_NT, 0, s>>>(dp);
 }
 
+int cg_iter_span() { return 4 * CG_IT; }
+
 void launch_cg_iter(const CgDev *dp, int bilap, int grid, hipStream_t s)
 {
-	if (bilap) k_cg_spmv<1, 4><<<grid, CG_NT, 0, s>>>(dp);
-	else k_cg_spmv<0, 4><<<grid, CG_NT, 0, s>>>(dp);
-	k_cg_update<4><<<grid, CG_NT, 0, s>>>(dp);
-	k_cg_dir<4><<<grid, CG_NT, 0, s>>>(dp);
+	if (bilap) k_cg_spmv<1, 4><<<grid, CG_IT, 0, s>>>(dp);
+	else k_cg_spmv<0, 4><<<grid, CG_IT, 0, s>>>(dp);
+	k_cg_update<4><<<grid, CG_IT, 0, s>>>(dp);
+	k_cg_dir<4><<<grid, CG_IT, 0, s>>>(dp);
 }
 
 void launch_cg_finish(const CgDev *dp, int nblk, hipStream_t s)
