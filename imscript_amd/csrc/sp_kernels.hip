@@ -529,6 +529,7 @@ constexpr int CG_NT = 256;
 // against 19.7 ms per solve: half the partial sums of 256, more blocks
 // per CU than 1024)
 constexpr int CG_IT = 512;
+constexpr int CG_ITU = 4;   // unknowns per thread and pass (2: 19.2, 8: 20.2 ms)
 
 // stencil of A at (i,j) over values val(q) (float), Poisson = -(5-point),
 // biharmonic = 13-point, in the reference's term order
@@ -955,14 +956,14 @@ void launch_cg_setup(const CgDev *dp, int nb, hipStream_t s)
 	k_cg_rr0<<<1, CG_NT, 0, s>>>(dp);
 }
 
-int cg_iter_span() { return 4 * CG_IT; }
+int cg_iter_span() { return CG_ITU * CG_IT; }
 
 void launch_cg_iter(const CgDev *dp, int bilap, int grid, hipStream_t s)
 {
-	if (bilap) k_cg_spmv<1, 4><<<grid, CG_IT, 0, s>>>(dp);
-	else k_cg_spmv<0, 4><<<grid, CG_IT, 0, s>>>(dp);
-	k_cg_update<4><<<grid, CG_IT, 0, s>>>(dp);
-	k_cg_dir<4><<<grid, CG_IT, 0, s>>>(dp);
+	if (bilap) k_cg_spmv<1, CG_ITU><<<grid, CG_IT, 0, s>>>(dp);
+	else k_cg_spmv<0, CG_ITU><<<grid, CG_IT, 0, s>>>(dp);
+	k_cg_update<CG_ITU><<<grid, CG_IT, 0, s>>>(dp);
+	k_cg_dir<CG_ITU><<<grid, CG_IT, 0, s>>>(dp);
 }
 
 void launch_cg_finish(const CgDev *dp, int nblk, hipStream_t s)
