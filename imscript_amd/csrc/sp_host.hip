@@ -419,10 +419,12 @@ struct Plan {
 	int cg_rr_cap = 0;
 	int *cg_done = nullptr, *h_done = nullptr;
 	spk::CgDev *cg_dev = nullptr;       // descriptor the FP64 CG kernels read
-	// FP64 CG iterations captured in chunks of 16 << i (i = 0..2), shared by
-	// every level and plane (they read *cg_dev); built on first use
-	hipGraphExec_t cg64_exec[3] = {};
-	int cg64_bilap = -1, cg64_grid = 0;
+	// FP64 CG iterations captured in chunks of 16 << i (i = 0..2), one set
+	// per operator and grid (the work blocks of a level's unknowns), shared
+	// by the levels and planes of that grid (they read *cg_dev); built on
+	// first use
+	struct Cg64Graphs { int bilap, grid; hipGraphExec_t exec[3]; };
+	std::vector<Cg64Graphs> cg64;
 	// pinned staging of small per-solve copies (descriptors in, results out):
 	// copies to and from pageable memory each cost a host round trip
 	void *h_stage = nullptr;
@@ -457,8 +459,9 @@ struct Plan {
 	{
 		for (auto &L : lv)
 			if (L.cg.exec) (void)hipGraphExecDestroy(L.cg.exec);
-		for (auto &e : cg64_exec)
-			if (e) (void)hipGraphExecDestroy(e);
+		for (auto &g : cg64)
+			for (auto &e : g.exec)
+				if (e) (void)hipGraphExecDestroy(e);
 		for (auto &a : allocs) dev_free(a.first, a.second);
 		if (h_done) (void)hipHostFree(h_done);
 		if (h_gs_err) (void)hipHostFree(h_gs_err);
@@ -989,32 +992,33 @@ static int cg64_ensure(Plan &pl, int maxit)
 	return SP_OK;
 }
 
-// The FP64 CG's captured chunks: 16, 32 and 64 iterations over a
-// grid that covers the plan's largest plane (a level with fewer blocks
-// leaves the rest idle)
-static int cg64_graphs(Plan &pl, int bilap)
+// The FP64 CG's captured chunks: 16, 32 and 64 iterations on grid work
+// blocks (a level's own count: a grid sized for the largest plane left
+// most blocks of the smaller counts returning at once, three launches per
+// iteration)
+static int cg64_graphs(Plan &pl, int bilap, int grid, hipGraphExec_t **out)
 {
-	if (pl.cg64_bilap == bilap) return SP_OK;
-	for (auto &e : pl.cg64_exec)
-		if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
-	size_t n0 = 0;
-	for (auto &p : pl.planes) n0 = std::max(n0, (size_t)p.w * p.h);
-	const size_t span = (size_t)spk::cg_iter_span();
-	pl.cg64_grid = (int)std::min<size_t>(1024, std::max<size_t>(1, (n0 + span - 1) / span));
+	for (auto &g : pl.cg64)
+		if (g.bilap == bilap && g.grid == grid) { *out = g.exec; return SP_OK; }
 	if (!pl.cap_stream)
 		HIPCHK(hipStreamCreateWithFlags(&pl.cap_stream, hipStreamNonBlocking));
+	Plan::Cg64Graphs G{bilap, grid, {}};
 	for (int i = 0; i < 3; i++) {
 		hipGraph_t graph;
 		HIPCHK(hipStreamBeginCapture(pl.cap_stream, hipStreamCaptureModeThreadLocal));
 		for (int z = 0; z < (16 << i); z++)
-			spk::launch_cg_iter(pl.cg_dev, bilap, pl.cg64_grid, pl.cap_stream);
+			spk::launch_cg_iter(pl.cg_dev, bilap, grid, pl.cap_stream);
 		HIPCHK(hipStreamEndCapture(pl.cap_stream, &graph));
-		const hipError_t ie = hipGraphInstantiate(&pl.cg64_exec[i], graph, nullptr,
-				nullptr, 0);
+		const hipError_t ie = hipGraphInstantiate(&G.exec[i], graph, nullptr, nullptr, 0);
 		(void)hipGraphDestroy(graph);
-		HIPCHK(ie);
+		if (ie != hipSuccess) {
+			for (auto &e : G.exec)
+				if (e) (void)hipGraphExecDestroy(e);
+			HIPCHK(ie);
+		}
 	}
-	pl.cg64_bilap = bilap;
+	pl.cg64.push_back(G);
+	*out = pl.cg64.back().exec;
 	return SP_OK;
 }
 
@@ -1048,7 +1052,8 @@ static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 	d.nb = spk::cg_setup_blocks(S.w, S.h);
 	d.nc = nmask;
 	d.stop = (double)e;
-	TRY(cg64_graphs(pl, bilap));
+	hipGraphExec_t *exec = nullptr;
+	TRY(cg64_graphs(pl, bilap, nblk, &exec));
 	// the descriptor into device memory, in stream order (no copy in
 	// flight uses the staging area: the hole count above was waited for)
 	TRY(pl.stage(sizeof d));
@@ -1067,7 +1072,7 @@ static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 		int c;
 		if ((16 << ci) <= maxit - it) {
 			c = 16 << ci;
-			HIPCHK(hipGraphLaunch(pl.cg64_exec[ci], s));
+			HIPCHK(hipGraphLaunch(exec[ci], s));
 			if (ci < 2) ci++;
 		} else {
 			c = maxit - it;
