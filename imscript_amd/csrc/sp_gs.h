@@ -455,14 +455,21 @@ SP_HD long small_nsteps(int R, int W, int H, int T)
 // four neighbours from the level's values in LDS, inside a frame of NaN
 // (skipped as non-finite, as in the tiles).  Same updates, same order of
 // terms, same results as the raster sweeps.
+//
+// A sweep that leaves every unknown's bits unchanged is a fixed point: each
+// later sweep reads the same values and repeats it, so the level's values
+// are final from there on and the remaining steps are skipped (gsc_final).
+// Coarse pyramid levels reach one long before their last sweep (a level
+// without unknowns at its first).
 
 constexpr int GSC_MAXCELLS = 16384;   // pixels of a level: <= 8192 per parity
 constexpr int GSC_MAXNT = 1024;
 
-// LDS bytes: the framed values, the list of unknowns and 16 scan words
+// LDS bytes: the framed values, the list of unknowns, 16 scan words and
+// the two change marks
 SP_HD long gsc_lds_bytes(int W, int H)
 {
-	return 4L * ((long)(W + 2) * (H + 2) + (long)W * H + 16);
+	return 4L * ((long)(W + 2) * (H + 2) + (long)W * H + 18);
 }
 
 // threads for a level: one per pixel of a parity (64..1024)
@@ -573,9 +580,12 @@ SP_HD float gsc_update(const float *U, int q, int P, float x, float cv, float t)
 	return x + t * (r - cv);
 }
 
+// the thread's updates of step s (parity PAR); true when one of them
+// changed its pixel's bits
 template <int MAXS, int PAR>
-SP_HD void gsc_step_par(GscThread<MAXS> &th, float *U, int P, int s, int T2, float t)
+SP_HD bool gsc_step_par(GscThread<MAXS> &th, float *U, int P, int s, int T2, float t)
 {
+	bool ch = false;
 #pragma unroll
 	for (int m = 0; m < MAXS; m++) {
 		if (m >= th.n[PAR]) break;
@@ -583,18 +593,30 @@ SP_HD void gsc_step_par(GscThread<MAXS> &th, float *U, int P, int s, int T2, flo
 		if ((unsigned)d >= (unsigned)T2) continue;
 		const int q = (int)(th.qij[PAR][m] & 0xffffu);
 		const float y = gsc_update(U, q, P, th.x[PAR][m], th.c[PAR][m], t);
+		ch |= sp_bits(y) != sp_bits(th.x[PAR][m]);
 		U[q] = y;
 		th.x[PAR][m] = y;
 	}
+	return ch;
 }
 
 // phase 5, step s of small_nsteps(1, W, H, T): steps run in increasing s
 // with a barrier between them
 template <int MAXS>
-SP_HD void gsc_step(GscThread<MAXS> &th, float *U, int W, int T, float t, int s)
+SP_HD bool gsc_step(GscThread<MAXS> &th, float *U, int W, int T, float t, int s)
 {
-	if (s & 1) gsc_step_par<MAXS, 1>(th, U, W + 2, s, 2 * T, t);
-	else gsc_step_par<MAXS, 0>(th, U, W + 2, s, 2 * T, t);
+	if (s & 1) return gsc_step_par<MAXS, 1>(th, U, W + 2, s, 2 * T, t);
+	return gsc_step_par<MAXS, 0>(th, U, W + 2, s, 2 * T, t);
+}
+
+// true when the level's values are final after step s, L being the last
+// step on which an update changed its pixel (-1: none so far): sweep k's
+// updates lie on steps [2k, 2k + W + H - 2], so the first sweep wholly
+// after L has ended and changed nothing (a fixed point).  Such a sweep is
+// always below the sweep count when s is a step of the schedule.
+SP_HD bool gsc_final(int L, int W, int H, int s)
+{
+	return ((L + 2) & ~1) + W + H - 2 <= s;
 }
 
 // phase 6: the thread's pixels back to the image

@@ -456,6 +456,9 @@ __global__ __launch_bounds__(GSC_MAXNT) void k_gs_cells(const GsInst *insts, con
 	float *U = smem;
 	int *L = (int *)(smem + (W + 2) * (H + 2));
 	int *wsum = L + W * H;
+	// the last step of each parity on which an update changed its pixel
+	int *mark = wsum + 16;
+	if (tid < 2) mark[tid] = -1;
 	gsc_load(U, in, tid, nt);
 	int tot;
 	const int pre = block_exscan(gsc_count(in, tid, nt), wsum, &tot);
@@ -465,15 +468,24 @@ __global__ __launch_bounds__(GSC_MAXNT) void k_gs_cells(const GsInst *insts, con
 	gsc_take<MAXS>(th, U, L, in, tid, nt, tot);
 	// even and odd steps in turn, each with its parity known at compile
 	// time (a step of either parity behind one branch made the compiler
-	// index the values by parity, through scratch memory)
+	// index the values by parity, through scratch memory).  An even step's
+	// mark is read during the next (odd) step and an odd step's during the
+	// next even one: each word is stable between the barrier after its
+	// writes and the barrier after the following step, so every thread
+	// reads the same values and leaves the loop at the same step.
 	const int ns = (int)small_nsteps(1, W, H, T);
+	const bool lead = (tid & 63) == 0;   // one writer per wave
+	int last = -1;
 	for (int s = 0; s < ns; s += 2) {
-		gsc_step_par<MAXS, 0>(th, U, W + 2, s, 2 * T, in.t);
+		if (__any(gsc_step_par<MAXS, 0>(th, U, W + 2, s, 2 * T, in.t)) && lead) mark[0] = s;
 		__syncthreads();
-		if (s + 1 < ns) {
-			gsc_step_par<MAXS, 1>(th, U, W + 2, s + 1, 2 * T, in.t);
-			__syncthreads();
-		}
+		last = max(last, __builtin_amdgcn_readfirstlane(mark[0]));
+		if (gsc_final(last, W, H, s) || s + 1 >= ns) break;
+		if (__any(gsc_step_par<MAXS, 1>(th, U, W + 2, s + 1, 2 * T, in.t)) && lead)
+			mark[1] = s + 1;
+		__syncthreads();
+		last = max(last, __builtin_amdgcn_readfirstlane(mark[1]));
+		if (gsc_final(last, W, H, s + 1)) break;
 	}
 	gsc_store<MAXS>(th, in);
 }

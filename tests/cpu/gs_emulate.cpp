@@ -211,9 +211,17 @@ static int run_cells(float *u, const float *c, int W, int H, int T, float t)
 	for (int tid = 0; tid < nt; tid++) gsc_fill(L, in, tid, nt, pre[tid], tot);
 	std::vector<GscThread<MAXS>> th(nt);
 	for (int tid = 0; tid < nt; tid++) gsc_take<MAXS>(th[tid], U, L, in, tid, nt, tot);
+	// the device's loop, left at the first step after which the values are
+	// final (a sweep that changed no pixel)
 	const long ns = small_nsteps(1, W, H, T);
-	for (long s = 0; s < ns; s++)
-		for (int tid = 0; tid < nt; tid++) gsc_step<MAXS>(th[tid], U, W, T, t, (int)s);
+	long s = 0;
+	for (int last = -1; s < ns; s++) {
+		bool ch = false;
+		for (int tid = 0; tid < nt; tid++) ch |= gsc_step<MAXS>(th[tid], U, W, T, t, (int)s);
+		if (ch) last = (int)s;
+		if (gsc_final(last, W, H, (int)s)) break;
+	}
+	fprintf(stderr, "CELLS LEFT AT STEP %ld OF %ld\n", s < ns ? s + 1 : ns, ns);
 	for (int tid = 0; tid < nt; tid++) gsc_store<MAXS>(th[tid], in);
 	return 0;
 }

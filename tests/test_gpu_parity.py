@@ -176,6 +176,37 @@ def test_whole_level_by_unknown_pixel_against_oracle(h, w, hole, n, seed):
     assert bits_equal(out, ref)
 
 
+def _smooth_discs(h, w, seed, ndisc, rmax):
+    """A smooth field with ndisc NaN discs of radius 1..1+rmax."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
+    g = (100 + 60 * np.sin(x / 9 + rng.random() * 6)
+         * np.cos(y / 13 + rng.random() * 6)).astype(np.float32)
+    for _ in range(ndisc):
+        cy, cx, r = rng.random() * h, rng.random() * w, 1 + rng.random() * rmax
+        g[(y - cy) ** 2 + (x - cx) ** 2 < r * r] = np.nan
+    return g[None]
+
+
+@pytest.mark.parametrize("h,w,nd,n,seed", [
+    (128, 128, 20, 2000, 71),    # bitwise fixed point at sweep 278
+    (64, 64, 8, 2000, 73),       # at sweep 131
+    (40, 30, 0, 50, 62),         # no unknown pixel: final after one sweep
+])
+def test_whole_level_fixed_point_against_oracle(h, w, nd, n, seed):
+    """Laplace fills of small smooth levels whose sweeps reach a fixed
+    point (a sweep that changes no pixel's bits), where the whole-level
+    kernel stops early: bit-identical to all n of the oracle's raster
+    sweeps.  (Random values at isolated holes cycle in the last bit
+    instead and run every sweep; the other whole-level cases cover them.)"""
+    m = _imscript()
+    g = _smooth_discs(h, w, seed, nd, 6)
+    print(f"fixed point {h}x{w} discs {nd} n={n}", flush=True)
+    ref, _ = oracle_solve(g, None, 0.25, n, 1, 0)
+    out = m.simpois(g, None, tstep=0.25, niter=n, scale=1, cgit=0, trace=False)
+    assert bits_equal(out, ref)
+
+
 def test_device_api_matches_host_api_and_is_deterministic():
     import torch
     m = _imscript()

@@ -154,6 +154,27 @@ def test_whole_level_kernel_by_unknown_pixel(emulator, args):
     assert p.stdout.startswith("OK")
 
 
+# a fixed point (a sweep that changes no pixel's bits) ends the level's
+# steps early; the values still match all T raster sweeps bit for bit.
+# W H T ..., and the step the level is left at: a level without unknowns
+# after its first sweep's W + H - 1 steps, a Laplace fill of 20x20 after
+# 163 of its 6037 steps
+CELLS_FIXED_CASES = [
+    ("40 30 50 0.25 1 0.0 1 cells", 69, 167),
+    ("20 20 3000 0.25 2 0.3 0 cells", 163, 6037),
+]
+
+
+@pytest.mark.parametrize("args,left,total", CELLS_FIXED_CASES)
+def test_whole_level_kernel_stops_at_a_fixed_point(emulator, args, left, total):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    p = subprocess.run([emulator, *args.split()], capture_output=True,
+                       text=True, timeout=600, env=env)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert p.stdout.startswith("OK")
+    assert f"CELLS LEFT AT STEP {left} OF {total}" in p.stderr
+
+
 def test_whole_level_kernel_by_unknown_pixel_limits(emulator):
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
     p = subprocess.run([emulator, *"129 128 5 0.25 1 0.4 1 cells".split()],
