@@ -121,9 +121,16 @@ static const int g_gs_flow = [] {
 	const char *e = getenv("SIMPOIS_HIP_GS_FLOW");
 	return e && !strcmp(e, "0") ? 0 : 1;
 }();
+// SIMPOIS_HIP_CG_PERSIST=0 (read once) runs the FP64 CG's iterations as
+// three launches each (captured in graphs) instead of one persistent launch
+static const int g_cg_persist = [] {
+	const char *e = getenv("SIMPOIS_HIP_CG_PERSIST");
+	return e && !strcmp(e, "0") ? 0 : 1;
+}();
 // polls of one predecessor flag before k_gs_flow gives up (a poll is a
 // read from memory and a 128-cycle sleep, about 1 us: some 0.25 s in all,
-// far beyond the few tile run-times a predecessor can take).
+// far beyond the few tile run-times a predecessor can take); also the bound
+// of each grid barrier wait of the persistent FP64 CG.
 // SIMPOIS_HIP_GS_FLOW_SPIN=n (read once) sets it, for tests of the error
 // path: with n = 0 any wait that is not met at once gives up.
 static const int GS_FLOW_SPIN = [] {
@@ -412,12 +419,14 @@ struct Plan {
 	// FP64 CG scratch, sized for the largest plane
 	float *cg_q = nullptr;
 	double *cg_x = nullptr, *cg_r = nullptr, *cg_p = nullptr,
-		*cg_part = nullptr, *cg_rr = nullptr;
+		*cg_part = nullptr, *cg_rr = nullptr, *cg_rrp = nullptr;
 	unsigned *cg_idx = nullptr;         // pixel of each unknown of the plane solved
 	int *cg_bcnt = nullptr;             // unknowns per setup block, prefix sums
 	float *cg_pf = nullptr;             // direction rounded to float (full grid)
 	int cg_rr_cap = 0;
+	int cg_cap[2] = {-1, -1};           // resident k_cg_persist work-groups (-1: not asked)
 	int *cg_done = nullptr, *h_done = nullptr;
+	unsigned *cg_bar = nullptr;         // k_cg_persist's barrier words
 	spk::CgDev *cg_dev = nullptr;       // descriptor the FP64 CG kernels read
 	// FP64 CG iterations captured in chunks of 16 << i (i = 0..2), one set
 	// per operator and grid (the work blocks of a level's unknowns), shared
@@ -787,7 +796,7 @@ static int plan_create(Plan &pl, const std::vector<PlaneIn> &planes,
 		for (int p = 0; p < np; p++)
 			if (pl.depth[p] > k) { pl.lv[k].members.push_back(p); ncnt++; }
 	TRY(pl.alloc(&pl.d_cnt, ncnt));
-	HIPCHK(hipHostMalloc((void **)&pl.h_done, 4 * sizeof(int)));
+	HIPCHK(hipHostMalloc((void **)&pl.h_done, 8 * sizeof(int)));
 	pl.traces.assign(np, std::vector<CgTrace>(nlev));
 	pl.cg_iters_finest.assign(np, 0);
 	pl.status_finest.assign(np, SP_PLANE_SWEPT);
@@ -981,7 +990,9 @@ static int cg64_ensure(Plan &pl, int maxit)
 		TRY(pl.alloc(&pl.cg_bcnt, (size_t)nb0 + 1));
 		TRY(pl.alloc(&pl.cg_pf, n0));
 		TRY(pl.alloc(&pl.cg_part, std::max<size_t>(2 * 1024, nb0)));
-		TRY(pl.alloc(&pl.cg_done, 4));
+		TRY(pl.alloc(&pl.cg_done, 8));
+		TRY(pl.alloc(&pl.cg_bar, 640));
+		TRY(pl.alloc(&pl.cg_rrp, 2));
 		TRY(pl.alloc(&pl.cg_dev, 1));
 	}
 	if (maxit + 2 > pl.cg_rr_cap) {
@@ -1007,7 +1018,7 @@ static int cg64_graphs(Plan &pl, int bilap, int grid, hipGraphExec_t **out)
 		hipGraph_t graph;
 		HIPCHK(hipStreamBeginCapture(pl.cap_stream, hipStreamCaptureModeThreadLocal));
 		for (int z = 0; z < (16 << i); z++)
-			spk::launch_cg_iter(pl.cg_dev, bilap, grid, pl.cap_stream);
+			spk::launch_cg_iter(pl.cg_dev, bilap, grid, z & 1, pl.cap_stream);
 		HIPCHK(hipStreamEndCapture(pl.cap_stream, &graph));
 		const hipError_t ie = hipGraphInstantiate(&G.exec[i], graph, nullptr, nullptr, 0);
 		(void)hipGraphDestroy(graph);
@@ -1047,13 +1058,19 @@ static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 	d.g = S.g; d.f = S.f; d.u = S.u;
 	d.x = pl.cg_x; d.r = pl.cg_r; d.p = pl.cg_p; d.q = pl.cg_q;
 	d.idx = pl.cg_idx; d.bcnt = pl.cg_bcnt; d.pf = pl.cg_pf;
-	d.part = pl.cg_part; d.rr = pl.cg_rr; d.done = pl.cg_done;
+	d.part = pl.cg_part; d.rr = pl.cg_rr; d.rrp = pl.cg_rrp; d.done = pl.cg_done;
+	d.bar = pl.cg_bar;
 	d.w = S.w; d.h = S.h; d.bilap = bilap; d.nblk = nblk;
 	d.nb = spk::cg_setup_blocks(S.w, S.h);
 	d.nc = nmask;
 	d.stop = (double)e;
+	// the iterations in one persistent launch when the unknowns take one
+	// pass of its work-groups and those can all be resident
+	if (pl.cg_cap[bilap] < 0) pl.cg_cap[bilap] = spk::cg_persist_capacity(bilap);
+	const bool persist = g_cg_persist && (long)nmask <= (long)nblk * span
+		&& nblk <= pl.cg_cap[bilap];
 	hipGraphExec_t *exec = nullptr;
-	TRY(cg64_graphs(pl, bilap, nblk, &exec));
+	if (!persist) TRY(cg64_graphs(pl, bilap, nblk, &exec));
 	// the descriptor into device memory, in stream order (no copy in
 	// flight uses the staging area: the hole count above was waited for)
 	TRY(pl.stage(sizeof d));
@@ -1061,12 +1078,29 @@ static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 	HIPCHK(hipMemcpyAsync(pl.cg_dev, pl.h_stage, sizeof d, hipMemcpyHostToDevice, s));
 	ProfScope pr("cg64", s);
 	spk::launch_cg_setup(pl.cg_dev, d.nb, s);
-	// chunks of 16, 32, then 64 iterations from the captured graphs while a
-	// whole chunk fits under maxit, the rest launched one iteration at a
-	// time; the stop flag is read after each chunk (iterations after the
-	// stop return at once: 64 keeps those few against one host round trip
-	// per chunk)
 	int it = 0, ci = 0, ran = maxit;
+	bool persisted = false;
+	if (persist) {
+		if (spk::launch_cg_persist(pl.cg_dev, bilap, nblk, maxit, GS_FLOW_SPIN, s) != 0) {
+			(void)hipGetLastError();   // refused: the launches below instead
+			pl.cg_cap[bilap] = 0;
+			TRY(cg64_graphs(pl, bilap, nblk, &exec));
+		} else {
+			HIPCHK(hipMemcpyAsync(pl.h_done, pl.cg_done, 6 * sizeof(int),
+					hipMemcpyDeviceToHost, s));
+			HIPCHK(hipStreamSynchronize(s));
+			if (pl.h_done[5])
+				return set_err(SP_ERR_HIP, "FP64 CG grid barrier wait timed out");
+			it = maxit;
+			persisted = true;
+			if (pl.h_done[0]) ran = pl.h_done[1];
+		}
+	}
+	// otherwise chunks of 16, 32, then 64 iterations from the captured
+	// graphs while a whole chunk fits under maxit, the rest launched one
+	// iteration at a time; the stop flag is read after each chunk
+	// (iterations after the stop return at once: 64 keeps those few against
+	// one host round trip per chunk)
 	while (it < maxit) {
 		while (ci > 0 && (16 << ci) > maxit - it) ci--;
 		int c;
@@ -1077,7 +1111,7 @@ static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 		} else {
 			c = maxit - it;
 			for (int z = 0; z < c; z++)
-				spk::launch_cg_iter(pl.cg_dev, bilap, nblk, s);
+				spk::launch_cg_iter(pl.cg_dev, bilap, nblk, (it + z) & 1, s);
 		}
 		it += c;
 		HIPCHK(hipGetLastError());
@@ -1089,7 +1123,7 @@ static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 	spk::launch_cg_finish(pl.cg_dev, nblk, s);
 	HIPCHK(hipGetLastError());
 	// FP64 vectors: 88 B per unknown and iteration (twice the FP32 vectors)
-	pr.add(3L * it, 88.0 * ran * nmask);
+	pr.add(persisted ? 1 : 3L * it, 88.0 * ran * nmask);
 	CgTrace &tr = pl.traces[L.members[m]][k];
 	tr.n = nmask;
 	tr.rr.resize(ran);

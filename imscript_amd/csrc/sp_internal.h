@@ -297,8 +297,12 @@ struct CgDev {
 	                      // the stencil reads), 0 at known pixels
 	double *part;         // per-block partial sums (max(2 * nblk, nb))
 	double *rr;           // rr history, rr[0] = r0.r0, rr[k+1] = after iter k
-	int *done;            // [0] = stop flag, [1] = iterations run, [2] = index
-	                      // of the iteration under way
+	double *rrp;          // [k & 1] = r.r before iteration k (2 entries)
+	int *done;            // [0] = stop flag (for the host), [1] = iterations
+	                      // run, [2] = index of the iteration under way,
+	                      // [3 + (k & 1)] = stop flag iteration k reads,
+	                      // [5] = k_cg_persist's error word (8 entries)
+	unsigned *bar;        // k_cg_persist's barrier words (640 entries)
 	int w, h, bilap;
 	int nblk;             // blocks of the iteration kernels
 	int nb;               // setup blocks: cg_setup_blocks(w, h)
@@ -308,10 +312,17 @@ struct CgDev {
 // The kernels read the descriptor from device memory (dp), so captured
 // iterations serve every level and plane: the host rewrites *dp between
 // them.  An iteration launch's grid may exceed d.nblk (extra blocks return).
+// Iteration k is launched with par = k & 1 (chunks of even length keep the
+// parity of their nodes).
 int cg_setup_blocks(int w, int h);
 void launch_cg_setup(const CgDev *dp, int nb, hipStream_t s);
 int cg_iter_span();   // unknowns per block of the iteration kernels
-void launch_cg_iter(const CgDev *dp, int bilap, int grid, hipStream_t s);
+void launch_cg_iter(const CgDev *dp, int bilap, int grid, int par, hipStream_t s);
+// all iterations in one cooperative launch of nblk work-groups (nonzero:
+// the launch was refused); capacity = work-groups that can be resident
+int cg_persist_capacity(int bilap);
+int launch_cg_persist(const CgDev *dp, int bilap, int nblk, int maxit, int spin_max,
+		hipStream_t s);
 void launch_cg_finish(const CgDev *dp, int nblk, hipStream_t s);
 
 // adjoint of the converged Poisson fill

@@ -10,6 +10,7 @@
 #include "sp_internal.h"
 #include <math.h>
 #include <algorithm>
+#include <type_traits>
 
 namespace spk {
 
@@ -591,25 +592,6 @@ static __device__ double sum_partials(const double *part, int nblk, double *red)
 	return bc;
 }
 
-// sums of a[0..n) and b[0..n), each in exactly the order sum_partials uses
-template <int NT = CG_NT>
-static __device__ double2 sum_partials2(const double *a, const double *b, int n,
-		double *red)
-{
-	double va = 0, vb = 0;
-#pragma unroll 4
-	for (int k = threadIdx.x; k < n; k += NT) {
-		va += gload<double>(a + k);
-		vb += gload<double>(b + k);
-	}
-	const double sa = block_sum_dbl<NT>(va, red);
-	const double sb = block_sum_dbl<NT>(vb, red);
-	__shared__ double2 bc;
-	if (threadIdx.x == 0) bc = make_double2(sa, sb);
-	__syncthreads();
-	return bc;
-}
-
 // The FP64 CG keeps its vectors compact, one entry per unknown in raster
 // order (as the reference's mask3 list does, cleant_cgpois.c:6-25): the
 // setup lists the unknowns (idx), and the iteration kernels stream over that
@@ -731,15 +713,25 @@ __global__ __launch_bounds__(CG_NT) void k_cg_rr0(const CgDev *dp)
 	double s = sum_partials(d.part, d.nb, red);
 	if (threadIdx.x == 0) {
 		d.rr[0] = s;
+		d.rrp[0] = s;
 		d.done[0] = 0;
 		d.done[1] = 0;
 		d.done[2] = -1;   // iteration index, advanced by each k_cg_spmv
+		d.done[3] = 0;
+		d.done[4] = 0;
+		d.done[5] = 0;   // k_cg_persist's error word and barrier words
+		for (int k = 0; k < 10; k++) d.bar[64 * k] = 0;
 	}
 }
 
-static __device__ __forceinline__ bool cg_stopped(const CgDev &d)
+// The stop flag of iteration k's kernels is done[3 + (k & 1)]: k_cg_dir of
+// the stopping iteration k sets done[3 + ((k + 1) & 1)], and k_cg_spmv of
+// iteration k + 1, finding it set, sets done[3 + (k & 1)].  So no kernel
+// writes the word it reads (every block of the stopping k_cg_dir still
+// updates x, however late it starts), and all threads read one value.
+static __device__ __forceinline__ bool cg_stopped(const CgDev &d, int par)
 {
-	return __hip_atomic_load((__attribute__((address_space(1))) int *)d.done,
+	return __hip_atomic_load((__attribute__((address_space(1))) int *)d.done + 3 + par,
 			__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
@@ -787,37 +779,63 @@ static __device__ __forceinline__ double cg_apply_taps(const float (&t)[CgTaps<B
 	return (double)r;
 }
 
+// a relaxed atomic load (4 or 8 bytes): an ordered access, which the
+// compiler neither moves behind a later branch nor drops
+template <typename T>
+static __device__ __forceinline__ T cg_aload(const T *p)
+{
+	static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4 or 8 bytes");
+	using B = typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type;
+	const B v = __hip_atomic_load((__attribute__((address_space(1))) B *)p,
+			__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	return __builtin_bit_cast(T, v);
+}
+
 // q = A p, partial p.q
 template <int BILAP, int U>
-__global__ __launch_bounds__(CG_IT) void k_cg_spmv(const CgDev *dp)
+__global__ __launch_bounds__(CG_IT) void k_cg_spmv(const CgDev *dp, int par)
 {
 	const CgDev d = *dp;
 	__shared__ double red[CG_IT / 64];
 	// the launch may have more blocks than this level uses (a graph's grid
 	// is fixed); a block's whole work is skipped, so no barrier is split
-	if (blockIdx.x >= (unsigned)d.nblk || cg_stopped(d)) return;
+	if (blockIdx.x >= (unsigned)d.nblk) return;
+	const bool stopped = cg_stopped(d, par);
 	constexpr int NT = CgTaps<BILAP>::N;
 	const int w = d.w, h = d.h;
 	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_IT;
 	const float *pf = d.pf;
-	double acc = 0;
-	for (unsigned k0 = blockIdx.x * CG_IT + threadIdx.x; k0 < nc; k0 += U * S) {
-		bool on[U];
-		unsigned qq[U];
-		double pc[U];
+	unsigned k0 = blockIdx.x * CG_IT + threadIdx.x;
+	bool on[U];
+	unsigned qq[U];
+	double pc[U];
+	// unconditional loads at clamped indices, as atomic loads (see
+	// cg_part_load: plain ones were moved behind the stop test)
+	auto load = [&]() {
 #pragma unroll
 		for (int u = 0; u < U; u++) {
 			const unsigned k = k0 + u * S;
 			on[u] = k < nc;
-			if (on[u]) {
-				qq[u] = gload<unsigned>(d.idx + k);
-				pc[u] = gload<double>(d.p + k);
-			}
+			qq[u] = cg_aload(d.idx + min(k, nc - 1));
+			pc[u] = cg_aload(d.p + min(k, nc - 1));
 		}
+	};
+	// the first pass's list and p loads are in flight when the flag is
+	// tested
+	load();
+	if (stopped) {
+		// passed on to the next iteration's kernels (see cg_stopped)
+		if (blockIdx.x == 0 && threadIdx.x == 0) d.done[4 - par] = 1;
+		return;
+	}
+	double acc = 0;
+	for (;;) {
+		// every entry's taps, also those past the list's end (their qq is
+		// the last entry's, a valid pixel): loads under a per-entry branch
+		// made the compiler wait for one entry's taps before the next's
 		float t[U][NT];
 #pragma unroll
 		for (int u = 0; u < U; u++) {
-			if (!on[u]) continue;
 			const int j = (int)(qq[u] / (unsigned)w), i = (int)(qq[u] - (unsigned)j * w);
 #pragma unroll
 			for (int e = 0; e < NT; e++) {
@@ -834,46 +852,81 @@ __global__ __launch_bounds__(CG_IT) void k_cg_spmv(const CgDev *dp)
 			gstore<float>(d.q + k0 + u * S, (float)y);   // y is a float widened: exact
 			acc += y * pc[u];
 		}
+		k0 += U * S;
+		if (k0 >= nc) break;
+		load();
 	}
 	double s = block_sum_dbl<CG_IT>(acc, red);
 	if (threadIdx.x == 0) d.part[blockIdx.x] = s;
-	// the first kernel of an iteration advances the index the next two read
+	// the first kernel of an iteration advances the index k_cg_dir records
+	// the r.r history at
 	if (blockIdx.x == 0 && threadIdx.x == 0) d.done[2] += 1;
 }
 
-// alpha = rr_old / p.q ; r -= alpha q ; partial r.r.  The vectors' loads
-// issue before the partials are summed, so their latency overlaps the sum.
+// The partials of the previous kernel, two per thread (nblk <= 2 * CG_IT):
+// loaded first, summed later in sum_partials<CG_IT>'s order and tree.  The
+// loads are unconditional, at indices clamped into [0, nblk) (a load under a
+// branch made the compiler wait for it at the join), and the sum skips the
+// clamped ones.  They are relaxed atomic loads, which the compiler keeps
+// ahead of the stop test (it moved plain loads behind it).
+static __device__ __forceinline__ double cg_part_at(const double *p)
+{
+	return cg_aload(p);
+}
+static __device__ __forceinline__ void cg_part_load(const double *part, int nblk,
+		double (&pp)[2])
+{
+	const int k = (int)threadIdx.x;
+	pp[0] = cg_part_at(part + min(k, nblk - 1));
+	pp[1] = cg_part_at(part + min(k + CG_IT, nblk - 1));
+}
+static __device__ __forceinline__ double cg_part_sum(const double (&pp)[2], int nblk,
+		double *red)
+{
+	const int k = (int)threadIdx.x;
+	double v = 0;
+	if (k < nblk) v += pp[0];
+	if (k + CG_IT < nblk) v += pp[1];
+	return block_sum_dbl<CG_IT>(v, red);   // valid in thread 0
+}
+
+// The first pass of k_cg_update and k_cg_dir is written out before their
+// loop: the stop flag, r.r, the pass's vectors and the partials are all in
+// flight before the flag is tested (inside a loop the compiler tested the
+// flag ahead of the vector loads, one memory round trip more).
+
+// alpha = rr_old / p.q ; r -= alpha q ; partial r.r
 template <int U>
-__global__ __launch_bounds__(CG_IT) void k_cg_update(const CgDev *dp)
+__global__ __launch_bounds__(CG_IT) void k_cg_update(const CgDev *dp, int par)
 {
 	const CgDev d = *dp;
 	__shared__ double red[CG_IT / 64];
 	__shared__ double alpha_s;
-	if (blockIdx.x >= (unsigned)d.nblk || cg_stopped(d)) return;
-	const int it = gload<int>(d.done + 2);
-	const double rr0 = gload<double>(d.rr + it);
+	if (blockIdx.x >= (unsigned)d.nblk) return;
+	const bool stopped = cg_stopped(d, par);
+	const double rr0 = gload<double>(d.rrp + par);
 	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_IT;
-	double acc = 0;
-	for (unsigned k0 = blockIdx.x * CG_IT + threadIdx.x, first = 1; ; k0 += U * S, first = 0) {
-		// every thread of the block takes part in the first pass (the
-		// partials' sum has barriers), later passes only while in range
-		if (!first && k0 >= nc) break;
-		double rv[U];
-		float qv[U];   // widened at the use: a conversion here would wait
+	unsigned k0 = blockIdx.x * CG_IT + threadIdx.x;
+	double rv[U];
+	float qv[U];   // widened at the use: a conversion here would wait
+	auto load = [&]() {   // unconditional, clamped (as cg_part_load)
 #pragma unroll
 		for (int u = 0; u < U; u++) {
-			const unsigned k = k0 + u * S;
-			if (k < nc) {
-				rv[u] = gload<double>(d.r + k);
-				qv[u] = gload<float>(d.q + k);
-			}
+			const unsigned k = min(k0 + u * S, nc - 1);
+			rv[u] = gload<double>(d.r + k);
+			qv[u] = gload<float>(d.q + k);
 		}
-		if (first) {
-			const double pq = sum_partials<CG_IT>(d.part, d.nblk, red);
-			if (threadIdx.x == 0) alpha_s = rr0 / pq;
-			__syncthreads();
-		}
-		const double alpha = alpha_s;
+	};
+	load();
+	double pp[2];
+	cg_part_load(d.part, d.nblk, pp);
+	if (stopped) return;   // before any barrier, the whole block
+	const double pq = cg_part_sum(pp, d.nblk, red);
+	if (threadIdx.x == 0) alpha_s = rr0 / pq;
+	__syncthreads();
+	const double alpha = alpha_s;
+	double acc = 0;
+	for (;;) {
 #pragma unroll
 		for (int u = 0; u < U; u++) {
 			const unsigned k = k0 + u * S;
@@ -882,7 +935,9 @@ __global__ __launch_bounds__(CG_IT) void k_cg_update(const CgDev *dp)
 			gstore<double>(d.r + k, r);
 			acc += r * r;
 		}
-		if (k0 + U * S >= nc) break;
+		k0 += U * S;
+		if (k0 >= nc) break;
+		load();
 	}
 	double s = block_sum_dbl<CG_IT>(acc, red);
 	if (threadIdx.x == 0) d.part[d.nblk + blockIdx.x] = s;
@@ -893,37 +948,44 @@ __global__ __launch_bounds__(CG_IT) void k_cg_update(const CgDev *dp)
 // its float copy.  x is updated here, where p is read anyway, also in the
 // iteration that stops.
 template <int U>
-__global__ __launch_bounds__(CG_IT) void k_cg_dir(const CgDev *dp)
+__global__ __launch_bounds__(CG_IT) void k_cg_dir(const CgDev *dp, int par)
 {
 	const CgDev d = *dp;
 	__shared__ double red[CG_IT / 64];
 	__shared__ double alpha_s, rr1_s;
-	if (blockIdx.x >= (unsigned)d.nblk || cg_stopped(d)) return;
-	const int it = gload<int>(d.done + 2);
+	if (blockIdx.x >= (unsigned)d.nblk) return;
+	const bool stopped = cg_stopped(d, par);
+	// the history index, for block 0's record at the end
+	const int it = blockIdx.x == 0 ? gload<int>(d.done + 2) : 0;
 	const unsigned nc = (unsigned)d.nc, S = (unsigned)d.nblk * CG_IT;
-	const double rr0 = gload<double>(d.rr + it);
-	for (unsigned k0 = blockIdx.x * CG_IT + threadIdx.x, first = 1; ; k0 += U * S, first = 0) {
-		if (!first && k0 >= nc) break;
-		double rv[U], pv[U], xv[U];
-		unsigned qq[U];
+	const double rr0 = gload<double>(d.rrp + par);
+	unsigned k0 = blockIdx.x * CG_IT + threadIdx.x;
+	double rv[U], pv[U], xv[U];
+	unsigned qq[U];
+	auto load = [&]() {   // unconditional, clamped (as cg_part_load)
 #pragma unroll
 		for (int u = 0; u < U; u++) {
-			const unsigned k = k0 + u * S;
-			if (k < nc) {
-				rv[u] = gload<double>(d.r + k);
-				pv[u] = gload<double>(d.p + k);
-				xv[u] = gload<double>(d.x + k);
-				qq[u] = gload<unsigned>(d.idx + k);
-			}
+			const unsigned k = min(k0 + u * S, nc - 1);
+			rv[u] = gload<double>(d.r + k);
+			pv[u] = gload<double>(d.p + k);
+			xv[u] = gload<double>(d.x + k);
+			qq[u] = gload<unsigned>(d.idx + k);
 		}
-		if (first) {
-			const double2 t = sum_partials2<CG_IT>(d.part, d.part + d.nblk, d.nblk, red);
-			if (threadIdx.x == 0) { alpha_s = rr0 / t.x; rr1_s = t.y; }
-			__syncthreads();
-		}
-		const double alpha = alpha_s, rr1 = rr1_s;
-		const bool stop = sqrt(rr1) < d.stop;
-		const double beta = rr1 / rr0;
+	};
+	load();
+	double pa[2], pb[2];
+	cg_part_load(d.part, d.nblk, pa);
+	cg_part_load(d.part + d.nblk, d.nblk, pb);
+	if (stopped) return;   // before any barrier, the whole block
+	// p.q then r.r, each in sum_partials' order
+	const double spq = cg_part_sum(pa, d.nblk, red);
+	const double srr = cg_part_sum(pb, d.nblk, red);
+	if (threadIdx.x == 0) { alpha_s = rr0 / spq; rr1_s = srr; }
+	__syncthreads();
+	const double alpha = alpha_s, rr1 = rr1_s;
+	const bool stop = sqrt(rr1) < d.stop;
+	const double beta = rr1 / rr0;
+	for (;;) {
 #pragma unroll
 		for (int u = 0; u < U; u++) {
 			const unsigned k = k0 + u * S;
@@ -934,16 +996,229 @@ __global__ __launch_bounds__(CG_IT) void k_cg_dir(const CgDev *dp)
 			gstore<double>(d.p + k, pn);
 			gstore<float>(d.pf + qq[u], (float)pn);
 		}
-		if (k0 + U * S >= nc) break;
+		k0 += U * S;
+		if (k0 >= nc) break;
+		load();
 	}
 	if (blockIdx.x == 0 && threadIdx.x == 0) {
-		const double rr1 = rr1_s;
 		d.rr[it + 1] = rr1;
+		d.rrp[1 - par] = rr1;   // the next iteration's r.r (this one reads rrp[par])
 		if (sqrt(rr1) < d.stop) {
 			d.done[1] = it + 1;
-			__hip_atomic_store(d.done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			d.done[0] = 1;
+			d.done[4 - par] = 1;   // the next iteration's stop flag
 		}
 	}
+}
+
+// word 3 of a raw buffer resource on gfx9 (untyped 32-bit data, no
+// swizzle) and the cache policy bit sc1: an access coherent at agent scope,
+// the bit the compiler sets for cg_aload
+constexpr int CG_RSRC_WORD3 = 0x00020000;
+constexpr int CG_SC1 = 16;
+
+// a relaxed atomic store (4 or 8 bytes) at agent scope: coherent across the
+// XCDs' L2 caches, for values other work-groups of the launch read
+template <typename T>
+static __device__ __forceinline__ void cg_astore(T *p, T v)
+{
+	static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4 or 8 bytes");
+	using B = typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type;
+	__hip_atomic_store((__attribute__((address_space(1))) B *)p, __builtin_bit_cast(B, v),
+			__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid barrier of k_cg_persist, e-th (from 1): work-group b counts in at
+// group counter bar[64 * (b % 8)] (the group of one XCD: its blocks
+// i = b mod 8), the last of a group at the global counter bar[64 * 8], and
+// the last of all writes e to the release word bar[64 * 9], which the
+// work-groups poll (the words are 256 bytes apart: arrivals and polls do
+// not share a line).  Each work-group's waves have completed their stores
+// before it counts in.  A wait past spin_max polls, or one that finds *err
+// set, sets *err and returns false (the work-group leaves; so does every
+// other one at its next wait).
+static __device__ __forceinline__ bool cg_grid_sync(unsigned *bar, unsigned e, int nblk,
+		int *err, int spin_max)
+{
+	__shared__ int s_ok;
+	__builtin_amdgcn_s_waitcnt(0);   // this wave's stores have completed
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		const int g = (int)blockIdx.x & 7, ng = nblk < 8 ? nblk : 8;
+		const unsigned ing = (unsigned)(nblk / 8 + (g < nblk % 8));   // blocks in group g
+		typedef __attribute__((address_space(1))) unsigned gu;
+		const unsigned o = __hip_atomic_fetch_add((gu *)(bar + 64 * g), 1u,
+				__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (o == e * ing - 1) {
+			const unsigned o2 = __hip_atomic_fetch_add((gu *)(bar + 64 * 8), 1u,
+					__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			if (o2 == e * (unsigned)ng - 1) cg_astore(bar + 64 * 9, e);
+		}
+		int ok = 1, n = 0;
+		while ((int)(cg_aload(bar + 64 * 9) - e) < 0) {
+			if (++n > spin_max || ((n & 255) == 0 && cg_aload(err))) {
+				ok = 0;
+				cg_astore(err, 1);
+				break;
+			}
+			__builtin_amdgcn_s_sleep(1);
+		}
+		s_ok = ok;
+	}
+	__syncthreads();
+	return s_ok != 0;
+}
+
+// The FP64 CG's iterations in one launch whose work-groups are all resident
+// (a cooperative launch), for a level whose unknowns take one pass of the
+// iteration kernels (nc <= CG_ITU * nblk * CG_IT).  Each thread keeps its
+// entries' x, r and p in registers from the first iteration to the last;
+// only p's float copy (read by the neighbours' stencils) and the per-block
+// partial sums go through memory, with agent-scope coherent stores and
+// loads, and three grid barriers per iteration stand for the boundaries of
+// k_cg_spmv, k_cg_update and k_cg_dir.  Same entries per thread, same
+// arithmetic and the same partial sums in the same trees as those kernels:
+// the same iterates, r.r history and stop.  Writes x, the history and
+// done[0..1] as they do.  bar holds the barrier words and done[5] is the
+// error word, both cleared by k_cg_rr0.
+// (at most 128 registers: two work-groups resident per CU)
+template <int BILAP>
+__global__ __launch_bounds__(CG_IT) __attribute__((amdgpu_waves_per_eu(4)))
+void k_cg_persist(const CgDev *dp, int maxit, int spin_max)
+{
+	constexpr int U = CG_ITU, NT = CgTaps<BILAP>::N;
+	const CgDev d = *dp;
+	__shared__ double red[CG_IT / 64];
+	__shared__ double s_alpha, s_rr1;
+	const int nblk = d.nblk, w = d.w, h = d.h;
+	const unsigned nc = (unsigned)d.nc, S = (unsigned)nblk * CG_IT;
+	const unsigned k0 = blockIdx.x * CG_IT + threadIdx.x;
+	unsigned *bar = d.bar;
+	int *err = d.done + 5;
+	float *pf = d.pf;
+	// the stencils read pf through a buffer resource: a 32-bit byte offset
+	// per tap (one register per address instead of two; the plane has at
+	// most 2^30 pixels) and the agent-coherent cache policy of cg_aload
+	const unsigned long long pfb = (unsigned long long)w * (unsigned)h * 4u;
+	const __amdgpu_buffer_rsrc_t pfr = __builtin_amdgcn_make_buffer_rsrc(pf, (short)0,
+			(int)(unsigned)(pfb < 0xfffffffcull ? pfb : 0xfffffffcull), CG_RSRC_WORD3);
+	bool on[U];
+	unsigned qq[U];
+	double x[U], r[U], p[U];
+#pragma unroll
+	for (int u = 0; u < U; u++) {
+		const unsigned k = k0 + u * S;
+		on[u] = k < nc;
+		const unsigned kc = min(k, nc - 1);   // past the end: the last entry, unused
+		qq[u] = gload<unsigned>(d.idx + kc);
+		x[u] = gload<double>(d.x + kc);
+		r[u] = gload<double>(d.r + kc);
+		p[u] = gload<double>(d.p + kc);
+	}
+	double rr0 = gload<double>(d.rr);
+	unsigned epoch = 0;
+	int it = 0, stopped = 0;
+	while (it < maxit) {
+		// the 13 U tap offsets are recomputed here in every iteration: held
+		// from one to the next (the compiler's choice, as they do not
+		// change) they took 52 registers and made the kernel spill
+#pragma unroll
+		for (int u = 0; u < U; u++) asm volatile("" : "+v"(qq[u]));
+		// q = A p and the partial p.q (k_cg_spmv)
+		float t[U][NT];
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			const int j = (int)(qq[u] / (unsigned)w), i = (int)(qq[u] - (unsigned)j * w);
+#pragma unroll
+			for (int e = 0; e < NT; e++) {
+				int di, dj;
+				cg_tap_offset(e, di, dj);
+				const int a = clampi(i + di, 0, w - 1), b = clampi(j + dj, 0, h - 1);
+				const unsigned off = ((unsigned)b * (unsigned)w + (unsigned)a) << 2;
+				t[u][e] = __builtin_bit_cast(float,
+						__builtin_amdgcn_raw_buffer_load_b32(pfr, off, 0, CG_SC1));
+			}
+		}
+		double q[U], acc = 0;
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			q[u] = cg_apply_taps<BILAP>(t[u]);
+			if (on[u]) acc += q[u] * p[u];
+		}
+		double sb = block_sum_dbl<CG_IT>(acc, red);
+		if (threadIdx.x == 0) cg_astore(d.part + blockIdx.x, sb);
+		if (!cg_grid_sync(bar, ++epoch, nblk, err, spin_max)) return;
+		// alpha = rr0 / p.q, r -= alpha q and the partial r.r (k_cg_update)
+		double pa[2];
+		cg_part_load(d.part, nblk, pa);
+		const double pq = cg_part_sum(pa, nblk, red);
+		if (threadIdx.x == 0) s_alpha = rr0 / pq;
+		__syncthreads();
+		const double alpha = s_alpha;
+		acc = 0;
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			if (!on[u]) continue;
+			r[u] = r[u] - alpha * q[u];
+			acc += r[u] * r[u];
+		}
+		sb = block_sum_dbl<CG_IT>(acc, red);
+		if (threadIdx.x == 0) cg_astore(d.part + nblk + blockIdx.x, sb);
+		if (!cg_grid_sync(bar, ++epoch, nblk, err, spin_max)) return;
+		// r.r, x += alpha p, the stop test, p = r + beta p (k_cg_dir)
+		double pb[2];
+		cg_part_load(d.part + nblk, nblk, pb);
+		const double rs = cg_part_sum(pb, nblk, red);
+		if (threadIdx.x == 0) s_rr1 = rs;
+		__syncthreads();
+		const double rr1 = s_rr1;
+		const bool stop = sqrt(rr1) < d.stop;
+		const double beta = rr1 / rr0;
+#pragma unroll
+		for (int u = 0; u < U; u++) {
+			if (!on[u]) continue;
+			x[u] = x[u] + alpha * p[u];
+			if (stop) continue;
+			p[u] = r[u] + beta * p[u];
+			cg_astore(pf + qq[u], (float)p[u]);
+		}
+		if (blockIdx.x == 0 && threadIdx.x == 0) d.rr[it + 1] = rr1;
+		rr0 = rr1;
+		it++;
+		if (stop) { stopped = 1; break; }
+		if (it < maxit) {
+			// p's float copy complete before the next stencils read it
+			if (!cg_grid_sync(bar, ++epoch, nblk, err, spin_max)) return;
+		}
+	}
+#pragma unroll
+	for (int u = 0; u < U; u++)
+		if (on[u]) d.x[k0 + u * S] = x[u];
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		d.done[0] = stopped;
+		d.done[1] = it;
+	}
+}
+
+int cg_persist_capacity(int bilap)
+{
+	const void *f = bilap ? (const void *)k_cg_persist<1> : (const void *)k_cg_persist<0>;
+	int per = 0, dev = 0, ncu = 0;
+	if (hipGetDevice(&dev) != hipSuccess
+			|| hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, CG_IT, 0) != hipSuccess
+			|| hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)
+			!= hipSuccess)
+		return 0;
+	return per * ncu;
+}
+
+int launch_cg_persist(const CgDev *dp, int bilap, int nblk, int maxit, int spin_max,
+		hipStream_t s)
+{
+	const void *f = bilap ? (const void *)k_cg_persist<1> : (const void *)k_cg_persist<0>;
+	void *args[] = {(void *)&dp, (void *)&maxit, (void *)&spin_max};
+	return hipLaunchCooperativeKernel(f, dim3(nblk), dim3(CG_IT), args, 0, s) == hipSuccess
+		? 0 : -1;
 }
 
 // u[holes] = (float) x
@@ -970,12 +1245,12 @@ void launch_cg_setup(const CgDev *dp, int nb, hipStream_t s)
 
 int cg_iter_span() { return CG_ITU * CG_IT; }
 
-void launch_cg_iter(const CgDev *dp, int bilap, int grid, hipStream_t s)
+void launch_cg_iter(const CgDev *dp, int bilap, int grid, int par, hipStream_t s)
 {
-	if (bilap) k_cg_spmv<1, CG_ITU><<<grid, CG_IT, 0, s>>>(dp);
-	else k_cg_spmv<0, CG_ITU><<<grid, CG_IT, 0, s>>>(dp);
-	k_cg_update<CG_ITU><<<grid, CG_IT, 0, s>>>(dp);
-	k_cg_dir<CG_ITU><<<grid, CG_IT, 0, s>>>(dp);
+	if (bilap) k_cg_spmv<1, CG_ITU><<<grid, CG_IT, 0, s>>>(dp, par);
+	else k_cg_spmv<0, CG_ITU><<<grid, CG_IT, 0, s>>>(dp, par);
+	k_cg_update<CG_ITU><<<grid, CG_IT, 0, s>>>(dp, par);
+	k_cg_dir<CG_ITU><<<grid, CG_IT, 0, s>>>(dp, par);
 }
 
 void launch_cg_finish(const CgDev *dp, int nblk, hipStream_t s)
