@@ -421,6 +421,47 @@ def test_dependence_wait_timeout_is_reported():
     assert p.returncode == 0, p.stdout.decode()[-2000:] + p.stderr.decode()[-2000:]
 
 
+def test_fp64_cg_launch_forms_bit_identical(tmp_path):
+    """The FP64 CG's iterations as one persistent launch per level (the
+    default) and as three launches per iteration (SIMPOIS_HIP_CG_PERSIST=0),
+    each in a child process (the variable is read when the library loads):
+    the same outputs and r.r histories bit for bit, for biharmonic and
+    Poisson fills with CG at every pyramid level, an iteration-limited solve
+    and two planes."""
+    child = os.path.join(os.path.dirname(__file__), "cg_persist_child.py")
+    outs = {}
+    for form in ("0", "1"):
+        env = dict(os.environ, SIMPOIS_HIP_CG_PERSIST=form)
+        env.pop("SIMPOIS_HIP_GS_FLOW_SPIN", None)
+        out = tmp_path / f"cg_{form}.npz"
+        print(f"cg child persist={form}", flush=True)
+        p = subprocess.run([sys.executable, child, str(out)], env=env,
+                           capture_output=True, timeout=300)
+        assert p.returncode == 0, p.stdout.decode()[-2000:] + p.stderr.decode()[-2000:]
+        outs[form] = np.load(out)
+    a, b = outs["0"], outs["1"]
+    assert sorted(a.files) == sorted(b.files)
+    assert any("_rr_" in k for k in a.files)
+    for k in a.files:
+        assert a[k].shape == b[k].shape, k
+        assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
+
+
+def test_fp64_cg_barrier_timeout_is_reported():
+    """With the wait bound forced to 0 polls (SIMPOIS_HIP_GS_FLOW_SPIN, read
+    when the library loads, so in a child process) the persistent FP64 CG's
+    grid barriers give up: the solve returns an error.  Every work-group
+    still ends normally (it leaves at the first wait not met at once)."""
+    env = dict(os.environ, SIMPOIS_HIP_GS_FLOW_SPIN="0")
+    env.pop("SIMPOIS_HIP_CG_PERSIST", None)
+    child = os.path.join(os.path.dirname(__file__), "cg_persist_child.py")
+    print("cg barrier error child", flush=True)
+    p = subprocess.run([sys.executable, child, "err"], env=env, capture_output=True,
+                       timeout=300)
+    print(p.stdout.decode(), flush=True)
+    assert p.returncode == 0, p.stdout.decode()[-2000:] + p.stderr.decode()[-2000:]
+
+
 @pytest.mark.parametrize("h,w", [(2600, 2700), (1100, 40000)])
 def test_multigrid_fused_coarse_levels(h, w, tmp_path):
     """Planes whose multigrid levels 5 (2600x2700: 82x85) and 6 (1100x40000:
