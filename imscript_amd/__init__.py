@@ -22,7 +22,8 @@ from .api import (poisson_solver_separable, poisson_rec,
                   simpois, solve_device, solve_batch_device, residual_device,
                   profile, profile_read, autotune_report, stream_copy,
                   plambda_lap, amle, heatd, plan_cache_limit,
-                  plan_cache_clear, set_alloc_cap, device_bytes, last_cg_trace)
+                  plan_cache_clear, set_alloc_cap, device_bytes, last_cg_trace,
+                  cg_persist_counts)
 
 __all__ = ["lib", "lib_path", "SimpoisError", "sp_params", "sp_stats",
            "sp_image", "default_params", "exported_symbols",
@@ -33,4 +34,4 @@ __all__ = ["lib", "lib_path", "SimpoisError", "sp_params", "sp_stats",
            "sp_plane_stats", "PLANE_SWEPT", "PLANE_CONVERGED", "PLANE_MAXIT",
            "PLANE_BREAKDOWN", "PLANE_NONFINITE", "PLANE_STATUS_NAMES",
            "plan_cache_limit", "plan_cache_clear", "set_alloc_cap", "device_bytes",
-           "last_cg_trace"]
+           "last_cg_trace", "cg_persist_counts"]

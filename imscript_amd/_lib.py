@@ -94,6 +94,8 @@ _SIGNATURES = {
                              ctypes.POINTER(ctypes.c_long),
                              ctypes.POINTER(ctypes.c_double)]),
     "sp_autotune_report": (_I, [ctypes.c_char_p, _I]),
+    "sp_cg_persist_counts": (None, [ctypes.POINTER(ctypes.c_long),
+                                    ctypes.POINTER(ctypes.c_long)]),
     "sp_stream_copy": (_I, [_P, _P, ctypes.c_size_t, _P]),
     "sp_plambda_lap_device": (_I, [_P, _P, _I, _I, _I, _I, _I, _P]),
     "sp_plambda_lap_host": (_I, [_P, _P, _I, _I, _I, _I, _I]),

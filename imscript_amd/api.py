@@ -215,6 +215,15 @@ def autotune_report():
     return buf.value.decode()
 
 
+def cg_persist_counts():
+    """(levels, given up): FP64 CG levels solved by the persistent launch in
+    this process, and launches that found their work-groups not all resident
+    and left the iterations to separate launches."""
+    a, b = ctypes.c_long(), ctypes.c_long()
+    lib.sp_cg_persist_counts(ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
 def stream_copy(dst, src, stream=None):
     """dst[:] = src with the library's streaming copy kernel (device float32
     tensors of equal size, a multiple of 4 elements)."""

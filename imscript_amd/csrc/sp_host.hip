@@ -133,6 +133,9 @@ static const int g_cg_persist = [] {
 // of each grid barrier wait of the persistent FP64 CG.
 // SIMPOIS_HIP_GS_FLOW_SPIN=n (read once) sets it, for tests of the error
 // path: with n = 0 any wait that is not met at once gives up.
+// k_cg_persist launches in this process that ran the iterations / gave
+// the launch up at their start gate (sp_cg_persist_counts)
+static std::atomic<long> g_cg_persisted{0}, g_cg_gave_up{0};
 static const int GS_FLOW_SPIN = [] {
 	const char *e = getenv("SIMPOIS_HIP_GS_FLOW_SPIN");
 	if (!e || !*e) return 1 << 18;
@@ -726,6 +729,12 @@ static int gs_tune(Plan &pl, LevelPlan &L, hipStream_t s)
 	return SP_OK;
 }
 
+extern "C" void sp_cg_persist_counts(long *ran, long *gave_up)
+{
+	if (ran) *ran = g_cg_persisted.load();
+	if (gave_up) *gave_up = g_cg_gave_up.load();
+}
+
 extern "C" int sp_autotune_report(char *buf, int cap)
 {
 	std::string out;
@@ -991,7 +1000,7 @@ static int cg64_ensure(Plan &pl, int maxit)
 		TRY(pl.alloc(&pl.cg_pf, n0));
 		TRY(pl.alloc(&pl.cg_part, std::max<size_t>(2 * 1024, nb0)));
 		TRY(pl.alloc(&pl.cg_done, 8));
-		TRY(pl.alloc(&pl.cg_bar, 640));
+		TRY(pl.alloc(&pl.cg_bar, 768));
 		TRY(pl.alloc(&pl.cg_rrp, 2));
 		TRY(pl.alloc(&pl.cg_dev, 1));
 	}
@@ -1081,18 +1090,22 @@ static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 	int it = 0, ci = 0, ran = maxit;
 	bool persisted = false;
 	if (persist) {
-		if (spk::launch_cg_persist(pl.cg_dev, bilap, nblk, maxit, GS_FLOW_SPIN, s) != 0) {
-			(void)hipGetLastError();   // refused: the launches below instead
-			pl.cg_cap[bilap] = 0;
+		spk::launch_cg_persist(pl.cg_dev, bilap, nblk, maxit, GS_FLOW_SPIN, s);
+		HIPCHK(hipGetLastError());
+		HIPCHK(hipMemcpyAsync(pl.h_done, pl.cg_done, 8 * sizeof(int),
+				hipMemcpyDeviceToHost, s));
+		HIPCHK(hipStreamSynchronize(s));
+		if (pl.h_done[7]) {
+			// its work-groups were not all resident (other work held CUs):
+			// nothing was done, the launches below instead
+			g_cg_gave_up++;
 			TRY(cg64_graphs(pl, bilap, nblk, &exec));
 		} else {
-			HIPCHK(hipMemcpyAsync(pl.h_done, pl.cg_done, 6 * sizeof(int),
-					hipMemcpyDeviceToHost, s));
-			HIPCHK(hipStreamSynchronize(s));
 			if (pl.h_done[5])
 				return set_err(SP_ERR_HIP, "FP64 CG grid barrier wait timed out");
 			it = maxit;
 			persisted = true;
+			g_cg_persisted++;
 			if (pl.h_done[0]) ran = pl.h_done[1];
 		}
 	}

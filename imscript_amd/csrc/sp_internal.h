@@ -301,8 +301,9 @@ struct CgDev {
 	int *done;            // [0] = stop flag (for the host), [1] = iterations
 	                      // run, [2] = index of the iteration under way,
 	                      // [3 + (k & 1)] = stop flag iteration k reads,
-	                      // [5] = k_cg_persist's error word (8 entries)
-	unsigned *bar;        // k_cg_persist's barrier words (640 entries)
+	                      // [5] = k_cg_persist's error word, [7] = set when
+	                      // it gave the launch up at its start (8 entries)
+	unsigned *bar;        // k_cg_persist's gate and barrier words (768 entries)
 	int w, h, bilap;
 	int nblk;             // blocks of the iteration kernels
 	int nb;               // setup blocks: cg_setup_blocks(w, h)
@@ -318,10 +319,11 @@ int cg_setup_blocks(int w, int h);
 void launch_cg_setup(const CgDev *dp, int nb, hipStream_t s);
 int cg_iter_span();   // unknowns per block of the iteration kernels
 void launch_cg_iter(const CgDev *dp, int bilap, int grid, int par, hipStream_t s);
-// all iterations in one cooperative launch of nblk work-groups (nonzero:
-// the launch was refused); capacity = work-groups that can be resident
+// all iterations in one launch of nblk work-groups (done[7] set: it found
+// its work-groups not all resident and did nothing); capacity = work-groups
+// that can be resident at once on the device
 int cg_persist_capacity(int bilap);
-int launch_cg_persist(const CgDev *dp, int bilap, int nblk, int maxit, int spin_max,
+void launch_cg_persist(const CgDev *dp, int bilap, int nblk, int maxit, int spin_max,
 		hipStream_t s);
 void launch_cg_finish(const CgDev *dp, int nblk, hipStream_t s);
 

@@ -719,8 +719,9 @@ __global__ __launch_bounds__(CG_NT) void k_cg_rr0(const CgDev *dp)
 		d.done[2] = -1;   // iteration index, advanced by each k_cg_spmv
 		d.done[3] = 0;
 		d.done[4] = 0;
-		d.done[5] = 0;   // k_cg_persist's error word and barrier words
-		for (int k = 0; k < 10; k++) d.bar[64 * k] = 0;
+		d.done[5] = 0;   // k_cg_persist's error word, gate word and barrier words
+		d.done[7] = 0;
+		for (int k = 0; k < 12; k++) d.bar[64 * k] = 0;
 	}
 }
 
@@ -1069,8 +1070,48 @@ static __device__ __forceinline__ bool cg_grid_sync(unsigned *bar, unsigned e, i
 	return s_ok != 0;
 }
 
+// Start gate of k_cg_persist: every work-group counts in at bar[64 * 11]
+// and waits for the decision word bar[64 * 10]: 1 (go), set by the last of
+// the nblk to count in, or 2 (leave), set by a work-group that waited more
+// than spin_max polls; one compare-and-swap from 0 decides, and every
+// work-group acts on the one value.  Go means all work-groups were running
+// at once, and none leaves before the end, so all stay resident for the
+// barriers; with leave no work-group has done any work (a launch whose
+// work-groups cannot all be resident, as when other work holds CUs), and
+// the host runs the iterations as separate launches instead.
+static __device__ __forceinline__ bool cg_start_gate(unsigned *bar, int nblk, int *done,
+		int spin_max)
+{
+	__shared__ int s_go;
+	if (threadIdx.x == 0) {
+		typedef __attribute__((address_space(1))) unsigned gu;
+		gu *dec = (gu *)(bar + 64 * 10);
+		const unsigned o = __hip_atomic_fetch_add((gu *)(bar + 64 * 11), 1u,
+				__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (o == (unsigned)nblk - 1) {
+			unsigned z = 0;
+			__hip_atomic_compare_exchange_strong(dec, &z, 1u, __ATOMIC_RELAXED,
+					__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		}
+		unsigned f;
+		int n = 0;
+		while ((f = cg_aload(bar + 64 * 10)) == 0) {
+			if (++n > spin_max) {
+				unsigned z = 0;
+				__hip_atomic_compare_exchange_strong(dec, &z, 2u, __ATOMIC_RELAXED,
+						__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			}
+			__builtin_amdgcn_s_sleep(1);
+		}
+		s_go = f == 1;
+		if (f != 1) done[7] = 1;   // for the host: the launch did nothing
+	}
+	__syncthreads();
+	return s_go != 0;
+}
+
 // The FP64 CG's iterations in one launch whose work-groups are all resident
-// (a cooperative launch), for a level whose unknowns take one pass of the
+// (checked by cg_start_gate), for a level whose unknowns take one pass of the
 // iteration kernels (nc <= CG_ITU * nblk * CG_IT).  Each thread keeps its
 // entries' x, r and p in registers from the first iteration to the last;
 // only p's float copy (read by the neighbours' stencils) and the per-block
@@ -1084,7 +1125,7 @@ static __device__ __forceinline__ bool cg_grid_sync(unsigned *bar, unsigned e, i
 // (at most 128 registers: two work-groups resident per CU)
 template <int BILAP>
 __global__ __launch_bounds__(CG_IT) __attribute__((amdgpu_waves_per_eu(4)))
-void k_cg_persist(const CgDev *dp, int maxit, int spin_max)
+void k_cg_persist(const CgDev *dp, int maxit, int spin_max, int gate_spin)
 {
 	constexpr int U = CG_ITU, NT = CgTaps<BILAP>::N;
 	const CgDev d = *dp;
@@ -1095,6 +1136,7 @@ void k_cg_persist(const CgDev *dp, int maxit, int spin_max)
 	const unsigned k0 = blockIdx.x * CG_IT + threadIdx.x;
 	unsigned *bar = d.bar;
 	int *err = d.done + 5;
+	if (!cg_start_gate(bar, nblk, d.done, gate_spin)) return;
 	float *pf = d.pf;
 	// the stencils read pf through a buffer resource: a 32-bit byte offset
 	// per tap (one register per address instead of two; the plane has at
@@ -1212,13 +1254,16 @@ int cg_persist_capacity(int bilap)
 	return per * ncu;
 }
 
-int launch_cg_persist(const CgDev *dp, int bilap, int nblk, int maxit, int spin_max,
+// polls of the start gate before a work-group gives the launch up (about
+// 4 ms: the work-groups of a launch that can all be resident start within
+// microseconds of each other)
+constexpr int CG_GATE_SPIN = 1 << 12;
+
+void launch_cg_persist(const CgDev *dp, int bilap, int nblk, int maxit, int spin_max,
 		hipStream_t s)
 {
-	const void *f = bilap ? (const void *)k_cg_persist<1> : (const void *)k_cg_persist<0>;
-	void *args[] = {(void *)&dp, (void *)&maxit, (void *)&spin_max};
-	return hipLaunchCooperativeKernel(f, dim3(nblk), dim3(CG_IT), args, 0, s) == hipSuccess
-		? 0 : -1;
+	if (bilap) k_cg_persist<1><<<nblk, CG_IT, 0, s>>>(dp, maxit, spin_max, CG_GATE_SPIN);
+	else k_cg_persist<0><<<nblk, CG_IT, 0, s>>>(dp, maxit, spin_max, CG_GATE_SPIN);
 }
 
 // u[holes] = (float) x

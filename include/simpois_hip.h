@@ -344,6 +344,13 @@ int sp_stream_copy(float *d_dst, const float *d_src, size_t n, void *stream);
  * No reference counterpart. */
 int sp_autotune_report(char *buf, int cap);
 
+/* FP64 CG levels solved in this process by the persistent launch (all
+ * iterations of a level in one launch), and those whose launch found its
+ * work-groups not all resident at its start and left the iterations to
+ * separate launches (same results).  Either pointer may be NULL.
+ * No reference counterpart. */
+void sp_cg_persist_counts(long *ran, long *gave_up);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,8 @@ def main():
         except Exception as e:   # the library's error, as the Python layer raises it
             print(f"cg: {type(e).__name__}: {e}", flush=True)
             return 0 if "barrier wait" in str(e) else 1
+        finally:
+            print("persistent launches (ran, gave up):", m.cg_persist_counts(), flush=True)
     res = {}
     for name, g, f, t, cgit in cases():
         out = m.simpois(g, f, tstep=t, niter=0, scale=99, cgit=cgit, trace=False)
@@ -50,6 +52,9 @@ def main():
                     break
                 res[f"{name}_rr_{l}_{lev}"] = np.array(tr[1], np.float64)
         print(f"{name}: {g.shape}", flush=True)
+    ran, gave_up = m.cg_persist_counts()
+    print(f"persistent launches: ran {ran}, gave up {gave_up}", flush=True)
+    res["persist_ran"] = np.array([ran, gave_up], np.int64)
     np.savez(sys.argv[1], **res)
     return 0
 

@@ -442,7 +442,12 @@ def test_fp64_cg_launch_forms_bit_identical(tmp_path):
     a, b = outs["0"], outs["1"]
     assert sorted(a.files) == sorted(b.files)
     assert any("_rr_" in k for k in a.files)
+    # the persistent form ran every level in one launch, the other none
+    assert a["persist_ran"][0] == 0 and b["persist_ran"][0] > 0
+    assert b["persist_ran"][1] == 0
     for k in a.files:
+        if k == "persist_ran":
+            continue
         assert a[k].shape == b[k].shape, k
         assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
 
