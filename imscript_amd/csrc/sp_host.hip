@@ -133,6 +133,15 @@ static const int g_cg_persist = [] {
 // of each grid barrier wait of the persistent FP64 CG.
 // SIMPOIS_HIP_GS_FLOW_SPIN=n (read once) sets it, for tests of the error
 // path: with n = 0 any wait that is not met at once gives up.
+// log2 of the arrival groups of the persistent FP64 CG's grid barriers:
+// SIMPOIS_HIP_CG_GROUPS=1, 2, 4, 8, 16 or 32 (read once; default 8, the XCDs)
+static const int g_cg_groups = [] {
+	const char *e = getenv("SIMPOIS_HIP_CG_GROUPS");
+	const int v = e ? atoi(e) : 8;
+	for (int lg = 0; lg <= 5; lg++)
+		if (v == 1 << lg) return lg;
+	return 3;
+}();
 // k_cg_persist launches in this process that ran the iterations / gave
 // the launch up at their start gate (sp_cg_persist_counts)
 static std::atomic<long> g_cg_persisted{0}, g_cg_gave_up{0};
@@ -1000,7 +1009,7 @@ static int cg64_ensure(Plan &pl, int maxit)
 		TRY(pl.alloc(&pl.cg_pf, n0));
 		TRY(pl.alloc(&pl.cg_part, std::max<size_t>(2 * 1024, nb0)));
 		TRY(pl.alloc(&pl.cg_done, 8));
-		TRY(pl.alloc(&pl.cg_bar, 768));
+		TRY(pl.alloc(&pl.cg_bar, 64 * 36));
 		TRY(pl.alloc(&pl.cg_rrp, 2));
 		TRY(pl.alloc(&pl.cg_dev, 1));
 	}
@@ -1090,7 +1099,7 @@ static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 	int it = 0, ci = 0, ran = maxit;
 	bool persisted = false;
 	if (persist) {
-		spk::launch_cg_persist(pl.cg_dev, bilap, nblk, maxit, GS_FLOW_SPIN, s);
+		spk::launch_cg_persist(pl.cg_dev, bilap, nblk, maxit, GS_FLOW_SPIN, g_cg_groups, s);
 		HIPCHK(hipGetLastError());
 		HIPCHK(hipMemcpyAsync(pl.h_done, pl.cg_done, 8 * sizeof(int),
 				hipMemcpyDeviceToHost, s));

@@ -303,7 +303,7 @@ struct CgDev {
 	                      // [3 + (k & 1)] = stop flag iteration k reads,
 	                      // [5] = k_cg_persist's error word, [7] = set when
 	                      // it gave the launch up at its start (8 entries)
-	unsigned *bar;        // k_cg_persist's gate and barrier words (768 entries)
+	unsigned *bar;        // k_cg_persist's gate and barrier words (2304 entries)
 	int w, h, bilap;
 	int nblk;             // blocks of the iteration kernels
 	int nb;               // setup blocks: cg_setup_blocks(w, h)
@@ -324,7 +324,7 @@ void launch_cg_iter(const CgDev *dp, int bilap, int grid, int par, hipStream_t s
 // that can be resident at once on the device
 int cg_persist_capacity(int bilap);
 void launch_cg_persist(const CgDev *dp, int bilap, int nblk, int maxit, int spin_max,
-		hipStream_t s);
+		int lg, hipStream_t s);   // lg: log2 of the barrier's arrival groups (0..5)
 void launch_cg_finish(const CgDev *dp, int nblk, hipStream_t s);
 
 // adjoint of the converged Poisson fill

@@ -721,7 +721,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_rr0(const CgDev *dp)
 		d.done[4] = 0;
 		d.done[5] = 0;   // k_cg_persist's error word, gate word and barrier words
 		d.done[7] = 0;
-		for (int k = 0; k < 12; k++) d.bar[64 * k] = 0;
+		for (int k = 0; k < 36; k++) d.bar[64 * k] = 0;
 	}
 }
 
@@ -1030,33 +1030,34 @@ static __device__ __forceinline__ void cg_astore(T *p, T v)
 }
 
 // Grid barrier of k_cg_persist, e-th (from 1): work-group b counts in at
-// group counter bar[64 * (b % 8)] (the group of one XCD: its blocks
-// i = b mod 8), the last of a group at the global counter bar[64 * 8], and
-// the last of all writes e to the release word bar[64 * 9], which the
-// work-groups poll (the words are 256 bytes apart: arrivals and polls do
-// not share a line).  Each work-group's waves have completed their stores
-// before it counts in.  A wait past spin_max polls, or one that finds *err
-// set, sets *err and returns false (the work-group leaves; so does every
-// other one at its next wait).
+// group counter bar[64 * (b % G)] (G = 1 << lg groups; with G a multiple of
+// 8 a group's blocks share an XCD, block i running on XCD i mod 8), the
+// last of a group at the global counter bar[64 * 32], and the last of all
+// writes e to the release word bar[64 * 33], which the work-groups poll
+// (the words are 256 bytes apart: arrivals and polls do not share a line).
+// Each work-group's waves have completed their stores before it counts in.
+// A wait past spin_max polls, or one that finds *err set, sets *err and
+// returns false (the work-group leaves; so does every other one at its next
+// wait).
 static __device__ __forceinline__ bool cg_grid_sync(unsigned *bar, unsigned e, int nblk,
-		int *err, int spin_max)
+		int lg, int *err, int spin_max)
 {
 	__shared__ int s_ok;
 	__builtin_amdgcn_s_waitcnt(0);   // this wave's stores have completed
 	__syncthreads();
 	if (threadIdx.x == 0) {
-		const int g = (int)blockIdx.x & 7, ng = nblk < 8 ? nblk : 8;
-		const unsigned ing = (unsigned)(nblk / 8 + (g < nblk % 8));   // blocks in group g
+		const int G = 1 << lg, g = (int)blockIdx.x & (G - 1), ng = nblk < G ? nblk : G;
+		const unsigned ing = (unsigned)((nblk >> lg) + (g < (nblk & (G - 1))));   // in group g
 		typedef __attribute__((address_space(1))) unsigned gu;
 		const unsigned o = __hip_atomic_fetch_add((gu *)(bar + 64 * g), 1u,
 				__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		if (o == e * ing - 1) {
-			const unsigned o2 = __hip_atomic_fetch_add((gu *)(bar + 64 * 8), 1u,
+			const unsigned o2 = __hip_atomic_fetch_add((gu *)(bar + 64 * 32), 1u,
 					__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-			if (o2 == e * (unsigned)ng - 1) cg_astore(bar + 64 * 9, e);
+			if (o2 == e * (unsigned)ng - 1) cg_astore(bar + 64 * 33, e);
 		}
 		int ok = 1, n = 0;
-		while ((int)(cg_aload(bar + 64 * 9) - e) < 0) {
+		while ((int)(cg_aload(bar + 64 * 33) - e) < 0) {
 			if (++n > spin_max || ((n & 255) == 0 && cg_aload(err))) {
 				ok = 0;
 				cg_astore(err, 1);
@@ -1070,8 +1071,8 @@ static __device__ __forceinline__ bool cg_grid_sync(unsigned *bar, unsigned e, i
 	return s_ok != 0;
 }
 
-// Start gate of k_cg_persist: every work-group counts in at bar[64 * 11]
-// and waits for the decision word bar[64 * 10]: 1 (go), set by the last of
+// Start gate of k_cg_persist: every work-group counts in at bar[64 * 35]
+// and waits for the decision word bar[64 * 34]: 1 (go), set by the last of
 // the nblk to count in, or 2 (leave), set by a work-group that waited more
 // than spin_max polls; one compare-and-swap from 0 decides, and every
 // work-group acts on the one value.  Go means all work-groups were running
@@ -1085,8 +1086,8 @@ static __device__ __forceinline__ bool cg_start_gate(unsigned *bar, int nblk, in
 	__shared__ int s_go;
 	if (threadIdx.x == 0) {
 		typedef __attribute__((address_space(1))) unsigned gu;
-		gu *dec = (gu *)(bar + 64 * 10);
-		const unsigned o = __hip_atomic_fetch_add((gu *)(bar + 64 * 11), 1u,
+		gu *dec = (gu *)(bar + 64 * 34);
+		const unsigned o = __hip_atomic_fetch_add((gu *)(bar + 64 * 35), 1u,
 				__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		if (o == (unsigned)nblk - 1) {
 			unsigned z = 0;
@@ -1095,7 +1096,7 @@ static __device__ __forceinline__ bool cg_start_gate(unsigned *bar, int nblk, in
 		}
 		unsigned f;
 		int n = 0;
-		while ((f = cg_aload(bar + 64 * 10)) == 0) {
+		while ((f = cg_aload(bar + 64 * 34)) == 0) {
 			if (++n > spin_max) {
 				unsigned z = 0;
 				__hip_atomic_compare_exchange_strong(dec, &z, 2u, __ATOMIC_RELAXED,
@@ -1125,7 +1126,7 @@ static __device__ __forceinline__ bool cg_start_gate(unsigned *bar, int nblk, in
 // (at most 128 registers: two work-groups resident per CU)
 template <int BILAP>
 __global__ __launch_bounds__(CG_IT) __attribute__((amdgpu_waves_per_eu(4)))
-void k_cg_persist(const CgDev *dp, int maxit, int spin_max, int gate_spin)
+void k_cg_persist(const CgDev *dp, int maxit, int spin_max, int gate_spin, int lg)
 {
 	constexpr int U = CG_ITU, NT = CgTaps<BILAP>::N;
 	const CgDev d = *dp;
@@ -1189,7 +1190,7 @@ void k_cg_persist(const CgDev *dp, int maxit, int spin_max, int gate_spin)
 		}
 		double sb = block_sum_dbl<CG_IT>(acc, red);
 		if (threadIdx.x == 0) cg_astore(d.part + blockIdx.x, sb);
-		if (!cg_grid_sync(bar, ++epoch, nblk, err, spin_max)) return;
+		if (!cg_grid_sync(bar, ++epoch, nblk, lg, err, spin_max)) return;
 		// alpha = rr0 / p.q, r -= alpha q and the partial r.r (k_cg_update)
 		double pa[2];
 		cg_part_load(d.part, nblk, pa);
@@ -1206,7 +1207,7 @@ void k_cg_persist(const CgDev *dp, int maxit, int spin_max, int gate_spin)
 		}
 		sb = block_sum_dbl<CG_IT>(acc, red);
 		if (threadIdx.x == 0) cg_astore(d.part + nblk + blockIdx.x, sb);
-		if (!cg_grid_sync(bar, ++epoch, nblk, err, spin_max)) return;
+		if (!cg_grid_sync(bar, ++epoch, nblk, lg, err, spin_max)) return;
 		// r.r, x += alpha p, the stop test, p = r + beta p (k_cg_dir)
 		double pb[2];
 		cg_part_load(d.part + nblk, nblk, pb);
@@ -1230,7 +1231,7 @@ void k_cg_persist(const CgDev *dp, int maxit, int spin_max, int gate_spin)
 		if (stop) { stopped = 1; break; }
 		if (it < maxit) {
 			// p's float copy complete before the next stencils read it
-			if (!cg_grid_sync(bar, ++epoch, nblk, err, spin_max)) return;
+			if (!cg_grid_sync(bar, ++epoch, nblk, lg, err, spin_max)) return;
 		}
 	}
 #pragma unroll
@@ -1260,10 +1261,10 @@ int cg_persist_capacity(int bilap)
 constexpr int CG_GATE_SPIN = 1 << 12;
 
 void launch_cg_persist(const CgDev *dp, int bilap, int nblk, int maxit, int spin_max,
-		hipStream_t s)
+		int lg, hipStream_t s)
 {
-	if (bilap) k_cg_persist<1><<<nblk, CG_IT, 0, s>>>(dp, maxit, spin_max, CG_GATE_SPIN);
-	else k_cg_persist<0><<<nblk, CG_IT, 0, s>>>(dp, maxit, spin_max, CG_GATE_SPIN);
+	if (bilap) k_cg_persist<1><<<nblk, CG_IT, 0, s>>>(dp, maxit, spin_max, CG_GATE_SPIN, lg);
+	else k_cg_persist<0><<<nblk, CG_IT, 0, s>>>(dp, maxit, spin_max, CG_GATE_SPIN, lg);
 }
 
 // u[holes] = (float) x
