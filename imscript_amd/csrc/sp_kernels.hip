@@ -44,11 +44,60 @@ static __device__ int block_sum_int(int v, int *red)
 	return r;
 }
 
+// A double moved across lanes as its two halves: lane i takes lane i + 32
+// (lanes 0-31), lane i + 16 (lanes 0-15 and 32-47: the swapped halves of
+// permlane32 / permlane16 swaps), or lane i + o within its row of 16 (a
+// DPP row shift left by o = 8, 4, 2, 1: CTRL 0x100 + o; lanes whose source
+// leaves the row keep their value)
+static __device__ __forceinline__ double dbl_of(unsigned lo, unsigned hi)
+{
+	return __builtin_bit_cast(double, (unsigned long long)hi << 32 | lo);
+}
+static __device__ __forceinline__ double lanes_up32(double x)
+{
+	const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+	const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
+	const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32),
+			false, false);
+	return dbl_of(lo[1], hi[1]);
+}
+static __device__ __forceinline__ double lanes_up16(double x)
+{
+	const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+	const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)b, (unsigned)b, false, false);
+	const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32),
+			false, false);
+	return dbl_of(lo[1], hi[1]);
+}
+template <int CTRL>
+static __device__ __forceinline__ double lanes_up_row(double x)
+{
+	const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+	const unsigned lo = (unsigned)b, hi = (unsigned)(b >> 32);
+	return dbl_of(__builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false),
+			__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false));
+}
+
+// wave sum in lane 0 by the tree v += (v of lane + o), o = 32, 16, ..., 1
+// (that of __shfl_down, value for value: lane 0's result reads at step o
+// only lanes below 2 o, which the moves above serve, with no trip through
+// the LDS crossbar)
+static __device__ __forceinline__ double wave_sum_dbl(double v)
+{
+	v = v + lanes_up32(v);
+	v = v + lanes_up16(v);
+	v = v + lanes_up_row<0x108>(v);
+	v = v + lanes_up_row<0x104>(v);
+	v = v + lanes_up_row<0x102>(v);
+	v = v + lanes_up_row<0x101>(v);
+	return v;
+}
+
 // block-wide sum of a double (fixed tree), result valid in thread 0
 template <int NT>
 static __device__ double block_sum_dbl(double v, double *red)
 {
-	for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+	v = wave_sum_dbl(v);
 	int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 	if (lane == 0) red[wv] = v;
 	__syncthreads();
