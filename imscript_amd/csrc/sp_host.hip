@@ -142,6 +142,18 @@ static const int g_cg_groups = [] {
 		if (v == 1 << lg) return lg;
 	return 3;
 }();
+// polls of the persistent FP64 CG's start gate before a work-group gives
+// the launch up (default 2^12, about 4 ms: the work-groups of a launch that
+// can all be resident start within microseconds of each other);
+// SIMPOIS_HIP_CG_GATE_SPIN=n (read once) sets it, for tests of the fallback
+// (with 0 a launch goes ahead only if its last work-group counts in before
+// any other polls)
+static const int g_cg_gate_spin = [] {
+	const char *e = getenv("SIMPOIS_HIP_CG_GATE_SPIN");
+	if (!e || !*e) return 1 << 12;
+	const long v = strtol(e, nullptr, 10);
+	return v < 0 ? 0 : v > (1L << 30) ? 1 << 30 : (int)v;
+}();
 // k_cg_persist launches in this process that ran the iterations / gave
 // the launch up at their start gate (sp_cg_persist_counts)
 static std::atomic<long> g_cg_persisted{0}, g_cg_gave_up{0};
@@ -1099,7 +1111,8 @@ static int run_cg64(Plan &pl, int k, int m, int cgit, int bilap, float eps,
 	int it = 0, ci = 0, ran = maxit;
 	bool persisted = false;
 	if (persist) {
-		spk::launch_cg_persist(pl.cg_dev, bilap, nblk, maxit, GS_FLOW_SPIN, g_cg_groups, s);
+		spk::launch_cg_persist(pl.cg_dev, bilap, nblk, maxit, GS_FLOW_SPIN, g_cg_gate_spin,
+				g_cg_groups, s);
 		HIPCHK(hipGetLastError());
 		HIPCHK(hipMemcpyAsync(pl.h_done, pl.cg_done, 8 * sizeof(int),
 				hipMemcpyDeviceToHost, s));

@@ -323,8 +323,10 @@ void launch_cg_iter(const CgDev *dp, int bilap, int grid, int par, hipStream_t s
 // its work-groups not all resident and did nothing); capacity = work-groups
 // that can be resident at once on the device
 int cg_persist_capacity(int bilap);
+// spin_max bounds each barrier wait, gate_spin the start gate's wait; lg:
+// log2 of the barriers' arrival groups (0..5)
 void launch_cg_persist(const CgDev *dp, int bilap, int nblk, int maxit, int spin_max,
-		int lg, hipStream_t s);   // lg: log2 of the barrier's arrival groups (0..5)
+		int gate_spin, int lg, hipStream_t s);
 void launch_cg_finish(const CgDev *dp, int nblk, hipStream_t s);
 
 // adjoint of the converged Poisson fill

@@ -1304,16 +1304,11 @@ int cg_persist_capacity(int bilap)
 	return per * ncu;
 }
 
-// polls of the start gate before a work-group gives the launch up (about
-// 4 ms: the work-groups of a launch that can all be resident start within
-// microseconds of each other)
-constexpr int CG_GATE_SPIN = 1 << 12;
-
 void launch_cg_persist(const CgDev *dp, int bilap, int nblk, int maxit, int spin_max,
-		int lg, hipStream_t s)
+		int gate_spin, int lg, hipStream_t s)
 {
-	if (bilap) k_cg_persist<1><<<nblk, CG_IT, 0, s>>>(dp, maxit, spin_max, CG_GATE_SPIN, lg);
-	else k_cg_persist<0><<<nblk, CG_IT, 0, s>>>(dp, maxit, spin_max, CG_GATE_SPIN, lg);
+	if (bilap) k_cg_persist<1><<<nblk, CG_IT, 0, s>>>(dp, maxit, spin_max, gate_spin, lg);
+	else k_cg_persist<0><<<nblk, CG_IT, 0, s>>>(dp, maxit, spin_max, gate_spin, lg);
 }
 
 // u[holes] = (float) x

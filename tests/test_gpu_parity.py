@@ -452,6 +452,37 @@ def test_fp64_cg_launch_forms_bit_identical(tmp_path):
         assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
 
 
+def test_fp64_cg_start_gate_fallback(tmp_path):
+    """With the start gate's wait forced to 0 polls (SIMPOIS_HIP_CG_GATE_SPIN,
+    read when the library loads, so in a child process) the persistent
+    launches give up before any work and the iterations run as separate
+    launches: the same outputs and r.r histories, bit for bit, as with the
+    separate launches chosen from the start (SIMPOIS_HIP_CG_PERSIST=0)."""
+    child = os.path.join(os.path.dirname(__file__), "cg_persist_child.py")
+    outs = {}
+    for name, extra in (("sep", {"SIMPOIS_HIP_CG_PERSIST": "0"}),
+                        ("gate", {"SIMPOIS_HIP_CG_GATE_SPIN": "0"})):
+        env = dict(os.environ, **extra)
+        if name == "gate":
+            env.pop("SIMPOIS_HIP_CG_PERSIST", None)
+        env.pop("SIMPOIS_HIP_GS_FLOW_SPIN", None)
+        out = tmp_path / f"cg_{name}.npz"
+        print(f"cg child {name}", flush=True)
+        p = subprocess.run([sys.executable, child, str(out)], env=env,
+                           capture_output=True, timeout=300)
+        print(p.stdout.decode(), flush=True)
+        assert p.returncode == 0, p.stdout.decode()[-2000:] + p.stderr.decode()[-2000:]
+        outs[name] = np.load(out)
+    a, b = outs["sep"], outs["gate"]
+    assert sorted(a.files) == sorted(b.files)
+    assert b["persist_ran"][1] > 0   # launches that gave up at the gate
+    for k in a.files:
+        if k == "persist_ran":
+            continue
+        assert a[k].shape == b[k].shape, k
+        assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
+
+
 def test_fp64_cg_barrier_timeout_is_reported():
     """With the wait bound forced to 0 polls (SIMPOIS_HIP_GS_FLOW_SPIN, read
     when the library loads, so in a child process) the persistent FP64 CG's
