@@ -1170,8 +1170,9 @@ static __device__ __forceinline__ bool cg_start_gate(unsigned *bar, int nblk, in
 // k_cg_spmv, k_cg_update and k_cg_dir.  Same entries per thread, same
 // arithmetic and the same partial sums in the same trees as those kernels:
 // the same iterates, r.r history and stop.  Writes x, the history and
-// done[0..1] as they do.  bar holds the barrier words and done[5] is the
-// error word, both cleared by k_cg_rr0.
+// done[0..1] as they do.  bar holds the gate and barrier words, done[5] is
+// the error word and done[7] the gate's give-up mark, all cleared by
+// k_cg_rr0.
 // (at most 128 registers: two work-groups resident per CU)
 template <int BILAP>
 __global__ __launch_bounds__(CG_IT) __attribute__((amdgpu_waves_per_eu(4)))
